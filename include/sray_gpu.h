@@ -1,0 +1,374 @@
+/*
+ * sray_gpu.h — C ABI of the MI355X (gfx950) ray-tracing hot path.
+ *
+ * Plain C, POD structs, plain pointers and sizes, int status codes (0 = ok,
+ * < 0 = error, text from sray_gpu_last_error()).  No torch, no C++ types.
+ *
+ * Every entry point names the reference interface it replaces (S-ray 0.1,
+ * file:line).  Host buffers are caller-owned and copied in/out; scenes and
+ * photon maps uploaded to a context are owned by the context until released.
+ * One context = one HIP stream; calls on one context are serialised by an
+ * internal mutex, several contexts may share a device.
+ */
+#ifndef SRAY_GPU_H_
+#define SRAY_GPU_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status codes                                                        */
+/* ------------------------------------------------------------------ */
+#define SRAY_OK              0
+#define SRAY_ERR_ARG        -1   /* invalid argument / size */
+#define SRAY_ERR_HIP        -2   /* a HIP runtime call failed */
+#define SRAY_ERR_NOMEM      -3   /* allocation failed or admission cap exceeded */
+#define SRAY_ERR_NOTFOUND   -4   /* unknown scene / photon map / ticket */
+#define SRAY_ERR_LIMIT      -5   /* a compiled limit (stack depth, ior stack) is exceeded */
+#define SRAY_ERR_PARSE      -6   /* scene XML could not be parsed/loaded */
+#define SRAY_ERR_PENDING    -7   /* sray_gpu_poll: request not finished yet */
+#define SRAY_ERR_NODEVICE   -8   /* no HIP device */
+
+/* ------------------------------------------------------------------ */
+/* constants shared with the reference                                 */
+/* ------------------------------------------------------------------ */
+/* Rays are segments t in (epsilon, 1]; the direction carries the length
+ * (RAY_MAGNITUDE, src/opt.h:21).  ERROR_MARGIN / XSMALL_NUMBER live in the
+ * un-vendored libvmath; these are the values this build fixes. */
+#define SRAY_RAY_MAGNITUDE   10000.0
+#define SRAY_ERROR_MARGIN    1e-6
+#define SRAY_XSMALL_NUMBER   1e-8
+
+/* object types, src/object.cc:28-29 */
+#define SRAY_OBJ_SPHERE      0
+#define SRAY_OBJ_CYLINDER    1
+#define SRAY_OBJ_MESH        2
+
+/* light types, src/light.cc:23-29 */
+#define SRAY_LIGHT_POINT     0
+#define SRAY_LIGHT_SPHERE    1
+#define SRAY_LIGHT_BOX       2
+
+/* material flags: which attributes the <material> defines
+ * (Material::have_attribute, src/material.cc:170-173) and its shader
+ * (src/shader.cc:28-34). */
+#define SRAY_MAT_HAS_SPECULAR  0x1u
+#define SRAY_MAT_HAS_REFLECT   0x2u
+#define SRAY_MAT_HAS_REFRACT   0x4u
+#define SRAY_MAT_PHONG         0x8u   /* shader="phong"; otherwise shade_undef */
+
+/* ------------------------------------------------------------------ */
+/* flat scene description (host side, what Scene::load produces)       */
+/* ------------------------------------------------------------------ */
+/* Matrices are row-major 3x4 (rows x,y,z of a 4x4 affine matrix whose last
+ * row is 0 0 0 1), applied to column vectors: p' = M * (p, 1). */
+typedef struct sray_object_desc {
+    int32_t type;          /* SRAY_OBJ_* */
+    int32_t material;      /* index into materials, -1 = Scene::default_mat */
+    int32_t mesh;          /* index into meshes (SRAY_OBJ_MESH), else -1 */
+    float   radius;        /* sphere / cylinder radius */
+    float   start, end;    /* cylinder extent along local y */
+    float   xform[12];     /* object -> world at the frame time */
+    float   inv_xform[12]; /* world -> object */
+    /* linear translation over [motion_t0, motion_t1] msec (motion blur):
+     * xform(t) = xform with translation + motion * s, s = clamp((t-t0)/(t1-t0)).
+     * all zero = static object. */
+    float   motion[3];
+    int32_t motion_t0, motion_t1;
+} sray_object_desc;
+
+typedef struct sray_mesh_desc {
+    uint32_t tri_start;    /* first triangle in the scene triangle arrays */
+    uint32_t tri_count;
+} sray_mesh_desc;
+
+typedef struct sray_material_desc {
+    float    diffuse[4];   /* kd */
+    float    specular[4];  /* ks */
+    float    shininess;    /* default 1 (Material::def_attr, src/material.cc:77) */
+    float    reflect;
+    float    refract;
+    float    ior;
+    uint32_t flags;        /* SRAY_MAT_* */
+    uint32_t pad_;
+} sray_material_desc;
+
+typedef struct sray_light_desc {
+    int32_t type;          /* SRAY_LIGHT_* */
+    float   pos[3];        /* get_position(frame time) */
+    float   color[4];
+    float   radius;        /* sphere light */
+    float   dim[3];        /* box light */
+    float   xform[12];     /* box light: get_xform_matrix() at time 0 */
+} sray_light_desc;
+
+typedef struct sray_camera_desc {
+    float   xform[12];     /* camera -> world (Camera::get_matrix) */
+    double  vfov;          /* radians (default QUARTER_PI, src/camera.cc:32) */
+    float   max_dist;      /* RAY_MAGNITUDE */
+    int32_t shutter;       /* msec */
+    /* the aspect ratio is not a scene property: Camera() takes it from the
+     * render size (src/camera.cc:31), so it comes from sray_render_params */
+} sray_camera_desc;
+
+typedef struct sray_scene_desc {
+    uint32_t                  num_objects;
+    const sray_object_desc   *objects;
+    uint32_t                  num_meshes;
+    const sray_mesh_desc     *meshes;
+    uint32_t                  num_triangles;
+    const float              *tri_pos;      /* 9 floats/tri, v0 v1 v2, mesh-local */
+    const float              *tri_normal;   /* 9 floats/tri or NULL */
+    const float              *tri_tangent;  /* 9 floats/tri or NULL */
+    const float              *tri_texcoord; /* 6 floats/tri or NULL */
+    uint32_t                  num_materials;
+    const sray_material_desc *materials;
+    uint32_t                  num_lights;
+    const sray_light_desc    *lights;
+    sray_camera_desc          camera;
+    float                     env_ambient[4];
+    float                     env_color[4];
+    float                     env_ior;
+    int32_t                   scene_oct_depth;  /* -soctdepth (default 5) */
+    int32_t                   scene_oct_items;  /* -soctitems (default 5) */
+    int32_t                   mesh_oct_depth;   /* -moctdepth (default 7) */
+    int32_t                   mesh_oct_items;   /* -moctitems (default 14) */
+    int32_t                   frame_t0, frame_t1; /* build_tree(t0, t1) interval */
+} sray_scene_desc;
+
+/* ------------------------------------------------------------------ */
+/* rays and hits (the parity entry)                                    */
+/* ------------------------------------------------------------------ */
+typedef struct sray_ray {
+    float   origin[3];
+    float   dir[3];        /* segment end = origin + dir */
+    int32_t time;          /* msec */
+    int32_t pad_;
+} sray_ray;
+
+typedef struct sray_hit {
+    float    t;            /* parametric distance, FLT_MAX on miss */
+    int32_t  obj;          /* object index, -1 on miss */
+    int32_t  prim;         /* triangle index within the mesh, -1 otherwise */
+    uint32_t rank;         /* occurrence rank of the winner in the scene octree's
+                              depth-first leaf order (tie-break key) */
+} sray_hit;
+
+#define SRAY_TRACE_CLOSEST   0u    /* Scene::cast_ray with a SurfPoint */
+#define SRAY_TRACE_ANY       1u    /* Scene::cast_ray(ray) == NULL test (shadow) */
+#define SRAY_TRACE_COUNT     2u    /* also fill visit counters (slower) */
+
+typedef struct sray_trace_counts {
+    uint64_t nodes;        /* octree nodes whose box was tested */
+    uint64_t spheres;
+    uint64_t cylinders;
+    uint64_t triangles;
+    uint64_t item_refs;    /* leaf item indices read */
+    uint64_t mesh_enters;  /* mesh (second level) traversals started */
+} sray_trace_counts;
+
+/* ------------------------------------------------------------------ */
+/* rendering                                                           */
+/* ------------------------------------------------------------------ */
+#define SRAY_RENDER_REF_Q1   0x1u  /* reproduce src/render.cc:240 as written */
+#define SRAY_RENDER_COUNT    0x2u  /* fill sray_stats visit counters (slower) */
+
+typedef struct sray_render_params {
+    int32_t  width, height;         /* full image size (camera aspect) */
+    int32_t  x0, y0, x1, y1;        /* pixel rectangle to render (tile); x1/y1 exclusive */
+    int32_t  min_samples, max_samples;  /* -r min[-max] */
+    float    max_var;               /* -d (default 0.005) */
+    float    min_energy;            /* -e (default 1e-4) */
+    int32_t  iter;                  /* -i (default 7) */
+    int32_t  shadow_samples;        /* -shadowrays (default 1) */
+    int32_t  diffuse_samples;       /* -diffuserays (default 1) */
+    int32_t  frame_time;            /* msec, render(msec) */
+    int32_t  mblur;                 /* -m */
+    int32_t  caust_photons;         /* -c : enables caustics gather */
+    int32_t  gi_photons;            /* -g : enables GI diffuse rays */
+    float    gather_dist;           /* -gatherdisc, relative to the scene octree diagonal */
+    uint64_t seed;                  /* counter-based RNG seed (sample jitter, area lights, GI) */
+    uint32_t flags;                 /* SRAY_RENDER_* */
+    uint32_t pad_;
+} sray_render_params;
+
+typedef struct sray_stats {
+    uint64_t rays_primary;
+    uint64_t rays_shadow;
+    uint64_t rays_reflect;
+    uint64_t rays_refract;
+    uint64_t rays_diffuse;
+    uint64_t samples;               /* primary samples over all pixels */
+    double   render_ms;             /* device time of the render phase */
+    double   accel_ms;              /* scene octree build / upload time */
+    sray_trace_counts closest;      /* SRAY_RENDER_COUNT: closest-hit traversal visits */
+    sray_trace_counts shadow;       /* SRAY_RENDER_COUNT: shadow (any-hit) traversal visits */
+} sray_stats;
+
+/* ------------------------------------------------------------------ */
+/* photon maps                                                         */
+/* ------------------------------------------------------------------ */
+typedef struct sray_photon {
+    float pos[3];
+    float dir[3];
+    float norm[3];
+    float col[3];
+} sray_photon;
+
+#define SRAY_PMAP_CAUSTIC    0
+#define SRAY_PMAP_GLOBAL     1
+#define SRAY_GATHER_IRRADIANCE 0   /* PhotonMap::irradiance_est, src/pmap.cc:109-139 */
+#define SRAY_GATHER_RADIANCE   1   /* PhotonMap::radiance_est,  src/pmap.cc:143-178 */
+
+typedef struct sray_gather_query {
+    float pos[3];
+    float norm[3];
+    float dir[3];          /* radiance mode only */
+    float pad_;
+} sray_gather_query;
+
+/* ------------------------------------------------------------------ */
+/* batching / serving requests                                         */
+/* ------------------------------------------------------------------ */
+typedef struct sray_request {
+    uint32_t           scene_id;
+    sray_render_params params;      /* tile rectangle inside params */
+    sray_camera_desc   camera;      /* per-request camera pose */
+    int32_t            use_camera;  /* 0 = the scene's own camera */
+    int32_t            pad_;
+} sray_request;
+
+typedef struct sray_result {
+    float   *rgba;                  /* caller-owned, (x1-x0)*(y1-y0)*4 floats */
+    uint64_t rays;                  /* ray queries made for this request */
+    double   latency_ms;            /* submit -> result on host */
+    double   queue_ms;              /* submit -> admitted */
+} sray_result;
+
+/* ------------------------------------------------------------------ */
+/* opaque context                                                      */
+/* ------------------------------------------------------------------ */
+typedef struct sray_gpu_ctx sray_gpu_ctx;
+
+/* Replaces rend_init(Image*) + ThreadPool::start (src/render.cc:22-32,
+ * src/tpool.cc:48-74): one HIP stream instead of opt.threads workers. */
+int  sray_gpu_create(int device, uint32_t flags, sray_gpu_ctx **out);
+void sray_gpu_destroy(sray_gpu_ctx *ctx);
+const char *sray_gpu_last_error(sray_gpu_ctx *ctx);
+
+/* Replaces Scene::build_tree (src/scene.cc:353-390) + Mesh::build_tree
+ * (src/mesh.cc:210-228) + the per-object data the Object::intersect
+ * overrides read: builds both octree levels on the host, flattens them and
+ * uploads everything to HBM. */
+int  sray_gpu_scene_upload(sray_gpu_ctx *ctx, const sray_scene_desc *s, uint32_t *scene_id);
+int  sray_gpu_scene_release(sray_gpu_ctx *ctx, uint32_t scene_id);
+/* bytes of device memory held by a scene (admission accounting) */
+int  sray_gpu_scene_bytes(sray_gpu_ctx *ctx, uint32_t scene_id, uint64_t *bytes);
+
+/* Replaces Scene::cast_ray (src/scene.cc:764-794) over the octree path
+ * (Octree<T>::intersect, src/octree.inl:194-241).  Host buffers. */
+int  sray_gpu_trace(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_ray *rays, uint32_t n,
+                    sray_hit *hits, uint32_t flags, sray_trace_counts *counts);
+/* Same, on device-resident buffers, enqueued on the context stream. */
+int  sray_gpu_trace_device(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_ray *d_rays,
+                           uint32_t n, sray_hit *d_hits, uint32_t flags);
+
+/* Replaces render_frame/start_frame/render_block + Scene::trace_ray +
+ * shade_phong (src/render.cc:124-244, src/scene.cc:551-569,
+ * src/shader.cc:83-283): wavefront render of params' pixel rectangle into
+ * caller-owned RGBA float (x1-x0)*(y1-y0)*4, row-major. */
+int  sray_gpu_render(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_render_params *p,
+                     float *rgba, sray_stats *st);
+/* Same, into a device buffer, timed with HIP events on the context stream. */
+int  sray_gpu_render_device(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_render_params *p,
+                            float *d_rgba, sray_stats *st);
+
+/* Replaces PhotonMap::add_photon/scale_photon_power + the kdtree build
+ * (src/pmap.cc:73-100): uploads an identical photon array, builds a
+ * balanced kd-tree on the device. */
+int  sray_gpu_pmap_upload(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_photon *p,
+                          uint32_t n, int kind, uint32_t *pmap_id);
+int  sray_gpu_pmap_release(sray_gpu_ctx *ctx, uint32_t pmap_id);
+/* Replaces PhotonMap::irradiance_est / radiance_est (src/pmap.cc:107-178).
+ * k == 0: fixed-radius range gather (reference semantics);
+ * k > 0 : the k nearest within radius, ties broken by photon index. */
+int  sray_gpu_pmap_gather(sray_gpu_ctx *ctx, uint32_t pmap_id, const sray_gather_query *q,
+                          uint32_t n, float radius, int k, int mode, float *rgb);
+
+/* Batching / serving (no reference counterpart: the reference renders one
+ * frame of one scene at a time, src/sray.cc:56-65).  submit admits a tile
+ * request by its memory footprint; requests in flight share ray queues. */
+int  sray_gpu_submit(sray_gpu_ctx *ctx, const sray_request *r, float *rgba, uint64_t *ticket);
+int  sray_gpu_poll(sray_gpu_ctx *ctx, uint64_t ticket, int wait, sray_result *out);
+int  sray_gpu_set_admission_cap(sray_gpu_ctx *ctx, uint64_t bytes);
+
+/* ------------------------------------------------------------------ */
+/* host-only entry points (no device needed)                           */
+/* ------------------------------------------------------------------ */
+typedef struct sray_host_scene sray_host_scene;
+
+/* Replaces Scene::load(const char*) / load(FILE*) (src/scene.cc:75-183):
+ * parses reference-format scene XML into a flat description owned by the
+ * returned handle. */
+int  sray_scene_load_xml(const char *path, const char *xml_text, sray_host_scene **out);
+const sray_scene_desc *sray_scene_desc_of(const sray_host_scene *hs);
+void sray_scene_free(sray_host_scene *hs);
+const char *sray_host_last_error(void);
+
+/* Flattened octree of one level, as the device sees it (for inspection and
+ * host-side tests).  level = -1 for the scene octree, else a mesh index. */
+typedef struct sray_octree_info {
+    uint32_t num_nodes;
+    uint32_t num_leaves;
+    uint32_t num_item_refs;
+    uint32_t height;
+    float    root_min[3], root_max[3];
+} sray_octree_info;
+int  sray_octree_build_host(const sray_scene_desc *s, int level, sray_octree_info *info,
+                            float *node_lo, float *node_hi, uint32_t *items,
+                            uint32_t cap_nodes, uint32_t cap_items);
+
+/* Seeded synthetic workloads (no reference counterpart; the reference ships
+ * no scenes): config 0 = 64 spheres, 1 = fBm heightfield of 1M triangles +
+ * 40 glass/mirror spheres, 2 = photon room, 3 = batch scene with `variant`
+ * triangles.  For configs 0-2, variant > 1 shrinks the scene for tests. */
+int  sray_scene_generate(int config, uint64_t seed, int variant, sray_host_scene **out);
+/* reference-format XML of a host scene (%.9g floats); *len receives the
+ * size including the terminating NUL, buf may be NULL to query it */
+int  sray_scene_write_xml(const sray_host_scene *hs, char *buf, size_t cap, size_t *len);
+
+/* Per-kernel device time, measured with HIP events around each launch on
+ * the context's stream while profiling is on (no reference counterpart;
+ * the reference only prints total wall time, src/sray.cc:67-76). */
+typedef struct sray_kernel_stat {
+    char     name[32];
+    uint64_t launches;
+    double   total_ms;
+} sray_kernel_stat;
+int  sray_gpu_set_profiling(sray_gpu_ctx *ctx, int on);
+/* copies up to cap entries, *n = entries available; reset != 0 clears them */
+int  sray_gpu_kernel_stats(sray_gpu_ctx *ctx, sray_kernel_stat *out, int cap, int *n, int reset);
+
+/* sizeof() of the public structs, in declaration order (sray_object_desc,
+ * sray_mesh_desc, sray_material_desc, sray_light_desc, sray_camera_desc,
+ * sray_scene_desc, sray_ray, sray_hit, sray_trace_counts,
+ * sray_render_params, sray_stats, sray_photon, sray_gather_query,
+ * sray_request, sray_result, sray_octree_info, sray_kernel_stat) — lets
+ * bindings check their
+ * layout.  Returns the number of entries written (at most n). */
+int  sray_abi_sizes(uint32_t *out, int n);
+
+/* Primary rays of one sample pass (Camera::get_primary_ray,
+ * src/camera.cc:119-142) generated by the device raygen kernel into a
+ * device buffer of (x1-x0)*(y1-y0) rays, row-major. */
+int  sray_gpu_primary_rays_device(sray_gpu_ctx *ctx, uint32_t scene_id,
+                                  const sray_render_params *p, int sample, sray_ray *d_rays);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SRAY_GPU_H_ */

@@ -1,0 +1,149 @@
+"""CPU restatement of the S-ray 0.1 hot path — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package, and only as the checker (or as the timed CPU baseline).
+The product (sray_amd/, libsray_gpu.so) never imports, links or calls it.
+
+Which reference function each oracle function restates is cited in
+oracle/oracle.cc.  The reference itself cannot be compiled here (it needs
+libvmath, kdtree and libimago, none of which is present), so there is no
+oracle/_ref build; the restatement is pinned by closed forms, brute force
+and invariants in tests/ (see the parity notes in the test suite).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(_HERE, "_build")
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+class _Ext(C.Structure):
+    _fields_ = [("tie", C.c_int32), ("pad_", C.c_int32), ("nodes", C.c_uint64), ("prims", C.c_uint64)]
+
+
+class _TreeInfo(C.Structure):
+    _fields_ = [("num_nodes", C.c_uint32), ("num_leaves", C.c_uint32), ("num_item_refs", C.c_uint32),
+                ("height", C.c_uint32), ("root_min", C.c_double * 3), ("root_max", C.c_double * 3)]
+
+
+EXT_DTYPE = np.dtype([("tie", np.int32), ("pad", np.int32), ("nodes", np.uint64), ("prims", np.uint64)])
+
+_libs = {}
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(BUILD, name)
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.oracle_scene_create.restype = vp
+        L.oracle_scene_create.argtypes = [vp, C.c_int, C.c_uint32]
+        L.oracle_scene_destroy.argtypes = [vp]
+        L.oracle_trace.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_int, C.c_int]
+        L.oracle_tree.argtypes = [vp, C.c_int, C.POINTER(_TreeInfo), vp, vp, C.c_uint32, C.c_uint32]
+        L.oracle_primary_rays.argtypes = [vp, vp, C.c_int, vp]
+        L.oracle_render.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.oracle_set_photons.argtypes = [vp, C.c_int, vp, C.c_uint32]
+        L.oracle_gather.argtypes = [vp, C.c_int, vp, C.c_uint32, C.c_float, C.c_int, C.c_int, vp]
+        L.oracle_scene_diag.restype = C.c_double
+        L.oracle_scene_diag.argtypes = [vp]
+        L.oracle_rng_u32.restype = C.c_uint32
+        L.oracle_rng_u32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.oracle_splitmix64.restype = C.c_uint64
+        L.oracle_splitmix64.argtypes = [C.c_uint64]
+        _libs[name] = L
+    return _libs[name]
+
+
+Q9_RETRAVERSE = 0x1
+BRUTE_FORCE = 0x2
+
+
+class OracleScene:
+    """The oracle's scene built from an sray_scene_desc (prec 32 or 64)."""
+
+    def __init__(self, desc_ptr, prec=32, flags=0, fast=False):
+        self.L = lib(fast)
+        self.h = self.L.oracle_scene_create(C.cast(desc_ptr, C.c_void_p), prec, flags)
+        if not self.h:
+            raise RuntimeError("oracle_scene_create failed")
+        self.prec = prec
+
+    def trace(self, rays, any_hit=False, nthreads=8):
+        from sray_amd import HIT_DTYPE
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        ext = np.zeros(len(rays), EXT_DTYPE)
+        rc = self.L.oracle_trace(self.h, rays.ctypes.data, len(rays), hits.ctypes.data, ext.ctypes.data,
+                                 1 if any_hit else 0, nthreads)
+        if rc:
+            raise RuntimeError("oracle_trace failed")
+        return hits, ext
+
+    def tree(self, level):
+        info = _TreeInfo()
+        self.L.oracle_tree(self.h, level, C.byref(info), None, None, 0, 0)
+        items = np.zeros(max(info.num_item_refs, 1), np.uint32)
+        counts = np.zeros(max(info.num_leaves, 1), np.uint32)
+        self.L.oracle_tree(self.h, level, C.byref(info), items.ctypes.data, counts.ctypes.data,
+                           info.num_item_refs, info.num_leaves)
+        return info, items[:info.num_item_refs], counts[:info.num_leaves]
+
+    def primary_rays(self, params, sample=0):
+        from sray_amd import RAY_DTYPE
+        n = (params.x1 - params.x0) * (params.y1 - params.y0)
+        rays = np.zeros(n, RAY_DTYPE)
+        if self.L.oracle_primary_rays(self.h, C.addressof(params), sample, rays.ctypes.data):
+            raise RuntimeError("oracle_primary_rays failed")
+        return rays
+
+    def render(self, params, nthreads=8):
+        from sray_amd.abi import Stats
+        w, h = params.x1 - params.x0, params.y1 - params.y0
+        out = np.zeros((h, w, 4), np.float32)
+        st = Stats()
+        if self.L.oracle_render(self.h, C.addressof(params), out.ctypes.data, nthreads, C.addressof(st)):
+            raise RuntimeError("oracle_render failed")
+        return out, st
+
+    def set_photons(self, kind, photons):
+        photons = np.ascontiguousarray(photons, np.float32).reshape(-1, 12)
+        self.L.oracle_set_photons(self.h, kind, photons.ctypes.data, len(photons))
+
+    def gather(self, kind, queries, radius, k=0, mode=0):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 10)
+        out = np.zeros((len(q), 3), np.float32)
+        self.L.oracle_gather(self.h, kind, q.ctypes.data, len(q), radius, k, mode, out.ctypes.data)
+        return out
+
+    def diag(self):
+        return self.L.oracle_scene_diag(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.oracle_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rng_u32(seed, a, b, c, d):
+    return lib().oracle_rng_u32(seed, a, b, c, d)
+
+
+def splitmix64(x):
+    return lib().oracle_splitmix64(x)

@@ -1,0 +1,243 @@
+"""sray_amd — Python bindings (ctypes) over the C ABI of libsray_gpu.so.
+
+The product is the native library (gfx950 HIP kernels + host C++); this
+module only loads it and converts numpy arrays.  Loading fails loudly if the
+library has not been built: there is no Python or CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (Hit, OctreeInfo, Ray, RenderParams, SceneDesc, Stats, TraceCounts)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD_DIR = os.path.join(_HERE, "_build")
+LIB_PATH = os.path.join(BUILD_DIR, "libsray_gpu.so")
+HOST_LIB_PATH = os.path.join(BUILD_DIR, "libsray_host.so")
+
+_lib = None
+
+
+class SrayError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sray error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded libsray_gpu.so (raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                              "(or make -C sray_amd/csrc)")
+        _lib = abi.declare(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _host_err():
+    return lib().sray_host_last_error().decode(errors="replace")
+
+
+def abi_sizes():
+    n = len(abi.ABI_STRUCTS)
+    out = (C.c_uint32 * n)()
+    k = lib().sray_abi_sizes(out, n)
+    return list(out)[:k]
+
+
+# ---------------------------------------------------------------------------
+# host scenes
+# ---------------------------------------------------------------------------
+class HostScene:
+    """A flat scene description owned by the native library (loaded from
+    reference-format XML or generated)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def load_xml(cls, path=None, text=None):
+        h = C.c_void_p()
+        rc = lib().sray_scene_load_xml(path.encode() if path else None, text.encode() if text else None,
+                                       C.byref(h))
+        if rc:
+            raise SrayError(rc, _host_err())
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, config, seed=None, variant=0):
+        """config 0 spheres, 1 heightfield, 2 photon room, 3 batch scene."""
+        if seed is None:
+            seed = 0x5EED0000 + config
+        h = C.c_void_p()
+        rc = lib().sray_scene_generate(int(config), int(seed), int(variant), C.byref(h))
+        if rc:
+            raise SrayError(rc, _host_err())
+        return cls(h.value)
+
+    @property
+    def desc(self):
+        return lib().sray_scene_desc_of(self._h).contents
+
+    def desc_ptr(self):
+        return lib().sray_scene_desc_of(self._h)
+
+    def to_xml(self):
+        n = C.c_size_t()
+        lib().sray_scene_write_xml(self._h, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value)
+        rc = lib().sray_scene_write_xml(self._h, buf, n.value, C.byref(n))
+        if rc:
+            raise SrayError(rc, "write_xml")
+        return buf.value.decode()
+
+    def close(self):
+        if self._h:
+            lib().sray_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def octree_host(desc_ptr, level):
+    """Flattened octree of one level as built on the host:
+    (info, lo[N,4] float32, hi[N,4] float32, items[M] uint32)."""
+    info = OctreeInfo()
+    rc = lib().sray_octree_build_host(desc_ptr, level, C.byref(info), None, None, None, 0, 0)
+    if rc:
+        raise SrayError(rc, "octree_build_host")
+    lo = np.zeros((info.num_nodes, 4), np.float32)
+    hi = np.zeros((info.num_nodes, 4), np.float32)
+    items = np.zeros(max(info.num_item_refs, 1), np.uint32)
+    fp = C.POINTER(C.c_float)
+    rc = lib().sray_octree_build_host(desc_ptr, level, C.byref(info), lo.ctypes.data_as(fp),
+                                      hi.ctypes.data_as(fp), items.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                      info.num_nodes, info.num_item_refs)
+    if rc:
+        raise SrayError(rc, "octree_build_host")
+    return info, lo, hi, items[:info.num_item_refs]
+
+
+# ---------------------------------------------------------------------------
+# numpy <-> struct arrays
+# ---------------------------------------------------------------------------
+RAY_DTYPE = np.dtype([("origin", np.float32, 3), ("dir", np.float32, 3), ("time", np.int32), ("pad", np.int32)])
+HIT_DTYPE = np.dtype([("t", np.float32), ("obj", np.int32), ("prim", np.int32), ("rank", np.uint32)])
+
+
+def make_rays(origins, dirs, time=0):
+    r = np.zeros(len(origins), RAY_DTYPE)
+    r["origin"] = origins
+    r["dir"] = dirs
+    r["time"] = time
+    return r
+
+
+def render_params(width, height, rect=None, samples=(1, 1), max_var=0.005, min_energy=1e-4, iter=7,
+                  shadow_samples=1, diffuse_samples=1, frame_time=0, mblur=0, caust_photons=0, gi_photons=0,
+                  gather_dist=0.001, seed=0x5EED, flags=0):
+    """sray_render_params with the reference's option defaults (src/opt.c:357-388)."""
+    p = RenderParams()
+    p.width, p.height = width, height
+    x0, y0, x1, y1 = rect if rect else (0, 0, width, height)
+    p.x0, p.y0, p.x1, p.y1 = x0, y0, x1, y1
+    p.min_samples, p.max_samples = samples
+    p.max_var, p.min_energy, p.iter = max_var, min_energy, iter
+    p.shadow_samples, p.diffuse_samples = shadow_samples, diffuse_samples
+    p.frame_time, p.mblur = frame_time, mblur
+    p.caust_photons, p.gi_photons, p.gather_dist = caust_photons, gi_photons, gather_dist
+    p.seed, p.flags = seed, flags
+    return p
+
+
+# ---------------------------------------------------------------------------
+# device context
+# ---------------------------------------------------------------------------
+class Context:
+    """One sray_gpu context (one HIP stream) on a device."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().sray_gpu_create(int(device), 0, C.byref(h))
+        if rc:
+            raise SrayError(rc, "sray_gpu_create failed (no HIP device?)")
+        self._h = h
+
+    def _check(self, rc):
+        if rc:
+            raise SrayError(rc, lib().sray_gpu_last_error(self._h).decode(errors="replace"))
+
+    def upload(self, scene):
+        """scene: HostScene or SceneDesc pointer -> scene id"""
+        sid = C.c_uint32()
+        ptr = scene.desc_ptr() if isinstance(scene, HostScene) else scene
+        self._check(lib().sray_gpu_scene_upload(self._h, ptr, C.byref(sid)))
+        return sid.value
+
+    def release(self, sid):
+        self._check(lib().sray_gpu_scene_release(self._h, sid))
+
+    def scene_bytes(self, sid):
+        b = C.c_uint64()
+        self._check(lib().sray_gpu_scene_bytes(self._h, sid, C.byref(b)))
+        return b.value
+
+    def trace(self, sid, rays, any_hit=False, counts=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        cnt = TraceCounts()
+        self._check(lib().sray_gpu_trace(self._h, sid, rays.ctypes.data_as(C.POINTER(Ray)), len(rays),
+                                         hits.ctypes.data_as(C.POINTER(Hit)),
+                                         abi.SRAY_TRACE_ANY if any_hit else 0,
+                                         C.byref(cnt) if counts else None))
+        return (hits, cnt) if counts else hits
+
+    def trace_device(self, sid, d_rays, n, d_hits, any_hit=False):
+        self._check(lib().sray_gpu_trace_device(self._h, sid, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
+                                                abi.SRAY_TRACE_ANY if any_hit else 0))
+
+    def primary_rays_device(self, sid, params, sample, d_rays):
+        self._check(lib().sray_gpu_primary_rays_device(self._h, sid, C.byref(params), int(sample),
+                                                       C.c_void_p(d_rays)))
+
+    def render(self, sid, params):
+        w, h = params.x1 - params.x0, params.y1 - params.y0
+        out = np.zeros((h, w, 4), np.float32)
+        st = Stats()
+        self._check(lib().sray_gpu_render(self._h, sid, C.byref(params),
+                                          out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(st)))
+        return out, st
+
+    def render_device(self, sid, params, d_out):
+        st = Stats()
+        self._check(lib().sray_gpu_render_device(self._h, sid, C.byref(params), C.c_void_p(d_out), C.byref(st)))
+        return st
+
+    def set_profiling(self, on=True):
+        self._check(lib().sray_gpu_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self, reset=False):
+        """{kernel name: (launches, total_ms)} from in-library HIP events"""
+        n = C.c_int()
+        self._check(lib().sray_gpu_kernel_stats(self._h, None, 0, C.byref(n), 0))
+        arr = (abi.KernelStat * max(n.value, 1))()
+        self._check(lib().sray_gpu_kernel_stats(self._h, arr, n.value, C.byref(n), 1 if reset else 0))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
+
+    def close(self):
+        if self._h:
+            lib().sray_gpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,132 @@
+/*
+ * capi_host.cc — host-only part of the C ABI (no device needed): scene XML
+ * loading (Scene::load, src/scene.cc:75-183), octree inspection and the
+ * synthetic workload generators.
+ */
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+
+#include "host.h"
+
+struct sray_host_scene {
+    sray::HostScene hs;
+};
+
+static thread_local std::string g_host_err;
+
+extern "C" {
+
+const char *sray_host_last_error(void) { return g_host_err.c_str(); }
+
+int sray_scene_load_xml(const char *path, const char *xml_text, sray_host_scene **out)
+{
+    if (!out || (!path && !xml_text)) {
+        g_host_err = "sray_scene_load_xml: need a path or XML text";
+        return SRAY_ERR_ARG;
+    }
+    *out = nullptr;
+    std::string text;
+    if (xml_text) {
+        text = xml_text;
+    } else {
+        std::ifstream f(path, std::ios::binary);
+        if (!f) {
+            g_host_err = std::string("failed to open file: ") + path;
+            return SRAY_ERR_PARSE;
+        }
+        std::ostringstream ss;
+        ss << f.rdbuf();
+        text = ss.str();
+    }
+    auto *h = new sray_host_scene;
+    std::string err;
+    if (!sray::load_scene_xml(text, h->hs, err)) {
+        g_host_err = "failed to load scene: " + err;
+        delete h;
+        return SRAY_ERR_PARSE;
+    }
+    *out = h;
+    return SRAY_OK;
+}
+
+const sray_scene_desc *sray_scene_desc_of(const sray_host_scene *h) { return h ? &h->hs.desc : nullptr; }
+
+void sray_scene_free(sray_host_scene *h) { delete h; }
+
+int sray_scene_generate(int config, uint64_t seed, int variant, sray_host_scene **out)
+{
+    if (!out) return SRAY_ERR_ARG;
+    *out = nullptr;
+    auto *h = new sray_host_scene;
+    std::string err;
+    if (!sray::generate_scene(config, seed, variant, h->hs, err)) {
+        g_host_err = "generate: " + err;
+        delete h;
+        return SRAY_ERR_ARG;
+    }
+    *out = h;
+    return SRAY_OK;
+}
+
+int sray_scene_write_xml(const sray_host_scene *h, char *buf, size_t cap, size_t *len)
+{
+    if (!h || !len) return SRAY_ERR_ARG;
+    std::string s = sray::write_scene_xml(h->hs);
+    *len = s.size() + 1;
+    if (buf) {
+        if (cap < s.size() + 1) return SRAY_ERR_ARG;
+        memcpy(buf, s.c_str(), s.size() + 1);
+    }
+    return SRAY_OK;
+}
+
+int sray_octree_build_host(const sray_scene_desc *s, int level, sray_octree_info *info, float *node_lo,
+                           float *node_hi, uint32_t *items, uint32_t cap_nodes, uint32_t cap_items)
+{
+    if (!s || !info) return SRAY_ERR_ARG;
+    std::vector<float> boxes;
+    sray::FlatTree ft;
+    if (level < 0) {
+        sray::object_boxes(s, boxes);
+        sray::build_flat_octree(boxes, s->scene_oct_items, s->scene_oct_depth, 0, ft);
+    } else {
+        if ((uint32_t)level >= s->num_meshes) return SRAY_ERR_ARG;
+        sray::triangle_boxes(s, (uint32_t)level, boxes);
+        sray::build_flat_octree(boxes, s->mesh_oct_items, s->mesh_oct_depth, 0, ft);
+    }
+    info->num_nodes = (uint32_t)(ft.lo.size() / 4);
+    info->num_leaves = ft.num_leaves;
+    info->num_item_refs = (uint32_t)ft.items.size();
+    info->height = ft.height;
+    for (int k = 0; k < 3; k++) {
+        info->root_min[k] = ft.root_min[k];
+        info->root_max[k] = ft.root_max[k];
+    }
+    if (node_lo && node_hi && info->num_nodes <= cap_nodes) {
+        memcpy(node_lo, ft.lo.data(), ft.lo.size() * 4);
+        memcpy(node_hi, ft.hi.data(), ft.hi.size() * 4);
+    }
+    if (items && info->num_item_refs <= cap_items) memcpy(items, ft.items.data(), ft.items.size() * 4);
+    return SRAY_OK;
+}
+
+} /* extern "C" */
+
+extern "C" int sray_abi_sizes(uint32_t *out, int n)
+{
+    const uint32_t sz[] = {(uint32_t)sizeof(sray_object_desc),   (uint32_t)sizeof(sray_mesh_desc),
+                           (uint32_t)sizeof(sray_material_desc), (uint32_t)sizeof(sray_light_desc),
+                           (uint32_t)sizeof(sray_camera_desc),   (uint32_t)sizeof(sray_scene_desc),
+                           (uint32_t)sizeof(sray_ray),           (uint32_t)sizeof(sray_hit),
+                           (uint32_t)sizeof(sray_trace_counts),  (uint32_t)sizeof(sray_render_params),
+                           (uint32_t)sizeof(sray_stats),         (uint32_t)sizeof(sray_photon),
+                           (uint32_t)sizeof(sray_gather_query),  (uint32_t)sizeof(sray_request),
+                           (uint32_t)sizeof(sray_result),        (uint32_t)sizeof(sray_octree_info),
+                           (uint32_t)sizeof(sray_kernel_stat)};
+    int k = 0;
+    for (; k < n && k < (int)(sizeof sz / sizeof sz[0]); k++) out[k] = sz[k];
+    return k;
+}

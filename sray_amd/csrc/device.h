@@ -1,0 +1,161 @@
+/*
+ * device.h — device-side data layout and FP32 math of the gfx950 kernels.
+ *
+ * All arithmetic is IEEE binary32 with contraction off (the file is built
+ * with -ffp-contract=off and the pragma below), in the operation order the
+ * CPU restatement's float path uses, so hit IDs and spawned rays agree
+ * bit for bit:
+ *   dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z
+ *   point  M*p = ((m0*x + m1*y) + m2*z) + m3,  vector: (m0*x + m1*y) + m2*z
+ *   normal (M^-1)^T n = (i0*x + i4*y) + i8*z, ...
+ *   normalize = per-component true division by sqrt(dot(v,v))
+ */
+#ifndef SRAY_DEVICE_H_
+#define SRAY_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sray_gpu.h"
+
+#pragma clang fp contract(off)
+
+namespace srd {
+
+constexpr uint32_t LEAF = 0x80000000u;
+constexpr int STACK = 20;      /* per-lane traversal stack entries (LDS) */
+constexpr int TRACE_BLOCK = 128;
+constexpr int IOR_DEPTH = 4;   /* ior stack carried by queued rays */
+constexpr float EPS = (float)SRAY_ERROR_MARGIN;
+constexpr float XSMALL = (float)SRAY_XSMALL_NUMBER;
+
+/* object record: traversal reads the first 80 bytes */
+struct alignas(16) DevObj {
+    float inv[12];   /* world -> object */
+    float rad, start, end;
+    int32_t type;
+    int32_t mat;     /* material index (default material resolved on upload) */
+    uint32_t root;   /* mesh: root node of its octree */
+    uint32_t tri_base;
+    int32_t moving;
+    float xf[12];    /* object -> world */
+    float motion[3];
+    int32_t mt0;
+    int32_t mt1;
+    int32_t mesh;
+    int32_t has_attr; /* mesh: 1 normals, 2 texcoords, 4 tangents */
+    int32_t pad;
+};
+
+struct alignas(16) DevMat {
+    float4 kd, ks;
+    float shininess, reflect, refract, ior;
+    uint32_t flags, pad0, pad1, pad2;
+};
+
+struct alignas(16) DevLight {
+    int32_t type;
+    float pos[3];
+    float color[4];
+    float radius;
+    float dim[3];
+    float xf[12];
+};
+
+struct SceneDev {
+    const float4 *nlo;
+    const float4 *nhi;
+    const uint32_t *items;
+    const DevObj *objs;
+    const float4 *tri;       /* 3 per triangle: v0, v1, v2 (w unused) */
+    const float *tri_n;      /* 9 per triangle or null */
+    const float *tri_g;      /* 9 per triangle or null */
+    const float *tri_uv;     /* 6 per triangle or null */
+    const DevMat *mats;
+    const DevLight *lights;
+    uint32_t scene_root;
+    int32_t nlights;
+    int32_t nobjs;
+    int32_t pad;
+    float4 env_ambient;
+    float4 env_color;
+    float cam[12];
+    float fz;                /* 1/tan(vfov/2), from double on the host */
+    float max_dist;
+    int32_t shutter;
+    float gather_abs;        /* gather radius (absolute) */
+};
+
+/* ------------------------------------------------------------------ */
+/* small vector helpers (explicit order, no contraction)                */
+/* ------------------------------------------------------------------ */
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 mul(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 dvs(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ V3 neg(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b)
+{
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalized(V3 a)
+{
+    float l = length(a);
+    return V3{a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ V3 reflect(V3 d, V3 n)
+{
+    float s = 2.0f * dot(d, n);
+    return V3{d.x - n.x * s, d.y - n.y * s, d.z - n.z * s};
+}
+__device__ __forceinline__ V3 xf_point(const float *m, V3 p)
+{
+    return V3{m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+}
+__device__ __forceinline__ V3 xf_vec(const float *m, V3 v)
+{
+    return V3{m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+              m[8] * v.x + m[9] * v.y + m[10] * v.z};
+}
+__device__ __forceinline__ V3 xf_normal(const float *i, V3 n)
+{
+    return V3{i[0] * n.x + i[4] * n.y + i[8] * n.z, i[1] * n.x + i[5] * n.y + i[9] * n.z,
+              i[2] * n.x + i[6] * n.y + i[10] * n.z};
+}
+
+/* ------------------------------------------------------------------ */
+/* counter-based RNG: SplitMix64 chain (same definition as the checker) */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint32_t rng_u32(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    uint64_t h = splitmix64(seed);
+    h = splitmix64(h ^ (uint64_t)a);
+    h = splitmix64(h ^ (((uint64_t)b << 32) | (uint64_t)c));
+    h = splitmix64(h ^ (uint64_t)d);
+    return (uint32_t)(h >> 32);
+}
+__device__ __forceinline__ float rng_unit(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    return (float)(rng_u32(seed, a, b, c, d) >> 8) * (1.0f / 16777216.0f);
+}
+constexpr uint32_t SLOT_CAMERA = 0u;
+constexpr uint32_t SLOT_LIGHT = 1u << 20;
+constexpr uint32_t SLOT_DIFFUSE = 2u << 20;
+
+} // namespace srd
+
+#endif

@@ -1,0 +1,124 @@
+/*
+ * host.h — host-side C++ of the S-ray MI355X path (internal).
+ *
+ * What the reference keeps on the host stays here, in C++: the scene XML
+ * loader (Scene::load, src/scene.cc:75-183), the octree build of both levels
+ * (src/octree.inl:100-191, src/scene.cc:353-390, src/mesh.cc:210-228) and its
+ * flattening into the device node arrays, plus the synthetic scene
+ * generators used by tests and the benchmark.
+ */
+#ifndef SRAY_HOST_H_
+#define SRAY_HOST_H_
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sray_gpu.h"
+
+namespace sray {
+
+/* ------------------------------------------------------------------ */
+/* an owning flat scene: the vectors behind an sray_scene_desc           */
+/* ------------------------------------------------------------------ */
+struct HostScene {
+    std::vector<sray_object_desc> objects;
+    std::vector<sray_mesh_desc> meshes;
+    std::vector<float> tri_pos, tri_normal, tri_tangent, tri_texcoord;
+    std::vector<sray_material_desc> materials;
+    std::vector<std::string> material_names;
+    std::vector<sray_light_desc> lights;
+    sray_camera_desc camera{};
+    bool cam_has_target = false; /* target camera: position + target point */
+    float cam_pos[3] = {0, 0, 0}, cam_target[3] = {0, 0, 0};
+    float env_ambient[4] = {0, 0, 0, 0};
+    float env_color[4] = {0, 0, 0, 0};
+    float env_ior = 1.0f;
+    int scene_oct_depth = 5, scene_oct_items = 5;
+    int mesh_oct_depth = 7, mesh_oct_items = 14;
+    int frame_t0 = 0, frame_t1 = 0;
+    sray_scene_desc desc{};
+
+    /* refresh desc's pointers/counts from the vectors */
+    const sray_scene_desc *finish();
+};
+
+/* TargetCamera::get_matrix (src/camera.cc:216-234), double precision */
+void target_camera_matrix(const double pos[3], const double target[3], double roll, double m[12]);
+
+/* default camera: Camera() (src/camera.cc:29-35) with the identity matrix */
+sray_camera_desc default_camera();
+
+/* row-major 3x4 helpers (double precision, host) */
+void mat_identity(double m[12]);
+void mat_mul(const double a[12], const double b[12], double out[12]);
+bool mat_inverse(const double m[12], double out[12]);
+void mat_to_float(const double m[12], float out[12]);
+
+/* ------------------------------------------------------------------ */
+/* octree: build (reference rules) + flatten (device layout)             */
+/* ------------------------------------------------------------------ */
+/* Device node layout, two float4 arrays ("lo" and "hi"):
+ *   lo = (min.x, min.y, min.z, bits(first))   hi = (max.x, max.y, max.z, bits(meta))
+ *   inner node: first = index of its first child, meta = child mask (bit k =
+ *               child whose octant bits (x | y<<1 | z<<2) equal k is stored;
+ *               children are contiguous in increasing k)
+ *   leaf:       first = first item index, meta = SR_LEAF | item count
+ * Item ranges are assigned in the reference's depth-first order (children
+ * 0..7 of src/octree.inl:161-169), so an item's position in the item array
+ * is its occurrence rank — the tie-break key of src/octree.inl:211-234.
+ * Empty leaves are not stored. */
+constexpr uint32_t SR_LEAF = 0x80000000u;
+
+struct FlatTree {
+    std::vector<float> lo, hi; /* 4 floats per node */
+    std::vector<uint32_t> items;
+    uint32_t height = 0;
+    uint32_t num_leaves = 0;
+    float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
+};
+
+/* Builds an octree over item boxes (6 floats per item: min xyz, max xyz)
+ * with the reference rules and appends its flattened form to out (node
+ * indices offset by out's current node count, item values = item_base +
+ * local item index).  Returns the root node index. */
+uint32_t build_flat_octree(const std::vector<float> &boxes, int max_items, int max_depth, uint32_t item_base,
+                           FlatTree &out);
+
+/* item boxes in float, same arithmetic as the device:
+ * scene level = union of the object's bounds at t0 and t1 */
+void object_boxes(const sray_scene_desc *s, std::vector<float> &boxes);
+void triangle_boxes(const sray_scene_desc *s, uint32_t mesh, std::vector<float> &boxes);
+
+/* ------------------------------------------------------------------ */
+/* XML                                                                  */
+/* ------------------------------------------------------------------ */
+/* attribute typing of src/xmltree.c:51-99 */
+enum { ATYPE_STR, ATYPE_INT, ATYPE_FLT, ATYPE_VEC };
+struct XmlAttr {
+    std::string name, str;
+    int type = ATYPE_STR;
+    int ival = 0;
+    float fval = 0;
+    float vval[4] = {0, 0, 0, 0};
+};
+struct XmlNode {
+    std::string name;
+    std::vector<XmlAttr> attr;
+    std::vector<XmlNode> chld;
+    const XmlAttr *get(const char *n) const;
+};
+bool xml_parse(const std::string &text, XmlNode &root, std::string &err);
+XmlAttr xml_make_attr(const std::string &name, const std::string &val);
+
+bool load_scene_xml(const std::string &text, HostScene &hs, std::string &err);
+std::string write_scene_xml(const HostScene &hs);
+
+/* ------------------------------------------------------------------ */
+/* synthetic workloads (seeded)                                         */
+/* ------------------------------------------------------------------ */
+bool generate_scene(int config, uint64_t seed, int variant, HostScene &hs, std::string &err);
+
+} // namespace sray
+
+#endif

@@ -1,0 +1,1529 @@
+/*
+ * sray_gpu.hip — gfx950 kernels and the C ABI runtime of the S-ray hot path.
+ *
+ * One context = one HIP stream (replaces the reference's pthread tile pool,
+ * src/tpool.cc:48-225).  Scenes are built on the host (octrees of both
+ * levels, src/octree.inl:100-191) and kept resident in HBM; the render is a
+ * wavefront of bounce levels (render.h).  Built with -ffp-contract=off so
+ * every float operation is the IEEE one the CPU checker performs.
+ */
+#include <hip/hip_runtime.h>
+#include <hipcub/device/device_select.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "host.h"
+#include "render.h"
+
+using namespace srd;
+
+/* ================================================================== */
+/* kernels                                                              */
+/* ================================================================== */
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneDev S, const sray_ray *__restrict__ rays, uint32_t n,
+                                                          sray_hit *__restrict__ hits,
+                                                          unsigned long long *__restrict__ counts)
+{
+    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        sray_ray r = rays[i];
+        HitRec h = traverse<ANY, COUNT>(S, v3(r.origin[0], r.origin[1], r.origin[2]), v3(r.dir[0], r.dir[1], r.dir[2]),
+                                        r.time, &stk[threadIdx.x], TRACE_BLOCK, cnt);
+        sray_hit o;
+        o.t = h.t;
+        o.obj = h.obj;
+        o.prim = (h.obj >= 0 && h.tri != 0xffffffffu) ? (int32_t)(h.tri - S.objs[h.obj].tri_base) : -1;
+        o.rank = h.obj >= 0 ? h.rank : 0u;
+        if (h.obj < 0) {
+            o.t = FLT_MAX;
+            o.rank = 0;
+        }
+        hits[i] = o;
+    }
+    if (COUNT) {
+        atomicAdd(&counts[0], (unsigned long long)cnt.nodes);
+        atomicAdd(&counts[1], (unsigned long long)cnt.spheres);
+        atomicAdd(&counts[2], (unsigned long long)cnt.cylinders);
+        atomicAdd(&counts[3], (unsigned long long)cnt.triangles);
+        atomicAdd(&counts[4], (unsigned long long)cnt.item_refs);
+        atomicAdd(&counts[5], (unsigned long long)cnt.mesh_enters);
+    }
+}
+
+/* Camera::get_primary_ray, src/camera.cc:119-142 */
+__device__ __forceinline__ void primary_ray(const SceneDev &S, const RenderDev &P, int x, int y, int sample, V3 &o,
+                                            V3 &d, int &time)
+{
+    float ysz = 2.0f;
+    float xsz = P.aspect * ysz;
+    float px = ((float)x / (float)P.width) * xsz - xsz / 2.0f;
+    float py = 1.0f - ((float)y / (float)P.height) * ysz;
+    uint32_t pix = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
+    if (sample > 0) {
+        float f0 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, SLOT_CAMERA | 0u);
+        float f1 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, SLOT_CAMERA | 1u);
+        px = px + xsz * (f0 - 0.5f) / (float)P.width;
+        py = py + ysz * (f1 - 0.5f) / (float)P.height;
+    }
+    V3 dl = v3(px * S.max_dist, py * S.max_dist, S.fz * S.max_dist);
+    time = P.frame_time;
+    if (P.mblur && sample > 0 && S.shutter > 0) {
+        uint32_t u = rng_u32(P.seed, pix, (uint32_t)sample, 0u, SLOT_CAMERA | 2u);
+        time += (int)(u % (uint32_t)S.shutter) - S.shutter / 2;
+    }
+    o = xf_point(S.cam, v3(0.0f, 0.0f, 0.0f));
+    d = xf_vec(S.cam, dl);
+}
+
+__global__ void k_primary(SceneDev S, RenderDev P, int sample, sray_ray *out, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x = P.x0 + (int)(i % (uint32_t)P.W), y = P.y0 + (int)(i / (uint32_t)P.W);
+    V3 o, d;
+    int time;
+    primary_ray(S, P, x, y, sample, o, d, time);
+    sray_ray r;
+    r.origin[0] = o.x;
+    r.origin[1] = o.y;
+    r.origin[2] = o.z;
+    r.dir[0] = d.x;
+    r.dir[1] = d.y;
+    r.dir[2] = d.z;
+    r.time = time;
+    r.pad_ = 0;
+    out[i] = r;
+}
+
+__global__ void k_raygen(SceneDev S, RenderDev P, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m,
+                         int sample, QRay *__restrict__ q)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint32_t pr = active[c0 + i];
+    int x = P.x0 + (int)(pr % (uint32_t)P.W), y = P.y0 + (int)(pr / (uint32_t)P.W);
+    V3 o, d;
+    int time;
+    primary_ray(S, P, x, y, sample, o, d, time);
+    QRay r;
+    r.o = make_float4(o.x, o.y, o.z, __int_as_float(time));
+    r.d = make_float4(d.x, d.y, d.z, 1.0f);
+    r.w = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    r.m = make_uint4(i, 1u, (uint32_t)P.iter, 0u);
+    r.stk = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q[i] = r;
+}
+
+template <bool COUNT> __device__ __forceinline__ void flush_counts(const TraceCounts &cnt, unsigned long long *c)
+{
+    if (!COUNT) return;
+    atomicAdd(&c[0], (unsigned long long)cnt.nodes);
+    atomicAdd(&c[1], (unsigned long long)cnt.spheres);
+    atomicAdd(&c[2], (unsigned long long)cnt.cylinders);
+    atomicAdd(&c[3], (unsigned long long)cnt.triangles);
+    atomicAdd(&c[4], (unsigned long long)cnt.item_refs);
+    atomicAdd(&c[5], (unsigned long long)cnt.mesh_enters);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_q(SceneDev S, const QRay *__restrict__ q, uint32_t n,
+                                                         int4 *__restrict__ hits, unsigned long long *counts)
+{
+    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 o = q[i].o, d = q[i].d;
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    HitRec h = traverse<false, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
+                                      TRACE_BLOCK, cnt);
+    hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+    flush_counts<COUNT>(cnt, counts);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_shadow(SceneDev S, const SRay *__restrict__ sq, uint32_t n,
+                                                              uint8_t *__restrict__ occ, unsigned long long *counts)
+{
+    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 d = sq[i].d;
+    if (d.w == 0.0f) {
+        occ[i] = 0;
+        return;
+    }
+    float4 o = sq[i].o;
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
+                                     TRACE_BLOCK, cnt);
+    occ[i] = h.obj >= 0 ? 1 : 0;
+    flush_counts<COUNT>(cnt, counts);
+}
+
+/* light sample point: Light::get_point src/light.cc:106-109, SphLight
+ * :272-275, BoxLight :346-354 (matrix at time 0) */
+__device__ __forceinline__ V3 sph_rand(uint64_t seed, uint32_t pix, uint32_t sample, uint32_t path, uint32_t base,
+                                       float r)
+{
+    /* Marsaglia (1972): uniform on the sphere from a rejection pair */
+    for (uint32_t a = 0; a < 64; a++) {
+        float u1 = rng_unit(seed, pix, sample, path, base ^ (0x800u | (a << 4) | 0u));
+        float u2 = rng_unit(seed, pix, sample, path, base ^ (0x800u | (a << 4) | 1u));
+        float x1 = 2.0f * u1 - 1.0f, x2 = 2.0f * u2 - 1.0f;
+        float s = x1 * x1 + x2 * x2;
+        if (s < 1.0f && s > 0.0f) {
+            float qq = 2.0f * sqrtf(1.0f - s);
+            V3 d = v3(x1 * qq, x2 * qq, 1.0f - 2.0f * s);
+            return mul(d, r);
+        }
+    }
+    return v3(0.0f, 0.0f, r);
+}
+
+__device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uint32_t pix, uint32_t sample,
+                                          uint32_t path, uint32_t li, uint32_t si)
+{
+    uint32_t base = SLOT_LIGHT | (li << 12) | (si << 2);
+    if (lt.type == SRAY_LIGHT_BOX) {
+        float fx = rng_unit(seed, pix, sample, path, base | 0u);
+        float fy = rng_unit(seed, pix, sample, path, base | 1u);
+        float fz = rng_unit(seed, pix, sample, path, base | 2u);
+        V3 v = v3(fx * lt.dim[0] - 0.5f * lt.dim[0], fy * lt.dim[1] - 0.5f * lt.dim[1],
+                  fz * lt.dim[2] - 0.5f * lt.dim[2]);
+        return xf_point(lt.xf, v);
+    }
+    if (lt.type == SRAY_LIGHT_SPHERE) {
+        V3 s = sph_rand(seed, pix, sample, path, base, lt.radius);
+        return add(v3(lt.pos[0], lt.pos[1], lt.pos[2]), s);
+    }
+    return v3(lt.pos[0], lt.pos[1], lt.pos[2]);
+}
+
+/* interp_face, src/mesh.cc:321-445 (inverse bilinear u,v; lerped attributes) */
+__device__ void interp_face(const float4 *tri, const float *tn, const float *tg, const float *tuv, uint32_t ti, V3 ispos,
+                            V3 normal, V3 &on, V3 &og, float &ou, float &ov)
+{
+    float4 A = tri[(size_t)ti * 3], B = tri[(size_t)ti * 3 + 1], C = tri[(size_t)ti * 3 + 2];
+    V3 p00 = v3(A.x, A.y, A.z), p10 = v3(B.x, B.y, B.z), p01 = v3(C.x, C.y, C.z), p11 = p01;
+    V3 pa = add(sub(p00, p10), sub(p11, p01));
+    V3 pb = sub(p10, p00);
+    V3 pc = sub(p01, p00);
+    V3 pd = p00;
+    V3 na = cross(pa, normal), nb = cross(pb, normal), nc = cross(pc, normal);
+    float du0 = dot(nc, pd);
+    float du1 = dot(na, pd) + dot(nc, pb);
+    float du2 = dot(na, pb);
+    float dv0 = dot(nb, pd);
+    float dv1 = dot(na, pd) + dot(nb, pc);
+    float dv2 = dot(na, pc);
+    float ru, rv;
+    if (fabsf(du2) <= XSMALL) {
+        float divisor = du1 - dot(na, ispos);
+        ru = fabsf(divisor) > XSMALL ? ((dot(nc, ispos) - du0) / divisor) : 0.0f;
+    } else {
+        V3 qux = dvs(na, 2.0f * du2);
+        V3 quy = dvs(neg(nc), du2);
+        float dux = -du1 / (2.0f * du2);
+        float duy = du0 / du2;
+        float ka = dux + dot(qux, ispos);
+        float kb = duy + dot(quy, ispos);
+        float u0 = ka - sqrtf(ka * ka - kb);
+        float u1 = ka + sqrtf(ka * ka - kb);
+        ru = (u0 < 0.0f || u0 > 1.0f) ? u1 : u0;
+    }
+    if (fabsf(dv2) <= XSMALL) {
+        float divisor = dv1 - dot(na, ispos);
+        rv = fabsf(divisor) > XSMALL ? ((dot(nb, ispos) - dv0) / divisor) : 0.0f;
+    } else {
+        V3 qvx = dvs(na, 2.0f * dv2);
+        V3 qvy = dvs(neg(nb), dv2);
+        float dvx = -dv1 / (2.0f * dv2);
+        float dvy = dv0 / dv2;
+        float ka = dvx + dot(qvx, ispos);
+        float kb = dvy + dot(qvy, ispos);
+        float v0 = ka - sqrtf(ka * ka - kb);
+        float v1 = ka + sqrtf(ka * ka - kb);
+        rv = (v0 < 0.0f || v0 > 1.0f) ? v1 : v0;
+    }
+    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+    auto l2 = [&](float a00, float a10, float a01) {
+        float s0 = lerp(a00, a01, rv);
+        float s1 = lerp(a10, a01, rv);
+        return lerp(s0, s1, ru);
+    };
+    if (tuv) {
+        const float *t = tuv + (size_t)ti * 6;
+        ou = l2(t[0], t[2], t[4]);
+        ov = l2(t[1], t[3], t[5]);
+    } else {
+        ou = ov = 0.0f;
+    }
+    if (tn) {
+        const float *n = tn + (size_t)ti * 9;
+        on = v3(l2(n[0], n[3], n[6]), l2(n[1], n[4], n[7]), l2(n[2], n[5], n[8]));
+    } else on = v3(0.0f, 0.0f, 0.0f);
+    if (tg) {
+        const float *g = tg + (size_t)ti * 9;
+        og = v3(l2(g[0], g[3], g[6]), l2(g[1], g[4], g[7]), l2(g[2], g[5], g[8]));
+    } else og = v3(0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ float fresnel(float r, float cosa)
+{
+    float inv = 1.0f - cosa;
+    float inv_sq = inv * inv;
+    return r + (1.0f - r) * inv_sq * inv_sq * inv;
+}
+
+/* refraction of the unit incident d about n; total internal reflection
+ * returns the mirror direction (libvmath Vector3::refraction, fixed here) */
+__device__ __forceinline__ V3 refract(V3 d, V3 n, float ior)
+{
+    float cos_inc = -dot(d, n);
+    float radical = 1.0f + ior * ior * (cos_inc * cos_inc - 1.0f);
+    if (radical < 0.0f) return reflect(d, n);
+    float beta = ior * cos_inc - sqrtf(radical);
+    return v3(d.x * ior + n.x * beta, d.y * ior + n.y * beta, d.z * ior + n.z * beta);
+}
+
+/* per-ray ior stack (vmath Ray::enter/leave/calc_ior) */
+struct IorStack {
+    float cur;
+    float s[IOR_DEPTH];
+    uint32_t n;
+    bool overflow;
+    __device__ void enter(float x)
+    {
+        if (n < (uint32_t)IOR_DEPTH) s[n++] = cur;
+        else overflow = true;
+        cur = x;
+    }
+    __device__ void leave() { cur = n > 0 ? s[--n] : 1.0f; }
+    __device__ float calc(bool entering, float mat_ior) const
+    {
+        if (entering) return cur / mat_ior;
+        float to = n > 0 ? s[n - 1] : 1.0f;
+        return cur / to;
+    }
+};
+
+__device__ __forceinline__ uint32_t pixel_id(const RenderDev &P, const uint32_t *active, uint32_t c0, uint32_t slot)
+{
+    uint32_t pr = active[c0 + slot];
+    uint32_t x = (uint32_t)P.x0 + pr % (uint32_t)P.W, y = (uint32_t)P.y0 + pr / (uint32_t)P.W;
+    return y * (uint32_t)P.width + x;
+}
+
+/* shade_phong (src/shader.cc:83-247) / shade_undef (:36-71) for one hit,
+ * emitting direct terms, shadow rays and children */
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, const uint32_t *__restrict__ active,
+                                               uint32_t c0, int sample, const QRay *__restrict__ q,
+                                               const int4 *__restrict__ hits, uint32_t n, float4 *__restrict__ direct,
+                                               SRay *__restrict__ sq, QRay *__restrict__ kids,
+                                               uint8_t *__restrict__ kflag, unsigned long long *__restrict__ stats,
+                                               uint32_t *__restrict__ err)
+{
+    __shared__ uint32_t bcnt[5];
+    if (threadIdx.x < 5) bcnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
+    if (i < n) {
+        QRay R = q[i];
+        int4 h = hits[i];
+        uint32_t slot = R.m.x, path = R.m.y, flg = R.m.w >> 8;
+        int iter = (int)R.m.z;
+        V3 W = v3(R.w.x, R.w.y, R.w.z);
+        float energy = R.d.w;
+        int time = __float_as_int(R.o.w);
+        V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
+        float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int l = 0; l < P.L; l++) sq[(size_t)i * P.L + l].d.w = 0.0f;
+        for (int k = 0; k < P.K; k++) kflag[(size_t)i * P.K + k] = 0;
+        if (h.y < 0) {
+            /* miss: env_color (src/scene.cc:568) */
+            dir.x = W.x * S.env_color.x;
+            dir.y = W.y * S.env_color.y;
+            dir.z = W.z * S.env_color.z;
+            if (path == 1u) dir.w = S.env_color.w;
+            direct[i] = dir;
+        } else {
+            float t = __int_as_float(h.x);
+            const DevObj &ob = S.objs[h.y];
+            float I[12], M[12];
+            obj_inv_at(ob, time, I);
+            obj_xf_at(ob, time, M);
+            V3 lo = xf_point(I, ro), ld = xf_vec(I, rd);
+            V3 p = add(lo, mul(ld, t));
+            V3 N;
+            float tu = 0.0f, tv = 0.0f;
+            const DevMat &mat = S.mats[ob.mat];
+            bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
+            if (ob.type == SRAY_OBJ_SPHERE) {
+                V3 nrm = dvs(p, ob.rad);
+                N = xf_normal(I, nrm);
+                if (!phong) {
+                    float r = length(p);
+                    tu = (atan2f(p.z, p.x) + 3.14159265358979f) / 6.28318530717959f;
+                    tv = r > 0.0f ? acosf(fmaxf(-1.0f, fminf(1.0f, p.y / r))) / 3.14159265358979f : 0.0f;
+                }
+            } else if (ob.type == SRAY_OBJ_CYLINDER) {
+                V3 nrm = dvs(v3(p.x, 0.0f * p.y, p.z), ob.rad);
+                N = xf_normal(I, nrm);
+                if (!phong) {
+                    tu = (atan2f(p.z, p.x) + 3.14159265358979f) / 6.28318530717959f;
+                    tv = p.y;
+                }
+            } else {
+                uint32_t ti = (uint32_t)h.z;
+                float4 A = S.tri[(size_t)ti * 3], B = S.tri[(size_t)ti * 3 + 1], C = S.tri[(size_t)ti * 3 + 2];
+                V3 e1 = v3(B.x - A.x, B.y - A.y, B.z - A.z), e2 = v3(C.x - A.x, C.y - A.y, C.z - A.z);
+                V3 gn = cross(e1, e2);
+                V3 nrm = gn;
+                if (ob.has_attr) {
+                    V3 in, ig;
+                    interp_face(S.tri, (ob.has_attr & 1) ? S.tri_n : nullptr, (ob.has_attr & 4) ? S.tri_g : nullptr,
+                                (ob.has_attr & 2) ? S.tri_uv : nullptr, ti, p, gn, in, ig, tu, tv);
+                    nrm = dot(in, in) < XSMALL ? gn : in;
+                }
+                N = normalized(xf_normal(I, nrm));
+            }
+            V3 Pw = xf_point(M, p);
+            if (path == 1u) dir.w = 1.0f;
+            uint32_t pix = pixel_id(P, active, c0, slot);
+            int nl = S.nlights;
+            if (!phong) {
+                /* shade_undef: checker texture from texcoords */
+                int cx = (int)(tu * 20.0f), cy = (int)(tv * 10.0f);
+                V3 tex = ((cx % 2) == (cy % 2)) ? v3(1.0f, 0.3f, 0.2f) : v3(0.2f, 0.3f, 1.0f);
+                dir.x = W.x * (0.1f * tex.x);
+                dir.y = W.y * (0.1f * tex.y);
+                dir.z = W.z * (0.1f * tex.z);
+                for (int l = 0; l < nl; l++) {
+                    const DevLight &lt = S.lights[l];
+                    V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u);
+                    V3 ldir = sub(lpos, Pw);
+                    float ldist = length(ldir);
+                    V3 sd = ldir;
+                    ldir = dvs(ldir, ldist);
+                    V3 vdir = neg(normalized(rd));
+                    V3 hdir = normalized(add(ldir, vdir));
+                    float ndotl = fmaxf(dot(N, ldir), 0.0f);
+                    float ndoth = fmaxf(dot(N, hdir), 0.0f);
+                    float sp = powf(ndoth, 50.0f);
+                    SRay s;
+                    s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
+                    s.d = make_float4(sd.x, sd.y, sd.z, 1.0f);
+                    s.c = make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
+                                      W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
+                                      W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp), 0.0f);
+                    sq[(size_t)i * P.L + l] = s;
+                    c_sh++;
+                }
+                direct[i] = dir;
+            } else {
+                bool entering;
+                V3 normal;
+                if (dot(rd, N) > 0.0f) {
+                    normal = neg(N);
+                    entering = false;
+                } else {
+                    normal = N;
+                    entering = true;
+                }
+                float ray_mag = length(rd);
+                V3 incident = dvs(rd, ray_mag);
+                float4 kd = mat.kd;
+                float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                if (flg & QF_DIFFUSE) {
+                    /* diffuse GI ray: global-map radiance estimate (gathered by a
+                     * separate pass; zero when no map is bound) */
+                    direct[i] = dir;
+                } else {
+                    float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
+                    float trans_fact = (mat.flags & SRAY_MAT_HAS_REFRACT) ? mat.refract : 0.0f;
+                    float mat_ior = mat.ior;
+                    IorStack st;
+                    st.cur = R.w.w;
+                    st.s[0] = R.stk.x;
+                    st.s[1] = R.stk.y;
+                    st.s[2] = R.stk.z;
+                    st.s[3] = R.stk.w;
+                    st.n = R.m.w & 0xffu;
+                    st.overflow = false;
+                    float ior = st.calc(entering, mat_ior);
+                    /* ambient: (env_ambient * kd) * kd, src/shader.cc:142,246 */
+                    dir.x = W.x * ((S.env_ambient.x * kd.x) * kd.x);
+                    dir.y = W.y * ((S.env_ambient.y * kd.y) * kd.y);
+                    dir.z = W.z * ((S.env_ambient.z * kd.z) * kd.z);
+                    int slotk = 0;
+                    V3 vref = reflect(incident, normal);
+                    for (int l = 0; l < nl; l++) {
+                        const DevLight &lt = S.lights[l];
+                        int ns = lt.type != SRAY_LIGHT_POINT ? P.shadow_samples : 1;
+                        float scale = ns > 1 ? 1.0f / (float)ns : 1.0f;
+                        for (int si = 0; si < ns; si++) {
+                            V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si);
+                            V3 ldir = sub(lpos, Pw);
+                            float ldist = length(ldir);
+                            V3 sd = ldir;
+                            ldir = dvs(ldir, ldist);
+                            float ndotl = fmaxf(dot(normal, ldir), 0.0f);
+                            float rdotl = fmaxf(dot(vref, ldir), 0.0f);
+                            float sp = powf(rdotl, mat.shininess);
+                            SRay s;
+                            s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
+                            s.d = make_float4(sd.x, sd.y, sd.z, 1.0f);
+                            s.c = make_float4(
+                                W.x * ((kd.x * (lt.color[0] * ndotl) + ks.x * (lt.color[0] * sp)) * scale),
+                                W.y * ((kd.y * (lt.color[1] * ndotl) + ks.y * (lt.color[1] * sp)) * scale),
+                                W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale), 0.0f);
+                            sq[(size_t)i * P.L + slotk] = s;
+                            slotk++;
+                            c_sh++;
+                        }
+                    }
+                    V3 nrd = dvs(neg(rd), ray_mag);
+                    float ray_dot_n = dot(nrd, normal);
+                    float sf0 = trans_fact > 0.0f ? (ior - 1.0f) / (ior + 1.0f) : 1.0f;
+                    float fres = fresnel(sf0 * sf0, ray_dot_n);
+                    float refl_energy = refl_fact * energy;
+                    if (refl_energy > P.min_energy && iter > 0) {
+                        V3 nd = reflect(rd, normal);
+                        QRay c;
+                        c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                        c.d = make_float4(nd.x, nd.y, nd.z, refl_energy);
+                        float wf = fres * refl_fact;
+                        c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), R.w.w);
+                        c.m = make_uint4(slot, path * 2u, (uint32_t)(iter - 1), R.m.w);
+                        c.stk = R.stk;
+                        kids[(size_t)i * P.K + 0] = c;
+                        kflag[(size_t)i * P.K + 0] = 1;
+                        c_rl++;
+                    }
+                    float trans_energy = trans_fact * energy;
+                    if (trans_energy > P.min_energy && iter > 0) {
+                        IorStack ts = st;
+                        if (entering) ts.enter(mat_ior);
+                        else ts.leave();
+                        V3 td = mul(refract(dvs(rd, ray_mag), normal, ior), ray_mag);
+                        if (dot(td, normal) > 0.0f) {
+                            if (entering) ts.leave();
+                            else ts.enter(mat_ior);
+                        }
+                        if (ts.overflow) atomicOr(err, 1u);
+                        QRay c;
+                        c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                        c.d = make_float4(td.x, td.y, td.z, trans_energy);
+                        float wf = (1.0f - fres) * trans_fact;
+                        c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), ts.cur);
+                        c.m = make_uint4(slot, path * 2u + 1u, (uint32_t)(iter - 1), ts.n | (flg << 8));
+                        c.stk = make_float4(ts.s[0], ts.s[1], ts.s[2], ts.s[3]);
+                        kids[(size_t)i * P.K + 1] = c;
+                        kflag[(size_t)i * P.K + 1] = 1;
+                        c_rr++;
+                    }
+                    direct[i] = dir;
+                }
+            }
+        }
+    }
+    if (c_sh) atomicAdd(&bcnt[0], c_sh);
+    if (c_rl) atomicAdd(&bcnt[1], c_rl);
+    if (c_rr) atomicAdd(&bcnt[2], c_rr);
+    if (c_df) atomicAdd(&bcnt[3], c_df);
+    __syncthreads();
+    if (threadIdx.x < 4 && bcnt[threadIdx.x]) atomicAdd(&stats[1 + threadIdx.x], (unsigned long long)bcnt[threadIdx.x]);
+}
+
+/* per-sample accumulation: the first ray of each run of equal sample slots
+ * sums the run (queues are ordered by slot) */
+__global__ void k_accum(const QRay *__restrict__ q, const float4 *__restrict__ direct, const SRay *__restrict__ sq,
+                        const uint8_t *__restrict__ occ, uint32_t n, int L, float4 *__restrict__ acc)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t slot = q[i].m.x;
+    if (i > 0 && q[i - 1].m.x == slot) return;
+    float4 s = acc[slot];
+    for (uint32_t j = i; j < n && q[j].m.x == slot; j++) {
+        float4 d = direct[j];
+        s.x = s.x + d.x;
+        s.y = s.y + d.y;
+        s.z = s.z + d.z;
+        s.w = s.w + d.w;
+        for (int l = 0; l < L; l++) {
+            size_t k = (size_t)j * L + l;
+            if (sq[k].d.w != 0.0f && !occ[k]) {
+                float4 c = sq[k].c;
+                s.x = s.x + c.x;
+                s.y = s.y + c.y;
+                s.z = s.z + c.z;
+            }
+        }
+    }
+    acc[slot] = s;
+}
+
+__global__ void k_fill_f4(float4 *p, size_t n, float4 v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_iota(uint32_t *p, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+/* end of one sample pass: framebuffer accumulation in float
+ * (src/render.cc:226-229) and the variance stop test (:232-237, 261-270) */
+__global__ void k_pass_finish(RenderDev P, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m, int sample,
+                              const float4 *__restrict__ acc, float4 *__restrict__ fb, float4 *__restrict__ subpix,
+                              int *__restrict__ nsamp, uint8_t *__restrict__ keep)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint32_t pr = active[c0 + i];
+    float4 c = acc[i];
+    float4 f = fb[pr];
+    f.x = f.x + c.x;
+    f.y = f.y + c.y;
+    f.z = f.z + c.z;
+    f.w = f.w + c.w;
+    fb[pr] = f;
+    int ns = sample + 1;
+    nsamp[pr] = ns;
+    int maxs = P.max_samples;
+    bool k = ns < maxs;
+    if (maxs > 1) {
+        subpix[(size_t)pr * maxs + sample] = c;
+        if (ns >= P.min_samples && ns > 1) {
+            float rcp = 1.0f / (float)ns;
+            float4 mean = make_float4(f.x * rcp, f.y * rcp, f.z * rcp, f.w * rcp);
+            float sq = 0.0f;
+            for (int j = 0; j < ns; j++) {
+                float4 s = subpix[(size_t)pr * maxs + j];
+                float dx = mean.x - s.x, dy = mean.y - s.y, dz = mean.z - s.z, dw = mean.w - s.w;
+                float l2 = dx * dx + dy * dy + dz * dz + dw * dw;
+                sq = sq + l2;
+            }
+            float v = sq * rcp;
+            if (v < P.max_var) k = false;
+        }
+    }
+    keep[c0 + i] = k ? 1 : 0;
+}
+
+/* final pixel value: divide all channels by the sample count, or the
+ * reference's as-written img[x] *= rcp_lut[i] (src/render.cc:240, Q1) */
+__global__ void k_finalize(RenderDev P, const float4 *__restrict__ fb, const int *__restrict__ nsamp,
+                           float *__restrict__ out, uint32_t npix)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    float4 f = fb[i];
+    if (P.flags & SRAY_RENDER_REF_Q1) {
+        out[(size_t)i * 4 + 0] = f.x;
+        out[(size_t)i * 4 + 1] = f.y;
+        out[(size_t)i * 4 + 2] = f.z;
+        out[(size_t)i * 4 + 3] = f.w;
+        return;
+    }
+    float rcp = 1.0f / (float)nsamp[i];
+    out[(size_t)i * 4 + 0] = f.x * rcp;
+    out[(size_t)i * 4 + 1] = f.y * rcp;
+    out[(size_t)i * 4 + 2] = f.z * rcp;
+    out[(size_t)i * 4 + 3] = f.w * rcp;
+}
+
+__global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__restrict__ out, uint32_t npix)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    int xr = (int)(i % (uint32_t)P.W), yr = (int)(i / (uint32_t)P.W);
+    int ax = P.x0 + xr;
+    int tx = (ax / 48) * 48;
+    int org = tx > P.x0 ? tx : P.x0; /* tile origin clipped to the rectangle */
+    int f = ax - org;                /* pixel offset within the tile row */
+    size_t fi = ((size_t)yr * P.W + (size_t)(org - P.x0)) * 4 + (size_t)f;
+    float rcp = 1.0f / (float)nsamp[i];
+    out[fi] = out[fi] * rcp;
+}
+
+/* ================================================================== */
+/* runtime                                                              */
+/* ================================================================== */
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+template <class T> T *dptr(DevBuf &b) { return (T *)b.p; }
+
+struct GpuScene {
+    std::vector<DevBuf> bufs;
+    SceneDev dev{};
+    uint64_t bytes = 0;
+    double diag = 0;
+    uint32_t scene_height = 0, mesh_height = 0;
+    uint32_t nobjs = 0;
+};
+
+} // namespace
+
+struct sray_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    std::string err;
+    std::map<uint32_t, std::unique_ptr<GpuScene>> scenes;
+    uint32_t next_scene = 1;
+    uint64_t admission_cap = 200ull << 30;
+    /* render scratch, grown on demand */
+    std::vector<DevBuf> qlev;
+    DevBuf hits, direct, sq, occ, kids, kflag, acc, fb, subpix, nsamp, act[2], keep, stats, errf, cubtmp, nsel, rays,
+        hitbuf, counts;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    /* per-kernel HIP-event timing */
+    bool profiling = false;
+    std::vector<hipEvent_t> evpool;
+    struct Pending {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    struct KStat {
+        uint64_t launches = 0;
+        double ms = 0;
+    };
+    std::map<std::string, KStat> kstats;
+    DevBuf rcounts;
+};
+
+namespace {
+
+int fail(sray_gpu_ctx *c, int code, const char *fmt, ...)
+{
+    char b[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(b, sizeof b, fmt, ap);
+    va_end(ap);
+    if (c) c->err = b;
+    return code;
+}
+
+#define HIPCHK(c, x)                                                                                                  \
+    do {                                                                                                              \
+        hipError_t e_ = (x);                                                                                          \
+        if (e_ != hipSuccess) return fail((c), SRAY_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));            \
+    } while (0)
+
+int ensure(sray_gpu_ctx *c, DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return SRAY_OK;
+    if (b.p) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, SRAY_ERR_NOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    b.bytes = want;
+    return SRAY_OK;
+}
+
+void release(DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int upload(sray_gpu_ctx *c, GpuScene &g, const void *src, size_t bytes, const void **dst)
+{
+    DevBuf b;
+    if (bytes == 0) {
+        *dst = nullptr;
+        return SRAY_OK;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(c, SRAY_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    b.bytes = bytes;
+    g.bufs.push_back(b);
+    g.bytes += bytes;
+    HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    *dst = b.p;
+    return SRAY_OK;
+}
+
+inline uint32_t blocks(size_t n, int bs) { return (uint32_t)((n + (size_t)bs - 1) / (size_t)bs); }
+
+hipEvent_t prof_event(sray_gpu_ctx *c)
+{
+    if (!c->evpool.empty()) {
+        hipEvent_t e = c->evpool.back();
+        c->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+/* records an event pair around one launch when profiling is on */
+struct ProfScope {
+    sray_gpu_ctx *c;
+    const char *name;
+    hipEvent_t a = nullptr;
+    ProfScope(sray_gpu_ctx *cc, const char *n) : c(cc), name(n)
+    {
+        if (c->profiling && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
+    }
+    ~ProfScope()
+    {
+        if (!a) return;
+        hipEvent_t b = prof_event(c);
+        if (!b) {
+            c->evpool.push_back(a);
+            return;
+        }
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back({name, a, b});
+    }
+};
+
+/* after a stream synchronisation: fold pending event pairs into the stats */
+void prof_resolve(sray_gpu_ctx *c)
+{
+    for (auto &p : c->pending) {
+        float ms = 0;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto &k = c->kstats[p.name];
+            k.launches++;
+            k.ms += ms;
+        }
+        c->evpool.push_back(p.a);
+        c->evpool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+GpuScene *find_scene(sray_gpu_ctx *c, uint32_t id)
+{
+    auto it = c->scenes.find(id);
+    return it == c->scenes.end() ? nullptr : it->second.get();
+}
+
+int validate_desc(sray_gpu_ctx *c, const sray_scene_desc *s)
+{
+    if (!s) return fail(c, SRAY_ERR_ARG, "scene description is NULL");
+    if (s->num_objects && !s->objects) return fail(c, SRAY_ERR_ARG, "objects is NULL");
+    if (s->num_meshes && !s->meshes) return fail(c, SRAY_ERR_ARG, "meshes is NULL");
+    if (s->num_triangles && !s->tri_pos) return fail(c, SRAY_ERR_ARG, "tri_pos is NULL");
+    if (s->num_lights && !s->lights) return fail(c, SRAY_ERR_ARG, "lights is NULL");
+    if (s->num_materials && !s->materials) return fail(c, SRAY_ERR_ARG, "materials is NULL");
+    if (s->scene_oct_depth < 0 || s->mesh_oct_depth < 0 || s->scene_oct_items < 0 || s->mesh_oct_items < 0)
+        return fail(c, SRAY_ERR_ARG, "negative octree limits");
+    if (s->num_lights > 255) return fail(c, SRAY_ERR_LIMIT, "more than 255 lights");
+    for (uint32_t i = 0; i < s->num_objects; i++) {
+        const sray_object_desc &o = s->objects[i];
+        if (o.type < 0 || o.type > 2) return fail(c, SRAY_ERR_ARG, "object %u: bad type %d", i, o.type);
+        if (o.material >= (int)s->num_materials) return fail(c, SRAY_ERR_ARG, "object %u: bad material", i);
+        if (o.type == SRAY_OBJ_MESH && (o.mesh < 0 || (uint32_t)o.mesh >= s->num_meshes))
+            return fail(c, SRAY_ERR_ARG, "object %u: bad mesh index", i);
+    }
+    for (uint32_t m = 0; m < s->num_meshes; m++) {
+        uint64_t e = (uint64_t)s->meshes[m].tri_start + s->meshes[m].tri_count;
+        if (e > s->num_triangles) return fail(c, SRAY_ERR_ARG, "mesh %u: triangle range out of bounds", m);
+    }
+    return SRAY_OK;
+}
+
+int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_id)
+{
+    int rc = validate_desc(c, s);
+    if (rc) return rc;
+    auto g = std::make_unique<GpuScene>();
+    /* octrees: scene level first (root 0), then one per mesh */
+    sray::FlatTree ft;
+    std::vector<float> boxes;
+    sray::object_boxes(s, boxes);
+    uint32_t sroot = sray::build_flat_octree(boxes, s->scene_oct_items, s->scene_oct_depth, 0, ft);
+    g->scene_height = ft.height;
+    double dx = (double)ft.root_max[0] - ft.root_min[0], dy = (double)ft.root_max[1] - ft.root_min[1],
+           dz = (double)ft.root_max[2] - ft.root_min[2];
+    g->diag = s->num_objects ? std::sqrt(dx * dx + dy * dy + dz * dz) : 0.0;
+    std::vector<uint32_t> mesh_root(s->num_meshes);
+    uint32_t mh = 0;
+    for (uint32_t m = 0; m < s->num_meshes; m++) {
+        sray::triangle_boxes(s, m, boxes);
+        sray::FlatTree tmp;
+        /* append into ft: build into tmp then splice with offsets */
+        uint32_t node_off = (uint32_t)(ft.lo.size() / 4), item_off = (uint32_t)ft.items.size();
+        sray::build_flat_octree(boxes, s->mesh_oct_items, s->mesh_oct_depth, s->meshes[m].tri_start, tmp);
+        mh = std::max(mh, tmp.height);
+        for (size_t k = 0; k < tmp.lo.size() / 4; k++) {
+            uint32_t first, meta;
+            memcpy(&first, &tmp.lo[k * 4 + 3], 4);
+            memcpy(&meta, &tmp.hi[k * 4 + 3], 4);
+            first += (meta & LEAF) ? item_off : node_off;
+            memcpy(&tmp.lo[k * 4 + 3], &first, 4);
+        }
+        ft.lo.insert(ft.lo.end(), tmp.lo.begin(), tmp.lo.end());
+        ft.hi.insert(ft.hi.end(), tmp.hi.begin(), tmp.hi.end());
+        ft.items.insert(ft.items.end(), tmp.items.begin(), tmp.items.end());
+        mesh_root[m] = node_off;
+    }
+    g->mesh_height = mh;
+    if ((int)(4 + g->scene_height + g->mesh_height) > STACK)
+        return fail(c, SRAY_ERR_LIMIT, "octree heights %u + %u exceed the traversal stack (%d entries)",
+                    g->scene_height, g->mesh_height, STACK);
+    if (ft.items.size() >= (1u << 30)) return fail(c, SRAY_ERR_LIMIT, "more than 2^30 octree item references");
+    /* objects */
+    uint32_t nmat = s->num_materials;
+    std::vector<DevObj> objs(s->num_objects);
+    for (uint32_t i = 0; i < s->num_objects; i++) {
+        const sray_object_desc &o = s->objects[i];
+        DevObj &d = objs[i];
+        memset(&d, 0, sizeof d);
+        memcpy(d.inv, o.inv_xform, sizeof d.inv);
+        memcpy(d.xf, o.xform, sizeof d.xf);
+        d.rad = o.radius;
+        d.start = o.start;
+        d.end = o.end;
+        d.type = o.type;
+        d.mat = o.material >= 0 ? o.material : (int32_t)nmat; /* default material appended */
+        d.moving = ((o.motion[0] != 0 || o.motion[1] != 0 || o.motion[2] != 0) && o.motion_t1 > o.motion_t0) ? 1 : 0;
+        memcpy(d.motion, o.motion, sizeof d.motion);
+        d.mt0 = o.motion_t0;
+        d.mt1 = o.motion_t1;
+        d.mesh = o.mesh;
+        if (o.type == SRAY_OBJ_MESH) {
+            d.root = mesh_root[o.mesh];
+            d.tri_base = s->meshes[o.mesh].tri_start;
+            d.has_attr = (s->tri_normal ? 1 : 0) | (s->tri_texcoord ? 2 : 0) | (s->tri_tangent ? 4 : 0);
+        }
+    }
+    std::vector<DevMat> mats(nmat + 1);
+    for (uint32_t i = 0; i <= nmat; i++) {
+        sray_material_desc md;
+        if (i < nmat) md = s->materials[i];
+        else {
+            /* Scene::default_mat = Material(): shade_undef, src/material.cc:79-89 */
+            memset(&md, 0, sizeof md);
+            md.diffuse[0] = md.diffuse[1] = md.diffuse[2] = md.diffuse[3] = 1.0f;
+            md.specular[3] = 1.0f;
+            md.shininess = 1.0f;
+            md.ior = 1.5f;
+            md.flags = SRAY_MAT_HAS_SPECULAR | SRAY_MAT_HAS_REFLECT | SRAY_MAT_HAS_REFRACT;
+        }
+        DevMat &d = mats[i];
+        memset(&d, 0, sizeof d);
+        d.kd = make_float4(md.diffuse[0], md.diffuse[1], md.diffuse[2], md.diffuse[3]);
+        d.ks = make_float4(md.specular[0], md.specular[1], md.specular[2], md.specular[3]);
+        d.shininess = md.shininess;
+        d.reflect = md.reflect;
+        d.refract = md.refract;
+        d.ior = md.ior;
+        d.flags = md.flags;
+    }
+    std::vector<DevLight> lights(s->num_lights);
+    for (uint32_t i = 0; i < s->num_lights; i++) {
+        const sray_light_desc &l = s->lights[i];
+        DevLight &d = lights[i];
+        memset(&d, 0, sizeof d);
+        d.type = l.type;
+        memcpy(d.pos, l.pos, sizeof d.pos);
+        memcpy(d.color, l.color, sizeof d.color);
+        d.radius = l.radius;
+        memcpy(d.dim, l.dim, sizeof d.dim);
+        memcpy(d.xf, l.xform, sizeof d.xf);
+    }
+    /* triangles as 3 float4 */
+    std::vector<float4> tri((size_t)s->num_triangles * 3);
+    for (size_t t = 0; t < s->num_triangles; t++)
+        for (int k = 0; k < 3; k++) {
+            const float *p = s->tri_pos + t * 9 + k * 3;
+            tri[t * 3 + k] = make_float4(p[0], p[1], p[2], 0.0f);
+        }
+    SceneDev &D = g->dev;
+    memset(&D, 0, sizeof D);
+    const void *p;
+    if ((rc = upload(c, *g, ft.lo.data(), ft.lo.size() * 4, &p))) return rc;
+    D.nlo = (const float4 *)p;
+    if ((rc = upload(c, *g, ft.hi.data(), ft.hi.size() * 4, &p))) return rc;
+    D.nhi = (const float4 *)p;
+    if ((rc = upload(c, *g, ft.items.data(), ft.items.size() * 4, &p))) return rc;
+    D.items = (const uint32_t *)p;
+    if ((rc = upload(c, *g, objs.data(), objs.size() * sizeof(DevObj), &p))) return rc;
+    D.objs = (const DevObj *)p;
+    if ((rc = upload(c, *g, tri.data(), tri.size() * sizeof(float4), &p))) return rc;
+    D.tri = (const float4 *)p;
+    if (s->tri_normal) {
+        if ((rc = upload(c, *g, s->tri_normal, (size_t)s->num_triangles * 9 * 4, &p))) return rc;
+        D.tri_n = (const float *)p;
+    }
+    if (s->tri_tangent) {
+        if ((rc = upload(c, *g, s->tri_tangent, (size_t)s->num_triangles * 9 * 4, &p))) return rc;
+        D.tri_g = (const float *)p;
+    }
+    if (s->tri_texcoord) {
+        if ((rc = upload(c, *g, s->tri_texcoord, (size_t)s->num_triangles * 6 * 4, &p))) return rc;
+        D.tri_uv = (const float *)p;
+    }
+    if ((rc = upload(c, *g, mats.data(), mats.size() * sizeof(DevMat), &p))) return rc;
+    D.mats = (const DevMat *)p;
+    if ((rc = upload(c, *g, lights.data(), lights.size() * sizeof(DevLight), &p))) return rc;
+    D.lights = (const DevLight *)p;
+    D.scene_root = sroot;
+    D.nlights = (int32_t)s->num_lights;
+    D.nobjs = (int32_t)s->num_objects;
+    D.env_ambient = make_float4(s->env_ambient[0], s->env_ambient[1], s->env_ambient[2], s->env_ambient[3]);
+    D.env_color = make_float4(s->env_color[0], s->env_color[1], s->env_color[2], s->env_color[3]);
+    memcpy(D.cam, s->camera.xform, sizeof D.cam);
+    D.fz = (float)(1.0 / std::tan(0.5 * s->camera.vfov));
+    D.max_dist = s->camera.max_dist;
+    D.shutter = s->camera.shutter;
+    g->nobjs = s->num_objects;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    uint32_t id = c->next_scene++;
+    c->scenes[id] = std::move(g);
+    *scene_id = id;
+    return SRAY_OK;
+}
+
+void fill_render_dev(const sray_render_params *p, RenderDev &R)
+{
+    memset(&R, 0, sizeof R);
+    R.width = p->width;
+    R.height = p->height;
+    R.x0 = p->x0;
+    R.y0 = p->y0;
+    R.x1 = p->x1;
+    R.y1 = p->y1;
+    R.W = p->x1 - p->x0;
+    R.min_samples = p->min_samples;
+    R.max_samples = std::max(1, p->max_samples);
+    R.max_var = p->max_var;
+    R.min_energy = p->min_energy;
+    R.iter = p->iter;
+    R.shadow_samples = std::max(1, p->shadow_samples);
+    R.diffuse_samples = std::max(0, p->diffuse_samples);
+    R.frame_time = p->frame_time;
+    R.mblur = p->mblur;
+    R.caust = p->caust_photons;
+    R.gi = p->gi_photons;
+    R.flags = p->flags;
+    R.seed = p->seed;
+    R.aspect = (float)((double)p->width / (double)p->height);
+}
+
+int validate_params(sray_gpu_ctx *c, const sray_render_params *p)
+{
+    if (!p) return fail(c, SRAY_ERR_ARG, "render params is NULL");
+    if (p->width <= 0 || p->height <= 0) return fail(c, SRAY_ERR_ARG, "bad image size %dx%d", p->width, p->height);
+    if (p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height || p->x0 >= p->x1 || p->y0 >= p->y1)
+        return fail(c, SRAY_ERR_ARG, "bad pixel rectangle");
+    if (p->max_samples > 256) return fail(c, SRAY_ERR_LIMIT, "max_samples > 256");
+    if (p->iter < 0 || p->iter > 30) return fail(c, SRAY_ERR_LIMIT, "iter outside [0,30]");
+    if (p->shadow_samples > 1024) return fail(c, SRAY_ERR_LIMIT, "shadow_samples > 1024");
+    if (p->gi_photons || p->caust_photons)
+        return fail(c, SRAY_ERR_ARG, "photon-mapped rendering needs a bound photon map (not available in this build)");
+    return SRAY_OK;
+}
+
+constexpr uint32_t CHUNK = 1u << 21; /* rays per level sub-chunk */
+
+struct LevelCtx {
+    sray_gpu_ctx *c;
+    GpuScene *g;
+    RenderDev R;
+    const uint32_t *active;
+    uint32_t c0;
+    int sample;
+    unsigned long long *stats;
+};
+
+int compact(sray_gpu_ctx *c, const QRay *in, const uint8_t *flags, QRay *out, uint32_t n, uint32_t *count)
+{
+    size_t tmp = 0;
+    HIPCHK(c, hipcub::DeviceSelect::Flagged(nullptr, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
+    int rc = ensure(c, c->cubtmp, tmp);
+    if (rc) return rc;
+    tmp = c->cubtmp.bytes;
+    HIPCHK(c, hipcub::DeviceSelect::Flagged(c->cubtmp.p, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
+    HIPCHK(c, hipMemcpyAsync(count, c->nsel.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SRAY_OK;
+}
+
+int compact_u32(sray_gpu_ctx *c, const uint32_t *in, const uint8_t *flags, uint32_t *out, uint32_t n, uint32_t *count)
+{
+    size_t tmp = 0;
+    HIPCHK(c, hipcub::DeviceSelect::Flagged(nullptr, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
+    int rc = ensure(c, c->cubtmp, tmp);
+    if (rc) return rc;
+    tmp = c->cubtmp.bytes;
+    HIPCHK(c, hipcub::DeviceSelect::Flagged(c->cubtmp.p, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
+    HIPCHK(c, hipMemcpyAsync(count, c->nsel.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SRAY_OK;
+}
+
+/* one bounce level over qlev[b][0,count): sub-chunks of CHUNK rays, each
+ * followed depth-first by its children's levels */
+int process_level(LevelCtx &L, int b, uint32_t count)
+{
+    sray_gpu_ctx *c = L.c;
+    const SceneDev &S = L.g->dev;
+    const RenderDev &R = L.R;
+    int rc;
+    QRay *Q = dptr<QRay>(c->qlev[b]);
+    for (uint32_t off = 0; off < count; off += CHUNK) {
+        uint32_t m = std::min(CHUNK, count - off);
+        const QRay *q = Q + off;
+        bool count = (R.flags & SRAY_RENDER_COUNT) != 0;
+        unsigned long long *rc6 = dptr<unsigned long long>(c->rcounts);
+        {
+            ProfScope ps(c, "k_trace_q");
+            if (count) k_trace_q<true><<<blocks(m, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(S, q, m, dptr<int4>(c->hits), rc6);
+            else k_trace_q<false><<<blocks(m, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(S, q, m, dptr<int4>(c->hits), rc6);
+        }
+        {
+            ProfScope ps(c, "k_shade");
+            k_shade<<<blocks(m, 256), 256, 0, c->stream>>>(S, R, L.active, L.c0, L.sample, q, dptr<int4>(c->hits), m,
+                                                           dptr<float4>(c->direct), dptr<SRay>(c->sq),
+                                                           dptr<QRay>(c->kids), dptr<uint8_t>(c->kflag), L.stats,
+                                                           dptr<uint32_t>(c->errf));
+        }
+        size_t ns = (size_t)m * R.L;
+        if (ns) {
+            ProfScope ps(c, "k_trace_shadow");
+            if (count)
+                k_trace_shadow<true><<<blocks(ns, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(
+                    S, dptr<SRay>(c->sq), (uint32_t)ns, dptr<uint8_t>(c->occ), rc6 + 6);
+            else
+                k_trace_shadow<false><<<blocks(ns, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(
+                    S, dptr<SRay>(c->sq), (uint32_t)ns, dptr<uint8_t>(c->occ), rc6 + 6);
+        }
+        {
+            ProfScope ps(c, "k_accum");
+            k_accum<<<blocks(m, 256), 256, 0, c->stream>>>(q, dptr<float4>(c->direct), dptr<SRay>(c->sq),
+                                                          dptr<uint8_t>(c->occ), m, R.L, dptr<float4>(c->acc));
+        }
+        if (b + 1 > R.iter) continue;
+        uint32_t nk = 0;
+        if ((rc = compact(c, dptr<QRay>(c->kids), dptr<uint8_t>(c->kflag), dptr<QRay>(c->qlev[b + 1]),
+                          m * (uint32_t)R.K, &nk)))
+            return rc;
+        HIPCHK(c, hipGetLastError());
+        if (nk) {
+            if ((rc = process_level(L, b + 1, nk))) return rc;
+        }
+    }
+    return SRAY_OK;
+}
+
+int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *d_out, sray_stats *st)
+{
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    GpuScene *g = find_scene(c, scene_id);
+    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    RenderDev R;
+    fill_render_dev(p, R);
+    const SceneDev &S = g->dev;
+    int L = 0;
+    /* shadow slots per ray: phong samples area lights shadow_samples times */
+    std::vector<sray_light_desc> dummy;
+    (void)dummy;
+    {
+        /* light types live on the device; keep a host mirror via the count of
+         * area lights, recomputed from DevLight on upload would need a copy:
+         * use the upper bound lights * max(1, shadow_samples) */
+        L = S.nlights * R.shadow_samples;
+    }
+    R.L = L;
+    R.K = 2;
+    uint32_t npix = (uint32_t)R.W * (uint32_t)(p->y1 - p->y0);
+    uint32_t levels = (uint32_t)R.iter + 1;
+    if (c->qlev.size() < levels) c->qlev.resize(levels);
+    for (uint32_t b = 0; b < levels; b++)
+        if ((rc = ensure(c, c->qlev[b], (size_t)CHUNK * R.K * sizeof(QRay)))) return rc;
+    if ((rc = ensure(c, c->hits, (size_t)CHUNK * sizeof(int4)))) return rc;
+    if ((rc = ensure(c, c->direct, (size_t)CHUNK * sizeof(float4)))) return rc;
+    if ((rc = ensure(c, c->sq, (size_t)CHUNK * std::max(L, 1) * sizeof(SRay)))) return rc;
+    if ((rc = ensure(c, c->occ, (size_t)CHUNK * std::max(L, 1)))) return rc;
+    if ((rc = ensure(c, c->kids, (size_t)CHUNK * R.K * sizeof(QRay)))) return rc;
+    if ((rc = ensure(c, c->kflag, (size_t)CHUNK * R.K))) return rc;
+    if ((rc = ensure(c, c->acc, (size_t)CHUNK * sizeof(float4)))) return rc;
+    if ((rc = ensure(c, c->fb, (size_t)npix * sizeof(float4)))) return rc;
+    if ((rc = ensure(c, c->nsamp, (size_t)npix * sizeof(int)))) return rc;
+    if (R.max_samples > 1 && (rc = ensure(c, c->subpix, (size_t)npix * R.max_samples * sizeof(float4)))) return rc;
+    for (int k = 0; k < 2; k++)
+        if ((rc = ensure(c, c->act[k], (size_t)npix * 4))) return rc;
+    if ((rc = ensure(c, c->keep, npix))) return rc;
+    if ((rc = ensure(c, c->stats, 8 * sizeof(unsigned long long)))) return rc;
+    if ((rc = ensure(c, c->errf, 4))) return rc;
+    if ((rc = ensure(c, c->nsel, 4))) return rc;
+    if ((rc = ensure(c, c->rcounts, 12 * sizeof(unsigned long long)))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->rcounts.p, 0, 12 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->errf.p, 0, 4, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    k_fill_f4<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<float4>(c->fb), npix, make_float4(0, 0, 0, 0));
+    k_iota<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(c->act[0]), npix);
+    uint32_t nact = npix;
+    int cur = 0;
+    uint64_t prim = 0, samples = 0;
+    LevelCtx LC{c, g, R, nullptr, 0, 0, dptr<unsigned long long>(c->stats)};
+    for (int s = 0; s < R.max_samples && nact > 0; s++) {
+        const uint32_t *A = dptr<uint32_t>(c->act[cur]);
+        for (uint32_t c0 = 0; c0 < nact; c0 += CHUNK) {
+            uint32_t m = std::min(CHUNK, nact - c0);
+            k_fill_f4<<<blocks(m, 256), 256, 0, c->stream>>>(dptr<float4>(c->acc), m, make_float4(0, 0, 0, 0));
+            {
+                ProfScope ps(c, "k_raygen");
+                k_raygen<<<blocks(m, 256), 256, 0, c->stream>>>(S, R, A, c0, m, s, dptr<QRay>(c->qlev[0]));
+            }
+            LC.active = A;
+            LC.c0 = c0;
+            LC.sample = s;
+            if ((rc = process_level(LC, 0, m))) return rc;
+            k_pass_finish<<<blocks(m, 256), 256, 0, c->stream>>>(R, A, c0, m, s, dptr<float4>(c->acc),
+                                                                dptr<float4>(c->fb), dptr<float4>(c->subpix),
+                                                                dptr<int>(c->nsamp), dptr<uint8_t>(c->keep));
+            prim += m;
+        }
+        samples += nact;
+        if (s + 1 < R.max_samples) {
+            uint32_t nn = 0;
+            if ((rc = compact_u32(c, A, dptr<uint8_t>(c->keep), dptr<uint32_t>(c->act[cur ^ 1]), nact, &nn))) return rc;
+            nact = nn;
+            cur ^= 1;
+        }
+    }
+    k_finalize<<<blocks(npix, 256), 256, 0, c->stream>>>(R, dptr<float4>(c->fb), dptr<int>(c->nsamp), d_out, npix);
+    if (R.flags & SRAY_RENDER_REF_Q1)
+        k_q1_scale<<<blocks(npix, 256), 256, 0, c->stream>>>(R, dptr<int>(c->nsamp), d_out, npix);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    prof_resolve(c);
+    unsigned long long hs[8], rcn[12];
+    uint32_t errf = 0;
+    HIPCHK(c, hipMemcpy(hs, c->stats.p, sizeof hs, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rcn, c->rcounts.p, sizeof rcn, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&errf, c->errf.p, 4, hipMemcpyDeviceToHost));
+    if (errf) return fail(c, SRAY_ERR_LIMIT, "ior stack deeper than %d media", IOR_DEPTH);
+    if (st) {
+        memset(st, 0, sizeof *st);
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        st->render_ms = ms;
+        st->rays_primary = prim;
+        st->rays_shadow = hs[1];
+        st->rays_reflect = hs[2];
+        st->rays_refract = hs[3];
+        st->rays_diffuse = hs[4];
+        st->samples = samples;
+        sray_trace_counts *tc[2] = {&st->closest, &st->shadow};
+        for (int k = 0; k < 2; k++) {
+            tc[k]->nodes = rcn[k * 6 + 0];
+            tc[k]->spheres = rcn[k * 6 + 1];
+            tc[k]->cylinders = rcn[k * 6 + 2];
+            tc[k]->triangles = rcn[k * 6 + 3];
+            tc[k]->item_refs = rcn[k * 6 + 4];
+            tc[k]->mesh_enters = rcn[k * 6 + 5];
+        }
+    }
+    return SRAY_OK;
+}
+
+int do_trace_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *d_rays, uint32_t n, sray_hit *d_hits,
+                    uint32_t flags, unsigned long long *d_counts)
+{
+    GpuScene *g = find_scene(c, scene_id);
+    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    if (n == 0) return SRAY_OK;
+    bool any = flags & SRAY_TRACE_ANY, count = (flags & SRAY_TRACE_COUNT) && d_counts;
+    uint32_t nb = blocks(n, TRACE_BLOCK);
+    ProfScope ps(c, any ? "k_trace_rays_any" : "k_trace_rays");
+    if (any && count) k_trace_rays<true, true><<<nb, TRACE_BLOCK, 0, c->stream>>>(g->dev, d_rays, n, d_hits, d_counts);
+    else if (any) k_trace_rays<true, false><<<nb, TRACE_BLOCK, 0, c->stream>>>(g->dev, d_rays, n, d_hits, d_counts);
+    else if (count) k_trace_rays<false, true><<<nb, TRACE_BLOCK, 0, c->stream>>>(g->dev, d_rays, n, d_hits, d_counts);
+    else k_trace_rays<false, false><<<nb, TRACE_BLOCK, 0, c->stream>>>(g->dev, d_rays, n, d_hits, d_counts);
+    HIPCHK(c, hipGetLastError());
+    return SRAY_OK;
+}
+
+} // namespace
+
+/* ================================================================== */
+/* C ABI                                                                */
+/* ================================================================== */
+extern "C" {
+
+int sray_gpu_create(int device, uint32_t flags, sray_gpu_ctx **out)
+{
+    (void)flags;
+    if (!out) return SRAY_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SRAY_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return SRAY_ERR_ARG;
+    auto *c = new sray_gpu_ctx;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return SRAY_ERR_HIP;
+    }
+    *out = c;
+    return SRAY_OK;
+}
+
+void sray_gpu_destroy(sray_gpu_ctx *c)
+{
+    if (!c) return;
+    {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (auto &kv : c->scenes)
+            for (auto &b : kv.second->bufs) release(b);
+        c->scenes.clear();
+        for (auto &b : c->qlev) release(b);
+        DevBuf *bs[] = {&c->hits, &c->direct, &c->sq, &c->occ, &c->kids, &c->kflag, &c->acc, &c->fb, &c->subpix,
+                        &c->nsamp, &c->act[0], &c->act[1], &c->keep, &c->stats, &c->errf, &c->cubtmp, &c->nsel,
+                        &c->rays, &c->hitbuf, &c->counts, &c->rcounts};
+        for (DevBuf *b : bs) release(*b);
+        prof_resolve(c);
+        for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(c->ev0);
+        (void)hipEventDestroy(c->ev1);
+        (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+const char *sray_gpu_last_error(sray_gpu_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int sray_gpu_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_id)
+{
+    if (!c || !scene_id) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    return do_scene_upload(c, s, scene_id);
+}
+
+int sray_gpu_scene_release(sray_gpu_ctx *c, uint32_t scene_id)
+{
+    if (!c) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    auto it = c->scenes.find(scene_id);
+    if (it == c->scenes.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &b : it->second->bufs) release(b);
+    c->scenes.erase(it);
+    return SRAY_OK;
+}
+
+int sray_gpu_scene_bytes(sray_gpu_ctx *c, uint32_t scene_id, uint64_t *bytes)
+{
+    if (!c || !bytes) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    GpuScene *g = find_scene(c, scene_id);
+    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    *bytes = g->bytes;
+    return SRAY_OK;
+}
+
+int sray_gpu_trace_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *d_rays, uint32_t n, sray_hit *d_hits,
+                          uint32_t flags)
+{
+    if (!c || (n && (!d_rays || !d_hits))) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    return do_trace_device(c, scene_id, d_rays, n, d_hits, flags & ~SRAY_TRACE_COUNT, nullptr);
+}
+
+int sray_gpu_trace(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *rays, uint32_t n, sray_hit *hits,
+                   uint32_t flags, sray_trace_counts *counts)
+{
+    if (!c || (n && (!rays || !hits))) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    if (counts) memset(counts, 0, sizeof *counts);
+    if (!find_scene(c, scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    if (n == 0) return SRAY_OK;
+    int rc;
+    if ((rc = ensure(c, c->rays, (size_t)n * sizeof(sray_ray)))) return rc;
+    if ((rc = ensure(c, c->hitbuf, (size_t)n * sizeof(sray_hit)))) return rc;
+    if ((rc = ensure(c, c->counts, 8 * sizeof(unsigned long long)))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->rays.p, rays, (size_t)n * sizeof(sray_ray), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->counts.p, 0, 8 * sizeof(unsigned long long), c->stream));
+    rc = do_trace_device(c, scene_id, dptr<sray_ray>(c->rays), n, dptr<sray_hit>(c->hitbuf),
+                         counts ? (flags | SRAY_TRACE_COUNT) : flags, dptr<unsigned long long>(c->counts));
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(hits, c->hitbuf.p, (size_t)n * sizeof(sray_hit), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long hc[8] = {0};
+    HIPCHK(c, hipMemcpyAsync(hc, c->counts.p, sizeof hc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_resolve(c);
+    if (counts) {
+        counts->nodes = hc[0];
+        counts->spheres = hc[1];
+        counts->cylinders = hc[2];
+        counts->triangles = hc[3];
+        counts->item_refs = hc[4];
+        counts->mesh_enters = hc[5];
+    }
+    return SRAY_OK;
+}
+
+int sray_gpu_primary_rays_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, int sample,
+                                 sray_ray *d_rays)
+{
+    if (!c || !p || !d_rays || sample < 0) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    GpuScene *g = find_scene(c, scene_id);
+    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    if (p->width <= 0 || p->height <= 0 || p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height ||
+        p->x0 >= p->x1 || p->y0 >= p->y1)
+        return fail(c, SRAY_ERR_ARG, "bad pixel rectangle");
+    RenderDev R;
+    fill_render_dev(p, R);
+    uint32_t n = (uint32_t)R.W * (uint32_t)(p->y1 - p->y0);
+    k_primary<<<blocks(n, 256), 256, 0, c->stream>>>(g->dev, R, sample, d_rays, n);
+    HIPCHK(c, hipGetLastError());
+    return SRAY_OK;
+}
+
+int sray_gpu_render_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *d_rgba,
+                           sray_stats *st)
+{
+    if (!c || !d_rgba) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    return do_render(c, scene_id, p, d_rgba, st);
+}
+
+int sray_gpu_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *rgba, sray_stats *st)
+{
+    if (!c || !rgba) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    size_t bytes = (size_t)(p->x1 - p->x0) * (size_t)(p->y1 - p->y0) * 4 * sizeof(float);
+    DevBuf out;
+    hipError_t e = hipMalloc(&out.p, bytes);
+    if (e != hipSuccess) return fail(c, SRAY_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    out.bytes = bytes;
+    rc = do_render(c, scene_id, p, (float *)out.p, st);
+    if (rc == SRAY_OK) {
+        e = hipMemcpy(rgba, out.p, bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, SRAY_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(e));
+    }
+    release(out);
+    return rc;
+}
+
+int sray_gpu_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *p, uint32_t n, int kind,
+                         uint32_t *pmap_id)
+{
+    (void)scene_id, (void)p, (void)n, (void)kind, (void)pmap_id;
+    return fail(c, SRAY_ERR_ARG, "photon maps are not available in this build");
+}
+
+int sray_gpu_pmap_release(sray_gpu_ctx *c, uint32_t pmap_id)
+{
+    (void)pmap_id;
+    return fail(c, SRAY_ERR_NOTFOUND, "photon maps are not available in this build");
+}
+
+int sray_gpu_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q, uint32_t n, float radius, int k,
+                         int mode, float *rgb)
+{
+    (void)pmap_id, (void)q, (void)n, (void)radius, (void)k, (void)mode, (void)rgb;
+    return fail(c, SRAY_ERR_NOTFOUND, "photon maps are not available in this build");
+}
+
+int sray_gpu_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *ticket)
+{
+    (void)r, (void)rgba, (void)ticket;
+    return fail(c, SRAY_ERR_ARG, "request serving is not available in this build");
+}
+
+int sray_gpu_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
+{
+    (void)ticket, (void)wait, (void)out;
+    return fail(c, SRAY_ERR_NOTFOUND, "request serving is not available in this build");
+}
+
+int sray_gpu_set_profiling(sray_gpu_ctx *c, int on)
+{
+    if (!c) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->profiling = on != 0;
+    return SRAY_OK;
+}
+
+int sray_gpu_kernel_stats(sray_gpu_ctx *c, sray_kernel_stat *out, int cap, int *n, int reset)
+{
+    if (!c || !n) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipStreamSynchronize(c->stream);
+    prof_resolve(c);
+    *n = (int)c->kstats.size();
+    int k = 0;
+    for (auto &kv : c->kstats) {
+        if (!out || k >= cap) break;
+        memset(&out[k], 0, sizeof out[k]);
+        snprintf(out[k].name, sizeof out[k].name, "%s", kv.first.c_str());
+        out[k].launches = kv.second.launches;
+        out[k].total_ms = kv.second.ms;
+        k++;
+    }
+    if (reset) c->kstats.clear();
+    return SRAY_OK;
+}
+
+int sray_gpu_set_admission_cap(sray_gpu_ctx *c, uint64_t bytes)
+{
+    if (!c) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->admission_cap = bytes;
+    return SRAY_OK;
+}
+
+} /* extern "C" */

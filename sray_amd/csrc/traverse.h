@@ -1,0 +1,361 @@
+/*
+ * traverse.h — two-level octree traversal on gfx950 (one ray per lane).
+ *
+ * Replaces Octree<T>::intersect_rec (src/octree.inl:200-241) at both levels:
+ * the scene octree over objects and, inside a mesh item, that mesh's octree
+ * over triangles (Mesh::intersect, src/mesh.cc:252-308).
+ *
+ * The reference walks every node whose box meets the segment, children in
+ * fixed order 0..7, and keeps the first occurrence of the smallest t (strict
+ * '<').  Here children are visited near-to-far (octant bits XOR the ray's
+ * sign mask), a node is skipped only when its entry distance is strictly
+ * greater than the best t found so far, and candidates are ranked by the
+ * pair (t, occurrence rank) — the rank being the item's position in the
+ * depth-first item array — so equal distances resolve to the same winner
+ * as the reference's first-occurrence rule.
+ *
+ * Per-lane stack in LDS, one 8-byte entry per open node:
+ *   ITER   x = first child, y = exist mask | pending (order space) << 8
+ *   RESUME x = next scene item, y = end        (scene leaf interrupted by a mesh)
+ *   MARKER x = scene item rank of the mesh being traversed
+ * A mesh hit by the same ray in several scene leaves (Q9) is traversed once:
+ * its result is memoised for the ray.
+ */
+#ifndef SRAY_TRAVERSE_H_
+#define SRAY_TRAVERSE_H_
+
+#include <cfloat>
+
+#include "device.h"
+
+namespace srd {
+
+constexpr uint32_t E_ITER = 0u, E_RESUME = 1u, E_MARKER = 2u;
+
+struct TRay {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+    uint32_t dmask;
+};
+
+__device__ __forceinline__ TRay make_tray(V3 o, V3 d)
+{
+    TRay r;
+    r.ox = o.x;
+    r.oy = o.y;
+    r.oz = o.z;
+    r.dx = d.x;
+    r.dy = d.y;
+    r.dz = d.z;
+    r.ix = 1.0f / d.x;
+    r.iy = 1.0f / d.y;
+    r.iz = 1.0f / d.z;
+    r.dmask = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    return r;
+}
+
+/* AABox::intersect, src/aabb.cc:81-125; tmin_out = entry distance */
+__device__ __forceinline__ bool slab(float4 lo, float4 hi, const TRay &r, float &tmin_out)
+{
+    bool xs = r.dx < 0.0f, ys = r.dy < 0.0f, zs = r.dz < 0.0f;
+    float tmin = ((xs ? hi.x : lo.x) - r.ox) * r.ix;
+    float tmax = ((xs ? lo.x : hi.x) - r.ox) * r.ix;
+    float tymin = ((ys ? hi.y : lo.y) - r.oy) * r.iy;
+    float tymax = ((ys ? lo.y : hi.y) - r.oy) * r.iy;
+    if ((tmin > tymax) || (tymin > tmax)) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = ((zs ? hi.z : lo.z) - r.oz) * r.iz;
+    float tzmax = ((zs ? lo.z : hi.z) - r.oz) * r.iz;
+    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    tmin_out = tmin;
+    return (tmin < 1.0f) && (tmax > 0.0f);
+}
+
+/* child mask permuted into near-to-far order: bit k <-> child (k ^ dmask) */
+__device__ __forceinline__ uint32_t order_mask(uint32_t m, uint32_t dm)
+{
+    if (dm & 1u) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
+    if (dm & 2u) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
+    if (dm & 4u) m = ((m & 0x0fu) << 4) | ((m >> 4) & 0x0fu);
+    return m;
+}
+
+/* object matrices at a ray time (linear translation keys) */
+__device__ __forceinline__ void obj_inv_at(const DevObj &o, int time, float I[12])
+{
+#pragma unroll
+    for (int k = 0; k < 12; k++) I[k] = o.inv[k];
+    if (!o.moving) return;
+    float s;
+    if (time <= o.mt0) s = 0.0f;
+    else if (time >= o.mt1) s = 1.0f;
+    else s = (float)(time - o.mt0) / (float)(o.mt1 - o.mt0);
+    V3 dl = v3(o.motion[0] * s, o.motion[1] * s, o.motion[2] * s);
+    V3 rd = xf_vec(o.inv, dl);
+    I[3] = I[3] - rd.x;
+    I[7] = I[7] - rd.y;
+    I[11] = I[11] - rd.z;
+}
+__device__ __forceinline__ void obj_xf_at(const DevObj &o, int time, float M[12])
+{
+#pragma unroll
+    for (int k = 0; k < 12; k++) M[k] = o.xf[k];
+    if (!o.moving) return;
+    float s;
+    if (time <= o.mt0) s = 0.0f;
+    else if (time >= o.mt1) s = 1.0f;
+    else s = (float)(time - o.mt0) / (float)(o.mt1 - o.mt0);
+    M[3] = M[3] + o.motion[0] * s;
+    M[7] = M[7] + o.motion[1] * s;
+    M[11] = M[11] + o.motion[2] * s;
+}
+
+/* Sphere::intersect, src/sphere.cc:85-132 (t only) */
+__device__ __forceinline__ bool sphere_t(float rad, V3 o, V3 d, float &t)
+{
+    float a = dot(d, d);
+    float b = 2.0f * d.x * o.x + 2.0f * d.y * o.y + 2.0f * d.z * o.z;
+    float c = dot(o, o) - rad * rad;
+    float disc = b * b - 4.0f * a * c;
+    if (disc < 0.0f) return false;
+    float sd = sqrtf(disc);
+    float t1 = (-b + sd) / (2.0f * a);
+    float t2 = (-b - sd) / (2.0f * a);
+    if ((t1 < EPS && t2 < EPS) || (t1 > 1.0f && t2 > 1.0f)) return false;
+    if (t1 < EPS) t1 = t2;
+    if (t2 < EPS) t2 = t1;
+    t = t1 < t2 ? t1 : t2;
+    return true;
+}
+
+/* Cylinder::intersect, src/cylinder.cc:125-185 (t only) */
+__device__ __forceinline__ bool cylinder_t(float rad, float ys, float ye, V3 o, V3 d, float &t)
+{
+    float a = d.x * d.x + d.z * d.z;
+    float b = 2.0f * d.x * o.x + 2.0f * d.z * o.z;
+    float c = o.x * o.x + o.z * o.z - rad * rad;
+    float disc = b * b - 4.0f * a * c;
+    if (disc < 0.0f) return false;
+    float sd = sqrtf(disc);
+    float t1 = (-b + sd) / (2.0f * a);
+    float t2 = (-b - sd) / (2.0f * a);
+    if ((t1 < EPS && t2 < EPS) || (t1 > 1.0f && t2 > 1.0f)) return false;
+    float y1 = o.y + d.y * t1;
+    float y2 = o.y + d.y * t2;
+    bool v0 = true, v1 = true;
+    if (t1 < EPS || y1 < ys || y1 >= ye) {
+        v0 = false;
+        t1 = t2;
+    }
+    if (t2 < EPS || y2 < ys || y2 >= ye) {
+        v1 = false;
+        t2 = t1;
+    }
+    if (!v0 && !v1) return false;
+    t = t1 < t2 ? t1 : t2;
+    return true;
+}
+
+/* Triangle::intersect + check_clock, src/mesh.cc:117-165, 310-315 */
+__device__ __forceinline__ bool tri_t(float4 A, float4 B, float4 C, const TRay &r, float &t)
+{
+    V3 v0 = v3(A.x, A.y, A.z), v1 = v3(B.x, B.y, B.z), v2 = v3(C.x, C.y, C.z);
+    V3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+    V3 n = cross(e1, e2);
+    V3 d = v3(r.dx, r.dy, r.dz), o = v3(r.ox, r.oy, r.oz);
+    float ndd = dot(n, d);
+    if (fabsf(ndd) < EPS) return false;
+    V3 vo = sub(o, v0);
+    float tt = -dot(n, vo) / ndd;
+    if (tt < EPS || tt > 1.0f) return false;
+    V3 p = add(o, mul(d, tt));
+    if (!(dot(cross(sub(v1, v0), sub(p, v0)), n) >= 0.0f)) return false;
+    if (!(dot(cross(sub(v2, v1), sub(p, v1)), n) >= 0.0f)) return false;
+    if (!(dot(cross(sub(v0, v2), sub(p, v2)), n) >= 0.0f)) return false;
+    t = tt;
+    return true;
+}
+
+struct TraceCounts {
+    uint32_t nodes, spheres, cylinders, triangles, item_refs, mesh_enters;
+};
+
+struct HitRec {
+    float t;
+    int32_t obj;
+    uint32_t tri;  /* global triangle index, 0xffffffff if none */
+    uint32_t rank;
+};
+
+/* Scene::cast_ray over the octrees.  ANY: stop at the first hit (the
+ * reference's shadow test cast_ray(ray) != NULL, src/shader.cc:266).
+ * stk points at this lane's entry 0; consecutive entries are `stride`
+ * apart (LDS, lane-interleaved). */
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int time, uint2 *stk, int stride,
+                                           TraceCounts &cnt)
+{
+    TRay wr = make_tray(wo, wd);
+    TRay r = wr;
+    HitRec best;
+    best.t = FLT_MAX;
+    best.obj = -1;
+    best.tri = 0xffffffffu;
+    best.rank = 0xffffffffu;
+    int level = 0;
+    float m_t = FLT_MAX;
+    uint32_t m_tri = 0xffffffffu, m_rank = 0xffffffffu;
+    int cache_obj = -1;
+    float cache_t = FLT_MAX;
+    uint32_t cache_tri = 0xffffffffu;
+    uint32_t item_cur = 0, item_end = 0;
+    int cur_mesh = -1;
+    int sp = 0;
+    /* pseudo entry whose only child is the scene root */
+    stk[0] = make_uint2(S.scene_root, 1u | ((1u << r.dmask) << 8));
+    sp = 1;
+    for (;;) {
+        if (item_cur < item_end) {
+            if (level) {
+                /* mesh leaf: all its triangles */
+                for (; item_cur < item_end; item_cur++) {
+                    uint32_t ti = S.items[item_cur];
+                    if (COUNT) {
+                        cnt.item_refs++;
+                        cnt.triangles++;
+                    }
+                    float4 A = S.tri[(size_t)ti * 3], B = S.tri[(size_t)ti * 3 + 1], C = S.tri[(size_t)ti * 3 + 2];
+                    float t;
+                    if (tri_t(A, B, C, r, t)) {
+                        if (ANY) {
+                            best.t = t;
+                            best.tri = ti;
+                            best.obj = cur_mesh;
+                            return best;
+                        }
+                        if (t < m_t || (t == m_t && item_cur < m_rank)) {
+                            m_t = t;
+                            m_tri = ti;
+                            m_rank = item_cur;
+                        }
+                    }
+                }
+                continue;
+            }
+            uint32_t rank = item_cur++;
+            int oi = (int)S.items[rank];
+            if (COUNT) cnt.item_refs++;
+            const DevObj &ob = S.objs[oi];
+            int type = ob.type;
+            if (type == SRAY_OBJ_MESH) {
+                if (!ANY && oi == cache_obj) {
+                    if (cache_tri != 0xffffffffu &&
+                        (cache_t < best.t || (cache_t == best.t && rank < best.rank))) {
+                        best.t = cache_t;
+                        best.obj = oi;
+                        best.tri = cache_tri;
+                        best.rank = rank;
+                    }
+                    continue;
+                }
+                if (COUNT) cnt.mesh_enters++;
+                if (item_cur < item_end) stk[(sp++) * stride] = make_uint2(item_cur, item_end | (E_RESUME << 30));
+                stk[(sp++) * stride] = make_uint2(rank, E_MARKER << 30);
+                float I[12];
+                obj_inv_at(ob, time, I);
+                r = make_tray(xf_point(I, wo), xf_vec(I, wd));
+                stk[(sp++) * stride] = make_uint2(ob.root, 1u | ((1u << r.dmask) << 8));
+                item_cur = item_end = 0;
+                level = 1;
+                cur_mesh = oi;
+                m_t = FLT_MAX;
+                m_tri = 0xffffffffu;
+                m_rank = 0xffffffffu;
+                continue;
+            }
+            float I[12];
+            obj_inv_at(ob, time, I);
+            V3 lo = xf_point(I, wo), ld = xf_vec(I, wd);
+            float t;
+            bool h;
+            if (type == SRAY_OBJ_SPHERE) {
+                if (COUNT) cnt.spheres++;
+                h = sphere_t(ob.rad, lo, ld, t);
+            } else {
+                if (COUNT) cnt.cylinders++;
+                h = cylinder_t(ob.rad, ob.start, ob.end, lo, ld, t);
+            }
+            if (h) {
+                if (ANY) {
+                    best.t = t;
+                    best.obj = oi;
+                    return best;
+                }
+                if (t < best.t || (t == best.t && rank < best.rank)) {
+                    best.t = t;
+                    best.obj = oi;
+                    best.tri = 0xffffffffu;
+                    best.rank = rank;
+                }
+            }
+            continue;
+        }
+        if (sp == 0) break;
+        uint2 e = stk[(sp - 1) * stride];
+        uint32_t ty = e.y >> 30;
+        if (ty == E_MARKER) {
+            sp--;
+            uint32_t rank = e.x;
+            int oi = (int)S.items[rank];
+            if (!ANY) {
+                cache_obj = oi;
+                cache_t = m_t;
+                cache_tri = m_tri;
+                if (m_tri != 0xffffffffu && (m_t < best.t || (m_t == best.t && rank < best.rank))) {
+                    best.t = m_t;
+                    best.obj = oi;
+                    best.tri = m_tri;
+                    best.rank = rank;
+                }
+            }
+            r = wr;
+            level = 0;
+            continue;
+        }
+        if (ty == E_RESUME) {
+            sp--;
+            item_cur = e.x;
+            item_end = e.y & 0x3fffffffu;
+            continue;
+        }
+        uint32_t exist = e.y & 0xffu, pend = (e.y >> 8) & 0xffu;
+        uint32_t k = (uint32_t)__builtin_ctz(pend);
+        pend &= pend - 1u;
+        if (pend == 0u) sp--;
+        else stk[(sp - 1) * stride].y = exist | (pend << 8);
+        uint32_t c = k ^ r.dmask;
+        uint32_t child = e.x + (uint32_t)__builtin_popcount(exist & ((1u << c) - 1u));
+        float4 lo = S.nlo[child], hi = S.nhi[child];
+        if (COUNT) cnt.nodes++;
+        float tmin;
+        if (!slab(lo, hi, r, tmin)) continue;
+        if (!ANY) {
+            float bound = level ? fminf(best.t, m_t) : best.t;
+            if (tmin > bound) continue;
+        }
+        uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
+        if (meta & LEAF) {
+            item_cur = first;
+            item_end = first + (meta & ~LEAF);
+        } else {
+            stk[(sp++) * stride] = make_uint2(first, meta | (order_mask(meta, r.dmask) << 8));
+        }
+    }
+    return best;
+}
+
+} // namespace srd
+
+#endif
