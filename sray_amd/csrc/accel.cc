@@ -354,9 +354,9 @@ const sray_scene_desc *HostScene::finish()
     desc.meshes = meshes.data();
     desc.num_triangles = (uint32_t)(tri_pos.size() / 9);
     desc.tri_pos = tri_pos.data();
-    desc.tri_normal = tri_normal.empty() ? nullptr : tri_normal.data();
-    desc.tri_tangent = tri_tangent.empty() ? nullptr : tri_tangent.data();
-    desc.tri_texcoord = tri_texcoord.empty() ? nullptr : tri_texcoord.data();
+    desc.tri_normal = has_normal ? tri_normal.data() : nullptr;
+    desc.tri_tangent = has_tangent ? tri_tangent.data() : nullptr;
+    desc.tri_texcoord = has_texcoord ? tri_texcoord.data() : nullptr;
     desc.num_materials = (uint32_t)materials.size();
     desc.materials = materials.data();
     desc.num_lights = (uint32_t)lights.size();

@@ -25,6 +25,7 @@ struct HostScene {
     std::vector<sray_object_desc> objects;
     std::vector<sray_mesh_desc> meshes;
     std::vector<float> tri_pos, tri_normal, tri_tangent, tri_texcoord;
+    bool has_normal = false, has_tangent = false, has_texcoord = false;
     std::vector<sray_material_desc> materials;
     std::vector<std::string> material_names;
     std::vector<sray_light_desc> lights;

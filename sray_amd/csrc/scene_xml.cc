@@ -598,12 +598,15 @@ bool load_mesh(const XmlNode &obj, HostScene &hs, sray_object_desc &o, std::stri
     /* attribute arrays are scene-wide: start them when the first mesh with
      * that attribute appears, zero-filled for earlier triangles */
     size_t ntri0 = hs.tri_pos.size() / 9;
-    auto ensure = [&](std::vector<float> &arr, size_t per) {
-        if (arr.empty()) arr.assign(ntri0 * per, 0.0f);
+    auto ensure = [&](bool &flag, std::vector<float> &arr, size_t per) {
+        if (!flag) {
+            flag = true;
+            arr.assign(ntri0 * per, 0.0f);
+        }
     };
-    if (has_n) ensure(hs.tri_normal, 9);
-    if (has_g) ensure(hs.tri_tangent, 9);
-    if (has_t) ensure(hs.tri_texcoord, 6);
+    if (has_n) ensure(hs.has_normal, hs.tri_normal, 9);
+    if (has_g) ensure(hs.has_tangent, hs.tri_tangent, 9);
+    if (has_t) ensure(hs.has_texcoord, hs.tri_texcoord, 6);
     sray_mesh_desc md;
     md.tri_start = (uint32_t)ntri0;
     md.tri_count = (uint32_t)faces.size();
@@ -615,15 +618,15 @@ bool load_mesh(const XmlNode &obj, HostScene &hs, sray_object_desc &o, std::stri
             }
             for (int k = 0; k < 3; k++) hs.tri_pos.push_back(ve[0][f.vi[j]].v[k]);
         }
-        if (!hs.tri_normal.empty())
+        if (hs.has_normal)
             for (int j = 0; j < 3; j++)
                 for (int k = 0; k < 3; k++)
                     hs.tri_normal.push_back(has_n && f.ni[j] >= 0 && f.ni[j] < (int)ve[1].size() ? ve[1][f.ni[j]].v[k] : 0.0f);
-        if (!hs.tri_tangent.empty())
+        if (hs.has_tangent)
             for (int j = 0; j < 3; j++)
                 for (int k = 0; k < 3; k++)
                     hs.tri_tangent.push_back(has_g && f.gi[j] >= 0 && f.gi[j] < (int)ve[3].size() ? ve[3][f.gi[j]].v[k] : 0.0f);
-        if (!hs.tri_texcoord.empty())
+        if (hs.has_texcoord)
             for (int j = 0; j < 3; j++)
                 for (int k = 0; k < 2; k++)
                     hs.tri_texcoord.push_back(has_t && f.ti[j] >= 0 && f.ti[j] < (int)ve[2].size() ? ve[2][f.ti[j]].v[k] : 0.0f);
@@ -962,7 +965,7 @@ std::string write_scene_xml(const HostScene &hs)
                     const float *v = &hs.tri_pos[(size_t)(md.tri_start + f) * 9 + c * 3];
                     o << "      <vertex id=\"" << (f * 3 + c) << "\" val=\"" << vec3(v) << "\"/>\n";
                 }
-            if (!hs.tri_normal.empty())
+            if (hs.has_normal)
                 for (uint32_t f = 0; f < md.tri_count; f++)
                     for (int c = 0; c < 3; c++) {
                         const float *v = &hs.tri_normal[(size_t)(md.tri_start + f) * 9 + c * 3];
@@ -973,7 +976,7 @@ std::string write_scene_xml(const HostScene &hs)
                 for (int c = 0; c < 3; c++) {
                     uint32_t id = f * 3 + c;
                     o << "<vref vertex=\"" << id << "\"";
-                    if (!hs.tri_normal.empty()) o << " normal=\"" << id << "\"";
+                    if (hs.has_normal) o << " normal=\"" << id << "\"";
                     o << "/>";
                 }
                 o << "</face>\n";

@@ -487,9 +487,14 @@ bool gen_batch_scene(uint64_t seed, int ntri, HostScene &hs)
             double dx = r.unit() - 0.5, dy = r.unit() - 0.5, dz = r.unit() - 0.5;
             double l = sqrt(dx * dx + dy * dy + dz * dz);
             double len = 0.5 * r.unit();
-            o.motion[0] = (float)(dx / l * len);
-            o.motion[1] = (float)(dy / l * len);
-            o.motion[2] = (float)(dz / l * len);
+            /* end position rounded to float first, so the scene file
+             * (two position keys) reproduces the motion exactly */
+            double m[3] = {dx / l * len, dy / l * len, dz / l * len};
+            const float p0[3] = {o.xform[3], o.xform[7], o.xform[11]};
+            for (int k = 0; k < 3; k++) {
+                float p1 = (float)((double)p0[k] + m[k]);
+                o.motion[k] = (float)((double)p1 - (double)p0[k]);
+            }
             o.motion_t0 = 0;
             o.motion_t1 = 40;
         }
