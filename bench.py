@@ -94,8 +94,6 @@ def main():
     if args.gpus > 1:
         sys.exit("bench.py measures one GPU only: every box here has one MI355X and no multi-GPU "
                  "figure is produced (run with --gpus 1)")
-    import torch
-
     import sray_amd as S
     wl = args.workload
     config, variant, W, H, it, samples = WORKLOADS[wl]
@@ -107,27 +105,27 @@ def main():
     log(args, f"scene {wl}: {hs.desc.num_triangles} triangles, {hs.desc.num_objects} objects, "
               f"upload+octree build {upload_s:.2f} s, {ctx.scene_bytes(sid) / 2**20:.1f} MiB resident")
     p = S.render_params(W, H, samples=samples, iter=it)
-    out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+    out = ctx.malloc(W * H * 4 * 4)
     for _ in range(args.warmup):
-        ctx.render_device(sid, p, out.data_ptr())
-    torch.cuda.synchronize()
+        ctx.render_device(sid, p, out)
+    ctx.synchronize()
     ctx.set_profiling(True)
     ctx.kernel_stats(reset=True)
     rays = 0
     dev_ms = 0.0
-    torch.cuda.synchronize()
+    ctx.synchronize()   # device-wide barrier before and after the timed steps (one GPU)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st = ctx.render_device(sid, p, out.data_ptr())
+        st = ctx.render_device(sid, p, out)
         rays += st.rays()
         dev_ms += st.render_ms
-    torch.cuda.synchronize()
+    ctx.synchronize()
     wall = time.perf_counter() - t0
     ctx.set_profiling(False)
     ks = ctx.kernel_stats(reset=True)
     # visit counters of the same frame (separate, untimed run)
     pc = S.render_params(W, H, samples=samples, iter=it, flags=S.abi.SRAY_RENDER_COUNT)
-    cst = ctx.render_device(sid, pc, out.data_ptr())
+    cst = ctx.render_device(sid, pc, out)
     cl = cst.closest
     closest_rays = cst.rays_primary + cst.rays_reflect + cst.rays_refract + cst.rays_diffuse
     alg_bytes = (32 + 16) * closest_rays + 32 * cl.nodes + 36 * cl.triangles + 16 * cl.spheres + \
@@ -169,6 +167,7 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    ctx.free(out)
     ctx.close()
 
 

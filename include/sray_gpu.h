@@ -341,6 +341,15 @@ int  sray_scene_generate(int config, uint64_t seed, int variant, sray_host_scene
  * size including the terminating NUL, buf may be NULL to query it */
 int  sray_scene_write_xml(const sray_host_scene *hs, char *buf, size_t cap, size_t *len);
 
+/* Device memory and synchronisation on the context's device, so callers
+ * (tests, benchmarks, the serve loop) need no second GPU runtime in the
+ * process.  kind: 0 host->device, 1 device->host, 2 device->device.
+ * No reference counterpart (the reference has no device). */
+int  sray_gpu_malloc(sray_gpu_ctx *ctx, size_t bytes, void **dptr);
+int  sray_gpu_free(sray_gpu_ctx *ctx, void *dptr);
+int  sray_gpu_memcpy(sray_gpu_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+int  sray_gpu_synchronize(sray_gpu_ctx *ctx);
+
 /* Per-kernel device time, measured with HIP events around each launch on
  * the context's stream while profiling is on (no reference counterpart;
  * the reference only prints total wall time, src/sray.cc:67-76). */

@@ -65,12 +65,16 @@ static inline float rng_unit(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, 
     return (float)(rng_u32(seed, a, b, c, d) >> 8) * (1.0f / 16777216.0f);
 }
 
-/* RNG "slot" words: which random site a draw belongs to */
-enum : uint32_t {
-    SLOT_CAMERA = 0u,          /* dims 0,1 jitter, 2 motion-blur time */
-    SLOT_LIGHT = 1u << 20,     /* | light<<12 | shadow sample<<2 | dim */
-    SLOT_DIFFUSE = 2u << 20,   /* | sample<<10 | attempt<<1 | dim */
-};
+/* RNG "slot" word: which random site a draw belongs to
+ *   kind<<30 | i1<<20 | i2<<10 | attempt<<3 | dim
+ * kind 0 camera (dim 0,1 jitter, 2 motion-blur time), 1 light sample
+ * (i1 light, i2 shadow sample), 2 diffuse GI direction (i1 sample index),
+ * 3 photon emission/bounce.  i1, i2 < 1024, attempt < 128, dim < 8. */
+static inline uint32_t slot(uint32_t kind, uint32_t i1, uint32_t i2, uint32_t att, uint32_t dim)
+{
+    return (kind << 30) | (i1 << 20) | (i2 << 10) | (att << 3) | dim;
+}
+enum : uint32_t { K_CAMERA = 0u, K_LIGHT = 1u, K_DIFFUSE = 2u, K_PHOTON = 3u };
 
 /* ------------------------------------------------------------------ */
 /* vector math                                                          */
@@ -428,8 +432,8 @@ template <class T> struct Scene {
     Octree<T> tree;
     bool use_tree = true;
     bool q9_retraverse = false;
-    T gather_dist_abs = 0;
-    T diag = 0;
+    double gather_dist_abs = 0;
+    double diag = 0;
     std::vector<Photon> pm[2];
     static constexpr T eps = (T)SRAY_ERROR_MARGIN;
 
@@ -513,49 +517,72 @@ template <class T> struct Scene {
 
     /* ---------------- primitives ---------------- */
 
-    /* Sphere::intersect, src/sphere.cc:85-132 (t only; attributes later) */
-    bool sphere_t(const Object<T> &o, const Ray<T> &lr, T &t) const
+    /* Sphere::intersect, src/sphere.cc:85-132 (t only; attributes later).
+     * The quadratic is evaluated in double for both precisions: with
+     * |dir| = 1e4, b*b - 4ac cancels about three digits, which in float
+     * puts hit points 1e-5 off the surface (self-shadowing the reference,
+     * computing in double, does not have). */
+    static bool sphere_td(double rad, V3<T> lo, V3<T> ld, double &t)
     {
-        T a = dot(lr.d, lr.d);
-        T b = (T)2 * lr.d.x * lr.o.x + (T)2 * lr.d.y * lr.o.y + (T)2 * lr.d.z * lr.o.z;
-        T c = dot(lr.o, lr.o) - o.rad * o.rad;
-        T d = b * b - (T)4 * a * c;
-        if (d < (T)0) return false;
-        T sd = std::sqrt(d);
-        T t1 = (-b + sd) / ((T)2 * a);
-        T t2 = (-b - sd) / ((T)2 * a);
-        if ((t1 < eps && t2 < eps) || (t1 > (T)1 && t2 > (T)1)) return false;
-        if (t1 < eps) t1 = t2;
-        if (t2 < eps) t2 = t1;
+        double ox = lo.x, oy = lo.y, oz = lo.z, dx = ld.x, dy = ld.y, dz = ld.z;
+        double a = dx * dx + dy * dy + dz * dz;
+        double b = 2.0 * dx * ox + 2.0 * dy * oy + 2.0 * dz * oz;
+        double c = ox * ox + oy * oy + oz * oz - rad * rad;
+        double d = b * b - 4.0 * a * c;
+        if (d < 0.0) return false;
+        double sd = std::sqrt(d);
+        double t1 = (-b + sd) / (2.0 * a);
+        double t2 = (-b - sd) / (2.0 * a);
+        const double e = SRAY_ERROR_MARGIN;
+        if ((t1 < e && t2 < e) || (t1 > 1.0 && t2 > 1.0)) return false;
+        if (t1 < e) t1 = t2;
+        if (t2 < e) t2 = t1;
         t = t1 < t2 ? t1 : t2; /* may exceed 1 (Q4) */
         return true;
     }
-
-    /* Cylinder::intersect, src/cylinder.cc:125-185 */
-    bool cylinder_t(const Object<T> &o, const Ray<T> &lr, T &t) const
+    bool sphere_t(const Object<T> &o, const Ray<T> &lr, T &t) const
     {
-        T a = lr.d.x * lr.d.x + lr.d.z * lr.d.z;
-        T b = (T)2 * lr.d.x * lr.o.x + (T)2 * lr.d.z * lr.o.z;
-        T c = lr.o.x * lr.o.x + lr.o.z * lr.o.z - o.rad * o.rad;
-        T d = b * b - (T)4 * a * c;
-        if (d < (T)0) return false;
-        T sd = std::sqrt(d);
-        T t1 = (-b + sd) / ((T)2 * a);
-        T t2 = (-b - sd) / ((T)2 * a);
-        if ((t1 < eps && t2 < eps) || (t1 > (T)1 && t2 > (T)1)) return false;
-        T y1 = lr.o.y + lr.d.y * t1;
-        T y2 = lr.o.y + lr.d.y * t2;
+        double td;
+        if (!sphere_td((double)o.rad, lr.o, lr.d, td)) return false;
+        t = (T)td;
+        return true;
+    }
+
+    /* Cylinder::intersect, src/cylinder.cc:125-185 (quadratic in double, as
+     * for the sphere) */
+    static bool cylinder_td(double rad, double ys, double ye, V3<T> lo, V3<T> ld, double &t)
+    {
+        double ox = lo.x, oy = lo.y, oz = lo.z, dx = ld.x, dy = ld.y, dz = ld.z;
+        double a = dx * dx + dz * dz;
+        double b = 2.0 * dx * ox + 2.0 * dz * oz;
+        double c = ox * ox + oz * oz - rad * rad;
+        double d = b * b - 4.0 * a * c;
+        if (d < 0.0) return false;
+        double sd = std::sqrt(d);
+        double t1 = (-b + sd) / (2.0 * a);
+        double t2 = (-b - sd) / (2.0 * a);
+        const double e = SRAY_ERROR_MARGIN;
+        if ((t1 < e && t2 < e) || (t1 > 1.0 && t2 > 1.0)) return false;
+        double y1 = oy + dy * t1;
+        double y2 = oy + dy * t2;
         bool v0 = true, v1 = true;
-        if (t1 < eps || y1 < o.start || y1 >= o.end) {
+        if (t1 < e || y1 < ys || y1 >= ye) {
             v0 = false;
             t1 = t2;
         }
-        if (t2 < eps || y2 < o.start || y2 >= o.end) {
+        if (t2 < e || y2 < ys || y2 >= ye) {
             v1 = false;
             t2 = t1;
         }
         if (!v0 && !v1) return false;
         t = t1 < t2 ? t1 : t2;
+        return true;
+    }
+    bool cylinder_t(const Object<T> &o, const Ray<T> &lr, T &t) const
+    {
+        double td;
+        if (!cylinder_td((double)o.rad, (double)o.start, (double)o.end, lr.o, lr.d, td)) return false;
+        t = (T)td;
         return true;
     }
 
@@ -791,6 +818,16 @@ template <class T> struct Scene {
         SurfPoint<T> sp;
         sp.dist = t;
         V3<T> p = lo + ld * t;
+        if (o.type != SRAY_OBJ_MESH) {
+            /* the surface point of a quadric from the double-precision root */
+            double td;
+            bool ok = o.type == SRAY_OBJ_SPHERE
+                          ? sphere_td((double)o.rad, lo, ld, td)
+                          : cylinder_td((double)o.rad, (double)o.start, (double)o.end, lo, ld, td);
+            if (ok)
+                p = v3<T>((T)((double)lo.x + (double)ld.x * td), (T)((double)lo.y + (double)ld.y * td),
+                          (T)((double)lo.z + (double)ld.z * td));
+        }
         if (o.type == SRAY_OBJ_SPHERE) {
             V3<T> nrm = p / o.rad;
             V3<T> tg = cross(v3<T>(0, 1, 0), nrm);
@@ -833,11 +870,13 @@ template <class T> struct Scene {
     }
 
     /* ---------------- photon gather, src/pmap.cc:107-178 ---------------- */
-    void gather_list(int kind, V3<T> pos, T radius, int k, std::vector<uint32_t> &out) const
+    /* kd_nearest_range3f takes float coordinates and a float radius */
+    void gather_list(int kind, V3<T> pos, double radius, int k, std::vector<uint32_t> &out) const
     {
         out.clear();
         const std::vector<Photon> &ph = pm[kind];
-        double r2 = (double)radius * (double)radius;
+        double rf = (double)(float)radius;
+        double r2 = rf * rf;
         std::vector<std::pair<double, uint32_t>> cand;
         for (uint32_t i = 0; i < ph.size(); i++) {
             double dx = (double)ph[i].pos[0] - (double)(float)pos.x;
@@ -853,7 +892,7 @@ template <class T> struct Scene {
         std::sort(cand.begin(), cand.end(), [](auto &a, auto &b) { return a.second < b.second; });
         for (auto &c : cand) out.push_back(c.second);
     }
-    C4<T> irradiance(V3<T> pos, V3<T> norm, T radius, int k) const
+    C4<T> irradiance(V3<T> pos, V3<T> norm, double radius, int k) const
     {
         std::vector<uint32_t> l;
         gather_list(SRAY_PMAP_CAUSTIC, pos, radius, k, l);
@@ -869,14 +908,14 @@ template <class T> struct Scene {
                 sz += (T)p.col[2];
             }
         }
-        T sc = (T)1 / ((T)M_PI * (radius * radius));
+        T sc = (T)(1.0 / (M_PI * (radius * radius)));
         irr.x = sx * sc;
         irr.y = sy * sc;
         irr.z = sz * sc;
         irr.w = 0;
         return irr;
     }
-    C4<T> radiance(V3<T> pos, V3<T> norm, T radius, int k) const
+    C4<T> radiance(V3<T> pos, V3<T> norm, double radius, int k) const
     {
         static const T max_cos = (T)std::cos(M_PI / 2.5);
         std::vector<uint32_t> l;
@@ -893,7 +932,7 @@ template <class T> struct Scene {
                 sz += (T)p.col[2] * ndotl;
             }
         }
-        T sc = (T)1 / ((T)M_PI * (radius * radius));
+        T sc = (T)(1.0 / (M_PI * (radius * radius)));
         return c4<T>(sx * sc, sy * sc, sz * sc, 0);
     }
 
@@ -910,17 +949,16 @@ template <class T> struct Scene {
      * SphLight src/light.cc:272-275, BoxLight src/light.cc:346-354 (time 0) */
     V3<T> light_point(const Light<T> &lt, const ShadeCtx &sc, uint32_t path, uint32_t li, uint32_t si) const
     {
-        uint32_t base = SLOT_LIGHT | (li << 12) | (si << 2);
         if (lt.type == SRAY_LIGHT_BOX) {
-            T fx = (T)rng_unit(sc.seed, sc.pixel, sc.sample, path, base | 0);
-            T fy = (T)rng_unit(sc.seed, sc.pixel, sc.sample, path, base | 1);
-            T fz = (T)rng_unit(sc.seed, sc.pixel, sc.sample, path, base | 2);
+            T fx = (T)rng_unit(sc.seed, sc.pixel, sc.sample, path, slot(K_LIGHT, li, si, 0, 0));
+            T fy = (T)rng_unit(sc.seed, sc.pixel, sc.sample, path, slot(K_LIGHT, li, si, 0, 1));
+            T fz = (T)rng_unit(sc.seed, sc.pixel, sc.sample, path, slot(K_LIGHT, li, si, 0, 2));
             V3<T> v = v3<T>(fx * lt.dim.x - (T)0.5 * lt.dim.x, fy * lt.dim.y - (T)0.5 * lt.dim.y,
                             fz * lt.dim.z - (T)0.5 * lt.dim.z);
             return xf_point(lt.xf, v);
         }
         if (lt.type == SRAY_LIGHT_SPHERE) {
-            V3<T> s = sph_rand(sc, path, base, lt.radius);
+            V3<T> s = sph_rand(sc, path, K_LIGHT, li, si, lt.radius);
             return lt.pos + s;
         }
         return lt.pos;
@@ -928,11 +966,11 @@ template <class T> struct Scene {
 
     /* uniform point on a sphere of radius r (Marsaglia 1972), a bounded
      * rejection loop over counter-based draws */
-    V3<T> sph_rand(const ShadeCtx &sc, uint32_t path, uint32_t base, T r) const
+    V3<T> sph_rand(const ShadeCtx &sc, uint32_t path, uint32_t kind, uint32_t i1, uint32_t i2, T r) const
     {
         for (uint32_t a = 0; a < 64; a++) {
-            float u1 = rng_unit(sc.seed, sc.pixel, sc.sample, path, base ^ (0x800u | (a << 4) | 0));
-            float u2 = rng_unit(sc.seed, sc.pixel, sc.sample, path, base ^ (0x800u | (a << 4) | 1));
+            float u1 = rng_unit(sc.seed, sc.pixel, sc.sample, path, slot(kind, i1, i2, a, 0));
+            float u2 = rng_unit(sc.seed, sc.pixel, sc.sample, path, slot(kind, i1, i2, a, 1));
             float x1 = 2.0f * u1 - 1.0f, x2 = 2.0f * u2 - 1.0f;
             float s = x1 * x1 + x2 * x2;
             if (s < 1.0f && s > 0.0f) {
@@ -1061,8 +1099,7 @@ template <class T> struct Scene {
         C4<T> gi = c4<T>(0, 0, 0, 0);
         if (sc.p->gi_photons) {
             for (int i = 0; i < sc.p->diffuse_samples; i++) {
-                uint32_t base = SLOT_DIFFUSE | ((uint32_t)i << 10);
-                V3<T> dir = sph_rand(sc, path, base, (T)1);
+                V3<T> dir = sph_rand(sc, path, K_DIFFUSE, (uint32_t)i, 0, (T)1);
                 T ndotl = dot(dir, normal);
                 if (ndotl < (T)0) dir = -dir;
                 T diff_energy = ndotl * ray.energy;
@@ -1073,7 +1110,7 @@ template <class T> struct Scene {
                     dr.o = sp.pos;
                     dr.d = dir * ray_mag;
                     sc.ray_counts[4]++;
-                    gi = gi + trace(dr, sc, path) * ndotl;
+                    gi = gi + trace(dr, sc, path | 0x40000000u | ((uint32_t)i << 20)) * ndotl;
                 }
             }
             if (sc.p->diffuse_samples > 1) gi = gi * ((T)1 / (T)sc.p->diffuse_samples);
@@ -1146,8 +1183,8 @@ template <class T> struct Scene {
         T py = (T)1 - ((T)y / (T)p->height) * ysz;
         uint32_t pix = (uint32_t)y * (uint32_t)p->width + (uint32_t)x;
         if (sub > 0) {
-            T f0 = (T)rng_unit(seed, pix, (uint32_t)sub, 0, SLOT_CAMERA | 0);
-            T f1 = (T)rng_unit(seed, pix, (uint32_t)sub, 0, SLOT_CAMERA | 1);
+            T f0 = (T)rng_unit(seed, pix, (uint32_t)sub, 0, slot(K_CAMERA, 0, 0, 0, 0));
+            T f1 = (T)rng_unit(seed, pix, (uint32_t)sub, 0, slot(K_CAMERA, 0, 0, 0, 1));
             px = px + xsz * (f0 - (T)0.5) / (T)p->width;
             py = py + ysz * (f1 - (T)0.5) / (T)p->height;
         }
@@ -1159,7 +1196,7 @@ template <class T> struct Scene {
         r.d = v3<T>(px * max_dist, py * max_dist, fz * max_dist);
         r.time = time;
         if (p->mblur && sub > 0 && shutter > 0) {
-            uint32_t u = rng_u32(seed, pix, (uint32_t)sub, 0, SLOT_CAMERA | 2);
+            uint32_t u = rng_u32(seed, pix, (uint32_t)sub, 0, slot(K_CAMERA, 0, 0, 0, 2));
             r.time += (int)(u % (uint32_t)shutter) - shutter / 2;
         }
         Ray<T> w = r;
@@ -1266,8 +1303,15 @@ template <class T> static void load(Scene<T> &S, const sray_scene_desc *d, uint3
         S.tree.add(u);
     }
     S.tree.build();
+    /* diagonal in double from the root box (src/scene.cc:384-387) */
     const Box<T> &rb = S.tree.nodes[0].box;
-    S.diag = length(rb.mx - rb.mn);
+    if (d->num_objects) {
+        double dx = (double)rb.mx.x - (double)rb.mn.x, dy = (double)rb.mx.y - (double)rb.mn.y,
+               dz = (double)rb.mx.z - (double)rb.mn.z;
+        S.diag = std::sqrt(dx * dx + dy * dy + dz * dz);
+    } else {
+        S.diag = 0;
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1489,7 +1533,7 @@ template <class T> static int do_tree(Scene<T> &S, int level, oracle_tree_info *
 template <class T> static int do_render(Scene<T> &S, const sray_render_params *p, float *rgba, int nthreads, sray_stats *st)
 {
     /* gather radius relative to the scene octree diagonal (Q19) */
-    S.gather_dist_abs = S.diag * (T)p->gather_dist;
+    S.gather_dist_abs = S.diag * (double)p->gather_dist;
     Renderer<T> R(S, p, rgba);
     R.run(nthreads);
     if (st) {
@@ -1533,11 +1577,11 @@ template <class T> static int do_gather(Scene<T> &S, int kind, const sray_gather
         if (mode == SRAY_GATHER_IRRADIANCE) {
             /* the irradiance estimate reads the map of the given kind */
             std::swap(S.pm[SRAY_PMAP_CAUSTIC], S.pm[kind]);
-            c = S.irradiance(pos, nrm, (T)radius, k);
+            c = S.irradiance(pos, nrm, (double)radius, k);
             std::swap(S.pm[SRAY_PMAP_CAUSTIC], S.pm[kind]);
         } else {
             std::swap(S.pm[SRAY_PMAP_GLOBAL], S.pm[kind]);
-            c = S.radiance(pos, nrm, (T)radius, k);
+            c = S.radiance(pos, nrm, (double)radius, k);
             std::swap(S.pm[SRAY_PMAP_GLOBAL], S.pm[kind]);
         }
         rgb[i * 3 + 0] = (float)c.x;
@@ -1627,7 +1671,7 @@ int oracle_gather(oracle_scene *s, int kind, const sray_gather_query *q, uint32_
 double oracle_scene_diag(oracle_scene *s)
 {
     if (!s) return 0;
-    return s->s.prec == 32 ? (double)s->s.f->diag : s->s.d->diag;
+    return s->s.prec == 32 ? s->s.f->diag : s->s.d->diag;
 }
 
 uint32_t oracle_rng_u32(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d)

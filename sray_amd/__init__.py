@@ -220,6 +220,29 @@ class Context:
         self._check(lib().sray_gpu_render_device(self._h, sid, C.byref(params), C.c_void_p(d_out), C.byref(st)))
         return st
 
+    # ---- device memory (the library's own runtime; no torch needed)
+    def malloc(self, nbytes):
+        p = C.c_void_p()
+        self._check(lib().sray_gpu_malloc(self._h, int(nbytes), C.byref(p)))
+        return p.value
+
+    def free(self, dptr):
+        self._check(lib().sray_gpu_free(self._h, C.c_void_p(dptr)))
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        d = self.malloc(arr.nbytes)
+        self._check(lib().sray_gpu_memcpy(self._h, C.c_void_p(d), C.c_void_p(arr.ctypes.data), arr.nbytes, 0))
+        return d
+
+    def to_host(self, dptr, nbytes, dtype):
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+        self._check(lib().sray_gpu_memcpy(self._h, C.c_void_p(out.ctypes.data), C.c_void_p(dptr), nbytes, 1))
+        return out
+
+    def synchronize(self):
+        self._check(lib().sray_gpu_synchronize(self._h))
+
     def set_profiling(self, on=True):
         self._check(lib().sray_gpu_set_profiling(self._h, 1 if on else 0))
 

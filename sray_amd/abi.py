@@ -133,6 +133,7 @@ EXPORTS = [
     "sray_scene_load_xml", "sray_scene_desc_of", "sray_scene_free", "sray_host_last_error",
     "sray_octree_build_host", "sray_scene_generate", "sray_scene_write_xml", "sray_abi_sizes",
     "sray_gpu_primary_rays_device", "sray_gpu_set_profiling", "sray_gpu_kernel_stats",
+    "sray_gpu_malloc", "sray_gpu_free", "sray_gpu_memcpy", "sray_gpu_synchronize",
 ]
 
 
@@ -167,6 +168,10 @@ def declare(lib):
         "sray_scene_write_xml": (i32, [vp, C.c_char_p, C.c_size_t, P(C.c_size_t)]),
         "sray_abi_sizes": (i32, [P(u32), C.c_int]),
         "sray_gpu_set_profiling": (i32, [vp, C.c_int]),
+        "sray_gpu_malloc": (i32, [vp, C.c_size_t, P(vp)]),
+        "sray_gpu_free": (i32, [vp, vp]),
+        "sray_gpu_memcpy": (i32, [vp, vp, vp, C.c_size_t, C.c_int]),
+        "sray_gpu_synchronize": (i32, [vp]),
         "sray_gpu_kernel_stats": (i32, [vp, P(KernelStat), C.c_int, P(C.c_int), C.c_int]),
     }
     for name, (res, args) in sig.items():

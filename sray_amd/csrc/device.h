@@ -152,9 +152,13 @@ __device__ __forceinline__ float rng_unit(uint64_t seed, uint32_t a, uint32_t b,
 {
     return (float)(rng_u32(seed, a, b, c, d) >> 8) * (1.0f / 16777216.0f);
 }
-constexpr uint32_t SLOT_CAMERA = 0u;
-constexpr uint32_t SLOT_LIGHT = 1u << 20;
-constexpr uint32_t SLOT_DIFFUSE = 2u << 20;
+/* slot word: kind<<30 | i1<<20 | i2<<10 | attempt<<3 | dim (kind 0 camera,
+ * 1 light sample, 2 diffuse GI direction, 3 photon emission/bounce) */
+__device__ __forceinline__ uint32_t rslot(uint32_t kind, uint32_t i1, uint32_t i2, uint32_t att, uint32_t dim)
+{
+    return (kind << 30) | (i1 << 20) | (i2 << 10) | (att << 3) | dim;
+}
+constexpr uint32_t K_CAMERA = 0u, K_LIGHT = 1u, K_DIFFUSE = 2u, K_PHOTON = 3u;
 
 } // namespace srd
 

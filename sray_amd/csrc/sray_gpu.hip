@@ -8,6 +8,8 @@
  * every float operation is the IEEE one the CPU checker performs.
  */
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_scan.hpp>
 #include <hipcub/device/device_select.hpp>
 
 #include <algorithm>
@@ -27,6 +29,7 @@
 #include <vector>
 
 #include "host.h"
+#include "pmap.h"
 #include "render.h"
 
 using namespace srd;
@@ -78,15 +81,15 @@ __device__ __forceinline__ void primary_ray(const SceneDev &S, const RenderDev &
     float py = 1.0f - ((float)y / (float)P.height) * ysz;
     uint32_t pix = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
     if (sample > 0) {
-        float f0 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, SLOT_CAMERA | 0u);
-        float f1 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, SLOT_CAMERA | 1u);
+        float f0 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 0));
+        float f1 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 1));
         px = px + xsz * (f0 - 0.5f) / (float)P.width;
         py = py + ysz * (f1 - 0.5f) / (float)P.height;
     }
     V3 dl = v3(px * S.max_dist, py * S.max_dist, S.fz * S.max_dist);
     time = P.frame_time;
     if (P.mblur && sample > 0 && S.shutter > 0) {
-        uint32_t u = rng_u32(P.seed, pix, (uint32_t)sample, 0u, SLOT_CAMERA | 2u);
+        uint32_t u = rng_u32(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 2));
         time += (int)(u % (uint32_t)S.shutter) - S.shutter / 2;
     }
     o = xf_point(S.cam, v3(0.0f, 0.0f, 0.0f));
@@ -180,13 +183,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_shadow(SceneDev S, const 
 
 /* light sample point: Light::get_point src/light.cc:106-109, SphLight
  * :272-275, BoxLight :346-354 (matrix at time 0) */
-__device__ __forceinline__ V3 sph_rand(uint64_t seed, uint32_t pix, uint32_t sample, uint32_t path, uint32_t base,
-                                       float r)
+__device__ __forceinline__ V3 sph_rand(uint64_t seed, uint32_t pix, uint32_t sample, uint32_t path, uint32_t kind,
+                                       uint32_t i1, uint32_t i2, float r)
 {
     /* Marsaglia (1972): uniform on the sphere from a rejection pair */
     for (uint32_t a = 0; a < 64; a++) {
-        float u1 = rng_unit(seed, pix, sample, path, base ^ (0x800u | (a << 4) | 0u));
-        float u2 = rng_unit(seed, pix, sample, path, base ^ (0x800u | (a << 4) | 1u));
+        float u1 = rng_unit(seed, pix, sample, path, rslot(kind, i1, i2, a, 0));
+        float u2 = rng_unit(seed, pix, sample, path, rslot(kind, i1, i2, a, 1));
         float x1 = 2.0f * u1 - 1.0f, x2 = 2.0f * u2 - 1.0f;
         float s = x1 * x1 + x2 * x2;
         if (s < 1.0f && s > 0.0f) {
@@ -201,17 +204,16 @@ __device__ __forceinline__ V3 sph_rand(uint64_t seed, uint32_t pix, uint32_t sam
 __device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uint32_t pix, uint32_t sample,
                                           uint32_t path, uint32_t li, uint32_t si)
 {
-    uint32_t base = SLOT_LIGHT | (li << 12) | (si << 2);
     if (lt.type == SRAY_LIGHT_BOX) {
-        float fx = rng_unit(seed, pix, sample, path, base | 0u);
-        float fy = rng_unit(seed, pix, sample, path, base | 1u);
-        float fz = rng_unit(seed, pix, sample, path, base | 2u);
+        float fx = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 0));
+        float fy = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 1));
+        float fz = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 2));
         V3 v = v3(fx * lt.dim[0] - 0.5f * lt.dim[0], fy * lt.dim[1] - 0.5f * lt.dim[1],
                   fz * lt.dim[2] - 0.5f * lt.dim[2]);
         return xf_point(lt.xf, v);
     }
     if (lt.type == SRAY_LIGHT_SPHERE) {
-        V3 s = sph_rand(seed, pix, sample, path, base, lt.radius);
+        V3 s = sph_rand(seed, pix, sample, path, K_LIGHT, li, si, lt.radius);
         return add(v3(lt.pos[0], lt.pos[1], lt.pos[2]), s);
     }
     return v3(lt.pos[0], lt.pos[1], lt.pos[2]);
@@ -334,7 +336,24 @@ __device__ __forceinline__ uint32_t pixel_id(const RenderDev &P, const uint32_t 
 
 /* shade_phong (src/shader.cc:83-247) / shade_undef (:36-71) for one hit,
  * emitting direct terms, shadow rays and children */
-__global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, const uint32_t *__restrict__ active,
+struct ShadePmaps {
+    PmapTree caust, glob;
+    float radius;     /* inclusion radius (float, as kd_nearest_range3f) */
+    double scale;     /* 1 / (pi r^2) with the double gather distance */
+};
+
+__device__ __forceinline__ V3 gather_est(const PmapTree &T, V3 pos, V3 n, float radius, double scale, int mode,
+                                         uint2 *stk)
+{
+    GatherAcc a;
+    gather_range(T, pos.x, pos.y, pos.z, radius, mode, (double)n.x, (double)n.y, (double)n.z, a, stk, 256);
+    if (mode == SRAY_GATHER_IRRADIANCE && a.count < 8) return v3(0.0f, 0.0f, 0.0f);
+    return v3((float)(a.r * scale), (float)(a.g * scale), (float)(a.b * scale));
+}
+
+template <bool PHOTONS>
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, ShadePmaps PM,
+                                               const uint32_t *__restrict__ active,
                                                uint32_t c0, int sample, const QRay *__restrict__ q,
                                                const int4 *__restrict__ hits, uint32_t n, float4 *__restrict__ direct,
                                                SRay *__restrict__ sq, QRay *__restrict__ kids,
@@ -342,6 +361,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, const ui
                                                uint32_t *__restrict__ err)
 {
     __shared__ uint32_t bcnt[5];
+    __shared__ uint2 gstk[PHOTONS ? PMAP_STACK * 256 : 1];
     if (threadIdx.x < 5) bcnt[threadIdx.x] = 0;
     __syncthreads();
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -373,6 +393,16 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, const ui
             obj_xf_at(ob, time, M);
             V3 lo = xf_point(I, ro), ld = xf_vec(I, rd);
             V3 p = add(lo, mul(ld, t));
+            if (ob.type != SRAY_OBJ_MESH) {
+                /* surface point of a quadric from the double-precision root */
+                double td;
+                bool ok = ob.type == SRAY_OBJ_SPHERE
+                              ? sphere_td((double)ob.rad, lo, ld, td)
+                              : cylinder_td((double)ob.rad, (double)ob.start, (double)ob.end, lo, ld, td);
+                if (ok)
+                    p = v3((float)((double)lo.x + (double)ld.x * td), (float)((double)lo.y + (double)ld.y * td),
+                           (float)((double)lo.z + (double)ld.z * td));
+            }
             V3 N;
             float tu = 0.0f, tv = 0.0f;
             const DevMat &mat = S.mats[ob.mat];
@@ -454,8 +484,14 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, const ui
                 float4 kd = mat.kd;
                 float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
                 if (flg & QF_DIFFUSE) {
-                    /* diffuse GI ray: global-map radiance estimate (gathered by a
-                     * separate pass; zero when no map is bound) */
+                    /* diffuse GI ray: radiance estimate from the global map
+                     * (src/shader.cc:119-123) */
+                    V3 rad = PHOTONS ? gather_est(PM.glob, Pw, normal, PM.radius, PM.scale, SRAY_GATHER_RADIANCE,
+                                                  &gstk[threadIdx.x])
+                                     : v3(0.0f, 0.0f, 0.0f);
+                    dir.x = W.x * (rad.x * kd.x);
+                    dir.y = W.y * (rad.y * kd.y);
+                    dir.z = W.z * (rad.z * kd.z);
                     direct[i] = dir;
                 } else {
                     float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
@@ -500,6 +536,40 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, const ui
                             slotk++;
                             c_sh++;
                         }
+                    }
+                    if (PHOTONS && P.gi) {
+                        /* diffuse hemisphere rays (src/shader.cc:158-185); a
+                         * direction flipped into the hemisphere keeps its
+                         * negative ndotl and so spawns no ray */
+                        float dsc = P.diffuse_samples > 1 ? 1.0f / (float)P.diffuse_samples : 1.0f;
+                        for (int di = 0; di < P.diffuse_samples; di++) {
+                            V3 gd = sph_rand(P.seed, pix, (uint32_t)sample, path, K_DIFFUSE, (uint32_t)di, 0u, 1.0f);
+                            float ndotl = dot(gd, normal);
+                            if (ndotl < 0.0f) gd = neg(gd);
+                            float diff_energy = ndotl * energy;
+                            if (diff_energy > P.min_energy) {
+                                QRay cr;
+                                V3 dd = mul(gd, ray_mag);
+                                cr.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                                cr.d = make_float4(dd.x, dd.y, dd.z, diff_energy);
+                                float wf = ndotl * dsc;
+                                cr.w = make_float4(W.x * (kd.x * wf), W.y * (kd.y * wf), W.z * (kd.z * wf), R.w.w);
+                                cr.m = make_uint4(slot, path | 0x40000000u | ((uint32_t)di << 20), 0x7fffffffu,
+                                                  (R.m.w & 0xffu) | (QF_DIFFUSE << 8));
+                                cr.stk = R.stk;
+                                kids[(size_t)i * P.K + 2 + di] = cr;
+                                kflag[(size_t)i * P.K + 2 + di] = 1;
+                                c_df++;
+                            }
+                        }
+                    }
+                    if (PHOTONS && P.caust) {
+                        /* caustics: irradiance from the caustic map (src/shader.cc:231-236) */
+                        V3 irr = gather_est(PM.caust, Pw, normal, PM.radius, PM.scale, SRAY_GATHER_IRRADIANCE,
+                                            &gstk[threadIdx.x]);
+                        dir.x = dir.x + W.x * irr.x;
+                        dir.y = dir.y + W.y * irr.y;
+                        dir.z = dir.z + W.z * irr.z;
                     }
                     V3 nrd = dvs(neg(rd), ray_mag);
                     float ray_dot_n = dot(nrd, normal);
@@ -670,6 +740,259 @@ __global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__
     out[fi] = out[fi] * rcp;
 }
 
+/* ------------------------------------------------------------------ */
+/* photon-map kd-tree build (level by level) and gathers                */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ uint32_t fkey(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void k_pm_init(const sray_photon *__restrict__ ph, uint32_t n, float4 *__restrict__ pos0,
+                          uint32_t *__restrict__ ord)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos0[i] = make_float4(ph[i].pos[0], ph[i].pos[1], ph[i].pos[2], 0.0f);
+    ord[i] = i;
+}
+
+/* segment of every array position at this level (-1: a finished node) */
+__global__ void k_pm_seg(const uint32_t *__restrict__ slo, const uint32_t *__restrict__ shi, uint32_t nseg,
+                         uint32_t n, int32_t *__restrict__ seg_of)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t a = 0, b = nseg;
+    while (b - a > 1) {
+        uint32_t m = (a + b) >> 1;
+        if (slo[m] <= i) a = m;
+        else b = m;
+    }
+    seg_of[i] = (nseg && slo[a] <= i && i < shi[a]) ? (int32_t)a : -1;
+}
+
+__global__ void k_pm_bbox(const uint32_t *__restrict__ ord, const float4 *__restrict__ pos0,
+                          const int32_t *__restrict__ seg_of, uint32_t n, uint32_t *__restrict__ bb)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = i < n ? seg_of[i] : -1;
+    float4 p = s >= 0 ? pos0[ord[i]] : make_float4(0, 0, 0, 0);
+    uint32_t k[3] = {fkey(p.x), fkey(p.y), fkey(p.z)};
+    /* whole-wave fast path: one segment for all active lanes */
+    int s0 = __shfl(s, 0);
+    bool same = __all(s == s0) && s0 >= 0;
+    if (same) {
+        for (int a = 0; a < 3; a++) {
+            uint32_t mn = k[a], mx = k[a];
+            for (int off = 32; off > 0; off >>= 1) {
+                mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+                mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+            }
+            if ((threadIdx.x & 63) == 0) {
+                atomicMin(&bb[s0 * 6 + a], mn);
+                atomicMax(&bb[s0 * 6 + 3 + a], mx);
+            }
+        }
+    } else if (s >= 0) {
+        for (int a = 0; a < 3; a++) {
+            atomicMin(&bb[s * 6 + a], k[a]);
+            atomicMax(&bb[s * 6 + 3 + a], k[a]);
+        }
+    }
+}
+
+__device__ __forceinline__ float unkey(uint32_t k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void k_pm_bbinit(uint32_t *bb, uint32_t nseg)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg * 6) return;
+    bb[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
+}
+
+/* split axis = largest extent of the segment (ties: lowest axis) */
+__global__ void k_pm_axis(const uint32_t *__restrict__ bb, uint32_t nseg, uint8_t *__restrict__ saxis)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    float e[3];
+    for (int a = 0; a < 3; a++) e[a] = unkey(bb[s * 6 + 3 + a]) - unkey(bb[s * 6 + a]);
+    uint8_t ax = 0;
+    if (e[1] > e[ax]) ax = 1;
+    if (e[2] > e[ax]) ax = 2;
+    saxis[s] = ax;
+}
+
+__global__ void k_pm_keys(const uint32_t *__restrict__ ord, const float4 *__restrict__ pos0,
+                          const int32_t *__restrict__ seg_of, const uint32_t *__restrict__ slo,
+                          const uint8_t *__restrict__ saxis, uint32_t n, unsigned long long *__restrict__ key)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = seg_of[i];
+    if (s < 0) {
+        key[i] = (unsigned long long)i << 32;
+        return;
+    }
+    float4 p = pos0[ord[i]];
+    uint32_t a = saxis[s];
+    float c = a == 0 ? p.x : a == 1 ? p.y : p.z;
+    key[i] = ((unsigned long long)slo[s] << 32) | fkey(c);
+}
+
+__global__ void k_pm_nodes(const uint32_t *__restrict__ slo, const uint32_t *__restrict__ shi,
+                           const uint8_t *__restrict__ saxis, uint32_t nseg, uint8_t *__restrict__ axis)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    axis[(slo[s] + shi[s]) >> 1] = saxis[s];
+}
+
+__global__ void k_pm_final(const sray_photon *__restrict__ ph, const uint32_t *__restrict__ ord, uint32_t n,
+                           float4 *__restrict__ pos, float4 *__restrict__ dir, float4 *__restrict__ nrm,
+                           float *__restrict__ colb)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t o = ord[i];
+    const sray_photon &p = ph[o];
+    pos[i] = make_float4(p.pos[0], p.pos[1], p.pos[2], __uint_as_float(o));
+    dir[i] = make_float4(p.dir[0], p.dir[1], p.dir[2], p.col[0]);
+    nrm[i] = make_float4(p.norm[0], p.norm[1], p.norm[2], p.col[1]);
+    colb[i] = p.col[2];
+}
+
+__device__ __forceinline__ void finish_estimate(const GatherAcc &a, int mode, double scale, float *out)
+{
+    if (mode == SRAY_GATHER_IRRADIANCE && a.count < 8) {
+        out[0] = out[1] = out[2] = 0.0f;
+        return;
+    }
+    out[0] = (float)(a.r * scale);
+    out[1] = (float)(a.g * scale);
+    out[2] = (float)(a.b * scale);
+}
+
+/* range gather; with k > 0, queries holding more than k photons are flagged
+ * for the k-nearest pass */
+__global__ __launch_bounds__(128) void k_pm_gather(PmapTree T, const sray_gather_query *__restrict__ q, uint32_t n,
+                                                   float radius, int mode, int k, float *__restrict__ rgb,
+                                                   uint32_t *__restrict__ count, uint8_t *__restrict__ need)
+{
+    __shared__ uint2 stk[PMAP_STACK * 128];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const sray_gather_query &Q = q[i];
+    GatherAcc a;
+    gather_range(T, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
+                 (double)Q.norm[2], a, &stk[threadIdx.x], 128);
+    count[i] = a.count;
+    bool sel = k > 0 && a.count > (uint32_t)k;
+    need[i] = sel ? 1 : 0;
+    if (!sel) finish_estimate(a, mode, 1.0 / (M_PI * ((double)radius * (double)radius)), rgb + (size_t)i * 3);
+}
+
+struct Cand {
+    double d2;
+    uint32_t idx;  /* original photon index (tie-break) */
+    uint32_t pos;  /* tree position */
+};
+
+__global__ __launch_bounds__(128) void k_pm_collect(PmapTree T, const sray_gather_query *__restrict__ q,
+                                                    const uint32_t *__restrict__ fq, uint32_t nf,
+                                                    const uint32_t *__restrict__ off, float radius,
+                                                    Cand *__restrict__ cand)
+{
+    __shared__ uint2 stk[PMAP_STACK * 128];
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nf) return;
+    const sray_gather_query &Q = q[fq[j]];
+    uint2 *st = &stk[threadIdx.x];
+    const int stride = 128;
+    double r2 = (double)radius * (double)radius;
+    double qq[3] = {(double)Q.pos[0], (double)Q.pos[1], (double)Q.pos[2]};
+    uint32_t w = off[j];
+    int sp = 1;
+    st[0] = make_uint2(0u, T.n);
+    while (sp > 0) {
+        uint2 s = st[(--sp) * stride];
+        uint32_t lo = s.x, hi = s.y;
+        while (lo < hi) {
+            uint32_t m = (lo + hi) >> 1;
+            float4 p = T.pos[m];
+            double dx = (double)p.x - qq[0], dy = (double)p.y - qq[1], dz = (double)p.z - qq[2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 <= r2) {
+                Cand c;
+                c.d2 = d2;
+                c.idx = __float_as_uint(p.w);
+                c.pos = m;
+                cand[w++] = c;
+            }
+            uint32_t ax = T.axis[m];
+            double pa = ax == 0 ? (double)p.x : ax == 1 ? (double)p.y : (double)p.z;
+            double diff = qq[ax] - pa;
+            uint32_t nlo, nhi, flo, fhi;
+            if (diff <= 0.0) {
+                nlo = lo, nhi = m, flo = m + 1, fhi = hi;
+            } else {
+                nlo = m + 1, nhi = hi, flo = lo, fhi = m;
+            }
+            if (diff * diff <= r2 && flo < fhi && sp < PMAP_STACK) st[(sp++) * stride] = make_uint2(flo, fhi);
+            lo = nlo;
+            hi = nhi;
+        }
+    }
+}
+
+__device__ __forceinline__ bool cand_less(const Cand &a, const Cand &b)
+{
+    return a.d2 < b.d2 || (a.d2 == b.d2 && a.idx < b.idx);
+}
+
+/* k smallest (d2, photon index) of each flagged query's candidates
+ * (in-place quickselect), then the estimate over them */
+__global__ void k_pm_select(PmapTree T, const sray_gather_query *__restrict__ q, const uint32_t *__restrict__ fq,
+                            uint32_t nf, const uint32_t *__restrict__ off, const uint32_t *__restrict__ cnt,
+                            Cand *__restrict__ cand, int k, float radius, int mode, float *__restrict__ rgb)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nf) return;
+    uint32_t qi = fq[j];
+    Cand *c = cand + off[j];
+    int lo = 0, hi = (int)cnt[qi] - 1, kk = k - 1;
+    while (lo < hi) {
+        Cand pv = c[(lo + hi) >> 1];
+        int a = lo, b = hi;
+        while (a <= b) {
+            while (cand_less(c[a], pv)) a++;
+            while (cand_less(pv, c[b])) b--;
+            if (a <= b) {
+                Cand t = c[a];
+                c[a] = c[b];
+                c[b] = t;
+                a++;
+                b--;
+            }
+        }
+        if (kk <= b) hi = b;
+        else if (kk >= a) lo = a;
+        else break;
+    }
+    const sray_gather_query &Q = q[qi];
+    GatherAcc acc;
+    acc.r = acc.g = acc.b = 0.0;
+    acc.count = 0;
+    for (int e = 0; e < k; e++)
+        gather_photon(T, c[e].pos, mode, (double)Q.norm[0], (double)Q.norm[1], (double)Q.norm[2], acc);
+    finish_estimate(acc, mode, 1.0 / (M_PI * ((double)radius * (double)radius)), rgb + (size_t)qi * 3);
+}
+
 /* ================================================================== */
 /* runtime                                                              */
 /* ================================================================== */
@@ -682,6 +1005,14 @@ struct DevBuf {
 
 template <class T> T *dptr(DevBuf &b) { return (T *)b.p; }
 
+struct GpuPmap {
+    std::vector<DevBuf> bufs;
+    PmapTree tree{};
+    uint32_t scene_id = 0;
+    int kind = 0;
+    uint64_t bytes = 0;
+};
+
 struct GpuScene {
     std::vector<DevBuf> bufs;
     SceneDev dev{};
@@ -689,6 +1020,7 @@ struct GpuScene {
     double diag = 0;
     uint32_t scene_height = 0, mesh_height = 0;
     uint32_t nobjs = 0;
+    uint32_t pmap[2] = {0, 0}; /* bound caustic / global photon maps */
 };
 
 } // namespace
@@ -700,11 +1032,13 @@ struct sray_gpu_ctx {
     std::string err;
     std::map<uint32_t, std::unique_ptr<GpuScene>> scenes;
     uint32_t next_scene = 1;
+    std::map<uint32_t, std::unique_ptr<GpuPmap>> pmaps;
+    uint32_t next_pmap = 1;
     uint64_t admission_cap = 200ull << 30;
     /* render scratch, grown on demand */
     std::vector<DevBuf> qlev;
     DevBuf hits, direct, sq, occ, kids, kflag, acc, fb, subpix, nsamp, act[2], keep, stats, errf, cubtmp, nsel, rays,
-        hitbuf, counts;
+        hitbuf, counts, gq, grgb, gcnt, gneed, gfq, goff, gcand, giota;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     /* per-kernel HIP-event timing */
     bool profiling = false;
@@ -1056,8 +1390,9 @@ int validate_params(sray_gpu_ctx *c, const sray_render_params *p)
     if (p->max_samples > 256) return fail(c, SRAY_ERR_LIMIT, "max_samples > 256");
     if (p->iter < 0 || p->iter > 30) return fail(c, SRAY_ERR_LIMIT, "iter outside [0,30]");
     if (p->shadow_samples > 1024) return fail(c, SRAY_ERR_LIMIT, "shadow_samples > 1024");
-    if (p->gi_photons || p->caust_photons)
-        return fail(c, SRAY_ERR_ARG, "photon-mapped rendering needs a bound photon map (not available in this build)");
+    if (p->diffuse_samples > 1023 || p->diffuse_samples < 0) return fail(c, SRAY_ERR_LIMIT, "diffuse_samples outside [0,1023]");
+    if ((p->gi_photons || p->caust_photons) && !(p->gather_dist > 0.0f))
+        return fail(c, SRAY_ERR_ARG, "gather_dist must be > 0 with photon maps");
     return SRAY_OK;
 }
 
@@ -1067,6 +1402,7 @@ struct LevelCtx {
     sray_gpu_ctx *c;
     GpuScene *g;
     RenderDev R;
+    ShadePmaps PM;
     const uint32_t *active;
     uint32_t c0;
     int sample;
@@ -1120,10 +1456,11 @@ int process_level(LevelCtx &L, int b, uint32_t count)
         }
         {
             ProfScope ps(c, "k_shade");
-            k_shade<<<blocks(m, 256), 256, 0, c->stream>>>(S, R, L.active, L.c0, L.sample, q, dptr<int4>(c->hits), m,
-                                                           dptr<float4>(c->direct), dptr<SRay>(c->sq),
-                                                           dptr<QRay>(c->kids), dptr<uint8_t>(c->kflag), L.stats,
-                                                           dptr<uint32_t>(c->errf));
+            auto kern = (R.gi || R.caust) ? k_shade<true> : k_shade<false>;
+            kern<<<blocks(m, 256), 256, 0, c->stream>>>(S, R, L.PM, L.active, L.c0, L.sample, q, dptr<int4>(c->hits), m,
+                                                        dptr<float4>(c->direct), dptr<SRay>(c->sq),
+                                                        dptr<QRay>(c->kids), dptr<uint8_t>(c->kflag), L.stats,
+                                                        dptr<uint32_t>(c->errf));
         }
         size_t ns = (size_t)m * R.L;
         if (ns) {
@@ -1173,7 +1510,22 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
         L = S.nlights * R.shadow_samples;
     }
     R.L = L;
-    R.K = 2;
+    R.K = 2 + (R.gi ? R.diffuse_samples : 0);
+    ShadePmaps PM;
+    memset(&PM, 0, sizeof PM);
+    for (int kind = 0; kind < 2; kind++) {
+        bool want = kind == SRAY_PMAP_CAUSTIC ? R.caust != 0 : R.gi != 0;
+        if (!want) continue;
+        auto it = c->pmaps.find(g->pmap[kind]);
+        if (it == c->pmaps.end())
+            return fail(c, SRAY_ERR_NOTFOUND, "render asks for the %s photon map but none is bound to scene %u",
+                        kind == SRAY_PMAP_CAUSTIC ? "caustic" : "global", scene_id);
+        (kind == SRAY_PMAP_CAUSTIC ? PM.caust : PM.glob) = it->second->tree;
+    }
+    /* gather distance relative to the scene octree diagonal (src/scene.cc:384-387) */
+    double gabs = g->diag * (double)p->gather_dist;
+    PM.radius = (float)gabs;
+    PM.scale = gabs > 0.0 ? 1.0 / (M_PI * (gabs * gabs)) : 0.0;
     uint32_t npix = (uint32_t)R.W * (uint32_t)(p->y1 - p->y0);
     uint32_t levels = (uint32_t)R.iter + 1;
     if (c->qlev.size() < levels) c->qlev.resize(levels);
@@ -1205,7 +1557,7 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
     uint32_t nact = npix;
     int cur = 0;
     uint64_t prim = 0, samples = 0;
-    LevelCtx LC{c, g, R, nullptr, 0, 0, dptr<unsigned long long>(c->stats)};
+    LevelCtx LC{c, g, R, PM, nullptr, 0, 0, dptr<unsigned long long>(c->stats)};
     for (int s = 0; s < R.max_samples && nact > 0; s++) {
         const uint32_t *A = dptr<uint32_t>(c->act[cur]);
         for (uint32_t c0 = 0; c0 < nact; c0 += CHUNK) {
@@ -1269,6 +1621,209 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
     return SRAY_OK;
 }
 
+/* per-level open segments of the balanced tree over n photons (host) */
+void pm_levels(uint32_t n, std::vector<std::vector<uint32_t>> &lo, std::vector<std::vector<uint32_t>> &hi)
+{
+    lo.clear();
+    hi.clear();
+    std::vector<uint32_t> cl = {0}, ch = {n};
+    while (!cl.empty()) {
+        std::vector<uint32_t> l2, h2, nl, nh;
+        for (size_t k = 0; k < cl.size(); k++) {
+            uint32_t a = cl[k], b = ch[k];
+            if (b - a >= 2) {
+                l2.push_back(a);
+                h2.push_back(b);
+            }
+            uint32_t m = (a + b) >> 1;
+            if (m > a + 1) {  /* left child with >= 2 elements still needs sorting */
+                nl.push_back(a);
+                nh.push_back(m);
+            }
+            if (b > m + 2) {
+                nl.push_back(m + 1);
+                nh.push_back(b);
+            }
+        }
+        if (!l2.empty()) {
+            lo.push_back(l2);
+            hi.push_back(h2);
+        }
+        cl.swap(nl);
+        ch.swap(nh);
+    }
+}
+
+int do_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, uint32_t n, int kind, uint32_t *pmap_id)
+{
+    if (kind != SRAY_PMAP_CAUSTIC && kind != SRAY_PMAP_GLOBAL) return fail(c, SRAY_ERR_ARG, "bad photon map kind");
+    if (scene_id && !find_scene(c, scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    if (n && !ph) return fail(c, SRAY_ERR_ARG, "photons is NULL");
+    if (n >= (1u << 30)) return fail(c, SRAY_ERR_LIMIT, "more than 2^30 photons");
+    auto g = std::make_unique<GpuPmap>();
+    g->scene_id = scene_id;
+    g->kind = kind;
+    int rc;
+    auto alloc = [&](size_t bytes, void **p) -> int {
+        DevBuf b;
+        hipError_t e = hipMalloc(&b.p, bytes ? bytes : 16);
+        if (e != hipSuccess) return fail(c, SRAY_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+        b.bytes = bytes;
+        g->bufs.push_back(b);
+        g->bytes += bytes;
+        *p = b.p;
+        return SRAY_OK;
+    };
+    void *pos, *dir, *nrm, *colb, *axis;
+    if ((rc = alloc((size_t)n * 16, &pos)) || (rc = alloc((size_t)n * 16, &dir)) || (rc = alloc((size_t)n * 16, &nrm)) ||
+        (rc = alloc((size_t)n * 4, &colb)) || (rc = alloc((size_t)n, &axis)))
+        return rc;
+    uint32_t depth = 0;
+    if (n) {
+        /* scratch for the build */
+        DevBuf dph, pos0, ord[2], key[2], seg, bb, slo, shi, sax, tmp;
+        auto cleanup = [&]() {
+            DevBuf *bs[] = {&dph, &pos0, &ord[0], &ord[1], &key[0], &key[1], &seg, &bb, &slo, &shi, &sax, &tmp};
+            for (DevBuf *b : bs) release(*b);
+        };
+        std::vector<std::vector<uint32_t>> L, H;
+        pm_levels(n, L, H);
+        depth = (uint32_t)L.size() + 1;
+        size_t maxseg = 1;
+        for (auto &v : L) maxseg = std::max(maxseg, v.size());
+        if ((rc = ensure(c, dph, (size_t)n * sizeof(sray_photon))) || (rc = ensure(c, pos0, (size_t)n * 16)) ||
+            (rc = ensure(c, ord[0], (size_t)n * 4)) || (rc = ensure(c, ord[1], (size_t)n * 4)) ||
+            (rc = ensure(c, key[0], (size_t)n * 8)) || (rc = ensure(c, key[1], (size_t)n * 8)) ||
+            (rc = ensure(c, seg, (size_t)n * 4)) || (rc = ensure(c, bb, maxseg * 24)) ||
+            (rc = ensure(c, slo, maxseg * 4)) || (rc = ensure(c, shi, maxseg * 4)) || (rc = ensure(c, sax, maxseg))) {
+            cleanup();
+            return rc;
+        }
+        hipError_t e = hipMemcpyAsync(dph.p, ph, (size_t)n * sizeof(sray_photon), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) {
+            cleanup();
+            return fail(c, SRAY_ERR_HIP, "photon upload failed");
+        }
+        (void)hipMemsetAsync(axis, 0, n, c->stream);
+        k_pm_init<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<sray_photon>(dph), n, dptr<float4>(pos0),
+                                                          dptr<uint32_t>(ord[0]));
+        int endbit = 32;
+        while ((1ull << (endbit - 32)) <= n) endbit++;
+        int cur = 0;
+        for (size_t l = 0; l < L.size(); l++) {
+            uint32_t ns = (uint32_t)L[l].size();
+            (void)hipMemcpyAsync(slo.p, L[l].data(), ns * 4, hipMemcpyHostToDevice, c->stream);
+            (void)hipMemcpyAsync(shi.p, H[l].data(), ns * 4, hipMemcpyHostToDevice, c->stream);
+            k_pm_seg<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<uint32_t>(slo), dptr<uint32_t>(shi), ns, n,
+                                                             dptr<int32_t>(seg));
+            k_pm_bbinit<<<blocks((size_t)ns * 6, 256), 256, 0, c->stream>>>(dptr<uint32_t>(bb), ns);
+            k_pm_bbox<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<uint32_t>(ord[cur]), dptr<float4>(pos0),
+                                                              dptr<int32_t>(seg), n, dptr<uint32_t>(bb));
+            k_pm_axis<<<blocks(ns, 256), 256, 0, c->stream>>>(dptr<uint32_t>(bb), ns, dptr<uint8_t>(sax));
+            k_pm_keys<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<uint32_t>(ord[cur]), dptr<float4>(pos0),
+                                                              dptr<int32_t>(seg), dptr<uint32_t>(slo),
+                                                              dptr<uint8_t>(sax), n,
+                                                              dptr<unsigned long long>(key[0]));
+            size_t tb = 0;
+            (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, dptr<unsigned long long>(key[0]),
+                                                     dptr<unsigned long long>(key[1]), dptr<uint32_t>(ord[cur]),
+                                                     dptr<uint32_t>(ord[cur ^ 1]), (int)n, 0, endbit, c->stream);
+            if ((rc = ensure(c, tmp, tb))) {
+                cleanup();
+                return rc;
+            }
+            tb = tmp.bytes;
+            e = hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, dptr<unsigned long long>(key[0]),
+                                                   dptr<unsigned long long>(key[1]), dptr<uint32_t>(ord[cur]),
+                                                   dptr<uint32_t>(ord[cur ^ 1]), (int)n, 0, endbit, c->stream);
+            if (e != hipSuccess) {
+                cleanup();
+                return fail(c, SRAY_ERR_HIP, "kd-tree level sort failed: %s", hipGetErrorString(e));
+            }
+            cur ^= 1;
+            k_pm_nodes<<<blocks(ns, 256), 256, 0, c->stream>>>(dptr<uint32_t>(slo), dptr<uint32_t>(shi),
+                                                                dptr<uint8_t>(sax), ns, (uint8_t *)axis);
+        }
+        k_pm_final<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<sray_photon>(dph), dptr<uint32_t>(ord[cur]), n,
+                                                           (float4 *)pos, (float4 *)dir, (float4 *)nrm, (float *)colb);
+        e = hipStreamSynchronize(c->stream);
+        cleanup();
+        if (e != hipSuccess) return fail(c, SRAY_ERR_HIP, "kd-tree build failed: %s", hipGetErrorString(e));
+        if (depth + 1 > (uint32_t)PMAP_STACK) return fail(c, SRAY_ERR_LIMIT, "kd-tree deeper than the gather stack");
+    }
+    PmapTree &T = g->tree;
+    T.pos = (const float4 *)pos;
+    T.dir = (const float4 *)dir;
+    T.nrm = (const float4 *)nrm;
+    T.colb = (const float *)colb;
+    T.axis = (const uint8_t *)axis;
+    T.n = n;
+    T.depth = depth;
+    uint32_t id = c->next_pmap++;
+    c->pmaps[id] = std::move(g);
+    if (scene_id) find_scene(c, scene_id)->pmap[kind] = id; /* the scene's caustic/global map */
+    *pmap_id = id;
+    return SRAY_OK;
+}
+
+int do_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q, uint32_t n, float radius, int k,
+                   int mode, float *rgb)
+{
+    auto it = c->pmaps.find(pmap_id);
+    if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
+    if (mode != SRAY_GATHER_IRRADIANCE && mode != SRAY_GATHER_RADIANCE) return fail(c, SRAY_ERR_ARG, "bad gather mode");
+    if (k < 0) return fail(c, SRAY_ERR_ARG, "k < 0");
+    if (!(radius > 0.0f)) return fail(c, SRAY_ERR_ARG, "radius must be > 0");
+    if (n == 0) return SRAY_OK;
+    const PmapTree &T = it->second->tree;
+    int rc;
+    if ((rc = ensure(c, c->gq, (size_t)n * sizeof(sray_gather_query))) || (rc = ensure(c, c->grgb, (size_t)n * 12)) ||
+        (rc = ensure(c, c->gcnt, (size_t)n * 4)) || (rc = ensure(c, c->gneed, n)) ||
+        (rc = ensure(c, c->giota, (size_t)n * 4)) || (rc = ensure(c, c->gfq, (size_t)n * 4)) ||
+        (rc = ensure(c, c->goff, (size_t)n * 4 + 4)) || (rc = ensure(c, c->nsel, 4)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->gq.p, q, (size_t)n * sizeof(sray_gather_query), hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, "k_pm_gather");
+        k_pm_gather<<<blocks(n, 128), 128, 0, c->stream>>>(T, dptr<sray_gather_query>(c->gq), n, radius, mode, k,
+                                                          dptr<float>(c->grgb), dptr<uint32_t>(c->gcnt),
+                                                          dptr<uint8_t>(c->gneed));
+    }
+    if (k > 0) {
+        k_iota<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<uint32_t>(c->giota), n);
+        uint32_t nf = 0;
+        if ((rc = compact_u32(c, dptr<uint32_t>(c->giota), dptr<uint8_t>(c->gneed), dptr<uint32_t>(c->gfq), n, &nf)))
+            return rc;
+        if (nf) {
+            /* candidate offsets: exclusive scan of the flagged queries' counts */
+            std::vector<uint32_t> fq(nf), cnt(n);
+            HIPCHK(c, hipMemcpy(fq.data(), c->gfq.p, (size_t)nf * 4, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(cnt.data(), c->gcnt.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+            std::vector<uint32_t> off(nf);
+            uint64_t tot = 0;
+            for (uint32_t j = 0; j < nf; j++) {
+                off[j] = (uint32_t)tot;
+                tot += cnt[fq[j]];
+            }
+            if (tot >= (1ull << 32)) return fail(c, SRAY_ERR_LIMIT, "too many k-NN candidates");
+            if ((rc = ensure(c, c->gcand, (size_t)tot * sizeof(Cand)))) return rc;
+            HIPCHK(c, hipMemcpyAsync(c->goff.p, off.data(), (size_t)nf * 4, hipMemcpyHostToDevice, c->stream));
+            k_pm_collect<<<blocks(nf, 128), 128, 0, c->stream>>>(T, dptr<sray_gather_query>(c->gq),
+                                                                dptr<uint32_t>(c->gfq), nf, dptr<uint32_t>(c->goff),
+                                                                radius, dptr<Cand>(c->gcand));
+            k_pm_select<<<blocks(nf, 64), 64, 0, c->stream>>>(T, dptr<sray_gather_query>(c->gq), dptr<uint32_t>(c->gfq),
+                                                              nf, dptr<uint32_t>(c->goff), dptr<uint32_t>(c->gcnt),
+                                                              dptr<Cand>(c->gcand), k, radius, mode,
+                                                              dptr<float>(c->grgb));
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(rgb, c->grgb.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_resolve(c);
+    return SRAY_OK;
+}
+
 int do_trace_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *d_rays, uint32_t n, sray_hit *d_hits,
                     uint32_t flags, unsigned long long *d_counts)
 {
@@ -1322,10 +1877,14 @@ void sray_gpu_destroy(sray_gpu_ctx *c)
         for (auto &kv : c->scenes)
             for (auto &b : kv.second->bufs) release(b);
         c->scenes.clear();
+        for (auto &kv : c->pmaps)
+            for (auto &b : kv.second->bufs) release(b);
+        c->pmaps.clear();
         for (auto &b : c->qlev) release(b);
         DevBuf *bs[] = {&c->hits, &c->direct, &c->sq, &c->occ, &c->kids, &c->kflag, &c->acc, &c->fb, &c->subpix,
                         &c->nsamp, &c->act[0], &c->act[1], &c->keep, &c->stats, &c->errf, &c->cubtmp, &c->nsel,
-                        &c->rays, &c->hitbuf, &c->counts, &c->rcounts};
+                        &c->rays, &c->hitbuf, &c->counts, &c->rcounts, &c->gq, &c->grgb, &c->gcnt,
+                        &c->gneed, &c->gfq, &c->goff, &c->gcand, &c->giota};
         for (DevBuf *b : bs) release(*b);
         prof_resolve(c);
         for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -1461,21 +2020,31 @@ int sray_gpu_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params
 int sray_gpu_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *p, uint32_t n, int kind,
                          uint32_t *pmap_id)
 {
-    (void)scene_id, (void)p, (void)n, (void)kind, (void)pmap_id;
-    return fail(c, SRAY_ERR_ARG, "photon maps are not available in this build");
+    if (!c || !pmap_id) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    return do_pmap_upload(c, scene_id, p, n, kind, pmap_id);
 }
 
 int sray_gpu_pmap_release(sray_gpu_ctx *c, uint32_t pmap_id)
 {
-    (void)pmap_id;
-    return fail(c, SRAY_ERR_NOTFOUND, "photon maps are not available in this build");
+    if (!c) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    auto it = c->pmaps.find(pmap_id);
+    if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &b : it->second->bufs) release(b);
+    c->pmaps.erase(it);
+    return SRAY_OK;
 }
 
 int sray_gpu_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q, uint32_t n, float radius, int k,
                          int mode, float *rgb)
 {
-    (void)pmap_id, (void)q, (void)n, (void)radius, (void)k, (void)mode, (void)rgb;
-    return fail(c, SRAY_ERR_NOTFOUND, "photon maps are not available in this build");
+    if (!c || (n && (!q || !rgb))) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    return do_pmap_gather(c, pmap_id, q, n, radius, k, mode, rgb);
 }
 
 int sray_gpu_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *ticket)
@@ -1488,6 +2057,48 @@ int sray_gpu_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
 {
     (void)ticket, (void)wait, (void)out;
     return fail(c, SRAY_ERR_NOTFOUND, "request serving is not available in this build");
+}
+
+int sray_gpu_malloc(sray_gpu_ctx *c, size_t bytes, void **dp)
+{
+    if (!c || !dp) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    hipError_t e = hipMalloc(dp, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *dp = nullptr;
+        return fail(c, SRAY_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return SRAY_OK;
+}
+
+int sray_gpu_free(sray_gpu_ctx *c, void *dp)
+{
+    if (!c) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (!dp) return SRAY_OK;
+    (void)hipStreamSynchronize(c->stream);
+    HIPCHK(c, hipFree(dp));
+    return SRAY_OK;
+}
+
+int sray_gpu_memcpy(sray_gpu_ctx *c, void *dst, const void *src, size_t bytes, int kind)
+{
+    if (!c || (bytes && (!dst || !src)) || kind < 0 || kind > 2) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SRAY_OK;
+}
+
+int sray_gpu_synchronize(sray_gpu_ctx *c)
+{
+    if (!c) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipDeviceSynchronize());
+    return SRAY_OK;
 }
 
 int sray_gpu_set_profiling(sray_gpu_ctx *c, int on)

@@ -112,49 +112,70 @@ __device__ __forceinline__ void obj_xf_at(const DevObj &o, int time, float M[12]
     M[11] = M[11] + o.motion[2] * s;
 }
 
-/* Sphere::intersect, src/sphere.cc:85-132 (t only) */
-__device__ __forceinline__ bool sphere_t(float rad, V3 o, V3 d, float &t)
+/* Sphere::intersect, src/sphere.cc:85-132 (t only).  The quadratic is
+ * evaluated in double: with |dir| = 1e4 the discriminant cancels about
+ * three digits, which in float would put hit points 1e-5 off the surface
+ * and self-shadow where the reference (double) does not. */
+__device__ __forceinline__ bool sphere_td(double rad, V3 o, V3 d, double &t)
 {
-    float a = dot(d, d);
-    float b = 2.0f * d.x * o.x + 2.0f * d.y * o.y + 2.0f * d.z * o.z;
-    float c = dot(o, o) - rad * rad;
-    float disc = b * b - 4.0f * a * c;
-    if (disc < 0.0f) return false;
-    float sd = sqrtf(disc);
-    float t1 = (-b + sd) / (2.0f * a);
-    float t2 = (-b - sd) / (2.0f * a);
-    if ((t1 < EPS && t2 < EPS) || (t1 > 1.0f && t2 > 1.0f)) return false;
-    if (t1 < EPS) t1 = t2;
-    if (t2 < EPS) t2 = t1;
+    double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
+    double a = dx * dx + dy * dy + dz * dz;
+    double b = 2.0 * dx * ox + 2.0 * dy * oy + 2.0 * dz * oz;
+    double c = ox * ox + oy * oy + oz * oz - rad * rad;
+    double disc = b * b - 4.0 * a * c;
+    if (disc < 0.0) return false;
+    double sd = sqrt(disc);
+    double t1 = (-b + sd) / (2.0 * a);
+    double t2 = (-b - sd) / (2.0 * a);
+    const double e = SRAY_ERROR_MARGIN;
+    if ((t1 < e && t2 < e) || (t1 > 1.0 && t2 > 1.0)) return false;
+    if (t1 < e) t1 = t2;
+    if (t2 < e) t2 = t1;
     t = t1 < t2 ? t1 : t2;
     return true;
 }
-
-/* Cylinder::intersect, src/cylinder.cc:125-185 (t only) */
-__device__ __forceinline__ bool cylinder_t(float rad, float ys, float ye, V3 o, V3 d, float &t)
+__device__ __forceinline__ bool sphere_t(float rad, V3 o, V3 d, float &t)
 {
-    float a = d.x * d.x + d.z * d.z;
-    float b = 2.0f * d.x * o.x + 2.0f * d.z * o.z;
-    float c = o.x * o.x + o.z * o.z - rad * rad;
-    float disc = b * b - 4.0f * a * c;
-    if (disc < 0.0f) return false;
-    float sd = sqrtf(disc);
-    float t1 = (-b + sd) / (2.0f * a);
-    float t2 = (-b - sd) / (2.0f * a);
-    if ((t1 < EPS && t2 < EPS) || (t1 > 1.0f && t2 > 1.0f)) return false;
-    float y1 = o.y + d.y * t1;
-    float y2 = o.y + d.y * t2;
+    double td;
+    if (!sphere_td((double)rad, o, d, td)) return false;
+    t = (float)td;
+    return true;
+}
+
+/* Cylinder::intersect, src/cylinder.cc:125-185 (t only; double as above) */
+__device__ __forceinline__ bool cylinder_td(double rad, double ys, double ye, V3 o, V3 d, double &t)
+{
+    double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
+    double a = dx * dx + dz * dz;
+    double b = 2.0 * dx * ox + 2.0 * dz * oz;
+    double c = ox * ox + oz * oz - rad * rad;
+    double disc = b * b - 4.0 * a * c;
+    if (disc < 0.0) return false;
+    double sd = sqrt(disc);
+    double t1 = (-b + sd) / (2.0 * a);
+    double t2 = (-b - sd) / (2.0 * a);
+    const double e = SRAY_ERROR_MARGIN;
+    if ((t1 < e && t2 < e) || (t1 > 1.0 && t2 > 1.0)) return false;
+    double y1 = oy + dy * t1;
+    double y2 = oy + dy * t2;
     bool v0 = true, v1 = true;
-    if (t1 < EPS || y1 < ys || y1 >= ye) {
+    if (t1 < e || y1 < ys || y1 >= ye) {
         v0 = false;
         t1 = t2;
     }
-    if (t2 < EPS || y2 < ys || y2 >= ye) {
+    if (t2 < e || y2 < ys || y2 >= ye) {
         v1 = false;
         t2 = t1;
     }
     if (!v0 && !v1) return false;
     t = t1 < t2 ? t1 : t2;
+    return true;
+}
+__device__ __forceinline__ bool cylinder_t(float rad, float ys, float ye, V3 o, V3 d, float &t)
+{
+    double td;
+    if (!cylinder_td((double)rad, (double)ys, (double)ye, o, d, td)) return false;
+    t = (float)td;
     return true;
 }
 

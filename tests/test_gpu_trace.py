@@ -54,21 +54,21 @@ def test_primary_spheres(ctx):
 def test_device_raygen_matches_checker(ctx):
     """raygen kernel == Camera::get_primary_ray restatement, bit for bit
     (sample 0 and a jittered sample)"""
-    import torch
     hs = _scene(0)
     sid = ctx.upload(hs)
     ref = oracle.OracleScene(hs.desc_ptr(), 32)
     p = S.render_params(320, 240, rect=(10, 20, 300, 200), seed=1234)
     n = (p.x1 - p.x0) * (p.y1 - p.y0)
+    d = ctx.malloc(n * S.RAY_DTYPE.itemsize)
     for sample in (0, 3):
-        d = torch.empty(n * 8, dtype=torch.float32, device="cuda")
-        ctx.primary_rays_device(sid, p, sample, d.data_ptr())
-        torch.cuda.synchronize()
-        got = d.cpu().numpy().view(S.RAY_DTYPE)
+        ctx.primary_rays_device(sid, p, sample, d)
+        ctx.synchronize()
+        got = ctx.to_host(d, n * S.RAY_DTYPE.itemsize, S.RAY_DTYPE)
         want = ref.primary_rays(p, sample)
         assert np.array_equal(got["origin"].view(np.uint32), want["origin"].view(np.uint32))
         assert np.array_equal(got["dir"].view(np.uint32), want["dir"].view(np.uint32)), f"sample {sample}"
         assert np.array_equal(got["time"], want["time"])
+    ctx.free(d)
     ctx.release(sid)
 
 
