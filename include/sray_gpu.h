@@ -231,6 +231,32 @@ typedef struct sray_gather_query {
     float pad_;
 } sray_gather_query;
 
+/* photon shooting (shoot_photons, src/render.cc:67-122) */
+typedef struct sray_photon_params {
+    int32_t  caust_photons;   /* -c: caustic photons shot */
+    int32_t  gi_photons;      /* -g: global photons shot */
+    float    photon_energy;   /* -photonenergy (default 300) */
+    int32_t  iter;            /* -i: bounce limit (ray.iter) */
+    int32_t  t0, t1;          /* frame interval (msec) */
+    uint64_t seed;            /* counter-based RNG seed */
+} sray_photon_params;
+
+typedef struct sray_photon_stats {
+    uint64_t shot[2];         /* caustic, global photons shot */
+    uint64_t stored[2];       /* caustic, global photons stored */
+    uint64_t segments;        /* photon ray segments traced */
+    double   trace_ms;        /* device time of photon tracing */
+    double   build_ms;        /* device time of the kd-tree builds */
+} sray_photon_stats;
+
+/* The 64x32 projection-map cell of a direction, without transcendental
+ * functions: the polar index counts cos(k*pi/32) boundaries below y/|d|,
+ * the azimuth index counts cos(2*pi*k/64) boundaries in the half plane of
+ * z (ProjMap::cell_from_dir, src/projmap.cc:146-163, uses atan2/acos; the
+ * result differs only exactly on a cell border). */
+#define SRAY_PROJ_U 64
+#define SRAY_PROJ_V 32
+
 /* ------------------------------------------------------------------ */
 /* batching / serving requests                                         */
 /* ------------------------------------------------------------------ */
@@ -298,6 +324,18 @@ int  sray_gpu_pmap_release(sray_gpu_ctx *ctx, uint32_t pmap_id);
  * k > 0 : the k nearest within radius, ties broken by photon index. */
 int  sray_gpu_pmap_gather(sray_gpu_ctx *ctx, uint32_t pmap_id, const sray_gather_query *q,
                           uint32_t n, float radius, int k, int mode, float *rgb);
+
+/* Replaces shoot_photons + Scene::build_caustics_map/build_global_map +
+ * Light::gen_photon/build_projmap (src/render.cc:67-122, src/scene.cc:445-760,
+ * src/light.cc:111-205, src/projmap.cc): projection maps on the host,
+ * photon paths traced on the device, both maps built as device kd-trees and
+ * bound to the scene.  *caustic / *global receive the photon map ids (0 if
+ * that map was not requested). */
+int  sray_gpu_shoot_photons(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_photon_params *pp,
+                            uint32_t *caustic, uint32_t *global, sray_photon_stats *st);
+/* photon array of a map in storage order (the order add_photon saw them) */
+int  sray_gpu_pmap_download(sray_gpu_ctx *ctx, uint32_t pmap_id, sray_photon *out, uint32_t cap,
+                            uint32_t *n);
 
 /* Batching / serving (no reference counterpart: the reference renders one
  * frame of one scene at a time, src/sray.cc:56-65).  submit admits a tile

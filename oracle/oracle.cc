@@ -24,6 +24,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cfloat>
 #include <climits>
@@ -1207,6 +1208,465 @@ template <class T> struct Scene {
 };
 
 /* ------------------------------------------------------------------ */
+/* photon shooting, src/render.cc:67-122, src/scene.cc:445-760,          */
+/* src/light.cc:111-205, 392-432, src/projmap.cc                         */
+/* ------------------------------------------------------------------ */
+enum : uint32_t { P_TIME = 0, P_LPOINT = 1, P_EMIT = 2, P_RR = 3, P_DIFF = 4, P_PROJ = 5 };
+constexpr uint32_t PHOTON_ATTEMPTS = 1u << 17;
+constexpr int PHOTON_MAX_BOUNCES = 1024;
+
+/* photon draws: a = emission index, b = kind<<24 | light, c = bounce */
+static inline float prand(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t purpose, uint32_t att,
+                          uint32_t dim)
+{
+    return rng_unit(seed, a, b, c, slot(K_PHOTON, purpose, att >> 7, att & 127u, dim));
+}
+
+/* one Marsaglia attempt; false if rejected */
+static inline bool marsaglia(float u1, float u2, float out[3])
+{
+    float x1 = 2.0f * u1 - 1.0f, x2 = 2.0f * u2 - 1.0f;
+    float s = x1 * x1 + x2 * x2;
+    if (!(s < 1.0f && s > 0.0f)) return false;
+    float q = 2.0f * std::sqrt(1.0f - s);
+    out[0] = x1 * q;
+    out[1] = x2 * q;
+    out[2] = 1.0f - 2.0f * s;
+    return true;
+}
+
+/* ProjMap::cell_from_dir without atan2/acos: boundary cosines in float */
+struct CellTables {
+    float cv[SRAY_PROJ_V + 1];  /* cos(k*pi/32) */
+    float cu[SRAY_PROJ_U + 1];  /* cos(2*pi*k/64) */
+    CellTables()
+    {
+        for (int k = 0; k <= SRAY_PROJ_V; k++) cv[k] = (float)std::cos((double)k * M_PI / SRAY_PROJ_V);
+        for (int k = 0; k <= SRAY_PROJ_U; k++) cu[k] = (float)std::cos(2.0 * M_PI * (double)k / SRAY_PROJ_U);
+    }
+};
+static const CellTables kCell;
+
+static inline void cell_from_dir(float x, float y, float z, int &uc, int &vc)
+{
+    float r = std::sqrt(x * x + y * y + z * z);
+    float cy = y / r;
+    int v = 0;
+    for (int k = 1; k < SRAY_PROJ_V; k++) v += (cy <= kCell.cv[k]) ? 1 : 0;
+    float rho = std::sqrt(x * x + z * z);
+    int u;
+    if (!(rho > 0.0f)) {
+        u = x < 0.0f ? SRAY_PROJ_U / 2 : 0;
+    } else {
+        float cx = x / rho;
+        if (!(z < 0.0f)) {
+            u = 0;
+            for (int k = 1; k <= SRAY_PROJ_U / 2; k++) u += (cx <= kCell.cu[k]) ? 1 : 0;
+        } else {
+            u = SRAY_PROJ_U / 2;
+            for (int k = SRAY_PROJ_U / 2 + 1; k < SRAY_PROJ_U; k++) u += (cx >= kCell.cu[k]) ? 1 : 0;
+        }
+    }
+    uc = std::min(u, SRAY_PROJ_U - 1);
+    vc = std::min(v, SRAY_PROJ_V - 1);
+}
+
+/* 64x32 map of directions from a light towards objects */
+struct ProjMapT {
+    std::vector<uint8_t> cells;
+    std::vector<double> area;
+    double total = 0, coverage = 0;
+    ProjMapT() : cells(SRAY_PROJ_U * SRAY_PROJ_V, 0), area(SRAY_PROJ_U * SRAY_PROJ_V)
+    {
+        /* ProjMap::set_subdiv, src/projmap.cc:44-74 */
+        for (int i = 0; i < SRAY_PROJ_U * SRAY_PROJ_V; i++) {
+            int uc = i % SRAY_PROJ_U, vc = i / SRAY_PROJ_U;
+            double v[4][3];
+            for (int k = 0; k < 4; k++) corner(uc, vc, k, v[k]);
+            auto tri = [&](const double *a, const double *b, const double *c) {
+                double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+                double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                       cz = e1[0] * e2[1] - e1[1] * e2[0];
+                return std::sqrt(cx * cx + cy * cy + cz * cz) * 0.5;
+            };
+            area[i] = tri(v[0], v[1], v[2]) + tri(v[0], v[2], v[3]);
+            total += area[i];
+        }
+    }
+    /* ProjMap::cell_corner_dir, src/projmap.cc:129-144 */
+    static void corner(int uc, int vc, int k, double out[3])
+    {
+        static const int offs[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+        double u = (double)(uc + offs[k][0]) * (1.0 / SRAY_PROJ_U);
+        double v = (double)(vc + offs[k][1]) * (1.0 / SRAY_PROJ_V);
+        double th = u * M_PI * 2.0, ph = v * M_PI;
+        out[0] = std::sin(ph) * std::cos(th);
+        out[1] = std::cos(ph);
+        out[2] = std::sin(ph) * std::sin(th);
+    }
+    void set(int i, bool val)
+    {
+        if ((cells[i] != 0) != val) coverage += val ? area[i] / total : -area[i] / total;
+        cells[i] = val ? 1 : 0;
+    }
+    void set_all() { std::fill(cells.begin(), cells.end(), 1), coverage = 1.0; }
+    double get_coverage() const { return std::min(std::max(coverage, 0.0), 1.0); }
+};
+
+/* ::build_projmap, src/light.cc:392-432 */
+static void build_projmap(ProjMapT &map, const double lpos[3], const std::vector<std::array<double, 6>> &boxes)
+{
+    for (const auto &b : boxes) {
+        if (lpos[0] >= b[0] && lpos[1] >= b[1] && lpos[2] >= b[2] && lpos[0] < b[3] && lpos[1] < b[4] && lpos[2] < b[5]) {
+            map.set_all();
+            return;
+        }
+        double c[3] = {(b[0] + b[3]) / 2.0, (b[1] + b[4]) / 2.0, (b[2] + b[5]) / 2.0};
+        double r2 = (b[0] - c[0]) * (b[0] - c[0]) + (b[1] - c[1]) * (b[1] - c[1]) + (b[2] - c[2]) * (b[2] - c[2]);
+        double v[3] = {c[0] - lpos[0], c[1] - lpos[1], c[2] - lpos[2]};
+        double dist = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        int uc, vc;
+        cell_from_dir((float)v[0], (float)v[1], (float)v[2], uc, vc);
+        map.set(vc * SRAY_PROJ_U + uc, true);
+        for (int j = 0; j < SRAY_PROJ_U * SRAY_PROJ_V; j++) {
+            int u = j % SRAY_PROJ_U, w = j / SRAY_PROJ_U;
+            for (int k = 0; k < 4; k++) {
+                double d[3];
+                ProjMapT::corner(u, w, k, d);
+                double p[3] = {lpos[0] + d[0] * dist - c[0], lpos[1] + d[1] * dist - c[1], lpos[2] + d[2] * dist - c[2]};
+                if (p[0] * p[0] + p[1] * p[1] + p[2] * p[2] <= r2) {
+                    map.set(j, true);
+                    break;
+                }
+            }
+        }
+    }
+}
+
+template <class T> struct PhotonTracer {
+    Scene<T> &S;
+    const sray_photon_params *pp;
+    uint64_t segments = 0;
+    std::vector<ProjMapT> proj, spec;
+
+    PhotonTracer(Scene<T> &s, const sray_photon_params *p) : S(s), pp(p) {}
+
+    /* light point: point = position, sphere = pos + sphrand(r), box = box
+     * point at time 0 (src/light.cc:106-109, 272-275, 346-354) */
+    bool light_point(const Light<T> &lt, uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t purpose,
+                     double out[3]) const
+    {
+        if (lt.type == SRAY_LIGHT_BOX) {
+            T f[3];
+            for (int k = 0; k < 3; k++) f[k] = (T)prand(seed, a, b, c, purpose, 0, (uint32_t)k);
+            V3<T> v = v3<T>(f[0] * lt.dim.x - (T)0.5 * lt.dim.x, f[1] * lt.dim.y - (T)0.5 * lt.dim.y,
+                            f[2] * lt.dim.z - (T)0.5 * lt.dim.z);
+            V3<T> w = xf_point(lt.xf, v);
+            out[0] = (double)w.x, out[1] = (double)w.y, out[2] = (double)w.z;
+            return true;
+        }
+        if (lt.type == SRAY_LIGHT_SPHERE) {
+            for (uint32_t at = 0; at < 128; at++) {
+                float d[3];
+                if (marsaglia(prand(seed, a, b, c, purpose, at, 0), prand(seed, a, b, c, purpose, at, 1), d)) {
+                    V3<T> w = lt.pos + v3<T>((T)d[0], (T)d[1], (T)d[2]) * lt.radius;
+                    out[0] = (double)w.x, out[1] = (double)w.y, out[2] = (double)w.z;
+                    return true;
+                }
+            }
+        }
+        out[0] = (double)lt.pos.x, out[1] = (double)lt.pos.y, out[2] = (double)lt.pos.z;
+        return true;
+    }
+
+    /* Light::build_proj_intern, src/light.cc:151-205 */
+    void build_maps(int t0, int t1)
+    {
+        proj.assign(S.lights.size(), ProjMapT());
+        spec.assign(S.lights.size(), ProjMapT());
+        int tm[2] = {t0, t1};
+        int ntm = t0 == t1 ? 1 : 2;
+        for (size_t li = 0; li < S.lights.size(); li++) {
+            const Light<T> &lt = S.lights[li];
+            int samples = lt.type == SRAY_LIGHT_SPHERE ? 8 : 1;
+            for (int ti = 0; ti < ntm; ti++) {
+                std::vector<std::array<double, 6>> all, caust;
+                for (size_t i = 0; i < S.objs.size(); i++) {
+                    Box<T> b = S.bounds(S.objs[i], tm[ti]);
+                    std::array<double, 6> bb = {(double)b.mn.x, (double)b.mn.y, (double)b.mn.z,
+                                                (double)b.mx.x, (double)b.mx.y, (double)b.mx.z};
+                    all.push_back(bb);
+                    int m = S.objs[i].mat;
+                    bool specobj = m >= 0 && (attr_reflect(S.mats[m]) > 0 || attr_refract(S.mats[m]) > 0);
+                    if (specobj) caust.push_back(bb);
+                }
+                for (int sm = 0; sm < samples; sm++) {
+                    double lp[3];
+                    light_point(lt, pp->seed, 0xffffffffu, (uint32_t)li, (uint32_t)(ti << 8 | sm), P_PROJ, lp);
+                    build_projmap(proj[li], lp, all);
+                    build_projmap(spec[li], lp, caust);
+                }
+            }
+        }
+    }
+
+    struct Ph {
+        int type; /* 0 direct, 1 caustic, 2 global */
+        V3<T> pos, dir, norm;
+        T col[3];
+    };
+
+    /* material-driven interaction shared by both tracers */
+    struct Hit {
+        SurfPoint<T> sp;
+        const Material<T> *mat;
+        V3<T> normal;
+        bool entering;
+        T ray_mag;
+    };
+    bool cast_hit(const Ray<T> &r, Hit &h)
+    {
+        Counters cnt;
+        Best<T> b = S.cast(r, cnt, false);
+        segments++;
+        if (!b.found()) return false;
+        h.sp = S.surf(r, (int)b.item, b.sub, b.t);
+        int m = S.objs[b.item].mat;
+        static Material<T> defm = [] {
+            Material<T> d;
+            d.kd = c4<T>(1, 1, 1, 1);
+            d.ks = c4<T>(0, 0, 0, 1);
+            d.shininess = 1;
+            d.reflect = 0;
+            d.refract = 0;
+            d.ior = (T)1.5;
+            d.flags = SRAY_MAT_HAS_SPECULAR | SRAY_MAT_HAS_REFLECT | SRAY_MAT_HAS_REFRACT;
+            return d;
+        }();
+        h.mat = m >= 0 ? &S.mats[m] : &defm;
+        h.ray_mag = length(r.d);
+        if (dot(r.d, h.sp.normal) > (T)0) {
+            h.normal = -h.sp.normal;
+            h.entering = false;
+        } else {
+            h.normal = h.sp.normal;
+            h.entering = true;
+        }
+        return true;
+    }
+
+    /* Material::get_color / get_value return the default attribute
+     * (1,1,1,1) for a missing attribute (src/material.cc:77,164-168) */
+    static C4<T> attr_specular(const Material<T> &m)
+    {
+        return (m.flags & SRAY_MAT_HAS_SPECULAR) ? m.ks : c4<T>(1, 1, 1, 1);
+    }
+    static T attr_reflect(const Material<T> &m) { return (m.flags & SRAY_MAT_HAS_REFLECT) ? m.reflect : (T)1; }
+    static T attr_refract(const Material<T> &m) { return (m.flags & SRAY_MAT_HAS_REFRACT) ? m.refract : (T)1; }
+
+    /* reflect_ray / refract_ray (libvmath, fixed here as in shade_phong) */
+    static Ray<T> reflect_ray(const Ray<T> &r, V3<T> n)
+    {
+        Ray<T> o = r;
+        o.d = reflect(r.d, n);
+        return o;
+    }
+    static Ray<T> refract_ray(const Ray<T> &r, V3<T> n, T mat_ior, bool entering, T ray_mag)
+    {
+        T ior = r.calc_ior(entering, mat_ior);
+        Ray<T> o = r;
+        if (entering) o.enter(mat_ior);
+        else o.leave();
+        o.d = refract(r.d / ray_mag, n, ior) * ray_mag;
+        if (dot(o.d, n) > (T)0) {
+            if (entering) o.leave();
+            else o.enter(mat_ior);
+        }
+        return o;
+    }
+
+    /* Scene::trace_caustics_photon, src/scene.cc:571-647 (iterative) */
+    bool trace_caustics(Ray<T> r, Ph &ph, uint32_t a, uint32_t b, uint32_t &bounce)
+    {
+        for (int guard = 0; guard < PHOTON_MAX_BOUNCES; guard++) {
+            if (r.iter <= 0) return false;
+            Hit h;
+            if (!cast_hit(r, h)) return false;
+            const Material<T> &m = *h.mat;
+            T refl = attr_reflect(m), refr = attr_refract(m), mat_ior = m.ior;
+            C4<T> spc = attr_specular(m);
+            T ior = r.calc_ior(h.entering, mat_ior);
+            T ray_dot_n = dot(-r.d / h.ray_mag, h.normal);
+            T sf0 = refr > (T)0 ? (ior - (T)1) / (ior + (T)1) : (T)1;
+            T fres = Scene<T>::fresnel(sf0 * sf0, ray_dot_n);
+            refl = fres * refl;
+            refr = ((T)1 - fres) * refr;
+            T rnum = (T)prand(pp->seed, a, b, bounce, P_RR, 0, 0);
+            bounce++;
+            if (rnum < refl) {
+                Ray<T> nr = reflect_ray(r, h.normal);
+                nr.iter--;
+                nr.o = h.sp.pos;
+                ph.type = 1;
+                for (int k = 0; k < 3; k++) ph.col[k] *= (&spc.x)[k];
+                r = nr;
+            } else if (rnum < refr) {
+                Ray<T> nr = refract_ray(r, h.normal, mat_ior, h.entering, h.ray_mag);
+                nr.o = h.sp.pos;
+                ph.type = 1;
+                for (int k = 0; k < 3; k++) ph.col[k] *= (&spc.x)[k];
+                r = nr;
+            } else {
+                ph.dir = normalized(r.d);
+                ph.pos = h.sp.pos;
+                ph.norm = h.normal;
+                return true;
+            }
+        }
+        return false;
+    }
+
+    /* Scene::trace_global_photon, src/scene.cc:650-760 (iterative) */
+    bool trace_global(Ray<T> r, Ph &ph, uint32_t a, uint32_t b, uint32_t &bounce)
+    {
+        for (int guard = 0; guard < PHOTON_MAX_BOUNCES; guard++) {
+            if (r.iter <= 0) return false;
+            V3<T> incident = normalized(r.d);
+            Hit h;
+            if (!cast_hit(r, h)) return false;
+            const Material<T> &m = *h.mat;
+            C4<T> diff = m.kd, spc = attr_specular(m);
+            T diff_avg = (diff.x + diff.y + diff.z) / (T)3;
+            T spec_avg = (spc.x + spc.y + spc.z) / (T)3;
+            T rnum = (T)prand(pp->seed, a, b, bounce, P_RR, 0, 0);
+            uint32_t here = bounce++;
+            if (rnum < diff_avg) {
+                /* diffuse reflection: uniform direction flipped into the hemisphere */
+                float d[3] = {0, 0, 1};
+                for (uint32_t at = 0; at < 128; at++)
+                    if (marsaglia(prand(pp->seed, a, b, here, P_DIFF, at, 0), prand(pp->seed, a, b, here, P_DIFF, at, 1), d))
+                        break;
+                V3<T> dir = v3<T>((T)d[0], (T)d[1], (T)d[2]);
+                if (dot(dir, h.normal) < (T)0) dir = -dir;
+                T ndotl = dot(-incident, h.normal);
+                Ray<T> nr = r;
+                nr.d = dir * (T)SRAY_RAY_MAGNITUDE;
+                nr.iter--;
+                nr.o = h.sp.pos;
+                ph.type = 2;
+                ph.col[0] *= ndotl * (diff.x / diff_avg);
+                ph.col[1] *= ndotl * (diff.y / diff_avg);
+                ph.col[2] *= ndotl * (diff.z / diff_avg);
+                r = nr;
+            } else if (rnum < diff_avg + spec_avg) {
+                T refl = attr_reflect(m), refr = attr_refract(m), mat_ior = m.ior;
+                T ior = r.calc_ior(h.entering, mat_ior);
+                T ray_dot_n = dot(-r.d / h.ray_mag, h.normal);
+                T sf0 = refr > (T)0 ? (ior - (T)1) / (ior + (T)1) : (T)1;
+                T fres = Scene<T>::fresnel(sf0 * sf0, ray_dot_n);
+                refl = fres * refl;
+                refr = ((T)1 - fres) * refr;
+                T rn2 = (T)prand(pp->seed, a, b, here, P_RR, 0, 1);
+                if (rn2 < refl) {
+                    Ray<T> nr = reflect_ray(r, h.normal);
+                    nr.iter--;
+                    nr.o = h.sp.pos;
+                    ph.type = 1;
+                    for (int k = 0; k < 3; k++) ph.col[k] *= (&spc.x)[k];
+                    return trace_caustics(nr, ph, a, b, bounce);
+                } else if (rn2 < refr) {
+                    Ray<T> nr = refract_ray(r, h.normal, mat_ior, h.entering, h.ray_mag);
+                    nr.o = h.sp.pos;
+                    ph.type = 1;
+                    for (int k = 0; k < 3; k++) ph.col[k] *= (&spc.x)[k];
+                    return trace_caustics(nr, ph, a, b, bounce);
+                }
+                return false; /* specular BRDF lobe: not implemented in the reference either */
+            } else {
+                ph.dir = incident;
+                ph.pos = h.sp.pos;
+                ph.norm = h.normal;
+                return true;
+            }
+        }
+        return false;
+    }
+
+    /* Light::gen_photon, src/light.cc:111-148, with the direction drawn by
+     * rejection towards full projection-map cells (src/projmap.cc:194-208) */
+    Ph gen_photon(size_t li, int kind, uint32_t a, uint32_t b, int msec, Ray<T> &r)
+    {
+        const Light<T> &lt = S.lights[li];
+        (void)msec;
+        Ph ph;
+        ph.type = 0;
+        double lp[3];
+        light_point(lt, pp->seed, a, b, 0, P_LPOINT, lp);
+        const ProjMapT &pm = kind == 0 ? spec[li] : proj[li];
+        double cov = pm.get_coverage();
+        bool use = cov != 0.0;
+        double e = (double)pp->photon_energy * (use ? cov : 1.0);
+        for (int k = 0; k < 3; k++) ph.col[k] = (T)((double)(&lt.color.x)[k] * e);
+        float d[3] = {0, 0, 1};
+        for (uint32_t at = 0; at < PHOTON_ATTEMPTS; at++) {
+            if (!marsaglia(prand(pp->seed, a, b, 0, P_EMIT, at, 0), prand(pp->seed, a, b, 0, P_EMIT, at, 1), d)) continue;
+            if (!use) break;
+            int uc, vc;
+            cell_from_dir(d[0], d[1], d[2], uc, vc);
+            if (pm.cells[vc * SRAY_PROJ_U + uc]) break;
+        }
+        r.o = v3<T>((T)lp[0], (T)lp[1], (T)lp[2]);
+        r.d = v3<T>((T)d[0], (T)d[1], (T)d[2]) * (T)SRAY_RAY_MAGNITUDE;
+        return ph;
+    }
+
+    /* Scene::build_caustics_map (kind 0) / build_global_map (kind 1) */
+    void shoot(int kind, int num, std::vector<sray_photon> &out)
+    {
+        out.clear();
+        if (num <= 0 || S.lights.empty()) return;
+        double acc = 0;
+        std::vector<double> inten(S.lights.size());
+        for (size_t i = 0; i < S.lights.size(); i++) {
+            const Light<T> &l = S.lights[i];
+            inten[i] = ((double)l.color.x + (double)l.color.y + (double)l.color.z) / 3.0;
+            acc += inten[i];
+        }
+        int t0 = pp->t0, t1 = pp->t1;
+        for (size_t li = 0; li < S.lights.size(); li++) {
+            int nphot = (int)std::ceil((double)num * (inten[li] / acc));
+            size_t first = out.size();
+            uint32_t b = ((uint32_t)kind << 24) | (uint32_t)li;
+            for (int j = 0; j < nphot; j++) {
+                uint32_t a = (uint32_t)j;
+                int msec = t0 == t1 ? t0 : (int)(rng_u32(pp->seed, a, b, 0, slot(K_PHOTON, P_TIME, 0, 0, 0)) %
+                                                 (uint32_t)(t1 - t0)) + t0;
+                Ray<T> r;
+                Ph ph = gen_photon(li, kind, a, b, msec, r);
+                r.time = msec;
+                r.iter = pp->iter;
+                uint32_t bounce = 1;
+                bool ok = kind == 0 ? (trace_caustics(r, ph, a, b, bounce) && ph.type == 1)
+                                    : trace_global(r, ph, a, b, bounce);
+                if (!ok) continue;
+                sray_photon sp;
+                sp.pos[0] = (float)ph.pos.x, sp.pos[1] = (float)ph.pos.y, sp.pos[2] = (float)ph.pos.z;
+                sp.dir[0] = (float)ph.dir.x, sp.dir[1] = (float)ph.dir.y, sp.dir[2] = (float)ph.dir.z;
+                sp.norm[0] = (float)ph.norm.x, sp.norm[1] = (float)ph.norm.y, sp.norm[2] = (float)ph.norm.z;
+                for (int k = 0; k < 3; k++) sp.col[k] = (float)ph.col[k];
+                out.push_back(sp);
+            }
+            /* PhotonMap::scale_photon_power(1/nphot), src/pmap.cc:90-100 */
+            if (nphot) {
+                double sc = 1.0 / (double)nphot;
+                for (size_t k = first; k < out.size(); k++)
+                    for (int c = 0; c < 3; c++) out[k].col[c] = (float)((double)out[k].col[c] * sc);
+            }
+        }
+    }
+};
+
+/* ------------------------------------------------------------------ */
 /* scene construction from the flat description                         */
 /* ------------------------------------------------------------------ */
 template <class T> static void load(Scene<T> &S, const sray_scene_desc *d, uint32_t flags)
@@ -1591,6 +2051,43 @@ template <class T> static int do_gather(Scene<T> &S, int kind, const sray_gather
     return 0;
 }
 
+template <class T>
+static int do_shoot(Scene<T> &S, const sray_photon_params *pp, sray_photon *caust, uint32_t cap_c, uint32_t *n_c,
+                    sray_photon *glob, uint32_t cap_g, uint32_t *n_g, uint64_t *segments)
+{
+    PhotonTracer<T> tr(S, pp);
+    tr.build_maps(pp->t0, pp->t1);
+    std::vector<sray_photon> c, g;
+    tr.shoot(0, pp->caust_photons, c);
+    tr.shoot(1, pp->gi_photons, g);
+    if (n_c) *n_c = (uint32_t)c.size();
+    if (n_g) *n_g = (uint32_t)g.size();
+    if (caust && c.size() <= cap_c) std::copy(c.begin(), c.end(), caust);
+    if (glob && g.size() <= cap_g) std::copy(g.begin(), g.end(), glob);
+    if (segments) *segments = tr.segments;
+    /* the maps also feed this scene's renders */
+    S.pm[0].resize(c.size());
+    S.pm[1].resize(g.size());
+    for (size_t i = 0; i < c.size(); i++) memcpy(&S.pm[0][i], &c[i], sizeof(Photon));
+    for (size_t i = 0; i < g.size(); i++) memcpy(&S.pm[1][i], &g[i], sizeof(Photon));
+    return 0;
+}
+
+template <class T>
+static int do_projmap(Scene<T> &S, int light, int kind, int t0, int t1, uint64_t seed, uint8_t *cells, double *cov)
+{
+    if (light < 0 || (size_t)light >= S.lights.size()) return -1;
+    sray_photon_params pp;
+    memset(&pp, 0, sizeof pp);
+    pp.seed = seed;
+    PhotonTracer<T> tr(S, &pp);
+    tr.build_maps(t0, t1);
+    const ProjMapT &m = kind == 0 ? tr.proj[light] : tr.spec[light];
+    if (cells) std::copy(m.cells.begin(), m.cells.end(), cells);
+    if (cov) *cov = m.get_coverage();
+    return 0;
+}
+
 } // namespace orc
 
 using namespace orc;
@@ -1666,6 +2163,31 @@ int oracle_gather(oracle_scene *s, int kind, const sray_gather_query *q, uint32_
     if (!s || kind < 0 || kind > 1) return -1;
     if (s->s.prec == 32) return do_gather(*s->s.f, kind, q, n, radius, k, mode, rgb);
     return do_gather(*s->s.d, kind, q, n, radius, k, mode, rgb);
+}
+
+int oracle_shoot_photons(oracle_scene *s, const sray_photon_params *pp, sray_photon *caust, uint32_t cap_c,
+                         uint32_t *n_c, sray_photon *glob, uint32_t cap_g, uint32_t *n_g, uint64_t *segments)
+{
+    if (!s || !pp) return -1;
+    if (s->s.prec == 32) return do_shoot(*s->s.f, pp, caust, cap_c, n_c, glob, cap_g, n_g, segments);
+    return do_shoot(*s->s.d, pp, caust, cap_c, n_c, glob, cap_g, n_g, segments);
+}
+
+int oracle_projmap(oracle_scene *s, int light, int kind, int t0, int t1, uint64_t seed, uint8_t *cells,
+                   double *coverage)
+{
+    if (!s) return -1;
+    if (s->s.prec == 32) return do_projmap(*s->s.f, light, kind, t0, t1, seed, cells, coverage);
+    return do_projmap(*s->s.d, light, kind, t0, t1, seed, cells, coverage);
+}
+
+int oracle_cell_from_dir(float x, float y, float z, int *uc, int *vc)
+{
+    int u, v;
+    cell_from_dir(x, y, z, u, v);
+    if (uc) *uc = u;
+    if (vc) *vc = v;
+    return 0;
 }
 
 double oracle_scene_diag(oracle_scene *s)

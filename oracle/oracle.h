@@ -76,6 +76,18 @@ int oracle_set_photons(oracle_scene *s, int kind, const sray_photon *p, uint32_t
 int oracle_gather(oracle_scene *s, int kind, const sray_gather_query *q, uint32_t n,
                   float radius, int k, int mode, float *rgb);
 
+/* shoot_photons (src/render.cc:67-122): projection maps, emission and
+ * tracing of caustic and global photons; stored photons in storage order.
+ * caust/glob: capacity cap_c / cap_g photons; counts returned in n_c / n_g. */
+int oracle_shoot_photons(oracle_scene *s, const sray_photon_params *pp, sray_photon *caust, uint32_t cap_c,
+                         uint32_t *n_c, sray_photon *glob, uint32_t cap_g, uint32_t *n_g, uint64_t *segments);
+/* projection maps of a light (kind 0 = all objects, 1 = specular objects):
+ * 64x32 cells as bytes (v-major), and the coverage */
+int oracle_projmap(oracle_scene *s, int light, int kind, int t0, int t1, uint64_t seed, uint8_t *cells,
+                   double *coverage);
+/* ProjMap cell of a direction (the transcendental-free definition) */
+int oracle_cell_from_dir(float x, float y, float z, int *uc, int *vc);
+
 /* scene octree root diagonal (Scene::build_tree, src/scene.cc:384-387) */
 double oracle_scene_diag(oracle_scene *s);
 
