@@ -370,6 +370,18 @@ int  sray_octree_build_host(const sray_scene_desc *s, int level, sray_octree_inf
                             float *node_lo, float *node_hi, uint32_t *items,
                             uint32_t cap_nodes, uint32_t cap_items);
 
+/* Replaces Light::build_projmap (src/light.cc:151-205, 392-432): the
+ * projection map of one light over the frame interval [t0, t1] as
+ * SRAY_PROJ_U x SRAY_PROJ_V cell bytes (v-major, 1 = full) and its
+ * coverage; kind 0 = all objects (global photons), 1 = reflecting or
+ * refracting objects (caustic photons).  Host only, the same maps
+ * sray_gpu_shoot_photons builds. */
+int  sray_projmap_build(const sray_scene_desc *s, int light, int kind, int t0, int t1, uint64_t seed,
+                        uint8_t *cells, double *coverage);
+/* Replaces ProjMap::cell_from_dir (src/projmap.cc:146-163): the cell of a
+ * direction (see SRAY_PROJ_U above). */
+int  sray_projmap_cell(float x, float y, float z, int *uc, int *vc);
+
 /* Seeded synthetic workloads (no reference counterpart; the reference ships
  * no scenes): config 0 = 64 spheres, 1 = fBm heightfield of 1M triangles +
  * 40 glass/mirror spheres, 2 = photon room, 3 = batch scene with `variant`
@@ -404,7 +416,8 @@ int  sray_gpu_kernel_stats(sray_gpu_ctx *ctx, sray_kernel_stat *out, int cap, in
  * sray_mesh_desc, sray_material_desc, sray_light_desc, sray_camera_desc,
  * sray_scene_desc, sray_ray, sray_hit, sray_trace_counts,
  * sray_render_params, sray_stats, sray_photon, sray_gather_query,
- * sray_request, sray_result, sray_octree_info, sray_kernel_stat) — lets
+ * sray_request, sray_result, sray_octree_info, sray_kernel_stat,
+ * sray_photon_params, sray_photon_stats) — lets
  * bindings check their
  * layout.  Returns the number of entries written (at most n). */
 int  sray_abi_sizes(uint32_t *out, int n);

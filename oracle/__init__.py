@@ -61,6 +61,12 @@ def lib(fast=False):
         L.oracle_rng_u32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
         L.oracle_splitmix64.restype = C.c_uint64
         L.oracle_splitmix64.argtypes = [C.c_uint64]
+        L.oracle_shoot_photons.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(C.c_uint32), vp, C.c_uint32,
+                                           C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
+        L.oracle_projmap.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, vp,
+                                     C.POINTER(C.c_double)]
+        L.oracle_cell_from_dir.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]
         _libs[name] = L
     return _libs[name]
 
@@ -126,6 +132,28 @@ class OracleScene:
         self.L.oracle_gather(self.h, kind, q.ctypes.data, len(q), radius, k, mode, out.ctypes.data)
         return out
 
+    def shoot_photons(self, params):
+        """shoot_photons: (caustic photons, global photons, segments traced);
+        photons as (n, 12) float32 rows pos, dir, norm, col.  The maps also
+        become this scene's photon maps for render/gather."""
+        nc, ng, seg = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        pp = C.addressof(params)
+        if self.L.oracle_shoot_photons(self.h, pp, None, 0, C.byref(nc), None, 0, C.byref(ng), C.byref(seg)):
+            raise RuntimeError("oracle_shoot_photons failed")
+        c = np.zeros((max(nc.value, 1), 12), np.float32)
+        g = np.zeros((max(ng.value, 1), 12), np.float32)
+        if self.L.oracle_shoot_photons(self.h, pp, c.ctypes.data, nc.value, C.byref(nc), g.ctypes.data, ng.value,
+                                       C.byref(ng), C.byref(seg)):
+            raise RuntimeError("oracle_shoot_photons failed")
+        return c[:nc.value], g[:ng.value], seg.value
+
+    def projmap(self, light, kind, t0=0, t1=0, seed=0):
+        cells = np.zeros(64 * 32, np.uint8)
+        cov = C.c_double()
+        if self.L.oracle_projmap(self.h, light, kind, t0, t1, seed, cells.ctypes.data, C.byref(cov)):
+            raise RuntimeError("oracle_projmap failed")
+        return cells.reshape(32, 64), cov.value
+
     def diag(self):
         return self.L.oracle_scene_diag(self.h)
 
@@ -147,3 +175,9 @@ def rng_u32(seed, a, b, c, d):
 
 def splitmix64(x):
     return lib().oracle_splitmix64(x)
+
+
+def cell_from_dir(x, y, z):
+    u, v = C.c_int(), C.c_int()
+    lib().oracle_cell_from_dir(x, y, z, C.byref(u), C.byref(v))
+    return u.value, v.value

@@ -157,6 +157,30 @@ def render_params(width, height, rect=None, samples=(1, 1), max_var=0.005, min_e
     return p
 
 
+def photon_params(caust=0, gi=0, energy=300.0, iter=7, t0=0, t1=0, seed=0x5EED):
+    """sray_photon_params (-c, -g, -photonenergy, -i of src/opt.c)"""
+    pp = abi.PhotonParams()
+    pp.caust_photons, pp.gi_photons, pp.photon_energy = caust, gi, energy
+    pp.iter, pp.t0, pp.t1, pp.seed = iter, t0, t1, seed
+    return pp
+
+
+def projmap(desc_ptr, light, kind, t0=0, t1=0, seed=0x5EED):
+    """projection map of one light (host): (cells[32, 64] uint8, coverage)"""
+    cells = np.zeros(64 * 32, np.uint8)
+    cov = C.c_double()
+    rc = lib().sray_projmap_build(desc_ptr, light, kind, t0, t1, seed, cells.ctypes.data, C.byref(cov))
+    if rc:
+        raise SrayError(rc, _host_err())
+    return cells.reshape(32, 64), cov.value
+
+
+def projmap_cell(x, y, z):
+    u, v = C.c_int(), C.c_int()
+    lib().sray_projmap_cell(x, y, z, C.byref(u), C.byref(v))
+    return u.value, v.value
+
+
 # ---------------------------------------------------------------------------
 # device context
 # ---------------------------------------------------------------------------
@@ -214,6 +238,35 @@ class Context:
         self._check(lib().sray_gpu_render(self._h, sid, C.byref(params),
                                           out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(st)))
         return out, st
+
+    def shoot_photons(self, sid, caust=0, gi=0, energy=300.0, iter=7, t0=0, t1=0, seed=0x5EED):
+        """shoot_photons on the device: ((caustic map id, global map id), PhotonStats);
+        the maps are bound to the scene for its renders (id 0 = not shot)"""
+        pp = photon_params(caust, gi, energy, iter, t0, t1, seed)
+        ids = (C.c_uint32(), C.c_uint32())
+        st = abi.PhotonStats()
+        self._check(lib().sray_gpu_shoot_photons(self._h, sid, C.byref(pp), C.byref(ids[0]), C.byref(ids[1]),
+                                                 C.byref(st)))
+        return (ids[0].value, ids[1].value), st
+
+    def pmap_upload(self, sid, photons, kind):
+        """photons: (n, 12) float32 rows pos, dir, norm, col -> map id (bound to scene sid if sid)"""
+        arr = np.ascontiguousarray(photons, np.float32).reshape(-1, 12)
+        pid = C.c_uint32()
+        self._check(lib().sray_gpu_pmap_upload(self._h, sid, arr.ctypes.data_as(C.POINTER(abi.Photon)), len(arr),
+                                               kind, C.byref(pid)))
+        return pid.value
+
+    def pmap_download(self, pid):
+        """a map's photons in storage order as (n, 12) float32"""
+        n = C.c_uint32()
+        self._check(lib().sray_gpu_pmap_download(self._h, pid, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 12), np.float32)
+        self._check(lib().sray_gpu_pmap_download(self._h, pid, C.c_void_p(out.ctypes.data), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def pmap_release(self, pid):
+        self._check(lib().sray_gpu_pmap_release(self._h, pid))
 
     def render_device(self, sid, params, d_out):
         st = Stats()

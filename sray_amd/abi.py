@@ -99,6 +99,15 @@ class Photon(C.Structure):
     _fields_ = [("pos", f32 * 3), ("dir", f32 * 3), ("norm", f32 * 3), ("col", f32 * 3)]
 
 
+class PhotonParams(C.Structure):
+    _fields_ = [("caust_photons", i32), ("gi_photons", i32), ("photon_energy", f32), ("iter", i32), ("t0", i32),
+                ("t1", i32), ("seed", u64)]
+
+
+class PhotonStats(C.Structure):
+    _fields_ = [("shot", u64 * 2), ("stored", u64 * 2), ("segments", u64), ("trace_ms", f64), ("build_ms", f64)]
+
+
 class GatherQuery(C.Structure):
     _fields_ = [("pos", f32 * 3), ("norm", f32 * 3), ("dir", f32 * 3), ("pad_", f32)]
 
@@ -122,7 +131,8 @@ class KernelStat(C.Structure):
 
 
 ABI_STRUCTS = [ObjectDesc, MeshDesc, MaterialDesc, LightDesc, CameraDesc, SceneDesc, Ray, Hit, TraceCounts,
-               RenderParams, Stats, Photon, GatherQuery, Request, Result, OctreeInfo, KernelStat]
+               RenderParams, Stats, Photon, GatherQuery, Request, Result, OctreeInfo, KernelStat, PhotonParams,
+               PhotonStats]
 
 # every extern "C" entry point declared in include/sray_gpu.h
 EXPORTS = [
@@ -134,6 +144,7 @@ EXPORTS = [
     "sray_octree_build_host", "sray_scene_generate", "sray_scene_write_xml", "sray_abi_sizes",
     "sray_gpu_primary_rays_device", "sray_gpu_set_profiling", "sray_gpu_kernel_stats",
     "sray_gpu_malloc", "sray_gpu_free", "sray_gpu_memcpy", "sray_gpu_synchronize",
+    "sray_gpu_shoot_photons", "sray_gpu_pmap_download", "sray_projmap_build", "sray_projmap_cell",
 ]
 
 
@@ -173,6 +184,10 @@ def declare(lib):
         "sray_gpu_memcpy": (i32, [vp, vp, vp, C.c_size_t, C.c_int]),
         "sray_gpu_synchronize": (i32, [vp]),
         "sray_gpu_kernel_stats": (i32, [vp, P(KernelStat), C.c_int, P(C.c_int), C.c_int]),
+        "sray_gpu_shoot_photons": (i32, [vp, u32, P(PhotonParams), P(u32), P(u32), P(PhotonStats)]),
+        "sray_gpu_pmap_download": (i32, [vp, u32, vp, u32, P(u32)]),
+        "sray_projmap_build": (i32, [P(SceneDesc), C.c_int, C.c_int, C.c_int, C.c_int, u64, vp, P(f64)]),
+        "sray_projmap_cell": (i32, [f32, f32, f32, P(C.c_int), P(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

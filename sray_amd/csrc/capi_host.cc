@@ -113,6 +113,30 @@ int sray_octree_build_host(const sray_scene_desc *s, int level, sray_octree_info
     return SRAY_OK;
 }
 
+int sray_projmap_build(const sray_scene_desc *s, int light, int kind, int t0, int t1, uint64_t seed, uint8_t *cells,
+                       double *coverage)
+{
+    if (!s || !cells || !coverage || light < 0 || (uint32_t)light >= s->num_lights || (kind != 0 && kind != 1)) {
+        g_host_err = "sray_projmap_build: bad arguments";
+        return SRAY_ERR_ARG;
+    }
+    sray::PhotonScene ps;
+    sray::photon_scene_from_desc(s, ps);
+    sray::ProjMap all, spec;
+    sray::build_projmaps(ps, light, t0, t1, seed, all, spec);
+    const sray::ProjMap &m = kind == 0 ? all : spec;
+    memcpy(cells, m.cells, sizeof m.cells);
+    *coverage = m.coverage;
+    return SRAY_OK;
+}
+
+int sray_projmap_cell(float x, float y, float z, int *uc, int *vc)
+{
+    if (!uc || !vc) return SRAY_ERR_ARG;
+    sray::cell_from_dir(x, y, z, *uc, *vc);
+    return SRAY_OK;
+}
+
 } /* extern "C" */
 
 extern "C" int sray_abi_sizes(uint32_t *out, int n)
@@ -125,7 +149,8 @@ extern "C" int sray_abi_sizes(uint32_t *out, int n)
                            (uint32_t)sizeof(sray_stats),         (uint32_t)sizeof(sray_photon),
                            (uint32_t)sizeof(sray_gather_query),  (uint32_t)sizeof(sray_request),
                            (uint32_t)sizeof(sray_result),        (uint32_t)sizeof(sray_octree_info),
-                           (uint32_t)sizeof(sray_kernel_stat)};
+                           (uint32_t)sizeof(sray_kernel_stat),   (uint32_t)sizeof(sray_photon_params),
+                           (uint32_t)sizeof(sray_photon_stats)};
     int k = 0;
     for (; k < n && k < (int)(sizeof sz / sizeof sz[0]); k++) out[k] = sz[k];
     return k;

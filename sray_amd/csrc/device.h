@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "../../include/sray_gpu.h"
+#include "rng.h"
 
 #pragma clang fp contract(off)
 
@@ -130,35 +131,6 @@ __device__ __forceinline__ V3 xf_normal(const float *i, V3 n)
               i[2] * n.x + i[6] * n.y + i[10] * n.z};
 }
 
-/* ------------------------------------------------------------------ */
-/* counter-based RNG: SplitMix64 chain (same definition as the checker) */
-/* ------------------------------------------------------------------ */
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
-{
-    uint64_t z = x + 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-__device__ __forceinline__ uint32_t rng_u32(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
-{
-    uint64_t h = splitmix64(seed);
-    h = splitmix64(h ^ (uint64_t)a);
-    h = splitmix64(h ^ (((uint64_t)b << 32) | (uint64_t)c));
-    h = splitmix64(h ^ (uint64_t)d);
-    return (uint32_t)(h >> 32);
-}
-__device__ __forceinline__ float rng_unit(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
-{
-    return (float)(rng_u32(seed, a, b, c, d) >> 8) * (1.0f / 16777216.0f);
-}
-/* slot word: kind<<30 | i1<<20 | i2<<10 | attempt<<3 | dim (kind 0 camera,
- * 1 light sample, 2 diffuse GI direction, 3 photon emission/bounce) */
-__device__ __forceinline__ uint32_t rslot(uint32_t kind, uint32_t i1, uint32_t i2, uint32_t att, uint32_t dim)
-{
-    return (kind << 30) | (i1 << 20) | (i2 << 10) | (att << 3) | dim;
-}
-constexpr uint32_t K_CAMERA = 0u, K_LIGHT = 1u, K_DIFFUSE = 2u, K_PHOTON = 3u;
 
 } // namespace srd
 

@@ -92,6 +92,38 @@ void object_boxes(const sray_scene_desc *s, std::vector<float> &boxes);
 void triangle_boxes(const sray_scene_desc *s, uint32_t mesh, std::vector<float> &boxes);
 
 /* ------------------------------------------------------------------ */
+/* photon shooting, host half (photon_host.cc)                           */
+/* ------------------------------------------------------------------ */
+struct PhotonObj {
+    int32_t type = 0;
+    float rad = 0, start = 0, end = 0;
+    float xform[12] = {};
+    float motion[3] = {};
+    int32_t mt0 = 0, mt1 = 0;
+    bool moving = false;
+    bool specular = false; /* caustic caster (reflect or refract attribute > 0) */
+    float lin[6] = {};     /* mesh: vertex bounds under the matrix without translation */
+};
+/* what the projection maps need of a scene, kept after upload */
+struct PhotonScene {
+    std::vector<PhotonObj> objs;
+    std::vector<sray_light_desc> lights;
+};
+struct ProjMap {
+    uint8_t cells[SRAY_PROJ_U * SRAY_PROJ_V]; /* v-major */
+    double coverage;
+};
+void photon_scene_from_desc(const sray_scene_desc *s, PhotonScene &ps);
+void photon_object_box(const PhotonObj &o, int time, float b[6]);
+void cell_from_dir(float x, float y, float z, int &uc, int &vc);
+void cell_tables(float cv[SRAY_PROJ_V + 1], float cu[SRAY_PROJ_U + 1]);
+void photon_light_point(const sray_light_desc &l, uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t purpose,
+                        float out[3]);
+/* all = map of every object, spec = map of the caustic casters */
+void build_projmaps(const PhotonScene &ps, int light, int t0, int t1, uint64_t seed, ProjMap &all, ProjMap &spec);
+void photon_counts(const PhotonScene &ps, int num, std::vector<uint32_t> &nphot);
+
+/* ------------------------------------------------------------------ */
 /* XML                                                                  */
 /* ------------------------------------------------------------------ */
 /* attribute typing of src/xmltree.c:51-99 */

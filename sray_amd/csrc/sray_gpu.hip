@@ -31,6 +31,8 @@
 #include "host.h"
 #include "pmap.h"
 #include "render.h"
+#include "surface.h"
+#include "photon.h"
 
 using namespace srd;
 
@@ -219,114 +221,6 @@ __device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uin
     return v3(lt.pos[0], lt.pos[1], lt.pos[2]);
 }
 
-/* interp_face, src/mesh.cc:321-445 (inverse bilinear u,v; lerped attributes) */
-__device__ void interp_face(const float4 *tri, const float *tn, const float *tg, const float *tuv, uint32_t ti, V3 ispos,
-                            V3 normal, V3 &on, V3 &og, float &ou, float &ov)
-{
-    float4 A = tri[(size_t)ti * 3], B = tri[(size_t)ti * 3 + 1], C = tri[(size_t)ti * 3 + 2];
-    V3 p00 = v3(A.x, A.y, A.z), p10 = v3(B.x, B.y, B.z), p01 = v3(C.x, C.y, C.z), p11 = p01;
-    V3 pa = add(sub(p00, p10), sub(p11, p01));
-    V3 pb = sub(p10, p00);
-    V3 pc = sub(p01, p00);
-    V3 pd = p00;
-    V3 na = cross(pa, normal), nb = cross(pb, normal), nc = cross(pc, normal);
-    float du0 = dot(nc, pd);
-    float du1 = dot(na, pd) + dot(nc, pb);
-    float du2 = dot(na, pb);
-    float dv0 = dot(nb, pd);
-    float dv1 = dot(na, pd) + dot(nb, pc);
-    float dv2 = dot(na, pc);
-    float ru, rv;
-    if (fabsf(du2) <= XSMALL) {
-        float divisor = du1 - dot(na, ispos);
-        ru = fabsf(divisor) > XSMALL ? ((dot(nc, ispos) - du0) / divisor) : 0.0f;
-    } else {
-        V3 qux = dvs(na, 2.0f * du2);
-        V3 quy = dvs(neg(nc), du2);
-        float dux = -du1 / (2.0f * du2);
-        float duy = du0 / du2;
-        float ka = dux + dot(qux, ispos);
-        float kb = duy + dot(quy, ispos);
-        float u0 = ka - sqrtf(ka * ka - kb);
-        float u1 = ka + sqrtf(ka * ka - kb);
-        ru = (u0 < 0.0f || u0 > 1.0f) ? u1 : u0;
-    }
-    if (fabsf(dv2) <= XSMALL) {
-        float divisor = dv1 - dot(na, ispos);
-        rv = fabsf(divisor) > XSMALL ? ((dot(nb, ispos) - dv0) / divisor) : 0.0f;
-    } else {
-        V3 qvx = dvs(na, 2.0f * dv2);
-        V3 qvy = dvs(neg(nb), dv2);
-        float dvx = -dv1 / (2.0f * dv2);
-        float dvy = dv0 / dv2;
-        float ka = dvx + dot(qvx, ispos);
-        float kb = dvy + dot(qvy, ispos);
-        float v0 = ka - sqrtf(ka * ka - kb);
-        float v1 = ka + sqrtf(ka * ka - kb);
-        rv = (v0 < 0.0f || v0 > 1.0f) ? v1 : v0;
-    }
-    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
-    auto l2 = [&](float a00, float a10, float a01) {
-        float s0 = lerp(a00, a01, rv);
-        float s1 = lerp(a10, a01, rv);
-        return lerp(s0, s1, ru);
-    };
-    if (tuv) {
-        const float *t = tuv + (size_t)ti * 6;
-        ou = l2(t[0], t[2], t[4]);
-        ov = l2(t[1], t[3], t[5]);
-    } else {
-        ou = ov = 0.0f;
-    }
-    if (tn) {
-        const float *n = tn + (size_t)ti * 9;
-        on = v3(l2(n[0], n[3], n[6]), l2(n[1], n[4], n[7]), l2(n[2], n[5], n[8]));
-    } else on = v3(0.0f, 0.0f, 0.0f);
-    if (tg) {
-        const float *g = tg + (size_t)ti * 9;
-        og = v3(l2(g[0], g[3], g[6]), l2(g[1], g[4], g[7]), l2(g[2], g[5], g[8]));
-    } else og = v3(0.0f, 0.0f, 0.0f);
-}
-
-__device__ __forceinline__ float fresnel(float r, float cosa)
-{
-    float inv = 1.0f - cosa;
-    float inv_sq = inv * inv;
-    return r + (1.0f - r) * inv_sq * inv_sq * inv;
-}
-
-/* refraction of the unit incident d about n; total internal reflection
- * returns the mirror direction (libvmath Vector3::refraction, fixed here) */
-__device__ __forceinline__ V3 refract(V3 d, V3 n, float ior)
-{
-    float cos_inc = -dot(d, n);
-    float radical = 1.0f + ior * ior * (cos_inc * cos_inc - 1.0f);
-    if (radical < 0.0f) return reflect(d, n);
-    float beta = ior * cos_inc - sqrtf(radical);
-    return v3(d.x * ior + n.x * beta, d.y * ior + n.y * beta, d.z * ior + n.z * beta);
-}
-
-/* per-ray ior stack (vmath Ray::enter/leave/calc_ior) */
-struct IorStack {
-    float cur;
-    float s[IOR_DEPTH];
-    uint32_t n;
-    bool overflow;
-    __device__ void enter(float x)
-    {
-        if (n < (uint32_t)IOR_DEPTH) s[n++] = cur;
-        else overflow = true;
-        cur = x;
-    }
-    __device__ void leave() { cur = n > 0 ? s[--n] : 1.0f; }
-    __device__ float calc(bool entering, float mat_ior) const
-    {
-        if (entering) return cur / mat_ior;
-        float to = n > 0 ? s[n - 1] : 1.0f;
-        return cur / to;
-    }
-};
-
 __device__ __forceinline__ uint32_t pixel_id(const RenderDev &P, const uint32_t *active, uint32_t c0, uint32_t slot)
 {
     uint32_t pr = active[c0 + slot];
@@ -388,55 +282,11 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, ShadePma
         } else {
             float t = __int_as_float(h.x);
             const DevObj &ob = S.objs[h.y];
-            float I[12], M[12];
-            obj_inv_at(ob, time, I);
-            obj_xf_at(ob, time, M);
-            V3 lo = xf_point(I, ro), ld = xf_vec(I, rd);
-            V3 p = add(lo, mul(ld, t));
-            if (ob.type != SRAY_OBJ_MESH) {
-                /* surface point of a quadric from the double-precision root */
-                double td;
-                bool ok = ob.type == SRAY_OBJ_SPHERE
-                              ? sphere_td((double)ob.rad, lo, ld, td)
-                              : cylinder_td((double)ob.rad, (double)ob.start, (double)ob.end, lo, ld, td);
-                if (ok)
-                    p = v3((float)((double)lo.x + (double)ld.x * td), (float)((double)lo.y + (double)ld.y * td),
-                           (float)((double)lo.z + (double)ld.z * td));
-            }
-            V3 N;
-            float tu = 0.0f, tv = 0.0f;
             const DevMat &mat = S.mats[ob.mat];
             bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
-            if (ob.type == SRAY_OBJ_SPHERE) {
-                V3 nrm = dvs(p, ob.rad);
-                N = xf_normal(I, nrm);
-                if (!phong) {
-                    float r = length(p);
-                    tu = (atan2f(p.z, p.x) + 3.14159265358979f) / 6.28318530717959f;
-                    tv = r > 0.0f ? acosf(fmaxf(-1.0f, fminf(1.0f, p.y / r))) / 3.14159265358979f : 0.0f;
-                }
-            } else if (ob.type == SRAY_OBJ_CYLINDER) {
-                V3 nrm = dvs(v3(p.x, 0.0f * p.y, p.z), ob.rad);
-                N = xf_normal(I, nrm);
-                if (!phong) {
-                    tu = (atan2f(p.z, p.x) + 3.14159265358979f) / 6.28318530717959f;
-                    tv = p.y;
-                }
-            } else {
-                uint32_t ti = (uint32_t)h.z;
-                float4 A = S.tri[(size_t)ti * 3], B = S.tri[(size_t)ti * 3 + 1], C = S.tri[(size_t)ti * 3 + 2];
-                V3 e1 = v3(B.x - A.x, B.y - A.y, B.z - A.z), e2 = v3(C.x - A.x, C.y - A.y, C.z - A.z);
-                V3 gn = cross(e1, e2);
-                V3 nrm = gn;
-                if (ob.has_attr) {
-                    V3 in, ig;
-                    interp_face(S.tri, (ob.has_attr & 1) ? S.tri_n : nullptr, (ob.has_attr & 4) ? S.tri_g : nullptr,
-                                (ob.has_attr & 2) ? S.tri_uv : nullptr, ti, p, gn, in, ig, tu, tv);
-                    nrm = dot(in, in) < XSMALL ? gn : in;
-                }
-                N = normalized(xf_normal(I, nrm));
-            }
-            V3 Pw = xf_point(M, p);
+            V3 Pw, N;
+            float tu, tv;
+            hit_surface(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong, Pw, N, tu, tv);
             if (path == 1u) dir.w = 1.0f;
             uint32_t pix = pixel_id(P, active, c0, slot);
             int nl = S.nlights;
@@ -1008,6 +858,7 @@ template <class T> T *dptr(DevBuf &b) { return (T *)b.p; }
 struct GpuPmap {
     std::vector<DevBuf> bufs;
     PmapTree tree{};
+    sray_photon *store = nullptr; /* photons in storage order */
     uint32_t scene_id = 0;
     int kind = 0;
     uint64_t bytes = 0;
@@ -1021,6 +872,7 @@ struct GpuScene {
     uint32_t scene_height = 0, mesh_height = 0;
     uint32_t nobjs = 0;
     uint32_t pmap[2] = {0, 0}; /* bound caustic / global photon maps */
+    sray::PhotonScene photon;  /* what the projection maps need */
 };
 
 } // namespace
@@ -1348,6 +1200,7 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     D.max_dist = s->camera.max_dist;
     D.shutter = s->camera.shutter;
     g->nobjs = s->num_objects;
+    sray::photon_scene_from_desc(s, g->photon);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     uint32_t id = c->next_scene++;
     c->scenes[id] = std::move(g);
@@ -1409,7 +1262,9 @@ struct LevelCtx {
     unsigned long long *stats;
 };
 
-int compact(sray_gpu_ctx *c, const QRay *in, const uint8_t *flags, QRay *out, uint32_t n, uint32_t *count)
+/* stable selection of the flagged elements (order kept), count to the host */
+template <class T>
+int select_flagged(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, uint32_t n, uint32_t *count)
 {
     size_t tmp = 0;
     HIPCHK(c, hipcub::DeviceSelect::Flagged(nullptr, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
@@ -1421,18 +1276,13 @@ int compact(sray_gpu_ctx *c, const QRay *in, const uint8_t *flags, QRay *out, ui
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SRAY_OK;
 }
-
+int compact(sray_gpu_ctx *c, const QRay *in, const uint8_t *flags, QRay *out, uint32_t n, uint32_t *count)
+{
+    return select_flagged(c, in, flags, out, n, count);
+}
 int compact_u32(sray_gpu_ctx *c, const uint32_t *in, const uint8_t *flags, uint32_t *out, uint32_t n, uint32_t *count)
 {
-    size_t tmp = 0;
-    HIPCHK(c, hipcub::DeviceSelect::Flagged(nullptr, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
-    int rc = ensure(c, c->cubtmp, tmp);
-    if (rc) return rc;
-    tmp = c->cubtmp.bytes;
-    HIPCHK(c, hipcub::DeviceSelect::Flagged(c->cubtmp.p, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
-    HIPCHK(c, hipMemcpyAsync(count, c->nsel.p, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return SRAY_OK;
+    return select_flagged(c, in, flags, out, n, count);
 }
 
 /* one bounce level over qlev[b][0,count): sub-chunks of CHUNK rays, each
@@ -1654,7 +1504,10 @@ void pm_levels(uint32_t n, std::vector<std::vector<uint32_t>> &lo, std::vector<s
     }
 }
 
-int do_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, uint32_t n, int kind, uint32_t *pmap_id)
+/* photon map from n photons in storage order (host or device memory): the
+ * array is kept as given (for download) and a kd-tree copy is built */
+int do_pmap_build(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, bool on_device, uint32_t n, int kind,
+                  uint32_t *pmap_id)
 {
     if (kind != SRAY_PMAP_CAUSTIC && kind != SRAY_PMAP_GLOBAL) return fail(c, SRAY_ERR_ARG, "bad photon map kind");
     if (scene_id && !find_scene(c, scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
@@ -1674,16 +1527,17 @@ int do_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, ui
         *p = b.p;
         return SRAY_OK;
     };
-    void *pos, *dir, *nrm, *colb, *axis;
-    if ((rc = alloc((size_t)n * 16, &pos)) || (rc = alloc((size_t)n * 16, &dir)) || (rc = alloc((size_t)n * 16, &nrm)) ||
+    void *pos, *dir, *nrm, *colb, *axis, *store;
+    if ((rc = alloc((size_t)n * sizeof(sray_photon), &store)) || (rc = alloc((size_t)n * 16, &pos)) || (rc = alloc((size_t)n * 16, &dir)) || (rc = alloc((size_t)n * 16, &nrm)) ||
         (rc = alloc((size_t)n * 4, &colb)) || (rc = alloc((size_t)n, &axis)))
         return rc;
     uint32_t depth = 0;
     if (n) {
         /* scratch for the build */
-        DevBuf dph, pos0, ord[2], key[2], seg, bb, slo, shi, sax, tmp;
+        DevBuf pos0, ord[2], key[2], seg, bb, slo, shi, sax, tmp;
+        sray_photon *dph = (sray_photon *)store;
         auto cleanup = [&]() {
-            DevBuf *bs[] = {&dph, &pos0, &ord[0], &ord[1], &key[0], &key[1], &seg, &bb, &slo, &shi, &sax, &tmp};
+            DevBuf *bs[] = {&pos0, &ord[0], &ord[1], &key[0], &key[1], &seg, &bb, &slo, &shi, &sax, &tmp};
             for (DevBuf *b : bs) release(*b);
         };
         std::vector<std::vector<uint32_t>> L, H;
@@ -1691,7 +1545,7 @@ int do_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, ui
         depth = (uint32_t)L.size() + 1;
         size_t maxseg = 1;
         for (auto &v : L) maxseg = std::max(maxseg, v.size());
-        if ((rc = ensure(c, dph, (size_t)n * sizeof(sray_photon))) || (rc = ensure(c, pos0, (size_t)n * 16)) ||
+        if ((rc = ensure(c, pos0, (size_t)n * 16)) ||
             (rc = ensure(c, ord[0], (size_t)n * 4)) || (rc = ensure(c, ord[1], (size_t)n * 4)) ||
             (rc = ensure(c, key[0], (size_t)n * 8)) || (rc = ensure(c, key[1], (size_t)n * 8)) ||
             (rc = ensure(c, seg, (size_t)n * 4)) || (rc = ensure(c, bb, maxseg * 24)) ||
@@ -1699,13 +1553,14 @@ int do_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, ui
             cleanup();
             return rc;
         }
-        hipError_t e = hipMemcpyAsync(dph.p, ph, (size_t)n * sizeof(sray_photon), hipMemcpyHostToDevice, c->stream);
+        hipError_t e = hipMemcpyAsync(dph, ph, (size_t)n * sizeof(sray_photon),
+                                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) {
             cleanup();
             return fail(c, SRAY_ERR_HIP, "photon upload failed");
         }
         (void)hipMemsetAsync(axis, 0, n, c->stream);
-        k_pm_init<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<sray_photon>(dph), n, dptr<float4>(pos0),
+        k_pm_init<<<blocks(n, 256), 256, 0, c->stream>>>(dph, n, dptr<float4>(pos0),
                                                           dptr<uint32_t>(ord[0]));
         int endbit = 32;
         while ((1ull << (endbit - 32)) <= n) endbit++;
@@ -1744,7 +1599,7 @@ int do_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, ui
             k_pm_nodes<<<blocks(ns, 256), 256, 0, c->stream>>>(dptr<uint32_t>(slo), dptr<uint32_t>(shi),
                                                                 dptr<uint8_t>(sax), ns, (uint8_t *)axis);
         }
-        k_pm_final<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<sray_photon>(dph), dptr<uint32_t>(ord[cur]), n,
+        k_pm_final<<<blocks(n, 256), 256, 0, c->stream>>>(dph, dptr<uint32_t>(ord[cur]), n,
                                                            (float4 *)pos, (float4 *)dir, (float4 *)nrm, (float *)colb);
         e = hipStreamSynchronize(c->stream);
         cleanup();
@@ -1759,6 +1614,7 @@ int do_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, ui
     T.axis = (const uint8_t *)axis;
     T.n = n;
     T.depth = depth;
+    g->store = (sray_photon *)store;
     uint32_t id = c->next_pmap++;
     c->pmaps[id] = std::move(g);
     if (scene_id) find_scene(c, scene_id)->pmap[kind] = id; /* the scene's caustic/global map */
@@ -1821,6 +1677,138 @@ int do_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q
     HIPCHK(c, hipMemcpyAsync(rgb, c->grgb.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     prof_resolve(c);
+    return SRAY_OK;
+}
+
+/* shoot_photons (src/render.cc:67-122): projection maps on the host, one
+ * device lane per emitted photon, stable compaction in emission order, then
+ * the kd-tree build; the maps are bound to the scene for its renders */
+int do_shoot(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon_params *pp, uint32_t *ids, sray_photon_stats *st)
+{
+    GpuScene *g = find_scene(c, scene_id);
+    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    if (pp->caust_photons < 0 || pp->gi_photons < 0) return fail(c, SRAY_ERR_ARG, "negative photon count");
+    if (pp->t1 < pp->t0) return fail(c, SRAY_ERR_ARG, "photon interval t1 < t0");
+    const sray::PhotonScene &ps = g->photon;
+    size_t nl = ps.lights.size();
+    if (st) memset(st, 0, sizeof *st);
+    ids[0] = ids[1] = 0;
+    if (nl == 0 || (pp->caust_photons == 0 && pp->gi_photons == 0)) return SRAY_OK;
+    std::vector<sray::ProjMap> all(nl), spec(nl);
+    for (size_t li = 0; li < nl; li++) sray::build_projmaps(ps, (int)li, pp->t0, pp->t1, pp->seed, all[li], spec[li]);
+    PhotonJob J;
+    memset(&J, 0, sizeof J);
+    sray::cell_tables(J.tab.cv, J.tab.cu);
+    J.nlights = (int32_t)nl;
+    J.t0 = pp->t0;
+    J.t1 = pp->t1;
+    J.iter = pp->iter;
+    J.seed = pp->seed;
+    int rc;
+    DevBuf dl, dc, dout, dkeep, dsel, dseg;
+    auto cleanup = [&]() {
+        DevBuf *bs[] = {&dl, &dc, &dout, &dkeep, &dsel, &dseg};
+        for (DevBuf *b : bs) release(*b);
+    };
+    if ((rc = ensure(c, dl, nl * sizeof(PhotonLightDev))) || (rc = ensure(c, dc, nl * SRAY_PROJ_U * SRAY_PROJ_V)) ||
+        (rc = ensure(c, dseg, sizeof(unsigned long long))) || (rc = ensure(c, c->nsel, 4))) {
+        cleanup();
+        return rc;
+    }
+    hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    double trace_ms = 0, build_ms = 0;
+    for (int kind = 0; kind < 2; kind++) {
+        int num = kind == SRAY_PMAP_CAUSTIC ? pp->caust_photons : pp->gi_photons;
+        if (num <= 0) continue;
+        std::vector<uint32_t> nphot;
+        sray::photon_counts(ps, num, nphot);
+        std::vector<PhotonLightDev> L(nl);
+        std::vector<uint8_t> cells(nl * SRAY_PROJ_U * SRAY_PROJ_V);
+        uint64_t total = 0;
+        for (size_t li = 0; li < nl; li++) {
+            /* caustic photons aim at the specular objects' map, global ones at all objects */
+            const sray::ProjMap &m = kind == SRAY_PMAP_CAUSTIC ? spec[li] : all[li];
+            bool use = m.coverage != 0.0;
+            double e = (double)pp->photon_energy * (use ? m.coverage : 1.0);
+            for (int k = 0; k < 3; k++) L[li].col0[k] = (float)((double)ps.lights[li].color[k] * e);
+            L[li].use_map = use ? 1 : 0;
+            L[li].first = (uint32_t)total;
+            L[li].count = nphot[li];
+            L[li].scale = nphot[li] ? 1.0 / (double)nphot[li] : 0.0;
+            memcpy(&cells[li * SRAY_PROJ_U * SRAY_PROJ_V], m.cells, SRAY_PROJ_U * SRAY_PROJ_V);
+            total += nphot[li];
+        }
+        if (total >= (1ull << 30)) {
+            cleanup();
+            return fail(c, SRAY_ERR_LIMIT, "more than 2^30 photons");
+        }
+        if (st) st->shot[kind] = total;
+        if (total == 0) continue;
+        uint32_t n = (uint32_t)total;
+        if ((rc = ensure(c, dout, (size_t)n * sizeof(sray_photon))) || (rc = ensure(c, dkeep, n)) ||
+            (rc = ensure(c, dsel, (size_t)n * sizeof(sray_photon)))) {
+            cleanup();
+            return rc;
+        }
+        HIPCHK(c, hipMemcpyAsync(dl.p, L.data(), nl * sizeof(PhotonLightDev), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dc.p, cells.data(), cells.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemsetAsync(dseg.p, 0, sizeof(unsigned long long), c->stream));
+        J.lights = dptr<PhotonLightDev>(dl);
+        J.cells = dptr<uint8_t>(dc);
+        J.kind = kind;
+        J.total = n;
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        {
+            ProfScope ps_(c, "k_photon");
+            k_photon<true><<<blocks(n, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(
+                g->dev, J, dptr<sray_photon>(dout), dptr<uint8_t>(dkeep), dptr<unsigned long long>(dseg));
+        }
+        HIPCHK(c, hipGetLastError());
+        uint32_t stored = 0;
+        if ((rc = select_flagged(c, dptr<sray_photon>(dout), dptr<uint8_t>(dkeep), dptr<sray_photon>(dsel), n,
+                                 &stored))) {
+            cleanup();
+            return rc;
+        }
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        trace_ms += ms;
+        unsigned long long segs = 0;
+        HIPCHK(c, hipMemcpy(&segs, dseg.p, sizeof segs, hipMemcpyDeviceToHost));
+        if (st) {
+            st->stored[kind] = stored;
+            st->segments += segs;
+        }
+        /* the previous map of this kind stays alive until released by its id */
+        auto t0 = std::chrono::steady_clock::now();
+        if ((rc = do_pmap_build(c, scene_id, dptr<sray_photon>(dsel), true, stored, kind, &ids[kind]))) {
+            cleanup();
+            return rc;
+        }
+        build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    cleanup();
+    prof_resolve(c);
+    if (st) {
+        st->trace_ms = trace_ms;
+        st->build_ms = build_ms;
+    }
+    return SRAY_OK;
+}
+
+int do_pmap_download(sray_gpu_ctx *c, uint32_t pmap_id, sray_photon *out, uint32_t cap, uint32_t *n)
+{
+    auto it = c->pmaps.find(pmap_id);
+    if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
+    uint32_t cnt = it->second->tree.n;
+    *n = cnt;
+    if (!out || cnt == 0) return SRAY_OK;
+    if (cap < cnt) return fail(c, SRAY_ERR_ARG, "photon buffer holds %u, map has %u", cap, cnt);
+    HIPCHK(c, hipMemcpyAsync(out, it->second->store, (size_t)cnt * sizeof(sray_photon), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return SRAY_OK;
 }
 
@@ -2023,7 +2011,7 @@ int sray_gpu_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *
     if (!c || !pmap_id) return SRAY_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
-    return do_pmap_upload(c, scene_id, p, n, kind, pmap_id);
+    return do_pmap_build(c, scene_id, p, false, n, kind, pmap_id);
 }
 
 int sray_gpu_pmap_release(sray_gpu_ctx *c, uint32_t pmap_id)
@@ -2045,6 +2033,26 @@ int sray_gpu_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_qu
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_pmap_gather(c, pmap_id, q, n, radius, k, mode, rgb);
+}
+
+int sray_gpu_shoot_photons(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon_params *pp, uint32_t *caustic,
+                           uint32_t *global, sray_photon_stats *st)
+{
+    if (!c || !pp) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    uint32_t ids[2] = {0, 0};
+    int rc = do_shoot(c, scene_id, pp, ids, st);
+    if (caustic) *caustic = ids[0];
+    if (global) *global = ids[1];
+    return rc;
+}
+
+int sray_gpu_pmap_download(sray_gpu_ctx *c, uint32_t pmap_id, sray_photon *out, uint32_t cap, uint32_t *n)
+{
+    if (!c || !n) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    return do_pmap_download(c, pmap_id, out, cap, n);
 }
 
 int sray_gpu_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *ticket)

@@ -40,10 +40,26 @@ def phong(kd=(1, 1, 1), ks=0.0, shin=1.0, refl=0.0, refr=0.0, ior=1.5):
     return m
 
 
+def plane(y, half, up=True):
+    """two triangles covering [-half, half]^2 at height y (geometric normal +y if up)"""
+    a, b, c, d = (-half, y, -half), (-half, y, half), (half, y, half), (half, y, -half)
+    t = [[a, b, c], [a, c, d]] if up else [[a, c, b], [a, d, c]]
+    return np.asarray(t, np.float32)
+
+
+def material(kd=(0, 0, 0), ks=0.0, refl=0.0, refr=0.0, ior=1.5, flags=None):
+    """phong material; flags=None sets every attribute (no default-1 fallbacks)"""
+    m = phong(kd, ks, 1.0, refl, refr, ior)
+    if flags is not None:
+        m.flags = flags
+    return m
+
+
 def build(spheres=(), cylinders=(), meshes=(), materials=None, lights=(), camera=None, ambient=(0, 0, 0, 0),
           bg=(0, 0, 0, 0), scene_oct=(5, 5), mesh_oct=(7, 14)):
     """spheres: (center, radius, mat); cylinders: (center, radius, start, end, mat);
-    meshes: (triangles [N,3,3] local, translation, mat); lights: (pos, color)"""
+    meshes: (triangles [N,3,3] local, translation, mat);
+    lights: (pos, color[, type, radius, dim])"""
     b = Built()
     objs, mesh_descs, tris = [], [], []
     for c, r, mat in spheres:
@@ -92,13 +108,15 @@ def build(spheres=(), cylinders=(), meshes=(), materials=None, lights=(), camera
     b.keep.append(mt)
     d.num_materials, d.materials = len(mats), C.cast(mt, C.POINTER(abi.MaterialDesc))
     lts = []
-    for pos, col in lights:
+    for light in lights:
+        pos, col = light[0], light[1]
+        ltype = light[2] if len(light) > 2 else abi.SRAY_LIGHT_POINT
         lt = abi.LightDesc()
-        lt.type = abi.SRAY_LIGHT_POINT
+        lt.type = ltype
         lt.pos[:] = pos
         lt.color[:] = (*col, 1.0)
-        lt.radius = 1.0
-        lt.dim[:] = (1, 1, 1)
+        lt.radius = light[3] if len(light) > 3 else 1.0
+        lt.dim[:] = light[4] if len(light) > 4 else (1, 1, 1)
         lt.xform[:] = translate(*pos).astype(np.float32).ravel()
         lts.append(lt)
     LA = abi.LightDesc * max(len(lts), 1)
