@@ -292,7 +292,8 @@ const char *sray_gpu_last_error(sray_gpu_ctx *ctx);
  * uploads everything to HBM. */
 int  sray_gpu_scene_upload(sray_gpu_ctx *ctx, const sray_scene_desc *s, uint32_t *scene_id);
 int  sray_gpu_scene_release(sray_gpu_ctx *ctx, uint32_t scene_id);
-/* bytes of device memory held by a scene (admission accounting) */
+/* bytes of device memory held by a scene (admission accounting);
+ * scene_id 0: everything the context holds (scenes, photon maps, queues) */
 int  sray_gpu_scene_bytes(sray_gpu_ctx *ctx, uint32_t scene_id, uint64_t *bytes);
 
 /* Replaces Scene::cast_ray (src/scene.cc:764-794) over the octree path
@@ -338,8 +339,18 @@ int  sray_gpu_pmap_download(sray_gpu_ctx *ctx, uint32_t pmap_id, sray_photon *ou
                             uint32_t *n);
 
 /* Batching / serving (no reference counterpart: the reference renders one
- * frame of one scene at a time, src/sray.cc:56-65).  submit admits a tile
- * request by its memory footprint; requests in flight share ray queues. */
+ * frame of one scene at a time, src/sray.cc:56-65).  submit queues a tile
+ * request (its pixel rectangle of params, through the scene's camera or
+ * r->camera) and returns at once; a worker thread admits queued requests
+ * first come first served while the context's device memory (scenes,
+ * photon maps, queues, requests in flight) stays under the admission cap
+ * (default 200 GB), and every pass of the renderer takes one sample of
+ * every active pixel of every admitted request, so requests that arrive
+ * while others are on the GPU join at the next pass.  rgba must stay valid
+ * until poll reports the ticket done.  poll: SRAY_ERR_PENDING if not done
+ * and wait == 0; otherwise the request's status (SRAY_ERR_NOMEM if it can
+ * never fit the cap), after which the ticket is forgotten.  A request's
+ * image is the same alone and in any mix. */
 int  sray_gpu_submit(sray_gpu_ctx *ctx, const sray_request *r, float *rgba, uint64_t *ticket);
 int  sray_gpu_poll(sray_gpu_ctx *ctx, uint64_t ticket, int wait, sray_result *out);
 int  sray_gpu_set_admission_cap(sray_gpu_ctx *ctx, uint64_t bytes);

@@ -273,6 +273,39 @@ class Context:
         self._check(lib().sray_gpu_render_device(self._h, sid, C.byref(params), C.c_void_p(d_out), C.byref(st)))
         return st
 
+    # ---- serving: continuous batching behind submit / poll
+    def submit(self, sid, params, camera=None):
+        """queue a tile request; returns a ticket (the output array is kept
+        until poll returns it)"""
+        r = abi.Request()
+        r.scene_id = sid
+        r.params = params
+        if camera is not None:
+            r.camera = camera
+            r.use_camera = 1
+        w, h = params.x1 - params.x0, params.y1 - params.y0
+        out = np.zeros((h, w, 4), np.float32)
+        t = C.c_uint64()
+        self._check(lib().sray_gpu_submit(self._h, C.byref(r), out.ctypes.data_as(C.POINTER(C.c_float)),
+                                          C.byref(t)))
+        if not hasattr(self, "_outs"):
+            self._outs = {}
+        self._outs[t.value] = out
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        """(image, sray_result) when done; None if not done and wait is False"""
+        res = abi.Result()
+        rc = lib().sray_gpu_poll(self._h, ticket, 1 if wait else 0, C.byref(res))
+        if rc == abi.SRAY_ERR_PENDING and not wait:
+            return None
+        out = self._outs.pop(ticket, None) if hasattr(self, "_outs") else None
+        self._check(rc)
+        return out, res
+
+    def set_admission_cap(self, nbytes):
+        self._check(lib().sray_gpu_set_admission_cap(self._h, int(nbytes)))
+
     # ---- device memory (the library's own runtime; no torch needed)
     def malloc(self, nbytes):
         p = C.c_void_p()

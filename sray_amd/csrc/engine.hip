@@ -1,0 +1,1423 @@
+/*
+ * engine.hip — the wavefront renderer of the S-ray hot path on gfx950, for
+ * one request (sray_gpu_render) or many in flight (sray_gpu_submit/poll).
+ *
+ * Replaces render_frame / start_frame / render_block (src/render.cc:124-244),
+ * Scene::trace_ray (src/scene.cc:551-569) and the shade_phong recursion
+ * (src/shader.cc:83-283).  A request is a pixel rectangle of one resident
+ * scene seen through a camera; its pixels live in a pixel arena (per-pixel
+ * framebuffer sum, sample count, request slot).  A pass takes one sample of
+ * every active pixel of every request in flight: primary rays, then bounce
+ * levels of closest-hit trace -> shade -> shadow trace -> accumulation ->
+ * stable compaction of the children, depth first over sub-chunks.  After a
+ * pass the pixels that stop (sample budget or variance) leave the active
+ * list; a request whose pixels have all stopped is written out and its
+ * slot and arena ranges are reused.  New requests join at pass boundaries
+ * (continuous batching).  Every random draw is keyed by the request's seed,
+ * pixel, sample and path, and every pixel's contributions are summed by one
+ * thread in queue order, so a request's image does not depend on what else
+ * is in flight.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+
+#include "runtime.h"
+#include "surface.h"
+
+using namespace srd;
+using namespace srt;
+
+namespace {
+
+/* Camera::get_primary_ray, src/camera.cc:119-142 */
+__device__ __forceinline__ void primary_ray(const CamDev &C, const RenderDev &P, int x, int y, int sample, V3 &o,
+                                            V3 &d, int &time)
+{
+    float ysz = 2.0f;
+    float xsz = P.aspect * ysz;
+    float px = ((float)x / (float)P.width) * xsz - xsz / 2.0f;
+    float py = 1.0f - ((float)y / (float)P.height) * ysz;
+    uint32_t pix = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
+    if (sample > 0) {
+        float f0 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 0));
+        float f1 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 1));
+        px = px + xsz * (f0 - 0.5f) / (float)P.width;
+        py = py + ysz * (f1 - 0.5f) / (float)P.height;
+    }
+    V3 dl = v3(px * C.max_dist, py * C.max_dist, C.fz * C.max_dist);
+    time = P.frame_time;
+    if (P.mblur && sample > 0 && C.shutter > 0) {
+        uint32_t u = rng_u32(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 2));
+        time += (int)(u % (uint32_t)C.shutter) - C.shutter / 2;
+    }
+    o = xf_point(C.cam, v3(0.0f, 0.0f, 0.0f));
+    d = xf_vec(C.cam, dl);
+}
+
+__global__ void k_primary(CamDev C, RenderDev P, int sample, sray_ray *out, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x = P.x0 + (int)(i % (uint32_t)P.W), y = P.y0 + (int)(i / (uint32_t)P.W);
+    V3 o, d;
+    int time;
+    primary_ray(C, P, x, y, sample, o, d, time);
+    sray_ray r;
+    r.origin[0] = o.x;
+    r.origin[1] = o.y;
+    r.origin[2] = o.z;
+    r.dir[0] = d.x;
+    r.dir[1] = d.y;
+    r.dir[2] = d.z;
+    r.time = time;
+    r.pad_ = 0;
+    out[i] = r;
+}
+
+/* one primary ray per active pixel of the chunk [c0, c0+m), at the
+ * pixel's next sample index */
+template <bool UNI>
+__global__ void k_raygen(Tabs T, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m, QRay *__restrict__ q)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint32_t pr = active[c0 + i];
+    uint32_t rq = UNI ? 0u : T.preq[pr];
+    const ReqDev &Q = T.reqs[rq];
+    const RenderDev &P = Q.P;
+    uint32_t lp = pr - Q.pix_base;
+    int x = P.x0 + (int)(lp % (uint32_t)P.W), y = P.y0 + (int)(lp / (uint32_t)P.W);
+    int sample = T.nsamp[pr];
+    V3 o, d;
+    int time;
+    primary_ray(Q.C, P, x, y, sample, o, d, time);
+    QRay r;
+    r.o = make_float4(o.x, o.y, o.z, __int_as_float(time));
+    r.d = make_float4(d.x, d.y, d.z, 1.0f);
+    r.w = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    r.m = make_uint4(i, 1u, (uint32_t)P.iter, rq << SLOT_SHIFT);
+    r.stk = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q[i] = r;
+}
+
+/* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
+template <bool COUNT, bool UNI>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_q(Tabs T, const QRay *__restrict__ q, uint32_t n,
+                                                         int4 *__restrict__ hits, unsigned long long *counts)
+{
+    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 o = q[i].o, d = q[i].d;
+    const SceneDev S = T.scenes[UNI ? 0u : T.reqs[q[i].m.w >> SLOT_SHIFT].scene];
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    HitRec h = traverse<false, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
+                                      TRACE_BLOCK, cnt);
+    hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+    flush_counts<COUNT>(cnt, counts);
+}
+
+/* occlusion of every shadow ray (any hit: cast_ray(ray) != NULL, src/shader.cc:266) */
+template <bool COUNT, bool UNI>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq, uint32_t n,
+                                                              uint8_t *__restrict__ occ, unsigned long long *counts)
+{
+    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 d = sq[i].d;
+    if (d.w == 0.0f) {
+        occ[i] = 0;
+        return;
+    }
+    float4 o = sq[i].o;
+    const SceneDev S = T.scenes[UNI ? 0u : T.reqs[__float_as_uint(sq[i].c.w)].scene];
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
+                                     TRACE_BLOCK, cnt);
+    occ[i] = h.obj >= 0 ? 1 : 0;
+    flush_counts<COUNT>(cnt, counts);
+}
+
+/* light sample point: Light::get_point src/light.cc:106-109, SphLight
+ * :272-275, BoxLight :346-354 (matrix at time 0) */
+__device__ __forceinline__ V3 sph_rand(uint64_t seed, uint32_t pix, uint32_t sample, uint32_t path, uint32_t kind,
+                                       uint32_t i1, uint32_t i2, float r)
+{
+    /* Marsaglia (1972): uniform on the sphere from a rejection pair */
+    for (uint32_t a = 0; a < 64; a++) {
+        float u1 = rng_unit(seed, pix, sample, path, rslot(kind, i1, i2, a, 0));
+        float u2 = rng_unit(seed, pix, sample, path, rslot(kind, i1, i2, a, 1));
+        float x1 = 2.0f * u1 - 1.0f, x2 = 2.0f * u2 - 1.0f;
+        float s = x1 * x1 + x2 * x2;
+        if (s < 1.0f && s > 0.0f) {
+            float qq = 2.0f * sqrtf(1.0f - s);
+            V3 d = v3(x1 * qq, x2 * qq, 1.0f - 2.0f * s);
+            return mul(d, r);
+        }
+    }
+    return v3(0.0f, 0.0f, r);
+}
+
+__device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uint32_t pix, uint32_t sample,
+                                          uint32_t path, uint32_t li, uint32_t si)
+{
+    if (lt.type == SRAY_LIGHT_BOX) {
+        float fx = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 0));
+        float fy = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 1));
+        float fz = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 2));
+        V3 v = v3(fx * lt.dim[0] - 0.5f * lt.dim[0], fy * lt.dim[1] - 0.5f * lt.dim[1],
+                  fz * lt.dim[2] - 0.5f * lt.dim[2]);
+        return xf_point(lt.xf, v);
+    }
+    if (lt.type == SRAY_LIGHT_SPHERE) {
+        V3 s = sph_rand(seed, pix, sample, path, K_LIGHT, li, si, lt.radius);
+        return add(v3(lt.pos[0], lt.pos[1], lt.pos[2]), s);
+    }
+    return v3(lt.pos[0], lt.pos[1], lt.pos[2]);
+}
+
+/* shade_phong (src/shader.cc:83-247) / shade_undef (:36-71) for one hit,
+ * emitting direct terms, shadow rays and children */
+__device__ __forceinline__ V3 gather_est(const PmapTree &T, V3 pos, V3 n, float radius, double scale, int mode,
+                                         uint2 *stk)
+{
+    GatherAcc a;
+    gather_range(T, pos.x, pos.y, pos.z, radius, mode, (double)n.x, (double)n.y, (double)n.z, a, stk, 256);
+    if (mode == SRAY_GATHER_IRRADIANCE && a.count < 8) return v3(0.0f, 0.0f, 0.0f);
+    return v3((float)(a.r * scale), (float)(a.g * scale), (float)(a.b * scale));
+}
+
+template <bool PHOTONS, bool UNI>
+__global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restrict__ active, uint32_t c0,
+                                               const QRay *__restrict__ q, const int4 *__restrict__ hits, uint32_t n,
+                                               int L, int K, float4 *__restrict__ direct, SRay *__restrict__ sq,
+                                               QRay *__restrict__ kids, uint8_t *__restrict__ kflag,
+                                               unsigned long long *__restrict__ stats,
+                                               unsigned long long *__restrict__ slot_rays, uint32_t *__restrict__ err)
+{
+    __shared__ uint32_t bcnt[5];
+    __shared__ uint2 gstk[PHOTONS ? PMAP_STACK * 256 : 1];
+    if (threadIdx.x < 5) bcnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
+    if (i < n) {
+        QRay R = q[i];
+        int4 h = hits[i];
+        uint32_t slot = R.m.x, path = R.m.y, flg = (R.m.w >> 8) & 0xffu;
+        uint32_t rq = UNI ? 0u : R.m.w >> SLOT_SHIFT;
+        const ReqDev &Q = T.reqs[rq];
+        const RenderDev &P = Q.P;
+        const ShadePmaps &PM = Q.PM;
+        const SceneDev S = T.scenes[UNI ? 0u : Q.scene];
+        uint32_t pr = active[c0 + slot];
+        uint32_t lp = pr - Q.pix_base;
+        int sample = T.nsamp[pr];
+        int iter = (int)R.m.z;
+        V3 W = v3(R.w.x, R.w.y, R.w.z);
+        float energy = R.d.w;
+        int time = __float_as_int(R.o.w);
+        V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
+        float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int l = 0; l < L; l++) sq[(size_t)i * L + l].d.w = 0.0f;
+        for (int k = 0; k < K; k++) kflag[(size_t)i * K + k] = 0;
+        if (h.y < 0) {
+            /* miss: env_color (src/scene.cc:568) */
+            dir.x = W.x * S.env_color.x;
+            dir.y = W.y * S.env_color.y;
+            dir.z = W.z * S.env_color.z;
+            if (path == 1u) dir.w = S.env_color.w;
+            direct[i] = dir;
+        } else {
+            float t = __int_as_float(h.x);
+            const DevObj &ob = S.objs[h.y];
+            const DevMat &mat = S.mats[ob.mat];
+            bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
+            V3 Pw, N;
+            float tu, tv;
+            hit_surface(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong, Pw, N, tu, tv);
+            if (path == 1u) dir.w = 1.0f;
+            uint32_t pix = (uint32_t)(P.y0 + (int)(lp / (uint32_t)P.W)) * (uint32_t)P.width + (uint32_t)(P.x0 + (int)(lp % (uint32_t)P.W));
+            int nl = S.nlights;
+            if (!phong) {
+                /* shade_undef: checker texture from texcoords */
+                int cx = (int)(tu * 20.0f), cy = (int)(tv * 10.0f);
+                V3 tex = ((cx % 2) == (cy % 2)) ? v3(1.0f, 0.3f, 0.2f) : v3(0.2f, 0.3f, 1.0f);
+                dir.x = W.x * (0.1f * tex.x);
+                dir.y = W.y * (0.1f * tex.y);
+                dir.z = W.z * (0.1f * tex.z);
+                for (int l = 0; l < nl; l++) {
+                    const DevLight &lt = S.lights[l];
+                    V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u);
+                    V3 ldir = sub(lpos, Pw);
+                    float ldist = length(ldir);
+                    V3 sd = ldir;
+                    ldir = dvs(ldir, ldist);
+                    V3 vdir = neg(normalized(rd));
+                    V3 hdir = normalized(add(ldir, vdir));
+                    float ndotl = fmaxf(dot(N, ldir), 0.0f);
+                    float ndoth = fmaxf(dot(N, hdir), 0.0f);
+                    float sp = powf(ndoth, 50.0f);
+                    SRay s;
+                    s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
+                    s.d = make_float4(sd.x, sd.y, sd.z, 1.0f);
+                    s.c = make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
+                                      W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
+                                      W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp), __uint_as_float(rq));
+                    sq[(size_t)i * L + l] = s;
+                    c_sh++;
+                }
+                direct[i] = dir;
+            } else {
+                bool entering;
+                V3 normal;
+                if (dot(rd, N) > 0.0f) {
+                    normal = neg(N);
+                    entering = false;
+                } else {
+                    normal = N;
+                    entering = true;
+                }
+                float ray_mag = length(rd);
+                V3 incident = dvs(rd, ray_mag);
+                float4 kd = mat.kd;
+                float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                if (flg & QF_DIFFUSE) {
+                    /* diffuse GI ray: radiance estimate from the global map
+                     * (src/shader.cc:119-123) */
+                    V3 rad = PHOTONS ? gather_est(PM.glob, Pw, normal, PM.radius, PM.scale, SRAY_GATHER_RADIANCE,
+                                                  &gstk[threadIdx.x])
+                                     : v3(0.0f, 0.0f, 0.0f);
+                    dir.x = W.x * (rad.x * kd.x);
+                    dir.y = W.y * (rad.y * kd.y);
+                    dir.z = W.z * (rad.z * kd.z);
+                    direct[i] = dir;
+                } else {
+                    float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
+                    float trans_fact = (mat.flags & SRAY_MAT_HAS_REFRACT) ? mat.refract : 0.0f;
+                    float mat_ior = mat.ior;
+                    IorStack st;
+                    st.cur = R.w.w;
+                    st.s[0] = R.stk.x;
+                    st.s[1] = R.stk.y;
+                    st.s[2] = R.stk.z;
+                    st.s[3] = R.stk.w;
+                    st.n = R.m.w & 0xffu;
+                    st.overflow = false;
+                    float ior = st.calc(entering, mat_ior);
+                    /* ambient: (env_ambient * kd) * kd, src/shader.cc:142,246 */
+                    dir.x = W.x * ((S.env_ambient.x * kd.x) * kd.x);
+                    dir.y = W.y * ((S.env_ambient.y * kd.y) * kd.y);
+                    dir.z = W.z * ((S.env_ambient.z * kd.z) * kd.z);
+                    int slotk = 0;
+                    V3 vref = reflect(incident, normal);
+                    for (int l = 0; l < nl; l++) {
+                        const DevLight &lt = S.lights[l];
+                        int ns = lt.type != SRAY_LIGHT_POINT ? P.shadow_samples : 1;
+                        float scale = ns > 1 ? 1.0f / (float)ns : 1.0f;
+                        for (int si = 0; si < ns; si++) {
+                            V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si);
+                            V3 ldir = sub(lpos, Pw);
+                            float ldist = length(ldir);
+                            V3 sd = ldir;
+                            ldir = dvs(ldir, ldist);
+                            float ndotl = fmaxf(dot(normal, ldir), 0.0f);
+                            float rdotl = fmaxf(dot(vref, ldir), 0.0f);
+                            float sp = powf(rdotl, mat.shininess);
+                            SRay s;
+                            s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
+                            s.d = make_float4(sd.x, sd.y, sd.z, 1.0f);
+                            s.c = make_float4(
+                                W.x * ((kd.x * (lt.color[0] * ndotl) + ks.x * (lt.color[0] * sp)) * scale),
+                                W.y * ((kd.y * (lt.color[1] * ndotl) + ks.y * (lt.color[1] * sp)) * scale),
+                                W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale), __uint_as_float(rq));
+                            sq[(size_t)i * L + slotk] = s;
+                            slotk++;
+                            c_sh++;
+                        }
+                    }
+                    if (PHOTONS && P.gi) {
+                        /* diffuse hemisphere rays (src/shader.cc:158-185); a
+                         * direction flipped into the hemisphere keeps its
+                         * negative ndotl and so spawns no ray */
+                        float dsc = P.diffuse_samples > 1 ? 1.0f / (float)P.diffuse_samples : 1.0f;
+                        for (int di = 0; di < P.diffuse_samples; di++) {
+                            V3 gd = sph_rand(P.seed, pix, (uint32_t)sample, path, K_DIFFUSE, (uint32_t)di, 0u, 1.0f);
+                            float ndotl = dot(gd, normal);
+                            if (ndotl < 0.0f) gd = neg(gd);
+                            float diff_energy = ndotl * energy;
+                            if (diff_energy > P.min_energy) {
+                                QRay cr;
+                                V3 dd = mul(gd, ray_mag);
+                                cr.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                                cr.d = make_float4(dd.x, dd.y, dd.z, diff_energy);
+                                float wf = ndotl * dsc;
+                                cr.w = make_float4(W.x * (kd.x * wf), W.y * (kd.y * wf), W.z * (kd.z * wf), R.w.w);
+                                cr.m = make_uint4(slot, path | 0x40000000u | ((uint32_t)di << 20), 0x7fffffffu,
+                                                  (R.m.w & 0xffu) | (QF_DIFFUSE << 8) | (rq << SLOT_SHIFT));
+                                cr.stk = R.stk;
+                                kids[(size_t)i * K + 2 + di] = cr;
+                                kflag[(size_t)i * K + 2 + di] = 1;
+                                c_df++;
+                            }
+                        }
+                    }
+                    if (PHOTONS && P.caust) {
+                        /* caustics: irradiance from the caustic map (src/shader.cc:231-236) */
+                        V3 irr = gather_est(PM.caust, Pw, normal, PM.radius, PM.scale, SRAY_GATHER_IRRADIANCE,
+                                            &gstk[threadIdx.x]);
+                        dir.x = dir.x + W.x * irr.x;
+                        dir.y = dir.y + W.y * irr.y;
+                        dir.z = dir.z + W.z * irr.z;
+                    }
+                    V3 nrd = dvs(neg(rd), ray_mag);
+                    float ray_dot_n = dot(nrd, normal);
+                    float sf0 = trans_fact > 0.0f ? (ior - 1.0f) / (ior + 1.0f) : 1.0f;
+                    float fres = fresnel(sf0 * sf0, ray_dot_n);
+                    float refl_energy = refl_fact * energy;
+                    if (refl_energy > P.min_energy && iter > 0) {
+                        V3 nd = reflect(rd, normal);
+                        QRay c;
+                        c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                        c.d = make_float4(nd.x, nd.y, nd.z, refl_energy);
+                        float wf = fres * refl_fact;
+                        c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), R.w.w);
+                        c.m = make_uint4(slot, path * 2u, (uint32_t)(iter - 1), R.m.w);
+                        c.stk = R.stk;
+                        kids[(size_t)i * K + 0] = c;
+                        kflag[(size_t)i * K + 0] = 1;
+                        c_rl++;
+                    }
+                    float trans_energy = trans_fact * energy;
+                    if (trans_energy > P.min_energy && iter > 0) {
+                        IorStack ts = st;
+                        if (entering) ts.enter(mat_ior);
+                        else ts.leave();
+                        V3 td = mul(refract(dvs(rd, ray_mag), normal, ior), ray_mag);
+                        if (dot(td, normal) > 0.0f) {
+                            if (entering) ts.leave();
+                            else ts.enter(mat_ior);
+                        }
+                        if (ts.overflow) atomicOr(err, 1u);
+                        QRay c;
+                        c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                        c.d = make_float4(td.x, td.y, td.z, trans_energy);
+                        float wf = (1.0f - fres) * trans_fact;
+                        c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), ts.cur);
+                        c.m = make_uint4(slot, path * 2u + 1u, (uint32_t)(iter - 1), ts.n | (flg << 8) | (rq << SLOT_SHIFT));
+                        c.stk = make_float4(ts.s[0], ts.s[1], ts.s[2], ts.s[3]);
+                        kids[(size_t)i * K + 1] = c;
+                        kflag[(size_t)i * K + 1] = 1;
+                        c_rr++;
+                    }
+                    direct[i] = dir;
+                }
+            }
+        }
+    }
+    if (!UNI && i < n) {
+        uint32_t tot = c_sh + c_rl + c_rr + c_df;
+        if (tot) atomicAdd(&slot_rays[q[i].m.w >> SLOT_SHIFT], (unsigned long long)tot);
+    }
+    if (c_sh) atomicAdd(&bcnt[0], c_sh);
+    if (c_rl) atomicAdd(&bcnt[1], c_rl);
+    if (c_rr) atomicAdd(&bcnt[2], c_rr);
+    if (c_df) atomicAdd(&bcnt[3], c_df);
+    __syncthreads();
+    if (threadIdx.x < 4 && bcnt[threadIdx.x]) atomicAdd(&stats[1 + threadIdx.x], (unsigned long long)bcnt[threadIdx.x]);
+}
+
+/* per-sample accumulation: the first ray of each run of equal sample slots
+ * sums the run (queues are ordered by slot) */
+__global__ void k_accum(const QRay *__restrict__ q, const float4 *__restrict__ direct, const SRay *__restrict__ sq,
+                        const uint8_t *__restrict__ occ, uint32_t n, int L, float4 *__restrict__ acc)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t slot = q[i].m.x;
+    if (i > 0 && q[i - 1].m.x == slot) return;
+    float4 s = acc[slot];
+    for (uint32_t j = i; j < n && q[j].m.x == slot; j++) {
+        float4 d = direct[j];
+        s.x = s.x + d.x;
+        s.y = s.y + d.y;
+        s.z = s.z + d.z;
+        s.w = s.w + d.w;
+        for (int l = 0; l < L; l++) {
+            size_t k = (size_t)j * L + l;
+            if (sq[k].d.w != 0.0f && !occ[k]) {
+                float4 c = sq[k].c;
+                s.x = s.x + c.x;
+                s.y = s.y + c.y;
+                s.z = s.z + c.z;
+            }
+        }
+    }
+    acc[slot] = s;
+}
+
+__global__ void k_fill_f4(float4 *p, size_t n, float4 v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_iota(uint32_t *p, uint32_t n, uint32_t base)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = base + i;
+}
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+/* active pixels per request slot */
+__global__ void k_slot_count(const uint32_t *__restrict__ active, uint32_t n, const uint32_t *__restrict__ preq,
+                             uint32_t *__restrict__ cnt)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[preq[active[i]]], 1u);
+}
+
+/* end of one sample pass: framebuffer accumulation in float
+ * (src/render.cc:226-229) and the variance stop test (:232-237, 261-270) */
+template <bool UNI>
+__global__ void k_pass_finish(Tabs T, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m,
+                              const float4 *__restrict__ acc, float4 *__restrict__ fb, float4 *__restrict__ subpix,
+                              int *__restrict__ nsamp, uint8_t *__restrict__ keep)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint32_t pr = active[c0 + i];
+    uint32_t rq = UNI ? 0u : T.preq[pr];
+    const ReqDev &Q = T.reqs[rq];
+    const RenderDev &P = Q.P;
+    int sample = nsamp[pr];
+    float4 c = acc[i];
+    float4 f = fb[pr];
+    f.x = f.x + c.x;
+    f.y = f.y + c.y;
+    f.z = f.z + c.z;
+    f.w = f.w + c.w;
+    fb[pr] = f;
+    int ns = sample + 1;
+    nsamp[pr] = ns;
+    int maxs = P.max_samples;
+    bool k = ns < maxs;
+    if (maxs > 1) {
+        float4 *sp = subpix + Q.sub_base + (size_t)(pr - Q.pix_base) * (size_t)maxs;
+        sp[sample] = c;
+        if (ns >= P.min_samples && ns > 1) {
+            float rcp = 1.0f / (float)ns;
+            float4 mean = make_float4(f.x * rcp, f.y * rcp, f.z * rcp, f.w * rcp);
+            float sq = 0.0f;
+            for (int j = 0; j < ns; j++) {
+                float4 s = sp[j];
+                float dx = mean.x - s.x, dy = mean.y - s.y, dz = mean.z - s.z, dw = mean.w - s.w;
+                float l2 = dx * dx + dy * dy + dz * dz + dw * dw;
+                sq = sq + l2;
+            }
+            float v = sq * rcp;
+            if (v < P.max_var) k = false;
+        }
+    }
+    keep[c0 + i] = k ? 1 : 0;
+}
+
+/* final pixel value: divide all channels by the sample count, or the
+ * reference's as-written img[x] *= rcp_lut[i] (src/render.cc:240, Q1) */
+__global__ void k_finalize(RenderDev P, const float4 *__restrict__ fb, const int *__restrict__ nsamp,
+                           float *__restrict__ out, uint32_t npix)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    float4 f = fb[i];
+    if (P.flags & SRAY_RENDER_REF_Q1) {
+        out[(size_t)i * 4 + 0] = f.x;
+        out[(size_t)i * 4 + 1] = f.y;
+        out[(size_t)i * 4 + 2] = f.z;
+        out[(size_t)i * 4 + 3] = f.w;
+        return;
+    }
+    float rcp = 1.0f / (float)nsamp[i];
+    out[(size_t)i * 4 + 0] = f.x * rcp;
+    out[(size_t)i * 4 + 1] = f.y * rcp;
+    out[(size_t)i * 4 + 2] = f.z * rcp;
+    out[(size_t)i * 4 + 3] = f.w * rcp;
+}
+
+__global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__restrict__ out, uint32_t npix)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    int xr = (int)(i % (uint32_t)P.W), yr = (int)(i / (uint32_t)P.W);
+    int ax = P.x0 + xr;
+    int tx = (ax / 48) * 48;
+    int org = tx > P.x0 ? tx : P.x0; /* tile origin clipped to the rectangle */
+    int f = ax - org;                /* pixel offset within the tile row */
+    size_t fi = ((size_t)yr * P.W + (size_t)(org - P.x0)) * 4 + (size_t)f;
+    float rcp = 1.0f / (float)nsamp[i];
+    out[fi] = out[fi] * rcp;
+}
+
+
+} // namespace
+
+/* ================================================================== */
+/* host side                                                            */
+/* ================================================================== */
+namespace srt {
+
+constexpr uint32_t CHUNK = 1u << 21; /* rays per level sub-chunk */
+using Clock = std::chrono::steady_clock;
+
+void fill_render_dev(const sray_render_params *p, RenderDev &R)
+{
+    memset(&R, 0, sizeof R);
+    R.width = p->width;
+    R.height = p->height;
+    R.x0 = p->x0;
+    R.y0 = p->y0;
+    R.x1 = p->x1;
+    R.y1 = p->y1;
+    R.W = p->x1 - p->x0;
+    R.min_samples = p->min_samples;
+    R.max_samples = std::max(1, p->max_samples);
+    R.max_var = p->max_var;
+    R.min_energy = p->min_energy;
+    R.iter = p->iter;
+    R.shadow_samples = std::max(1, p->shadow_samples);
+    R.diffuse_samples = std::max(0, p->diffuse_samples);
+    R.frame_time = p->frame_time;
+    R.mblur = p->mblur;
+    R.caust = p->caust_photons;
+    R.gi = p->gi_photons;
+    R.flags = p->flags;
+    R.seed = p->seed;
+    R.aspect = (float)((double)p->width / (double)p->height);
+}
+
+int validate_params(sray_gpu_ctx *c, const sray_render_params *p)
+{
+    if (!p) return fail(c, SRAY_ERR_ARG, "render params is NULL");
+    if (p->width <= 0 || p->height <= 0) return fail(c, SRAY_ERR_ARG, "bad image size %dx%d", p->width, p->height);
+    if (p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height || p->x0 >= p->x1 || p->y0 >= p->y1)
+        return fail(c, SRAY_ERR_ARG, "bad pixel rectangle");
+    if (p->max_samples > 256) return fail(c, SRAY_ERR_LIMIT, "max_samples > 256");
+    if (p->iter < 0 || p->iter > 30) return fail(c, SRAY_ERR_LIMIT, "iter outside [0,30]");
+    if (p->shadow_samples > 1024) return fail(c, SRAY_ERR_LIMIT, "shadow_samples > 1024");
+    if (p->diffuse_samples > 1023 || p->diffuse_samples < 0) return fail(c, SRAY_ERR_LIMIT, "diffuse_samples outside [0,1023]");
+    if ((p->gi_photons || p->caust_photons) && !(p->gather_dist > 0.0f))
+        return fail(c, SRAY_ERR_ARG, "gather_dist must be > 0 with photon maps");
+    return SRAY_OK;
+}
+
+
+/* first-fit allocator of [base, base+len) ranges of an arena */
+struct RangeAlloc {
+    uint64_t cap = 0;
+    std::map<uint64_t, uint64_t> fr; /* free ranges: base -> len */
+    bool alloc(uint64_t n, uint64_t &base)
+    {
+        for (auto it = fr.begin(); it != fr.end(); ++it) {
+            if (it->second < n) continue;
+            base = it->first;
+            uint64_t len = it->second;
+            fr.erase(it);
+            if (len > n) fr[base + n] = len - n;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t base, uint64_t n)
+    {
+        if (n == 0) return;
+        auto it = fr.emplace(base, n).first;
+        auto nx = std::next(it);
+        if (nx != fr.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            fr.erase(nx);
+        }
+        if (it != fr.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                fr.erase(it);
+            }
+        }
+    }
+    void grow(uint64_t ncap)
+    {
+        if (ncap > cap) release(cap, ncap - cap);
+        cap = ncap;
+    }
+    /* largest free range touching the end of the arena */
+    uint64_t tail_free() const
+    {
+        if (fr.empty()) return 0;
+        auto it = std::prev(fr.end());
+        return it->first + it->second == cap ? it->second : 0;
+    }
+};
+
+struct Job {
+    uint64_t ticket = 0;
+    uint32_t scene_id = 0;
+    sray_render_params p{};
+    sray_camera_desc cam{};
+    bool use_cam = false;
+    float *host_out = nullptr; /* result to host memory, or */
+    float *dev_out = nullptr;  /* to device memory */
+    uint32_t slot = 0;
+    uint64_t pix_base = 0, npix = 0, sub_base = 0, nsub = 0;
+    uint64_t rays = 0;
+    int state = 0; /* 0 waiting, 1 in flight, 2 done */
+    int rc = SRAY_OK;
+    std::string err;
+    bool limit_hit = false;
+    Clock::time_point t_submit, t_admit;
+    double queue_ms = 0, latency_ms = 0;
+};
+
+struct Engine {
+    sray_gpu_ctx *c = nullptr;
+    /* pixel arena */
+    RangeAlloc pix, sub;
+    DevBuf fb, nsamp, preq, keep, act[2], subpix;
+    int cur = 0;
+    uint32_t nact = 0;
+    /* ray queues and per-level scratch */
+    std::vector<DevBuf> qlev;
+    DevBuf hits, direct, sq, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs, dscenes, stage;
+    /* requests in flight */
+    std::vector<ReqDev> hreqs = std::vector<ReqDev>(MAX_SLOTS);
+    std::vector<SceneDev> hscenes;
+    Job *slot_job[MAX_SLOTS] = {};
+    uint32_t slot_active[MAX_SLOTS] = {};
+    uint32_t live = 0, hi = 0; /* slots in use, 1 + highest slot in use */
+    /* this pass's launch shape */
+    int L = 0, K = 2, maxiter = 0;
+    bool photons = false, uni = true, counting = false;
+    /* submit / poll */
+    std::thread worker;
+    std::mutex qmu;
+    std::condition_variable qcv;
+    std::deque<Job *> pending;
+    std::map<uint64_t, std::unique_ptr<Job>> jobs;
+    uint64_t next_ticket = 1;
+    bool stop = false, started = false;
+
+    uint64_t live_bytes = 0; /* footprints of the requests in flight */
+
+    /* everything held on the device */
+    uint64_t bytes() const { return fixed_bytes() + arena_bytes(); }
+    /* ray queues and tables (sized by the largest pass so far) */
+    uint64_t fixed_bytes() const
+    {
+        uint64_t b = 0;
+        const DevBuf *bs[] = {&hits, &direct, &sq, &occ, &kids, &kflag, &acc, &stats, &errf, &rcounts,
+                              &slotcnt, &slotrays, &dreqs, &dscenes, &stage};
+        for (const DevBuf *d : bs) b += d->bytes;
+        for (const DevBuf &d : qlev) b += d.bytes;
+        return b;
+    }
+    /* the pixel and sample arenas */
+    uint64_t arena_bytes() const
+    {
+        uint64_t b = 0;
+        const DevBuf *bs[] = {&fb, &nsamp, &preq, &keep, &act[0], &act[1], &subpix};
+        for (const DevBuf *d : bs) b += d->bytes;
+        return b;
+    }
+};
+
+/* grows b to bytes keeping its first keep bytes */
+int regrow(sray_gpu_ctx *c, DevBuf &b, size_t bytes, size_t keep)
+{
+    if (b.bytes >= bytes) return SRAY_OK;
+    DevBuf n;
+    hipError_t e = hipMalloc(&n.p, bytes);
+    if (e != hipSuccess) return fail(c, SRAY_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    n.bytes = bytes;
+    if (b.p && keep) HIPCHK(c, hipMemcpyAsync(n.p, b.p, std::min(keep, b.bytes), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    release(b);
+    b = n;
+    return SRAY_OK;
+}
+
+int grow_pixels(Engine &E, uint64_t need)
+{
+    sray_gpu_ctx *c = E.c;
+    uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(E.pix.cap * 2, 1u << 20), E.pix.cap + need);
+    if (cap > 0xfffff000ull) return fail(c, SRAY_ERR_LIMIT, "pixel arena beyond 2^32 pixels");
+    uint64_t old = E.pix.cap;
+    int rc;
+    if ((rc = regrow(c, E.fb, cap * 16, old * 16)) || (rc = regrow(c, E.nsamp, cap * 4, old * 4)) ||
+        (rc = regrow(c, E.preq, cap * 4, old * 4)) || (rc = regrow(c, E.keep, cap, 0)) ||
+        (rc = regrow(c, E.act[E.cur], cap * 4, (size_t)E.nact * 4)) || (rc = regrow(c, E.act[E.cur ^ 1], cap * 4, 0)))
+        return rc;
+    E.pix.grow(cap);
+    return SRAY_OK;
+}
+
+int grow_sub(Engine &E, uint64_t need)
+{
+    uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(E.sub.cap * 2, 1u << 20), E.sub.cap + need);
+    int rc = regrow(E.c, E.subpix, cap * 16, E.sub.cap * 16);
+    if (rc) return rc;
+    E.sub.grow(cap);
+    return SRAY_OK;
+}
+
+CamDev cam_dev(const sray_camera_desc &d)
+{
+    CamDev C;
+    memset(&C, 0, sizeof C);
+    memcpy(C.cam, d.xform, sizeof C.cam);
+    C.fz = (float)(1.0 / std::tan(0.5 * d.vfov));
+    C.max_dist = d.max_dist;
+    C.shutter = d.shutter;
+    return C;
+}
+
+/* device bytes a request needs while in flight (admission accounting) */
+uint64_t job_bytes(const Job &j)
+{
+    uint64_t npix = (uint64_t)(j.p.x1 - j.p.x0) * (uint64_t)(j.p.y1 - j.p.y0);
+    uint64_t maxs = (uint64_t)std::max(1, j.p.max_samples);
+    return npix * (16 + 4 + 4 + 1 + 8 + 16) + (maxs > 1 ? npix * maxs * 16 : 0);
+}
+
+/* what admission counts as taken: resident scenes and photon maps, the
+ * engines' queues, and the footprints of the requests in flight */
+uint64_t admitted_bytes(sray_gpu_ctx *c)
+{
+    uint64_t b = 0;
+    for (auto &kv : c->scenes) b += kv.second->bytes;
+    for (auto &kv : c->pmaps) b += kv.second->bytes;
+    if (c->plain) b += c->plain->bytes();
+    if (c->serve) b += c->serve->fixed_bytes() + c->serve->live_bytes;
+    return b;
+}
+
+void job_done(Engine &E, Job *j, int rc, const char *msg)
+{
+    std::lock_guard<std::mutex> lk(E.qmu);
+    if (rc) {
+        j->rc = rc;
+        if (msg) j->err = msg;
+    }
+    j->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - j->t_submit).count();
+    j->state = 2;
+    E.qcv.notify_all();
+}
+
+/* places a request in the arena and the active list */
+int admit(Engine &E, Job *j)
+{
+    sray_gpu_ctx *c = E.c;
+    int rc = validate_params(c, &j->p);
+    if (rc) return rc;
+    GpuScene *g = find_scene(c, j->scene_id);
+    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", j->scene_id);
+    uint32_t s = 0;
+    while (s < MAX_SLOTS && E.slot_job[s]) s++;
+    if (s == MAX_SLOTS) return fail(c, SRAY_ERR_LIMIT, "more than %u requests in flight", MAX_SLOTS);
+    ReqDev Q;
+    memset(&Q, 0, sizeof Q);
+    fill_render_dev(&j->p, Q.P);
+    Q.C = cam_dev(j->use_cam ? j->cam : g->camera);
+    for (int kind = 0; kind < 2; kind++) {
+        bool want = kind == SRAY_PMAP_CAUSTIC ? Q.P.caust != 0 : Q.P.gi != 0;
+        if (!want) continue;
+        auto it = c->pmaps.find(g->pmap[kind]);
+        if (it == c->pmaps.end())
+            return fail(c, SRAY_ERR_NOTFOUND, "render asks for the %s photon map but none is bound to scene %u",
+                        kind == SRAY_PMAP_CAUSTIC ? "caustic" : "global", j->scene_id);
+        (kind == SRAY_PMAP_CAUSTIC ? Q.PM.caust : Q.PM.glob) = it->second->tree;
+    }
+    /* gather distance relative to the scene octree diagonal (src/scene.cc:384-387) */
+    double gabs = g->diag * (double)j->p.gather_dist;
+    Q.PM.radius = (float)gabs;
+    Q.PM.scale = gabs > 0.0 ? 1.0 / (M_PI * (gabs * gabs)) : 0.0;
+    uint64_t npix = (uint64_t)Q.P.W * (uint64_t)(Q.P.y1 - Q.P.y0);
+    uint64_t maxs = (uint64_t)Q.P.max_samples;
+    uint64_t base = 0, sbase = 0;
+    if (!E.pix.alloc(npix, base)) {
+        if ((rc = grow_pixels(E, npix))) return rc;
+        if (!E.pix.alloc(npix, base)) return fail(c, SRAY_ERR_NOMEM, "pixel arena");
+    }
+    if (maxs > 1 && !E.sub.alloc(npix * maxs, sbase)) {
+        if ((rc = grow_sub(E, npix * maxs)) || !E.sub.alloc(npix * maxs, sbase)) {
+            E.pix.release(base, npix);
+            return rc ? rc : fail(c, SRAY_ERR_NOMEM, "sample arena");
+        }
+    }
+    Q.pix_base = (uint32_t)base;
+    Q.sub_base = sbase;
+    k_fill_f4<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<float4>(E.fb) + base, npix, make_float4(0, 0, 0, 0));
+    k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.nsamp) + base, npix, 0u);
+    k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.preq) + base, npix, s);
+    k_iota<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.act[E.cur]) + E.nact, (uint32_t)npix,
+                                                     (uint32_t)base);
+    HIPCHK(c, hipGetLastError());
+    E.nact += (uint32_t)npix;
+    E.hreqs[s] = Q;
+    E.slot_job[s] = j;
+    E.slot_active[s] = (uint32_t)npix;
+    E.live++;
+    E.live_bytes += job_bytes(*j);
+    E.hi = std::max(E.hi, s + 1);
+    j->slot = s;
+    j->pix_base = base;
+    j->npix = npix;
+    j->sub_base = sbase;
+    j->nsub = maxs > 1 ? npix * maxs : 0;
+    j->state = 1;
+    j->t_admit = Clock::now();
+    j->queue_ms = std::chrono::duration<double, std::milli>(j->t_admit - j->t_submit).count();
+    return SRAY_OK;
+}
+
+void retire(Engine &E, uint32_t s)
+{
+    Job *j = E.slot_job[s];
+    E.pix.release(j->pix_base, j->npix);
+    if (j->nsub) E.sub.release(j->sub_base, j->nsub);
+    E.slot_job[s] = nullptr;
+    E.slot_active[s] = 0;
+    E.live--;
+    E.live_bytes -= job_bytes(*j);
+    while (E.hi > 0 && !E.slot_job[E.hi - 1]) E.hi--;
+}
+
+/* drops every request in flight (after a device error) */
+void abort_all(Engine &E)
+{
+    for (uint32_t s = 0; s < E.hi; s++)
+        if (E.slot_job[s]) retire(E, s);
+    E.nact = 0;
+}
+
+int build_tables(Engine &E)
+{
+    sray_gpu_ctx *c = E.c;
+    E.hscenes.clear();
+    std::map<uint32_t, uint32_t> idx;
+    E.L = 0;
+    E.K = 2;
+    E.maxiter = 0;
+    E.photons = false;
+    for (uint32_t s = 0; s < E.hi; s++) {
+        Job *j = E.slot_job[s];
+        if (!j) continue;
+        GpuScene *g = find_scene(c, j->scene_id);
+        if (!g) return fail(c, SRAY_ERR_NOTFOUND, "scene %u released with requests in flight", j->scene_id);
+        auto it = idx.find(j->scene_id);
+        if (it == idx.end()) {
+            it = idx.emplace(j->scene_id, (uint32_t)E.hscenes.size()).first;
+            E.hscenes.push_back(g->dev);
+        }
+        ReqDev &Q = E.hreqs[s];
+        Q.scene = it->second;
+        /* shadow slots per ray: phong samples area lights shadow_samples times */
+        E.L = std::max(E.L, g->dev.nlights * Q.P.shadow_samples);
+        E.K = std::max(E.K, 2 + (Q.P.gi ? Q.P.diffuse_samples : 0));
+        E.maxiter = std::max(E.maxiter, Q.P.iter);
+        E.photons = E.photons || Q.P.gi || Q.P.caust;
+    }
+    E.uni = E.live == 1 && E.slot_job[0] != nullptr;
+    int rc;
+    if ((rc = ensure(c, E.dreqs, (size_t)E.hi * sizeof(ReqDev))) ||
+        (rc = ensure(c, E.dscenes, E.hscenes.size() * sizeof(SceneDev))))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(E.dreqs.p, E.hreqs.data(), (size_t)E.hi * sizeof(ReqDev), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(E.dscenes.p, E.hscenes.data(), E.hscenes.size() * sizeof(SceneDev),
+                             hipMemcpyHostToDevice, c->stream));
+    return SRAY_OK;
+}
+
+struct Level {
+    Engine *E;
+    Tabs T;
+    const uint32_t *active;
+    uint32_t c0;
+};
+
+/* one bounce level over qlev[b][0,count): sub-chunks of CHUNK rays, each
+ * followed depth-first by its children's levels */
+int process_level(Level &LV, int b, uint32_t count)
+{
+    Engine &E = *LV.E;
+    sray_gpu_ctx *c = E.c;
+    int rc;
+    QRay *Q = dptr<QRay>(E.qlev[b]);
+    unsigned long long *rc6 = dptr<unsigned long long>(E.rcounts);
+    unsigned long long *stats = dptr<unsigned long long>(E.stats);
+    unsigned long long *srays = dptr<unsigned long long>(E.slotrays);
+    for (uint32_t off = 0; off < count; off += CHUNK) {
+        uint32_t m = std::min(CHUNK, count - off);
+        const QRay *q = Q + off;
+        {
+            ProfScope ps(c, "k_trace_q");
+            auto kern = E.counting ? (E.uni ? k_trace_q<true, true> : k_trace_q<true, false>)
+                                   : (E.uni ? k_trace_q<false, true> : k_trace_q<false, false>);
+            kern<<<blocks(m, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(LV.T, q, m, dptr<int4>(E.hits), rc6);
+        }
+        {
+            ProfScope ps(c, "k_shade");
+            auto kern = E.photons ? (E.uni ? k_shade<true, true> : k_shade<true, false>)
+                                  : (E.uni ? k_shade<false, true> : k_shade<false, false>);
+            kern<<<blocks(m, 256), 256, 0, c->stream>>>(LV.T, LV.active, LV.c0, q, dptr<int4>(E.hits), m, E.L, E.K,
+                                                        dptr<float4>(E.direct), dptr<SRay>(E.sq),
+                                                        dptr<QRay>(E.kids), dptr<uint8_t>(E.kflag), stats, srays,
+                                                        dptr<uint32_t>(E.errf));
+        }
+        size_t ns = (size_t)m * E.L;
+        if (ns) {
+            ProfScope ps(c, "k_trace_shadow");
+            auto kern = E.counting ? (E.uni ? k_trace_shadow<true, true> : k_trace_shadow<true, false>)
+                                   : (E.uni ? k_trace_shadow<false, true> : k_trace_shadow<false, false>);
+            kern<<<blocks(ns, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), (uint32_t)ns,
+                                                                         dptr<uint8_t>(E.occ), rc6 + 6);
+        }
+        {
+            ProfScope ps(c, "k_accum");
+            k_accum<<<blocks(m, 256), 256, 0, c->stream>>>(q, dptr<float4>(E.direct), dptr<SRay>(E.sq),
+                                                          dptr<uint8_t>(E.occ), m, E.L, dptr<float4>(E.acc));
+        }
+        if (b + 1 > E.maxiter) continue;
+        uint32_t nk = 0;
+        if ((rc = select_flagged(c, dptr<QRay>(E.kids), dptr<uint8_t>(E.kflag), dptr<QRay>(E.qlev[b + 1]),
+                                 m * (uint32_t)E.K, &nk)))
+            return rc;
+        HIPCHK(c, hipGetLastError());
+        if (nk) {
+            if ((rc = process_level(LV, b + 1, nk))) return rc;
+        }
+    }
+    return SRAY_OK;
+}
+
+/* one sample of every active pixel; retires the requests that finish */
+int run_pass(Engine &E, std::vector<Job *> &finished)
+{
+    sray_gpu_ctx *c = E.c;
+    int rc;
+    if ((rc = build_tables(E))) return rc;
+    uint32_t levels = (uint32_t)E.maxiter + 1;
+    if (E.qlev.size() < levels) E.qlev.resize(levels);
+    for (uint32_t b = 0; b < levels; b++)
+        if ((rc = ensure(c, E.qlev[b], (size_t)CHUNK * E.K * sizeof(QRay)))) return rc;
+    if ((rc = ensure(c, E.hits, (size_t)CHUNK * sizeof(int4))) || (rc = ensure(c, E.direct, (size_t)CHUNK * 16)) ||
+        (rc = ensure(c, E.sq, (size_t)CHUNK * std::max(E.L, 1) * sizeof(SRay))) ||
+        (rc = ensure(c, E.occ, (size_t)CHUNK * std::max(E.L, 1))) ||
+        (rc = ensure(c, E.kids, (size_t)CHUNK * E.K * sizeof(QRay))) || (rc = ensure(c, E.kflag, (size_t)CHUNK * E.K)) ||
+        (rc = ensure(c, E.acc, (size_t)CHUNK * 16)) || (rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) ||
+        (rc = ensure(c, E.slotrays, MAX_SLOTS * 8)) || (rc = ensure(c, c->nsel, 4)))
+        return rc;
+    if (E.sub.cap == 0 && (rc = ensure(c, E.subpix, 16))) return rc;
+    if ((rc = ensure(c, E.errf, 4)) || (rc = ensure(c, E.stats, 8 * sizeof(unsigned long long))) ||
+        (rc = ensure(c, E.rcounts, 12 * sizeof(unsigned long long))))
+        return rc;
+    HIPCHK(c, hipMemsetAsync(E.errf.p, 0, 4, c->stream));
+    Tabs T{dptr<SceneDev>(E.dscenes), dptr<ReqDev>(E.dreqs), dptr<uint32_t>(E.preq), dptr<int>(E.nsamp)};
+    if (!E.uni) HIPCHK(c, hipMemsetAsync(E.slotrays.p, 0, (size_t)E.hi * 8, c->stream));
+    const uint32_t *A = dptr<uint32_t>(E.act[E.cur]);
+    Level LV{&E, T, A, 0};
+    for (uint32_t c0 = 0; c0 < E.nact; c0 += CHUNK) {
+        uint32_t m = std::min(CHUNK, E.nact - c0);
+        k_fill_f4<<<blocks(m, 256), 256, 0, c->stream>>>(dptr<float4>(E.acc), m, make_float4(0, 0, 0, 0));
+        {
+            ProfScope ps(c, "k_raygen");
+            auto kern = E.uni ? k_raygen<true> : k_raygen<false>;
+            kern<<<blocks(m, 256), 256, 0, c->stream>>>(T, A, c0, m, dptr<QRay>(E.qlev[0]));
+        }
+        LV.c0 = c0;
+        if ((rc = process_level(LV, 0, m))) return rc;
+        auto fin = E.uni ? k_pass_finish<true> : k_pass_finish<false>;
+        fin<<<blocks(m, 256), 256, 0, c->stream>>>(T, A, c0, m, dptr<float4>(E.acc), dptr<float4>(E.fb),
+                                                  dptr<float4>(E.subpix), dptr<int>(E.nsamp), dptr<uint8_t>(E.keep));
+    }
+    /* primary rays of this pass: one per active pixel */
+    unsigned long long srays[MAX_SLOTS];
+    if (!E.uni) HIPCHK(c, hipMemcpyAsync(srays, E.slotrays.p, (size_t)E.hi * 8, hipMemcpyDeviceToHost, c->stream));
+    uint32_t nn = 0;
+    if ((rc = select_flagged(c, A, dptr<uint8_t>(E.keep), dptr<uint32_t>(E.act[E.cur ^ 1]), E.nact, &nn))) return rc;
+    E.cur ^= 1;
+    E.nact = nn;
+    uint32_t cnt[MAX_SLOTS];
+    if (E.uni) {
+        cnt[0] = nn;
+    } else {
+        k_fill_u32<<<blocks(E.hi, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.slotcnt), E.hi, 0u);
+        if (nn)
+            k_slot_count<<<blocks(nn, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.act[E.cur]), nn,
+                                                                 dptr<uint32_t>(E.preq), dptr<uint32_t>(E.slotcnt));
+        HIPCHK(c, hipMemcpyAsync(cnt, E.slotcnt.p, (size_t)E.hi * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    uint32_t errf = 0;
+    HIPCHK(c, hipMemcpy(&errf, E.errf.p, 4, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < E.hi; s++) {
+        Job *j = E.slot_job[s];
+        if (!j) continue;
+        j->rays += E.slot_active[s] + (E.uni ? 0ull : srays[s]);
+        if (errf) j->limit_hit = true;
+        E.slot_active[s] = cnt[s];
+        if (cnt[s]) continue;
+        /* all pixels stopped: the image (src/render.cc:238-241) */
+        const ReqDev &Q = E.hreqs[s];
+        uint32_t npix = (uint32_t)j->npix;
+        float *dst = j->dev_out;
+        if (!dst) {
+            if ((rc = ensure(c, E.stage, (size_t)npix * 16))) return rc;
+            dst = dptr<float>(E.stage);
+        }
+        k_finalize<<<blocks(npix, 256), 256, 0, c->stream>>>(Q.P, dptr<float4>(E.fb) + j->pix_base,
+                                                            dptr<int>(E.nsamp) + j->pix_base, dst, npix);
+        if (Q.P.flags & SRAY_RENDER_REF_Q1)
+            k_q1_scale<<<blocks(npix, 256), 256, 0, c->stream>>>(Q.P, dptr<int>(E.nsamp) + j->pix_base, dst, npix);
+        HIPCHK(c, hipGetLastError());
+        if (j->host_out) {
+            HIPCHK(c, hipMemcpyAsync(j->host_out, dst, (size_t)npix * 16, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        retire(E, s);
+        finished.push_back(j);
+    }
+    return SRAY_OK;
+}
+
+Engine *engine_get(sray_gpu_ctx *c, Engine *&slot)
+{
+    if (!slot) {
+        slot = new Engine;
+        slot->c = c;
+    }
+    return slot;
+}
+
+void engine_free(Engine *E)
+{
+    if (!E) return;
+    DevBuf *bs[] = {&E->fb, &E->nsamp, &E->preq, &E->keep, &E->act[0], &E->act[1], &E->subpix, &E->hits,
+                    &E->direct, &E->sq, &E->occ, &E->kids, &E->kflag, &E->acc, &E->stats, &E->errf, &E->rcounts,
+                    &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->stage};
+    for (DevBuf *b : bs) release(*b);
+    for (DevBuf &b : E->qlev) release(b);
+    delete E;
+}
+
+/* sray_gpu_render: one request through the plain engine */
+int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *d_out, sray_stats *st)
+{
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    Engine &E = *engine_get(c, c->plain);
+    if ((rc = ensure(c, E.stats, 8 * sizeof(unsigned long long))) || (rc = ensure(c, E.errf, 4)) ||
+        (rc = ensure(c, E.rcounts, 12 * sizeof(unsigned long long))))
+        return rc;
+    HIPCHK(c, hipMemsetAsync(E.rcounts.p, 0, 12 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(E.stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(E.errf.p, 0, 4, c->stream));
+    E.counting = (p->flags & SRAY_RENDER_COUNT) != 0;
+    Job j;
+    j.scene_id = scene_id;
+    j.p = *p;
+    j.dev_out = d_out;
+    j.t_submit = Clock::now();
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if ((rc = admit(E, &j))) return rc;
+    uint64_t prim = 0, samples = 0;
+    std::vector<Job *> fin;
+    while (j.state != 2 && rc == SRAY_OK) {
+        samples += E.nact;
+        rc = run_pass(E, fin);
+        for (Job *f : fin) f->state = 2;
+        fin.clear();
+    }
+    if (rc) {
+        abort_all(E);
+        return rc;
+    }
+    prim = j.rays; /* uni: one primary ray per active pixel per pass */
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    prof_resolve(c);
+    unsigned long long hs[8], rcn[12];
+    HIPCHK(c, hipMemcpy(hs, E.stats.p, sizeof hs, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rcn, E.rcounts.p, sizeof rcn, hipMemcpyDeviceToHost));
+    if (j.limit_hit) return fail(c, SRAY_ERR_LIMIT, "ior stack deeper than %d media", IOR_DEPTH);
+    if (st) {
+        memset(st, 0, sizeof *st);
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        st->render_ms = ms;
+        st->rays_primary = prim;
+        st->rays_shadow = hs[1];
+        st->rays_reflect = hs[2];
+        st->rays_refract = hs[3];
+        st->rays_diffuse = hs[4];
+        st->samples = samples;
+        sray_trace_counts *tc[2] = {&st->closest, &st->shadow};
+        for (int k = 0; k < 2; k++) {
+            tc[k]->nodes = rcn[k * 6 + 0];
+            tc[k]->spheres = rcn[k * 6 + 1];
+            tc[k]->cylinders = rcn[k * 6 + 2];
+            tc[k]->triangles = rcn[k * 6 + 3];
+            tc[k]->item_refs = rcn[k * 6 + 4];
+            tc[k]->mesh_enters = rcn[k * 6 + 5];
+        }
+    }
+    return SRAY_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* submit / poll: continuous batching on a worker thread                */
+/* ------------------------------------------------------------------ */
+void serve_loop(sray_gpu_ctx *c, Engine *E)
+{
+    (void)hipSetDevice(c->device);
+    std::vector<Job *> fin;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(E->qmu);
+            E->qcv.wait(lk, [&] { return E->stop || !E->pending.empty() || E->live > 0; });
+            if (E->stop) break;
+        }
+        std::lock_guard<std::recursive_mutex> g(c->mu);
+        /* admission: first come first served while the requests fit the cap */
+        for (;;) {
+            Job *j = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(E->qmu);
+                if (E->pending.empty()) break;
+                j = E->pending.front();
+            }
+            uint64_t need = job_bytes(*j);
+            uint64_t used = admitted_bytes(c);
+            if (used + need > c->admission_cap && E->live > 0) break; /* wait for requests to finish */
+            {
+                std::lock_guard<std::mutex> lk(E->qmu);
+                E->pending.pop_front();
+            }
+            int rc = used + need > c->admission_cap
+                         ? fail(c, SRAY_ERR_NOMEM, "request needs %llu bytes beyond the admission cap",
+                                (unsigned long long)need)
+                         : admit(*E, j);
+            if (rc) job_done(*E, j, rc, c->err.c_str());
+            if (E->live == MAX_SLOTS) break;
+        }
+        if (E->live == 0) continue;
+        int rc = run_pass(*E, fin);
+        if (rc) {
+            std::string msg = c->err;
+            for (uint32_t s = 0; s < E->hi; s++)
+                if (E->slot_job[s]) fin.push_back(E->slot_job[s]);
+            abort_all(*E);
+            for (Job *j : fin) job_done(*E, j, rc, msg.c_str());
+        } else {
+            for (Job *j : fin)
+                job_done(*E, j, j->limit_hit ? SRAY_ERR_LIMIT : SRAY_OK,
+                         j->limit_hit ? "ior stack deeper than the queued rays carry" : nullptr);
+        }
+        fin.clear();
+    }
+}
+
+int do_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *ticket)
+{
+    std::unique_ptr<Job> j(new Job);
+    j->scene_id = r->scene_id;
+    j->p = r->params;
+    j->cam = r->camera;
+    j->use_cam = r->use_camera != 0;
+    j->host_out = rgba;
+    int rc;
+    {
+        std::lock_guard<std::recursive_mutex> g(c->mu);
+        if ((rc = validate_params(c, &r->params))) return rc;
+        if (!find_scene(c, r->scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", r->scene_id);
+        if (j->use_cam && !(r->camera.vfov > 0.0)) return fail(c, SRAY_ERR_ARG, "request camera: vfov must be > 0");
+        Engine *E = engine_get(c, c->serve);
+        if (!E->started) {
+            E->started = true;
+            E->worker = std::thread(serve_loop, c, E);
+        }
+        if ((rc = ensure(c, E->stats, 8 * sizeof(unsigned long long))) || (rc = ensure(c, E->errf, 4)) ||
+            (rc = ensure(c, E->rcounts, 12 * sizeof(unsigned long long))))
+            return rc;
+    }
+    Engine *E = c->serve;
+    std::lock_guard<std::mutex> lk(E->qmu);
+    j->ticket = E->next_ticket++;
+    j->t_submit = Clock::now();
+    *ticket = j->ticket;
+    E->pending.push_back(j.get());
+    E->jobs[j->ticket] = std::move(j);
+    E->qcv.notify_all();
+    return SRAY_OK;
+}
+
+int do_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
+{
+    Engine *E = c->serve;
+    if (!E) return fail(c, SRAY_ERR_NOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
+    std::unique_lock<std::mutex> lk(E->qmu);
+    auto it = E->jobs.find(ticket);
+    if (it == E->jobs.end()) return SRAY_ERR_NOTFOUND;
+    Job *j = it->second.get();
+    if (j->state != 2) {
+        if (!wait) return SRAY_ERR_PENDING;
+        E->qcv.wait(lk, [&] { return j->state == 2; });
+    }
+    if (out) {
+        out->rgba = j->host_out;
+        out->rays = j->rays;
+        out->latency_ms = j->latency_ms;
+        out->queue_ms = j->queue_ms;
+    }
+    int rc = j->rc;
+    std::string err = j->err;
+    E->jobs.erase(it);
+    lk.unlock();
+    if (rc) {
+        std::lock_guard<std::recursive_mutex> g(c->mu);
+        c->err = err;
+    }
+    return rc;
+}
+
+void engine_destroy(sray_gpu_ctx *c)
+{
+    Engine *S = c->serve;
+    if (S && S->started) {
+        {
+            std::lock_guard<std::mutex> lk(S->qmu);
+            S->stop = true;
+            S->qcv.notify_all();
+        }
+        S->worker.join();
+    }
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    (void)hipStreamSynchronize(c->stream);
+    if (S) {
+        std::lock_guard<std::mutex> lk(S->qmu);
+        for (auto &kv : S->jobs)
+            if (kv.second->state != 2) {
+                kv.second->rc = SRAY_ERR_ARG;
+                kv.second->state = 2;
+            }
+        S->qcv.notify_all();
+    }
+    engine_free(c->plain);
+    engine_free(S);
+    c->plain = c->serve = nullptr;
+}
+
+uint64_t engine_bytes(sray_gpu_ctx *c)
+{
+    return (c->plain ? c->plain->bytes() : 0) + (c->serve ? c->serve->bytes() : 0);
+}
+
+int engine_scene_busy(sray_gpu_ctx *c, uint32_t scene_id)
+{
+    Engine *E = c->serve;
+    if (!E) return SRAY_OK;
+    std::lock_guard<std::mutex> lk(E->qmu);
+    for (auto &kv : E->jobs)
+        if (kv.second->state != 2 && kv.second->scene_id == scene_id)
+            return fail(c, SRAY_ERR_PENDING, "scene %u has requests in flight", scene_id);
+    return SRAY_OK;
+}
+
+} // namespace srt
+
+/* ================================================================== */
+/* C ABI                                                                */
+/* ================================================================== */
+extern "C" {
+
+int sray_gpu_primary_rays_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, int sample,
+                                 sray_ray *d_rays)
+{
+    if (!c || !p || !d_rays || sample < 0) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    GpuScene *g = find_scene(c, scene_id);
+    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    if (p->width <= 0 || p->height <= 0 || p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height ||
+        p->x0 >= p->x1 || p->y0 >= p->y1)
+        return fail(c, SRAY_ERR_ARG, "bad pixel rectangle");
+    RenderDev R;
+    fill_render_dev(p, R);
+    uint32_t n = (uint32_t)R.W * (uint32_t)(p->y1 - p->y0);
+    k_primary<<<blocks(n, 256), 256, 0, c->stream>>>(cam_dev(g->camera), R, sample, d_rays, n);
+    HIPCHK(c, hipGetLastError());
+    return SRAY_OK;
+}
+
+int sray_gpu_render_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *d_rgba,
+                           sray_stats *st)
+{
+    if (!c || !d_rgba) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    return do_render(c, scene_id, p, d_rgba, st);
+}
+
+int sray_gpu_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *rgba, sray_stats *st)
+{
+    if (!c || !rgba) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    size_t bytes = (size_t)(p->x1 - p->x0) * (size_t)(p->y1 - p->y0) * 4 * sizeof(float);
+    DevBuf out;
+    hipError_t e = hipMalloc(&out.p, bytes);
+    if (e != hipSuccess) return fail(c, SRAY_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    out.bytes = bytes;
+    rc = do_render(c, scene_id, p, (float *)out.p, st);
+    if (rc == SRAY_OK) {
+        e = hipMemcpy(rgba, out.p, bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, SRAY_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(e));
+    }
+    release(out);
+    return rc;
+}
+
+int sray_gpu_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *ticket)
+{
+    if (!c || !r || !rgba || !ticket) return SRAY_ERR_ARG;
+    return do_submit(c, r, rgba, ticket);
+}
+
+int sray_gpu_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
+{
+    if (!c) return SRAY_ERR_ARG;
+    return do_poll(c, ticket, wait, out);
+}
+
+int sray_gpu_set_admission_cap(sray_gpu_ctx *c, uint64_t bytes)
+{
+    if (!c) return SRAY_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    c->admission_cap = bytes;
+    return SRAY_OK;
+}
+
+} /* extern "C" */

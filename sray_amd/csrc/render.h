@@ -14,6 +14,7 @@
 #ifndef SRAY_RENDER_H_
 #define SRAY_RENDER_H_
 
+#include "pmap.h"
 #include "traverse.h"
 
 namespace srd {
@@ -22,14 +23,15 @@ struct alignas(16) QRay {
     float4 o;    /* origin, w = time (int bits) */
     float4 d;    /* direction, w = energy */
     float4 w;    /* throughput weight rgb, w = current ior */
-    uint4 m;     /* x = sample slot (chunk-local), y = path id, z = iter, w = nstack | flags<<8 */
+    uint4 m;     /* x = sample slot (chunk-local), y = path id, z = iter,
+                  * w = nstack | flags << 8 | request slot << 16 */
     float4 stk;  /* ior stack */
 };
 
 struct alignas(16) SRay {
     float4 o;    /* origin, w = time (int bits) */
     float4 d;    /* direction (unnormalised: segment to the light), w = 1 valid / 0 unused */
-    float4 c;    /* contribution rgb if unoccluded */
+    float4 c;    /* contribution rgb if unoccluded, w = request slot (bits) */
 };
 
 constexpr uint32_t QF_DIFFUSE = 1u;  /* diffuse GI ray (IS_DIFFUSE_RAY, src/shader.cc:80-81) */
@@ -42,20 +44,45 @@ struct RenderDev {
     uint32_t flags;
     uint64_t seed;
     float aspect;     /* (float)((double)width / height) */
-    int32_t L;        /* shadow slots per ray */
-    int32_t K;        /* child slots per ray */
 };
 
-/* photon-map gather inputs for shading (null when the render has none) */
-struct PmapDev {
-    const float4 *pos;     /* x,y,z, w = photon index (bits) — kd-tree order */
-    const float4 *dir;     /* dir.xyz, w unused */
-    const float4 *nrm;     /* norm.xyz */
-    const float4 *col;     /* col.rgb */
-    const float4 *split;   /* per kd node: split value in .x, axis in .y bits */
-    uint32_t n;
-    uint32_t depth;
+/* photon maps bound for shading (caustic irradiance, GI radiance) */
+struct ShadePmaps {
+    PmapTree caust, glob;
+    float radius; /* inclusion radius (float, as kd_nearest_range3f) */
+    double scale; /* 1 / (pi r^2) with the double gather distance */
 };
+
+/* the camera a request renders through (Camera::get_primary_ray inputs) */
+struct CamDev {
+    float cam[12];   /* camera -> world */
+    float fz;        /* 1/tan(vfov/2), from double on the host */
+    float max_dist;
+    int32_t shutter;
+    int32_t pad;
+};
+
+/* one request of a batch: its parameters, camera, photon maps and where its
+ * pixels live in the engine's pixel arena */
+struct ReqDev {
+    RenderDev P;
+    CamDev C;
+    ShadePmaps PM;
+    uint32_t scene;    /* index into the batch's scene table */
+    uint32_t pix_base; /* first arena pixel */
+    uint64_t sub_base; /* first per-sample record (max_samples per pixel) */
+};
+
+/* kernel view of a batch */
+struct Tabs {
+    const SceneDev *scenes;
+    const ReqDev *reqs;
+    const uint32_t *preq; /* arena pixel -> request slot */
+    const int *nsamp;     /* arena pixel -> samples taken */
+};
+
+constexpr int SLOT_SHIFT = 16;
+constexpr uint32_t MAX_SLOTS = 1024; /* requests in flight in one engine */
 
 } // namespace srd
 

@@ -28,13 +28,13 @@
 #include <thread>
 #include <vector>
 
-#include "host.h"
+#include "runtime.h"
 #include "pmap.h"
-#include "render.h"
 #include "surface.h"
 #include "photon.h"
 
 using namespace srd;
+using namespace srt;
 
 /* ================================================================== */
 /* kernels                                                              */
@@ -71,523 +71,6 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneDev S, const sr
         atomicAdd(&counts[4], (unsigned long long)cnt.item_refs);
         atomicAdd(&counts[5], (unsigned long long)cnt.mesh_enters);
     }
-}
-
-/* Camera::get_primary_ray, src/camera.cc:119-142 */
-__device__ __forceinline__ void primary_ray(const SceneDev &S, const RenderDev &P, int x, int y, int sample, V3 &o,
-                                            V3 &d, int &time)
-{
-    float ysz = 2.0f;
-    float xsz = P.aspect * ysz;
-    float px = ((float)x / (float)P.width) * xsz - xsz / 2.0f;
-    float py = 1.0f - ((float)y / (float)P.height) * ysz;
-    uint32_t pix = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
-    if (sample > 0) {
-        float f0 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 0));
-        float f1 = rng_unit(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 1));
-        px = px + xsz * (f0 - 0.5f) / (float)P.width;
-        py = py + ysz * (f1 - 0.5f) / (float)P.height;
-    }
-    V3 dl = v3(px * S.max_dist, py * S.max_dist, S.fz * S.max_dist);
-    time = P.frame_time;
-    if (P.mblur && sample > 0 && S.shutter > 0) {
-        uint32_t u = rng_u32(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 2));
-        time += (int)(u % (uint32_t)S.shutter) - S.shutter / 2;
-    }
-    o = xf_point(S.cam, v3(0.0f, 0.0f, 0.0f));
-    d = xf_vec(S.cam, dl);
-}
-
-__global__ void k_primary(SceneDev S, RenderDev P, int sample, sray_ray *out, uint32_t n)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int x = P.x0 + (int)(i % (uint32_t)P.W), y = P.y0 + (int)(i / (uint32_t)P.W);
-    V3 o, d;
-    int time;
-    primary_ray(S, P, x, y, sample, o, d, time);
-    sray_ray r;
-    r.origin[0] = o.x;
-    r.origin[1] = o.y;
-    r.origin[2] = o.z;
-    r.dir[0] = d.x;
-    r.dir[1] = d.y;
-    r.dir[2] = d.z;
-    r.time = time;
-    r.pad_ = 0;
-    out[i] = r;
-}
-
-__global__ void k_raygen(SceneDev S, RenderDev P, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m,
-                         int sample, QRay *__restrict__ q)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    uint32_t pr = active[c0 + i];
-    int x = P.x0 + (int)(pr % (uint32_t)P.W), y = P.y0 + (int)(pr / (uint32_t)P.W);
-    V3 o, d;
-    int time;
-    primary_ray(S, P, x, y, sample, o, d, time);
-    QRay r;
-    r.o = make_float4(o.x, o.y, o.z, __int_as_float(time));
-    r.d = make_float4(d.x, d.y, d.z, 1.0f);
-    r.w = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    r.m = make_uint4(i, 1u, (uint32_t)P.iter, 0u);
-    r.stk = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    q[i] = r;
-}
-
-template <bool COUNT> __device__ __forceinline__ void flush_counts(const TraceCounts &cnt, unsigned long long *c)
-{
-    if (!COUNT) return;
-    atomicAdd(&c[0], (unsigned long long)cnt.nodes);
-    atomicAdd(&c[1], (unsigned long long)cnt.spheres);
-    atomicAdd(&c[2], (unsigned long long)cnt.cylinders);
-    atomicAdd(&c[3], (unsigned long long)cnt.triangles);
-    atomicAdd(&c[4], (unsigned long long)cnt.item_refs);
-    atomicAdd(&c[5], (unsigned long long)cnt.mesh_enters);
-}
-
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_q(SceneDev S, const QRay *__restrict__ q, uint32_t n,
-                                                         int4 *__restrict__ hits, unsigned long long *counts)
-{
-    __shared__ uint2 stk[STACK * TRACE_BLOCK];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 o = q[i].o, d = q[i].d;
-    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
-    HitRec h = traverse<false, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
-                                      TRACE_BLOCK, cnt);
-    hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
-    flush_counts<COUNT>(cnt, counts);
-}
-
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_shadow(SceneDev S, const SRay *__restrict__ sq, uint32_t n,
-                                                              uint8_t *__restrict__ occ, unsigned long long *counts)
-{
-    __shared__ uint2 stk[STACK * TRACE_BLOCK];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 d = sq[i].d;
-    if (d.w == 0.0f) {
-        occ[i] = 0;
-        return;
-    }
-    float4 o = sq[i].o;
-    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
-    HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
-                                     TRACE_BLOCK, cnt);
-    occ[i] = h.obj >= 0 ? 1 : 0;
-    flush_counts<COUNT>(cnt, counts);
-}
-
-/* light sample point: Light::get_point src/light.cc:106-109, SphLight
- * :272-275, BoxLight :346-354 (matrix at time 0) */
-__device__ __forceinline__ V3 sph_rand(uint64_t seed, uint32_t pix, uint32_t sample, uint32_t path, uint32_t kind,
-                                       uint32_t i1, uint32_t i2, float r)
-{
-    /* Marsaglia (1972): uniform on the sphere from a rejection pair */
-    for (uint32_t a = 0; a < 64; a++) {
-        float u1 = rng_unit(seed, pix, sample, path, rslot(kind, i1, i2, a, 0));
-        float u2 = rng_unit(seed, pix, sample, path, rslot(kind, i1, i2, a, 1));
-        float x1 = 2.0f * u1 - 1.0f, x2 = 2.0f * u2 - 1.0f;
-        float s = x1 * x1 + x2 * x2;
-        if (s < 1.0f && s > 0.0f) {
-            float qq = 2.0f * sqrtf(1.0f - s);
-            V3 d = v3(x1 * qq, x2 * qq, 1.0f - 2.0f * s);
-            return mul(d, r);
-        }
-    }
-    return v3(0.0f, 0.0f, r);
-}
-
-__device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uint32_t pix, uint32_t sample,
-                                          uint32_t path, uint32_t li, uint32_t si)
-{
-    if (lt.type == SRAY_LIGHT_BOX) {
-        float fx = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 0));
-        float fy = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 1));
-        float fz = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 2));
-        V3 v = v3(fx * lt.dim[0] - 0.5f * lt.dim[0], fy * lt.dim[1] - 0.5f * lt.dim[1],
-                  fz * lt.dim[2] - 0.5f * lt.dim[2]);
-        return xf_point(lt.xf, v);
-    }
-    if (lt.type == SRAY_LIGHT_SPHERE) {
-        V3 s = sph_rand(seed, pix, sample, path, K_LIGHT, li, si, lt.radius);
-        return add(v3(lt.pos[0], lt.pos[1], lt.pos[2]), s);
-    }
-    return v3(lt.pos[0], lt.pos[1], lt.pos[2]);
-}
-
-__device__ __forceinline__ uint32_t pixel_id(const RenderDev &P, const uint32_t *active, uint32_t c0, uint32_t slot)
-{
-    uint32_t pr = active[c0 + slot];
-    uint32_t x = (uint32_t)P.x0 + pr % (uint32_t)P.W, y = (uint32_t)P.y0 + pr / (uint32_t)P.W;
-    return y * (uint32_t)P.width + x;
-}
-
-/* shade_phong (src/shader.cc:83-247) / shade_undef (:36-71) for one hit,
- * emitting direct terms, shadow rays and children */
-struct ShadePmaps {
-    PmapTree caust, glob;
-    float radius;     /* inclusion radius (float, as kd_nearest_range3f) */
-    double scale;     /* 1 / (pi r^2) with the double gather distance */
-};
-
-__device__ __forceinline__ V3 gather_est(const PmapTree &T, V3 pos, V3 n, float radius, double scale, int mode,
-                                         uint2 *stk)
-{
-    GatherAcc a;
-    gather_range(T, pos.x, pos.y, pos.z, radius, mode, (double)n.x, (double)n.y, (double)n.z, a, stk, 256);
-    if (mode == SRAY_GATHER_IRRADIANCE && a.count < 8) return v3(0.0f, 0.0f, 0.0f);
-    return v3((float)(a.r * scale), (float)(a.g * scale), (float)(a.b * scale));
-}
-
-template <bool PHOTONS>
-__global__ __launch_bounds__(256) void k_shade(SceneDev S, RenderDev P, ShadePmaps PM,
-                                               const uint32_t *__restrict__ active,
-                                               uint32_t c0, int sample, const QRay *__restrict__ q,
-                                               const int4 *__restrict__ hits, uint32_t n, float4 *__restrict__ direct,
-                                               SRay *__restrict__ sq, QRay *__restrict__ kids,
-                                               uint8_t *__restrict__ kflag, unsigned long long *__restrict__ stats,
-                                               uint32_t *__restrict__ err)
-{
-    __shared__ uint32_t bcnt[5];
-    __shared__ uint2 gstk[PHOTONS ? PMAP_STACK * 256 : 1];
-    if (threadIdx.x < 5) bcnt[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
-    if (i < n) {
-        QRay R = q[i];
-        int4 h = hits[i];
-        uint32_t slot = R.m.x, path = R.m.y, flg = R.m.w >> 8;
-        int iter = (int)R.m.z;
-        V3 W = v3(R.w.x, R.w.y, R.w.z);
-        float energy = R.d.w;
-        int time = __float_as_int(R.o.w);
-        V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
-        float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        for (int l = 0; l < P.L; l++) sq[(size_t)i * P.L + l].d.w = 0.0f;
-        for (int k = 0; k < P.K; k++) kflag[(size_t)i * P.K + k] = 0;
-        if (h.y < 0) {
-            /* miss: env_color (src/scene.cc:568) */
-            dir.x = W.x * S.env_color.x;
-            dir.y = W.y * S.env_color.y;
-            dir.z = W.z * S.env_color.z;
-            if (path == 1u) dir.w = S.env_color.w;
-            direct[i] = dir;
-        } else {
-            float t = __int_as_float(h.x);
-            const DevObj &ob = S.objs[h.y];
-            const DevMat &mat = S.mats[ob.mat];
-            bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
-            V3 Pw, N;
-            float tu, tv;
-            hit_surface(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong, Pw, N, tu, tv);
-            if (path == 1u) dir.w = 1.0f;
-            uint32_t pix = pixel_id(P, active, c0, slot);
-            int nl = S.nlights;
-            if (!phong) {
-                /* shade_undef: checker texture from texcoords */
-                int cx = (int)(tu * 20.0f), cy = (int)(tv * 10.0f);
-                V3 tex = ((cx % 2) == (cy % 2)) ? v3(1.0f, 0.3f, 0.2f) : v3(0.2f, 0.3f, 1.0f);
-                dir.x = W.x * (0.1f * tex.x);
-                dir.y = W.y * (0.1f * tex.y);
-                dir.z = W.z * (0.1f * tex.z);
-                for (int l = 0; l < nl; l++) {
-                    const DevLight &lt = S.lights[l];
-                    V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u);
-                    V3 ldir = sub(lpos, Pw);
-                    float ldist = length(ldir);
-                    V3 sd = ldir;
-                    ldir = dvs(ldir, ldist);
-                    V3 vdir = neg(normalized(rd));
-                    V3 hdir = normalized(add(ldir, vdir));
-                    float ndotl = fmaxf(dot(N, ldir), 0.0f);
-                    float ndoth = fmaxf(dot(N, hdir), 0.0f);
-                    float sp = powf(ndoth, 50.0f);
-                    SRay s;
-                    s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
-                    s.d = make_float4(sd.x, sd.y, sd.z, 1.0f);
-                    s.c = make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
-                                      W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
-                                      W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp), 0.0f);
-                    sq[(size_t)i * P.L + l] = s;
-                    c_sh++;
-                }
-                direct[i] = dir;
-            } else {
-                bool entering;
-                V3 normal;
-                if (dot(rd, N) > 0.0f) {
-                    normal = neg(N);
-                    entering = false;
-                } else {
-                    normal = N;
-                    entering = true;
-                }
-                float ray_mag = length(rd);
-                V3 incident = dvs(rd, ray_mag);
-                float4 kd = mat.kd;
-                float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-                if (flg & QF_DIFFUSE) {
-                    /* diffuse GI ray: radiance estimate from the global map
-                     * (src/shader.cc:119-123) */
-                    V3 rad = PHOTONS ? gather_est(PM.glob, Pw, normal, PM.radius, PM.scale, SRAY_GATHER_RADIANCE,
-                                                  &gstk[threadIdx.x])
-                                     : v3(0.0f, 0.0f, 0.0f);
-                    dir.x = W.x * (rad.x * kd.x);
-                    dir.y = W.y * (rad.y * kd.y);
-                    dir.z = W.z * (rad.z * kd.z);
-                    direct[i] = dir;
-                } else {
-                    float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
-                    float trans_fact = (mat.flags & SRAY_MAT_HAS_REFRACT) ? mat.refract : 0.0f;
-                    float mat_ior = mat.ior;
-                    IorStack st;
-                    st.cur = R.w.w;
-                    st.s[0] = R.stk.x;
-                    st.s[1] = R.stk.y;
-                    st.s[2] = R.stk.z;
-                    st.s[3] = R.stk.w;
-                    st.n = R.m.w & 0xffu;
-                    st.overflow = false;
-                    float ior = st.calc(entering, mat_ior);
-                    /* ambient: (env_ambient * kd) * kd, src/shader.cc:142,246 */
-                    dir.x = W.x * ((S.env_ambient.x * kd.x) * kd.x);
-                    dir.y = W.y * ((S.env_ambient.y * kd.y) * kd.y);
-                    dir.z = W.z * ((S.env_ambient.z * kd.z) * kd.z);
-                    int slotk = 0;
-                    V3 vref = reflect(incident, normal);
-                    for (int l = 0; l < nl; l++) {
-                        const DevLight &lt = S.lights[l];
-                        int ns = lt.type != SRAY_LIGHT_POINT ? P.shadow_samples : 1;
-                        float scale = ns > 1 ? 1.0f / (float)ns : 1.0f;
-                        for (int si = 0; si < ns; si++) {
-                            V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si);
-                            V3 ldir = sub(lpos, Pw);
-                            float ldist = length(ldir);
-                            V3 sd = ldir;
-                            ldir = dvs(ldir, ldist);
-                            float ndotl = fmaxf(dot(normal, ldir), 0.0f);
-                            float rdotl = fmaxf(dot(vref, ldir), 0.0f);
-                            float sp = powf(rdotl, mat.shininess);
-                            SRay s;
-                            s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
-                            s.d = make_float4(sd.x, sd.y, sd.z, 1.0f);
-                            s.c = make_float4(
-                                W.x * ((kd.x * (lt.color[0] * ndotl) + ks.x * (lt.color[0] * sp)) * scale),
-                                W.y * ((kd.y * (lt.color[1] * ndotl) + ks.y * (lt.color[1] * sp)) * scale),
-                                W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale), 0.0f);
-                            sq[(size_t)i * P.L + slotk] = s;
-                            slotk++;
-                            c_sh++;
-                        }
-                    }
-                    if (PHOTONS && P.gi) {
-                        /* diffuse hemisphere rays (src/shader.cc:158-185); a
-                         * direction flipped into the hemisphere keeps its
-                         * negative ndotl and so spawns no ray */
-                        float dsc = P.diffuse_samples > 1 ? 1.0f / (float)P.diffuse_samples : 1.0f;
-                        for (int di = 0; di < P.diffuse_samples; di++) {
-                            V3 gd = sph_rand(P.seed, pix, (uint32_t)sample, path, K_DIFFUSE, (uint32_t)di, 0u, 1.0f);
-                            float ndotl = dot(gd, normal);
-                            if (ndotl < 0.0f) gd = neg(gd);
-                            float diff_energy = ndotl * energy;
-                            if (diff_energy > P.min_energy) {
-                                QRay cr;
-                                V3 dd = mul(gd, ray_mag);
-                                cr.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                                cr.d = make_float4(dd.x, dd.y, dd.z, diff_energy);
-                                float wf = ndotl * dsc;
-                                cr.w = make_float4(W.x * (kd.x * wf), W.y * (kd.y * wf), W.z * (kd.z * wf), R.w.w);
-                                cr.m = make_uint4(slot, path | 0x40000000u | ((uint32_t)di << 20), 0x7fffffffu,
-                                                  (R.m.w & 0xffu) | (QF_DIFFUSE << 8));
-                                cr.stk = R.stk;
-                                kids[(size_t)i * P.K + 2 + di] = cr;
-                                kflag[(size_t)i * P.K + 2 + di] = 1;
-                                c_df++;
-                            }
-                        }
-                    }
-                    if (PHOTONS && P.caust) {
-                        /* caustics: irradiance from the caustic map (src/shader.cc:231-236) */
-                        V3 irr = gather_est(PM.caust, Pw, normal, PM.radius, PM.scale, SRAY_GATHER_IRRADIANCE,
-                                            &gstk[threadIdx.x]);
-                        dir.x = dir.x + W.x * irr.x;
-                        dir.y = dir.y + W.y * irr.y;
-                        dir.z = dir.z + W.z * irr.z;
-                    }
-                    V3 nrd = dvs(neg(rd), ray_mag);
-                    float ray_dot_n = dot(nrd, normal);
-                    float sf0 = trans_fact > 0.0f ? (ior - 1.0f) / (ior + 1.0f) : 1.0f;
-                    float fres = fresnel(sf0 * sf0, ray_dot_n);
-                    float refl_energy = refl_fact * energy;
-                    if (refl_energy > P.min_energy && iter > 0) {
-                        V3 nd = reflect(rd, normal);
-                        QRay c;
-                        c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                        c.d = make_float4(nd.x, nd.y, nd.z, refl_energy);
-                        float wf = fres * refl_fact;
-                        c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), R.w.w);
-                        c.m = make_uint4(slot, path * 2u, (uint32_t)(iter - 1), R.m.w);
-                        c.stk = R.stk;
-                        kids[(size_t)i * P.K + 0] = c;
-                        kflag[(size_t)i * P.K + 0] = 1;
-                        c_rl++;
-                    }
-                    float trans_energy = trans_fact * energy;
-                    if (trans_energy > P.min_energy && iter > 0) {
-                        IorStack ts = st;
-                        if (entering) ts.enter(mat_ior);
-                        else ts.leave();
-                        V3 td = mul(refract(dvs(rd, ray_mag), normal, ior), ray_mag);
-                        if (dot(td, normal) > 0.0f) {
-                            if (entering) ts.leave();
-                            else ts.enter(mat_ior);
-                        }
-                        if (ts.overflow) atomicOr(err, 1u);
-                        QRay c;
-                        c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                        c.d = make_float4(td.x, td.y, td.z, trans_energy);
-                        float wf = (1.0f - fres) * trans_fact;
-                        c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), ts.cur);
-                        c.m = make_uint4(slot, path * 2u + 1u, (uint32_t)(iter - 1), ts.n | (flg << 8));
-                        c.stk = make_float4(ts.s[0], ts.s[1], ts.s[2], ts.s[3]);
-                        kids[(size_t)i * P.K + 1] = c;
-                        kflag[(size_t)i * P.K + 1] = 1;
-                        c_rr++;
-                    }
-                    direct[i] = dir;
-                }
-            }
-        }
-    }
-    if (c_sh) atomicAdd(&bcnt[0], c_sh);
-    if (c_rl) atomicAdd(&bcnt[1], c_rl);
-    if (c_rr) atomicAdd(&bcnt[2], c_rr);
-    if (c_df) atomicAdd(&bcnt[3], c_df);
-    __syncthreads();
-    if (threadIdx.x < 4 && bcnt[threadIdx.x]) atomicAdd(&stats[1 + threadIdx.x], (unsigned long long)bcnt[threadIdx.x]);
-}
-
-/* per-sample accumulation: the first ray of each run of equal sample slots
- * sums the run (queues are ordered by slot) */
-__global__ void k_accum(const QRay *__restrict__ q, const float4 *__restrict__ direct, const SRay *__restrict__ sq,
-                        const uint8_t *__restrict__ occ, uint32_t n, int L, float4 *__restrict__ acc)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t slot = q[i].m.x;
-    if (i > 0 && q[i - 1].m.x == slot) return;
-    float4 s = acc[slot];
-    for (uint32_t j = i; j < n && q[j].m.x == slot; j++) {
-        float4 d = direct[j];
-        s.x = s.x + d.x;
-        s.y = s.y + d.y;
-        s.z = s.z + d.z;
-        s.w = s.w + d.w;
-        for (int l = 0; l < L; l++) {
-            size_t k = (size_t)j * L + l;
-            if (sq[k].d.w != 0.0f && !occ[k]) {
-                float4 c = sq[k].c;
-                s.x = s.x + c.x;
-                s.y = s.y + c.y;
-                s.z = s.z + c.z;
-            }
-        }
-    }
-    acc[slot] = s;
-}
-
-__global__ void k_fill_f4(float4 *p, size_t n, float4 v)
-{
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__global__ void k_iota(uint32_t *p, uint32_t n)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = i;
-}
-
-/* end of one sample pass: framebuffer accumulation in float
- * (src/render.cc:226-229) and the variance stop test (:232-237, 261-270) */
-__global__ void k_pass_finish(RenderDev P, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m, int sample,
-                              const float4 *__restrict__ acc, float4 *__restrict__ fb, float4 *__restrict__ subpix,
-                              int *__restrict__ nsamp, uint8_t *__restrict__ keep)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    uint32_t pr = active[c0 + i];
-    float4 c = acc[i];
-    float4 f = fb[pr];
-    f.x = f.x + c.x;
-    f.y = f.y + c.y;
-    f.z = f.z + c.z;
-    f.w = f.w + c.w;
-    fb[pr] = f;
-    int ns = sample + 1;
-    nsamp[pr] = ns;
-    int maxs = P.max_samples;
-    bool k = ns < maxs;
-    if (maxs > 1) {
-        subpix[(size_t)pr * maxs + sample] = c;
-        if (ns >= P.min_samples && ns > 1) {
-            float rcp = 1.0f / (float)ns;
-            float4 mean = make_float4(f.x * rcp, f.y * rcp, f.z * rcp, f.w * rcp);
-            float sq = 0.0f;
-            for (int j = 0; j < ns; j++) {
-                float4 s = subpix[(size_t)pr * maxs + j];
-                float dx = mean.x - s.x, dy = mean.y - s.y, dz = mean.z - s.z, dw = mean.w - s.w;
-                float l2 = dx * dx + dy * dy + dz * dz + dw * dw;
-                sq = sq + l2;
-            }
-            float v = sq * rcp;
-            if (v < P.max_var) k = false;
-        }
-    }
-    keep[c0 + i] = k ? 1 : 0;
-}
-
-/* final pixel value: divide all channels by the sample count, or the
- * reference's as-written img[x] *= rcp_lut[i] (src/render.cc:240, Q1) */
-__global__ void k_finalize(RenderDev P, const float4 *__restrict__ fb, const int *__restrict__ nsamp,
-                           float *__restrict__ out, uint32_t npix)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npix) return;
-    float4 f = fb[i];
-    if (P.flags & SRAY_RENDER_REF_Q1) {
-        out[(size_t)i * 4 + 0] = f.x;
-        out[(size_t)i * 4 + 1] = f.y;
-        out[(size_t)i * 4 + 2] = f.z;
-        out[(size_t)i * 4 + 3] = f.w;
-        return;
-    }
-    float rcp = 1.0f / (float)nsamp[i];
-    out[(size_t)i * 4 + 0] = f.x * rcp;
-    out[(size_t)i * 4 + 1] = f.y * rcp;
-    out[(size_t)i * 4 + 2] = f.z * rcp;
-    out[(size_t)i * 4 + 3] = f.w * rcp;
-}
-
-__global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__restrict__ out, uint32_t npix)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npix) return;
-    int xr = (int)(i % (uint32_t)P.W), yr = (int)(i / (uint32_t)P.W);
-    int ax = P.x0 + xr;
-    int tx = (ax / 48) * 48;
-    int org = tx > P.x0 ? tx : P.x0; /* tile origin clipped to the rectangle */
-    int f = ax - org;                /* pixel offset within the tile row */
-    size_t fi = ((size_t)yr * P.W + (size_t)(org - P.x0)) * 4 + (size_t)f;
-    float rcp = 1.0f / (float)nsamp[i];
-    out[fi] = out[fi] * rcp;
 }
 
 /* ------------------------------------------------------------------ */
@@ -715,6 +198,12 @@ __global__ void k_pm_final(const sray_photon *__restrict__ ph, const uint32_t *_
     dir[i] = make_float4(p.dir[0], p.dir[1], p.dir[2], p.col[0]);
     nrm[i] = make_float4(p.norm[0], p.norm[1], p.norm[2], p.col[1]);
     colb[i] = p.col[2];
+}
+
+__global__ void k_iota(uint32_t *p, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
 }
 
 __device__ __forceinline__ void finish_estimate(const GatherAcc &a, int mode, double scale, float *out)
@@ -846,69 +335,7 @@ __global__ void k_pm_select(PmapTree T, const sray_gather_query *__restrict__ q,
 /* ================================================================== */
 /* runtime                                                              */
 /* ================================================================== */
-namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
-template <class T> T *dptr(DevBuf &b) { return (T *)b.p; }
-
-struct GpuPmap {
-    std::vector<DevBuf> bufs;
-    PmapTree tree{};
-    sray_photon *store = nullptr; /* photons in storage order */
-    uint32_t scene_id = 0;
-    int kind = 0;
-    uint64_t bytes = 0;
-};
-
-struct GpuScene {
-    std::vector<DevBuf> bufs;
-    SceneDev dev{};
-    uint64_t bytes = 0;
-    double diag = 0;
-    uint32_t scene_height = 0, mesh_height = 0;
-    uint32_t nobjs = 0;
-    uint32_t pmap[2] = {0, 0}; /* bound caustic / global photon maps */
-    sray::PhotonScene photon;  /* what the projection maps need */
-};
-
-} // namespace
-
-struct sray_gpu_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::recursive_mutex mu;
-    std::string err;
-    std::map<uint32_t, std::unique_ptr<GpuScene>> scenes;
-    uint32_t next_scene = 1;
-    std::map<uint32_t, std::unique_ptr<GpuPmap>> pmaps;
-    uint32_t next_pmap = 1;
-    uint64_t admission_cap = 200ull << 30;
-    /* render scratch, grown on demand */
-    std::vector<DevBuf> qlev;
-    DevBuf hits, direct, sq, occ, kids, kflag, acc, fb, subpix, nsamp, act[2], keep, stats, errf, cubtmp, nsel, rays,
-        hitbuf, counts, gq, grgb, gcnt, gneed, gfq, goff, gcand, giota;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    /* per-kernel HIP-event timing */
-    bool profiling = false;
-    std::vector<hipEvent_t> evpool;
-    struct Pending {
-        const char *name;
-        hipEvent_t a, b;
-    };
-    std::vector<Pending> pending;
-    struct KStat {
-        uint64_t launches = 0;
-        double ms = 0;
-    };
-    std::map<std::string, KStat> kstats;
-    DevBuf rcounts;
-};
-
-namespace {
+namespace srt {
 
 int fail(sray_gpu_ctx *c, int code, const char *fmt, ...)
 {
@@ -920,12 +347,6 @@ int fail(sray_gpu_ctx *c, int code, const char *fmt, ...)
     if (c) c->err = b;
     return code;
 }
-
-#define HIPCHK(c, x)                                                                                                  \
-    do {                                                                                                              \
-        hipError_t e_ = (x);                                                                                          \
-        if (e_ != hipSuccess) return fail((c), SRAY_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));            \
-    } while (0)
 
 int ensure(sray_gpu_ctx *c, DevBuf &b, size_t bytes)
 {
@@ -971,8 +392,6 @@ int upload(sray_gpu_ctx *c, GpuScene &g, const void *src, size_t bytes, const vo
     return SRAY_OK;
 }
 
-inline uint32_t blocks(size_t n, int bs) { return (uint32_t)((n + (size_t)bs - 1) / (size_t)bs); }
-
 hipEvent_t prof_event(sray_gpu_ctx *c)
 {
     if (!c->evpool.empty()) {
@@ -984,28 +403,6 @@ hipEvent_t prof_event(sray_gpu_ctx *c)
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
-
-/* records an event pair around one launch when profiling is on */
-struct ProfScope {
-    sray_gpu_ctx *c;
-    const char *name;
-    hipEvent_t a = nullptr;
-    ProfScope(sray_gpu_ctx *cc, const char *n) : c(cc), name(n)
-    {
-        if (c->profiling && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
-    }
-    ~ProfScope()
-    {
-        if (!a) return;
-        hipEvent_t b = prof_event(c);
-        if (!b) {
-            c->evpool.push_back(a);
-            return;
-        }
-        (void)hipEventRecord(b, c->stream);
-        c->pending.push_back({name, a, b});
-    }
-};
 
 /* after a stream synchronisation: fold pending event pairs into the stats */
 void prof_resolve(sray_gpu_ctx *c)
@@ -1201,273 +598,11 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     D.shutter = s->camera.shutter;
     g->nobjs = s->num_objects;
     sray::photon_scene_from_desc(s, g->photon);
+    g->camera = s->camera;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     uint32_t id = c->next_scene++;
     c->scenes[id] = std::move(g);
     *scene_id = id;
-    return SRAY_OK;
-}
-
-void fill_render_dev(const sray_render_params *p, RenderDev &R)
-{
-    memset(&R, 0, sizeof R);
-    R.width = p->width;
-    R.height = p->height;
-    R.x0 = p->x0;
-    R.y0 = p->y0;
-    R.x1 = p->x1;
-    R.y1 = p->y1;
-    R.W = p->x1 - p->x0;
-    R.min_samples = p->min_samples;
-    R.max_samples = std::max(1, p->max_samples);
-    R.max_var = p->max_var;
-    R.min_energy = p->min_energy;
-    R.iter = p->iter;
-    R.shadow_samples = std::max(1, p->shadow_samples);
-    R.diffuse_samples = std::max(0, p->diffuse_samples);
-    R.frame_time = p->frame_time;
-    R.mblur = p->mblur;
-    R.caust = p->caust_photons;
-    R.gi = p->gi_photons;
-    R.flags = p->flags;
-    R.seed = p->seed;
-    R.aspect = (float)((double)p->width / (double)p->height);
-}
-
-int validate_params(sray_gpu_ctx *c, const sray_render_params *p)
-{
-    if (!p) return fail(c, SRAY_ERR_ARG, "render params is NULL");
-    if (p->width <= 0 || p->height <= 0) return fail(c, SRAY_ERR_ARG, "bad image size %dx%d", p->width, p->height);
-    if (p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height || p->x0 >= p->x1 || p->y0 >= p->y1)
-        return fail(c, SRAY_ERR_ARG, "bad pixel rectangle");
-    if (p->max_samples > 256) return fail(c, SRAY_ERR_LIMIT, "max_samples > 256");
-    if (p->iter < 0 || p->iter > 30) return fail(c, SRAY_ERR_LIMIT, "iter outside [0,30]");
-    if (p->shadow_samples > 1024) return fail(c, SRAY_ERR_LIMIT, "shadow_samples > 1024");
-    if (p->diffuse_samples > 1023 || p->diffuse_samples < 0) return fail(c, SRAY_ERR_LIMIT, "diffuse_samples outside [0,1023]");
-    if ((p->gi_photons || p->caust_photons) && !(p->gather_dist > 0.0f))
-        return fail(c, SRAY_ERR_ARG, "gather_dist must be > 0 with photon maps");
-    return SRAY_OK;
-}
-
-constexpr uint32_t CHUNK = 1u << 21; /* rays per level sub-chunk */
-
-struct LevelCtx {
-    sray_gpu_ctx *c;
-    GpuScene *g;
-    RenderDev R;
-    ShadePmaps PM;
-    const uint32_t *active;
-    uint32_t c0;
-    int sample;
-    unsigned long long *stats;
-};
-
-/* stable selection of the flagged elements (order kept), count to the host */
-template <class T>
-int select_flagged(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, uint32_t n, uint32_t *count)
-{
-    size_t tmp = 0;
-    HIPCHK(c, hipcub::DeviceSelect::Flagged(nullptr, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
-    int rc = ensure(c, c->cubtmp, tmp);
-    if (rc) return rc;
-    tmp = c->cubtmp.bytes;
-    HIPCHK(c, hipcub::DeviceSelect::Flagged(c->cubtmp.p, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
-    HIPCHK(c, hipMemcpyAsync(count, c->nsel.p, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return SRAY_OK;
-}
-int compact(sray_gpu_ctx *c, const QRay *in, const uint8_t *flags, QRay *out, uint32_t n, uint32_t *count)
-{
-    return select_flagged(c, in, flags, out, n, count);
-}
-int compact_u32(sray_gpu_ctx *c, const uint32_t *in, const uint8_t *flags, uint32_t *out, uint32_t n, uint32_t *count)
-{
-    return select_flagged(c, in, flags, out, n, count);
-}
-
-/* one bounce level over qlev[b][0,count): sub-chunks of CHUNK rays, each
- * followed depth-first by its children's levels */
-int process_level(LevelCtx &L, int b, uint32_t count)
-{
-    sray_gpu_ctx *c = L.c;
-    const SceneDev &S = L.g->dev;
-    const RenderDev &R = L.R;
-    int rc;
-    QRay *Q = dptr<QRay>(c->qlev[b]);
-    for (uint32_t off = 0; off < count; off += CHUNK) {
-        uint32_t m = std::min(CHUNK, count - off);
-        const QRay *q = Q + off;
-        bool count = (R.flags & SRAY_RENDER_COUNT) != 0;
-        unsigned long long *rc6 = dptr<unsigned long long>(c->rcounts);
-        {
-            ProfScope ps(c, "k_trace_q");
-            if (count) k_trace_q<true><<<blocks(m, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(S, q, m, dptr<int4>(c->hits), rc6);
-            else k_trace_q<false><<<blocks(m, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(S, q, m, dptr<int4>(c->hits), rc6);
-        }
-        {
-            ProfScope ps(c, "k_shade");
-            auto kern = (R.gi || R.caust) ? k_shade<true> : k_shade<false>;
-            kern<<<blocks(m, 256), 256, 0, c->stream>>>(S, R, L.PM, L.active, L.c0, L.sample, q, dptr<int4>(c->hits), m,
-                                                        dptr<float4>(c->direct), dptr<SRay>(c->sq),
-                                                        dptr<QRay>(c->kids), dptr<uint8_t>(c->kflag), L.stats,
-                                                        dptr<uint32_t>(c->errf));
-        }
-        size_t ns = (size_t)m * R.L;
-        if (ns) {
-            ProfScope ps(c, "k_trace_shadow");
-            if (count)
-                k_trace_shadow<true><<<blocks(ns, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(
-                    S, dptr<SRay>(c->sq), (uint32_t)ns, dptr<uint8_t>(c->occ), rc6 + 6);
-            else
-                k_trace_shadow<false><<<blocks(ns, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(
-                    S, dptr<SRay>(c->sq), (uint32_t)ns, dptr<uint8_t>(c->occ), rc6 + 6);
-        }
-        {
-            ProfScope ps(c, "k_accum");
-            k_accum<<<blocks(m, 256), 256, 0, c->stream>>>(q, dptr<float4>(c->direct), dptr<SRay>(c->sq),
-                                                          dptr<uint8_t>(c->occ), m, R.L, dptr<float4>(c->acc));
-        }
-        if (b + 1 > R.iter) continue;
-        uint32_t nk = 0;
-        if ((rc = compact(c, dptr<QRay>(c->kids), dptr<uint8_t>(c->kflag), dptr<QRay>(c->qlev[b + 1]),
-                          m * (uint32_t)R.K, &nk)))
-            return rc;
-        HIPCHK(c, hipGetLastError());
-        if (nk) {
-            if ((rc = process_level(L, b + 1, nk))) return rc;
-        }
-    }
-    return SRAY_OK;
-}
-
-int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *d_out, sray_stats *st)
-{
-    int rc = validate_params(c, p);
-    if (rc) return rc;
-    GpuScene *g = find_scene(c, scene_id);
-    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
-    RenderDev R;
-    fill_render_dev(p, R);
-    const SceneDev &S = g->dev;
-    int L = 0;
-    /* shadow slots per ray: phong samples area lights shadow_samples times */
-    std::vector<sray_light_desc> dummy;
-    (void)dummy;
-    {
-        /* light types live on the device; keep a host mirror via the count of
-         * area lights, recomputed from DevLight on upload would need a copy:
-         * use the upper bound lights * max(1, shadow_samples) */
-        L = S.nlights * R.shadow_samples;
-    }
-    R.L = L;
-    R.K = 2 + (R.gi ? R.diffuse_samples : 0);
-    ShadePmaps PM;
-    memset(&PM, 0, sizeof PM);
-    for (int kind = 0; kind < 2; kind++) {
-        bool want = kind == SRAY_PMAP_CAUSTIC ? R.caust != 0 : R.gi != 0;
-        if (!want) continue;
-        auto it = c->pmaps.find(g->pmap[kind]);
-        if (it == c->pmaps.end())
-            return fail(c, SRAY_ERR_NOTFOUND, "render asks for the %s photon map but none is bound to scene %u",
-                        kind == SRAY_PMAP_CAUSTIC ? "caustic" : "global", scene_id);
-        (kind == SRAY_PMAP_CAUSTIC ? PM.caust : PM.glob) = it->second->tree;
-    }
-    /* gather distance relative to the scene octree diagonal (src/scene.cc:384-387) */
-    double gabs = g->diag * (double)p->gather_dist;
-    PM.radius = (float)gabs;
-    PM.scale = gabs > 0.0 ? 1.0 / (M_PI * (gabs * gabs)) : 0.0;
-    uint32_t npix = (uint32_t)R.W * (uint32_t)(p->y1 - p->y0);
-    uint32_t levels = (uint32_t)R.iter + 1;
-    if (c->qlev.size() < levels) c->qlev.resize(levels);
-    for (uint32_t b = 0; b < levels; b++)
-        if ((rc = ensure(c, c->qlev[b], (size_t)CHUNK * R.K * sizeof(QRay)))) return rc;
-    if ((rc = ensure(c, c->hits, (size_t)CHUNK * sizeof(int4)))) return rc;
-    if ((rc = ensure(c, c->direct, (size_t)CHUNK * sizeof(float4)))) return rc;
-    if ((rc = ensure(c, c->sq, (size_t)CHUNK * std::max(L, 1) * sizeof(SRay)))) return rc;
-    if ((rc = ensure(c, c->occ, (size_t)CHUNK * std::max(L, 1)))) return rc;
-    if ((rc = ensure(c, c->kids, (size_t)CHUNK * R.K * sizeof(QRay)))) return rc;
-    if ((rc = ensure(c, c->kflag, (size_t)CHUNK * R.K))) return rc;
-    if ((rc = ensure(c, c->acc, (size_t)CHUNK * sizeof(float4)))) return rc;
-    if ((rc = ensure(c, c->fb, (size_t)npix * sizeof(float4)))) return rc;
-    if ((rc = ensure(c, c->nsamp, (size_t)npix * sizeof(int)))) return rc;
-    if (R.max_samples > 1 && (rc = ensure(c, c->subpix, (size_t)npix * R.max_samples * sizeof(float4)))) return rc;
-    for (int k = 0; k < 2; k++)
-        if ((rc = ensure(c, c->act[k], (size_t)npix * 4))) return rc;
-    if ((rc = ensure(c, c->keep, npix))) return rc;
-    if ((rc = ensure(c, c->stats, 8 * sizeof(unsigned long long)))) return rc;
-    if ((rc = ensure(c, c->errf, 4))) return rc;
-    if ((rc = ensure(c, c->nsel, 4))) return rc;
-    if ((rc = ensure(c, c->rcounts, 12 * sizeof(unsigned long long)))) return rc;
-    HIPCHK(c, hipMemsetAsync(c->rcounts.p, 0, 12 * sizeof(unsigned long long), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->errf.p, 0, 4, c->stream));
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    k_fill_f4<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<float4>(c->fb), npix, make_float4(0, 0, 0, 0));
-    k_iota<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(c->act[0]), npix);
-    uint32_t nact = npix;
-    int cur = 0;
-    uint64_t prim = 0, samples = 0;
-    LevelCtx LC{c, g, R, PM, nullptr, 0, 0, dptr<unsigned long long>(c->stats)};
-    for (int s = 0; s < R.max_samples && nact > 0; s++) {
-        const uint32_t *A = dptr<uint32_t>(c->act[cur]);
-        for (uint32_t c0 = 0; c0 < nact; c0 += CHUNK) {
-            uint32_t m = std::min(CHUNK, nact - c0);
-            k_fill_f4<<<blocks(m, 256), 256, 0, c->stream>>>(dptr<float4>(c->acc), m, make_float4(0, 0, 0, 0));
-            {
-                ProfScope ps(c, "k_raygen");
-                k_raygen<<<blocks(m, 256), 256, 0, c->stream>>>(S, R, A, c0, m, s, dptr<QRay>(c->qlev[0]));
-            }
-            LC.active = A;
-            LC.c0 = c0;
-            LC.sample = s;
-            if ((rc = process_level(LC, 0, m))) return rc;
-            k_pass_finish<<<blocks(m, 256), 256, 0, c->stream>>>(R, A, c0, m, s, dptr<float4>(c->acc),
-                                                                dptr<float4>(c->fb), dptr<float4>(c->subpix),
-                                                                dptr<int>(c->nsamp), dptr<uint8_t>(c->keep));
-            prim += m;
-        }
-        samples += nact;
-        if (s + 1 < R.max_samples) {
-            uint32_t nn = 0;
-            if ((rc = compact_u32(c, A, dptr<uint8_t>(c->keep), dptr<uint32_t>(c->act[cur ^ 1]), nact, &nn))) return rc;
-            nact = nn;
-            cur ^= 1;
-        }
-    }
-    k_finalize<<<blocks(npix, 256), 256, 0, c->stream>>>(R, dptr<float4>(c->fb), dptr<int>(c->nsamp), d_out, npix);
-    if (R.flags & SRAY_RENDER_REF_Q1)
-        k_q1_scale<<<blocks(npix, 256), 256, 0, c->stream>>>(R, dptr<int>(c->nsamp), d_out, npix);
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    prof_resolve(c);
-    unsigned long long hs[8], rcn[12];
-    uint32_t errf = 0;
-    HIPCHK(c, hipMemcpy(hs, c->stats.p, sizeof hs, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(rcn, c->rcounts.p, sizeof rcn, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(&errf, c->errf.p, 4, hipMemcpyDeviceToHost));
-    if (errf) return fail(c, SRAY_ERR_LIMIT, "ior stack deeper than %d media", IOR_DEPTH);
-    if (st) {
-        memset(st, 0, sizeof *st);
-        float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        st->render_ms = ms;
-        st->rays_primary = prim;
-        st->rays_shadow = hs[1];
-        st->rays_reflect = hs[2];
-        st->rays_refract = hs[3];
-        st->rays_diffuse = hs[4];
-        st->samples = samples;
-        sray_trace_counts *tc[2] = {&st->closest, &st->shadow};
-        for (int k = 0; k < 2; k++) {
-            tc[k]->nodes = rcn[k * 6 + 0];
-            tc[k]->spheres = rcn[k * 6 + 1];
-            tc[k]->cylinders = rcn[k * 6 + 2];
-            tc[k]->triangles = rcn[k * 6 + 3];
-            tc[k]->item_refs = rcn[k * 6 + 4];
-            tc[k]->mesh_enters = rcn[k * 6 + 5];
-        }
-    }
     return SRAY_OK;
 }
 
@@ -1648,7 +783,7 @@ int do_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q
     if (k > 0) {
         k_iota<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<uint32_t>(c->giota), n);
         uint32_t nf = 0;
-        if ((rc = compact_u32(c, dptr<uint32_t>(c->giota), dptr<uint8_t>(c->gneed), dptr<uint32_t>(c->gfq), n, &nf)))
+        if ((rc = select_flagged(c, dptr<uint32_t>(c->giota), dptr<uint8_t>(c->gneed), dptr<uint32_t>(c->gfq), n, &nf)))
             return rc;
         if (nf) {
             /* candidate offsets: exclusive scan of the flagged queries' counts */
@@ -1858,6 +993,7 @@ int sray_gpu_create(int device, uint32_t flags, sray_gpu_ctx **out)
 void sray_gpu_destroy(sray_gpu_ctx *c)
 {
     if (!c) return;
+    engine_destroy(c);
     {
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         (void)hipSetDevice(c->device);
@@ -1868,10 +1004,7 @@ void sray_gpu_destroy(sray_gpu_ctx *c)
         for (auto &kv : c->pmaps)
             for (auto &b : kv.second->bufs) release(b);
         c->pmaps.clear();
-        for (auto &b : c->qlev) release(b);
-        DevBuf *bs[] = {&c->hits, &c->direct, &c->sq, &c->occ, &c->kids, &c->kflag, &c->acc, &c->fb, &c->subpix,
-                        &c->nsamp, &c->act[0], &c->act[1], &c->keep, &c->stats, &c->errf, &c->cubtmp, &c->nsel,
-                        &c->rays, &c->hitbuf, &c->counts, &c->rcounts, &c->gq, &c->grgb, &c->gcnt,
+        DevBuf *bs[] = {&c->cubtmp, &c->nsel, &c->rays, &c->hitbuf, &c->counts, &c->gq, &c->grgb, &c->gcnt,
                         &c->gneed, &c->gfq, &c->goff, &c->gcand, &c->giota};
         for (DevBuf *b : bs) release(*b);
         prof_resolve(c);
@@ -1899,6 +1032,8 @@ int sray_gpu_scene_release(sray_gpu_ctx *c, uint32_t scene_id)
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     auto it = c->scenes.find(scene_id);
     if (it == c->scenes.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
+    int rc = engine_scene_busy(c, scene_id);
+    if (rc) return rc;
     (void)hipStreamSynchronize(c->stream);
     for (auto &b : it->second->bufs) release(b);
     c->scenes.erase(it);
@@ -1909,6 +1044,17 @@ int sray_gpu_scene_bytes(sray_gpu_ctx *c, uint32_t scene_id, uint64_t *bytes)
 {
     if (!c || !bytes) return SRAY_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (scene_id == 0) {
+        /* everything the context holds on the device */
+        uint64_t t = engine_bytes(c);
+        for (auto &kv : c->scenes) t += kv.second->bytes;
+        for (auto &kv : c->pmaps) t += kv.second->bytes;
+        DevBuf *bs[] = {&c->cubtmp, &c->nsel, &c->rays, &c->hitbuf, &c->counts, &c->gq, &c->grgb, &c->gcnt,
+                        &c->gneed, &c->gfq, &c->goff, &c->gcand, &c->giota};
+        for (DevBuf *d : bs) t += d->bytes;
+        *bytes = t;
+        return SRAY_OK;
+    }
     GpuScene *g = find_scene(c, scene_id);
     if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
     *bytes = g->bytes;
@@ -1955,54 +1101,6 @@ int sray_gpu_trace(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *rays, uin
         counts->mesh_enters = hc[5];
     }
     return SRAY_OK;
-}
-
-int sray_gpu_primary_rays_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, int sample,
-                                 sray_ray *d_rays)
-{
-    if (!c || !p || !d_rays || sample < 0) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
-    GpuScene *g = find_scene(c, scene_id);
-    if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
-    if (p->width <= 0 || p->height <= 0 || p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height ||
-        p->x0 >= p->x1 || p->y0 >= p->y1)
-        return fail(c, SRAY_ERR_ARG, "bad pixel rectangle");
-    RenderDev R;
-    fill_render_dev(p, R);
-    uint32_t n = (uint32_t)R.W * (uint32_t)(p->y1 - p->y0);
-    k_primary<<<blocks(n, 256), 256, 0, c->stream>>>(g->dev, R, sample, d_rays, n);
-    HIPCHK(c, hipGetLastError());
-    return SRAY_OK;
-}
-
-int sray_gpu_render_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *d_rgba,
-                           sray_stats *st)
-{
-    if (!c || !d_rgba) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
-    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
-    return do_render(c, scene_id, p, d_rgba, st);
-}
-
-int sray_gpu_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *rgba, sray_stats *st)
-{
-    if (!c || !rgba) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
-    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
-    int rc = validate_params(c, p);
-    if (rc) return rc;
-    size_t bytes = (size_t)(p->x1 - p->x0) * (size_t)(p->y1 - p->y0) * 4 * sizeof(float);
-    DevBuf out;
-    hipError_t e = hipMalloc(&out.p, bytes);
-    if (e != hipSuccess) return fail(c, SRAY_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
-    out.bytes = bytes;
-    rc = do_render(c, scene_id, p, (float *)out.p, st);
-    if (rc == SRAY_OK) {
-        e = hipMemcpy(rgba, out.p, bytes, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = fail(c, SRAY_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(e));
-    }
-    release(out);
-    return rc;
 }
 
 int sray_gpu_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *p, uint32_t n, int kind,
@@ -2053,18 +1151,6 @@ int sray_gpu_pmap_download(sray_gpu_ctx *c, uint32_t pmap_id, sray_photon *out, 
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_pmap_download(c, pmap_id, out, cap, n);
-}
-
-int sray_gpu_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *ticket)
-{
-    (void)r, (void)rgba, (void)ticket;
-    return fail(c, SRAY_ERR_ARG, "request serving is not available in this build");
-}
-
-int sray_gpu_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
-{
-    (void)ticket, (void)wait, (void)out;
-    return fail(c, SRAY_ERR_NOTFOUND, "request serving is not available in this build");
 }
 
 int sray_gpu_malloc(sray_gpu_ctx *c, size_t bytes, void **dp)
@@ -2134,14 +1220,6 @@ int sray_gpu_kernel_stats(sray_gpu_ctx *c, sray_kernel_stat *out, int cap, int *
         k++;
     }
     if (reset) c->kstats.clear();
-    return SRAY_OK;
-}
-
-int sray_gpu_set_admission_cap(sray_gpu_ctx *c, uint64_t bytes)
-{
-    if (!c) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
-    c->admission_cap = bytes;
     return SRAY_OK;
 }
 

@@ -203,6 +203,19 @@ struct TraceCounts {
     uint32_t nodes, spheres, cylinders, triangles, item_refs, mesh_enters;
 };
 
+/* adds a lane's visit counts to the six device counters */
+template <bool COUNT>
+__device__ __forceinline__ void flush_counts(const TraceCounts &cnt, unsigned long long *c)
+{
+    if (!COUNT) return;
+    atomicAdd(&c[0], (unsigned long long)cnt.nodes);
+    atomicAdd(&c[1], (unsigned long long)cnt.spheres);
+    atomicAdd(&c[2], (unsigned long long)cnt.cylinders);
+    atomicAdd(&c[3], (unsigned long long)cnt.triangles);
+    atomicAdd(&c[4], (unsigned long long)cnt.item_refs);
+    atomicAdd(&c[5], (unsigned long long)cnt.mesh_enters);
+}
+
 struct HitRec {
     float t;
     int32_t obj;

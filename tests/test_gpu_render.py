@@ -87,3 +87,24 @@ def test_render_deterministic(ctx):
     b, _ = ctx.render(sid, p)
     assert np.array_equal(a, b)
     ctx.release(sid)
+
+
+def test_render_motion_blur(ctx):
+    """moving objects and a camera shutter: sample times drawn per pixel and
+    sample (Camera::get_primary_ray, src/camera.cc:137-139), object matrices
+    at the ray time"""
+    import ctypes as C
+    from sray_amd import abi
+    hs = S.HostScene.generate(3, variant=300)
+    d = abi.SceneDesc()
+    C.memmove(C.byref(d), hs.desc_ptr(), C.sizeof(d))
+    d.camera.shutter = 40
+    p = S.render_params(128, 96, samples=(2, 6), max_var=0.005, iter=3, mblur=1, seed=21)
+    sid = ctx.upload(C.pointer(d))
+    img, st = ctx.render(sid, p)
+    ref = oracle.OracleScene(C.pointer(d), 32)
+    oimg, ost = ref.render(p)
+    e = rmse(img, oimg)
+    print(f"motion blur: rmse {e:.3e}, samples {st.samples}/{ost.samples}")
+    assert e <= RMSE_TOL and st.samples == ost.samples and st.rays_primary == ost.rays_primary
+    ctx.release(sid)
