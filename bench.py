@@ -13,10 +13,14 @@ baseline (the CPU restatement of the reference renderer, double precision,
 -O3 -ffast-math, reference work pattern, on a bounded sample of rows).
 """
 import argparse
+import ctypes
 import json
+import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -26,6 +30,9 @@ WORKLOADS = {
     # name: (config, variant, width, height, iter, samples)
     "heightfield": (1, 0, 1920, 1080, 6, (1, 1)),
     "spheres": (0, 0, 640, 480, 5, (1, 1)),
+    "room": (2, 0, 1280, 720, 5, (1, 16)),
+    "batch": (3, 0, 0, 0, 4, (8, 8)),
+    "serve": (2, 0, 0, 0, 5, (1, 16)),
 }
 
 
@@ -37,6 +44,10 @@ def parse():
     ap.add_argument("--workload", default="heightfield", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--frames", type=int, default=8, help="room: frames in flight")
+    ap.add_argument("--scenes", type=int, default=None, help="batch: scenes (256); serve: resident scenes (32)")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrival rate (req/s)")
+    ap.add_argument("--duration", type=float, default=0.25, help="serve: arrival window (s)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -65,28 +76,273 @@ def read_traffic(workload):
     return None
 
 
-def cpu_baseline(args, hs, wl):
-    """the CPU restatement (reference semantics) timed on this host's cores"""
+def cpu_baseline(args, hs, wl, p=None, full=False):
+    """the CPU restatement (reference semantics) timed on this host's cores:
+    the whole frame (full) or bands of rows around the centre until about
+    --cpu-seconds of work"""
     import oracle
     import sray_amd as S
     config, variant, W, H, it, samples = WORKLOADS[wl]
     nthreads = min(os.cpu_count() or 1, 16)
     ref = oracle.OracleScene(hs.desc_ptr(), 64, oracle.Q9_RETRAVERSE, fast=True)
-    rows, y, rays, elapsed = 0, H // 2 - 24, 0, 0.0
-    band = 16
-    while elapsed < args.cpu_seconds and rows < H:
-        y0 = max(0, min(H - band, y))
-        p = S.render_params(W, H, rect=(0, y0, W, y0 + band), samples=samples, iter=it)
+    band = H if full else 16
+    rows, rays, elapsed = 0, 0, 0.0
+    y0 = 0 if full else max(0, H // 2 - band)
+    while rows < H and (full or elapsed < args.cpu_seconds):
+        y1 = min(H, y0 + band)
+        q = S.render_params(W, H, rect=(0, y0, W, y1), samples=samples, iter=it)
+        if p is not None:
+            for f in ("max_var", "shadow_samples", "diffuse_samples", "caust_photons", "gi_photons",
+                      "gather_dist", "seed"):
+                setattr(q, f, getattr(p, f))
         t0 = time.perf_counter()
-        _, st = ref.render(p, nthreads=nthreads)
+        _, st = ref.render(q, nthreads=nthreads)
         elapsed += time.perf_counter() - t0
         rays += st.rays()
-        rows += band
-        y += band
+        rows += y1 - y0
+        y0 = y1 if y1 < H else 0
     log(args, f"cpu baseline: {rows} rows, {rays} rays, {elapsed:.1f} s, {nthreads} threads")
+    what = "the whole frame" if full else f"{rows} rows x {W} px ({rows * W} pixels) of the {W}x{H} frame"
     return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": nthreads, "kind": "port",
-            "sample": f"{rows} rows x {W} px ({rows * W} pixels, {rays} ray queries) of the {W}x{H} frame, "
-                      f"double precision, -O3 -ffast-math, mesh re-traversal per scene leaf as the reference"}
+            "sample": f"{what}, {rays} ray queries, double precision, -O3 -ffast-math, "
+                      f"mesh re-traversal per scene leaf as the reference"}
+
+
+def pct(v, q):
+    return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else None
+
+
+def base_line(metric, value, steps, warmup, ms, dtype, workload, extra):
+    line = {"metric": metric, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": {"workload": workload}}
+    line.update(extra)
+    return line
+
+
+def look_at(pos, target, vfov_deg, shutter=0):
+    """camera -> world matrix of a camera at pos looking at target (+z forward, +y up)"""
+    import sray_amd as S
+    f = np.asarray(target, np.float64) - np.asarray(pos, np.float64)
+    f /= np.linalg.norm(f)
+    r = np.cross([0.0, 1.0, 0.0], f)
+    r /= np.linalg.norm(r)
+    u = np.cross(f, r)
+    cam = S.abi.CameraDesc()
+    cam.xform[:] = np.array([[r[0], u[0], f[0], pos[0]], [r[1], u[1], f[1], pos[1]],
+                             [r[2], u[2], f[2], pos[2]]], np.float32).ravel()
+    cam.vfov = math.radians(vfov_deg)
+    cam.max_dist = 10000.0
+    cam.shutter = shutter
+    return cam
+
+
+def run_spheres(args):
+    """64 spheres, 640x480, 1 spp, depth 5: plain GPU path against the CPU
+    renderer on the whole frame"""
+    import sray_amd as S
+    config, variant, W, H, it, samples = WORKLOADS["spheres"]
+    hs = S.HostScene.generate(config)
+    ctx = S.Context(0)
+    sid = ctx.upload(hs)
+    p = S.render_params(W, H, samples=samples, iter=it)
+    out = ctx.malloc(W * H * 16)
+    for _ in range(args.warmup):
+        ctx.render_device(sid, p, out)
+    ctx.synchronize()
+    rays, t0 = 0, time.perf_counter()
+    for _ in range(args.steps):
+        rays += ctx.render_device(sid, p, out).rays()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    value = rays / wall / 1e6
+    cpu = None if args.no_cpu else cpu_baseline(args, hs, "spheres", full=True)
+    extra = {"cpu_baseline": cpu, "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None}
+    print(json.dumps(base_line("Mrays/s on 1 GPU", value, args.steps, args.warmup, wall * 1e3 / args.steps, "f32",
+                               f"spheres: 64 spheres, {W}x{H}, 1 spp, depth {it}", extra)), flush=True)
+    ctx.free(out)
+    ctx.close()
+
+
+def room_params(W, H, diag, seed):
+    import sray_amd as S
+    return S.render_params(W, H, samples=(1, 16), max_var=0.05, iter=5, shadow_samples=4, diffuse_samples=1,
+                           caust_photons=500_000, gi_photons=2_000_000, gather_dist=0.1 / diag, seed=seed)
+
+
+def run_room(args):
+    """photon-mapped room, 1280x720, adaptive 1-16 spp, 2M global + 500k
+    caustic photons shot on the device: K frames one after another
+    (plain) against the same K frames in flight together (batched)"""
+    import sray_amd as S
+    config, variant, W, H, it, samples = WORKLOADS["room"]
+    hs = S.HostScene.generate(config, variant=variant)
+    ctx = S.Context(0)
+    sid = ctx.upload(hs)
+    diag = S.scene_diag(hs.desc_ptr())
+    t0 = time.perf_counter()
+    ids, pst = ctx.shoot_photons(sid, 500_000, 2_000_000, 300.0, 5, 0, 0, 0x5EED)
+    shoot_wall = time.perf_counter() - t0
+    log(args, f"photons: shot {pst.shot[0]}+{pst.shot[1]}, stored {pst.stored[0]}+{pst.stored[1]}, "
+              f"{pst.segments} segments, trace {pst.trace_ms:.1f} ms, kd build {pst.build_ms:.1f} ms")
+    K = args.frames
+    frames = [room_params(W, H, diag, 100 + k) for k in range(K)]
+    out = ctx.malloc(W * H * 16)
+    ctx.render_device(sid, frames[0], out)  # warm-up (allocations)
+    ctx.synchronize()
+    rays, t0 = 0, time.perf_counter()
+    for p in frames:
+        rays += ctx.render_device(sid, p, out).rays()
+    ctx.synchronize()
+    plain_wall = time.perf_counter() - t0
+    plain = rays / plain_wall / 1e6
+    ctx.poll(ctx.submit(sid, frames[0]))  # warm-up of the serving engine
+    t0 = time.perf_counter()
+    tickets = [ctx.submit(sid, p) for p in frames]
+    res = [ctx.poll(t)[1] for t in tickets]
+    batch_wall = time.perf_counter() - t0
+    brays = sum(r.rays for r in res)
+    batched = brays / batch_wall / 1e6
+    lat = [r.latency_ms for r in res]
+    log(args, f"room: plain {plain:.2f} Mrays/s ({plain_wall:.2f} s), batched {batched:.2f} Mrays/s "
+              f"({batch_wall:.2f} s), rays/frame {rays // K}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, hs, "room", p=frames[0])
+    extra = {"batched": {"value": round(batched, 3), "frames_in_flight": K, "wall_s": round(batch_wall, 3),
+                         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)},
+             "plain": {"value": round(plain, 3), "wall_s": round(plain_wall, 3)},
+             "batched_vs_plain": round(batched / plain, 3),
+             "photons": {"shot": [int(pst.shot[0]), int(pst.shot[1])],
+                         "stored": [int(pst.stored[0]), int(pst.stored[1])],
+                         "segments": int(pst.segments), "trace_ms": round(pst.trace_ms, 2),
+                         "kd_build_ms": round(pst.build_ms, 2),
+                         "Msegments_per_s": round(pst.segments / (pst.trace_ms * 1e-3) / 1e6, 2),
+                         "wall_s": round(shoot_wall, 3)},
+             "cpu_baseline": cpu}
+    print(json.dumps(base_line("Mrays/s on 1 GPU (batched)", batched, K, 1, batch_wall * 1e3 / K, "f32",
+                               f"room: 12 quads + 2 glass spheres + 50k-triangle icosphere, box light, "
+                               f"{W}x{H}, 1-16 spp (variance 0.05), 2M+500k photons, {K} frames", extra)),
+          flush=True)
+    ctx.free(out)
+    ctx.close()
+
+
+def run_batch(args):
+    """many independent scenes (triangle counts log-uniform in [1k, 200k]
+    plus 8-32 spheres, resolutions 64^2..512^2, half with photon maps of
+    100k-500k photons, 8 motion-blurred time samples per pixel): one after
+    another (plain) against all in flight (batched)"""
+    import sray_amd as S
+    rng = np.random.default_rng(0x5EED0004)
+    n = args.scenes
+    scenes, sids, params = [], [], []
+    t0 = time.perf_counter()
+    ctx = S.Context(0)
+    for k in range(n):
+        ntri = int(math.exp(rng.uniform(math.log(1000), math.log(200_000))))
+        hs = S.HostScene.generate(3, seed=0x5EED0400 + k, variant=ntri)
+        d = S.abi.SceneDesc()
+        ctypes.memmove(ctypes.byref(d), hs.desc_ptr(), ctypes.sizeof(d))
+        d.camera.shutter = 40
+        sid = ctx.upload(ctypes.pointer(d))
+        res = 64 * int(rng.integers(1, 9))
+        photons = k % 2 == 0
+        kw = {}
+        if photons:
+            nph = int(rng.integers(100_000, 500_001))
+            ctx.shoot_photons(sid, nph // 5, nph - nph // 5, 300.0, 5, 0, 40, 0x5EED + k)
+            diag = S.scene_diag(ctypes.pointer(d))
+            kw = dict(caust_photons=1, gi_photons=1, gather_dist=0.1 / diag)
+        p = S.render_params(res, res, samples=(8, 8), iter=4, mblur=1, frame_time=20, seed=k, **kw)
+        scenes.append((hs, d))
+        sids.append(sid)
+        params.append(p)
+    setup = time.perf_counter() - t0
+    log(args, f"batch: {n} scenes set up in {setup:.1f} s")
+    out = ctx.malloc(512 * 512 * 16)
+    ctx.render_device(sids[0], params[0], out)
+    ctx.synchronize()
+    rays, t0 = 0, time.perf_counter()
+    for sid, p in zip(sids, params):
+        rays += ctx.render_device(sid, p, out).rays()
+    ctx.synchronize()
+    plain_wall = time.perf_counter() - t0
+    plain = rays / plain_wall / 1e6
+    ctx.poll(ctx.submit(sids[0], params[0]))
+    t0 = time.perf_counter()
+    tickets = [ctx.submit(sid, p) for sid, p in zip(sids, params)]
+    res = [ctx.poll(t)[1] for t in tickets]
+    batch_wall = time.perf_counter() - t0
+    brays = sum(r.rays for r in res)
+    batched = brays / batch_wall / 1e6
+    lat = [r.latency_ms for r in res]
+    log(args, f"batch: plain {plain:.2f} Mrays/s ({plain_wall:.2f} s) batched {batched:.2f} Mrays/s "
+              f"({batch_wall:.2f} s)")
+    extra = {"batched": {"value": round(batched, 3), "wall_s": round(batch_wall, 3), "p50_ms": pct(lat, 50),
+                         "p99_ms": pct(lat, 99)},
+             "plain": {"value": round(plain, 3), "wall_s": round(plain_wall, 3)},
+             "batched_vs_plain": round(batched / plain, 3), "setup_s": round(setup, 1)}
+    print(json.dumps(base_line("Mrays/s on 1 GPU (batched)", batched, n, 1, batch_wall * 1e3 / n, "f32",
+                               f"batch: {n} scenes (1k-200k triangles + 8-32 spheres, 64^2-512^2, half with "
+                               f"100k-500k photons, 8 motion-blur samples/pixel)", extra)), flush=True)
+    ctx.free(out)
+    ctx.close()
+
+
+def run_serve(args):
+    """Poisson request stream over resident photon-mapped rooms: tiles of
+    64x64 / 256x256 / 1920x1080 (p = .6/.3/.1) through random cameras on a
+    shell around the room centre"""
+    import sray_amd as S
+    rng = np.random.default_rng(0x5EED0005)
+    ctx = S.Context(0)
+    sids, diags = [], []
+    t0 = time.perf_counter()
+    for k in range(args.scenes):
+        hs = S.HostScene.generate(2, seed=k)
+        sid = ctx.upload(hs)
+        ctx.shoot_photons(sid, 500_000, 2_000_000, 300.0, 5, 0, 0, 0x5EED + k)
+        diags.append(S.scene_diag(hs.desc_ptr()))
+        sids.append(sid)
+    setup = time.perf_counter() - t0
+    sizes = [(64, 64), (256, 256), (1920, 1080)]
+    n = int(args.rate * args.duration)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    reqs = []
+    for k in range(n):
+        sc = int(rng.integers(0, len(sids)))
+        w, h = sizes[int(rng.choice(3, p=[0.6, 0.3, 0.1]))]
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        pos = np.array([0.0, 4.0, 0.0]) + 3.5 * v
+        cam = look_at(pos, (0.0, 4.0, 0.0), 60.0)
+        p = S.render_params(w, h, samples=(1, 16), max_var=0.05, iter=5, shadow_samples=4, caust_photons=1,
+                            gi_photons=1, gather_dist=0.1 / diags[sc], seed=k)
+        reqs.append((sids[sc], p, cam))
+    ctx.poll(ctx.submit(*reqs[0]))  # warm-up
+    tickets = []
+    t0 = time.perf_counter()
+    for k, r in enumerate(reqs):
+        wait = arrivals[k] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        tickets.append(ctx.submit(*r))
+    res = [ctx.poll(t)[1] for t in tickets]
+    wall = time.perf_counter() - t0
+    rays = sum(r.rays for r in res)
+    lat = [r.latency_ms for r in res]
+    value = rays / wall / 1e6
+    log(args, f"serve: {n} requests in {wall:.2f} s, p50 {pct(lat, 50):.1f} ms p99 {pct(lat, 99):.1f} ms")
+    extra = {"latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
+             "queue_ms": {"p50": pct([r.queue_ms for r in res], 50), "p99": pct([r.queue_ms for r in res], 99)},
+             "requests": n, "offered_rate_per_s": args.rate, "arrival_window_s": args.duration,
+             "completed_rate_per_s": round(n / wall, 1), "setup_s": round(setup, 1)}
+    print(json.dumps(base_line("Mrays/s on 1 GPU (served)", value, n, 1, wall * 1e3 / n, "f32",
+                               f"serve: Poisson {args.rate:g} req/s for {args.duration:g} s over "
+                               f"{args.scenes} resident photon-mapped rooms (2M+500k photons each), tiles "
+                               f"64^2/256^2/1920x1080 at .6/.3/.1, 1-16 spp", extra)), flush=True)
+    ctx.close()
 
 
 def main():
@@ -94,6 +350,10 @@ def main():
     if args.gpus > 1:
         sys.exit("bench.py measures one GPU only: every box here has one MI355X and no multi-GPU "
                  "figure is produced (run with --gpus 1)")
+    if args.scenes is None:
+        args.scenes = 256 if args.workload == "batch" else 32
+    if args.workload != "heightfield":
+        return {"spheres": run_spheres, "room": run_room, "batch": run_batch, "serve": run_serve}[args.workload](args)
     import sray_amd as S
     wl = args.workload
     config, variant, W, H, it, samples = WORKLOADS[wl]

@@ -125,6 +125,17 @@ def octree_host(desc_ptr, level):
     return info, lo, hi, items[:info.num_item_refs]
 
 
+def scene_diag(desc_ptr):
+    """diagonal of the scene octree's root box (Scene::build_tree,
+    src/scene.cc:384-387), the unit of render_params.gather_dist"""
+    info = OctreeInfo()
+    rc = lib().sray_octree_build_host(desc_ptr, -1, C.byref(info), None, None, None, 0, 0)
+    if rc:
+        raise SrayError(rc, "octree_build_host")
+    d = [float(info.root_max[k]) - float(info.root_min[k]) for k in range(3)]
+    return float(np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]))
+
+
 # ---------------------------------------------------------------------------
 # numpy <-> struct arrays
 # ---------------------------------------------------------------------------

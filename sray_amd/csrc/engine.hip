@@ -680,7 +680,7 @@ struct Job {
     float *dev_out = nullptr;  /* to device memory */
     uint32_t slot = 0;
     uint64_t pix_base = 0, npix = 0, sub_base = 0, nsub = 0;
-    uint64_t rays = 0;
+    uint64_t rays = 0, prim = 0; /* ray queries of all kinds / primary rays */
     int state = 0; /* 0 waiting, 1 in flight, 2 done */
     int rc = SRAY_OK;
     std::string err;
@@ -708,6 +708,7 @@ struct Engine {
     /* this pass's launch shape */
     int L = 0, K = 2, maxiter = 0;
     bool photons = false, uni = true, counting = false;
+    unsigned long long sec_seen = 0; /* secondary rays counted before this pass */
     /* submit / poll */
     std::thread worker;
     std::mutex qmu;
@@ -1071,10 +1072,16 @@ int run_pass(Engine &E, std::vector<Job *> &finished)
     }
     uint32_t errf = 0;
     HIPCHK(c, hipMemcpy(&errf, E.errf.p, 4, hipMemcpyDeviceToHost));
+    unsigned long long hs[8];
+    HIPCHK(c, hipMemcpy(hs, E.stats.p, sizeof hs, hipMemcpyDeviceToHost));
+    unsigned long long sec = hs[1] + hs[2] + hs[3] + hs[4];
+    if (E.uni) srays[0] = sec - E.sec_seen;
+    E.sec_seen = sec;
     for (uint32_t s = 0; s < E.hi; s++) {
         Job *j = E.slot_job[s];
         if (!j) continue;
-        j->rays += E.slot_active[s] + (E.uni ? 0ull : srays[s]);
+        j->prim += E.slot_active[s];
+        j->rays += E.slot_active[s] + srays[s];
         if (errf) j->limit_hit = true;
         E.slot_active[s] = cnt[s];
         if (cnt[s]) continue;
@@ -1133,6 +1140,7 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
     HIPCHK(c, hipMemsetAsync(E.rcounts.p, 0, 12 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemsetAsync(E.stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemsetAsync(E.errf.p, 0, 4, c->stream));
+    E.sec_seen = 0;
     E.counting = (p->flags & SRAY_RENDER_COUNT) != 0;
     Job j;
     j.scene_id = scene_id;
@@ -1153,7 +1161,7 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
         abort_all(E);
         return rc;
     }
-    prim = j.rays; /* uni: one primary ray per active pixel per pass */
+    prim = j.prim;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -1199,6 +1207,8 @@ void serve_loop(sray_gpu_ctx *c, Engine *E)
             E->qcv.wait(lk, [&] { return E->stop || !E->pending.empty() || E->live > 0; });
             if (E->stop) break;
         }
+        /* API calls waiting for the context (submit, upload, ...) go first */
+        while (c->api_waiting.load() > 0) std::this_thread::yield();
         std::lock_guard<std::recursive_mutex> g(c->mu);
         /* admission: first come first served while the requests fit the cap */
         for (;;) {
@@ -1249,7 +1259,7 @@ int do_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *tic
     j->host_out = rgba;
     int rc;
     {
-        std::lock_guard<std::recursive_mutex> g(c->mu);
+        ApiLock g(c);
         if ((rc = validate_params(c, &r->params))) return rc;
         if (!find_scene(c, r->scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", r->scene_id);
         if (j->use_cam && !(r->camera.vfov > 0.0)) return fail(c, SRAY_ERR_ARG, "request camera: vfov must be > 0");
@@ -1356,7 +1366,7 @@ int sray_gpu_primary_rays_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_
                                  sray_ray *d_rays)
 {
     if (!c || !p || !d_rays || sample < 0) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     GpuScene *g = find_scene(c, scene_id);
     if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
     if (p->width <= 0 || p->height <= 0 || p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height ||
@@ -1374,7 +1384,7 @@ int sray_gpu_render_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_render
                            sray_stats *st)
 {
     if (!c || !d_rgba) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_render(c, scene_id, p, d_rgba, st);
 }
@@ -1382,7 +1392,7 @@ int sray_gpu_render_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_render
 int sray_gpu_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *rgba, sray_stats *st)
 {
     if (!c || !rgba) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     int rc = validate_params(c, p);
     if (rc) return rc;
@@ -1415,7 +1425,7 @@ int sray_gpu_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
 int sray_gpu_set_admission_cap(sray_gpu_ctx *c, uint64_t bytes)
 {
     if (!c) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     c->admission_cap = bytes;
     return SRAY_OK;
 }

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_select.hpp>
 
+#include <atomic>
 #include <condition_variable>
 #include <map>
 #include <memory>
@@ -61,6 +62,7 @@ struct sray_gpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
+    std::atomic<int> api_waiting{0}; /* callers waiting for mu (served before the next render pass) */
     std::string err;
     std::map<uint32_t, std::unique_ptr<srt::GpuScene>> scenes;
     uint32_t next_scene = 1;
@@ -89,6 +91,21 @@ struct sray_gpu_ctx {
 };
 
 namespace srt {
+
+/* the context lock as taken by API calls: a caller waiting here is served
+ * before the serving thread starts its next render pass */
+struct ApiLock {
+    sray_gpu_ctx *c;
+    explicit ApiLock(sray_gpu_ctx *cc) : c(cc)
+    {
+        c->api_waiting++;
+        c->mu.lock();
+        c->api_waiting--;
+    }
+    ~ApiLock() { c->mu.unlock(); }
+    ApiLock(const ApiLock &) = delete;
+    ApiLock &operator=(const ApiLock &) = delete;
+};
 
 int fail(sray_gpu_ctx *c, int code, const char *fmt, ...);
 

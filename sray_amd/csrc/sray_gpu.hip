@@ -995,7 +995,7 @@ void sray_gpu_destroy(sray_gpu_ctx *c)
     if (!c) return;
     engine_destroy(c);
     {
-        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        ApiLock lk(c);
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
         for (auto &kv : c->scenes)
@@ -1021,7 +1021,7 @@ const char *sray_gpu_last_error(sray_gpu_ctx *c) { return c ? c->err.c_str() : "
 int sray_gpu_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_id)
 {
     if (!c || !scene_id) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_scene_upload(c, s, scene_id);
 }
@@ -1029,7 +1029,7 @@ int sray_gpu_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *s
 int sray_gpu_scene_release(sray_gpu_ctx *c, uint32_t scene_id)
 {
     if (!c) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     auto it = c->scenes.find(scene_id);
     if (it == c->scenes.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
     int rc = engine_scene_busy(c, scene_id);
@@ -1043,7 +1043,7 @@ int sray_gpu_scene_release(sray_gpu_ctx *c, uint32_t scene_id)
 int sray_gpu_scene_bytes(sray_gpu_ctx *c, uint32_t scene_id, uint64_t *bytes)
 {
     if (!c || !bytes) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (scene_id == 0) {
         /* everything the context holds on the device */
         uint64_t t = engine_bytes(c);
@@ -1065,7 +1065,7 @@ int sray_gpu_trace_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *d_
                           uint32_t flags)
 {
     if (!c || (n && (!d_rays || !d_hits))) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     return do_trace_device(c, scene_id, d_rays, n, d_hits, flags & ~SRAY_TRACE_COUNT, nullptr);
 }
 
@@ -1073,7 +1073,7 @@ int sray_gpu_trace(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *rays, uin
                    uint32_t flags, sray_trace_counts *counts)
 {
     if (!c || (n && (!rays || !hits))) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     if (counts) memset(counts, 0, sizeof *counts);
     if (!find_scene(c, scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
@@ -1107,7 +1107,7 @@ int sray_gpu_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *
                          uint32_t *pmap_id)
 {
     if (!c || !pmap_id) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_pmap_build(c, scene_id, p, false, n, kind, pmap_id);
 }
@@ -1115,7 +1115,7 @@ int sray_gpu_pmap_upload(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *
 int sray_gpu_pmap_release(sray_gpu_ctx *c, uint32_t pmap_id)
 {
     if (!c) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     auto it = c->pmaps.find(pmap_id);
     if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
     (void)hipStreamSynchronize(c->stream);
@@ -1128,7 +1128,7 @@ int sray_gpu_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_qu
                          int mode, float *rgb)
 {
     if (!c || (n && (!q || !rgb))) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_pmap_gather(c, pmap_id, q, n, radius, k, mode, rgb);
 }
@@ -1137,7 +1137,7 @@ int sray_gpu_shoot_photons(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon
                            uint32_t *global, sray_photon_stats *st)
 {
     if (!c || !pp) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     uint32_t ids[2] = {0, 0};
     int rc = do_shoot(c, scene_id, pp, ids, st);
     if (caustic) *caustic = ids[0];
@@ -1148,7 +1148,7 @@ int sray_gpu_shoot_photons(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon
 int sray_gpu_pmap_download(sray_gpu_ctx *c, uint32_t pmap_id, sray_photon *out, uint32_t cap, uint32_t *n)
 {
     if (!c || !n) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_pmap_download(c, pmap_id, out, cap, n);
 }
@@ -1156,7 +1156,7 @@ int sray_gpu_pmap_download(sray_gpu_ctx *c, uint32_t pmap_id, sray_photon *out, 
 int sray_gpu_malloc(sray_gpu_ctx *c, size_t bytes, void **dp)
 {
     if (!c || !dp) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     hipError_t e = hipMalloc(dp, bytes ? bytes : 16);
     if (e != hipSuccess) {
@@ -1169,7 +1169,7 @@ int sray_gpu_malloc(sray_gpu_ctx *c, size_t bytes, void **dp)
 int sray_gpu_free(sray_gpu_ctx *c, void *dp)
 {
     if (!c) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     if (!dp) return SRAY_OK;
     (void)hipStreamSynchronize(c->stream);
     HIPCHK(c, hipFree(dp));
@@ -1179,7 +1179,7 @@ int sray_gpu_free(sray_gpu_ctx *c, void *dp)
 int sray_gpu_memcpy(sray_gpu_ctx *c, void *dst, const void *src, size_t bytes, int kind)
 {
     if (!c || (bytes && (!dst || !src)) || kind < 0 || kind > 2) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, k, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1189,7 +1189,7 @@ int sray_gpu_memcpy(sray_gpu_ctx *c, void *dst, const void *src, size_t bytes, i
 int sray_gpu_synchronize(sray_gpu_ctx *c)
 {
     if (!c) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipDeviceSynchronize());
     return SRAY_OK;
@@ -1198,7 +1198,7 @@ int sray_gpu_synchronize(sray_gpu_ctx *c)
 int sray_gpu_set_profiling(sray_gpu_ctx *c, int on)
 {
     if (!c) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     c->profiling = on != 0;
     return SRAY_OK;
 }
@@ -1206,7 +1206,7 @@ int sray_gpu_set_profiling(sray_gpu_ctx *c, int on)
 int sray_gpu_kernel_stats(sray_gpu_ctx *c, sray_kernel_stat *out, int cap, int *n, int reset)
 {
     if (!c || !n) return SRAY_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    ApiLock lk(c);
     (void)hipStreamSynchronize(c->stream);
     prof_resolve(c);
     *n = (int)c->kstats.size();

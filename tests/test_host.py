@@ -184,3 +184,12 @@ def test_empty_scene_tree():
     h = S.HostScene.load_xml(text="<scene/>")
     info, lo, hi, items = S.octree_host(h.desc_ptr(), -1)
     assert info.num_nodes == 1 and info.num_item_refs == 0
+
+
+def test_scene_diag_equals_checker():
+    """the gather-distance unit (scene octree root diagonal) from the host
+    octree build equals the checker's"""
+    import oracle
+    for cfg, var in ((0, 0), (2, 10), (3, 500)):
+        hs = S.HostScene.generate(cfg, variant=var)
+        assert S.scene_diag(hs.desc_ptr()) == oracle.OracleScene(hs.desc_ptr(), 32).diag()
