@@ -223,6 +223,10 @@ typedef struct sray_photon {
 #define SRAY_PMAP_GLOBAL     1
 #define SRAY_GATHER_IRRADIANCE 0   /* PhotonMap::irradiance_est, src/pmap.cc:109-139 */
 #define SRAY_GATHER_RADIANCE   1   /* PhotonMap::radiance_est,  src/pmap.cc:143-178 */
+/* or-ed into the gather mode: fixed-radius gather over a uniform grid of
+ * radius-sized cells (built per radius on first use, the structure the
+ * renderer's shading gathers use) instead of the kd-tree; same found set */
+#define SRAY_GATHER_GRID       0x10
 
 typedef struct sray_gather_query {
     float pos[3];

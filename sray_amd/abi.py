@@ -20,6 +20,7 @@ SRAY_RENDER_REF_Q1 = 1
 SRAY_RENDER_COUNT = 2
 SRAY_PMAP_CAUSTIC, SRAY_PMAP_GLOBAL = 0, 1
 SRAY_GATHER_IRRADIANCE, SRAY_GATHER_RADIANCE = 0, 1
+SRAY_GATHER_GRID = 0x10
 
 RAY_MAGNITUDE = 10000.0
 ERROR_MARGIN = 1e-6

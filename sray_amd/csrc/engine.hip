@@ -184,26 +184,16 @@ __device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uin
 }
 
 /* shade_phong (src/shader.cc:83-247) / shade_undef (:36-71) for one hit,
- * emitting direct terms, shadow rays and children */
-__device__ __forceinline__ V3 gather_est(const PmapTree &T, V3 pos, V3 n, float radius, double scale, int mode,
-                                         uint2 *stk)
-{
-    GatherAcc a;
-    gather_range(T, pos.x, pos.y, pos.z, radius, mode, (double)n.x, (double)n.y, (double)n.z, a, stk, 256);
-    if (mode == SRAY_GATHER_IRRADIANCE && a.count < 8) return v3(0.0f, 0.0f, 0.0f);
-    return v3((float)(a.r * scale), (float)(a.g * scale), (float)(a.b * scale));
-}
-
+ * emitting direct terms, shadow rays, children and photon-gather requests */
 template <bool PHOTONS, bool UNI>
 __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restrict__ active, uint32_t c0,
                                                const QRay *__restrict__ q, const int4 *__restrict__ hits, uint32_t n,
                                                int L, int K, float4 *__restrict__ direct, SRay *__restrict__ sq,
                                                QRay *__restrict__ kids, uint8_t *__restrict__ kflag,
-                                               unsigned long long *__restrict__ stats,
+                                               GatherReq *__restrict__ greq, unsigned long long *__restrict__ stats,
                                                unsigned long long *__restrict__ slot_rays, uint32_t *__restrict__ err)
 {
     __shared__ uint32_t bcnt[5];
-    __shared__ uint2 gstk[PHOTONS ? PMAP_STACK * 256 : 1];
     if (threadIdx.x < 5) bcnt[threadIdx.x] = 0;
     __syncthreads();
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -215,7 +205,6 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
         uint32_t rq = UNI ? 0u : R.m.w >> SLOT_SHIFT;
         const ReqDev &Q = T.reqs[rq];
         const RenderDev &P = Q.P;
-        const ShadePmaps &PM = Q.PM;
         const SceneDev S = T.scenes[UNI ? 0u : Q.scene];
         uint32_t pr = active[c0 + slot];
         uint32_t lp = pr - Q.pix_base;
@@ -227,6 +216,7 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
         V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
         float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         for (int l = 0; l < L; l++) sq[(size_t)i * L + l].d.w = 0.0f;
+        if (PHOTONS) greq[i].pos = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
         for (int k = 0; k < K; k++) kflag[(size_t)i * K + k] = 0;
         if (h.y < 0) {
             /* miss: env_color (src/scene.cc:568) */
@@ -292,12 +282,18 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
                 if (flg & QF_DIFFUSE) {
                     /* diffuse GI ray: radiance estimate from the global map
                      * (src/shader.cc:119-123) */
-                    V3 rad = PHOTONS ? gather_est(PM.glob, Pw, normal, PM.radius, PM.scale, SRAY_GATHER_RADIANCE,
-                                                  &gstk[threadIdx.x])
-                                     : v3(0.0f, 0.0f, 0.0f);
-                    dir.x = W.x * (rad.x * kd.x);
-                    dir.y = W.y * (rad.y * kd.y);
-                    dir.z = W.z * (rad.z * kd.z);
+                    if (PHOTONS) {
+                        GatherReq g;
+                        g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(2u));
+                        g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
+                        g.w = make_float4(W.x, W.y, W.z, 0.0f);
+                        g.kd = kd;
+                        greq[i] = g;
+                    } else {
+                        dir.x = W.x * (0.0f * kd.x);
+                        dir.y = W.y * (0.0f * kd.y);
+                        dir.z = W.z * (0.0f * kd.z);
+                    }
                     direct[i] = dir;
                 } else {
                     float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
@@ -370,12 +366,14 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
                         }
                     }
                     if (PHOTONS && P.caust) {
-                        /* caustics: irradiance from the caustic map (src/shader.cc:231-236) */
-                        V3 irr = gather_est(PM.caust, Pw, normal, PM.radius, PM.scale, SRAY_GATHER_IRRADIANCE,
-                                            &gstk[threadIdx.x]);
-                        dir.x = dir.x + W.x * irr.x;
-                        dir.y = dir.y + W.y * irr.y;
-                        dir.z = dir.z + W.z * irr.z;
+                        /* caustics: irradiance from the caustic map (src/shader.cc:231-236),
+                         * added by k_gather */
+                        GatherReq g;
+                        g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(1u));
+                        g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
+                        g.w = make_float4(W.x, W.y, W.z, 0.0f);
+                        g.kd = kd;
+                        greq[i] = g;
                     }
                     V3 nrd = dvs(neg(rd), ray_mag);
                     float ray_dot_n = dot(nrd, normal);
@@ -432,6 +430,43 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
     if (c_df) atomicAdd(&bcnt[3], c_df);
     __syncthreads();
     if (threadIdx.x < 4 && bcnt[threadIdx.x]) atomicAdd(&stats[1 + threadIdx.x], (unsigned long long)bcnt[threadIdx.x]);
+}
+
+/* photon gathers requested by k_shade: caustic irradiance (>= 8 photons,
+ * src/pmap.cc:109-139) added to the direct term, or the radiance estimate
+ * (src/pmap.cc:141-178) of a diffuse GI ray times kd */
+template <bool UNI>
+__global__ __launch_bounds__(128) void k_gather(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
+                                                uint32_t n, float4 *__restrict__ direct)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 pos = greq[i].pos;
+    uint32_t mode = __float_as_uint(pos.w);
+    if (mode == 0u) return;
+    uint32_t rq = UNI ? 0u : q[i].m.w >> SLOT_SHIFT;
+    const ShadePmaps &PM = T.reqs[rq].PM;
+    float4 nn = greq[i].nrm, w = greq[i].w;
+    GatherAcc a;
+    float4 d = direct[i];
+    if (mode == 1u) {
+        gather_grid(PM.caust, pos.x, pos.y, pos.z, PM.radius, SRAY_GATHER_IRRADIANCE, (double)nn.x, (double)nn.y,
+                    (double)nn.z, a);
+        V3 irr = a.count < 8 ? v3(0.0f, 0.0f, 0.0f)
+                             : v3((float)(a.r * PM.scale), (float)(a.g * PM.scale), (float)(a.b * PM.scale));
+        d.x = d.x + w.x * irr.x;
+        d.y = d.y + w.y * irr.y;
+        d.z = d.z + w.z * irr.z;
+    } else {
+        gather_grid(PM.glob, pos.x, pos.y, pos.z, PM.radius, SRAY_GATHER_RADIANCE, (double)nn.x, (double)nn.y,
+                    (double)nn.z, a);
+        V3 rad = v3((float)(a.r * PM.scale), (float)(a.g * PM.scale), (float)(a.b * PM.scale));
+        float4 kd = greq[i].kd;
+        d.x = w.x * (rad.x * kd.x);
+        d.y = w.y * (rad.y * kd.y);
+        d.z = w.z * (rad.z * kd.z);
+    }
+    direct[i] = d;
 }
 
 /* per-sample accumulation: the first ray of each run of equal sample slots
@@ -681,6 +716,7 @@ struct Job {
     uint32_t slot = 0;
     uint64_t pix_base = 0, npix = 0, sub_base = 0, nsub = 0;
     uint64_t rays = 0, prim = 0; /* ray queries of all kinds / primary rays */
+    uint32_t pmaps[2] = {0, 0};  /* photon maps read while in flight */
     int state = 0; /* 0 waiting, 1 in flight, 2 done */
     int rc = SRAY_OK;
     std::string err;
@@ -698,7 +734,8 @@ struct Engine {
     uint32_t nact = 0;
     /* ray queues and per-level scratch */
     std::vector<DevBuf> qlev;
-    DevBuf hits, direct, sq, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs, dscenes, stage;
+    DevBuf hits, direct, sq, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs, dscenes, stage,
+        greq;
     /* requests in flight */
     std::vector<ReqDev> hreqs = std::vector<ReqDev>(MAX_SLOTS);
     std::vector<SceneDev> hscenes;
@@ -727,7 +764,7 @@ struct Engine {
     {
         uint64_t b = 0;
         const DevBuf *bs[] = {&hits, &direct, &sq, &occ, &kids, &kflag, &acc, &stats, &errf, &rcounts,
-                              &slotcnt, &slotrays, &dreqs, &dscenes, &stage};
+                              &slotcnt, &slotrays, &dreqs, &dscenes, &stage, &greq};
         for (const DevBuf *d : bs) b += d->bytes;
         for (const DevBuf &d : qlev) b += d.bytes;
         return b;
@@ -842,16 +879,21 @@ int admit(Engine &E, Job *j)
     for (int kind = 0; kind < 2; kind++) {
         bool want = kind == SRAY_PMAP_CAUSTIC ? Q.P.caust != 0 : Q.P.gi != 0;
         if (!want) continue;
-        auto it = c->pmaps.find(g->pmap[kind]);
-        if (it == c->pmaps.end())
+        if (c->pmaps.find(g->pmap[kind]) == c->pmaps.end())
             return fail(c, SRAY_ERR_NOTFOUND, "render asks for the %s photon map but none is bound to scene %u",
                         kind == SRAY_PMAP_CAUSTIC ? "caustic" : "global", j->scene_id);
-        (kind == SRAY_PMAP_CAUSTIC ? Q.PM.caust : Q.PM.glob) = it->second->tree;
     }
     /* gather distance relative to the scene octree diagonal (src/scene.cc:384-387) */
     double gabs = g->diag * (double)j->p.gather_dist;
     Q.PM.radius = (float)gabs;
     Q.PM.scale = gabs > 0.0 ? 1.0 / (M_PI * (gabs * gabs)) : 0.0;
+    for (int kind = 0; kind < 2; kind++) {
+        bool want = kind == SRAY_PMAP_CAUSTIC ? Q.P.caust != 0 : Q.P.gi != 0;
+        if (want && (rc = pmap_grid_get(c, g->pmap[kind], Q.PM.radius,
+                                        kind == SRAY_PMAP_CAUSTIC ? &Q.PM.caust : &Q.PM.glob)))
+            return rc;
+        j->pmaps[kind] = want ? g->pmap[kind] : 0u;
+    }
     uint64_t npix = (uint64_t)Q.P.W * (uint64_t)(Q.P.y1 - Q.P.y0);
     uint64_t maxs = (uint64_t)Q.P.max_samples;
     uint64_t base = 0, sbase = 0;
@@ -983,8 +1025,14 @@ int process_level(Level &LV, int b, uint32_t count)
                                   : (E.uni ? k_shade<false, true> : k_shade<false, false>);
             kern<<<blocks(m, 256), 256, 0, c->stream>>>(LV.T, LV.active, LV.c0, q, dptr<int4>(E.hits), m, E.L, E.K,
                                                         dptr<float4>(E.direct), dptr<SRay>(E.sq),
-                                                        dptr<QRay>(E.kids), dptr<uint8_t>(E.kflag), stats, srays,
+                                                        dptr<QRay>(E.kids), dptr<uint8_t>(E.kflag),
+                                                        dptr<GatherReq>(E.greq), stats, srays,
                                                         dptr<uint32_t>(E.errf));
+        }
+        if (E.photons) {
+            ProfScope ps(c, "k_gather");
+            auto kern = E.uni ? k_gather<true> : k_gather<false>;
+            kern<<<blocks(m, 128), 128, 0, c->stream>>>(LV.T, q, dptr<GatherReq>(E.greq), m, dptr<float4>(E.direct));
         }
         size_t ns = (size_t)m * E.L;
         if (ns) {
@@ -1027,6 +1075,7 @@ int run_pass(Engine &E, std::vector<Job *> &finished)
         (rc = ensure(c, E.occ, (size_t)CHUNK * std::max(E.L, 1))) ||
         (rc = ensure(c, E.kids, (size_t)CHUNK * E.K * sizeof(QRay))) || (rc = ensure(c, E.kflag, (size_t)CHUNK * E.K)) ||
         (rc = ensure(c, E.acc, (size_t)CHUNK * 16)) || (rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) ||
+        (E.photons && (rc = ensure(c, E.greq, (size_t)CHUNK * sizeof(GatherReq)))) ||
         (rc = ensure(c, E.slotrays, MAX_SLOTS * 8)) || (rc = ensure(c, c->nsel, 4)))
         return rc;
     if (E.sub.cap == 0 && (rc = ensure(c, E.subpix, 16))) return rc;
@@ -1122,7 +1171,7 @@ void engine_free(Engine *E)
     if (!E) return;
     DevBuf *bs[] = {&E->fb, &E->nsamp, &E->preq, &E->keep, &E->act[0], &E->act[1], &E->subpix, &E->hits,
                     &E->direct, &E->sq, &E->occ, &E->kids, &E->kflag, &E->acc, &E->stats, &E->errf, &E->rcounts,
-                    &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->stage};
+                    &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->stage, &E->greq};
     for (DevBuf *b : bs) release(*b);
     for (DevBuf &b : E->qlev) release(b);
     delete E;
@@ -1342,6 +1391,16 @@ void engine_destroy(sray_gpu_ctx *c)
 uint64_t engine_bytes(sray_gpu_ctx *c)
 {
     return (c->plain ? c->plain->bytes() : 0) + (c->serve ? c->serve->bytes() : 0);
+}
+
+int engine_pmap_busy(sray_gpu_ctx *c, uint32_t pmap_id)
+{
+    Engine *E = c->serve;
+    if (!E || !pmap_id) return SRAY_OK;
+    for (uint32_t s = 0; s < E->hi; s++)
+        if (E->slot_job[s] && (E->slot_job[s]->pmaps[0] == pmap_id || E->slot_job[s]->pmaps[1] == pmap_id))
+            return fail(c, SRAY_ERR_PENDING, "photon map %u is read by requests in flight", pmap_id);
+    return SRAY_OK;
 }
 
 int engine_scene_busy(sray_gpu_ctx *c, uint32_t scene_id)

@@ -104,6 +104,53 @@ __device__ __forceinline__ void gather_range(const PmapTree &T, float qx, float 
     }
 }
 
+/* Uniform grid over a photon map for fixed-radius gathers in shading: the
+ * photons sorted by cell (x fastest), cell_start[c] = first photon of cell
+ * c (ncells + 1 entries).  A query with radius r scans, for every (y, z)
+ * row of cells its box meets, the one contiguous photon range of the x
+ * run, with the same inclusion test and estimate as the kd-tree gather,
+ * so the found set is the same. */
+struct PmapGrid {
+    PmapTree v;               /* photon arrays in cell order (axis unused) */
+    const uint32_t *cell_start;
+    float ox, oy, oz;         /* grid origin */
+    float inv_cs;             /* 1 / cell size */
+    int32_t dx, dy, dz;
+};
+
+__device__ __forceinline__ int grid_cell(float x, float o, float inv, int d)
+{
+    float f = floorf((x - o) * inv);
+    if (!(f > 0.0f)) return 0; /* below the grid (or NaN) */
+    return f > (float)(d - 1) ? d - 1 : (int)f;
+}
+
+__device__ __forceinline__ void gather_grid(const PmapGrid &G, float qx, float qy, float qz, float radius, int mode,
+                                            double nx, double ny, double nz, GatherAcc &a)
+{
+    a.r = a.g = a.b = 0.0;
+    a.count = 0;
+    if (G.v.n == 0) return;
+    double r2 = (double)radius * (double)radius;
+    /* the cell range of the query box, widened so float rounding of the
+     * box ends cannot drop a cell holding an included photon */
+    float mx = 1e-5f * (fabsf(qx) + radius), my = 1e-5f * (fabsf(qy) + radius), mz = 1e-5f * (fabsf(qz) + radius);
+    int x0 = grid_cell(qx - radius - mx, G.ox, G.inv_cs, G.dx), x1 = grid_cell(qx + radius + mx, G.ox, G.inv_cs, G.dx);
+    int y0 = grid_cell(qy - radius - my, G.oy, G.inv_cs, G.dy), y1 = grid_cell(qy + radius + my, G.oy, G.inv_cs, G.dy);
+    int z0 = grid_cell(qz - radius - mz, G.oz, G.inv_cs, G.dz), z1 = grid_cell(qz + radius + mz, G.oz, G.inv_cs, G.dz);
+    for (int z = z0; z <= z1; z++)
+        for (int y = y0; y <= y1; y++) {
+            uint32_t row = ((uint32_t)z * (uint32_t)G.dy + (uint32_t)y) * (uint32_t)G.dx;
+            uint32_t b = G.cell_start[row + (uint32_t)x0], e = G.cell_start[row + (uint32_t)x1 + 1u];
+            for (uint32_t m = b; m < e; m++) {
+                float4 p = G.v.pos[m];
+                double ddx = (double)p.x - (double)qx, ddy = (double)p.y - (double)qy, ddz = (double)p.z - (double)qz;
+                double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (d2 <= r2) gather_photon(G.v, m, mode, nx, ny, nz, a);
+            }
+        }
+}
+
 } // namespace srd
 
 #endif

@@ -48,9 +48,20 @@ struct RenderDev {
 
 /* photon maps bound for shading (caustic irradiance, GI radiance) */
 struct ShadePmaps {
-    PmapTree caust, glob;
+    PmapGrid caust, glob;
     float radius; /* inclusion radius (float, as kd_nearest_range3f) */
     double scale; /* 1 / (pi r^2) with the double gather distance */
+};
+
+/* a photon gather a shaded ray needs, done by its own kernel after shading:
+ * mode 1 adds W * caustic irradiance to the ray's direct term
+ * (src/shader.cc:231-236), mode 2 sets it to W * (GI radiance * kd) for a
+ * diffuse GI ray (src/shader.cc:119-123) */
+struct alignas(16) GatherReq {
+    float4 pos; /* hit point, w = mode (bits; 0 = none) */
+    float4 nrm; /* shading normal (faces the ray) */
+    float4 w;   /* throughput weight */
+    float4 kd;  /* diffuse colour (mode 2) */
 };
 
 /* the camera a request renders through (Camera::get_primary_ray inputs) */

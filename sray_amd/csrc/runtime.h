@@ -40,6 +40,13 @@ struct GpuPmap {
     uint32_t scene_id = 0;
     int kind = 0;
     uint64_t bytes = 0;
+    /* shading-gather grids, one per gather radius (float bits), built on
+     * first use and kept until the map is released */
+    struct Grid {
+        DevBuf pos, dir, nrm, colb, cell;
+        PmapGrid g{};
+    };
+    std::map<uint32_t, std::unique_ptr<Grid>> grids;
 };
 
 struct GpuScene {
@@ -162,10 +169,15 @@ int select_flagged(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, u
     return SRAY_OK;
 }
 
+/* the shading-gather grid of a photon map for a radius (built or cached) */
+int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, PmapGrid *out);
+
 /* engine.hip */
 void engine_destroy(sray_gpu_ctx *c);
 /* device bytes held by the renderers (queues, tables, pixel arenas) */
 uint64_t engine_bytes(sray_gpu_ctx *c);
+/* the photon map is about to be released: fails if requests in flight use it */
+int engine_pmap_busy(sray_gpu_ctx *c, uint32_t pmap_id);
 /* the scene is about to be released: fails if requests in flight use it */
 int engine_scene_busy(sray_gpu_ctx *c, uint32_t scene_id);
 

@@ -200,6 +200,56 @@ __global__ void k_pm_final(const sray_photon *__restrict__ ph, const uint32_t *_
     colb[i] = p.col[2];
 }
 
+/* shading-gather grid build */
+__global__ void k_grid_bbox(const float4 *__restrict__ pos, uint32_t n, uint32_t *__restrict__ bb)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos[i];
+    uint32_t k[3] = {fkey(p.x), fkey(p.y), fkey(p.z)};
+    for (int a = 0; a < 3; a++) {
+        atomicMin(&bb[a], k[a]);
+        atomicMax(&bb[3 + a], k[a]);
+    }
+}
+
+__global__ void k_grid_key(const float4 *__restrict__ pos, uint32_t n, PmapGrid G, uint32_t *__restrict__ key,
+                           uint32_t *__restrict__ val)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos[i];
+    uint32_t x = (uint32_t)grid_cell(p.x, G.ox, G.inv_cs, G.dx), y = (uint32_t)grid_cell(p.y, G.oy, G.inv_cs, G.dy),
+             z = (uint32_t)grid_cell(p.z, G.oz, G.inv_cs, G.dz);
+    key[i] = (z * (uint32_t)G.dy + y) * (uint32_t)G.dx + x;
+    val[i] = i;
+}
+
+/* cell_start[c] = first sorted photon with key >= c, for c in [0, ncell] */
+__global__ void k_grid_start(const uint32_t *__restrict__ key, uint32_t n, uint32_t ncell, uint32_t *__restrict__ start)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = key[i];
+    uint32_t prev = i == 0 ? 0u : key[i - 1] + 1u;
+    if (i == 0 || key[i - 1] != k)
+        for (uint32_t c = prev; c <= k; c++) start[c] = i;
+    if (i == n - 1)
+        for (uint32_t c = k + 1; c <= ncell; c++) start[c] = n;
+}
+
+__global__ void k_grid_final(PmapTree T, const uint32_t *__restrict__ order, uint32_t n, float4 *__restrict__ pos,
+                             float4 *__restrict__ dir, float4 *__restrict__ nrm, float *__restrict__ colb)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t o = order[j];
+    pos[j] = T.pos[o];
+    dir[j] = T.dir[o];
+    nrm[j] = T.nrm[o];
+    colb[j] = T.colb[o];
+}
+
 __global__ void k_iota(uint32_t *p, uint32_t n)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -234,6 +284,19 @@ __global__ __launch_bounds__(128) void k_pm_gather(PmapTree T, const sray_gather
     bool sel = k > 0 && a.count > (uint32_t)k;
     need[i] = sel ? 1 : 0;
     if (!sel) finish_estimate(a, mode, 1.0 / (M_PI * ((double)radius * (double)radius)), rgb + (size_t)i * 3);
+}
+
+/* fixed-radius gather over the uniform grid (same found set and estimate) */
+__global__ __launch_bounds__(128) void k_pm_gather_grid(PmapGrid G, const sray_gather_query *__restrict__ q,
+                                                        uint32_t n, float radius, int mode, float *__restrict__ rgb)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const sray_gather_query &Q = q[i];
+    GatherAcc a;
+    gather_grid(G, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
+                (double)Q.norm[2], a);
+    finish_estimate(a, mode, 1.0 / (M_PI * ((double)radius * (double)radius)), rgb + (size_t)i * 3);
 }
 
 struct Cand {
@@ -336,6 +399,14 @@ __global__ void k_pm_select(PmapTree T, const sray_gather_query *__restrict__ q,
 /* runtime                                                              */
 /* ================================================================== */
 namespace srt {
+
+float host_unkey(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
 
 int fail(sray_gpu_ctx *c, int code, const char *fmt, ...)
 {
@@ -762,12 +833,32 @@ int do_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q
 {
     auto it = c->pmaps.find(pmap_id);
     if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
+    bool grid = (mode & SRAY_GATHER_GRID) != 0;
+    mode &= ~SRAY_GATHER_GRID;
     if (mode != SRAY_GATHER_IRRADIANCE && mode != SRAY_GATHER_RADIANCE) return fail(c, SRAY_ERR_ARG, "bad gather mode");
     if (k < 0) return fail(c, SRAY_ERR_ARG, "k < 0");
+    if (grid && k > 0) return fail(c, SRAY_ERR_ARG, "the grid gather is fixed-radius (k must be 0)");
     if (!(radius > 0.0f)) return fail(c, SRAY_ERR_ARG, "radius must be > 0");
     if (n == 0) return SRAY_OK;
     const PmapTree &T = it->second->tree;
     int rc;
+    if (grid) {
+        PmapGrid G;
+        if ((rc = pmap_grid_get(c, pmap_id, radius, &G)) ||
+            (rc = ensure(c, c->gq, (size_t)n * sizeof(sray_gather_query))) || (rc = ensure(c, c->grgb, (size_t)n * 12)))
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(c->gq.p, q, (size_t)n * sizeof(sray_gather_query), hipMemcpyHostToDevice, c->stream));
+        {
+            ProfScope ps(c, "k_pm_gather_grid");
+            k_pm_gather_grid<<<blocks(n, 128), 128, 0, c->stream>>>(G, dptr<sray_gather_query>(c->gq), n, radius, mode,
+                                                                    dptr<float>(c->grgb));
+        }
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(rgb, c->grgb.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        prof_resolve(c);
+        return SRAY_OK;
+    }
     if ((rc = ensure(c, c->gq, (size_t)n * sizeof(sray_gather_query))) || (rc = ensure(c, c->grgb, (size_t)n * 12)) ||
         (rc = ensure(c, c->gcnt, (size_t)n * 4)) || (rc = ensure(c, c->gneed, n)) ||
         (rc = ensure(c, c->giota, (size_t)n * 4)) || (rc = ensure(c, c->gfq, (size_t)n * 4)) ||
@@ -947,6 +1038,135 @@ int do_pmap_download(sray_gpu_ctx *c, uint32_t pmap_id, sray_photon *out, uint32
     return SRAY_OK;
 }
 
+void free_pmap(GpuPmap &m)
+{
+    for (auto &b : m.bufs) release(b);
+    for (auto &g : m.grids) {
+        DevBuf *bs[] = {&g.second->pos, &g.second->dir, &g.second->nrm, &g.second->colb, &g.second->cell};
+        for (DevBuf *d : bs) release(*d);
+    }
+    m.grids.clear();
+}
+
+/* the shading-gather grid of a photon map for one radius: cells of the
+ * radius (larger if the photons' box would need more than 2^24 cells),
+ * photons sorted by cell with a device radix sort */
+int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, PmapGrid *out)
+{
+    auto it = c->pmaps.find(pmap_id);
+    if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
+    GpuPmap &pm = *it->second;
+    uint32_t key;
+    memcpy(&key, &radius, 4);
+    auto g = pm.grids.find(key);
+    if (g != pm.grids.end()) {
+        *out = g->second->g;
+        return SRAY_OK;
+    }
+    if (!(radius > 0.0f)) return fail(c, SRAY_ERR_ARG, "gather radius must be > 0");
+    auto G = std::make_unique<GpuPmap::Grid>();
+    const PmapTree &T = pm.tree;
+    uint32_t n = T.n;
+    PmapGrid &pg = G->g;
+    memset(&pg, 0, sizeof pg);
+    int rc;
+    if (n == 0) {
+        if ((rc = ensure(c, G->cell, 8))) return rc;
+        HIPCHK(c, hipMemsetAsync(G->cell.p, 0, 8, c->stream));
+        pg.cell_start = dptr<uint32_t>(G->cell);
+        pg.dx = pg.dy = pg.dz = 1;
+        pg.inv_cs = 1.0f;
+        *out = pg;
+        pm.grids[key] = std::move(G);
+        return SRAY_OK;
+    }
+    DevBuf bb, keys[2], vals[2], tmp;
+    auto cleanup = [&]() {
+        DevBuf *bs[] = {&bb, &keys[0], &keys[1], &vals[0], &vals[1], &tmp};
+        for (DevBuf *b : bs) release(*b);
+    };
+    if ((rc = ensure(c, bb, 24))) return rc;
+    k_pm_bbinit<<<1, 64, 0, c->stream>>>(dptr<uint32_t>(bb), 1);
+    k_grid_bbox<<<blocks(n, 256), 256, 0, c->stream>>>(T.pos, n, dptr<uint32_t>(bb));
+    uint32_t hb[6];
+    if (hipMemcpyAsync(hb, bb.p, 24, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        cleanup();
+        return fail(c, SRAY_ERR_HIP, "photon bounds failed");
+    }
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = host_unkey(hb[k]);
+        hi[k] = host_unkey(hb[3 + k]);
+    }
+    /* cell size: the radius, or larger to keep the cell count bounded */
+    const double MAXC = (double)(1u << 24);
+    double cs = radius;
+    auto cells = [&](double s, int64_t d[3]) {
+        double t = 1;
+        for (int k = 0; k < 3; k++) {
+            d[k] = (int64_t)std::floor(((double)hi[k] - (double)lo[k]) / s) + 1;
+            t *= (double)d[k];
+        }
+        return t;
+    };
+    int64_t dd[3];
+    while (cells(cs, dd) > MAXC) cs *= 1.25;
+    pg.ox = lo[0];
+    pg.oy = lo[1];
+    pg.oz = lo[2];
+    pg.inv_cs = (float)(1.0 / cs);
+    pg.dx = (int32_t)dd[0];
+    pg.dy = (int32_t)dd[1];
+    pg.dz = (int32_t)dd[2];
+    uint32_t ncell = (uint32_t)(dd[0] * dd[1] * dd[2]);
+    if ((rc = ensure(c, keys[0], (size_t)n * 4)) || (rc = ensure(c, keys[1], (size_t)n * 4)) ||
+        (rc = ensure(c, vals[0], (size_t)n * 4)) || (rc = ensure(c, vals[1], (size_t)n * 4)) ||
+        (rc = ensure(c, G->cell, ((size_t)ncell + 1) * 4)) || (rc = ensure(c, G->pos, (size_t)n * 16)) ||
+        (rc = ensure(c, G->dir, (size_t)n * 16)) || (rc = ensure(c, G->nrm, (size_t)n * 16)) ||
+        (rc = ensure(c, G->colb, (size_t)n * 4))) {
+        cleanup();
+        return rc;
+    }
+    k_grid_key<<<blocks(n, 256), 256, 0, c->stream>>>(T.pos, n, pg, dptr<uint32_t>(keys[0]), dptr<uint32_t>(vals[0]));
+    int endbit = 1;
+    while (endbit < 32 && (1ull << endbit) <= ncell) endbit++;
+    size_t tb = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, dptr<uint32_t>(keys[0]), dptr<uint32_t>(keys[1]),
+                                                      dptr<uint32_t>(vals[0]), dptr<uint32_t>(vals[1]), (int)n, 0,
+                                                      endbit, c->stream);
+    if (e != hipSuccess || (rc = ensure(c, tmp, tb))) {
+        cleanup();
+        return rc ? rc : fail(c, SRAY_ERR_HIP, "grid sort sizing failed");
+    }
+    tb = tmp.bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, dptr<uint32_t>(keys[0]), dptr<uint32_t>(keys[1]),
+                                           dptr<uint32_t>(vals[0]), dptr<uint32_t>(vals[1]), (int)n, 0, endbit,
+                                           c->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(c, SRAY_ERR_HIP, "grid sort failed: %s", hipGetErrorString(e));
+    }
+    k_grid_start<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<uint32_t>(keys[1]), n, ncell, dptr<uint32_t>(G->cell));
+    k_grid_final<<<blocks(n, 256), 256, 0, c->stream>>>(T, dptr<uint32_t>(vals[1]), n, dptr<float4>(G->pos),
+                                                        dptr<float4>(G->dir), dptr<float4>(G->nrm),
+                                                        dptr<float>(G->colb));
+    e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(c, SRAY_ERR_HIP, "grid build failed: %s", hipGetErrorString(e));
+    pg.v.pos = dptr<float4>(G->pos);
+    pg.v.dir = dptr<float4>(G->dir);
+    pg.v.nrm = dptr<float4>(G->nrm);
+    pg.v.colb = dptr<float>(G->colb);
+    pg.v.axis = nullptr;
+    pg.v.n = n;
+    pg.v.depth = 0;
+    pg.cell_start = dptr<uint32_t>(G->cell);
+    *out = pg;
+    pm.grids[key] = std::move(G);
+    return SRAY_OK;
+}
+
 int do_trace_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *d_rays, uint32_t n, sray_hit *d_hits,
                     uint32_t flags, unsigned long long *d_counts)
 {
@@ -1001,8 +1221,7 @@ void sray_gpu_destroy(sray_gpu_ctx *c)
         for (auto &kv : c->scenes)
             for (auto &b : kv.second->bufs) release(b);
         c->scenes.clear();
-        for (auto &kv : c->pmaps)
-            for (auto &b : kv.second->bufs) release(b);
+        for (auto &kv : c->pmaps) free_pmap(*kv.second);
         c->pmaps.clear();
         DevBuf *bs[] = {&c->cubtmp, &c->nsel, &c->rays, &c->hitbuf, &c->counts, &c->gq, &c->grgb, &c->gcnt,
                         &c->gneed, &c->gfq, &c->goff, &c->gcand, &c->giota};
@@ -1048,7 +1267,12 @@ int sray_gpu_scene_bytes(sray_gpu_ctx *c, uint32_t scene_id, uint64_t *bytes)
         /* everything the context holds on the device */
         uint64_t t = engine_bytes(c);
         for (auto &kv : c->scenes) t += kv.second->bytes;
-        for (auto &kv : c->pmaps) t += kv.second->bytes;
+        for (auto &kv : c->pmaps) {
+            t += kv.second->bytes;
+            for (auto &g : kv.second->grids)
+                t += g.second->pos.bytes + g.second->dir.bytes + g.second->nrm.bytes + g.second->colb.bytes +
+                     g.second->cell.bytes;
+        }
         DevBuf *bs[] = {&c->cubtmp, &c->nsel, &c->rays, &c->hitbuf, &c->counts, &c->gq, &c->grgb, &c->gcnt,
                         &c->gneed, &c->gfq, &c->goff, &c->gcand, &c->giota};
         for (DevBuf *d : bs) t += d->bytes;
@@ -1118,9 +1342,14 @@ int sray_gpu_pmap_release(sray_gpu_ctx *c, uint32_t pmap_id)
     ApiLock lk(c);
     auto it = c->pmaps.find(pmap_id);
     if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
+    int rc = engine_pmap_busy(c, pmap_id);
+    if (rc) return rc;
     (void)hipStreamSynchronize(c->stream);
-    for (auto &b : it->second->bufs) release(b);
+    free_pmap(*it->second);
     c->pmaps.erase(it);
+    for (auto &kv : c->scenes)
+        for (uint32_t &b : kv.second->pmap)
+            if (b == pmap_id) b = 0;
     return SRAY_OK;
 }
 

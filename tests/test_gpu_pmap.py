@@ -87,6 +87,29 @@ def test_range_gather_matches_checker(ctx, n, planar):
     S.lib().sray_gpu_pmap_release(ctx._h, pid)
 
 
+@pytest.mark.parametrize("planar", [False, True])
+@pytest.mark.parametrize("n", [1, 1000, 200_000])
+def test_grid_gather_matches_checker(ctx, n, planar):
+    """the uniform-grid gather (the renderer's shading path) finds the same
+    photons as the reference's range query"""
+    rng = np.random.default_rng(100 + n + planar)
+    ph = _photons(rng, n, planar)
+    q = _queries(rng, 2000, planar)
+    q[:50, 0:3] = ph[np.arange(50) % n, 0:3]  # queries on photons
+    q[50:60, 0:3] = 50.0  # far outside the photons' box
+    pid = _upload(ctx, ph)
+    b = build()
+    ref = oracle.OracleScene(b.ptr(), 64)
+    ref.set_photons(0, ph)
+    for radius in (0.02, 0.1, 0.4, 3.0):
+        for mode in (0, 1):
+            got = _gpu_gather(ctx, pid, q, radius, 0, mode | S.abi.SRAY_GATHER_GRID)
+            want = ref.gather(0, q, radius, 0, mode)
+            assert np.allclose(got, want, rtol=RTOL, atol=ATOL), (n, planar, radius, mode,
+                                                                  np.abs(got - want).max())
+    S.lib().sray_gpu_pmap_release(ctx._h, pid)
+
+
 @pytest.mark.parametrize("k", [1, 8, 20, 200])
 def test_knn_gather_tie_break_by_index(ctx, k):
     rng = np.random.default_rng(k)
