@@ -132,10 +132,11 @@ struct Builder {
         uint32_t mask = 0;
         for (int r = 0; r < 8; r++)
             if (nd.child[r] >= 0 && stored(nd.child[r])) mask |= 1u << kRefToBits[r];
-        uint32_t cnt = (uint32_t)__builtin_popcount(mask);
+        /* eight slots, child with octant bits k at first + k; slots of
+         * children that are not stored stay zero and are never visited */
         first = (uint32_t)(out.lo.size() / 4);
-        out.lo.resize(out.lo.size() + (size_t)cnt * 4);
-        out.hi.resize(out.hi.size() + (size_t)cnt * 4);
+        out.lo.resize(out.lo.size() + 8 * 4, 0.0f);
+        out.hi.resize(out.hi.size() + 8 * 4, 0.0f);
         lo = &out.lo[(size_t)at * 4];
         hi = &out.hi[(size_t)at * 4];
         meta = mask;
@@ -144,9 +145,7 @@ struct Builder {
         for (int r = 0; r < 8; r++) {
             int ch = nd.child[r];
             if (ch < 0 || !stored(ch)) continue;
-            uint32_t b = (uint32_t)kRefToBits[r];
-            uint32_t slot = first + (uint32_t)__builtin_popcount(mask & ((1u << b) - 1u));
-            flatten(ch, slot, item_base, out, leaves);
+            flatten(ch, first + (uint32_t)kRefToBits[r], item_base, out, leaves);
         }
     }
 };

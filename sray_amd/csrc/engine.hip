@@ -111,7 +111,7 @@ template <bool COUNT, bool UNI>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_q(Tabs T, const QRay *__restrict__ q, uint32_t n,
                                                          int4 *__restrict__ hits, unsigned long long *counts)
 {
-    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    __shared__ uint32_t stk[STACK * TRACE_BLOCK];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 o = q[i].o, d = q[i].d;
@@ -128,7 +128,7 @@ template <bool COUNT, bool UNI>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq, uint32_t n,
                                                               uint8_t *__restrict__ occ, unsigned long long *counts)
 {
-    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    __shared__ uint32_t stk[STACK * TRACE_BLOCK];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 d = sq[i].d;
@@ -435,31 +435,36 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
 /* photon gathers requested by k_shade: caustic irradiance (>= 8 photons,
  * src/pmap.cc:109-139) added to the direct term, or the radiance estimate
  * (src/pmap.cc:141-178) of a diffuse GI ray times kd */
+constexpr int GATHER_LANES = 16; /* lanes sharing one gather query */
+
 template <bool UNI>
-__global__ __launch_bounds__(128) void k_gather(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
+__global__ __launch_bounds__(256) void k_gather(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
                                                 uint32_t n, float4 *__restrict__ direct)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = t / GATHER_LANES;
+    int lane = (int)(t % GATHER_LANES);
+    if (i >= n) return; /* whole groups leave together */
     float4 pos = greq[i].pos;
     uint32_t mode = __float_as_uint(pos.w);
     if (mode == 0u) return;
     uint32_t rq = UNI ? 0u : q[i].m.w >> SLOT_SHIFT;
     const ShadePmaps &PM = T.reqs[rq].PM;
-    float4 nn = greq[i].nrm, w = greq[i].w;
+    float4 nn = greq[i].nrm;
     GatherAcc a;
+    gather_grid_coop<GATHER_LANES>(mode == 1u ? PM.caust : PM.glob, pos.x, pos.y, pos.z, PM.radius,
+                                   mode == 1u ? SRAY_GATHER_IRRADIANCE : SRAY_GATHER_RADIANCE, (double)nn.x,
+                                   (double)nn.y, (double)nn.z, lane, a);
+    if (lane != 0) return;
+    float4 w = greq[i].w;
     float4 d = direct[i];
     if (mode == 1u) {
-        gather_grid(PM.caust, pos.x, pos.y, pos.z, PM.radius, SRAY_GATHER_IRRADIANCE, (double)nn.x, (double)nn.y,
-                    (double)nn.z, a);
         V3 irr = a.count < 8 ? v3(0.0f, 0.0f, 0.0f)
                              : v3((float)(a.r * PM.scale), (float)(a.g * PM.scale), (float)(a.b * PM.scale));
         d.x = d.x + w.x * irr.x;
         d.y = d.y + w.y * irr.y;
         d.z = d.z + w.z * irr.z;
     } else {
-        gather_grid(PM.glob, pos.x, pos.y, pos.z, PM.radius, SRAY_GATHER_RADIANCE, (double)nn.x, (double)nn.y,
-                    (double)nn.z, a);
         V3 rad = v3((float)(a.r * PM.scale), (float)(a.g * PM.scale), (float)(a.b * PM.scale));
         float4 kd = greq[i].kd;
         d.x = w.x * (rad.x * kd.x);
@@ -1032,7 +1037,8 @@ int process_level(Level &LV, int b, uint32_t count)
         if (E.photons) {
             ProfScope ps(c, "k_gather");
             auto kern = E.uni ? k_gather<true> : k_gather<false>;
-            kern<<<blocks(m, 128), 128, 0, c->stream>>>(LV.T, q, dptr<GatherReq>(E.greq), m, dptr<float4>(E.direct));
+            kern<<<blocks((size_t)m * GATHER_LANES, 256), 256, 0, c->stream>>>(LV.T, q, dptr<GatherReq>(E.greq), m,
+                                                                               dptr<float4>(E.direct));
         }
         size_t ns = (size_t)m * E.L;
         if (ns) {

@@ -61,9 +61,9 @@ void mat_to_float(const double m[12], float out[12]);
 /* ------------------------------------------------------------------ */
 /* Device node layout, two float4 arrays ("lo" and "hi"):
  *   lo = (min.x, min.y, min.z, bits(first))   hi = (max.x, max.y, max.z, bits(meta))
- *   inner node: first = index of its first child, meta = child mask (bit k =
- *               child whose octant bits (x | y<<1 | z<<2) equal k is stored;
- *               children are contiguous in increasing k)
+ *   inner node: first = index of an 8-slot child block, meta = child mask
+ *               (bit k = the child whose octant bits (x | y<<1 | z<<2) equal
+ *               k is stored, at slot first + k; other slots are unused)
  *   leaf:       first = first item index, meta = SR_LEAF | item count
  * Item ranges are assigned in the reference's depth-first order (children
  * 0..7 of src/octree.inl:161-169), so an item's position in the item array

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon(SceneDev S, PhotonJob J,
                                                         uint8_t *__restrict__ keep,
                                                         unsigned long long *__restrict__ segs)
 {
-    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    __shared__ uint32_t stk[STACK * TRACE_BLOCK];
     __shared__ unsigned long long bseg;
     if (threadIdx.x == 0) bseg = 0;
     __syncthreads();

@@ -151,6 +151,42 @@ __device__ __forceinline__ void gather_grid(const PmapGrid &G, float qx, float q
         }
 }
 
+/* the same gather by a group of G consecutive lanes (G a power of two, at
+ * most 64, all G lanes active): the lanes stride over each contiguous
+ * photon range and their partial sums are combined with cross-lane
+ * shuffles, so a query's photons are read G at a time.  Every lane of the
+ * group returns the total. */
+template <int G>
+__device__ __forceinline__ void gather_grid_coop(const PmapGrid &Gd, float qx, float qy, float qz, float radius,
+                                                 int mode, double nx, double ny, double nz, int lane, GatherAcc &a)
+{
+    a.r = a.g = a.b = 0.0;
+    a.count = 0;
+    if (Gd.v.n == 0) return;
+    double r2 = (double)radius * (double)radius;
+    float mx = 1e-5f * (fabsf(qx) + radius), my = 1e-5f * (fabsf(qy) + radius), mz = 1e-5f * (fabsf(qz) + radius);
+    int x0 = grid_cell(qx - radius - mx, Gd.ox, Gd.inv_cs, Gd.dx), x1 = grid_cell(qx + radius + mx, Gd.ox, Gd.inv_cs, Gd.dx);
+    int y0 = grid_cell(qy - radius - my, Gd.oy, Gd.inv_cs, Gd.dy), y1 = grid_cell(qy + radius + my, Gd.oy, Gd.inv_cs, Gd.dy);
+    int z0 = grid_cell(qz - radius - mz, Gd.oz, Gd.inv_cs, Gd.dz), z1 = grid_cell(qz + radius + mz, Gd.oz, Gd.inv_cs, Gd.dz);
+    for (int z = z0; z <= z1; z++)
+        for (int y = y0; y <= y1; y++) {
+            uint32_t row = ((uint32_t)z * (uint32_t)Gd.dy + (uint32_t)y) * (uint32_t)Gd.dx;
+            uint32_t b = Gd.cell_start[row + (uint32_t)x0], e = Gd.cell_start[row + (uint32_t)x1 + 1u];
+            for (uint32_t m = b + (uint32_t)lane; m < e; m += (uint32_t)G) {
+                float4 p = Gd.v.pos[m];
+                double ddx = (double)p.x - (double)qx, ddy = (double)p.y - (double)qy, ddz = (double)p.z - (double)qz;
+                double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (d2 <= r2) gather_photon(Gd.v, m, mode, nx, ny, nz, a);
+            }
+        }
+    for (int off = G / 2; off > 0; off >>= 1) {
+        a.r += __shfl_xor(a.r, off, G);
+        a.g += __shfl_xor(a.g, off, G);
+        a.b += __shfl_xor(a.b, off, G);
+        a.count += (uint32_t)__shfl_xor((int)a.count, off, G);
+    }
+}
+
 } // namespace srd
 
 #endif

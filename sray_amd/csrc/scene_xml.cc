@@ -971,12 +971,20 @@ std::string write_scene_xml(const HostScene &hs)
                         const float *v = &hs.tri_normal[(size_t)(md.tri_start + f) * 9 + c * 3];
                         o << "      <normal id=\"" << (f * 3 + c) << "\" val=\"" << vec3(v) << "\"/>\n";
                     }
+            if (hs.has_texcoord)
+                for (uint32_t f = 0; f < md.tri_count; f++)
+                    for (int c = 0; c < 3; c++) {
+                        const float *v = &hs.tri_texcoord[(size_t)(md.tri_start + f) * 6 + c * 2];
+                        o << "      <texcoord id=\"" << (f * 3 + c) << "\" val=\"" << f9(v[0]) << " " << f9(v[1])
+                          << "\"/>\n";
+                    }
             for (uint32_t f = 0; f < md.tri_count; f++) {
                 o << "      <face>";
                 for (int c = 0; c < 3; c++) {
                     uint32_t id = f * 3 + c;
                     o << "<vref vertex=\"" << id << "\"";
                     if (hs.has_normal) o << " normal=\"" << id << "\"";
+                    if (hs.has_texcoord) o << " texcoord=\"" << id << "\"";
                     o << "/>";
                 }
                 o << "</face>\n";

@@ -469,6 +469,18 @@ bool gen_batch_scene(uint64_t seed, int ntri, HostScene &hs)
     uint32_t start = (uint32_t)(hs.tri_pos.size() / 9);
     uint32_t n = add_icosphere(hs, freq, 0.2, r.next(), 2.0);
     add_mesh_object(hs, mi, start, n, 0.0f, 0.0f, 0.0f);
+    /* vertex normals (radial) and spherical texture coordinates, so the
+     * batch exercises interpolated shading normals (interp_face) */
+    hs.has_normal = hs.has_texcoord = true;
+    hs.tri_normal.resize(hs.tri_pos.size());
+    hs.tri_texcoord.resize(hs.tri_pos.size() / 9 * 6);
+    for (size_t v = 0; v < hs.tri_pos.size() / 3; v++) {
+        const float *p = &hs.tri_pos[v * 3];
+        double l = sqrt((double)p[0] * p[0] + (double)p[1] * p[1] + (double)p[2] * p[2]);
+        for (int k = 0; k < 3; k++) hs.tri_normal[v * 3 + k] = (float)(p[k] / l);
+        hs.tri_texcoord[v * 2] = (float)(atan2((double)p[2], (double)p[0]) / (2.0 * M_PI) + 0.5);
+        hs.tri_texcoord[v * 2 + 1] = (float)(acos(std::max(-1.0, std::min(1.0, p[1] / l))) / M_PI);
+    }
     int ns = 8 + (int)(r.next() % 25);
     for (int s = 0; s < ns; s++) {
         float kd[3] = {r.uni(0, 1), r.uni(0, 1), r.uni(0, 1)};

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneDev S, const sr
                                                           sray_hit *__restrict__ hits,
                                                           unsigned long long *__restrict__ counts)
 {
-    __shared__ uint2 stk[STACK * TRACE_BLOCK];
+    __shared__ uint32_t stk[STACK * TRACE_BLOCK];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
     if (i < n) {
@@ -286,17 +286,20 @@ __global__ __launch_bounds__(128) void k_pm_gather(PmapTree T, const sray_gather
     if (!sel) finish_estimate(a, mode, 1.0 / (M_PI * ((double)radius * (double)radius)), rgb + (size_t)i * 3);
 }
 
-/* fixed-radius gather over the uniform grid (same found set and estimate) */
-__global__ __launch_bounds__(128) void k_pm_gather_grid(PmapGrid G, const sray_gather_query *__restrict__ q,
+/* fixed-radius gather over the uniform grid (same found set and estimate),
+ * 16 lanes per query */
+__global__ __launch_bounds__(256) void k_pm_gather_grid(PmapGrid G, const sray_gather_query *__restrict__ q,
                                                         uint32_t n, float radius, int mode, float *__restrict__ rgb)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = t / 16u;
+    int lane = (int)(t % 16u);
     if (i >= n) return;
     const sray_gather_query &Q = q[i];
     GatherAcc a;
-    gather_grid(G, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
-                (double)Q.norm[2], a);
-    finish_estimate(a, mode, 1.0 / (M_PI * ((double)radius * (double)radius)), rgb + (size_t)i * 3);
+    gather_grid_coop<16>(G, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
+                         (double)Q.norm[2], lane, a);
+    if (lane == 0) finish_estimate(a, mode, 1.0 / (M_PI * ((double)radius * (double)radius)), rgb + (size_t)i * 3);
 }
 
 struct Cand {
@@ -558,10 +561,13 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
         mesh_root[m] = node_off;
     }
     g->mesh_height = mh;
-    if ((int)(4 + g->scene_height + g->mesh_height) > STACK)
+    /* deepest stack: pseudo root + scene levels + resume (2) + marker +
+     * mesh pseudo root + mesh levels */
+    if ((int)(5 + g->scene_height + g->mesh_height) > STACK)
         return fail(c, SRAY_ERR_LIMIT, "octree heights %u + %u exceed the traversal stack (%d entries)",
                     g->scene_height, g->mesh_height, STACK);
     if (ft.items.size() >= (1u << 30)) return fail(c, SRAY_ERR_LIMIT, "more than 2^30 octree item references");
+    if (ft.lo.size() / 4 >= (1u << 23)) return fail(c, SRAY_ERR_LIMIT, "more than 2^23 octree nodes");
     /* objects */
     uint32_t nmat = s->num_materials;
     std::vector<DevObj> objs(s->num_objects);
@@ -850,8 +856,8 @@ int do_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q
         HIPCHK(c, hipMemcpyAsync(c->gq.p, q, (size_t)n * sizeof(sray_gather_query), hipMemcpyHostToDevice, c->stream));
         {
             ProfScope ps(c, "k_pm_gather_grid");
-            k_pm_gather_grid<<<blocks(n, 128), 128, 0, c->stream>>>(G, dptr<sray_gather_query>(c->gq), n, radius, mode,
-                                                                    dptr<float>(c->grgb));
+            k_pm_gather_grid<<<blocks((size_t)n * 16, 256), 256, 0, c->stream>>>(
+                G, dptr<sray_gather_query>(c->gq), n, radius, mode, dptr<float>(c->grgb));
         }
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(rgb, c->grgb.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));

@@ -14,10 +14,14 @@
  * depth-first item array — so equal distances resolve to the same winner
  * as the reference's first-occurrence rule.
  *
- * Per-lane stack in LDS, one 8-byte entry per open node:
- *   ITER   x = first child, y = exist mask | pending (order space) << 8
- *   RESUME x = next scene item, y = end        (scene leaf interrupted by a mesh)
- *   MARKER x = scene item rank of the mesh being traversed
+ * Per-lane stack in LDS, 32-bit entries:
+ *   ITER   first child block << 8 | pending children (order space)   (bit 31 = 0)
+ *   RESUME two words: the next scene item, then E_RES | end of the
+ *          leaf's items (a scene leaf interrupted by a mesh)
+ *   MARKER E_MARK | scene item rank of the mesh being traversed
+ * Children sit in eight-slot blocks (child with octant bits k at first + k),
+ * so an ITER entry needs no child mask; four bytes per entry keep the LDS
+ * footprint at 80 bytes per lane.
  * A mesh hit by the same ray in several scene leaves (Q9) is traversed once:
  * its result is memoised for the ray.
  */
@@ -30,7 +34,7 @@
 
 namespace srd {
 
-constexpr uint32_t E_ITER = 0u, E_RESUME = 1u, E_MARKER = 2u;
+constexpr uint32_t E_RES = 0x80000000u, E_MARK = 0xC0000000u;
 
 struct TRay {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
@@ -228,7 +232,7 @@ struct HitRec {
  * stk points at this lane's entry 0; consecutive entries are `stride`
  * apart (LDS, lane-interleaved). */
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int time, uint2 *stk, int stride,
+__device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int time, uint32_t *stk, int stride,
                                            TraceCounts &cnt)
 {
     TRay wr = make_tray(wo, wd);
@@ -247,8 +251,8 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
     uint32_t item_cur = 0, item_end = 0;
     int cur_mesh = -1;
     int sp = 0;
-    /* pseudo entry whose only child is the scene root */
-    stk[0] = make_uint2(S.scene_root, 1u | ((1u << r.dmask) << 8));
+    /* pseudo entry whose only child (slot 0) is the scene root */
+    stk[0] = (S.scene_root << 8) | (1u << r.dmask);
     sp = 1;
     for (;;) {
         if (item_cur < item_end) {
@@ -295,12 +299,15 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                     continue;
                 }
                 if (COUNT) cnt.mesh_enters++;
-                if (item_cur < item_end) stk[(sp++) * stride] = make_uint2(item_cur, item_end | (E_RESUME << 30));
-                stk[(sp++) * stride] = make_uint2(rank, E_MARKER << 30);
+                if (item_cur < item_end) {
+                    stk[(sp++) * stride] = item_cur;
+                    stk[(sp++) * stride] = E_RES | item_end;
+                }
+                stk[(sp++) * stride] = E_MARK | rank;
                 float I[12];
                 obj_inv_at(ob, time, I);
                 r = make_tray(xf_point(I, wo), xf_vec(I, wd));
-                stk[(sp++) * stride] = make_uint2(ob.root, 1u | ((1u << r.dmask) << 8));
+                stk[(sp++) * stride] = (ob.root << 8) | (1u << r.dmask);
                 item_cur = item_end = 0;
                 level = 1;
                 cur_mesh = oi;
@@ -337,11 +344,10 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
             continue;
         }
         if (sp == 0) break;
-        uint2 e = stk[(sp - 1) * stride];
-        uint32_t ty = e.y >> 30;
-        if (ty == E_MARKER) {
+        uint32_t e = stk[(sp - 1) * stride];
+        if ((e & E_MARK) == E_MARK) {
             sp--;
-            uint32_t rank = e.x;
+            uint32_t rank = e & 0x3fffffffu;
             int oi = (int)S.items[rank];
             if (!ANY) {
                 cache_obj = oi;
@@ -358,19 +364,18 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
             level = 0;
             continue;
         }
-        if (ty == E_RESUME) {
-            sp--;
-            item_cur = e.x;
-            item_end = e.y & 0x3fffffffu;
+        if (e & E_RES) {
+            item_end = e & 0x3fffffffu;
+            item_cur = stk[(sp - 2) * stride];
+            sp -= 2;
             continue;
         }
-        uint32_t exist = e.y & 0xffu, pend = (e.y >> 8) & 0xffu;
+        uint32_t pend = e & 0xffu;
         uint32_t k = (uint32_t)__builtin_ctz(pend);
         pend &= pend - 1u;
         if (pend == 0u) sp--;
-        else stk[(sp - 1) * stride].y = exist | (pend << 8);
-        uint32_t c = k ^ r.dmask;
-        uint32_t child = e.x + (uint32_t)__builtin_popcount(exist & ((1u << c) - 1u));
+        else stk[(sp - 1) * stride] = (e & ~0xffu) | pend;
+        uint32_t child = (e >> 8) + (k ^ r.dmask);
         float4 lo = S.nlo[child], hi = S.nhi[child];
         if (COUNT) cnt.nodes++;
         float tmin;
@@ -384,7 +389,7 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
             item_cur = first;
             item_end = first + (meta & ~LEAF);
         } else {
-            stk[(sp++) * stride] = make_uint2(first, meta | (order_mask(meta, r.dmask) << 8));
+            stk[(sp++) * stride] = (first << 8) | order_mask(meta, r.dmask);
         }
     }
     return best;

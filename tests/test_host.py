@@ -28,6 +28,12 @@ def _same_scene(a, b):
     n = da.num_triangles * 9
     if n:
         assert np.array_equal(np.ctypeslib.as_array(da.tri_pos, (n,)), np.ctypeslib.as_array(db.tri_pos, (n,)))
+    for name, per in (("tri_normal", 9), ("tri_texcoord", 6), ("tri_tangent", 9)):
+        pa, pb = getattr(da, name), getattr(db, name)
+        assert bool(pa) == bool(pb), name
+        if pa:
+            m = per * da.num_triangles
+            assert np.array_equal(np.ctypeslib.as_array(pa, (m,)), np.ctypeslib.as_array(pb, (m,))), name
     for i in range(da.num_materials):
         assert bytes(da.materials[i]) == bytes(db.materials[i])
     for i in range(da.num_lights):
@@ -161,14 +167,13 @@ def _leaf_rule(hs, level, lo, hi, items):
     for k in inner[:200]:
         c = (lo[k, :3] + hi[k, :3]) * np.float32(0.5)
         mask = meta[k]
-        slot = first[k]
         for bits in range(8):
             if not (mask >> bits) & 1:
                 continue
+            slot = first[k] + bits  # eight-slot child block
             want_lo = np.where([(bits >> a) & 1 for a in range(3)], c, lo[k, :3])
             want_hi = np.where([(bits >> a) & 1 for a in range(3)], hi[k, :3], c)
             assert np.array_equal(lo[slot, :3], want_lo) and np.array_equal(hi[slot, :3], want_hi)
-            slot += 1
 
 
 @pytest.mark.parametrize("config,variant,levels", [(0, 0, [-1]), (1, 8, [-1, 0]), (2, 10, [-1, 12]),
