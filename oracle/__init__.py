@@ -65,6 +65,7 @@ def lib(fast=False):
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
         L.oracle_projmap.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, vp,
                                      C.POINTER(C.c_double)]
+        L.oracle_surface.argtypes = [vp, vp, C.c_uint32, vp]
         L.oracle_cell_from_dir.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]
         _libs[name] = L
@@ -153,6 +154,14 @@ class OracleScene:
         if self.L.oracle_projmap(self.h, light, kind, t0, t1, seed, cells.ctypes.data, C.byref(cov)):
             raise RuntimeError("oracle_projmap failed")
         return cells.reshape(32, 64), cov.value
+
+    def surface(self, rays):
+        """(n, 8) float32: hit position, normal, u, v of each closest hit (NaN on a miss)"""
+        rays = np.ascontiguousarray(rays)
+        out = np.zeros((len(rays), 8), np.float32)
+        if self.L.oracle_surface(self.h, rays.ctypes.data, len(rays), out.ctypes.data):
+            raise RuntimeError("oracle_surface failed")
+        return out
 
     def diag(self):
         return self.L.oracle_scene_diag(self.h)

@@ -749,55 +749,62 @@ template <class T> struct Scene {
 
     /* interp_face, src/mesh.cc:321-445 (Glassner ch.2 s.3.3, E. Haines):
      * inverse bilinear (u,v) of the degenerate patch p00 p10 p01=p11 */
-    void interp_face(const Tri<T> &tr, V3<T> ispos, V3<T> normal, V3<T> &on, V3<T> &og, T &ou, T &ov) const
+    void interp_face(const Tri<T> &tr, V3<T> ispos_t, V3<T> normal_t, V3<T> &on, V3<T> &og, T &ou, T &ov) const
     {
-        V3<T> p00 = tr.v[0], p10 = tr.v[1], p01 = tr.v[2], p11 = tr.v[2];
-        V3<T> pa = (p00 - p10) + (p11 - p01);
-        V3<T> pb = p10 - p00;
-        V3<T> pc = p01 - p00;
-        V3<T> pd = p00;
-        V3<T> na = cross(pa, normal), nb = cross(pb, normal), nc = cross(pc, normal);
-        T du0 = dot(nc, pd);
-        T du1 = dot(na, pd) + dot(nc, pb);
-        T du2 = dot(na, pb);
-        T dv0 = dot(nb, pd);
-        T dv1 = dot(na, pd) + dot(nb, pc);
-        T dv2 = dot(na, pc);
-        const T xs = (T)SRAY_XSMALL_NUMBER;
-        T ru, rv;
+        /* evaluated in double for both precisions: for the degenerate quad
+         * (v0 v1 v2 v2) du2 is zero up to rounding, and in float that
+         * rounding noise passes the XSMALL test and the quadratic branch
+         * returns garbage; the reference computes in double */
+        typedef double D;
+        auto d3 = [](V3<T> a) { return v3<D>((D)a.x, (D)a.y, (D)a.z); };
+        V3<D> ispos = d3(ispos_t), normal = d3(normal_t);
+        V3<D> p00 = d3(tr.v[0]), p10 = d3(tr.v[1]), p01 = d3(tr.v[2]), p11 = d3(tr.v[2]);
+        V3<D> pa = (p00 - p10) + (p11 - p01);
+        V3<D> pb = p10 - p00;
+        V3<D> pc = p01 - p00;
+        V3<D> pd = p00;
+        V3<D> na = cross(pa, normal), nb = cross(pb, normal), nc = cross(pc, normal);
+        D du0 = dot(nc, pd);
+        D du1 = dot(na, pd) + dot(nc, pb);
+        D du2 = dot(na, pb);
+        D dv0 = dot(nb, pd);
+        D dv1 = dot(na, pd) + dot(nb, pc);
+        D dv2 = dot(na, pc);
+        const D xs = (D)SRAY_XSMALL_NUMBER;
+        D ru, rv;
         if (std::fabs(du2) <= xs) {
-            T divisor = du1 - dot(na, ispos);
-            ru = std::fabs(divisor) > xs ? ((dot(nc, ispos) - du0) / divisor) : (T)0;
+            D divisor = du1 - dot(na, ispos);
+            ru = std::fabs(divisor) > xs ? ((dot(nc, ispos) - du0) / divisor) : (D)0;
         } else {
-            V3<T> qux = na / ((T)2 * du2);
-            V3<T> quy = (-nc) / du2;
-            T dux = -du1 / ((T)2 * du2);
-            T duy = du0 / du2;
-            T ka = dux + dot(qux, ispos);
-            T kb = duy + dot(quy, ispos);
-            T u0 = ka - std::sqrt(ka * ka - kb);
-            T u1 = ka + std::sqrt(ka * ka - kb);
-            ru = (u0 < (T)0 || u0 > (T)1) ? u1 : u0;
+            V3<D> qux = na / ((D)2 * du2);
+            V3<D> quy = (-nc) / du2;
+            D dux = -du1 / ((D)2 * du2);
+            D duy = du0 / du2;
+            D ka = dux + dot(qux, ispos);
+            D kb = duy + dot(quy, ispos);
+            D u0 = ka - std::sqrt(ka * ka - kb);
+            D u1 = ka + std::sqrt(ka * ka - kb);
+            ru = (u0 < (D)0 || u0 > (D)1) ? u1 : u0;
         }
         if (std::fabs(dv2) <= xs) {
-            T divisor = dv1 - dot(na, ispos);
-            rv = std::fabs(divisor) > xs ? ((dot(nb, ispos) - dv0) / divisor) : (T)0;
+            D divisor = dv1 - dot(na, ispos);
+            rv = std::fabs(divisor) > xs ? ((dot(nb, ispos) - dv0) / divisor) : (D)0;
         } else {
-            V3<T> qvx = na / ((T)2 * dv2);
-            V3<T> qvy = (-nb) / dv2;
-            T dvx = -dv1 / ((T)2 * dv2);
-            T dvy = dv0 / dv2;
-            T ka = dvx + dot(qvx, ispos);
-            T kb = dvy + dot(qvy, ispos);
-            T v0 = ka - std::sqrt(ka * ka - kb);
-            T v1 = ka + std::sqrt(ka * ka - kb);
-            rv = (v0 < (T)0 || v0 > (T)1) ? v1 : v0;
+            V3<D> qvx = na / ((D)2 * dv2);
+            V3<D> qvy = (-nb) / dv2;
+            D dvx = -dv1 / ((D)2 * dv2);
+            D dvy = dv0 / dv2;
+            D ka = dvx + dot(qvx, ispos);
+            D kb = dvy + dot(qvy, ispos);
+            D v0 = ka - std::sqrt(ka * ka - kb);
+            D v1 = ka + std::sqrt(ka * ka - kb);
+            rv = (v0 < (D)0 || v0 > (D)1) ? v1 : v0;
         }
-        auto lerp = [](T a, T b, T t) { return a + (b - a) * t; };
+        auto lerp = [](D a, D b, D t) { return a + (b - a) * t; };
         auto l2 = [&](T a00, T a10, T a01) {
-            T s0 = lerp(a00, a01, rv);
-            T s1 = lerp(a10, a01, rv);
-            return lerp(s0, s1, ru);
+            D s0 = lerp((D)a00, (D)a01, rv);
+            D s1 = lerp((D)a10, (D)a01, rv);
+            return (T)lerp(s0, s1, ru);
         };
         ou = l2(tr.tu[0], tr.tu[1], tr.tu[2]);
         ov = l2(tr.tv[0], tr.tv[1], tr.tv[2]);
@@ -1905,6 +1912,31 @@ struct AnyScene {
     std::unique_ptr<Scene<double>> d;
 };
 
+/* SurfPoint of the closest hit of each ray (pos, normal, u, v); rays that
+ * miss get NaN (src/sphere.cc:107-129, src/cylinder.cc:166-182,
+ * src/mesh.cc:147-163, 293-306, interp_face :321-445) */
+template <class T> static int do_surface(Scene<T> &S, const sray_ray *rays, uint32_t n, float *out)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        Ray<T> r;
+        r.o = v3<T>(rays[i].origin[0], rays[i].origin[1], rays[i].origin[2]);
+        r.d = v3<T>(rays[i].dir[0], rays[i].dir[1], rays[i].dir[2]);
+        r.time = rays[i].time;
+        Counters cnt;
+        Best<T> b = S.cast(r, cnt, false);
+        float *o = out + (size_t)i * 8;
+        if (!b.found()) {
+            for (int k = 0; k < 8; k++) o[k] = NAN;
+            continue;
+        }
+        SurfPoint<T> sp = S.surf(r, (int)b.item, b.sub, b.t);
+        o[0] = (float)sp.pos.x, o[1] = (float)sp.pos.y, o[2] = (float)sp.pos.z;
+        o[3] = (float)sp.normal.x, o[4] = (float)sp.normal.y, o[5] = (float)sp.normal.z;
+        o[6] = (float)sp.u, o[7] = (float)sp.v;
+    }
+    return 0;
+}
+
 template <class T> static int do_trace(Scene<T> &S, const sray_ray *rays, uint32_t n, sray_hit *hits,
                                        oracle_hit_ext *ext, int any, int nthreads)
 {
@@ -2188,6 +2220,13 @@ int oracle_cell_from_dir(float x, float y, float z, int *uc, int *vc)
     if (uc) *uc = u;
     if (vc) *vc = v;
     return 0;
+}
+
+int oracle_surface(oracle_scene *s, const sray_ray *rays, uint32_t n, float *out)
+{
+    if (!s || (n && (!rays || !out))) return -1;
+    if (s->s.prec == 32) return do_surface(*s->s.f, rays, n, out);
+    return do_surface(*s->s.d, rays, n, out);
 }
 
 double oracle_scene_diag(oracle_scene *s)

@@ -88,6 +88,10 @@ int oracle_projmap(oracle_scene *s, int light, int kind, int t0, int t1, uint64_
 /* ProjMap cell of a direction (the transcendental-free definition) */
 int oracle_cell_from_dir(float x, float y, float z, int *uc, int *vc);
 
+/* surface point of each ray's closest hit: 8 floats per ray (pos, normal,
+ * u, v; NaN on a miss) */
+int oracle_surface(oracle_scene *s, const sray_ray *rays, uint32_t n, float *out);
+
 /* scene octree root diagonal (Scene::build_tree, src/scene.cc:384-387) */
 double oracle_scene_diag(oracle_scene *s);
 

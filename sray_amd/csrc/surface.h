@@ -12,57 +12,76 @@
 
 namespace srd {
 
-/* interp_face, src/mesh.cc:321-445 (inverse bilinear u,v; lerped attributes) */
-__device__ inline void interp_face(const float4 *tri, const float *tn, const float *tg, const float *tuv, uint32_t ti, V3 ispos,
-                            V3 normal, V3 &on, V3 &og, float &ou, float &ov)
+/* interp_face, src/mesh.cc:321-445 (inverse bilinear u,v of the quad v0 v1
+ * v2 v2; lerped attributes).  Evaluated in double: for this degenerate quad
+ * du2 is zero up to rounding, and float rounding noise would pass the
+ * XSMALL test into the quadratic branch; the reference computes in double. */
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 d3(float4 a) { return D3{(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ D3 d3(V3 a) { return D3{(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ D3 dsub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 dadd(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 dneg(D3 a) { return D3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ D3 ddiv(D3 a, double s) { return D3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 dcross(D3 a, D3 b)
 {
-    float4 A = tri[(size_t)ti * 3], B = tri[(size_t)ti * 3 + 1], C = tri[(size_t)ti * 3 + 2];
-    V3 p00 = v3(A.x, A.y, A.z), p10 = v3(B.x, B.y, B.z), p01 = v3(C.x, C.y, C.z), p11 = p01;
-    V3 pa = add(sub(p00, p10), sub(p11, p01));
-    V3 pb = sub(p10, p00);
-    V3 pc = sub(p01, p00);
-    V3 pd = p00;
-    V3 na = cross(pa, normal), nb = cross(pb, normal), nc = cross(pc, normal);
-    float du0 = dot(nc, pd);
-    float du1 = dot(na, pd) + dot(nc, pb);
-    float du2 = dot(na, pb);
-    float dv0 = dot(nb, pd);
-    float dv1 = dot(na, pd) + dot(nb, pc);
-    float dv2 = dot(na, pc);
-    float ru, rv;
-    if (fabsf(du2) <= XSMALL) {
-        float divisor = du1 - dot(na, ispos);
-        ru = fabsf(divisor) > XSMALL ? ((dot(nc, ispos) - du0) / divisor) : 0.0f;
+    return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+__device__ inline void interp_face(const float4 *tri, const float *tn, const float *tg, const float *tuv, uint32_t ti,
+                                   V3 ispos_f, V3 normal_f, V3 &on, V3 &og, float &ou, float &ov)
+{
+    const double XS = SRAY_XSMALL_NUMBER;
+    D3 p00 = d3(tri[(size_t)ti * 3]), p10 = d3(tri[(size_t)ti * 3 + 1]), p01 = d3(tri[(size_t)ti * 3 + 2]), p11 = p01;
+    D3 ispos = d3(ispos_f), normal = d3(normal_f);
+    D3 pa = dadd(dsub(p00, p10), dsub(p11, p01));
+    D3 pb = dsub(p10, p00);
+    D3 pc = dsub(p01, p00);
+    D3 pd = p00;
+    D3 na = dcross(pa, normal), nb = dcross(pb, normal), nc = dcross(pc, normal);
+    double du0 = ddot(nc, pd);
+    double du1 = ddot(na, pd) + ddot(nc, pb);
+    double du2 = ddot(na, pb);
+    double dv0 = ddot(nb, pd);
+    double dv1 = ddot(na, pd) + ddot(nb, pc);
+    double dv2 = ddot(na, pc);
+    double ru, rv;
+    if (fabs(du2) <= XS) {
+        double divisor = du1 - ddot(na, ispos);
+        ru = fabs(divisor) > XS ? ((ddot(nc, ispos) - du0) / divisor) : 0.0;
     } else {
-        V3 qux = dvs(na, 2.0f * du2);
-        V3 quy = dvs(neg(nc), du2);
-        float dux = -du1 / (2.0f * du2);
-        float duy = du0 / du2;
-        float ka = dux + dot(qux, ispos);
-        float kb = duy + dot(quy, ispos);
-        float u0 = ka - sqrtf(ka * ka - kb);
-        float u1 = ka + sqrtf(ka * ka - kb);
-        ru = (u0 < 0.0f || u0 > 1.0f) ? u1 : u0;
+        D3 qux = ddiv(na, 2.0 * du2);
+        D3 quy = ddiv(dneg(nc), du2);
+        double dux = -du1 / (2.0 * du2);
+        double duy = du0 / du2;
+        double ka = dux + ddot(qux, ispos);
+        double kb = duy + ddot(quy, ispos);
+        double u0 = ka - sqrt(ka * ka - kb);
+        double u1 = ka + sqrt(ka * ka - kb);
+        ru = (u0 < 0.0 || u0 > 1.0) ? u1 : u0;
     }
-    if (fabsf(dv2) <= XSMALL) {
-        float divisor = dv1 - dot(na, ispos);
-        rv = fabsf(divisor) > XSMALL ? ((dot(nb, ispos) - dv0) / divisor) : 0.0f;
+    if (fabs(dv2) <= XS) {
+        double divisor = dv1 - ddot(na, ispos);
+        rv = fabs(divisor) > XS ? ((ddot(nb, ispos) - dv0) / divisor) : 0.0;
     } else {
-        V3 qvx = dvs(na, 2.0f * dv2);
-        V3 qvy = dvs(neg(nb), dv2);
-        float dvx = -dv1 / (2.0f * dv2);
-        float dvy = dv0 / dv2;
-        float ka = dvx + dot(qvx, ispos);
-        float kb = dvy + dot(qvy, ispos);
-        float v0 = ka - sqrtf(ka * ka - kb);
-        float v1 = ka + sqrtf(ka * ka - kb);
-        rv = (v0 < 0.0f || v0 > 1.0f) ? v1 : v0;
+        D3 qvx = ddiv(na, 2.0 * dv2);
+        D3 qvy = ddiv(dneg(nb), dv2);
+        double dvx = -dv1 / (2.0 * dv2);
+        double dvy = dv0 / dv2;
+        double ka = dvx + ddot(qvx, ispos);
+        double kb = dvy + ddot(qvy, ispos);
+        double v0 = ka - sqrt(ka * ka - kb);
+        double v1 = ka + sqrt(ka * ka - kb);
+        rv = (v0 < 0.0 || v0 > 1.0) ? v1 : v0;
     }
-    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+    auto lerp = [](double a, double b, double t) { return a + (b - a) * t; };
     auto l2 = [&](float a00, float a10, float a01) {
-        float s0 = lerp(a00, a01, rv);
-        float s1 = lerp(a10, a01, rv);
-        return lerp(s0, s1, ru);
+        double s0 = lerp((double)a00, (double)a01, rv);
+        double s1 = lerp((double)a10, (double)a01, rv);
+        return (float)lerp(s0, s1, ru);
     };
     if (tuv) {
         const float *t = tuv + (size_t)ti * 6;

@@ -56,7 +56,7 @@ def material(kd=(0, 0, 0), ks=0.0, refl=0.0, refr=0.0, ior=1.5, flags=None):
 
 
 def build(spheres=(), cylinders=(), meshes=(), materials=None, lights=(), camera=None, ambient=(0, 0, 0, 0),
-          bg=(0, 0, 0, 0), scene_oct=(5, 5), mesh_oct=(7, 14)):
+          bg=(0, 0, 0, 0), scene_oct=(5, 5), mesh_oct=(7, 14), normals=None, texcoords=None):
     """spheres: (center, radius, mat); cylinders: (center, radius, start, end, mat);
     meshes: (triangles [N,3,3] local, translation, mat);
     lights: (pos, color[, type, radius, dim])"""
@@ -102,6 +102,14 @@ def build(spheres=(), cylinders=(), meshes=(), materials=None, lights=(), camera
     tp = np.ascontiguousarray(np.concatenate(tris).ravel() if tris else np.zeros(9), np.float32)
     b.keep.append(tp)
     d.num_triangles, d.tri_pos = ntri, tp.ctypes.data_as(C.POINTER(C.c_float))
+    if normals is not None:  # per corner, all triangles: [ntri, 3, 3]
+        tn = np.ascontiguousarray(np.asarray(normals, np.float32).ravel())
+        b.keep.append(tn)
+        d.tri_normal = tn.ctypes.data_as(C.POINTER(C.c_float))
+    if texcoords is not None:  # [ntri, 3, 2]
+        tt = np.ascontiguousarray(np.asarray(texcoords, np.float32).ravel())
+        b.keep.append(tt)
+        d.tri_texcoord = tt.ctypes.data_as(C.POINTER(C.c_float))
     mats = materials if materials is not None else [phong()]
     MT = abi.MaterialDesc * max(len(mats), 1)
     mt = MT(*mats)

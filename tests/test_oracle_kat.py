@@ -29,6 +29,10 @@ def trace(b, r, prec, flags=0, any_hit=False):
     return h, e
 
 
+def close(a, b, tol):
+    return np.allclose(np.asarray(a, np.float64), np.asarray(b, np.float64), rtol=0, atol=tol)
+
+
 def golden():
     with open(GOLDEN) as f:
         return json.load(f)
@@ -331,3 +335,79 @@ def test_gather_knn_reduces_to_range():
     assert np.allclose(o.gather(0, q, 0.5, k=10000), full)
     k20 = o.gather(0, q, 0.5, k=20)
     assert np.allclose(k20[0], 20 / (math.pi * 0.25))
+
+
+# ---------------------------------------------------------------- surface points
+@pytest.mark.parametrize("prec", PRECS)
+def test_surface_points_closed_forms(prec):
+    """SurfPoint of the winning item: sphere normal (p-c)/r with the
+    spherical texture coordinates, cylinder normal (x,0,z)/r, and a
+    triangle's interpolated normal and texcoords = barycentric interpolation
+    of its vertex attributes (interp_face's inverse bilinear map of the
+    degenerate quad v0 v1 v2 v2 reduces to it)"""
+    tol = 1e-4 if prec == 32 else 1e-9
+    rng = np.random.default_rng(17)
+    # sphere
+    c, r = np.array([1.0, 2.0, 3.0]), 1.5
+    b = build(spheres=[(tuple(c), r, 0)])
+    o = oracle.OracleScene(b.ptr(), prec)
+    dirs = rng.normal(size=(200, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    targets = c + r * dirs
+    origins = targets + 5.0 * dirs  # outside, looking back at the target point
+    sp = o.surface(rays(origins, (targets - origins) * 1.5))
+    assert close(sp[:, 0:3], targets, 1e-5 if prec == 32 else 1e-6)
+    assert close(sp[:, 3:6], dirs, 1e-5 if prec == 32 else 1e-6)
+    lp = targets - c
+    u = (np.arctan2(lp[:, 2], lp[:, 0]) + math.pi) / (2 * math.pi)
+    v = np.arccos(np.clip(lp[:, 1] / r, -1, 1)) / math.pi
+    du = np.abs(sp[:, 6] - u)
+    du = np.minimum(du, 1 - du)  # the seam at theta = +-pi
+    assert du.max() < 1e-5 and np.abs(sp[:, 7] - v).max() < 1e-4
+    # cylinder along y, radius 0.8, y in [-1, 2)
+    b = build(cylinders=[((0.5, 0.0, -0.5), 0.8, -1.0, 2.0, 0)])
+    o = oracle.OracleScene(b.ptr(), prec)
+    ang = rng.uniform(0, 2 * math.pi, 100)
+    ys = rng.uniform(-0.9, 1.9, 100)
+    nrm = np.stack([np.cos(ang), np.zeros(100), np.sin(ang)], 1)
+    targets = np.array([0.5, 0.0, -0.5]) + 0.8 * nrm + np.stack([np.zeros(100), ys, np.zeros(100)], 1)
+    origins = targets + 4.0 * nrm
+    sp = o.surface(rays(origins, (targets - origins) * 1.5))
+    assert close(sp[:, 0:3], targets, 1e-5 if prec == 32 else 1e-6)
+    assert close(sp[:, 3:6], nrm, 1e-5 if prec == 32 else 1e-6)
+    # triangles with vertex normals and texcoords
+    tris, nrms, uvs, pts, want_n, want_uv = [], [], [], [], [], []
+    for k in range(60):
+        v0 = rng.uniform(-1, 1, 3) + np.array([3.0 * (k % 8), 3.0 * (k // 8), 0.0])
+        e1, e2 = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        tri = np.array([v0, v0 + e1, v0 + e2])
+        n = rng.normal(size=(3, 3))
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        t = rng.uniform(0, 1, (3, 2))
+        a, bb = rng.uniform(0.1, 0.45, 2)
+        w = np.array([1 - a - bb, a, bb])
+        p = w @ tri
+        tris.append(tri), nrms.append(n), uvs.append(t), pts.append(p)
+        ni = w @ n
+        want_n.append(ni / np.linalg.norm(ni))
+        want_uv.append(w @ t)
+    b = build(meshes=[(np.array(tris), (0, 0, 0), 0)], normals=np.array(nrms), texcoords=np.array(uvs))
+    o = oracle.OracleScene(b.ptr(), prec, oracle.BRUTE_FORCE)
+    pts = np.array(pts)
+    gn = np.cross(np.array(tris)[:, 1] - np.array(tris)[:, 0], np.array(tris)[:, 2] - np.array(tris)[:, 0])
+    gn /= np.linalg.norm(gn, axis=1, keepdims=True)
+    origins = pts + 0.7 * gn
+    sp = o.surface(rays(origins, (pts - origins) * 1.5))
+    rel = 5e-6  # the rays and the stored positions are float
+    assert np.all(np.abs(sp[:, 0:3] - pts) <= rel * (1.0 + np.abs(pts)))
+    # expected attributes at the returned hit point, from the float-stored
+    # triangle and attributes: barycentric interpolation
+    T32 = np.array(tris, np.float32).astype(np.float64)
+    N32, U32 = np.array(nrms, np.float32).astype(np.float64), np.array(uvs, np.float32).astype(np.float64)
+    for k in range(len(tris)):
+        e = np.stack([T32[k, 1] - T32[k, 0], T32[k, 2] - T32[k, 0]], 1)
+        ab = np.linalg.lstsq(e, sp[k, 0:3].astype(np.float64) - T32[k, 0], rcond=None)[0]
+        w = np.array([1 - ab[0] - ab[1], ab[0], ab[1]])
+        n = w @ N32[k]
+        assert close(sp[k, 3:6], n / np.linalg.norm(n), 2e-5), k
+        assert close(sp[k, 6:8], w @ U32[k], 2e-5), k
