@@ -12,6 +12,16 @@
  *   prec = 32 : float, strict IEEE (built -ffp-contract=off, no fast-math),
  *               with the same operation order as the device kernels — the
  *               target for bit-exact hit IDs.
+ *
+ * Pins (tests/test_oracle_kat.py, tests/test_photons_cpu.py,
+ * tests/test_host.py): closed forms, brute force and invariants for every
+ * function here, except the following, which are parity unpinned:
+ *   - the sample points of sphere and box lights (Light::get_point,
+ *     src/light.cc:272-275, 346-354) beyond their use in the renders: their
+ *     distribution comes from the build's own generator;
+ *   - ERROR_MARGIN / XSMALL_NUMBER and the refraction formula, which live in
+ *     the un-vendored libvmath: the values and the textbook form are fixed
+ *     here (the glass and Snell closed forms pin the form, not libvmath).
  */
 #ifndef SRAY_ORACLE_H_
 #define SRAY_ORACLE_H_

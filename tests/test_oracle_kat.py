@@ -411,3 +411,43 @@ def test_surface_points_closed_forms(prec):
         n = w @ N32[k]
         assert close(sp[k, 3:6], n / np.linalg.norm(n), 2e-5), k
         assert close(sp[k, 6:8], w @ U32[k], 2e-5), k
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_shade_undef_checker_closed_form(prec):
+    """an object without a material uses shade_undef (src/shader.cc:36-71):
+    checker colour from the spherical texcoords (20 x 10 cells), diffuse 0.1
+    + ndotl, Blinn highlight ndoth^50"""
+    c0 = np.array([0.7, 0.4, 10.0])
+    b = build(spheres=[(tuple(c0), 2.0, -1)], materials=[], lights=[((0, 0, 0), (1, 1, 1))])
+    col, st = _center_pixel(b, prec, iter=2)
+    lp = np.array([-0.7, -0.4, -math.sqrt(4 - 0.49 - 0.16)])  # local hit point of the +z ray
+    u = (math.atan2(lp[2], lp[0]) + math.pi) / (2 * math.pi)
+    v = math.acos(lp[1] / 2.0) / math.pi
+    x, y = int(u * 20), int(v * 10)
+    tex = np.array([1.0, 0.3, 0.2]) if x % 2 == y % 2 else np.array([0.2, 0.3, 1.0])
+    n = lp / 2.0
+    ldir = np.array([0.0, 0.0, -1.0])  # light at the eye, hit on the axis
+    ndotl = max(float(n @ ldir), 0.0)
+    want = (0.1 + ndotl) * tex + ndotl ** 50
+    tol = 1e-6 if prec == 64 else 1e-5  # the framebuffer is float
+    assert np.allclose(col[:3], want, rtol=tol, atol=tol), (col, want)
+    assert col[3] == 1.0 and st.rays_shadow == 1
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_motion_translation_closed_form(prec):
+    """linear translation keys (src/anim.inl:188-261): a sphere moving by
+    (3,0,0) over [0,40] ms is met at the position of the ray's time, clamped
+    outside the key interval"""
+    b = build(spheres=[((0, 0, 10), 2.0, 0)])
+    b.desc.objects[0].motion[:] = (3.0, 0.0, 0.0)
+    b.desc.objects[0].motion_t0, b.desc.objects[0].motion_t1 = 0, 40
+    o = oracle.OracleScene(b.ptr(), prec, oracle.BRUTE_FORCE)
+    r = rays([(1.5, 0, 0)] * 5, [(0, 0, 100)] * 5)
+    r["time"] = [-10, 0, 20, 40, 70]
+    h, _ = o.trace(r)
+    off = np.abs(1.5 - np.array([0.0, 0.0, 1.5, 3.0, 3.0]))
+    want = (10.0 - np.sqrt(4.0 - off ** 2)) / 100.0
+    assert np.allclose(h["t"], want, rtol=1e-6), (h["t"], want)
+    assert np.all(h["obj"] == 0)
