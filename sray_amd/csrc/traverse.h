@@ -255,142 +255,145 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
     stk[0] = (S.scene_root << 8) | (1u << r.dmask);
     sp = 1;
     for (;;) {
-        if (item_cur < item_end) {
-            if (level) {
-                /* mesh leaf: all its triangles */
-                for (; item_cur < item_end; item_cur++) {
-                    uint32_t ti = S.items[item_cur];
-                    if (COUNT) {
-                        cnt.item_refs++;
-                        cnt.triangles++;
-                    }
-                    float4 A = S.tri[(size_t)ti * 3], B = S.tri[(size_t)ti * 3 + 1], C = S.tri[(size_t)ti * 3 + 2];
-                    float t;
-                    if (tri_t(A, B, C, r, t)) {
-                        if (ANY) {
-                            best.t = t;
-                            best.tri = ti;
-                            best.obj = cur_mesh;
-                            return best;
-                        }
-                        if (t < m_t || (t == m_t && item_cur < m_rank)) {
-                            m_t = t;
-                            m_tri = ti;
-                            m_rank = item_cur;
-                        }
-                    }
-                }
-                continue;
-            }
-            uint32_t rank = item_cur++;
-            int oi = (int)S.items[rank];
-            if (COUNT) cnt.item_refs++;
-            const DevObj &ob = S.objs[oi];
-            int type = ob.type;
-            if (type == SRAY_OBJ_MESH) {
-                if (!ANY && oi == cache_obj) {
-                    if (cache_tri != 0xffffffffu &&
-                        (cache_t < best.t || (cache_t == best.t && rank < best.rank))) {
-                        best.t = cache_t;
+        /* while-while: walk the stack until this lane has items to test, so
+         * lanes meet at the item tests instead of alternating node visits
+         * and leaf loops */
+        while (item_cur >= item_end && sp > 0) {
+            uint32_t e = stk[(sp - 1) * stride];
+            if ((e & E_MARK) == E_MARK) {
+                sp--;
+                uint32_t rank = e & 0x3fffffffu;
+                int oi = (int)S.items[rank];
+                if (!ANY) {
+                    cache_obj = oi;
+                    cache_t = m_t;
+                    cache_tri = m_tri;
+                    if (m_tri != 0xffffffffu && (m_t < best.t || (m_t == best.t && rank < best.rank))) {
+                        best.t = m_t;
                         best.obj = oi;
-                        best.tri = cache_tri;
+                        best.tri = m_tri;
                         best.rank = rank;
                     }
-                    continue;
                 }
-                if (COUNT) cnt.mesh_enters++;
-                if (item_cur < item_end) {
-                    stk[(sp++) * stride] = item_cur;
-                    stk[(sp++) * stride] = E_RES | item_end;
-                }
-                stk[(sp++) * stride] = E_MARK | rank;
-                float I[12];
-                obj_inv_at(ob, time, I);
-                r = make_tray(xf_point(I, wo), xf_vec(I, wd));
-                stk[(sp++) * stride] = (ob.root << 8) | (1u << r.dmask);
-                item_cur = item_end = 0;
-                level = 1;
-                cur_mesh = oi;
-                m_t = FLT_MAX;
-                m_tri = 0xffffffffu;
-                m_rank = 0xffffffffu;
+                r = wr;
+                level = 0;
                 continue;
             }
+            if (e & E_RES) {
+                item_end = e & 0x3fffffffu;
+                item_cur = stk[(sp - 2) * stride];
+                sp -= 2;
+                continue;
+            }
+            uint32_t pend = e & 0xffu;
+            uint32_t k = (uint32_t)__builtin_ctz(pend);
+            pend &= pend - 1u;
+            if (pend == 0u) sp--;
+            else stk[(sp - 1) * stride] = (e & ~0xffu) | pend;
+            uint32_t child = (e >> 8) + (k ^ r.dmask);
+            float4 lo = S.nlo[child], hi = S.nhi[child];
+            if (COUNT) cnt.nodes++;
+            float tmin;
+            if (!slab(lo, hi, r, tmin)) continue;
+            if (!ANY) {
+                float bound = level ? fminf(best.t, m_t) : best.t;
+                if (tmin > bound) continue;
+            }
+            uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
+            if (meta & LEAF) {
+                item_cur = first;
+                item_end = first + (meta & ~LEAF);
+            } else {
+                stk[(sp++) * stride] = (first << 8) | order_mask(meta, r.dmask);
+            }
+        }
+        if (item_cur >= item_end) break; /* stack empty */
+        if (level) {
+            /* mesh leaf: all its triangles */
+            for (; item_cur < item_end; item_cur++) {
+                uint32_t ti = S.items[item_cur];
+                if (COUNT) {
+                    cnt.item_refs++;
+                    cnt.triangles++;
+                }
+                float4 A = S.tri[(size_t)ti * 3], B = S.tri[(size_t)ti * 3 + 1], C = S.tri[(size_t)ti * 3 + 2];
+                float t;
+                if (tri_t(A, B, C, r, t)) {
+                    if (ANY) {
+                        best.t = t;
+                        best.tri = ti;
+                        best.obj = cur_mesh;
+                        return best;
+                    }
+                    if (t < m_t || (t == m_t && item_cur < m_rank)) {
+                        m_t = t;
+                        m_tri = ti;
+                        m_rank = item_cur;
+                    }
+                }
+            }
+            continue;
+        }
+        uint32_t rank = item_cur++;
+        int oi = (int)S.items[rank];
+        if (COUNT) cnt.item_refs++;
+        const DevObj &ob = S.objs[oi];
+        int type = ob.type;
+        if (type == SRAY_OBJ_MESH) {
+            if (!ANY && oi == cache_obj) {
+                if (cache_tri != 0xffffffffu &&
+                    (cache_t < best.t || (cache_t == best.t && rank < best.rank))) {
+                    best.t = cache_t;
+                    best.obj = oi;
+                    best.tri = cache_tri;
+                    best.rank = rank;
+                }
+                continue;
+            }
+            if (COUNT) cnt.mesh_enters++;
+            if (item_cur < item_end) {
+                stk[(sp++) * stride] = item_cur;
+                stk[(sp++) * stride] = E_RES | item_end;
+            }
+            stk[(sp++) * stride] = E_MARK | rank;
             float I[12];
             obj_inv_at(ob, time, I);
-            V3 lo = xf_point(I, wo), ld = xf_vec(I, wd);
-            float t;
-            bool h;
-            if (type == SRAY_OBJ_SPHERE) {
-                if (COUNT) cnt.spheres++;
-                h = sphere_t(ob.rad, lo, ld, t);
-            } else {
-                if (COUNT) cnt.cylinders++;
-                h = cylinder_t(ob.rad, ob.start, ob.end, lo, ld, t);
-            }
-            if (h) {
-                if (ANY) {
-                    best.t = t;
-                    best.obj = oi;
-                    return best;
-                }
-                if (t < best.t || (t == best.t && rank < best.rank)) {
-                    best.t = t;
-                    best.obj = oi;
-                    best.tri = 0xffffffffu;
-                    best.rank = rank;
-                }
-            }
+            r = make_tray(xf_point(I, wo), xf_vec(I, wd));
+            stk[(sp++) * stride] = (ob.root << 8) | (1u << r.dmask);
+            item_cur = item_end = 0;
+            level = 1;
+            cur_mesh = oi;
+            m_t = FLT_MAX;
+            m_tri = 0xffffffffu;
+            m_rank = 0xffffffffu;
             continue;
         }
-        if (sp == 0) break;
-        uint32_t e = stk[(sp - 1) * stride];
-        if ((e & E_MARK) == E_MARK) {
-            sp--;
-            uint32_t rank = e & 0x3fffffffu;
-            int oi = (int)S.items[rank];
-            if (!ANY) {
-                cache_obj = oi;
-                cache_t = m_t;
-                cache_tri = m_tri;
-                if (m_tri != 0xffffffffu && (m_t < best.t || (m_t == best.t && rank < best.rank))) {
-                    best.t = m_t;
-                    best.obj = oi;
-                    best.tri = m_tri;
-                    best.rank = rank;
-                }
-            }
-            r = wr;
-            level = 0;
-            continue;
-        }
-        if (e & E_RES) {
-            item_end = e & 0x3fffffffu;
-            item_cur = stk[(sp - 2) * stride];
-            sp -= 2;
-            continue;
-        }
-        uint32_t pend = e & 0xffu;
-        uint32_t k = (uint32_t)__builtin_ctz(pend);
-        pend &= pend - 1u;
-        if (pend == 0u) sp--;
-        else stk[(sp - 1) * stride] = (e & ~0xffu) | pend;
-        uint32_t child = (e >> 8) + (k ^ r.dmask);
-        float4 lo = S.nlo[child], hi = S.nhi[child];
-        if (COUNT) cnt.nodes++;
-        float tmin;
-        if (!slab(lo, hi, r, tmin)) continue;
-        if (!ANY) {
-            float bound = level ? fminf(best.t, m_t) : best.t;
-            if (tmin > bound) continue;
-        }
-        uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
-        if (meta & LEAF) {
-            item_cur = first;
-            item_end = first + (meta & ~LEAF);
+        float I[12];
+        obj_inv_at(ob, time, I);
+        V3 lo = xf_point(I, wo), ld = xf_vec(I, wd);
+        float t;
+        bool h;
+        if (type == SRAY_OBJ_SPHERE) {
+            if (COUNT) cnt.spheres++;
+            h = sphere_t(ob.rad, lo, ld, t);
         } else {
-            stk[(sp++) * stride] = (first << 8) | order_mask(meta, r.dmask);
+            if (COUNT) cnt.cylinders++;
+            h = cylinder_t(ob.rad, ob.start, ob.end, lo, ld, t);
         }
+        if (h) {
+            if (ANY) {
+                best.t = t;
+                best.obj = oi;
+                return best;
+            }
+            if (t < best.t || (t == best.t && rank < best.rank)) {
+                best.t = t;
+                best.obj = oi;
+                best.tri = 0xffffffffu;
+                best.rank = rank;
+            }
+        }
+        continue;
     }
     return best;
 }
