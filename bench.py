@@ -401,6 +401,9 @@ def main():
     log(args, f"per-kernel ms/step: " + ", ".join(f"{k} {v['ms_per_step']:.3f}" for k, v in kernels.items()))
     log(args, f"closest-hit visits/frame: rays {closest_rays} nodes {cl.nodes} tris {cl.triangles} "
               f"spheres {cl.spheres} idx {cl.item_refs}; alg bytes/frame {alg_bytes / 1e6:.1f} MB")
+    sh = cst.shadow
+    log(args, f"shadow visits/frame: rays {cst.rays_shadow} nodes {sh.nodes} tris {sh.triangles} "
+              f"spheres {sh.spheres} idx {sh.item_refs} mesh enters {sh.mesh_enters}")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, hs, wl)

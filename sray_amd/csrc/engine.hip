@@ -515,6 +515,23 @@ __global__ void k_iota(uint32_t *p, uint32_t n, uint32_t base)
     if (i < n) p[i] = base + i;
 }
 
+/* the arena pixels of a W x H rectangle in 8x8-tile order (rows of tiles,
+ * tiles left to right, row-major inside a tile), so the 64 primary rays of
+ * a wave cover a compact tile */
+__global__ void k_tile_order(uint32_t *__restrict__ act, uint32_t npix, uint32_t base, uint32_t W, uint32_t H)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    uint32_t ty = i / (8u * W);
+    uint32_t o = i - ty * 8u * W;
+    uint32_t hr = min(8u, H - 8u * ty);
+    uint32_t tx = o / (8u * hr);
+    uint32_t wt = min(8u, W - 8u * tx);
+    uint32_t j = o - tx * 8u * hr;
+    uint32_t y = 8u * ty + j / wt, x = 8u * tx + j % wt;
+    act[i] = base + y * W + x;
+}
+
 __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -917,8 +934,9 @@ int admit(Engine &E, Job *j)
     k_fill_f4<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<float4>(E.fb) + base, npix, make_float4(0, 0, 0, 0));
     k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.nsamp) + base, npix, 0u);
     k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.preq) + base, npix, s);
-    k_iota<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.act[E.cur]) + E.nact, (uint32_t)npix,
-                                                     (uint32_t)base);
+    k_tile_order<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.act[E.cur]) + E.nact, (uint32_t)npix,
+                                                           (uint32_t)base, (uint32_t)Q.P.W,
+                                                           (uint32_t)(Q.P.y1 - Q.P.y0));
     HIPCHK(c, hipGetLastError());
     E.nact += (uint32_t)npix;
     E.hreqs[s] = Q;

@@ -22,8 +22,8 @@
  * Children sit in eight-slot blocks (child with octant bits k at first + k),
  * so an ITER entry needs no child mask; four bytes per entry keep the LDS
  * footprint at 80 bytes per lane.
- * A mesh hit by the same ray in several scene leaves (Q9) is traversed once:
- * its result is memoised for the ray.
+ * A mesh met by the same ray in several scene leaves (Q9) is traversed once:
+ * its result is memoised for the ray (for an any-hit ray, its miss).
  */
 #ifndef SRAY_TRAVERSE_H_
 #define SRAY_TRAVERSE_H_
@@ -264,8 +264,8 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                 sp--;
                 uint32_t rank = e & 0x3fffffffu;
                 int oi = (int)S.items[rank];
+                cache_obj = oi;
                 if (!ANY) {
-                    cache_obj = oi;
                     cache_t = m_t;
                     cache_tri = m_tri;
                     if (m_tri != 0xffffffffu && (m_t < best.t || (m_t == best.t && rank < best.rank))) {
@@ -340,8 +340,11 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
         const DevObj &ob = S.objs[oi];
         int type = ob.type;
         if (type == SRAY_OBJ_MESH) {
-            if (!ANY && oi == cache_obj) {
-                if (cache_tri != 0xffffffffu &&
+            if (oi == cache_obj) {
+                /* this ray has traversed this mesh already (in another scene
+                 * leaf): same result; an any-hit ray only gets here after
+                 * finding nothing in it */
+                if (!ANY && cache_tri != 0xffffffffu &&
                     (cache_t < best.t || (cache_t == best.t && rank < best.rank))) {
                     best.t = cache_t;
                     best.obj = oi;
