@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--scenes", type=int, default=None, help="batch: scenes (256); serve: resident scenes (32)")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrival rate (req/s)")
     ap.add_argument("--duration", type=float, default=0.25, help="serve: arrival window (s)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device three times (scales the HBM byte counters)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -350,6 +352,19 @@ def main():
     if args.gpus > 1:
         sys.exit("bench.py measures one GPU only: every box here has one MI355X and no multi-GPU "
                  "figure is produced (run with --gpus 1)")
+    if args.copy_calibration:
+        import sray_amd as S
+        ctx = S.Context(0)
+        n = 256 << 20
+        a, b = ctx.malloc(n), ctx.malloc(n)
+        for _ in range(3):
+            ctx.memcpy_device(b, a, n)
+        ctx.synchronize()
+        ctx.free(a)
+        ctx.free(b)
+        ctx.close()
+        print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+        return
     if args.scenes is None:
         args.scenes = 256 if args.workload == "batch" else 32
     if args.workload != "heightfield":

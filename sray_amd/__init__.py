@@ -332,6 +332,10 @@ class Context:
         self._check(lib().sray_gpu_memcpy(self._h, C.c_void_p(d), C.c_void_p(arr.ctypes.data), arr.nbytes, 0))
         return d
 
+    def memcpy_device(self, dst, src, nbytes):
+        """device-to-device copy on the context's device"""
+        self._check(lib().sray_gpu_memcpy(self._h, C.c_void_p(dst), C.c_void_p(src), int(nbytes), 2))
+
     def to_host(self, dptr, nbytes, dtype):
         out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
         self._check(lib().sray_gpu_memcpy(self._h, C.c_void_p(out.ctypes.data), C.c_void_p(dptr), nbytes, 1))

@@ -515,20 +515,22 @@ __global__ void k_iota(uint32_t *p, uint32_t n, uint32_t base)
     if (i < n) p[i] = base + i;
 }
 
-/* the arena pixels of a W x H rectangle in 8x8-tile order (rows of tiles,
- * tiles left to right, row-major inside a tile), so the 64 primary rays of
- * a wave cover a compact tile */
-__global__ void k_tile_order(uint32_t *__restrict__ act, uint32_t npix, uint32_t base, uint32_t W, uint32_t H)
+/* the arena pixels of a W x H rectangle in tile order: tiles of tw x th
+ * pixels (tw * th = 64, one wave), rows of tiles top to bottom, tiles left
+ * to right, row-major inside a tile (tw = 64: plain row order) */
+__global__ void k_tile_order(uint32_t *__restrict__ act, uint32_t npix, uint32_t base, uint32_t W, uint32_t H,
+                             uint32_t tw)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npix) return;
-    uint32_t ty = i / (8u * W);
-    uint32_t o = i - ty * 8u * W;
-    uint32_t hr = min(8u, H - 8u * ty);
-    uint32_t tx = o / (8u * hr);
-    uint32_t wt = min(8u, W - 8u * tx);
-    uint32_t j = o - tx * 8u * hr;
-    uint32_t y = 8u * ty + j / wt, x = 8u * tx + j % wt;
+    uint32_t th = 64u / tw;
+    uint32_t ty = i / (th * W);
+    uint32_t o = i - ty * th * W;
+    uint32_t hr = min(th, H - th * ty);
+    uint32_t tx = o / (tw * hr);
+    uint32_t wt = min(tw, W - tw * tx);
+    uint32_t j = o - tx * tw * hr;
+    uint32_t y = th * ty + j / wt, x = tw * tx + j % wt;
     act[i] = base + y * W + x;
 }
 
@@ -767,6 +769,8 @@ struct Engine {
     /* this pass's launch shape */
     int L = 0, K = 2, maxiter = 0;
     bool photons = false, uni = true, counting = false;
+    /* tunables */
+    uint32_t tile_w = 64; /* primary-ray tile width (x 64/tile_w rows) */
     unsigned long long sec_seen = 0; /* secondary rays counted before this pass */
     /* submit / poll */
     std::thread worker;
@@ -936,7 +940,7 @@ int admit(Engine &E, Job *j)
     k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.preq) + base, npix, s);
     k_tile_order<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.act[E.cur]) + E.nact, (uint32_t)npix,
                                                            (uint32_t)base, (uint32_t)Q.P.W,
-                                                           (uint32_t)(Q.P.y1 - Q.P.y0));
+                                                           (uint32_t)(Q.P.y1 - Q.P.y0), E.tile_w);
     HIPCHK(c, hipGetLastError());
     E.nact += (uint32_t)npix;
     E.hreqs[s] = Q;

@@ -629,12 +629,16 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
         memcpy(d.xf, l.xform, sizeof d.xf);
     }
     /* triangles as 3 float4 */
+    /* three float4 per triangle: the vertices, and in their w components the
+     * unnormalised face normal (v1-v0) x (v2-v0), computed here with the
+     * float operations the intersection test would perform */
     std::vector<float4> tri((size_t)s->num_triangles * 3);
-    for (size_t t = 0; t < s->num_triangles; t++)
-        for (int k = 0; k < 3; k++) {
-            const float *p = s->tri_pos + t * 9 + k * 3;
-            tri[t * 3 + k] = make_float4(p[0], p[1], p[2], 0.0f);
-        }
+    for (size_t t = 0; t < s->num_triangles; t++) {
+        const float *v = s->tri_pos + t * 9;
+        float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        for (int k = 0; k < 3; k++) tri[t * 3 + k] = make_float4(v[k * 3], v[k * 3 + 1], v[k * 3 + 2], n[k]);
+    }
     SceneDev &D = g->dev;
     memset(&D, 0, sizeof D);
     const void *p;

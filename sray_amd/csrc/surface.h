@@ -183,8 +183,7 @@ __device__ __forceinline__ void hit_surface(const SceneDev &S, const DevObj &ob,
         }
     } else {
         float4 A = S.tri[(size_t)ti * 3], B = S.tri[(size_t)ti * 3 + 1], C = S.tri[(size_t)ti * 3 + 2];
-        V3 e1 = v3(B.x - A.x, B.y - A.y, B.z - A.z), e2 = v3(C.x - A.x, C.y - A.y, C.z - A.z);
-        V3 gn = cross(e1, e2);
+        V3 gn = v3(A.w, B.w, C.w); /* (v1-v0) x (v2-v0), stored at upload */
         V3 nrm = gn;
         if (ob.has_attr) {
             V3 in, ig;

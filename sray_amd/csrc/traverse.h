@@ -187,8 +187,7 @@ __device__ __forceinline__ bool cylinder_t(float rad, float ys, float ye, V3 o, 
 __device__ __forceinline__ bool tri_t(float4 A, float4 B, float4 C, const TRay &r, float &t)
 {
     V3 v0 = v3(A.x, A.y, A.z), v1 = v3(B.x, B.y, B.z), v2 = v3(C.x, C.y, C.z);
-    V3 e1 = sub(v1, v0), e2 = sub(v2, v0);
-    V3 n = cross(e1, e2);
+    V3 n = v3(A.w, B.w, C.w); /* (v1-v0) x (v2-v0), stored at upload */
     V3 d = v3(r.dx, r.dy, r.dz), o = v3(r.ox, r.oy, r.oz);
     float ndd = dot(n, d);
     if (fabsf(ndd) < EPS) return false;
