@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--workload", default="heightfield", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--frames", type=int, default=8, help="room: frames in flight")
+    ap.add_argument("--frames", type=int, default=None, help="room / spheres: frames in flight (8 / 64)")
     ap.add_argument("--scenes", type=int, default=None, help="batch: scenes (256); serve: resident scenes (32)")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrival rate (req/s)")
     ap.add_argument("--duration", type=float, default=0.25, help="serve: arrival window (s)")
@@ -158,8 +158,20 @@ def run_spheres(args):
     ctx.synchronize()
     wall = time.perf_counter() - t0
     value = rays / wall / 1e6
+    # the same frame, many in flight (different seeds): the serving engine
+    K = args.frames
+    frames = [S.render_params(W, H, samples=samples, iter=it, seed=0x5EED + k) for k in range(K)]
+    ctx.poll(ctx.submit(sid, frames[0]))
+    t0 = time.perf_counter()
+    tickets = [ctx.submit(sid, q) for q in frames]
+    res = [ctx.poll(t)[1] for t in tickets]
+    bwall = time.perf_counter() - t0
+    batched = sum(r.rays for r in res) / bwall / 1e6
+    log(args, f"spheres: plain {value:.1f} Mrays/s, {K} frames in flight {batched:.1f} Mrays/s")
     cpu = None if args.no_cpu else cpu_baseline(args, hs, "spheres", full=True)
-    extra = {"cpu_baseline": cpu, "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None}
+    extra = {"batched": {"value": round(batched, 3), "frames_in_flight": K, "wall_s": round(bwall, 4)},
+             "cpu_baseline": cpu, "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+             "batched_vs_cpu": round(batched / cpu["value"], 1) if cpu else None}
     print(json.dumps(base_line("Mrays/s on 1 GPU", value, args.steps, args.warmup, wall * 1e3 / args.steps, "f32",
                                f"spheres: 64 spheres, {W}x{H}, 1 spp, depth {it}", extra)), flush=True)
     ctx.free(out)
@@ -367,6 +379,8 @@ def main():
         return
     if args.scenes is None:
         args.scenes = 256 if args.workload == "batch" else 32
+    if args.frames is None:
+        args.frames = 64 if args.workload == "spheres" else 8
     if args.workload != "heightfield":
         return {"spheres": run_spheres, "room": run_room, "batch": run_batch, "serve": run_serve}[args.workload](args)
     import sray_amd as S

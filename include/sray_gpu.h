@@ -342,6 +342,34 @@ int  sray_gpu_shoot_photons(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_pho
 int  sray_gpu_pmap_download(sray_gpu_ctx *ctx, uint32_t pmap_id, sray_photon *out, uint32_t cap,
                             uint32_t *n);
 
+/* Tuning of the device path (no reference counterpart: the reference's
+ * knobs are its thread count and tile size, src/opt.c:338-342).  The values
+ * change speed only; every image stays the same, except that the grid cell
+ * scale changes the order in which a photon gather's sums are added. */
+typedef struct sray_tuning {
+    int32_t trace_block;      /* 64, 128 or 256 threads per traversal block */
+    int32_t tile_w;           /* 8, 16, 32 or 64: primary rays in tiles of tile_w x (64 / tile_w) pixels */
+    int32_t gather_lanes;     /* 8, 16 or 32 lanes sharing one photon-gather query */
+    int32_t chunk_log2;       /* 16..21: rays per bounce-level sub-chunk = 2^chunk_log2 */
+    float   gather_cell;      /* photon-grid cell size in gather radii: 0.5, 1 or 2 */
+    int32_t pad_;
+} sray_tuning;
+int  sray_gpu_set_tuning(sray_gpu_ctx *ctx, const sray_tuning *t);
+int  sray_gpu_get_tuning(sray_gpu_ctx *ctx, sray_tuning *t);
+/* Autotuner: renders params of the scene `rounds` times per candidate,
+ * varying one knob at a time from the current tuning (coordinate descent
+ * over the values above), keeps the fastest tuning in the context and
+ * returns it with its median render time. */
+int  sray_gpu_autotune(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_render_params *p, int rounds,
+                       sray_tuning *best, double *best_ms);
+/* The octree knobs of a scene description (leaf size and depth of the mesh
+ * and scene octrees, src/opt.c:383-387): renders params with each of
+ * candidates (mesh items, mesh depth) pairs and returns the fastest pair.
+ * Changing them changes only which of two exactly equal hit distances
+ * wins (the depth-first rank), never a hit's distance. */
+int  sray_gpu_autotune_octree(sray_gpu_ctx *ctx, const sray_scene_desc *s, const sray_render_params *p, int rounds,
+                              const int32_t *items, const int32_t *depths, int ncand, int *best, double *best_ms);
+
 /* Batching / serving (no reference counterpart: the reference renders one
  * frame of one scene at a time, src/sray.cc:56-65).  submit queues a tile
  * request (its pixel rectangle of params, through the scene's camera or
@@ -432,7 +460,7 @@ int  sray_gpu_kernel_stats(sray_gpu_ctx *ctx, sray_kernel_stat *out, int cap, in
  * sray_scene_desc, sray_ray, sray_hit, sray_trace_counts,
  * sray_render_params, sray_stats, sray_photon, sray_gather_query,
  * sray_request, sray_result, sray_octree_info, sray_kernel_stat,
- * sray_photon_params, sray_photon_stats) — lets
+ * sray_photon_params, sray_photon_stats, sray_tuning) — lets
  * bindings check their
  * layout.  Returns the number of entries written (at most n). */
 int  sray_abi_sizes(uint32_t *out, int n);

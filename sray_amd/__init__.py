@@ -314,6 +314,35 @@ class Context:
         self._check(rc)
         return out, res
 
+    # ---- tuning
+    def tuning(self):
+        t = abi.Tuning()
+        self._check(lib().sray_gpu_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._check(lib().sray_gpu_set_tuning(self._h, C.byref(t)))
+        return t
+
+    def autotune(self, sid, params, rounds=3):
+        """coordinate-descent search over the runtime knobs; keeps and returns (tuning, median ms)"""
+        t, ms = abi.Tuning(), C.c_double()
+        self._check(lib().sray_gpu_autotune(self._h, sid, C.byref(params), rounds, C.byref(t), C.byref(ms)))
+        return t, ms.value
+
+    def autotune_octree(self, desc_ptr, params, candidates, rounds=3):
+        """fastest (mesh items, mesh depth) of candidates: (index, median ms)"""
+        n = len(candidates)
+        items = (C.c_int32 * n)(*[c[0] for c in candidates])
+        depths = (C.c_int32 * n)(*[c[1] for c in candidates])
+        best, ms = C.c_int(), C.c_double()
+        self._check(lib().sray_gpu_autotune_octree(self._h, desc_ptr, C.byref(params), rounds, items, depths, n,
+                                                   C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
     def set_admission_cap(self, nbytes):
         self._check(lib().sray_gpu_set_admission_cap(self._h, int(nbytes)))
 

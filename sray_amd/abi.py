@@ -109,6 +109,11 @@ class PhotonStats(C.Structure):
     _fields_ = [("shot", u64 * 2), ("stored", u64 * 2), ("segments", u64), ("trace_ms", f64), ("build_ms", f64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("trace_block", i32), ("tile_w", i32), ("gather_lanes", i32), ("chunk_log2", i32),
+                ("gather_cell", f32), ("pad_", i32)]
+
+
 class GatherQuery(C.Structure):
     _fields_ = [("pos", f32 * 3), ("norm", f32 * 3), ("dir", f32 * 3), ("pad_", f32)]
 
@@ -133,7 +138,7 @@ class KernelStat(C.Structure):
 
 ABI_STRUCTS = [ObjectDesc, MeshDesc, MaterialDesc, LightDesc, CameraDesc, SceneDesc, Ray, Hit, TraceCounts,
                RenderParams, Stats, Photon, GatherQuery, Request, Result, OctreeInfo, KernelStat, PhotonParams,
-               PhotonStats]
+               PhotonStats, Tuning]
 
 # every extern "C" entry point declared in include/sray_gpu.h
 EXPORTS = [
@@ -146,6 +151,7 @@ EXPORTS = [
     "sray_gpu_primary_rays_device", "sray_gpu_set_profiling", "sray_gpu_kernel_stats",
     "sray_gpu_malloc", "sray_gpu_free", "sray_gpu_memcpy", "sray_gpu_synchronize",
     "sray_gpu_shoot_photons", "sray_gpu_pmap_download", "sray_projmap_build", "sray_projmap_cell",
+    "sray_gpu_set_tuning", "sray_gpu_get_tuning", "sray_gpu_autotune", "sray_gpu_autotune_octree",
 ]
 
 
@@ -189,6 +195,11 @@ def declare(lib):
         "sray_gpu_pmap_download": (i32, [vp, u32, vp, u32, P(u32)]),
         "sray_projmap_build": (i32, [P(SceneDesc), C.c_int, C.c_int, C.c_int, C.c_int, u64, vp, P(f64)]),
         "sray_projmap_cell": (i32, [f32, f32, f32, P(C.c_int), P(C.c_int)]),
+        "sray_gpu_set_tuning": (i32, [vp, P(Tuning)]),
+        "sray_gpu_get_tuning": (i32, [vp, P(Tuning)]),
+        "sray_gpu_autotune": (i32, [vp, u32, P(RenderParams), C.c_int, P(Tuning), P(f64)]),
+        "sray_gpu_autotune_octree": (i32, [vp, P(SceneDesc), P(RenderParams), C.c_int, P(i32), P(i32), C.c_int,
+                                           P(C.c_int), P(f64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -150,7 +150,7 @@ extern "C" int sray_abi_sizes(uint32_t *out, int n)
                            (uint32_t)sizeof(sray_gather_query),  (uint32_t)sizeof(sray_request),
                            (uint32_t)sizeof(sray_result),        (uint32_t)sizeof(sray_octree_info),
                            (uint32_t)sizeof(sray_kernel_stat),   (uint32_t)sizeof(sray_photon_params),
-                           (uint32_t)sizeof(sray_photon_stats)};
+                           (uint32_t)sizeof(sray_photon_stats),  (uint32_t)sizeof(sray_tuning)};
     int k = 0;
     for (; k < n && k < (int)(sizeof sz / sizeof sz[0]); k++) out[k] = sz[k];
     return k;

@@ -107,28 +107,28 @@ __global__ void k_raygen(Tabs T, const uint32_t *__restrict__ active, uint32_t c
 }
 
 /* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
-template <bool COUNT, bool UNI>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_q(Tabs T, const QRay *__restrict__ q, uint32_t n,
-                                                         int4 *__restrict__ hits, unsigned long long *counts)
+template <bool COUNT, bool UNI, int BS>
+__global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__ q, uint32_t n,
+                                                int4 *__restrict__ hits, unsigned long long *counts)
 {
-    __shared__ uint32_t stk[STACK * TRACE_BLOCK];
+    __shared__ uint32_t stk[STACK * BS];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 o = q[i].o, d = q[i].d;
     const SceneDev S = T.scenes[UNI ? 0u : T.reqs[q[i].m.w >> SLOT_SHIFT].scene];
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
     HitRec h = traverse<false, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
-                                      TRACE_BLOCK, cnt);
+                                      BS, cnt);
     hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
     flush_counts<COUNT>(cnt, counts);
 }
 
 /* occlusion of every shadow ray (any hit: cast_ray(ray) != NULL, src/shader.cc:266) */
-template <bool COUNT, bool UNI>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq, uint32_t n,
-                                                              uint8_t *__restrict__ occ, unsigned long long *counts)
+template <bool COUNT, bool UNI, int BS>
+__global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq, uint32_t n,
+                                                     uint8_t *__restrict__ occ, unsigned long long *counts)
 {
-    __shared__ uint32_t stk[STACK * TRACE_BLOCK];
+    __shared__ uint32_t stk[STACK * BS];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 d = sq[i].d;
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_shadow(Tabs T, const SRay
     const SceneDev S = T.scenes[UNI ? 0u : T.reqs[__float_as_uint(sq[i].c.w)].scene];
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
     HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
-                                     TRACE_BLOCK, cnt);
+                                     BS, cnt);
     occ[i] = h.obj >= 0 ? 1 : 0;
     flush_counts<COUNT>(cnt, counts);
 }
@@ -435,9 +435,7 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
 /* photon gathers requested by k_shade: caustic irradiance (>= 8 photons,
  * src/pmap.cc:109-139) added to the direct term, or the radiance estimate
  * (src/pmap.cc:141-178) of a diffuse GI ray times kd */
-constexpr int GATHER_LANES = 16; /* lanes sharing one gather query */
-
-template <bool UNI>
+template <bool UNI, int GATHER_LANES>
 __global__ __launch_bounds__(256) void k_gather(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
                                                 uint32_t n, float4 *__restrict__ direct)
 {
@@ -630,6 +628,21 @@ __global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__
 }
 
 
+/* launchable instances of the tunable kernels */
+template <bool COUNT, bool UNI> auto trace_q_of(int bs) -> decltype(&k_trace_q<COUNT, UNI, 128>)
+{
+    return bs == 64 ? k_trace_q<COUNT, UNI, 64> : bs == 256 ? k_trace_q<COUNT, UNI, 256> : k_trace_q<COUNT, UNI, 128>;
+}
+template <bool COUNT, bool UNI> auto trace_shadow_of(int bs) -> decltype(&k_trace_shadow<COUNT, UNI, 128>)
+{
+    return bs == 64 ? k_trace_shadow<COUNT, UNI, 64>
+                    : bs == 256 ? k_trace_shadow<COUNT, UNI, 256> : k_trace_shadow<COUNT, UNI, 128>;
+}
+template <bool UNI> auto gather_of(int lanes) -> decltype(&k_gather<UNI, 16>)
+{
+    return lanes == 8 ? k_gather<UNI, 8> : lanes == 32 ? k_gather<UNI, 32> : k_gather<UNI, 16>;
+}
+
 } // namespace
 
 /* ================================================================== */
@@ -637,7 +650,7 @@ __global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__
 /* ================================================================== */
 namespace srt {
 
-constexpr uint32_t CHUNK = 1u << 21; /* rays per level sub-chunk */
+constexpr uint32_t MAX_CHUNK = 1u << 21; /* largest rays per level sub-chunk */
 using Clock = std::chrono::steady_clock;
 
 void fill_render_dev(const sray_render_params *p, RenderDev &R)
@@ -769,8 +782,11 @@ struct Engine {
     /* this pass's launch shape */
     int L = 0, K = 2, maxiter = 0;
     bool photons = false, uni = true, counting = false;
-    /* tunables */
-    uint32_t tile_w = 64; /* primary-ray tile width (x 64/tile_w rows) */
+    /* tunables (sray_tuning), copied from the context at each pass */
+    uint32_t tile_w = 64;        /* primary-ray tile width (x 64/tile_w rows) */
+    uint32_t chunk = MAX_CHUNK;  /* rays per bounce-level sub-chunk */
+    int trace_block = 128;       /* threads per block of the traversal kernels */
+    int gather_lanes = 16;       /* lanes per photon-gather query */
     unsigned long long sec_seen = 0; /* secondary rays counted before this pass */
     /* submit / poll */
     std::thread worker;
@@ -887,10 +903,20 @@ void job_done(Engine &E, Job *j, int rc, const char *msg)
     E.qcv.notify_all();
 }
 
+void apply_tuning(Engine &E)
+{
+    const sray_tuning &t = E.c->tune;
+    E.trace_block = t.trace_block;
+    E.tile_w = (uint32_t)t.tile_w;
+    E.gather_lanes = t.gather_lanes;
+    E.chunk = 1u << t.chunk_log2;
+}
+
 /* places a request in the arena and the active list */
 int admit(Engine &E, Job *j)
 {
     sray_gpu_ctx *c = E.c;
+    apply_tuning(E);
     int rc = validate_params(c, &j->p);
     if (rc) return rc;
     GpuScene *g = find_scene(c, j->scene_id);
@@ -915,7 +941,7 @@ int admit(Engine &E, Job *j)
     Q.PM.scale = gabs > 0.0 ? 1.0 / (M_PI * (gabs * gabs)) : 0.0;
     for (int kind = 0; kind < 2; kind++) {
         bool want = kind == SRAY_PMAP_CAUSTIC ? Q.P.caust != 0 : Q.P.gi != 0;
-        if (want && (rc = pmap_grid_get(c, g->pmap[kind], Q.PM.radius,
+        if (want && (rc = pmap_grid_get(c, g->pmap[kind], Q.PM.radius, c->tune.gather_cell,
                                         kind == SRAY_PMAP_CAUSTIC ? &Q.PM.caust : &Q.PM.glob)))
             return rc;
         j->pmaps[kind] = want ? g->pmap[kind] : 0u;
@@ -983,6 +1009,7 @@ void abort_all(Engine &E)
 int build_tables(Engine &E)
 {
     sray_gpu_ctx *c = E.c;
+    apply_tuning(E);
     E.hscenes.clear();
     std::map<uint32_t, uint32_t> idx;
     E.L = 0;
@@ -1037,14 +1064,15 @@ int process_level(Level &LV, int b, uint32_t count)
     unsigned long long *rc6 = dptr<unsigned long long>(E.rcounts);
     unsigned long long *stats = dptr<unsigned long long>(E.stats);
     unsigned long long *srays = dptr<unsigned long long>(E.slotrays);
-    for (uint32_t off = 0; off < count; off += CHUNK) {
-        uint32_t m = std::min(CHUNK, count - off);
+    const int bs = E.trace_block;
+    for (uint32_t off = 0; off < count; off += E.chunk) {
+        uint32_t m = std::min(E.chunk, count - off);
         const QRay *q = Q + off;
         {
             ProfScope ps(c, "k_trace_q");
-            auto kern = E.counting ? (E.uni ? k_trace_q<true, true> : k_trace_q<true, false>)
-                                   : (E.uni ? k_trace_q<false, true> : k_trace_q<false, false>);
-            kern<<<blocks(m, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(LV.T, q, m, dptr<int4>(E.hits), rc6);
+            auto kern = E.counting ? (E.uni ? trace_q_of<true, true>(bs) : trace_q_of<true, false>(bs))
+                                   : (E.uni ? trace_q_of<false, true>(bs) : trace_q_of<false, false>(bs));
+            kern<<<blocks(m, bs), bs, 0, c->stream>>>(LV.T, q, m, dptr<int4>(E.hits), rc6);
         }
         {
             ProfScope ps(c, "k_shade");
@@ -1058,16 +1086,16 @@ int process_level(Level &LV, int b, uint32_t count)
         }
         if (E.photons) {
             ProfScope ps(c, "k_gather");
-            auto kern = E.uni ? k_gather<true> : k_gather<false>;
-            kern<<<blocks((size_t)m * GATHER_LANES, 256), 256, 0, c->stream>>>(LV.T, q, dptr<GatherReq>(E.greq), m,
-                                                                               dptr<float4>(E.direct));
+            auto kern = E.uni ? gather_of<true>(E.gather_lanes) : gather_of<false>(E.gather_lanes);
+            kern<<<blocks((size_t)m * E.gather_lanes, 256), 256, 0, c->stream>>>(LV.T, q, dptr<GatherReq>(E.greq), m,
+                                                                                 dptr<float4>(E.direct));
         }
         size_t ns = (size_t)m * E.L;
         if (ns) {
             ProfScope ps(c, "k_trace_shadow");
-            auto kern = E.counting ? (E.uni ? k_trace_shadow<true, true> : k_trace_shadow<true, false>)
-                                   : (E.uni ? k_trace_shadow<false, true> : k_trace_shadow<false, false>);
-            kern<<<blocks(ns, TRACE_BLOCK), TRACE_BLOCK, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), (uint32_t)ns,
+            auto kern = E.counting ? (E.uni ? trace_shadow_of<true, true>(bs) : trace_shadow_of<true, false>(bs))
+                                   : (E.uni ? trace_shadow_of<false, true>(bs) : trace_shadow_of<false, false>(bs));
+            kern<<<blocks(ns, bs), bs, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), (uint32_t)ns,
                                                                          dptr<uint8_t>(E.occ), rc6 + 6);
         }
         {
@@ -1097,13 +1125,13 @@ int run_pass(Engine &E, std::vector<Job *> &finished)
     uint32_t levels = (uint32_t)E.maxiter + 1;
     if (E.qlev.size() < levels) E.qlev.resize(levels);
     for (uint32_t b = 0; b < levels; b++)
-        if ((rc = ensure(c, E.qlev[b], (size_t)CHUNK * E.K * sizeof(QRay)))) return rc;
-    if ((rc = ensure(c, E.hits, (size_t)CHUNK * sizeof(int4))) || (rc = ensure(c, E.direct, (size_t)CHUNK * 16)) ||
-        (rc = ensure(c, E.sq, (size_t)CHUNK * std::max(E.L, 1) * sizeof(SRay))) ||
-        (rc = ensure(c, E.occ, (size_t)CHUNK * std::max(E.L, 1))) ||
-        (rc = ensure(c, E.kids, (size_t)CHUNK * E.K * sizeof(QRay))) || (rc = ensure(c, E.kflag, (size_t)CHUNK * E.K)) ||
-        (rc = ensure(c, E.acc, (size_t)CHUNK * 16)) || (rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) ||
-        (E.photons && (rc = ensure(c, E.greq, (size_t)CHUNK * sizeof(GatherReq)))) ||
+        if ((rc = ensure(c, E.qlev[b], (size_t)E.chunk * E.K * sizeof(QRay)))) return rc;
+    if ((rc = ensure(c, E.hits, (size_t)E.chunk * sizeof(int4))) || (rc = ensure(c, E.direct, (size_t)E.chunk * 16)) ||
+        (rc = ensure(c, E.sq, (size_t)E.chunk * std::max(E.L, 1) * sizeof(SRay))) ||
+        (rc = ensure(c, E.occ, (size_t)E.chunk * std::max(E.L, 1))) ||
+        (rc = ensure(c, E.kids, (size_t)E.chunk * E.K * sizeof(QRay))) || (rc = ensure(c, E.kflag, (size_t)E.chunk * E.K)) ||
+        (rc = ensure(c, E.acc, (size_t)E.chunk * 16)) || (rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) ||
+        (E.photons && (rc = ensure(c, E.greq, (size_t)E.chunk * sizeof(GatherReq)))) ||
         (rc = ensure(c, E.slotrays, MAX_SLOTS * 8)) || (rc = ensure(c, c->nsel, 4)))
         return rc;
     if (E.sub.cap == 0 && (rc = ensure(c, E.subpix, 16))) return rc;
@@ -1115,8 +1143,8 @@ int run_pass(Engine &E, std::vector<Job *> &finished)
     if (!E.uni) HIPCHK(c, hipMemsetAsync(E.slotrays.p, 0, (size_t)E.hi * 8, c->stream));
     const uint32_t *A = dptr<uint32_t>(E.act[E.cur]);
     Level LV{&E, T, A, 0};
-    for (uint32_t c0 = 0; c0 < E.nact; c0 += CHUNK) {
-        uint32_t m = std::min(CHUNK, E.nact - c0);
+    for (uint32_t c0 = 0; c0 < E.nact; c0 += E.chunk) {
+        uint32_t m = std::min(E.chunk, E.nact - c0);
         k_fill_f4<<<blocks(m, 256), 256, 0, c->stream>>>(dptr<float4>(E.acc), m, make_float4(0, 0, 0, 0));
         {
             ProfScope ps(c, "k_raygen");
@@ -1442,6 +1470,80 @@ int engine_scene_busy(sray_gpu_ctx *c, uint32_t scene_id)
     return SRAY_OK;
 }
 
+int check_tuning(sray_gpu_ctx *c, const sray_tuning &t)
+{
+    bool ok = (t.trace_block == 64 || t.trace_block == 128 || t.trace_block == 256) &&
+              (t.tile_w == 8 || t.tile_w == 16 || t.tile_w == 32 || t.tile_w == 64) &&
+              (t.gather_lanes == 8 || t.gather_lanes == 16 || t.gather_lanes == 32) && t.chunk_log2 >= 16 &&
+              t.chunk_log2 <= 21 && (t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f);
+    return ok ? SRAY_OK : fail(c, SRAY_ERR_ARG, "tuning value outside the supported set");
+}
+
+/* median device time of `rounds` renders of params (after one untimed render) */
+int time_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, int rounds, float *d_out, double *ms)
+{
+    sray_stats st;
+    int rc = do_render(c, scene_id, p, d_out, &st);
+    if (rc) return rc;
+    std::vector<double> v;
+    for (int r = 0; r < rounds; r++) {
+        if ((rc = do_render(c, scene_id, p, d_out, &st))) return rc;
+        v.push_back(st.render_ms);
+    }
+    std::sort(v.begin(), v.end());
+    *ms = v[v.size() / 2];
+    return SRAY_OK;
+}
+
+int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, int rounds, sray_tuning *best,
+                double *best_ms)
+{
+    int rc = validate_params(c, p);
+    if (rc) return rc;
+    if (rounds < 1) rounds = 1;
+    size_t bytes = (size_t)(p->x1 - p->x0) * (size_t)(p->y1 - p->y0) * 16;
+    DevBuf out;
+    if ((rc = ensure(c, out, bytes))) return rc;
+    sray_tuning cur = c->tune;
+    double cur_ms = 0;
+    if ((rc = time_render(c, scene_id, p, rounds, dptr<float>(out), &cur_ms))) {
+        release(out);
+        return rc;
+    }
+    static const int blocks_[] = {64, 128, 256}, tiles[] = {8, 16, 32, 64}, lanes[] = {8, 16, 32}, chunks[] = {18, 19, 20, 21};
+    static const float cells[] = {0.5f, 1.0f, 2.0f};
+    bool photons = p->caust_photons || p->gi_photons;
+    for (int knob = 0; knob < 5; knob++) {
+        if ((knob == 2 || knob == 4) && !photons) continue;
+        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : 3;
+        for (int k = 0; k < nv; k++) {
+            sray_tuning t = cur;
+            if (knob == 0) t.trace_block = blocks_[k];
+            else if (knob == 1) t.tile_w = tiles[k];
+            else if (knob == 2) t.gather_lanes = lanes[k];
+            else if (knob == 3) t.chunk_log2 = chunks[k];
+            else t.gather_cell = cells[k];
+            if (!memcmp(&t, &cur, sizeof t)) continue;
+            c->tune = t;
+            double ms = 0;
+            if ((rc = time_render(c, scene_id, p, rounds, dptr<float>(out), &ms))) {
+                c->tune = cur;
+                release(out);
+                return rc;
+            }
+            if (ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+        c->tune = cur;
+    }
+    release(out);
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return SRAY_OK;
+}
+
 } // namespace srt
 
 /* ================================================================== */
@@ -1507,6 +1609,33 @@ int sray_gpu_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
 {
     if (!c) return SRAY_ERR_ARG;
     return do_poll(c, ticket, wait, out);
+}
+
+int sray_gpu_set_tuning(sray_gpu_ctx *c, const sray_tuning *t)
+{
+    if (!c || !t) return SRAY_ERR_ARG;
+    ApiLock lk(c);
+    int rc = check_tuning(c, *t);
+    if (rc) return rc;
+    c->tune = *t;
+    return SRAY_OK;
+}
+
+int sray_gpu_get_tuning(sray_gpu_ctx *c, sray_tuning *t)
+{
+    if (!c || !t) return SRAY_ERR_ARG;
+    ApiLock lk(c);
+    *t = c->tune;
+    return SRAY_OK;
+}
+
+int sray_gpu_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, int rounds, sray_tuning *best,
+                      double *best_ms)
+{
+    if (!c || !p) return SRAY_ERR_ARG;
+    ApiLock lk(c);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    return do_autotune(c, scene_id, p, rounds, best, best_ms);
 }
 
 int sray_gpu_set_admission_cap(sray_gpu_ctx *c, uint64_t bytes)

@@ -40,13 +40,13 @@ struct GpuPmap {
     uint32_t scene_id = 0;
     int kind = 0;
     uint64_t bytes = 0;
-    /* shading-gather grids, one per gather radius (float bits), built on
+    /* shading-gather grids, one per (gather radius, cell size), built on
      * first use and kept until the map is released */
     struct Grid {
         DevBuf pos, dir, nrm, colb, cell;
         PmapGrid g{};
     };
-    std::map<uint32_t, std::unique_ptr<Grid>> grids;
+    std::map<uint64_t, std::unique_ptr<Grid>> grids; /* key: radius bits << 32 | cell bits */
 };
 
 struct GpuScene {
@@ -92,6 +92,8 @@ struct sray_gpu_ctx {
         double ms = 0;
     };
     std::map<std::string, KStat> kstats;
+    /* tuning of the renderers and photon grids */
+    sray_tuning tune{128, 64, 16, 21, 1.0f, 0};
     /* renderers: one for synchronous renders, one behind submit/poll */
     srt::Engine *plain = nullptr;
     srt::Engine *serve = nullptr;
@@ -169,10 +171,14 @@ int select_flagged(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, u
     return SRAY_OK;
 }
 
-/* the shading-gather grid of a photon map for a radius (built or cached) */
-int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, PmapGrid *out);
+/* the shading-gather grid of a photon map for a radius and a cell size in
+ * radii (built or cached) */
+int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, float cell, PmapGrid *out);
 
 /* engine.hip */
+int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, float *d_out, sray_stats *st);
+/* median device time of `rounds` renders (after one untimed render) */
+int time_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, int rounds, float *d_out, double *ms);
 void engine_destroy(sray_gpu_ctx *c);
 /* device bytes held by the renderers (queues, tables, pixel arenas) */
 uint64_t engine_bytes(sray_gpu_ctx *c);

@@ -854,7 +854,7 @@ int do_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q
     int rc;
     if (grid) {
         PmapGrid G;
-        if ((rc = pmap_grid_get(c, pmap_id, radius, &G)) ||
+        if ((rc = pmap_grid_get(c, pmap_id, radius, c->tune.gather_cell, &G)) ||
             (rc = ensure(c, c->gq, (size_t)n * sizeof(sray_gather_query))) || (rc = ensure(c, c->grgb, (size_t)n * 12)))
             return rc;
         HIPCHK(c, hipMemcpyAsync(c->gq.p, q, (size_t)n * sizeof(sray_gather_query), hipMemcpyHostToDevice, c->stream));
@@ -1061,13 +1061,16 @@ void free_pmap(GpuPmap &m)
 /* the shading-gather grid of a photon map for one radius: cells of the
  * radius (larger if the photons' box would need more than 2^24 cells),
  * photons sorted by cell with a device radix sort */
-int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, PmapGrid *out)
+int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, float cellr, PmapGrid *out)
 {
     auto it = c->pmaps.find(pmap_id);
     if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
     GpuPmap &pm = *it->second;
-    uint32_t key;
-    memcpy(&key, &radius, 4);
+    if (!(cellr > 0.0f)) cellr = 1.0f;
+    uint32_t kr, kc;
+    memcpy(&kr, &radius, 4);
+    memcpy(&kc, &cellr, 4);
+    uint64_t key = ((uint64_t)kr << 32) | kc;
     auto g = pm.grids.find(key);
     if (g != pm.grids.end()) {
         *out = g->second->g;
@@ -1109,9 +1112,9 @@ int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, PmapGrid *out
         lo[k] = host_unkey(hb[k]);
         hi[k] = host_unkey(hb[3 + k]);
     }
-    /* cell size: the radius, or larger to keep the cell count bounded */
+    /* cell size: cellr radii, or larger to keep the cell count bounded */
     const double MAXC = (double)(1u << 24);
-    double cs = radius;
+    double cs = (double)radius * (double)cellr;
     auto cells = [&](double s, int64_t d[3]) {
         double t = 1;
         for (int k = 0; k < 3; k++) {
@@ -1253,6 +1256,49 @@ int sray_gpu_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *s
     ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_scene_upload(c, s, scene_id);
+}
+
+int sray_gpu_autotune_octree(sray_gpu_ctx *c, const sray_scene_desc *s, const sray_render_params *p, int rounds,
+                             const int32_t *items, const int32_t *depths, int ncand, int *best, double *best_ms)
+{
+    if (!c || !s || !p || !items || !depths || ncand <= 0) return SRAY_ERR_ARG;
+    ApiLock lk(c);
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    if (rounds < 1) rounds = 1;
+    size_t bytes = (size_t)(p->x1 - p->x0) * (size_t)(p->y1 - p->y0) * 16;
+    DevBuf out;
+    int rc = ensure(c, out, bytes);
+    if (rc) return rc;
+    int bi = -1;
+    double bms = 0;
+    for (int k = 0; k < ncand; k++) {
+        sray_scene_desc d = *s;
+        d.mesh_oct_items = items[k];
+        d.mesh_oct_depth = depths[k];
+        uint32_t sid = 0;
+        double ms = 0;
+        if ((rc = do_scene_upload(c, &d, &sid)) == SRAY_OK) {
+            rc = time_render(c, sid, p, rounds, dptr<float>(out), &ms);
+            auto it = c->scenes.find(sid);
+            (void)hipStreamSynchronize(c->stream);
+            for (auto &b : it->second->bufs) release(b);
+            c->scenes.erase(it);
+        }
+        if (rc == SRAY_ERR_LIMIT) continue; /* this depth does not fit the traversal stack */
+        if (rc) {
+            release(out);
+            return rc;
+        }
+        if (bi < 0 || ms < bms) {
+            bi = k;
+            bms = ms;
+        }
+    }
+    release(out);
+    if (bi < 0) return fail(c, SRAY_ERR_LIMIT, "no octree candidate fits the traversal limits");
+    if (best) *best = bi;
+    if (best_ms) *best_ms = bms;
+    return SRAY_OK;
 }
 
 int sray_gpu_scene_release(sray_gpu_ctx *c, uint32_t scene_id)

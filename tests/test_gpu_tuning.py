@@ -1,0 +1,85 @@
+"""Tuning knobs and the autotuner: they change speed, not images.
+
+Bar: renders under every tuning equal the default tuning's render to 1e-6
+(sub-chunking of a bounce level may split one sample's rays over two
+sub-chunks, which changes the order of two float additions); the
+autotuner returns a supported tuning and keeps it in the context.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sray_amd as S
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-6
+
+
+def _variants():
+    return [dict(trace_block=64), dict(trace_block=256), dict(tile_w=8), dict(tile_w=16),
+            dict(chunk_log2=16), dict(gather_lanes=8), dict(gather_lanes=32), dict(gather_cell=0.5),
+            dict(gather_cell=2.0)]
+
+
+def test_tunings_give_the_same_image(ctx):
+    hs = S.HostScene.generate(2, variant=10)
+    sid = ctx.upload(hs)
+    ctx.shoot_photons(sid, 20000, 20000, 300.0, 5, 0, 0, 3)
+    p = S.render_params(160, 90, samples=(1, 4), max_var=0.05, iter=4, shadow_samples=2, caust_photons=1,
+                        gi_photons=1, gather_dist=0.03, seed=8)
+    default = ctx.tuning()
+    base, bst = ctx.render(sid, p)
+    for kw in _variants():
+        ctx.set_tuning(**kw)
+        img, st = ctx.render(sid, p)
+        ctx.set_tuning(**{k: getattr(default, k) for k in kw})
+        err = float(np.abs(img - base).max())
+        print(f"{kw}: max diff {err:.2e}")
+        assert err <= TOL, kw
+        assert st.rays() == bst.rays(), kw
+    ctx.release(sid)
+
+
+def test_sub_chunks_match_checker(ctx):
+    """chunks of 2^16 rays split the heightfield's bounce levels into
+    sub-chunks (depth first); the image still matches the checker"""
+    hs = S.HostScene.generate(1, variant=4)
+    sid = ctx.upload(hs)
+    p = S.render_params(480, 270, iter=6)
+    ctx.set_tuning(chunk_log2=16)
+    img, st = ctx.render(sid, p)
+    ctx.set_tuning(chunk_log2=21)
+    oimg, ost = oracle.OracleScene(hs.desc_ptr(), 32).render(p)
+    e = float(np.sqrt(np.mean((img.astype(np.float64) - oimg) ** 2)))
+    assert e <= 1e-3 and st.rays() == ost.rays(), e
+    ctx.release(sid)
+
+
+def test_bad_tuning_rejected(ctx):
+    for kw in (dict(trace_block=96), dict(tile_w=12), dict(gather_lanes=4), dict(chunk_log2=22),
+               dict(gather_cell=3.0)):
+        with pytest.raises(S.SrayError):
+            ctx.set_tuning(**kw)
+
+
+def test_autotune(ctx):
+    hs = S.HostScene.generate(0)
+    sid = ctx.upload(hs)
+    p = S.render_params(320, 240, iter=5)
+    before, _ = ctx.render(sid, p)
+    t, ms = ctx.autotune(sid, p, rounds=2)
+    assert ms > 0 and t.trace_block in (64, 128, 256) and t.tile_w in (8, 16, 32, 64)
+    kept = ctx.tuning()
+    assert (kept.trace_block, kept.tile_w, kept.chunk_log2) == (t.trace_block, t.tile_w, t.chunk_log2)
+    after, _ = ctx.render(sid, p)
+    assert float(np.abs(after - before).max()) <= TOL
+    ctx.set_tuning(trace_block=128, tile_w=64, gather_lanes=16, chunk_log2=21, gather_cell=1.0)
+    ctx.release(sid)
+
+
+def test_autotune_octree(ctx):
+    hs = S.HostScene.generate(1, variant=8)
+    p = S.render_params(240, 135, iter=3)
+    cands = [(14, 7), (32, 8), (8, 6), (64, 30)]  # the last is too deep for the traversal stack
+    k, ms = ctx.autotune_octree(hs.desc_ptr(), p, cands, rounds=1)
+    assert 0 <= k < 3 and ms > 0
