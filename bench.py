@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--workload", default="heightfield", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tuning (no untimed search)")
     ap.add_argument("--frames", type=int, default=None, help="room / spheres: frames in flight (8 / 64)")
     ap.add_argument("--scenes", type=int, default=None, help="batch: scenes (256); serve: resident scenes (32)")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrival rate (req/s)")
@@ -395,6 +396,11 @@ def main():
               f"upload+octree build {upload_s:.2f} s, {ctx.scene_bytes(sid) / 2**20:.1f} MiB resident")
     p = S.render_params(W, H, samples=samples, iter=it)
     out = ctx.malloc(W * H * 4 * 4)
+    tuning = None
+    if not args.no_autotune:
+        t, tms = ctx.autotune(sid, p, rounds=2)
+        tuning = {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2}
+        log(args, f"autotune: {tuning}, {tms:.2f} ms/frame")
     for _ in range(args.warmup):
         ctx.render_device(sid, p, out)
     ctx.synchronize()
@@ -451,7 +457,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{wl}: {hs.desc.num_triangles} triangles + {hs.desc.num_objects - 1} spheres, "
                                f"{W}x{H}, {samples[0]}-{samples[1]} spp, depth {it}",
-                   "rays_per_step": rays // args.steps, "device_ms_per_step": round(dev_ms / args.steps, 3)},
+                   "rays_per_step": rays // args.steps, "device_ms_per_step": round(dev_ms / args.steps, 3),
+                   "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_trace_q", "avg_launch_ms": round(avg_launch_ms, 4),
