@@ -162,12 +162,7 @@ def run_spheres(args):
     # the same frame, many in flight (different seeds): the serving engine
     K = args.frames
     frames = [S.render_params(W, H, samples=samples, iter=it, seed=0x5EED + k) for k in range(K)]
-    ctx.poll(ctx.submit(sid, frames[0]))
-    t0 = time.perf_counter()
-    tickets = [ctx.submit(sid, q) for q in frames]
-    res = [ctx.poll(t)[1] for t in tickets]
-    bwall = time.perf_counter() - t0
-    batched = sum(r.rays for r in res) / bwall / 1e6
+    batched, bwall, _ = batched_leg(ctx, [(sid, q, None) for q in frames])
     log(args, f"spheres: plain {value:.1f} Mrays/s, {K} frames in flight {batched:.1f} Mrays/s")
     cpu = None if args.no_cpu else cpu_baseline(args, hs, "spheres", full=True)
     extra = {"batched": {"value": round(batched, 3), "frames_in_flight": K, "wall_s": round(bwall, 4)},
@@ -177,6 +172,24 @@ def run_spheres(args):
                                f"spheres: 64 spheres, {W}x{H}, 1 spp, depth {it}", extra)), flush=True)
     ctx.free(out)
     ctx.close()
+
+
+def batched_leg(ctx, jobs, warm=True):
+    """all jobs ((scene, params, camera or None)) in flight together through
+    submit/poll, each writing its image to a device buffer of its own (as
+    the plain legs do); one untimed round first grows the engine's arenas
+    to the batch.  Returns (Mrays/s, wall s, per-request latencies ms)."""
+    outs = [ctx.malloc((p.x1 - p.x0) * (p.y1 - p.y0) * 16) for _, p, _ in jobs]
+    rounds = [False, True] if warm else [True]
+    for timed in rounds:
+        t0 = time.perf_counter()
+        tickets = [ctx.submit(sid, p, cam, out=o) for (sid, p, cam), o in zip(jobs, outs)]
+        res = [ctx.poll(t)[1] for t in tickets]
+        wall = time.perf_counter() - t0
+    for o in outs:
+        ctx.free(o)
+    rays = sum(r.rays for r in res)
+    return rays / wall / 1e6, wall, [r.latency_ms for r in res]
 
 
 def room_params(W, H, diag, seed):
@@ -211,14 +224,7 @@ def run_room(args):
     ctx.synchronize()
     plain_wall = time.perf_counter() - t0
     plain = rays / plain_wall / 1e6
-    ctx.poll(ctx.submit(sid, frames[0]))  # warm-up of the serving engine
-    t0 = time.perf_counter()
-    tickets = [ctx.submit(sid, p) for p in frames]
-    res = [ctx.poll(t)[1] for t in tickets]
-    batch_wall = time.perf_counter() - t0
-    brays = sum(r.rays for r in res)
-    batched = brays / batch_wall / 1e6
-    lat = [r.latency_ms for r in res]
+    batched, batch_wall, lat = batched_leg(ctx, [(sid, p, None) for p in frames])
     log(args, f"room: plain {plain:.2f} Mrays/s ({plain_wall:.2f} s), batched {batched:.2f} Mrays/s "
               f"({batch_wall:.2f} s), rays/frame {rays // K}")
     cpu = None
@@ -284,14 +290,7 @@ def run_batch(args):
     ctx.synchronize()
     plain_wall = time.perf_counter() - t0
     plain = rays / plain_wall / 1e6
-    ctx.poll(ctx.submit(sids[0], params[0]))
-    t0 = time.perf_counter()
-    tickets = [ctx.submit(sid, p) for sid, p in zip(sids, params)]
-    res = [ctx.poll(t)[1] for t in tickets]
-    batch_wall = time.perf_counter() - t0
-    brays = sum(r.rays for r in res)
-    batched = brays / batch_wall / 1e6
-    lat = [r.latency_ms for r in res]
+    batched, batch_wall, lat = batched_leg(ctx, [(sid, p, None) for sid, p in zip(sids, params)])
     log(args, f"batch: plain {plain:.2f} Mrays/s ({plain_wall:.2f} s) batched {batched:.2f} Mrays/s "
               f"({batch_wall:.2f} s)")
     extra = {"batched": {"value": round(batched, 3), "wall_s": round(batch_wall, 3), "p50_ms": pct(lat, 50),
@@ -423,8 +422,11 @@ def main():
     cst = ctx.render_device(sid, pc, out)
     cl = cst.closest
     closest_rays = cst.rays_primary + cst.rays_reflect + cst.rays_refract + cst.rays_diffuse
-    alg_bytes = (32 + 16) * closest_rays + 32 * cl.nodes + 36 * cl.triangles + 16 * cl.spheres + \
-        28 * cl.cylinders + 4 * cl.item_refs
+    # per ray 32 B in + 16 B hit out; per fetched node 32 B; per triangle its
+    # three vertices with the face normal (48 B); per sphere / cylinder the
+    # inverse matrix and radius (+ extent) (52 / 60 B); per scene item its index
+    alg_bytes = (32 + 16) * closest_rays + 32 * cl.nodes + 48 * cl.triangles + 52 * cl.spheres + \
+        60 * cl.cylinders + 4 * (cl.item_refs - cl.triangles)
     launches, tq_ms = ks.get("k_trace_q", (0, 0.0))
     per_frame_launches = launches / args.steps
     avg_launch_ms = tq_ms / max(launches, 1)

@@ -163,11 +163,13 @@ typedef struct sray_hit {
 #define SRAY_TRACE_COUNT     2u    /* also fill visit counters (slower) */
 
 typedef struct sray_trace_counts {
-    uint64_t nodes;        /* octree nodes whose box was tested */
+    uint64_t nodes;        /* octree node records fetched (children are
+                              tested from their parent's box; only those
+                              the segment meets are fetched) */
     uint64_t spheres;
     uint64_t cylinders;
     uint64_t triangles;
-    uint64_t item_refs;    /* leaf item indices read */
+    uint64_t item_refs;    /* leaf items tested (scene objects, triangles) */
     uint64_t mesh_enters;  /* mesh (second level) traversals started */
 } sray_trace_counts;
 
@@ -269,8 +271,12 @@ typedef struct sray_request {
     sray_render_params params;      /* tile rectangle inside params */
     sray_camera_desc   camera;      /* per-request camera pose */
     int32_t            use_camera;  /* 0 = the scene's own camera */
-    int32_t            pad_;
+    int32_t            flags;       /* SRAY_REQUEST_* */
 } sray_request;
+
+/* rgba of submit is device memory of the context's device (the image is
+ * written there by the device; latency then ends when it is written) */
+#define SRAY_REQUEST_DEVICE_OUT 0x1
 
 typedef struct sray_result {
     float   *rgba;                  /* caller-owned, (x1-x0)*(y1-y0)*4 floats */
@@ -378,8 +384,9 @@ int  sray_gpu_autotune_octree(sray_gpu_ctx *ctx, const sray_scene_desc *s, const
  * photon maps, queues, requests in flight) stays under the admission cap
  * (default 200 GB), and every pass of the renderer takes one sample of
  * every active pixel of every admitted request, so requests that arrive
- * while others are on the GPU join at the next pass.  rgba must stay valid
- * until poll reports the ticket done.  poll: SRAY_ERR_PENDING if not done
+ * while others are on the GPU join at the next pass.  rgba (host memory,
+ * or device memory with SRAY_REQUEST_DEVICE_OUT) must stay valid until
+ * poll reports the ticket done.  poll: SRAY_ERR_PENDING if not done
  * and wait == 0; otherwise the request's status (SRAY_ERR_NOMEM if it can
  * never fit the cap), after which the ticket is forgotten.  A request's
  * image is the same alone and in any mix. */

@@ -285,23 +285,30 @@ class Context:
         return st
 
     # ---- serving: continuous batching behind submit / poll
-    def submit(self, sid, params, camera=None):
-        """queue a tile request; returns a ticket (the output array is kept
-        until poll returns it)"""
+    def submit(self, sid, params, camera=None, out=None):
+        """queue a tile request; returns a ticket.  The image goes to a new
+        host array (kept until poll returns it), or to the device buffer out
+        (a pointer from malloc of (x1-x0)*(y1-y0)*16 bytes; poll then
+        returns None in place of the array)"""
         r = abi.Request()
         r.scene_id = sid
         r.params = params
         if camera is not None:
             r.camera = camera
             r.use_camera = 1
-        w, h = params.x1 - params.x0, params.y1 - params.y0
-        out = np.zeros((h, w, 4), np.float32)
         t = C.c_uint64()
-        self._check(lib().sray_gpu_submit(self._h, C.byref(r), out.ctypes.data_as(C.POINTER(C.c_float)),
+        if out is not None:
+            r.flags = abi.SRAY_REQUEST_DEVICE_OUT
+            self._check(lib().sray_gpu_submit(self._h, C.byref(r), C.cast(C.c_void_p(out), C.POINTER(C.c_float)),
+                                              C.byref(t)))
+            return t.value
+        w, h = params.x1 - params.x0, params.y1 - params.y0
+        host = np.empty((h, w, 4), np.float32)
+        self._check(lib().sray_gpu_submit(self._h, C.byref(r), host.ctypes.data_as(C.POINTER(C.c_float)),
                                           C.byref(t)))
         if not hasattr(self, "_outs"):
             self._outs = {}
-        self._outs[t.value] = out
+        self._outs[t.value] = host
         return t.value
 
     def poll(self, ticket, wait=True):

@@ -21,6 +21,7 @@ SRAY_RENDER_COUNT = 2
 SRAY_PMAP_CAUSTIC, SRAY_PMAP_GLOBAL = 0, 1
 SRAY_GATHER_IRRADIANCE, SRAY_GATHER_RADIANCE = 0, 1
 SRAY_GATHER_GRID = 0x10
+SRAY_REQUEST_DEVICE_OUT = 0x1
 
 RAY_MAGNITUDE = 10000.0
 ERROR_MARGIN = 1e-6
@@ -120,7 +121,7 @@ class GatherQuery(C.Structure):
 
 class Request(C.Structure):
     _fields_ = [("scene_id", u32), ("params", RenderParams), ("camera", CameraDesc), ("use_camera", i32),
-                ("pad_", i32)]
+                ("flags", i32)]
 
 
 class Result(C.Structure):

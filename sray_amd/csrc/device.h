@@ -68,7 +68,8 @@ struct SceneDev {
     const float4 *nhi;
     const uint32_t *items;
     const DevObj *objs;
-    const float4 *tri;       /* 3 per triangle: v0, v1, v2 (w unused) */
+    const float4 *tri;       /* 3 per triangle: v0, v1, v2; w = face normal (v1-v0) x (v2-v0) */
+    const float4 *ltri;      /* the same, 3 per item position: mesh leaves' triangles in leaf order */
     const float *tri_n;      /* 9 per triangle or null */
     const float *tri_g;      /* 9 per triangle or null */
     const float *tri_uv;     /* 6 per triangle or null */
@@ -131,6 +132,26 @@ __device__ __forceinline__ V3 xf_normal(const float *i, V3 n)
               i[2] * n.x + i[6] * n.y + i[10] * n.z};
 }
 
+/* base[key] += v over the lanes of a wave with act set, one atomic per
+ * distinct key of the wave instead of one per lane (the lanes of one
+ * request are contiguous, so a wave holds one or two keys).  Every lane of
+ * the wave must call it. */
+template <typename T>
+__device__ __forceinline__ void wave_keyed_add(T *base, uint32_t key, T v, bool act)
+{
+    const uint32_t lane = __lane_id();
+    unsigned long long left = __ballot(act && v != (T)0);
+    while (left) {
+        int lead = __ffsll(left) - 1;
+        uint32_t k0 = (uint32_t)__shfl((int)key, lead);
+        bool mine = ((left >> lane) & 1ull) && key == k0;
+        unsigned long long grp = __ballot(mine);
+        T s = mine ? v : (T)0;
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == (uint32_t)lead) atomicAdd(&base[k0], s);
+        left &= ~grp;
+    }
+}
 
 } // namespace srd
 

@@ -420,10 +420,9 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
             }
         }
     }
-    if (!UNI && i < n) {
-        uint32_t tot = c_sh + c_rl + c_rr + c_df;
-        if (tot) atomicAdd(&slot_rays[q[i].m.w >> SLOT_SHIFT], (unsigned long long)tot);
-    }
+    if (!UNI)
+        wave_keyed_add(slot_rays, i < n ? q[i].m.w >> SLOT_SHIFT : 0u,
+                       (unsigned long long)(c_sh + c_rl + c_rr + c_df), i < n);
     if (c_sh) atomicAdd(&bcnt[0], c_sh);
     if (c_rl) atomicAdd(&bcnt[1], c_rl);
     if (c_rr) atomicAdd(&bcnt[2], c_rr);
@@ -543,7 +542,7 @@ __global__ void k_slot_count(const uint32_t *__restrict__ active, uint32_t n, co
                              uint32_t *__restrict__ cnt)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[preq[active[i]]], 1u);
+    wave_keyed_add(cnt, i < n ? preq[active[i]] : 0u, 1u, i < n);
 }
 
 /* end of one sample pass: framebuffer accumulation in float
@@ -782,6 +781,7 @@ struct Engine {
     /* this pass's launch shape */
     int L = 0, K = 2, maxiter = 0;
     bool photons = false, uni = true, counting = false;
+    bool one_scene = true; /* every request in flight renders the same scene */
     /* tunables (sray_tuning), copied from the context at each pass */
     uint32_t tile_w = 64;        /* primary-ray tile width (x 64/tile_w rows) */
     uint32_t chunk = MAX_CHUNK;  /* rays per bounce-level sub-chunk */
@@ -1035,6 +1035,7 @@ int build_tables(Engine &E)
         E.photons = E.photons || Q.P.gi || Q.P.caust;
     }
     E.uni = E.live == 1 && E.slot_job[0] != nullptr;
+    E.one_scene = E.hscenes.size() == 1;
     int rc;
     if ((rc = ensure(c, E.dreqs, (size_t)E.hi * sizeof(ReqDev))) ||
         (rc = ensure(c, E.dscenes, E.hscenes.size() * sizeof(SceneDev))))
@@ -1070,8 +1071,9 @@ int process_level(Level &LV, int b, uint32_t count)
         const QRay *q = Q + off;
         {
             ProfScope ps(c, "k_trace_q");
-            auto kern = E.counting ? (E.uni ? trace_q_of<true, true>(bs) : trace_q_of<true, false>(bs))
-                                   : (E.uni ? trace_q_of<false, true>(bs) : trace_q_of<false, false>(bs));
+            /* the traversal reads only the scene: one scene = the uniform variant */
+            auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs) : trace_q_of<true, false>(bs))
+                                   : (E.one_scene ? trace_q_of<false, true>(bs) : trace_q_of<false, false>(bs));
             kern<<<blocks(m, bs), bs, 0, c->stream>>>(LV.T, q, m, dptr<int4>(E.hits), rc6);
         }
         {
@@ -1093,8 +1095,9 @@ int process_level(Level &LV, int b, uint32_t count)
         size_t ns = (size_t)m * E.L;
         if (ns) {
             ProfScope ps(c, "k_trace_shadow");
-            auto kern = E.counting ? (E.uni ? trace_shadow_of<true, true>(bs) : trace_shadow_of<true, false>(bs))
-                                   : (E.uni ? trace_shadow_of<false, true>(bs) : trace_shadow_of<false, false>(bs));
+            auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs) : trace_shadow_of<true, false>(bs))
+                                   : (E.one_scene ? trace_shadow_of<false, true>(bs)
+                                                  : trace_shadow_of<false, false>(bs));
             kern<<<blocks(ns, bs), bs, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), (uint32_t)ns,
                                                                          dptr<uint8_t>(E.occ), rc6 + 6);
         }
@@ -1210,6 +1213,8 @@ int run_pass(Engine &E, std::vector<Job *> &finished)
         retire(E, s);
         finished.push_back(j);
     }
+    /* device outputs are complete when the request is reported done */
+    if (!finished.empty()) HIPCHK(c, hipStreamSynchronize(c->stream));
     return SRAY_OK;
 }
 
@@ -1339,6 +1344,7 @@ void serve_loop(sray_gpu_ctx *c, Engine *E)
         }
         if (E->live == 0) continue;
         int rc = run_pass(*E, fin);
+        if (c->profiling) prof_resolve(c);
         if (rc) {
             std::string msg = c->err;
             for (uint32_t s = 0; s < E->hi; s++)
@@ -1361,10 +1367,14 @@ int do_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *tic
     j->p = r->params;
     j->cam = r->camera;
     j->use_cam = r->use_camera != 0;
-    j->host_out = rgba;
+    if (r->flags & SRAY_REQUEST_DEVICE_OUT)
+        j->dev_out = rgba;
+    else
+        j->host_out = rgba;
     int rc;
     {
         ApiLock g(c);
+        if (r->flags & ~SRAY_REQUEST_DEVICE_OUT) return fail(c, SRAY_ERR_ARG, "unknown request flags 0x%x", r->flags);
         if ((rc = validate_params(c, &r->params))) return rc;
         if (!find_scene(c, r->scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", r->scene_id);
         if (j->use_cam && !(r->camera.vfov > 0.0)) return fail(c, SRAY_ERR_ARG, "request camera: vfov must be > 0");
@@ -1401,7 +1411,7 @@ int do_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
         E->qcv.wait(lk, [&] { return j->state == 2; });
     }
     if (out) {
-        out->rgba = j->host_out;
+        out->rgba = j->host_out ? j->host_out : j->dev_out;
         out->rays = j->rays;
         out->latency_ms = j->latency_ms;
         out->queue_ms = j->queue_ms;

@@ -225,17 +225,21 @@ __global__ void k_grid_key(const float4 *__restrict__ pos, uint32_t n, PmapGrid 
     val[i] = i;
 }
 
-/* cell_start[c] = first sorted photon with key >= c, for c in [0, ncell] */
+/* cell_start[c] = first sorted photon with key >= c, for c in [0, ncell]:
+ * one lower-bound search per cell (runs of empty cells cost nothing extra) */
 __global__ void k_grid_start(const uint32_t *__restrict__ key, uint32_t n, uint32_t ncell, uint32_t *__restrict__ start)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t k = key[i];
-    uint32_t prev = i == 0 ? 0u : key[i - 1] + 1u;
-    if (i == 0 || key[i - 1] != k)
-        for (uint32_t c = prev; c <= k; c++) start[c] = i;
-    if (i == n - 1)
-        for (uint32_t c = k + 1; c <= ncell; c++) start[c] = n;
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncell) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (key[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    start[c] = lo;
 }
 
 __global__ void k_grid_final(PmapTree T, const uint32_t *__restrict__ order, uint32_t n, float4 *__restrict__ pos,
@@ -535,6 +539,7 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     std::vector<float> boxes;
     sray::object_boxes(s, boxes);
     uint32_t sroot = sray::build_flat_octree(boxes, s->scene_oct_items, s->scene_oct_depth, 0, ft);
+    const size_t scene_items = ft.items.size(); /* mesh item positions follow */
     g->scene_height = ft.height;
     double dx = (double)ft.root_max[0] - ft.root_min[0], dy = (double)ft.root_max[1] - ft.root_min[1],
            dz = (double)ft.root_max[2] - ft.root_min[2];
@@ -639,6 +644,17 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
         float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
         for (int k = 0; k < 3; k++) tri[t * 3 + k] = make_float4(v[k * 3], v[k * 3 + 1], v[k * 3 + 2], n[k]);
     }
+    /* mesh leaves' triangles copied in item order (scene-level item
+     * positions hold zeros and are never read as triangles) */
+    std::vector<float4> ltri(ft.items.size() * 3, make_float4(0, 0, 0, 0));
+    for (size_t k = scene_items; k < ft.items.size(); k++)
+        for (int v = 0; v < 3; v++) ltri[k * 3 + v] = tri[(size_t)ft.items[k] * 3 + v];
+    /* the host vectors above are the sources of asynchronous copies: on
+     * every way out (errors included) the stream drains before they go */
+    struct Drain {
+        sray_gpu_ctx *c;
+        ~Drain() { (void)hipStreamSynchronize(c->stream); }
+    } drain{c};
     SceneDev &D = g->dev;
     memset(&D, 0, sizeof D);
     const void *p;
@@ -652,6 +668,8 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     D.objs = (const DevObj *)p;
     if ((rc = upload(c, *g, tri.data(), tri.size() * sizeof(float4), &p))) return rc;
     D.tri = (const float4 *)p;
+    if ((rc = upload(c, *g, ltri.data(), ltri.size() * sizeof(float4), &p))) return rc;
+    D.ltri = (const float4 *)p;
     if (s->tri_normal) {
         if ((rc = upload(c, *g, s->tri_normal, (size_t)s->num_triangles * 9 * 4, &p))) return rc;
         D.tri_n = (const float *)p;
@@ -1160,7 +1178,8 @@ int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, float cellr, 
         cleanup();
         return fail(c, SRAY_ERR_HIP, "grid sort failed: %s", hipGetErrorString(e));
     }
-    k_grid_start<<<blocks(n, 256), 256, 0, c->stream>>>(dptr<uint32_t>(keys[1]), n, ncell, dptr<uint32_t>(G->cell));
+    k_grid_start<<<blocks((size_t)ncell + 1, 256), 256, 0, c->stream>>>(dptr<uint32_t>(keys[1]), n, ncell,
+                                                                        dptr<uint32_t>(G->cell));
     k_grid_final<<<blocks(n, 256), 256, 0, c->stream>>>(T, dptr<uint32_t>(vals[1]), n, dptr<float4>(G->pos),
                                                         dptr<float4>(G->dir), dptr<float4>(G->nrm),
                                                         dptr<float>(G->colb));

@@ -15,7 +15,8 @@
  * as the reference's first-occurrence rule.
  *
  * Per-lane stack in LDS, 32-bit entries:
- *   ITER   first child block << 8 | pending children (order space)   (bit 31 = 0)
+ *   ITER   first child block << 8 | pending children whose boxes the
+ *          segment meets (order space)                              (bit 31 = 0)
  *   RESUME two words: the next scene item, then E_RES | end of the
  *          leaf's items (a scene leaf interrupted by a mesh)
  *   MARKER E_MARK | scene item rank of the mesh being traversed
@@ -24,6 +25,9 @@
  * footprint at 80 bytes per lane.
  * A mesh met by the same ray in several scene leaves (Q9) is traversed once:
  * its result is memoised for the ray (for an any-hit ray, its miss).
+ * A mesh leaf's triangles are read from a copy in item order (no index
+ * load before the vertex loads); the triangle id of a hit is looked up
+ * from its item position once the mesh is done.
  */
 #ifndef SRAY_TRAVERSE_H_
 #define SRAY_TRAVERSE_H_
@@ -75,6 +79,45 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, const TRay &r, float 
     if (tzmax < tmax) tmax = tzmax;
     tmin_out = tmin;
     return (tmin < 1.0f) && (tmax > 0.0f);
+}
+
+/* The slab tests of a node's eight children at once.  Children are the
+ * octants around the box centre (lo + hi) * 0.5, cut exactly so by the
+ * host build, so the values slab() computes on a child's own box are
+ * products of the node's bounds and its centre: per axis the near half
+ * spans [N, C] and the far half [C, F] in ray parameter (N, F the node's
+ * entry / exit products, C the centre's).  Each child then goes through
+ * slab()'s own comparison sequence (same floats, same order, so the same
+ * result, NaN cases included).  exist and the result are in order space
+ * (bit k = child k ^ dmask, near to far); children whose entry distance
+ * exceeds bound are dropped. */
+__device__ __forceinline__ uint32_t expand_children(float4 lo, float4 hi, const TRay &r, uint32_t exist, float bound)
+{
+    const float cx = (lo.x + hi.x) * 0.5f, cy = (lo.y + hi.y) * 0.5f, cz = (lo.z + hi.z) * 0.5f;
+    const bool xs = r.dx < 0.0f, ys = r.dy < 0.0f, zs = r.dz < 0.0f;
+    const float x0 = (lo.x - r.ox) * r.ix, xc = (cx - r.ox) * r.ix, x1 = (hi.x - r.ox) * r.ix;
+    const float y0 = (lo.y - r.oy) * r.iy, yc = (cy - r.oy) * r.iy, y1 = (hi.y - r.oy) * r.iy;
+    const float z0 = (lo.z - r.oz) * r.iz, zc = (cz - r.oz) * r.iz, z1 = (hi.z - r.oz) * r.iz;
+    const float nx = xs ? x1 : x0, fx = xs ? x0 : x1;
+    const float ny = ys ? y1 : y0, fy = ys ? y0 : y1;
+    const float nz = zs ? z1 : z0, fz = zs ? z0 : z1;
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float txmin = (k & 1) ? xc : nx, txmax = (k & 1) ? fx : xc;
+        const float tymin = (k & 2) ? yc : ny, tymax = (k & 2) ? fy : yc;
+        const float tzmin = (k & 4) ? zc : nz, tzmax = (k & 4) ? fz : zc;
+        float tmin = txmin, tmax = txmax;
+        bool hit = !((tmin > tymax) || (tymin > tmax));
+        if (tymin > tmin) tmin = tymin;
+        if (tymax < tmax) tmax = tymax;
+        hit = hit && !((tmin > tzmax) || (tzmin > tmax));
+        if (tzmin > tmin) tmin = tzmin;
+        if (tzmax < tmax) tmax = tzmax;
+        hit = hit && (tmin < 1.0f) && (tmax > 0.0f) && !(tmin > bound);
+        out |= hit ? (1u << k) : 0u;
+    }
+    return out & exist;
 }
 
 /* child mask permuted into near-to-far order: bit k <-> child (k ^ dmask) */
@@ -229,7 +272,14 @@ struct HitRec {
 /* Scene::cast_ray over the octrees.  ANY: stop at the first hit (the
  * reference's shadow test cast_ray(ray) != NULL, src/shader.cc:266).
  * stk points at this lane's entry 0; consecutive entries are `stride`
- * apart (LDS, lane-interleaved). */
+ * apart (LDS, lane-interleaved).
+ *
+ * A node record is fetched only for a child whose box the segment meets:
+ * when a node is entered its children are tested from its own box
+ * (expand_children) and only the hit ones are pushed.  A popped child is
+ * tested again from its fetched box (the same floats, so it passes) to
+ * prune it against the best distance found since the push, and the pseudo
+ * entries of the roots get their real test there. */
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int time, uint32_t *stk, int stride,
                                            TraceCounts &cnt)
@@ -243,7 +293,7 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
     best.rank = 0xffffffffu;
     int level = 0;
     float m_t = FLT_MAX;
-    uint32_t m_tri = 0xffffffffu, m_rank = 0xffffffffu;
+    uint32_t m_rank = 0xffffffffu; /* item position of the mesh's best triangle */
     int cache_obj = -1;
     float cache_t = FLT_MAX;
     uint32_t cache_tri = 0xffffffffu;
@@ -265,6 +315,7 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                 int oi = (int)S.items[rank];
                 cache_obj = oi;
                 if (!ANY) {
+                    uint32_t m_tri = m_rank != 0xffffffffu ? S.items[m_rank] : 0xffffffffu;
                     cache_t = m_t;
                     cache_tri = m_tri;
                     if (m_tri != 0xffffffffu && (m_t < best.t || (m_t == best.t && rank < best.rank))) {
@@ -294,8 +345,9 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
             if (COUNT) cnt.nodes++;
             float tmin;
             if (!slab(lo, hi, r, tmin)) continue;
+            float bound = FLT_MAX;
             if (!ANY) {
-                float bound = level ? fminf(best.t, m_t) : best.t;
+                bound = level ? fminf(best.t, m_t) : best.t;
                 if (tmin > bound) continue;
             }
             uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
@@ -303,33 +355,36 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                 item_cur = first;
                 item_end = first + (meta & ~LEAF);
             } else {
-                stk[(sp++) * stride] = (first << 8) | order_mask(meta, r.dmask);
+                uint32_t hitm = expand_children(lo, hi, r, order_mask(meta, r.dmask), bound);
+                if (hitm) stk[(sp++) * stride] = (first << 8) | hitm;
             }
         }
         if (item_cur >= item_end) break; /* stack empty */
         if (level) {
-            /* mesh leaf: all its triangles */
-            for (; item_cur < item_end; item_cur++) {
-                uint32_t ti = S.items[item_cur];
+            /* mesh leaf: all its triangles, from the leaf-ordered copies
+             * (one per step: a second triangle's registers in flight cost
+             * a wave of occupancy and more than its latency hiding gains) */
+            for (; item_cur < item_end;) {
+                const float4 *L = S.ltri + (size_t)item_cur * 3;
+                float4 A = L[0], B = L[1], C = L[2];
                 if (COUNT) {
                     cnt.item_refs++;
                     cnt.triangles++;
                 }
-                float4 A = S.tri[(size_t)ti * 3], B = S.tri[(size_t)ti * 3 + 1], C = S.tri[(size_t)ti * 3 + 2];
                 float t;
                 if (tri_t(A, B, C, r, t)) {
                     if (ANY) {
                         best.t = t;
-                        best.tri = ti;
+                        best.tri = S.items[item_cur];
                         best.obj = cur_mesh;
                         return best;
                     }
                     if (t < m_t || (t == m_t && item_cur < m_rank)) {
                         m_t = t;
-                        m_tri = ti;
                         m_rank = item_cur;
                     }
                 }
+                item_cur++;
             }
             continue;
         }
@@ -366,7 +421,6 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
             level = 1;
             cur_mesh = oi;
             m_t = FLT_MAX;
-            m_tri = 0xffffffffu;
             m_rank = 0xffffffffu;
             continue;
         }

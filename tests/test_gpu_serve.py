@@ -167,3 +167,30 @@ def test_release_guard_and_bad_requests(ctx):
         ctx.submit(sid, p)
     with pytest.raises(S.SrayError):
         ctx.submit(999, S.render_params(16, 16))
+
+
+def test_device_outputs_one_scene(ctx):
+    """requests of one scene (the single-scene traversal variant) writing
+    their images to device buffers equal their images alone"""
+    hs = S.HostScene.generate(0)
+    sid = ctx.upload(hs)
+    ps = [S.render_params(96, 72, iter=5, seed=20 + k) for k in range(5)]
+    ps.append(S.render_params(120, 90, rect=(10, 5, 70, 60), samples=(1, 4), max_var=0.002, iter=4, seed=30))
+    alone = [ctx.render(sid, p)[0] for p in ps]
+    outs = [ctx.malloc((p.x1 - p.x0) * (p.y1 - p.y0) * 16) for p in ps]
+    tickets = [ctx.submit(sid, p, out=o) for p, o in zip(ps, outs)]
+    for k, t in enumerate(tickets):
+        img, res = ctx.poll(t)
+        assert img is None and res.rays > 0
+        p = ps[k]
+        got = ctx.to_host(outs[k], (p.x1 - p.x0) * (p.y1 - p.y0) * 16, np.float32).reshape(alone[k].shape)
+        assert float(np.abs(got - alone[k]).max()) <= TOL, k
+    for o in outs:
+        ctx.free(o)
+    r = abi.Request()
+    r.scene_id, r.params, r.flags = sid, ps[0], 0x100
+    t = C.c_uint64()
+    buf = np.empty(96 * 72 * 4, np.float32)
+    assert S.lib().sray_gpu_submit(ctx._h, C.byref(r), buf.ctypes.data_as(C.POINTER(C.c_float)),
+                                   C.byref(t)) == abi.SRAY_ERR_ARG
+    ctx.release(sid)
