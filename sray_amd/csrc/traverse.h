@@ -226,8 +226,10 @@ __device__ __forceinline__ bool cylinder_t(float rad, float ys, float ye, V3 o, 
     return true;
 }
 
-/* Triangle::intersect + check_clock, src/mesh.cc:117-165, 310-315 */
-__device__ __forceinline__ bool tri_t(float4 A, float4 B, float4 C, const TRay &r, float &t)
+/* Triangle::intersect + check_clock, src/mesh.cc:117-165, 310-315.  A
+ * plane distance strictly beyond bound cannot win (the caller's best is
+ * closer), so the edge tests are skipped for it; equal distances go on. */
+__device__ __forceinline__ bool tri_t(float4 A, float4 B, float4 C, const TRay &r, float &t, float bound = FLT_MAX)
 {
     V3 v0 = v3(A.x, A.y, A.z), v1 = v3(B.x, B.y, B.z), v2 = v3(C.x, C.y, C.z);
     V3 n = v3(A.w, B.w, C.w); /* (v1-v0) x (v2-v0), stored at upload */
@@ -236,7 +238,7 @@ __device__ __forceinline__ bool tri_t(float4 A, float4 B, float4 C, const TRay &
     if (fabsf(ndd) < EPS) return false;
     V3 vo = sub(o, v0);
     float tt = -dot(n, vo) / ndd;
-    if (tt < EPS || tt > 1.0f) return false;
+    if (tt < EPS || tt > 1.0f || tt > bound) return false;
     V3 p = add(o, mul(d, tt));
     if (!(dot(cross(sub(v1, v0), sub(p, v0)), n) >= 0.0f)) return false;
     if (!(dot(cross(sub(v2, v1), sub(p, v1)), n) >= 0.0f)) return false;
@@ -284,8 +286,7 @@ template <bool ANY, bool COUNT>
 __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int time, uint32_t *stk, int stride,
                                            TraceCounts &cnt)
 {
-    TRay wr = make_tray(wo, wd);
-    TRay r = wr;
+    TRay r = make_tray(wo, wd); /* world ray; the mesh's object ray inside a mesh */
     HitRec best;
     best.t = FLT_MAX;
     best.obj = -1;
@@ -325,7 +326,7 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                         best.rank = rank;
                     }
                 }
-                r = wr;
+                r = make_tray(wo, wd); /* back to the world ray (same floats) */
                 level = 0;
                 continue;
             }
@@ -372,7 +373,7 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                     cnt.triangles++;
                 }
                 float t;
-                if (tri_t(A, B, C, r, t)) {
+                if (tri_t(A, B, C, r, t, ANY ? FLT_MAX : fminf(m_t, best.t))) {
                     if (ANY) {
                         best.t = t;
                         best.tri = S.items[item_cur];
