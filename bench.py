@@ -398,13 +398,12 @@ def main():
     tuning = None
     if not args.no_autotune:
         t, tms = ctx.autotune(sid, p, rounds=2)
-        tuning = {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2}
+        tuning = {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2,
+                  "graphs": t.graphs, "persistent": t.persistent}
         log(args, f"autotune: {tuning}, {tms:.2f} ms/frame")
     for _ in range(args.warmup):
         ctx.render_device(sid, p, out)
     ctx.synchronize()
-    ctx.set_profiling(True)
-    ctx.kernel_stats(reset=True)
     rays = 0
     dev_ms = 0.0
     ctx.synchronize()   # device-wide barrier before and after the timed steps (one GPU)
@@ -415,6 +414,14 @@ def main():
         dev_ms += st.render_ms
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    # the same steps again with every launch bracketed by HIP events on the
+    # library's stream (per-kernel durations for the roofline; kernels are
+    # launched one by one while events are recorded)
+    ctx.set_profiling(True)
+    ctx.kernel_stats(reset=True)
+    for _ in range(args.steps):
+        ctx.render_device(sid, p, out)
+    ctx.synchronize()
     ctx.set_profiling(False)
     ks = ctx.kernel_stats(reset=True)
     # visit counters of the same frame (separate, untimed run)

@@ -358,6 +358,10 @@ typedef struct sray_tuning {
     int32_t gather_lanes;     /* 8, 16 or 32 lanes sharing one photon-gather query */
     int32_t chunk_log2;       /* 16..21: rays per bounce-level sub-chunk = 2^chunk_log2 */
     float   gather_cell;      /* photon-grid cell size in gather radii: 0.5, 1 or 2 */
+    int32_t graphs;           /* 1: each primary chunk's bounce levels are replayed as a captured
+                                 HIP graph; 0: launched kernel by kernel (same kernels, same order) */
+    int32_t persistent;       /* 1: traversal kernels run as many waves as stay resident, each
+                                 looping over runs of 64 rays; 0: one run per wave */
     int32_t pad_;
 } sray_tuning;
 int  sray_gpu_set_tuning(sray_gpu_ctx *ctx, const sray_tuning *t);

@@ -83,9 +83,11 @@ __global__ void k_primary(CamDev C, RenderDev P, int sample, sray_ray *out, uint
 /* one primary ray per active pixel of the chunk [c0, c0+m), at the
  * pixel's next sample index */
 template <bool UNI>
-__global__ void k_raygen(Tabs T, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m, QRay *__restrict__ q)
+__global__ void k_raygen(Tabs T, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m, QRay *__restrict__ q,
+                         uint32_t *__restrict__ count)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count = m; /* level 0's count for the level kernels */
     if (i >= m) return;
     uint32_t pr = active[c0 + i];
     uint32_t rq = UNI ? 0u : T.preq[pr];
@@ -106,42 +108,60 @@ __global__ void k_raygen(Tabs T, const uint32_t *__restrict__ active, uint32_t c
     q[i] = r;
 }
 
+/* The traversal kernels walk the step in runs of 64 rays per wave, wave w
+ * taking runs w, w + W, w + 2W, ... of the W waves of the grid: with a
+ * grid as large as the step (one run per wave) the hardware dispatcher
+ * balances the load; with a persistent grid (as many waves as stay
+ * resident) each wave loops.  Either way the launch shape does not depend
+ * on the step's count, which the kernels read from the device. */
+__device__ __forceinline__ uint32_t first_run() { return (blockIdx.x * blockDim.x + threadIdx.x) / 64u * 64u; }
+__device__ __forceinline__ uint32_t run_stride() { return gridDim.x * blockDim.x; }
+
 /* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
 template <bool COUNT, bool UNI, int BS>
-__global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__ q, uint32_t n,
+__global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
                                                 int4 *__restrict__ hits, unsigned long long *counts)
 {
     __shared__ uint32_t stk[STACK * BS];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 o = q[i].o, d = q[i].d;
-    const SceneDev S = T.scenes[UNI ? 0u : T.reqs[q[i].m.w >> SLOT_SHIFT].scene];
+    const uint32_t n = step_n(sn);
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
-    HitRec h = traverse<false, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
-                                      BS, cnt);
-    hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+    for (uint32_t base = first_run(); base < n; base += run_stride()) {
+        const uint32_t i = base + __lane_id();
+        if (i < n) {
+            float4 o = q[i].o, d = q[i].d;
+            const SceneDev S = T.scenes[UNI ? 0u : T.reqs[q[i].m.w >> SLOT_SHIFT].scene];
+            HitRec h = traverse<false, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w),
+                                              &stk[threadIdx.x], BS, cnt);
+            hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+        }
+    }
     flush_counts<COUNT>(cnt, counts);
 }
 
-/* occlusion of every shadow ray (any hit: cast_ray(ray) != NULL, src/shader.cc:266) */
+/* occlusion of every shadow ray (any hit: cast_ray(ray) != NULL,
+ * src/shader.cc:266), L slots per ray of the step */
 template <bool COUNT, bool UNI, int BS>
-__global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq, uint32_t n,
+__global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq, StepN sn, int L,
                                                      uint8_t *__restrict__ occ, unsigned long long *counts)
 {
     __shared__ uint32_t stk[STACK * BS];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 d = sq[i].d;
-    if (d.w == 0.0f) {
-        occ[i] = 0;
-        return;
-    }
-    float4 o = sq[i].o;
-    const SceneDev S = T.scenes[UNI ? 0u : T.reqs[__float_as_uint(sq[i].c.w)].scene];
+    const uint32_t n = step_n(sn) * (uint32_t)L;
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
-    HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w), &stk[threadIdx.x],
-                                     BS, cnt);
-    occ[i] = h.obj >= 0 ? 1 : 0;
+    for (uint32_t base = first_run(); base < n; base += run_stride()) {
+        const uint32_t i = base + __lane_id();
+        if (i < n) {
+            float4 d = sq[i].d;
+            if (d.w == 0.0f) {
+                occ[i] = 0;
+            } else {
+                float4 o = sq[i].o;
+                const SceneDev S = T.scenes[UNI ? 0u : T.reqs[__float_as_uint(sq[i].c.w)].scene];
+                HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w),
+                                                 &stk[threadIdx.x], BS, cnt);
+                occ[i] = h.obj >= 0 ? 1 : 0;
+            }
+        }
+    }
     flush_counts<COUNT>(cnt, counts);
 }
 
@@ -187,7 +207,7 @@ __device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uin
  * emitting direct terms, shadow rays, children and photon-gather requests */
 template <bool PHOTONS, bool UNI>
 __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restrict__ active, uint32_t c0,
-                                               const QRay *__restrict__ q, const int4 *__restrict__ hits, uint32_t n,
+                                               const QRay *__restrict__ q, const int4 *__restrict__ hits, StepN sn,
                                                int L, int K, float4 *__restrict__ direct, SRay *__restrict__ sq,
                                                QRay *__restrict__ kids, uint8_t *__restrict__ kflag,
                                                GatherReq *__restrict__ greq, unsigned long long *__restrict__ stats,
@@ -196,8 +216,12 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
     __shared__ uint32_t bcnt[5];
     if (threadIdx.x < 5) bcnt[threadIdx.x] = 0;
     __syncthreads();
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = step_n(sn);
     uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
+    /* grid-stride over the step, block-uniform bases (wave operations below) */
+    for (uint32_t gbase = blockIdx.x * blockDim.x; gbase < n; gbase += gridDim.x * blockDim.x) {
+    const uint32_t i = gbase + threadIdx.x;
+    const uint32_t before = c_sh + c_rl + c_rr + c_df;
     if (i < n) {
         QRay R = q[i];
         int4 h = hits[i];
@@ -422,7 +446,8 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
     }
     if (!UNI)
         wave_keyed_add(slot_rays, i < n ? q[i].m.w >> SLOT_SHIFT : 0u,
-                       (unsigned long long)(c_sh + c_rl + c_rr + c_df), i < n);
+                       (unsigned long long)(c_sh + c_rl + c_rr + c_df - before), i < n);
+    }
     if (c_sh) atomicAdd(&bcnt[0], c_sh);
     if (c_rl) atomicAdd(&bcnt[1], c_rl);
     if (c_rr) atomicAdd(&bcnt[2], c_rr);
@@ -435,13 +460,9 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
  * src/pmap.cc:109-139) added to the direct term, or the radiance estimate
  * (src/pmap.cc:141-178) of a diffuse GI ray times kd */
 template <bool UNI, int GATHER_LANES>
-__global__ __launch_bounds__(256) void k_gather(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
-                                                uint32_t n, float4 *__restrict__ direct)
+__device__ __forceinline__ void gather_one(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
+                                           uint32_t i, int lane, float4 *__restrict__ direct)
 {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t i = t / GATHER_LANES;
-    int lane = (int)(t % GATHER_LANES);
-    if (i >= n) return; /* whole groups leave together */
     float4 pos = greq[i].pos;
     uint32_t mode = __float_as_uint(pos.w);
     if (mode == 0u) return;
@@ -471,13 +492,24 @@ __global__ __launch_bounds__(256) void k_gather(Tabs T, const QRay *__restrict__
     direct[i] = d;
 }
 
-/* per-sample accumulation: the first ray of each run of equal sample slots
- * sums the run (queues are ordered by slot) */
-__global__ void k_accum(const QRay *__restrict__ q, const float4 *__restrict__ direct, const SRay *__restrict__ sq,
-                        const uint8_t *__restrict__ occ, uint32_t n, int L, float4 *__restrict__ acc)
+template <bool UNI, int GATHER_LANES>
+__global__ __launch_bounds__(256) void k_gather(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
+                                                StepN sn, float4 *__restrict__ direct)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t n = step_n(sn);
+    const uint32_t groups = gridDim.x * (blockDim.x / GATHER_LANES);
+    const int lane = (int)(threadIdx.x % GATHER_LANES);
+    /* grid-stride over the requests, one group of lanes each (whole groups
+     * move and leave together) */
+    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / GATHER_LANES; i < n; i += groups)
+        gather_one<UNI, GATHER_LANES>(T, q, greq, i, lane, direct);
+}
+
+/* the sum of the run of equal sample slots starting at i, if i starts one */
+__device__ __forceinline__ void accum_run(const QRay *__restrict__ q, const float4 *__restrict__ direct,
+                                          const SRay *__restrict__ sq, const uint8_t *__restrict__ occ, uint32_t n,
+                                          int L, float4 *__restrict__ acc, uint32_t i)
+{
     uint32_t slot = q[i].m.x;
     if (i > 0 && q[i - 1].m.x == slot) return;
     float4 s = acc[slot];
@@ -498,6 +530,104 @@ __global__ void k_accum(const QRay *__restrict__ q, const float4 *__restrict__ d
         }
     }
     acc[slot] = s;
+}
+
+/* per-sample accumulation: the first ray of each run of equal sample slots
+ * sums the run (queues are ordered by slot) */
+__global__ void k_accum(const QRay *__restrict__ q, const float4 *__restrict__ direct, const SRay *__restrict__ sq,
+                        const uint8_t *__restrict__ occ, StepN sn, int L, float4 *__restrict__ acc)
+{
+    const uint32_t n = step_n(sn);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        accum_run(q, direct, sq, occ, n, L, acc, i);
+}
+
+/* ------------------------------------------------------------------ */
+/* stable compaction of a level step's flagged children into the next   */
+/* level's queue, with the count read and written on the device         */
+/* ------------------------------------------------------------------ */
+constexpr uint32_t CMP_BLOCKS = 1024, CMP_THREADS = 256;
+
+/* block b's items: an equal share of whole 256-item tiles, in order */
+__device__ __forceinline__ void cmp_range(uint32_t items, uint32_t b, uint32_t &lo, uint32_t &hi)
+{
+    const uint32_t tiles = (items + CMP_THREADS - 1u) / CMP_THREADS;
+    const uint32_t per = (tiles + CMP_BLOCKS - 1u) / CMP_BLOCKS;
+    lo = min(b * per * CMP_THREADS, items);
+    hi = min((b + 1u) * per * CMP_THREADS, items);
+}
+
+/* pass 1: flagged items per block */
+__global__ __launch_bounds__(CMP_THREADS) void k_cmp_count(const uint8_t *__restrict__ flags, StepN sn, int K,
+                                                           uint32_t *__restrict__ bcount)
+{
+    __shared__ uint32_t wsum[CMP_THREADS / 64];
+    const uint32_t items = step_n(sn) * (uint32_t)K;
+    uint32_t lo, hi;
+    cmp_range(items, blockIdx.x, lo, hi);
+    uint32_t c = 0;
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += CMP_THREADS) c += flags[j] ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) c += (uint32_t)__shfl_xor((int)c, off);
+    if (__lane_id() == 0) wsum[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < CMP_THREADS / 64; w++) t += wsum[w];
+        bcount[blockIdx.x] = t;
+    }
+}
+
+/* pass 2 (one block of CMP_BLOCKS threads): exclusive offsets of the
+ * blocks, and the total = the next level's count */
+__global__ __launch_bounds__(CMP_BLOCKS) void k_cmp_scan(const uint32_t *__restrict__ bcount,
+                                                         uint32_t *__restrict__ boff, uint32_t *__restrict__ total)
+{
+    __shared__ uint32_t wsum[CMP_BLOCKS / 64];
+    const uint32_t t = threadIdx.x, lane = __lane_id(), w = t / 64u;
+    const uint32_t v = bcount[t];
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = (uint32_t)__shfl_up((int)x, o);
+        if ((int)lane >= o) x += y;
+    }
+    if (lane == 63u) wsum[w] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t k = 0; k < w; k++) pre += wsum[k];
+    const uint32_t ex = pre + x - v;
+    boff[t] = ex;
+    if (t == CMP_BLOCKS - 1u) *total = ex + v;
+}
+
+/* pass 3: each block writes its flagged items in order from its offset */
+__global__ __launch_bounds__(CMP_THREADS) void k_cmp_scatter(const QRay *__restrict__ in,
+                                                             const uint8_t *__restrict__ flags, StepN sn, int K,
+                                                             const uint32_t *__restrict__ boff,
+                                                             QRay *__restrict__ out)
+{
+    __shared__ uint32_t wsum[CMP_THREADS / 64];
+    const uint32_t items = step_n(sn) * (uint32_t)K;
+    uint32_t lo, hi;
+    cmp_range(items, blockIdx.x, lo, hi);
+    const uint32_t lane = __lane_id(), w = threadIdx.x / 64u;
+    uint32_t base = boff[blockIdx.x];
+    for (uint32_t t0 = lo; t0 < hi; t0 += CMP_THREADS) {
+        const uint32_t j = t0 + threadIdx.x;
+        const bool f = j < hi && flags[j] != 0;
+        const unsigned long long bal = __ballot(f);
+        const uint32_t r = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0u) wsum[w] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t wpre = 0, tot = 0;
+        for (uint32_t k = 0; k < CMP_THREADS / 64; k++) {
+            const uint32_t c = wsum[k];
+            wpre += k < w ? c : 0u;
+            tot += c;
+        }
+        if (f) out[base + wpre + r] = in[j];
+        base += tot;
+        __syncthreads();
+    }
 }
 
 __global__ void k_fill_f4(float4 *p, size_t n, float4 v)
@@ -772,6 +902,14 @@ struct Engine {
     std::vector<DevBuf> qlev;
     DevBuf hits, direct, sq, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs, dscenes, stage,
         greq;
+    /* device-driven levels: per-level ray counts, compaction block counts /
+     * offsets */
+    DevBuf lvcnt, cmpb;
+    std::map<const void *, uint32_t> pgrid; /* persistent grid per traversal kernel instance */
+    /* captured descents, keyed by everything their launches depend on */
+    std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
+    bool use_graphs = true;
+    bool persistent = false; /* traversal grids of resident size (each wave loops) */
     /* requests in flight */
     std::vector<ReqDev> hreqs = std::vector<ReqDev>(MAX_SLOTS);
     std::vector<SceneDev> hscenes;
@@ -806,7 +944,7 @@ struct Engine {
     {
         uint64_t b = 0;
         const DevBuf *bs[] = {&hits, &direct, &sq, &occ, &kids, &kflag, &acc, &stats, &errf, &rcounts,
-                              &slotcnt, &slotrays, &dreqs, &dscenes, &stage, &greq};
+                              &slotcnt, &slotrays, &dreqs, &dscenes, &stage, &greq, &lvcnt, &cmpb};
         for (const DevBuf *d : bs) b += d->bytes;
         for (const DevBuf &d : qlev) b += d.bytes;
         return b;
@@ -910,6 +1048,8 @@ void apply_tuning(Engine &E)
     E.tile_w = (uint32_t)t.tile_w;
     E.gather_lanes = t.gather_lanes;
     E.chunk = 1u << t.chunk_log2;
+    E.use_graphs = t.graphs != 0;
+    E.persistent = t.persistent != 0;
 }
 
 /* places a request in the arena and the active list */
@@ -1054,68 +1194,190 @@ struct Level {
     uint32_t c0;
 };
 
-/* one bounce level over qlev[b][0,count): sub-chunks of CHUNK rays, each
- * followed depth-first by its children's levels */
-int process_level(Level &LV, int b, uint32_t count)
+/* grid of a persistent traversal kernel: as many blocks as the device
+ * keeps resident at once */
+template <typename K> uint32_t persistent_grid(Engine &E, K kern, int bs)
+{
+    auto it = E.pgrid.find((const void *)kern);
+    if (it != E.pgrid.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, bs, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, E.c->device) != hipSuccess || cus < 1)
+        cus = 1;
+    uint32_t g = (uint32_t)per_cu * (uint32_t)cus;
+    E.pgrid[(const void *)kern] = g;
+    return g;
+}
+
+/* one level step: rays [off, off + chunk) of level b's queue (count on the
+ * device) through trace, shade, gather, shadow trace and accumulation,
+ * and (below the deepest level) the stable compaction of their children
+ * into level b + 1's queue, which also writes that level's count */
+int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
 {
     Engine &E = *LV.E;
     sray_gpu_ctx *c = E.c;
-    int rc;
-    QRay *Q = dptr<QRay>(E.qlev[b]);
+    const StepN sn{dptr<uint32_t>(E.lvcnt) + b, off, cap};
+    const QRay *q = dptr<QRay>(E.qlev[b]) + off;
     unsigned long long *rc6 = dptr<unsigned long long>(E.rcounts);
     unsigned long long *stats = dptr<unsigned long long>(E.stats);
     unsigned long long *srays = dptr<unsigned long long>(E.slotrays);
     const int bs = E.trace_block;
-    for (uint32_t off = 0; off < count; off += E.chunk) {
-        uint32_t m = std::min(E.chunk, count - off);
-        const QRay *q = Q + off;
-        {
-            ProfScope ps(c, "k_trace_q");
-            /* the traversal reads only the scene: one scene = the uniform variant */
-            auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs) : trace_q_of<true, false>(bs))
-                                   : (E.one_scene ? trace_q_of<false, true>(bs) : trace_q_of<false, false>(bs));
-            kern<<<blocks(m, bs), bs, 0, c->stream>>>(LV.T, q, m, dptr<int4>(E.hits), rc6);
-        }
-        {
-            ProfScope ps(c, "k_shade");
-            auto kern = E.photons ? (E.uni ? k_shade<true, true> : k_shade<true, false>)
-                                  : (E.uni ? k_shade<false, true> : k_shade<false, false>);
-            kern<<<blocks(m, 256), 256, 0, c->stream>>>(LV.T, LV.active, LV.c0, q, dptr<int4>(E.hits), m, E.L, E.K,
-                                                        dptr<float4>(E.direct), dptr<SRay>(E.sq),
-                                                        dptr<QRay>(E.kids), dptr<uint8_t>(E.kflag),
-                                                        dptr<GatherReq>(E.greq), stats, srays,
-                                                        dptr<uint32_t>(E.errf));
-        }
-        if (E.photons) {
-            ProfScope ps(c, "k_gather");
-            auto kern = E.uni ? gather_of<true>(E.gather_lanes) : gather_of<false>(E.gather_lanes);
-            kern<<<blocks((size_t)m * E.gather_lanes, 256), 256, 0, c->stream>>>(LV.T, q, dptr<GatherReq>(E.greq), m,
-                                                                                 dptr<float4>(E.direct));
-        }
-        size_t ns = (size_t)m * E.L;
-        if (ns) {
-            ProfScope ps(c, "k_trace_shadow");
-            auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs) : trace_shadow_of<true, false>(bs))
-                                   : (E.one_scene ? trace_shadow_of<false, true>(bs)
-                                                  : trace_shadow_of<false, false>(bs));
-            kern<<<blocks(ns, bs), bs, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), (uint32_t)ns,
-                                                                         dptr<uint8_t>(E.occ), rc6 + 6);
-        }
-        {
-            ProfScope ps(c, "k_accum");
-            k_accum<<<blocks(m, 256), 256, 0, c->stream>>>(q, dptr<float4>(E.direct), dptr<SRay>(E.sq),
-                                                          dptr<uint8_t>(E.occ), m, E.L, dptr<float4>(E.acc));
-        }
-        if (b + 1 > E.maxiter) continue;
-        uint32_t nk = 0;
-        if ((rc = select_flagged(c, dptr<QRay>(E.kids), dptr<uint8_t>(E.kflag), dptr<QRay>(E.qlev[b + 1]),
-                                 m * (uint32_t)E.K, &nk)))
-            return rc;
-        HIPCHK(c, hipGetLastError());
-        if (nk) {
-            if ((rc = process_level(LV, b + 1, nk))) return rc;
-        }
+    const uint32_t small_grid = std::min<uint32_t>(blocks(cap, 256), 2048u);
+    /* traversal grid: one run of 64 rays per wave, or the resident waves */
+    auto tgrid = [&](auto kern, size_t items) {
+        uint32_t g = blocks(items, bs);
+        return E.persistent ? std::min(g, persistent_grid(E, kern, bs)) : g;
+    };
+    {
+        ProfScope ps(c, "k_trace_q");
+        /* the traversal reads only the scene: one scene = the uniform variant */
+        auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs) : trace_q_of<true, false>(bs))
+                               : (E.one_scene ? trace_q_of<false, true>(bs) : trace_q_of<false, false>(bs));
+        kern<<<tgrid(kern, cap), bs, 0, c->stream>>>(LV.T, q, sn, dptr<int4>(E.hits), rc6);
     }
+    {
+        ProfScope ps(c, "k_shade");
+        auto kern = E.photons ? (E.uni ? k_shade<true, true> : k_shade<true, false>)
+                              : (E.uni ? k_shade<false, true> : k_shade<false, false>);
+        kern<<<small_grid, 256, 0, c->stream>>>(LV.T, LV.active, LV.c0, q, dptr<int4>(E.hits), sn, E.L, E.K,
+                                               dptr<float4>(E.direct), dptr<SRay>(E.sq), dptr<QRay>(E.kids),
+                                               dptr<uint8_t>(E.kflag), dptr<GatherReq>(E.greq), stats, srays,
+                                               dptr<uint32_t>(E.errf));
+    }
+    if (E.photons) {
+        ProfScope ps(c, "k_gather");
+        auto kern = E.uni ? gather_of<true>(E.gather_lanes) : gather_of<false>(E.gather_lanes);
+        const uint32_t g = std::min<uint32_t>(blocks((size_t)cap * E.gather_lanes, 256), 8192u);
+        kern<<<g, 256, 0, c->stream>>>(LV.T, q, dptr<GatherReq>(E.greq), sn, dptr<float4>(E.direct));
+    }
+    if (E.L) {
+        ProfScope ps(c, "k_trace_shadow");
+        auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs) : trace_shadow_of<true, false>(bs))
+                               : (E.one_scene ? trace_shadow_of<false, true>(bs)
+                                              : trace_shadow_of<false, false>(bs));
+        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), sn, E.L,
+                                                                   dptr<uint8_t>(E.occ), rc6 + 6);
+    }
+    {
+        ProfScope ps(c, "k_accum");
+        k_accum<<<small_grid, 256, 0, c->stream>>>(q, dptr<float4>(E.direct), dptr<SRay>(E.sq),
+                                                   dptr<uint8_t>(E.occ), sn, E.L, dptr<float4>(E.acc));
+    }
+    if (b < E.maxiter) {
+        ProfScope ps(c, "k_compact");
+        uint32_t *bc = dptr<uint32_t>(E.cmpb);
+        k_cmp_count<<<CMP_BLOCKS, CMP_THREADS, 0, c->stream>>>(dptr<uint8_t>(E.kflag), sn, E.K, bc);
+        k_cmp_scan<<<1, CMP_BLOCKS, 0, c->stream>>>(bc, bc + CMP_BLOCKS, dptr<uint32_t>(E.lvcnt) + b + 1);
+        k_cmp_scatter<<<CMP_BLOCKS, CMP_THREADS, 0, c->stream>>>(dptr<QRay>(E.kids), dptr<uint8_t>(E.kflag), sn,
+                                                                 E.K, bc + CMP_BLOCKS, dptr<QRay>(E.qlev[b + 1]));
+    }
+    HIPCHK(c, hipGetLastError());
+    return SRAY_OK;
+}
+
+/* the step (b, off) of at most cap rays, then the first sub-chunk of
+ * every deeper level: no host round trip between levels.  A level holds
+ * at most K times the rays of the one above, which bounds each step's
+ * launch size (small requests launch small grids). */
+int descend(Level &LV, int b, uint32_t off, uint32_t cap)
+{
+    Engine &E = *LV.E;
+    int rc;
+    uint64_t bound = cap;
+    for (int d = b; d <= E.maxiter; d++) {
+        if ((rc = launch_step(LV, d, d == b ? off : 0u, (uint32_t)std::min<uint64_t>(bound, E.chunk)))) return rc;
+        bound = std::min<uint64_t>(bound * (uint64_t)E.K, E.chunk);
+    }
+    return SRAY_OK;
+}
+
+/* After a descent from level b: the sub-chunks beyond the first of levels
+ * b+1 .. maxiter, deepest first, each followed by its own descent.  This
+ * is exactly the depth-first order of processing every level in chunks
+ * with each chunk's children before the next chunk (the queues of levels
+ * <= d are untouched while level d's subtree runs). */
+int finish_levels(Level &LV, int b)
+{
+    Engine &E = *LV.E;
+    sray_gpu_ctx *c = E.c;
+    if (b >= E.maxiter) return SRAY_OK;
+    uint32_t n[64];
+    const int nl = E.maxiter - b;
+    HIPCHK(c, hipMemcpyAsync(n, dptr<uint32_t>(E.lvcnt) + b + 1, (size_t)nl * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc;
+    for (int d = E.maxiter; d > b; d--)
+        for (uint64_t off = E.chunk; off < n[d - b - 1]; off += E.chunk) {
+            if ((rc = descend(LV, d, (uint32_t)off, E.chunk))) return rc;
+            if ((rc = finish_levels(LV, d))) return rc;
+        }
+    return SRAY_OK;
+}
+
+/* whether some level of a descent from m primary rays can hold more rays
+ * than one sub-chunk (a level holds at most K times the previous one) */
+bool may_overflow(const Engine &E, uint32_t m)
+{
+    uint64_t worst = m;
+    for (int d = 1; d <= E.maxiter; d++) {
+        worst *= (uint64_t)E.K;
+        if (worst > E.chunk) return true;
+    }
+    return false;
+}
+
+/* ray generation and the descent of the primary chunk [c0, c0 + m): as
+ * kernel launches, or as one captured graph replayed (same kernels, same
+ * order) */
+int launch_chunk(Level &LV, uint32_t m)
+{
+    Engine &E = *LV.E;
+    sray_gpu_ctx *c = E.c;
+    auto body = [&]() -> int {
+        k_fill_f4<<<blocks(m, 256), 256, 0, c->stream>>>(dptr<float4>(E.acc), m, make_float4(0, 0, 0, 0));
+        {
+            ProfScope ps(c, "k_raygen");
+            auto kern = E.uni ? k_raygen<true> : k_raygen<false>;
+            kern<<<blocks(m, 256), 256, 0, c->stream>>>(LV.T, LV.active, LV.c0, m, dptr<QRay>(E.qlev[0]),
+                                                        dptr<uint32_t>(E.lvcnt));
+        }
+        return descend(LV, 0, 0, m);
+    };
+    if (!E.use_graphs || c->profiling) return body();
+    std::vector<uint64_t> key = {LV.c0, m, (uint64_t)E.maxiter, (uint64_t)E.L, (uint64_t)E.K, E.chunk,
+                                 (uint64_t)E.trace_block, (uint64_t)E.gather_lanes,
+                                 (uint64_t)(E.photons | E.uni << 1 | E.one_scene << 2 | E.counting << 3 |
+                                            E.persistent << 4),
+                                 (uint64_t)(uintptr_t)LV.T.scenes, (uint64_t)(uintptr_t)LV.T.reqs,
+                                 (uint64_t)(uintptr_t)LV.T.preq, (uint64_t)(uintptr_t)LV.T.nsamp,
+                                 (uint64_t)(uintptr_t)LV.active};
+    const DevBuf *bufs[] = {&E.hits, &E.direct, &E.sq, &E.occ, &E.kids, &E.kflag, &E.acc, &E.stats, &E.errf,
+                            &E.rcounts, &E.slotrays, &E.greq, &E.lvcnt, &E.cmpb};
+    for (const DevBuf *d : bufs) key.push_back((uint64_t)(uintptr_t)d->p);
+    for (int b = 0; b <= E.maxiter; b++) key.push_back((uint64_t)(uintptr_t)E.qlev[b].p);
+    auto it = E.graphs.find(key);
+    if (it == E.graphs.end()) {
+        if (E.graphs.size() >= 64) {
+            for (auto &kv : E.graphs) (void)hipGraphExecDestroy(kv.second);
+            E.graphs.clear();
+        }
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = body();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, SRAY_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(c, SRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        it = E.graphs.emplace(key, x).first;
+    }
+    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
     return SRAY_OK;
 }
 
@@ -1139,8 +1401,10 @@ int run_pass(Engine &E, std::vector<Job *> &finished)
         return rc;
     if (E.sub.cap == 0 && (rc = ensure(c, E.subpix, 16))) return rc;
     if ((rc = ensure(c, E.errf, 4)) || (rc = ensure(c, E.stats, 8 * sizeof(unsigned long long))) ||
-        (rc = ensure(c, E.rcounts, 12 * sizeof(unsigned long long))))
+        (rc = ensure(c, E.rcounts, 12 * sizeof(unsigned long long))) || (rc = ensure(c, E.lvcnt, 64 * 4)) ||
+        (rc = ensure(c, E.cmpb, 2 * CMP_BLOCKS * 4)))
         return rc;
+
     HIPCHK(c, hipMemsetAsync(E.errf.p, 0, 4, c->stream));
     Tabs T{dptr<SceneDev>(E.dscenes), dptr<ReqDev>(E.dreqs), dptr<uint32_t>(E.preq), dptr<int>(E.nsamp)};
     if (!E.uni) HIPCHK(c, hipMemsetAsync(E.slotrays.p, 0, (size_t)E.hi * 8, c->stream));
@@ -1148,14 +1412,11 @@ int run_pass(Engine &E, std::vector<Job *> &finished)
     Level LV{&E, T, A, 0};
     for (uint32_t c0 = 0; c0 < E.nact; c0 += E.chunk) {
         uint32_t m = std::min(E.chunk, E.nact - c0);
-        k_fill_f4<<<blocks(m, 256), 256, 0, c->stream>>>(dptr<float4>(E.acc), m, make_float4(0, 0, 0, 0));
-        {
-            ProfScope ps(c, "k_raygen");
-            auto kern = E.uni ? k_raygen<true> : k_raygen<false>;
-            kern<<<blocks(m, 256), 256, 0, c->stream>>>(T, A, c0, m, dptr<QRay>(E.qlev[0]));
-        }
         LV.c0 = c0;
-        if ((rc = process_level(LV, 0, m))) return rc;
+        if ((rc = launch_chunk(LV, m))) return rc;
+        /* a level may have outgrown one sub-chunk: read the counts and run
+         * the remaining sub-chunks (never needed when m * K^maxiter fits) */
+        if (may_overflow(E, m) && (rc = finish_levels(LV, 0))) return rc;
         auto fin = E.uni ? k_pass_finish<true> : k_pass_finish<false>;
         fin<<<blocks(m, 256), 256, 0, c->stream>>>(T, A, c0, m, dptr<float4>(E.acc), dptr<float4>(E.fb),
                                                   dptr<float4>(E.subpix), dptr<int>(E.nsamp), dptr<uint8_t>(E.keep));
@@ -1230,9 +1491,11 @@ Engine *engine_get(sray_gpu_ctx *c, Engine *&slot)
 void engine_free(Engine *E)
 {
     if (!E) return;
+    for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
+    E->graphs.clear();
     DevBuf *bs[] = {&E->fb, &E->nsamp, &E->preq, &E->keep, &E->act[0], &E->act[1], &E->subpix, &E->hits,
                     &E->direct, &E->sq, &E->occ, &E->kids, &E->kflag, &E->acc, &E->stats, &E->errf, &E->rcounts,
-                    &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->stage, &E->greq};
+                    &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->stage, &E->greq, &E->lvcnt, &E->cmpb};
     for (DevBuf *b : bs) release(*b);
     for (DevBuf &b : E->qlev) release(b);
     delete E;
@@ -1485,7 +1748,8 @@ int check_tuning(sray_gpu_ctx *c, const sray_tuning &t)
     bool ok = (t.trace_block == 64 || t.trace_block == 128 || t.trace_block == 256) &&
               (t.tile_w == 8 || t.tile_w == 16 || t.tile_w == 32 || t.tile_w == 64) &&
               (t.gather_lanes == 8 || t.gather_lanes == 16 || t.gather_lanes == 32) && t.chunk_log2 >= 16 &&
-              t.chunk_log2 <= 21 && (t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f);
+              t.chunk_log2 <= 21 && (t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f) &&
+              (t.graphs == 0 || t.graphs == 1) && (t.persistent == 0 || t.persistent == 1);
     return ok ? SRAY_OK : fail(c, SRAY_ERR_ARG, "tuning value outside the supported set");
 }
 
@@ -1523,16 +1787,18 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
     static const int blocks_[] = {64, 128, 256}, tiles[] = {8, 16, 32, 64}, lanes[] = {8, 16, 32}, chunks[] = {18, 19, 20, 21};
     static const float cells[] = {0.5f, 1.0f, 2.0f};
     bool photons = p->caust_photons || p->gi_photons;
-    for (int knob = 0; knob < 5; knob++) {
+    for (int knob = 0; knob < 7; knob++) {
         if ((knob == 2 || knob == 4) && !photons) continue;
-        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : 3;
+        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 3 : 2;
         for (int k = 0; k < nv; k++) {
             sray_tuning t = cur;
             if (knob == 0) t.trace_block = blocks_[k];
             else if (knob == 1) t.tile_w = tiles[k];
             else if (knob == 2) t.gather_lanes = lanes[k];
             else if (knob == 3) t.chunk_log2 = chunks[k];
-            else t.gather_cell = cells[k];
+            else if (knob == 4) t.gather_cell = cells[k];
+            else if (knob == 5) t.graphs = k;
+            else t.persistent = k;
             if (!memcmp(&t, &cur, sizeof t)) continue;
             c->tune = t;
             double ms = 0;

@@ -92,6 +92,20 @@ struct Tabs {
     const int *nsamp;     /* arena pixel -> samples taken */
 };
 
+/* the rays of one level step: [off, off + n) of a level queue whose full
+ * count *cnt the device wrote (the previous level's compaction, or the ray
+ * generator), at most cap of them.  Kernels read the count themselves, so
+ * a bounce level's launches need nothing from the host. */
+struct StepN {
+    const uint32_t *cnt;
+    uint32_t off, cap;
+};
+__device__ __forceinline__ uint32_t step_n(const StepN &s)
+{
+    const uint32_t c = *s.cnt;
+    return c > s.off ? min(c - s.off, s.cap) : 0u;
+}
+
 constexpr int SLOT_SHIFT = 16;
 constexpr uint32_t MAX_SLOTS = 1024; /* requests in flight in one engine */
 

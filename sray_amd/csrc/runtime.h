@@ -93,7 +93,7 @@ struct sray_gpu_ctx {
     };
     std::map<std::string, KStat> kstats;
     /* tuning of the renderers and photon grids */
-    sray_tuning tune{128, 64, 16, 21, 1.0f, 0};
+    sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 0};
     /* renderers: one for synchronous renders, one behind submit/poll */
     srt::Engine *plain = nullptr;
     srt::Engine *serve = nullptr;
