@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tuning (no untimed search)")
+    ap.add_argument("--tuning", default=None,
+                    help="heightfield: set the tuning instead of searching, e.g. "
+                         "trace_block=64,tile_w=64,chunk_log2=21,graphs=1,persistent=0")
     ap.add_argument("--frames", type=int, default=None, help="room / spheres: frames in flight (8 / 64)")
     ap.add_argument("--scenes", type=int, default=None, help="batch: scenes (256); serve: resident scenes (32)")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrival rate (req/s)")
@@ -91,7 +94,9 @@ def cpu_baseline(args, hs, wl, p=None, full=False):
     band = H if full else 16
     rows, rays, elapsed = 0, 0, 0.0
     y0 = 0 if full else max(0, H // 2 - band)
-    while rows < H and (full or elapsed < args.cpu_seconds):
+    # whole frames (full) until about --cpu-seconds of work, or bands of rows
+    # until the frame is covered or the budget is spent
+    while (elapsed < args.cpu_seconds) if full else (rows < H and elapsed < args.cpu_seconds):
         y1 = min(H, y0 + band)
         q = S.render_params(W, H, rect=(0, y0, W, y1), samples=samples, iter=it)
         if p is not None:
@@ -105,7 +110,8 @@ def cpu_baseline(args, hs, wl, p=None, full=False):
         rows += y1 - y0
         y0 = y1 if y1 < H else 0
     log(args, f"cpu baseline: {rows} rows, {rays} rays, {elapsed:.1f} s, {nthreads} threads")
-    what = "the whole frame" if full else f"{rows} rows x {W} px ({rows * W} pixels) of the {W}x{H} frame"
+    what = (f"{rows // H} whole frames" if full else
+            f"{rows} rows x {W} px ({rows * W} pixels) of the {W}x{H} frame")
     return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": nthreads, "kind": "port",
             "sample": f"{what}, {rays} ray queries, double precision, -O3 -ffast-math, "
                       f"mesh re-traversal per scene leaf as the reference"}
@@ -396,7 +402,13 @@ def main():
     p = S.render_params(W, H, samples=samples, iter=it)
     out = ctx.malloc(W * H * 4 * 4)
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        kw = {k: (float(v) if k == "gather_cell" else int(v))
+              for k, v in (item.split("=") for item in args.tuning.split(","))}
+        t = ctx.set_tuning(**kw)
+        tuning = {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2,
+                  "graphs": t.graphs, "persistent": t.persistent}
+    elif not args.no_autotune:
         t, tms = ctx.autotune(sid, p, rounds=2)
         tuning = {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2,
                   "graphs": t.graphs, "persistent": t.persistent}
