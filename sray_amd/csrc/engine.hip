@@ -667,6 +667,15 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
+/* active pixels per request slot, the list's length read on the device */
+__global__ void k_slot_count_dev(const uint32_t *__restrict__ active, const uint32_t *__restrict__ np,
+                                 const uint32_t *__restrict__ preq, uint32_t *__restrict__ cnt)
+{
+    const uint32_t n = *np;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    wave_keyed_add(cnt, i < n ? preq[active[i]] : 0u, 1u, i < n);
+}
+
 /* active pixels per request slot */
 __global__ void k_slot_count(const uint32_t *__restrict__ active, uint32_t n, const uint32_t *__restrict__ preq,
                              uint32_t *__restrict__ cnt)
@@ -877,8 +886,9 @@ struct Job {
     sray_render_params p{};
     sray_camera_desc cam{};
     bool use_cam = false;
-    float *host_out = nullptr; /* result to host memory, or */
+    float *host_out = nullptr; /* result to host memory (through stage), or */
     float *dev_out = nullptr;  /* to device memory */
+    DevBuf stage;
     uint32_t slot = 0;
     uint64_t pix_base = 0, npix = 0, sub_base = 0, nsub = 0;
     uint64_t rays = 0, prim = 0; /* ray queries of all kinds / primary rays */
@@ -900,8 +910,7 @@ struct Engine {
     uint32_t nact = 0;
     /* ray queues and per-level scratch */
     std::vector<DevBuf> qlev;
-    DevBuf hits, direct, sq, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs, dscenes, stage,
-        greq;
+    DevBuf hits, direct, sq, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs, dscenes, greq;
     /* device-driven levels: per-level ray counts, compaction block counts /
      * offsets */
     DevBuf lvcnt, cmpb;
@@ -920,6 +929,8 @@ struct Engine {
     int L = 0, K = 2, maxiter = 0;
     bool photons = false, uni = true, counting = false;
     bool one_scene = true; /* every request in flight renders the same scene */
+    int Kb = 2;            /* most children one ray can spawn (growth bound of the levels) */
+    bool more = true;      /* some request in flight can take another sample */
     /* tunables (sray_tuning), copied from the context at each pass */
     uint32_t tile_w = 64;        /* primary-ray tile width (x 64/tile_w rows) */
     uint32_t chunk = MAX_CHUNK;  /* rays per bounce-level sub-chunk */
@@ -937,6 +948,18 @@ struct Engine {
 
     uint64_t live_bytes = 0; /* footprints of the requests in flight */
 
+    /* pinned host memory: the status of a pass, read back with one copy and
+     * one synchronisation, and the staging of the request / scene tables */
+    struct PassStatus {
+        uint32_t nsel, errf;
+        unsigned long long stats[8], rcounts[12];
+        uint32_t slotcnt[MAX_SLOTS];
+        unsigned long long srays[MAX_SLOTS];
+    };
+    PassStatus *hst = nullptr;
+    ReqDev *hreq_pin = nullptr;
+    SceneDev *hscn_pin = nullptr;
+
     /* everything held on the device */
     uint64_t bytes() const { return fixed_bytes() + arena_bytes(); }
     /* ray queues and tables (sized by the largest pass so far) */
@@ -944,7 +967,7 @@ struct Engine {
     {
         uint64_t b = 0;
         const DevBuf *bs[] = {&hits, &direct, &sq, &occ, &kids, &kflag, &acc, &stats, &errf, &rcounts,
-                              &slotcnt, &slotrays, &dreqs, &dscenes, &stage, &greq, &lvcnt, &cmpb};
+                              &slotcnt, &slotrays, &dreqs, &dscenes, &greq, &lvcnt, &cmpb};
         for (const DevBuf *d : bs) b += d->bytes;
         for (const DevBuf &d : qlev) b += d.bytes;
         return b;
@@ -1141,8 +1164,12 @@ void retire(Engine &E, uint32_t s)
 /* drops every request in flight (after a device error) */
 void abort_all(Engine &E)
 {
+    (void)hipStreamSynchronize(E.c->stream);
     for (uint32_t s = 0; s < E.hi; s++)
-        if (E.slot_job[s]) retire(E, s);
+        if (E.slot_job[s]) {
+            release(E.slot_job[s]->stage);
+            retire(E, s);
+        }
     E.nact = 0;
 }
 
@@ -1154,8 +1181,17 @@ int build_tables(Engine &E)
     std::map<uint32_t, uint32_t> idx;
     E.L = 0;
     E.K = 2;
+    E.Kb = 0;
     E.maxiter = 0;
     E.photons = false;
+    E.more = false;
+    if (!E.hst) {
+        if (hipHostMalloc((void **)&E.hst, sizeof(Engine::PassStatus), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&E.hreq_pin, MAX_SLOTS * sizeof(ReqDev), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&E.hscn_pin, MAX_SLOTS * sizeof(SceneDev), hipHostMallocDefault) != hipSuccess)
+            return fail(c, SRAY_ERR_NOMEM, "pinned host memory for the pass status");
+        memset(E.hst, 0, sizeof(Engine::PassStatus));
+    }
     for (uint32_t s = 0; s < E.hi; s++) {
         Job *j = E.slot_job[s];
         if (!j) continue;
@@ -1173,6 +1209,15 @@ int build_tables(Engine &E)
         E.K = std::max(E.K, 2 + (Q.P.gi ? Q.P.diffuse_samples : 0));
         E.maxiter = std::max(E.maxiter, Q.P.iter);
         E.photons = E.photons || Q.P.gi || Q.P.caust;
+        E.more = E.more || Q.P.max_samples > 1;
+        /* children a ray of this request can spawn: reflect / refract only
+         * with a Phong material whose factor is > 0 (or any Phong material
+         * when min_energy < 0 lets zero-energy children through), plus the
+         * diffuse GI rays */
+        const bool any = g->has_phong && Q.P.min_energy < 0.0f;
+        const int br = ((g->pos_reflect || any) ? 1 : 0) + ((g->pos_refract || any) ? 1 : 0) +
+                       (Q.P.gi ? Q.P.diffuse_samples : 0);
+        E.Kb = std::max(E.Kb, br);
     }
     E.uni = E.live == 1 && E.slot_job[0] != nullptr;
     E.one_scene = E.hscenes.size() == 1;
@@ -1180,10 +1225,13 @@ int build_tables(Engine &E)
     if ((rc = ensure(c, E.dreqs, (size_t)E.hi * sizeof(ReqDev))) ||
         (rc = ensure(c, E.dscenes, E.hscenes.size() * sizeof(SceneDev))))
         return rc;
-    HIPCHK(c, hipMemcpyAsync(E.dreqs.p, E.hreqs.data(), (size_t)E.hi * sizeof(ReqDev), hipMemcpyHostToDevice,
+    /* through pinned staging (the previous pass's copies are complete: every
+     * pass ends with a synchronisation) */
+    memcpy(E.hreq_pin, E.hreqs.data(), (size_t)E.hi * sizeof(ReqDev));
+    memcpy(E.hscn_pin, E.hscenes.data(), E.hscenes.size() * sizeof(SceneDev));
+    HIPCHK(c, hipMemcpyAsync(E.dreqs.p, E.hreq_pin, (size_t)E.hi * sizeof(ReqDev), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(E.dscenes.p, E.hscn_pin, E.hscenes.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
                              c->stream));
-    HIPCHK(c, hipMemcpyAsync(E.dscenes.p, E.hscenes.data(), E.hscenes.size() * sizeof(SceneDev),
-                             hipMemcpyHostToDevice, c->stream));
     return SRAY_OK;
 }
 
@@ -1285,9 +1333,9 @@ int descend(Level &LV, int b, uint32_t off, uint32_t cap)
     Engine &E = *LV.E;
     int rc;
     uint64_t bound = cap;
-    for (int d = b; d <= E.maxiter; d++) {
+    for (int d = b; d <= E.maxiter && bound > 0; d++) {
         if ((rc = launch_step(LV, d, d == b ? off : 0u, (uint32_t)std::min<uint64_t>(bound, E.chunk)))) return rc;
-        bound = std::min<uint64_t>(bound * (uint64_t)E.K, E.chunk);
+        bound = std::min<uint64_t>(bound * (uint64_t)E.Kb, E.chunk);
     }
     return SRAY_OK;
 }
@@ -1316,12 +1364,12 @@ int finish_levels(Level &LV, int b)
 }
 
 /* whether some level of a descent from m primary rays can hold more rays
- * than one sub-chunk (a level holds at most K times the previous one) */
+ * than one sub-chunk (a level holds at most Kb times the previous one) */
 bool may_overflow(const Engine &E, uint32_t m)
 {
     uint64_t worst = m;
     for (int d = 1; d <= E.maxiter; d++) {
-        worst *= (uint64_t)E.K;
+        worst *= (uint64_t)E.Kb;
         if (worst > E.chunk) return true;
     }
     return false;
@@ -1349,6 +1397,7 @@ int launch_chunk(Level &LV, uint32_t m)
                                  (uint64_t)E.trace_block, (uint64_t)E.gather_lanes,
                                  (uint64_t)(E.photons | E.uni << 1 | E.one_scene << 2 | E.counting << 3 |
                                             E.persistent << 4),
+                                 (uint64_t)E.Kb,
                                  (uint64_t)(uintptr_t)LV.T.scenes, (uint64_t)(uintptr_t)LV.T.reqs,
                                  (uint64_t)(uintptr_t)LV.T.preq, (uint64_t)(uintptr_t)LV.T.nsamp,
                                  (uint64_t)(uintptr_t)LV.active};
@@ -1381,8 +1430,32 @@ int launch_chunk(Level &LV, uint32_t m)
     return SRAY_OK;
 }
 
+/* the image of the request in slot s: average (or the reference's Q1
+ * scaling) into its output, through the staging buffer to host memory */
+int finalize_job(Engine &E, uint32_t s)
+{
+    sray_gpu_ctx *c = E.c;
+    Job *j = E.slot_job[s];
+    const ReqDev &Q = E.hreqs[s];
+    uint32_t npix = (uint32_t)j->npix;
+    float *dst = j->dev_out;
+    int rc;
+    if (!dst) {
+        if ((rc = ensure(c, j->stage, (size_t)npix * 16))) return rc;
+        dst = dptr<float>(j->stage);
+    }
+    k_finalize<<<blocks(npix, 256), 256, 0, c->stream>>>(Q.P, dptr<float4>(E.fb) + j->pix_base,
+                                                        dptr<int>(E.nsamp) + j->pix_base, dst, npix);
+    if (Q.P.flags & SRAY_RENDER_REF_Q1)
+        k_q1_scale<<<blocks(npix, 256), 256, 0, c->stream>>>(Q.P, dptr<int>(E.nsamp) + j->pix_base, dst, npix);
+    HIPCHK(c, hipGetLastError());
+    if (j->host_out)
+        HIPCHK(c, hipMemcpyAsync(j->host_out, dst, (size_t)npix * 16, hipMemcpyDeviceToHost, c->stream));
+    return SRAY_OK;
+}
+
 /* one sample of every active pixel; retires the requests that finish */
-int run_pass(Engine &E, std::vector<Job *> &finished)
+int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
 {
     sray_gpu_ctx *c = E.c;
     int rc;
@@ -1421,61 +1494,63 @@ int run_pass(Engine &E, std::vector<Job *> &finished)
         fin<<<blocks(m, 256), 256, 0, c->stream>>>(T, A, c0, m, dptr<float4>(E.acc), dptr<float4>(E.fb),
                                                   dptr<float4>(E.subpix), dptr<int>(E.nsamp), dptr<uint8_t>(E.keep));
     }
-    /* primary rays of this pass: one per active pixel */
-    unsigned long long srays[MAX_SLOTS];
-    if (!E.uni) HIPCHK(c, hipMemcpyAsync(srays, E.slotrays.p, (size_t)E.hi * 8, hipMemcpyDeviceToHost, c->stream));
-    uint32_t nn = 0;
-    if ((rc = select_flagged(c, A, dptr<uint8_t>(E.keep), dptr<uint32_t>(E.act[E.cur ^ 1]), E.nact, &nn))) return rc;
-    E.cur ^= 1;
-    E.nact = nn;
-    uint32_t cnt[MAX_SLOTS];
-    if (E.uni) {
-        cnt[0] = nn;
+    /* The pass status comes back with one copy into pinned memory and one
+     * synchronisation: the next active list's length (when some request can
+     * take another sample), the per-request counts, the error flag and the
+     * ray counters.  When no request can take another sample every pixel
+     * stops here, so the images are finalised before that synchronisation. */
+    Engine::PassStatus *H = E.hst;
+    unsigned long long *dstats = dptr<unsigned long long>(E.stats);
+    const size_t st_off = offsetof(Engine::PassStatus, stats);
+    if (E.more) {
+        if ((rc = select_flagged_async(c, A, dptr<uint8_t>(E.keep), dptr<uint32_t>(E.act[E.cur ^ 1]), E.nact,
+                                       dptr<uint32_t>(c->nsel))))
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(&H->nsel, c->nsel.p, 4, hipMemcpyDeviceToHost, c->stream));
+        if (!E.uni) {
+            /* per-request active counts of the new list, bounded by the old length */
+            k_fill_u32<<<blocks(E.hi, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.slotcnt), E.hi, 0u);
+            k_slot_count_dev<<<blocks(E.nact, 256), 256, 0, c->stream>>>(
+                dptr<uint32_t>(E.act[E.cur ^ 1]), dptr<uint32_t>(c->nsel), dptr<uint32_t>(E.preq),
+                dptr<uint32_t>(E.slotcnt));
+            HIPCHK(c, hipMemcpyAsync(H->slotcnt, E.slotcnt.p, (size_t)E.hi * 4, hipMemcpyDeviceToHost, c->stream));
+        }
     } else {
-        k_fill_u32<<<blocks(E.hi, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.slotcnt), E.hi, 0u);
-        if (nn)
-            k_slot_count<<<blocks(nn, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.act[E.cur]), nn,
-                                                                 dptr<uint32_t>(E.preq), dptr<uint32_t>(E.slotcnt));
-        HIPCHK(c, hipMemcpyAsync(cnt, E.slotcnt.p, (size_t)E.hi * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (uint32_t s = 0; s < E.hi; s++)
+            if (E.slot_job[s] && (rc = finalize_job(E, s))) return rc;
     }
-    uint32_t errf = 0;
-    HIPCHK(c, hipMemcpy(&errf, E.errf.p, 4, hipMemcpyDeviceToHost));
-    unsigned long long hs[8];
-    HIPCHK(c, hipMemcpy(hs, E.stats.p, sizeof hs, hipMemcpyDeviceToHost));
-    unsigned long long sec = hs[1] + hs[2] + hs[3] + hs[4];
-    if (E.uni) srays[0] = sec - E.sec_seen;
+    if (!E.uni) HIPCHK(c, hipMemcpyAsync(H->srays, E.slotrays.p, (size_t)E.hi * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&H->errf, E.errf.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync((char *)H + st_off, dstats, sizeof H->stats, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(H->rcounts, E.rcounts.p, sizeof H->rcounts, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const uint32_t nn = E.more ? H->nsel : 0u;
+    if (E.more) E.cur ^= 1;
+    E.nact = nn;
+    const unsigned long long sec = H->stats[1] + H->stats[2] + H->stats[3] + H->stats[4];
+    if (E.uni) H->srays[0] = sec - E.sec_seen;
     E.sec_seen = sec;
+    bool host_copies = false;
     for (uint32_t s = 0; s < E.hi; s++) {
         Job *j = E.slot_job[s];
         if (!j) continue;
+        const uint32_t cnt = !E.more ? 0u : E.uni ? nn : H->slotcnt[s];
         j->prim += E.slot_active[s];
-        j->rays += E.slot_active[s] + srays[s];
-        if (errf) j->limit_hit = true;
-        E.slot_active[s] = cnt[s];
-        if (cnt[s]) continue;
+        j->rays += E.slot_active[s] + H->srays[s];
+        if (H->errf) j->limit_hit = true;
+        E.slot_active[s] = cnt;
+        if (cnt) continue;
         /* all pixels stopped: the image (src/render.cc:238-241) */
-        const ReqDev &Q = E.hreqs[s];
-        uint32_t npix = (uint32_t)j->npix;
-        float *dst = j->dev_out;
-        if (!dst) {
-            if ((rc = ensure(c, E.stage, (size_t)npix * 16))) return rc;
-            dst = dptr<float>(E.stage);
-        }
-        k_finalize<<<blocks(npix, 256), 256, 0, c->stream>>>(Q.P, dptr<float4>(E.fb) + j->pix_base,
-                                                            dptr<int>(E.nsamp) + j->pix_base, dst, npix);
-        if (Q.P.flags & SRAY_RENDER_REF_Q1)
-            k_q1_scale<<<blocks(npix, 256), 256, 0, c->stream>>>(Q.P, dptr<int>(E.nsamp) + j->pix_base, dst, npix);
-        HIPCHK(c, hipGetLastError());
-        if (j->host_out) {
-            HIPCHK(c, hipMemcpyAsync(j->host_out, dst, (size_t)npix * 16, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-        }
+        if (E.more && (rc = finalize_job(E, s))) return rc;
+        host_copies = host_copies || j->host_out != nullptr;
         retire(E, s);
         finished.push_back(j);
     }
-    /* device outputs are complete when the request is reported done */
-    if (!finished.empty()) HIPCHK(c, hipStreamSynchronize(c->stream));
+    /* images complete when the request is reported done (a plain render
+     * synchronises on its own end event) */
+    if (!finished.empty() && (sync_done || host_copies)) HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (host_copies)
+        for (Job *j : finished) release(j->stage);
     return SRAY_OK;
 }
 
@@ -1495,8 +1570,11 @@ void engine_free(Engine *E)
     E->graphs.clear();
     DevBuf *bs[] = {&E->fb, &E->nsamp, &E->preq, &E->keep, &E->act[0], &E->act[1], &E->subpix, &E->hits,
                     &E->direct, &E->sq, &E->occ, &E->kids, &E->kflag, &E->acc, &E->stats, &E->errf, &E->rcounts,
-                    &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->stage, &E->greq, &E->lvcnt, &E->cmpb};
+                    &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->greq, &E->lvcnt, &E->cmpb};
     for (DevBuf *b : bs) release(*b);
+    if (E->hst) (void)hipHostFree(E->hst);
+    if (E->hreq_pin) (void)hipHostFree(E->hreq_pin);
+    if (E->hscn_pin) (void)hipHostFree(E->hscn_pin);
     for (DevBuf &b : E->qlev) release(b);
     delete E;
 }
@@ -1526,7 +1604,7 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
     std::vector<Job *> fin;
     while (j.state != 2 && rc == SRAY_OK) {
         samples += E.nact;
-        rc = run_pass(E, fin);
+        rc = run_pass(E, fin, false);
         for (Job *f : fin) f->state = 2;
         fin.clear();
     }
@@ -1539,9 +1617,8 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventSynchronize(c->ev1));
     prof_resolve(c);
-    unsigned long long hs[8], rcn[12];
-    HIPCHK(c, hipMemcpy(hs, E.stats.p, sizeof hs, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(rcn, E.rcounts.p, sizeof rcn, hipMemcpyDeviceToHost));
+    /* counters as the last pass read them back (nothing after it adds to them) */
+    const unsigned long long *hs = E.hst->stats, *rcn = E.hst->rcounts;
     if (j.limit_hit) return fail(c, SRAY_ERR_LIMIT, "ior stack deeper than %d media", IOR_DEPTH);
     if (st) {
         memset(st, 0, sizeof *st);
@@ -1606,7 +1683,7 @@ void serve_loop(sray_gpu_ctx *c, Engine *E)
             if (E->live == MAX_SLOTS) break;
         }
         if (E->live == 0) continue;
-        int rc = run_pass(*E, fin);
+        int rc = run_pass(*E, fin, true);
         if (c->profiling) prof_resolve(c);
         if (rc) {
             std::string msg = c->err;

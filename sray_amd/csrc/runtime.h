@@ -59,6 +59,10 @@ struct GpuScene {
     uint32_t pmap[2] = {0, 0}; /* bound caustic / global photon maps */
     sray::PhotonScene photon;  /* what the projection maps need */
     sray_camera_desc camera{}; /* the scene's own camera */
+    /* which children its Phong materials can spawn (bounds the growth of
+     * the bounce levels): any Phong material, one with reflect > 0, one
+     * with refract > 0 */
+    bool has_phong = false, pos_reflect = false, pos_refract = false;
 };
 
 struct Engine; /* multi-request wavefront renderer (engine.hip) */
@@ -168,6 +172,19 @@ int select_flagged(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, u
     HIPCHK(c, hipcub::DeviceSelect::Flagged(c->cubtmp.p, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
     HIPCHK(c, hipMemcpyAsync(count, c->nsel.p, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SRAY_OK;
+}
+
+/* as select_flagged, with the count left on the device (no synchronisation) */
+template <typename T>
+int select_flagged_async(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, uint32_t n, uint32_t *d_count)
+{
+    size_t tmp = 0;
+    HIPCHK(c, hipcub::DeviceSelect::Flagged(nullptr, tmp, in, flags, out, d_count, (int)n, c->stream));
+    int rc = ensure(c, c->cubtmp, tmp);
+    if (rc) return rc;
+    tmp = c->cubtmp.bytes;
+    HIPCHK(c, hipcub::DeviceSelect::Flagged(c->cubtmp.p, tmp, in, flags, out, d_count, (int)n, c->stream));
     return SRAY_OK;
 }
 

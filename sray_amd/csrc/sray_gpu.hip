@@ -611,6 +611,11 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
             md.ior = 1.5f;
             md.flags = SRAY_MAT_HAS_SPECULAR | SRAY_MAT_HAS_REFLECT | SRAY_MAT_HAS_REFRACT;
         }
+        if (md.flags & SRAY_MAT_PHONG) {
+            g->has_phong = true;
+            if ((md.flags & SRAY_MAT_HAS_REFLECT) && md.reflect > 0.0f) g->pos_reflect = true;
+            if ((md.flags & SRAY_MAT_HAS_REFRACT) && md.refract > 0.0f) g->pos_refract = true;
+        }
         DevMat &d = mats[i];
         memset(&d, 0, sizeof d);
         d.kd = make_float4(md.diffuse[0], md.diffuse[1], md.diffuse[2], md.diffuse[3]);
