@@ -126,17 +126,39 @@ template <int D> struct IorStackT {
     float s[D];
     uint32_t n;
     bool overflow;
+    /* entries are read and written with constant indices only (selects over
+     * the D slots), so the stack stays in registers */
+    __device__ float at(uint32_t k) const
+    {
+        float v = s[0];
+#pragma unroll
+        for (int j = 1; j < D; j++) v = k == (uint32_t)j ? s[j] : v;
+        return v;
+    }
     __device__ void enter(float x)
     {
-        if (n < (uint32_t)D) s[n++] = cur;
-        else overflow = true;
+        if (n < (uint32_t)D) {
+#pragma unroll
+            for (int j = 0; j < D; j++) s[j] = n == (uint32_t)j ? cur : s[j];
+            n++;
+        } else {
+            overflow = true;
+        }
         cur = x;
     }
-    __device__ void leave() { cur = n > 0 ? s[--n] : 1.0f; }
+    __device__ void leave()
+    {
+        if (n > 0) {
+            n--;
+            cur = at(n);
+        } else {
+            cur = 1.0f;
+        }
+    }
     __device__ float calc(bool entering, float mat_ior) const
     {
         if (entering) return cur / mat_ior;
-        float to = n > 0 ? s[n - 1] : 1.0f;
+        float to = n > 0 ? at(n - 1) : 1.0f;
         return cur / to;
     }
 };
