@@ -181,15 +181,16 @@ def run_spheres(args):
 
 
 def batched_leg(ctx, jobs, warm=True):
-    """all jobs ((scene, params, camera or None)) in flight together through
-    submit/poll, each writing its image to a device buffer of its own (as
-    the plain legs do); one untimed round first grows the engine's arenas
-    to the batch.  Returns (Mrays/s, wall s, per-request latencies ms)."""
+    """all jobs ((scene, params, camera or None)) in flight together: queued
+    with one submit_many call and polled, each writing its image to a
+    device buffer of its own (as the plain legs do); one untimed round
+    first grows the engine's arenas to the batch.  Returns (Mrays/s, wall
+    s, per-request latencies ms)."""
     outs = [ctx.malloc((p.x1 - p.x0) * (p.y1 - p.y0) * 16) for _, p, _ in jobs]
     rounds = [False, True] if warm else [True]
     for timed in rounds:
         t0 = time.perf_counter()
-        tickets = [ctx.submit(sid, p, cam, out=o) for (sid, p, cam), o in zip(jobs, outs)]
+        tickets = ctx.submit_many([(sid, p, cam, o) for (sid, p, cam), o in zip(jobs, outs)])
         res = [ctx.poll(t)[1] for t in tickets]
         wall = time.perf_counter() - t0
     for o in outs:

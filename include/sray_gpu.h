@@ -395,6 +395,11 @@ int  sray_gpu_autotune_octree(sray_gpu_ctx *ctx, const sray_scene_desc *s, const
  * never fit the cap), after which the ticket is forgotten.  A request's
  * image is the same alone and in any mix. */
 int  sray_gpu_submit(sray_gpu_ctx *ctx, const sray_request *r, float *rgba, uint64_t *ticket);
+/* n requests queued together (one ticket each, in order): all are checked
+ * before any is queued, so either all are queued or none (no reference
+ * counterpart, as submit) */
+int  sray_gpu_submit_many(sray_gpu_ctx *ctx, const sray_request *r, uint32_t n, float *const *rgba,
+                          uint64_t *tickets);
 int  sray_gpu_poll(sray_gpu_ctx *ctx, uint64_t ticket, int wait, sray_result *out);
 int  sray_gpu_set_admission_cap(sray_gpu_ctx *ctx, uint64_t bytes);
 

@@ -311,6 +311,35 @@ class Context:
         self._outs[t.value] = host
         return t.value
 
+    def submit_many(self, jobs):
+        """queue (scene, params, camera or None, device output or None)
+        requests together; returns their tickets in order"""
+        n = len(jobs)
+        reqs = (abi.Request * n)()
+        outs = (C.POINTER(C.c_float) * n)()
+        hosts = {}
+        for k, (sid, params, camera, out) in enumerate(jobs):
+            r = reqs[k]
+            r.scene_id = sid
+            r.params = params
+            if camera is not None:
+                r.camera = camera
+                r.use_camera = 1
+            if out is not None:
+                r.flags = abi.SRAY_REQUEST_DEVICE_OUT
+                outs[k] = C.cast(C.c_void_p(out), C.POINTER(C.c_float))
+            else:
+                w, h = params.x1 - params.x0, params.y1 - params.y0
+                hosts[k] = np.empty((h, w, 4), np.float32)
+                outs[k] = hosts[k].ctypes.data_as(C.POINTER(C.c_float))
+        t = (C.c_uint64 * n)()
+        self._check(lib().sray_gpu_submit_many(self._h, reqs, n, outs, t))
+        if not hasattr(self, "_outs"):
+            self._outs = {}
+        for k, arr in hosts.items():
+            self._outs[t[k]] = arr
+        return [t[k] for k in range(n)]
+
     def poll(self, ticket, wait=True):
         """(image, sray_result) when done; None if not done and wait is False"""
         res = abi.Result()

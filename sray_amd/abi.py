@@ -146,7 +146,7 @@ EXPORTS = [
     "sray_gpu_create", "sray_gpu_destroy", "sray_gpu_last_error", "sray_gpu_scene_upload",
     "sray_gpu_scene_release", "sray_gpu_scene_bytes", "sray_gpu_trace", "sray_gpu_trace_device",
     "sray_gpu_render", "sray_gpu_render_device", "sray_gpu_pmap_upload", "sray_gpu_pmap_release",
-    "sray_gpu_pmap_gather", "sray_gpu_submit", "sray_gpu_poll", "sray_gpu_set_admission_cap",
+    "sray_gpu_pmap_gather", "sray_gpu_submit", "sray_gpu_submit_many", "sray_gpu_poll", "sray_gpu_set_admission_cap",
     "sray_scene_load_xml", "sray_scene_desc_of", "sray_scene_free", "sray_host_last_error",
     "sray_octree_build_host", "sray_scene_generate", "sray_scene_write_xml", "sray_abi_sizes",
     "sray_gpu_primary_rays_device", "sray_gpu_set_profiling", "sray_gpu_kernel_stats",
@@ -176,6 +176,7 @@ def declare(lib):
         "sray_gpu_pmap_release": (i32, [vp, u32]),
         "sray_gpu_pmap_gather": (i32, [vp, u32, P(GatherQuery), u32, f32, C.c_int, C.c_int, P(f32)]),
         "sray_gpu_submit": (i32, [vp, P(Request), P(f32), P(u64)]),
+        "sray_gpu_submit_many": (i32, [vp, P(Request), u32, P(P(f32)), P(u64)]),
         "sray_gpu_poll": (i32, [vp, u64, C.c_int, P(Result)]),
         "sray_gpu_set_admission_cap": (i32, [vp, u64]),
         "sray_scene_load_xml": (i32, [C.c_char_p, C.c_char_p, P(vp)]),

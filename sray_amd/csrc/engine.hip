@@ -1700,24 +1700,34 @@ void serve_loop(sray_gpu_ctx *c, Engine *E)
     }
 }
 
-int do_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *ticket)
+/* queues n requests at once: all are validated first, then all enter the
+ * queue together (the worker admits them at one pass boundary when they fit) */
+int do_submit_many(sray_gpu_ctx *c, const sray_request *r, uint32_t n, float *const *rgba, uint64_t *tickets)
 {
-    std::unique_ptr<Job> j(new Job);
-    j->scene_id = r->scene_id;
-    j->p = r->params;
-    j->cam = r->camera;
-    j->use_cam = r->use_camera != 0;
-    if (r->flags & SRAY_REQUEST_DEVICE_OUT)
-        j->dev_out = rgba;
-    else
-        j->host_out = rgba;
+    std::vector<std::unique_ptr<Job>> jobs(n);
     int rc;
     {
         ApiLock g(c);
-        if (r->flags & ~SRAY_REQUEST_DEVICE_OUT) return fail(c, SRAY_ERR_ARG, "unknown request flags 0x%x", r->flags);
-        if ((rc = validate_params(c, &r->params))) return rc;
-        if (!find_scene(c, r->scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", r->scene_id);
-        if (j->use_cam && !(r->camera.vfov > 0.0)) return fail(c, SRAY_ERR_ARG, "request camera: vfov must be > 0");
+        for (uint32_t k = 0; k < n; k++) {
+            const sray_request &q = r[k];
+            if (!rgba[k]) return fail(c, SRAY_ERR_ARG, "request %u: output is NULL", k);
+            if (q.flags & ~SRAY_REQUEST_DEVICE_OUT)
+                return fail(c, SRAY_ERR_ARG, "request %u: unknown request flags 0x%x", k, q.flags);
+            if ((rc = validate_params(c, &q.params))) return rc;
+            if (!find_scene(c, q.scene_id)) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", q.scene_id);
+            if (q.use_camera && !(q.camera.vfov > 0.0))
+                return fail(c, SRAY_ERR_ARG, "request %u camera: vfov must be > 0", k);
+            std::unique_ptr<Job> j(new Job);
+            j->scene_id = q.scene_id;
+            j->p = q.params;
+            j->cam = q.camera;
+            j->use_cam = q.use_camera != 0;
+            if (q.flags & SRAY_REQUEST_DEVICE_OUT)
+                j->dev_out = rgba[k];
+            else
+                j->host_out = rgba[k];
+            jobs[k] = std::move(j);
+        }
         Engine *E = engine_get(c, c->serve);
         if (!E->started) {
             E->started = true;
@@ -1729,13 +1739,22 @@ int do_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *tic
     }
     Engine *E = c->serve;
     std::lock_guard<std::mutex> lk(E->qmu);
-    j->ticket = E->next_ticket++;
-    j->t_submit = Clock::now();
-    *ticket = j->ticket;
-    E->pending.push_back(j.get());
-    E->jobs[j->ticket] = std::move(j);
+    const Clock::time_point now = Clock::now();
+    for (uint32_t k = 0; k < n; k++) {
+        Job *j = jobs[k].get();
+        j->ticket = E->next_ticket++;
+        j->t_submit = now;
+        tickets[k] = j->ticket;
+        E->pending.push_back(j);
+        E->jobs[j->ticket] = std::move(jobs[k]);
+    }
     E->qcv.notify_all();
     return SRAY_OK;
+}
+
+int do_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_t *ticket)
+{
+    return do_submit_many(c, r, 1, &rgba, ticket);
 }
 
 int do_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)
@@ -1956,6 +1975,13 @@ int sray_gpu_submit(sray_gpu_ctx *c, const sray_request *r, float *rgba, uint64_
 {
     if (!c || !r || !rgba || !ticket) return SRAY_ERR_ARG;
     return do_submit(c, r, rgba, ticket);
+}
+
+int sray_gpu_submit_many(sray_gpu_ctx *c, const sray_request *r, uint32_t n, float *const *rgba, uint64_t *tickets)
+{
+    if (!c || (n && (!r || !rgba || !tickets))) return SRAY_ERR_ARG;
+    if (n == 0) return SRAY_OK;
+    return do_submit_many(c, r, n, rgba, tickets);
 }
 
 int sray_gpu_poll(sray_gpu_ctx *c, uint64_t ticket, int wait, sray_result *out)

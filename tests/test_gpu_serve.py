@@ -194,3 +194,21 @@ def test_device_outputs_one_scene(ctx):
     assert S.lib().sray_gpu_submit(ctx._h, C.byref(r), buf.ctypes.data_as(C.POINTER(C.c_float)),
                                    C.byref(t)) == abi.SRAY_ERR_ARG
     ctx.release(sid)
+
+
+def test_submit_many_equals_alone(ctx):
+    """requests queued together (one call) match their images alone; a bad
+    request anywhere in the list queues none of them"""
+    reqs, alone_sid, keep, sids = _requests(ctx)
+    alone = _alone(ctx, reqs, alone_sid)
+    tickets = ctx.submit_many([(sid, p, cam, None) for sid, p, cam in reqs])
+    assert len(tickets) == len(reqs) and tickets == sorted(tickets)
+    for k, t in enumerate(tickets):
+        img, res = ctx.poll(t)
+        assert float(np.abs(img - alone[k][0]).max()) <= TOL, k
+        assert res.rays == alone[k][1], k
+    bad = [(sids[0], reqs[0][1], None, None), (999, reqs[0][1], None, None)]
+    with pytest.raises(S.SrayError):
+        ctx.submit_many(bad)
+    for s in sids:
+        ctx.release(s)
