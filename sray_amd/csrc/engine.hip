@@ -118,14 +118,15 @@ __device__ __forceinline__ uint32_t first_run() { return (blockIdx.x * blockDim.
 __device__ __forceinline__ uint32_t run_stride() { return gridDim.x * blockDim.x; }
 
 /* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
-template <bool COUNT, bool UNI, int BS>
+template <bool COUNT, bool UNI, int BS, bool PERSIST>
 __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
                                                 int4 *__restrict__ hits, unsigned long long *counts)
 {
     __shared__ uint32_t stk[STACK * BS];
     const uint32_t n = step_n(sn);
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
-    for (uint32_t base = first_run(); base < n; base += run_stride()) {
+    /* without PERSIST the grid has one run per wave: no loop state is kept */
+    for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
         const uint32_t i = base + __lane_id();
         if (i < n) {
             float4 o = q[i].o, d = q[i].d;
@@ -140,22 +141,23 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
 
 /* occlusion of every shadow ray (any hit: cast_ray(ray) != NULL,
  * src/shader.cc:266), L slots per ray of the step */
-template <bool COUNT, bool UNI, int BS>
-__global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq, StepN sn, int L,
+template <bool COUNT, bool UNI, int BS, bool PERSIST>
+__global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq,
+                                                     const float4 *__restrict__ scon, StepN sn, int L,
                                                      uint8_t *__restrict__ occ, unsigned long long *counts)
 {
     __shared__ uint32_t stk[STACK * BS];
     const uint32_t n = step_n(sn) * (uint32_t)L;
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
-    for (uint32_t base = first_run(); base < n; base += run_stride()) {
+    for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
         const uint32_t i = base + __lane_id();
         if (i < n) {
             float4 d = sq[i].d;
-            if (d.w == 0.0f) {
-                occ[i] = 0;
+            if (((const float *)scon)[(size_t)i * 4 + 3] == 0.0f) {
+                occ[i] = 0; /* unused slot */
             } else {
                 float4 o = sq[i].o;
-                const SceneDev S = T.scenes[UNI ? 0u : T.reqs[__float_as_uint(sq[i].c.w)].scene];
+                const SceneDev S = T.scenes[UNI ? 0u : T.reqs[__float_as_uint(d.w)].scene];
                 HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w),
                                                  &stk[threadIdx.x], BS, cnt);
                 occ[i] = h.obj >= 0 ? 1 : 0;
@@ -205,10 +207,11 @@ __device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uin
 
 /* shade_phong (src/shader.cc:83-247) / shade_undef (:36-71) for one hit,
  * emitting direct terms, shadow rays, children and photon-gather requests */
-template <bool PHOTONS, bool UNI>
+template <bool PHOTONS, bool UNI, bool SCN1>
 __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restrict__ active, uint32_t c0,
                                                const QRay *__restrict__ q, const int4 *__restrict__ hits, StepN sn,
                                                int L, int K, float4 *__restrict__ direct, SRay *__restrict__ sq,
+                                               float4 *__restrict__ scon, uint32_t *__restrict__ rslot,
                                                QRay *__restrict__ kids, uint8_t *__restrict__ kflag,
                                                GatherReq *__restrict__ greq, unsigned long long *__restrict__ stats,
                                                unsigned long long *__restrict__ slot_rays, uint32_t *__restrict__ err)
@@ -229,7 +232,8 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
         uint32_t rq = UNI ? 0u : R.m.w >> SLOT_SHIFT;
         const ReqDev &Q = T.reqs[rq];
         const RenderDev &P = Q.P;
-        const SceneDev S = T.scenes[UNI ? 0u : Q.scene];
+        /* SCN1: every request in flight renders scene 0 */
+        const SceneDev S = T.scenes[(UNI || SCN1) ? 0u : Q.scene];
         uint32_t pr = active[c0 + slot];
         uint32_t lp = pr - Q.pix_base;
         int sample = T.nsamp[pr];
@@ -239,9 +243,11 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
         int time = __float_as_int(R.o.w);
         V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
         float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        for (int l = 0; l < L; l++) sq[(size_t)i * L + l].d.w = 0.0f;
+        int used = 0;        /* shadow slots written */
+        uint32_t spawn = 0;  /* reflect (bit 0) / refract (bit 1) child written */
+        int dwritten = 0;    /* diffuse child flags written */
+        rslot[i] = slot;
         if (PHOTONS) greq[i].pos = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-        for (int k = 0; k < K; k++) kflag[(size_t)i * K + k] = 0;
         if (h.y < 0) {
             /* miss: env_color (src/scene.cc:568) */
             dir.x = W.x * S.env_color.x;
@@ -281,13 +287,15 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
                     float sp = powf(ndoth, 50.0f);
                     SRay s;
                     s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
-                    s.d = make_float4(sd.x, sd.y, sd.z, 1.0f);
-                    s.c = make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
-                                      W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
-                                      W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp), __uint_as_float(rq));
+                    s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
                     sq[(size_t)i * L + l] = s;
+                    scon[(size_t)i * L + l] = make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
+                                                          W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
+                                                          W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp),
+                                                          1.0f);
                     c_sh++;
                 }
+                used = nl;
                 direct[i] = dir;
             } else {
                 bool entering;
@@ -353,16 +361,17 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
                             float sp = powf(rdotl, mat.shininess);
                             SRay s;
                             s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
-                            s.d = make_float4(sd.x, sd.y, sd.z, 1.0f);
-                            s.c = make_float4(
+                            s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
+                            sq[(size_t)i * L + slotk] = s;
+                            scon[(size_t)i * L + slotk] = make_float4(
                                 W.x * ((kd.x * (lt.color[0] * ndotl) + ks.x * (lt.color[0] * sp)) * scale),
                                 W.y * ((kd.y * (lt.color[1] * ndotl) + ks.y * (lt.color[1] * sp)) * scale),
-                                W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale), __uint_as_float(rq));
-                            sq[(size_t)i * L + slotk] = s;
+                                W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale), 1.0f);
                             slotk++;
                             c_sh++;
                         }
                     }
+                    used = slotk;
                     if (PHOTONS && P.gi) {
                         /* diffuse hemisphere rays (src/shader.cc:158-185); a
                          * direction flipped into the hemisphere keeps its
@@ -384,10 +393,11 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
                                                   (R.m.w & 0xffu) | (QF_DIFFUSE << 8) | (rq << SLOT_SHIFT));
                                 cr.stk = R.stk;
                                 kids[(size_t)i * K + 2 + di] = cr;
-                                kflag[(size_t)i * K + 2 + di] = 1;
                                 c_df++;
                             }
+                            kflag[(size_t)i * K + 2 + di] = diff_energy > P.min_energy ? 1 : 0;
                         }
+                        dwritten = P.diffuse_samples;
                     }
                     if (PHOTONS && P.caust) {
                         /* caustics: irradiance from the caustic map (src/shader.cc:231-236),
@@ -414,7 +424,7 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
                         c.m = make_uint4(slot, path * 2u, (uint32_t)(iter - 1), R.m.w);
                         c.stk = R.stk;
                         kids[(size_t)i * K + 0] = c;
-                        kflag[(size_t)i * K + 0] = 1;
+                        spawn |= 1u;
                         c_rl++;
                     }
                     float trans_energy = trans_fact * energy;
@@ -436,13 +446,18 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
                         c.m = make_uint4(slot, path * 2u + 1u, (uint32_t)(iter - 1), ts.n | (flg << 8) | (rq << SLOT_SHIFT));
                         c.stk = make_float4(ts.s[0], ts.s[1], ts.s[2], ts.s[3]);
                         kids[(size_t)i * K + 1] = c;
-                        kflag[(size_t)i * K + 1] = 1;
+                        spawn |= 2u;
                         c_rr++;
                     }
                     direct[i] = dir;
                 }
             }
         }
+        /* flags of the child slots and the unused shadow slots, written once */
+        kflag[(size_t)i * K + 0] = (uint8_t)(spawn & 1u);
+        kflag[(size_t)i * K + 1] = (uint8_t)((spawn >> 1) & 1u);
+        for (int k = 2 + dwritten; k < K; k++) kflag[(size_t)i * K + k] = 0;
+        for (int l = used; l < L; l++) scon[(size_t)i * L + l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     if (!UNI)
         wave_keyed_add(slot_rays, i < n ? q[i].m.w >> SLOT_SHIFT : 0u,
@@ -506,14 +521,14 @@ __global__ __launch_bounds__(256) void k_gather(Tabs T, const QRay *__restrict__
 }
 
 /* the sum of the run of equal sample slots starting at i, if i starts one */
-__device__ __forceinline__ void accum_run(const QRay *__restrict__ q, const float4 *__restrict__ direct,
-                                          const SRay *__restrict__ sq, const uint8_t *__restrict__ occ, uint32_t n,
-                                          int L, float4 *__restrict__ acc, uint32_t i)
+__device__ __forceinline__ void accum_run(const uint32_t *__restrict__ rslot, const float4 *__restrict__ direct,
+                                          const float4 *__restrict__ scon, const uint8_t *__restrict__ occ,
+                                          uint32_t n, int L, float4 *__restrict__ acc, uint32_t i)
 {
-    uint32_t slot = q[i].m.x;
-    if (i > 0 && q[i - 1].m.x == slot) return;
+    uint32_t slot = rslot[i];
+    if (i > 0 && rslot[i - 1] == slot) return;
     float4 s = acc[slot];
-    for (uint32_t j = i; j < n && q[j].m.x == slot; j++) {
+    for (uint32_t j = i; j < n && rslot[j] == slot; j++) {
         float4 d = direct[j];
         s.x = s.x + d.x;
         s.y = s.y + d.y;
@@ -521,8 +536,8 @@ __device__ __forceinline__ void accum_run(const QRay *__restrict__ q, const floa
         s.w = s.w + d.w;
         for (int l = 0; l < L; l++) {
             size_t k = (size_t)j * L + l;
-            if (sq[k].d.w != 0.0f && !occ[k]) {
-                float4 c = sq[k].c;
+            float4 c = scon[k];
+            if (c.w != 0.0f && !occ[k]) {
                 s.x = s.x + c.x;
                 s.y = s.y + c.y;
                 s.z = s.z + c.z;
@@ -534,12 +549,13 @@ __device__ __forceinline__ void accum_run(const QRay *__restrict__ q, const floa
 
 /* per-sample accumulation: the first ray of each run of equal sample slots
  * sums the run (queues are ordered by slot) */
-__global__ void k_accum(const QRay *__restrict__ q, const float4 *__restrict__ direct, const SRay *__restrict__ sq,
-                        const uint8_t *__restrict__ occ, StepN sn, int L, float4 *__restrict__ acc)
+__global__ void k_accum(const uint32_t *__restrict__ rslot, const float4 *__restrict__ direct,
+                        const float4 *__restrict__ scon, const uint8_t *__restrict__ occ, StepN sn, int L,
+                        float4 *__restrict__ acc)
 {
     const uint32_t n = step_n(sn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        accum_run(q, direct, sq, occ, n, L, acc, i);
+        accum_run(rslot, direct, scon, occ, n, L, acc, i);
 }
 
 /* ------------------------------------------------------------------ */
@@ -767,14 +783,27 @@ __global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__
 
 
 /* launchable instances of the tunable kernels */
-template <bool COUNT, bool UNI> auto trace_q_of(int bs) -> decltype(&k_trace_q<COUNT, UNI, 128>)
+template <bool COUNT, bool UNI, bool P> auto trace_q_of_p(int bs) -> decltype(&k_trace_q<COUNT, UNI, 128, P>)
 {
-    return bs == 64 ? k_trace_q<COUNT, UNI, 64> : bs == 256 ? k_trace_q<COUNT, UNI, 256> : k_trace_q<COUNT, UNI, 128>;
+    return bs == 64    ? k_trace_q<COUNT, UNI, 64, P>
+           : bs == 256 ? k_trace_q<COUNT, UNI, 256, P>
+                       : k_trace_q<COUNT, UNI, 128, P>;
 }
-template <bool COUNT, bool UNI> auto trace_shadow_of(int bs) -> decltype(&k_trace_shadow<COUNT, UNI, 128>)
+template <bool COUNT, bool UNI> auto trace_q_of(int bs, bool persist) -> decltype(&k_trace_q<COUNT, UNI, 128, false>)
 {
-    return bs == 64 ? k_trace_shadow<COUNT, UNI, 64>
-                    : bs == 256 ? k_trace_shadow<COUNT, UNI, 256> : k_trace_shadow<COUNT, UNI, 128>;
+    return persist ? trace_q_of_p<COUNT, UNI, true>(bs) : trace_q_of_p<COUNT, UNI, false>(bs);
+}
+template <bool COUNT, bool UNI, bool P>
+auto trace_shadow_of_p(int bs) -> decltype(&k_trace_shadow<COUNT, UNI, 128, P>)
+{
+    return bs == 64    ? k_trace_shadow<COUNT, UNI, 64, P>
+           : bs == 256 ? k_trace_shadow<COUNT, UNI, 256, P>
+                       : k_trace_shadow<COUNT, UNI, 128, P>;
+}
+template <bool COUNT, bool UNI>
+auto trace_shadow_of(int bs, bool persist) -> decltype(&k_trace_shadow<COUNT, UNI, 128, false>)
+{
+    return persist ? trace_shadow_of_p<COUNT, UNI, true>(bs) : trace_shadow_of_p<COUNT, UNI, false>(bs);
 }
 template <bool UNI> auto gather_of(int lanes) -> decltype(&k_gather<UNI, 16>)
 {
@@ -910,7 +939,8 @@ struct Engine {
     uint32_t nact = 0;
     /* ray queues and per-level scratch */
     std::vector<DevBuf> qlev;
-    DevBuf hits, direct, sq, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs, dscenes, greq;
+    DevBuf hits, direct, sq, scon, rslot, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs,
+        dscenes, greq;
     /* device-driven levels: per-level ray counts, compaction block counts /
      * offsets */
     DevBuf lvcnt, cmpb;
@@ -966,7 +996,7 @@ struct Engine {
     uint64_t fixed_bytes() const
     {
         uint64_t b = 0;
-        const DevBuf *bs[] = {&hits, &direct, &sq, &occ, &kids, &kflag, &acc, &stats, &errf, &rcounts,
+        const DevBuf *bs[] = {&hits, &direct, &sq, &scon, &rslot, &occ, &kids, &kflag, &acc, &stats, &errf, &rcounts,
                               &slotcnt, &slotrays, &dreqs, &dscenes, &greq, &lvcnt, &cmpb};
         for (const DevBuf *d : bs) b += d->bytes;
         for (const DevBuf &d : qlev) b += d.bytes;
@@ -1204,8 +1234,9 @@ int build_tables(Engine &E)
         }
         ReqDev &Q = E.hreqs[s];
         Q.scene = it->second;
-        /* shadow slots per ray: phong samples area lights shadow_samples times */
-        E.L = std::max(E.L, g->dev.nlights * Q.P.shadow_samples);
+        /* shadow slots per ray: Phong samples area lights shadow_samples
+         * times and point lights once; shade_undef samples each light once */
+        E.L = std::max(E.L, g->dev.nlights * std::max(1, Q.P.shadow_samples));
         E.K = std::max(E.K, 2 + (Q.P.gi ? Q.P.diffuse_samples : 0));
         E.maxiter = std::max(E.maxiter, Q.P.iter);
         E.photons = E.photons || Q.P.gi || Q.P.caust;
@@ -1280,16 +1311,20 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
     {
         ProfScope ps(c, "k_trace_q");
         /* the traversal reads only the scene: one scene = the uniform variant */
-        auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs) : trace_q_of<true, false>(bs))
-                               : (E.one_scene ? trace_q_of<false, true>(bs) : trace_q_of<false, false>(bs));
+        const bool P = E.persistent;
+        auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs, P) : trace_q_of<true, false>(bs, P))
+                               : (E.one_scene ? trace_q_of<false, true>(bs, P) : trace_q_of<false, false>(bs, P));
         kern<<<tgrid(kern, cap), bs, 0, c->stream>>>(LV.T, q, sn, dptr<int4>(E.hits), rc6);
     }
     {
         ProfScope ps(c, "k_shade");
-        auto kern = E.photons ? (E.uni ? k_shade<true, true> : k_shade<true, false>)
-                              : (E.uni ? k_shade<false, true> : k_shade<false, false>);
+        auto kern = E.photons ? (E.uni ? k_shade<true, true, true>
+                                       : E.one_scene ? k_shade<true, false, true> : k_shade<true, false, false>)
+                              : (E.uni ? k_shade<false, true, true>
+                                       : E.one_scene ? k_shade<false, false, true> : k_shade<false, false, false>);
         kern<<<small_grid, 256, 0, c->stream>>>(LV.T, LV.active, LV.c0, q, dptr<int4>(E.hits), sn, E.L, E.K,
-                                               dptr<float4>(E.direct), dptr<SRay>(E.sq), dptr<QRay>(E.kids),
+                                               dptr<float4>(E.direct), dptr<SRay>(E.sq), dptr<float4>(E.scon),
+                                               dptr<uint32_t>(E.rslot), dptr<QRay>(E.kids),
                                                dptr<uint8_t>(E.kflag), dptr<GatherReq>(E.greq), stats, srays,
                                                dptr<uint32_t>(E.errf));
     }
@@ -1301,16 +1336,19 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
     }
     if (E.L) {
         ProfScope ps(c, "k_trace_shadow");
-        auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs) : trace_shadow_of<true, false>(bs))
-                               : (E.one_scene ? trace_shadow_of<false, true>(bs)
-                                              : trace_shadow_of<false, false>(bs));
-        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), sn, E.L,
-                                                                   dptr<uint8_t>(E.occ), rc6 + 6);
+        const bool P = E.persistent;
+        auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs, P)
+                                              : trace_shadow_of<true, false>(bs, P))
+                               : (E.one_scene ? trace_shadow_of<false, true>(bs, P)
+                                              : trace_shadow_of<false, false>(bs, P));
+        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), dptr<float4>(E.scon),
+                                                                   sn, E.L, dptr<uint8_t>(E.occ), rc6 + 6);
     }
     {
         ProfScope ps(c, "k_accum");
-        k_accum<<<small_grid, 256, 0, c->stream>>>(q, dptr<float4>(E.direct), dptr<SRay>(E.sq),
-                                                   dptr<uint8_t>(E.occ), sn, E.L, dptr<float4>(E.acc));
+        k_accum<<<small_grid, 256, 0, c->stream>>>(dptr<uint32_t>(E.rslot), dptr<float4>(E.direct),
+                                                   dptr<float4>(E.scon), dptr<uint8_t>(E.occ), sn, E.L,
+                                                   dptr<float4>(E.acc));
     }
     if (b < E.maxiter) {
         ProfScope ps(c, "k_compact");
@@ -1401,7 +1439,7 @@ int launch_chunk(Level &LV, uint32_t m)
                                  (uint64_t)(uintptr_t)LV.T.scenes, (uint64_t)(uintptr_t)LV.T.reqs,
                                  (uint64_t)(uintptr_t)LV.T.preq, (uint64_t)(uintptr_t)LV.T.nsamp,
                                  (uint64_t)(uintptr_t)LV.active};
-    const DevBuf *bufs[] = {&E.hits, &E.direct, &E.sq, &E.occ, &E.kids, &E.kflag, &E.acc, &E.stats, &E.errf,
+    const DevBuf *bufs[] = {&E.hits, &E.direct, &E.sq, &E.scon, &E.rslot, &E.occ, &E.kids, &E.kflag, &E.acc, &E.stats, &E.errf,
                             &E.rcounts, &E.slotrays, &E.greq, &E.lvcnt, &E.cmpb};
     for (const DevBuf *d : bufs) key.push_back((uint64_t)(uintptr_t)d->p);
     for (int b = 0; b <= E.maxiter; b++) key.push_back((uint64_t)(uintptr_t)E.qlev[b].p);
@@ -1466,6 +1504,8 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
         if ((rc = ensure(c, E.qlev[b], (size_t)E.chunk * E.K * sizeof(QRay)))) return rc;
     if ((rc = ensure(c, E.hits, (size_t)E.chunk * sizeof(int4))) || (rc = ensure(c, E.direct, (size_t)E.chunk * 16)) ||
         (rc = ensure(c, E.sq, (size_t)E.chunk * std::max(E.L, 1) * sizeof(SRay))) ||
+        (rc = ensure(c, E.scon, (size_t)E.chunk * std::max(E.L, 1) * sizeof(float4))) ||
+        (rc = ensure(c, E.rslot, (size_t)E.chunk * sizeof(uint32_t))) ||
         (rc = ensure(c, E.occ, (size_t)E.chunk * std::max(E.L, 1))) ||
         (rc = ensure(c, E.kids, (size_t)E.chunk * E.K * sizeof(QRay))) || (rc = ensure(c, E.kflag, (size_t)E.chunk * E.K)) ||
         (rc = ensure(c, E.acc, (size_t)E.chunk * 16)) || (rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) ||
@@ -1569,7 +1609,7 @@ void engine_free(Engine *E)
     for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
     E->graphs.clear();
     DevBuf *bs[] = {&E->fb, &E->nsamp, &E->preq, &E->keep, &E->act[0], &E->act[1], &E->subpix, &E->hits,
-                    &E->direct, &E->sq, &E->occ, &E->kids, &E->kflag, &E->acc, &E->stats, &E->errf, &E->rcounts,
+                    &E->direct, &E->sq, &E->scon, &E->rslot, &E->occ, &E->kids, &E->kflag, &E->acc, &E->stats, &E->errf, &E->rcounts,
                     &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->greq, &E->lvcnt, &E->cmpb};
     for (DevBuf *b : bs) release(*b);
     if (E->hst) (void)hipHostFree(E->hst);

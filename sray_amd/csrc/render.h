@@ -28,10 +28,12 @@ struct alignas(16) QRay {
     float4 stk;  /* ior stack */
 };
 
+/* a shadow ray; its contribution (rgb if unoccluded, w = 1 for a used slot,
+ * 0 for an unused one) sits in a parallel float4 array, the only part the
+ * accumulation reads */
 struct alignas(16) SRay {
     float4 o;    /* origin, w = time (int bits) */
-    float4 d;    /* direction (unnormalised: segment to the light), w = 1 valid / 0 unused */
-    float4 c;    /* contribution rgb if unoccluded, w = request slot (bits) */
+    float4 d;    /* direction (unnormalised: segment to the light), w = request slot (bits) */
 };
 
 constexpr uint32_t QF_DIFFUSE = 1u;  /* diffuse GI ray (IS_DIFFUSE_RAY, src/shader.cc:80-81) */
