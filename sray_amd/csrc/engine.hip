@@ -835,7 +835,7 @@ void fill_render_dev(const sray_render_params *p, RenderDev &R)
     R.max_var = p->max_var;
     R.min_energy = p->min_energy;
     R.iter = p->iter;
-    R.shadow_samples = std::max(1, p->shadow_samples);
+    R.shadow_samples = p->shadow_samples; /* 0 or less: area lights take no samples, as the reference */
     R.diffuse_samples = std::max(0, p->diffuse_samples);
     R.frame_time = p->frame_time;
     R.mblur = p->mblur;

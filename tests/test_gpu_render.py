@@ -63,6 +63,14 @@ def test_render_room_adaptive_area_light(ctx):
     _check(ctx, S.HostScene.generate(2, variant=10), p, "room/10 160x90 1-8spp")
 
 
+def test_render_zero_shadow_samples(ctx):
+    """shadow_samples 0: area lights take no samples, point lights still one
+    (calc_lighting, src/shader.cc:252); shade_undef samples every light once"""
+    p = S.render_params(120, 70, iter=3, shadow_samples=0, seed=7)
+    _check(ctx, S.HostScene.generate(2, variant=10), p, "room/10 shadow_samples 0")
+    _check(ctx, S.HostScene.generate(0), S.render_params(96, 72, iter=3, shadow_samples=0), "spheres shadow_samples 0")
+
+
 def test_render_q1_as_written(ctx):
     """SRAY_RENDER_REF_Q1 reproduces img[x] *= rcp_lut[i] (src/render.cc:240)"""
     p = S.render_params(200, 100, samples=(2, 4), max_var=0.001, iter=4, flags=S.abi.SRAY_RENDER_REF_Q1)
