@@ -182,11 +182,12 @@ typedef struct sray_trace_counts {
 typedef struct sray_render_params {
     int32_t  width, height;         /* full image size (camera aspect) */
     int32_t  x0, y0, x1, y1;        /* pixel rectangle to render (tile); x1/y1 exclusive */
-    int32_t  min_samples, max_samples;  /* -r min[-max] */
+    int32_t  min_samples, max_samples;  /* -r min[-max]; max_samples below 1 renders 1 sample (the
+                                       reference would divide by zero samples) */
     float    max_var;               /* -d (default 0.005) */
     float    min_energy;            /* -e (default 1e-4) */
     int32_t  iter;                  /* -i (default 7) */
-    int32_t  shadow_samples;        /* -shadowrays (default 1) */
+    int32_t  shadow_samples;        /* -shadowrays (default 1); 0 or less: no area-light samples */
     int32_t  diffuse_samples;       /* -diffuserays (default 1) */
     int32_t  frame_time;            /* msec, render(msec) */
     int32_t  mblur;                 /* -m */
