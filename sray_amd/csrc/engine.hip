@@ -129,10 +129,11 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
     for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
         const uint32_t i = base + __lane_id();
         if (i < n) {
-            float4 o = q[i].o, d = q[i].d;
             const SceneDev S = T.scenes[UNI ? 0u : T.reqs[q[i].m.w >> SLOT_SHIFT].scene];
-            HitRec h = traverse<false, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w),
-                                              &stk[threadIdx.x], BS, cnt);
+            /* the world ray is re-read from the queue when needed instead of
+             * held in registers: 63 VGPRs, 8 waves per SIMD */
+            const RayRecord<QRay> src{q, i};
+            HitRec h = traverse_src<false, COUNT>(S, src, &stk[threadIdx.x], BS, cnt);
             hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
         }
     }
@@ -156,7 +157,9 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
             if (((const float *)scon)[(size_t)i * 4 + 3] == 0.0f) {
                 occ[i] = 0; /* unused slot */
             } else {
-                float4 o = sq[i].o;
+                /* the world ray stays in registers here: this kernel is at full
+                 * occupancy already, and re-reading it would only add loads */
+                const float4 o = sq[i].o;
                 const SceneDev S = T.scenes[UNI ? 0u : T.reqs[__float_as_uint(d.w)].scene];
                 HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w),
                                                  &stk[threadIdx.x], BS, cnt);

@@ -282,10 +282,40 @@ struct HitRec {
  * tested again from its fetched box (the same floats, so it passes) to
  * prune it against the best distance found since the push, and the pseudo
  * entries of the roots get their real test there. */
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int time, uint32_t *stk, int stride,
-                                           TraceCounts &cnt)
+/* where traverse() takes the world ray from: registers, or a queue record
+ * re-read when needed (at the start, entering and leaving a mesh), which
+ * keeps the world ray out of registers during the walk */
+struct RayRegs {
+    V3 o, d;
+    int time;
+    __device__ __forceinline__ void get(V3 &wo, V3 &wd, int &t) const
+    {
+        wo = o;
+        wd = d;
+        t = time;
+    }
+};
+template <class R> struct RayRecord {
+    const R *base;
+    uint32_t i;
+    __device__ __forceinline__ void get(V3 &wo, V3 &wd, int &t) const
+    {
+        uint32_t k = i;
+        asm volatile("" : "+v"(k)); /* re-read from memory each time, not kept */
+        float4 a = base[k].o, b = base[k].d;
+        wo = v3(a.x, a.y, a.z);
+        wd = v3(b.x, b.y, b.z);
+        t = __float_as_int(a.w);
+    }
+};
+
+template <bool ANY, bool COUNT, class SRC>
+__device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src, uint32_t *stk, int stride,
+                                               TraceCounts &cnt)
 {
+    V3 wo, wd;
+    int time;
+    src.get(wo, wd, time);
     TRay r = make_tray(wo, wd); /* world ray; the mesh's object ray inside a mesh */
     HitRec best;
     best.t = FLT_MAX;
@@ -326,7 +356,12 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                         best.rank = rank;
                     }
                 }
-                r = make_tray(wo, wd); /* back to the world ray (same floats) */
+                {
+                    V3 o2, d2;
+                    int t2;
+                    src.get(o2, d2, t2);
+                    r = make_tray(o2, d2); /* back to the world ray (same floats) */
+                }
                 level = 0;
                 continue;
             }
@@ -415,8 +450,13 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
             }
             stk[(sp++) * stride] = E_MARK | rank;
             float I[12];
-            obj_inv_at(ob, time, I);
-            r = make_tray(xf_point(I, wo), xf_vec(I, wd));
+            {
+                V3 o2, d2;
+                int t2;
+                src.get(o2, d2, t2);
+                obj_inv_at(ob, t2, I);
+                r = make_tray(xf_point(I, o2), xf_vec(I, d2));
+            }
             stk[(sp++) * stride] = (ob.root << 8) | (1u << r.dmask);
             item_cur = item_end = 0;
             level = 1;
@@ -426,8 +466,15 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
             continue;
         }
         float I[12];
-        obj_inv_at(ob, time, I);
-        V3 lo = xf_point(I, wo), ld = xf_vec(I, wd);
+        V3 lo, ld;
+        {
+            V3 o2, d2;
+            int t2;
+            src.get(o2, d2, t2);
+            obj_inv_at(ob, t2, I);
+            lo = xf_point(I, o2);
+            ld = xf_vec(I, d2);
+        }
         float t;
         bool h;
         if (type == SRAY_OBJ_SPHERE) {
@@ -453,6 +500,14 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
         continue;
     }
     return best;
+}
+
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int time, uint32_t *stk, int stride,
+                                           TraceCounts &cnt)
+{
+    RayRegs src{wo, wd, time};
+    return traverse_src<ANY, COUNT>(S, src, stk, stride, cnt);
 }
 
 } // namespace srd
