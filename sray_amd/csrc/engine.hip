@@ -226,245 +226,245 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
     uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
     /* grid-stride over the step, block-uniform bases (wave operations below) */
     for (uint32_t gbase = blockIdx.x * blockDim.x; gbase < n; gbase += gridDim.x * blockDim.x) {
-    const uint32_t i = gbase + threadIdx.x;
-    const uint32_t before = c_sh + c_rl + c_rr + c_df;
-    if (i < n) {
-        QRay R = q[i];
-        int4 h = hits[i];
-        uint32_t slot = R.m.x, path = R.m.y, flg = (R.m.w >> 8) & 0xffu;
-        uint32_t rq = UNI ? 0u : R.m.w >> SLOT_SHIFT;
-        const ReqDev &Q = T.reqs[rq];
-        const RenderDev &P = Q.P;
-        /* SCN1: every request in flight renders scene 0 */
-        const SceneDev S = T.scenes[(UNI || SCN1) ? 0u : Q.scene];
-        uint32_t pr = active[c0 + slot];
-        uint32_t lp = pr - Q.pix_base;
-        int sample = T.nsamp[pr];
-        int iter = (int)R.m.z;
-        V3 W = v3(R.w.x, R.w.y, R.w.z);
-        float energy = R.d.w;
-        int time = __float_as_int(R.o.w);
-        V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
-        float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        int used = 0;        /* shadow slots written */
-        uint32_t spawn = 0;  /* reflect (bit 0) / refract (bit 1) child written */
-        int dwritten = 0;    /* diffuse child flags written */
-        rslot[i] = slot;
-        if (PHOTONS) greq[i].pos = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-        if (h.y < 0) {
-            /* miss: env_color (src/scene.cc:568) */
-            dir.x = W.x * S.env_color.x;
-            dir.y = W.y * S.env_color.y;
-            dir.z = W.z * S.env_color.z;
-            if (path == 1u) dir.w = S.env_color.w;
-            direct[i] = dir;
-        } else {
-            float t = __int_as_float(h.x);
-            const DevObj &ob = S.objs[h.y];
-            const DevMat &mat = S.mats[ob.mat];
-            bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
-            V3 Pw, N;
-            float tu, tv;
-            hit_surface(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong, Pw, N, tu, tv);
-            if (path == 1u) dir.w = 1.0f;
-            uint32_t pix = (uint32_t)(P.y0 + (int)(lp / (uint32_t)P.W)) * (uint32_t)P.width + (uint32_t)(P.x0 + (int)(lp % (uint32_t)P.W));
-            int nl = S.nlights;
-            if (!phong) {
-                /* shade_undef: checker texture from texcoords */
-                int cx = (int)(tu * 20.0f), cy = (int)(tv * 10.0f);
-                V3 tex = ((cx % 2) == (cy % 2)) ? v3(1.0f, 0.3f, 0.2f) : v3(0.2f, 0.3f, 1.0f);
-                dir.x = W.x * (0.1f * tex.x);
-                dir.y = W.y * (0.1f * tex.y);
-                dir.z = W.z * (0.1f * tex.z);
-                for (int l = 0; l < nl; l++) {
-                    const DevLight &lt = S.lights[l];
-                    V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u);
-                    V3 ldir = sub(lpos, Pw);
-                    float ldist = length(ldir);
-                    V3 sd = ldir;
-                    ldir = dvs(ldir, ldist);
-                    V3 vdir = neg(normalized(rd));
-                    V3 hdir = normalized(add(ldir, vdir));
-                    float ndotl = fmaxf(dot(N, ldir), 0.0f);
-                    float ndoth = fmaxf(dot(N, hdir), 0.0f);
-                    float sp = powf(ndoth, 50.0f);
-                    SRay s;
-                    s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
-                    s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
-                    sq[(size_t)i * L + l] = s;
-                    scon[(size_t)i * L + l] = make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
-                                                          W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
-                                                          W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp),
-                                                          1.0f);
-                    c_sh++;
-                }
-                used = nl;
+        const uint32_t i = gbase + threadIdx.x;
+        const uint32_t before = c_sh + c_rl + c_rr + c_df;
+        if (i < n) {
+            QRay R = q[i];
+            int4 h = hits[i];
+            uint32_t slot = R.m.x, path = R.m.y, flg = (R.m.w >> 8) & 0xffu;
+            uint32_t rq = UNI ? 0u : R.m.w >> SLOT_SHIFT;
+            const ReqDev &Q = T.reqs[rq];
+            const RenderDev &P = Q.P;
+            /* SCN1: every request in flight renders scene 0 */
+            const SceneDev S = T.scenes[(UNI || SCN1) ? 0u : Q.scene];
+            uint32_t pr = active[c0 + slot];
+            uint32_t lp = pr - Q.pix_base;
+            int sample = T.nsamp[pr];
+            int iter = (int)R.m.z;
+            V3 W = v3(R.w.x, R.w.y, R.w.z);
+            float energy = R.d.w;
+            int time = __float_as_int(R.o.w);
+            V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
+            float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            int used = 0;        /* shadow slots written */
+            uint32_t spawn = 0;  /* reflect (bit 0) / refract (bit 1) child written */
+            int dwritten = 0;    /* diffuse child flags written */
+            rslot[i] = slot;
+            if (PHOTONS) greq[i].pos = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+            if (h.y < 0) {
+                /* miss: env_color (src/scene.cc:568) */
+                dir.x = W.x * S.env_color.x;
+                dir.y = W.y * S.env_color.y;
+                dir.z = W.z * S.env_color.z;
+                if (path == 1u) dir.w = S.env_color.w;
                 direct[i] = dir;
             } else {
-                bool entering;
-                V3 normal;
-                if (dot(rd, N) > 0.0f) {
-                    normal = neg(N);
-                    entering = false;
-                } else {
-                    normal = N;
-                    entering = true;
-                }
-                float ray_mag = length(rd);
-                V3 incident = dvs(rd, ray_mag);
-                float4 kd = mat.kd;
-                float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-                if (flg & QF_DIFFUSE) {
-                    /* diffuse GI ray: radiance estimate from the global map
-                     * (src/shader.cc:119-123) */
-                    if (PHOTONS) {
-                        GatherReq g;
-                        g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(2u));
-                        g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
-                        g.w = make_float4(W.x, W.y, W.z, 0.0f);
-                        g.kd = kd;
-                        greq[i] = g;
-                    } else {
-                        dir.x = W.x * (0.0f * kd.x);
-                        dir.y = W.y * (0.0f * kd.y);
-                        dir.z = W.z * (0.0f * kd.z);
-                    }
-                    direct[i] = dir;
-                } else {
-                    float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
-                    float trans_fact = (mat.flags & SRAY_MAT_HAS_REFRACT) ? mat.refract : 0.0f;
-                    float mat_ior = mat.ior;
-                    IorStack st;
-                    st.cur = R.w.w;
-                    st.s[0] = R.stk.x;
-                    st.s[1] = R.stk.y;
-                    st.s[2] = R.stk.z;
-                    st.s[3] = R.stk.w;
-                    st.n = R.m.w & 0xffu;
-                    st.overflow = false;
-                    float ior = st.calc(entering, mat_ior);
-                    /* ambient: (env_ambient * kd) * kd, src/shader.cc:142,246 */
-                    dir.x = W.x * ((S.env_ambient.x * kd.x) * kd.x);
-                    dir.y = W.y * ((S.env_ambient.y * kd.y) * kd.y);
-                    dir.z = W.z * ((S.env_ambient.z * kd.z) * kd.z);
-                    int slotk = 0;
-                    V3 vref = reflect(incident, normal);
+                float t = __int_as_float(h.x);
+                const DevObj &ob = S.objs[h.y];
+                const DevMat &mat = S.mats[ob.mat];
+                bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
+                V3 Pw, N;
+                float tu, tv;
+                hit_surface(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong, Pw, N, tu, tv);
+                if (path == 1u) dir.w = 1.0f;
+                uint32_t pix = (uint32_t)(P.y0 + (int)(lp / (uint32_t)P.W)) * (uint32_t)P.width + (uint32_t)(P.x0 + (int)(lp % (uint32_t)P.W));
+                int nl = S.nlights;
+                if (!phong) {
+                    /* shade_undef: checker texture from texcoords */
+                    int cx = (int)(tu * 20.0f), cy = (int)(tv * 10.0f);
+                    V3 tex = ((cx % 2) == (cy % 2)) ? v3(1.0f, 0.3f, 0.2f) : v3(0.2f, 0.3f, 1.0f);
+                    dir.x = W.x * (0.1f * tex.x);
+                    dir.y = W.y * (0.1f * tex.y);
+                    dir.z = W.z * (0.1f * tex.z);
                     for (int l = 0; l < nl; l++) {
                         const DevLight &lt = S.lights[l];
-                        int ns = lt.type != SRAY_LIGHT_POINT ? P.shadow_samples : 1;
-                        float scale = ns > 1 ? 1.0f / (float)ns : 1.0f;
-                        for (int si = 0; si < ns; si++) {
-                            V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si);
-                            V3 ldir = sub(lpos, Pw);
-                            float ldist = length(ldir);
-                            V3 sd = ldir;
-                            ldir = dvs(ldir, ldist);
-                            float ndotl = fmaxf(dot(normal, ldir), 0.0f);
-                            float rdotl = fmaxf(dot(vref, ldir), 0.0f);
-                            float sp = powf(rdotl, mat.shininess);
-                            SRay s;
-                            s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
-                            s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
-                            sq[(size_t)i * L + slotk] = s;
-                            scon[(size_t)i * L + slotk] = make_float4(
-                                W.x * ((kd.x * (lt.color[0] * ndotl) + ks.x * (lt.color[0] * sp)) * scale),
-                                W.y * ((kd.y * (lt.color[1] * ndotl) + ks.y * (lt.color[1] * sp)) * scale),
-                                W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale), 1.0f);
-                            slotk++;
-                            c_sh++;
-                        }
+                        V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u);
+                        V3 ldir = sub(lpos, Pw);
+                        float ldist = length(ldir);
+                        V3 sd = ldir;
+                        ldir = dvs(ldir, ldist);
+                        V3 vdir = neg(normalized(rd));
+                        V3 hdir = normalized(add(ldir, vdir));
+                        float ndotl = fmaxf(dot(N, ldir), 0.0f);
+                        float ndoth = fmaxf(dot(N, hdir), 0.0f);
+                        float sp = powf(ndoth, 50.0f);
+                        SRay s;
+                        s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
+                        s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
+                        sq[(size_t)i * L + l] = s;
+                        scon[(size_t)i * L + l] = make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
+                                                              W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
+                                                              W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp),
+                                                              1.0f);
+                        c_sh++;
                     }
-                    used = slotk;
-                    if (PHOTONS && P.gi) {
-                        /* diffuse hemisphere rays (src/shader.cc:158-185); a
-                         * direction flipped into the hemisphere keeps its
-                         * negative ndotl and so spawns no ray */
-                        float dsc = P.diffuse_samples > 1 ? 1.0f / (float)P.diffuse_samples : 1.0f;
-                        for (int di = 0; di < P.diffuse_samples; di++) {
-                            V3 gd = sph_rand(P.seed, pix, (uint32_t)sample, path, K_DIFFUSE, (uint32_t)di, 0u, 1.0f);
-                            float ndotl = dot(gd, normal);
-                            if (ndotl < 0.0f) gd = neg(gd);
-                            float diff_energy = ndotl * energy;
-                            if (diff_energy > P.min_energy) {
-                                QRay cr;
-                                V3 dd = mul(gd, ray_mag);
-                                cr.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                                cr.d = make_float4(dd.x, dd.y, dd.z, diff_energy);
-                                float wf = ndotl * dsc;
-                                cr.w = make_float4(W.x * (kd.x * wf), W.y * (kd.y * wf), W.z * (kd.z * wf), R.w.w);
-                                cr.m = make_uint4(slot, path | 0x40000000u | ((uint32_t)di << 20), 0x7fffffffu,
-                                                  (R.m.w & 0xffu) | (QF_DIFFUSE << 8) | (rq << SLOT_SHIFT));
-                                cr.stk = R.stk;
-                                kids[(size_t)i * K + 2 + di] = cr;
-                                c_df++;
-                            }
-                            kflag[(size_t)i * K + 2 + di] = diff_energy > P.min_energy ? 1 : 0;
-                        }
-                        dwritten = P.diffuse_samples;
-                    }
-                    if (PHOTONS && P.caust) {
-                        /* caustics: irradiance from the caustic map (src/shader.cc:231-236),
-                         * added by k_gather */
-                        GatherReq g;
-                        g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(1u));
-                        g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
-                        g.w = make_float4(W.x, W.y, W.z, 0.0f);
-                        g.kd = kd;
-                        greq[i] = g;
-                    }
-                    V3 nrd = dvs(neg(rd), ray_mag);
-                    float ray_dot_n = dot(nrd, normal);
-                    float sf0 = trans_fact > 0.0f ? (ior - 1.0f) / (ior + 1.0f) : 1.0f;
-                    float fres = fresnel(sf0 * sf0, ray_dot_n);
-                    float refl_energy = refl_fact * energy;
-                    if (refl_energy > P.min_energy && iter > 0) {
-                        V3 nd = reflect(rd, normal);
-                        QRay c;
-                        c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                        c.d = make_float4(nd.x, nd.y, nd.z, refl_energy);
-                        float wf = fres * refl_fact;
-                        c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), R.w.w);
-                        c.m = make_uint4(slot, path * 2u, (uint32_t)(iter - 1), R.m.w);
-                        c.stk = R.stk;
-                        kids[(size_t)i * K + 0] = c;
-                        spawn |= 1u;
-                        c_rl++;
-                    }
-                    float trans_energy = trans_fact * energy;
-                    if (trans_energy > P.min_energy && iter > 0) {
-                        IorStack ts = st;
-                        if (entering) ts.enter(mat_ior);
-                        else ts.leave();
-                        V3 td = mul(refract(dvs(rd, ray_mag), normal, ior), ray_mag);
-                        if (dot(td, normal) > 0.0f) {
-                            if (entering) ts.leave();
-                            else ts.enter(mat_ior);
-                        }
-                        if (ts.overflow) atomicOr(err, 1u);
-                        QRay c;
-                        c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                        c.d = make_float4(td.x, td.y, td.z, trans_energy);
-                        float wf = (1.0f - fres) * trans_fact;
-                        c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), ts.cur);
-                        c.m = make_uint4(slot, path * 2u + 1u, (uint32_t)(iter - 1), ts.n | (flg << 8) | (rq << SLOT_SHIFT));
-                        c.stk = make_float4(ts.s[0], ts.s[1], ts.s[2], ts.s[3]);
-                        kids[(size_t)i * K + 1] = c;
-                        spawn |= 2u;
-                        c_rr++;
-                    }
+                    used = nl;
                     direct[i] = dir;
+                } else {
+                    bool entering;
+                    V3 normal;
+                    if (dot(rd, N) > 0.0f) {
+                        normal = neg(N);
+                        entering = false;
+                    } else {
+                        normal = N;
+                        entering = true;
+                    }
+                    float ray_mag = length(rd);
+                    V3 incident = dvs(rd, ray_mag);
+                    float4 kd = mat.kd;
+                    float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+                    if (flg & QF_DIFFUSE) {
+                        /* diffuse GI ray: radiance estimate from the global map
+                         * (src/shader.cc:119-123) */
+                        if (PHOTONS) {
+                            GatherReq g;
+                            g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(2u));
+                            g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
+                            g.w = make_float4(W.x, W.y, W.z, 0.0f);
+                            g.kd = kd;
+                            greq[i] = g;
+                        } else {
+                            dir.x = W.x * (0.0f * kd.x);
+                            dir.y = W.y * (0.0f * kd.y);
+                            dir.z = W.z * (0.0f * kd.z);
+                        }
+                        direct[i] = dir;
+                    } else {
+                        float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
+                        float trans_fact = (mat.flags & SRAY_MAT_HAS_REFRACT) ? mat.refract : 0.0f;
+                        float mat_ior = mat.ior;
+                        IorStack st;
+                        st.cur = R.w.w;
+                        st.s[0] = R.stk.x;
+                        st.s[1] = R.stk.y;
+                        st.s[2] = R.stk.z;
+                        st.s[3] = R.stk.w;
+                        st.n = R.m.w & 0xffu;
+                        st.overflow = false;
+                        float ior = st.calc(entering, mat_ior);
+                        /* ambient: (env_ambient * kd) * kd, src/shader.cc:142,246 */
+                        dir.x = W.x * ((S.env_ambient.x * kd.x) * kd.x);
+                        dir.y = W.y * ((S.env_ambient.y * kd.y) * kd.y);
+                        dir.z = W.z * ((S.env_ambient.z * kd.z) * kd.z);
+                        int slotk = 0;
+                        V3 vref = reflect(incident, normal);
+                        for (int l = 0; l < nl; l++) {
+                            const DevLight &lt = S.lights[l];
+                            int ns = lt.type != SRAY_LIGHT_POINT ? P.shadow_samples : 1;
+                            float scale = ns > 1 ? 1.0f / (float)ns : 1.0f;
+                            for (int si = 0; si < ns; si++) {
+                                V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si);
+                                V3 ldir = sub(lpos, Pw);
+                                float ldist = length(ldir);
+                                V3 sd = ldir;
+                                ldir = dvs(ldir, ldist);
+                                float ndotl = fmaxf(dot(normal, ldir), 0.0f);
+                                float rdotl = fmaxf(dot(vref, ldir), 0.0f);
+                                float sp = powf(rdotl, mat.shininess);
+                                SRay s;
+                                s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
+                                s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
+                                sq[(size_t)i * L + slotk] = s;
+                                scon[(size_t)i * L + slotk] = make_float4(
+                                    W.x * ((kd.x * (lt.color[0] * ndotl) + ks.x * (lt.color[0] * sp)) * scale),
+                                    W.y * ((kd.y * (lt.color[1] * ndotl) + ks.y * (lt.color[1] * sp)) * scale),
+                                    W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale), 1.0f);
+                                slotk++;
+                                c_sh++;
+                            }
+                        }
+                        used = slotk;
+                        if (PHOTONS && P.gi) {
+                            /* diffuse hemisphere rays (src/shader.cc:158-185); a
+                             * direction flipped into the hemisphere keeps its
+                             * negative ndotl and so spawns no ray */
+                            float dsc = P.diffuse_samples > 1 ? 1.0f / (float)P.diffuse_samples : 1.0f;
+                            for (int di = 0; di < P.diffuse_samples; di++) {
+                                V3 gd = sph_rand(P.seed, pix, (uint32_t)sample, path, K_DIFFUSE, (uint32_t)di, 0u, 1.0f);
+                                float ndotl = dot(gd, normal);
+                                if (ndotl < 0.0f) gd = neg(gd);
+                                float diff_energy = ndotl * energy;
+                                if (diff_energy > P.min_energy) {
+                                    QRay cr;
+                                    V3 dd = mul(gd, ray_mag);
+                                    cr.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                                    cr.d = make_float4(dd.x, dd.y, dd.z, diff_energy);
+                                    float wf = ndotl * dsc;
+                                    cr.w = make_float4(W.x * (kd.x * wf), W.y * (kd.y * wf), W.z * (kd.z * wf), R.w.w);
+                                    cr.m = make_uint4(slot, path | 0x40000000u | ((uint32_t)di << 20), 0x7fffffffu,
+                                                      (R.m.w & 0xffu) | (QF_DIFFUSE << 8) | (rq << SLOT_SHIFT));
+                                    cr.stk = R.stk;
+                                    kids[(size_t)i * K + 2 + di] = cr;
+                                    c_df++;
+                                }
+                                kflag[(size_t)i * K + 2 + di] = diff_energy > P.min_energy ? 1 : 0;
+                            }
+                            dwritten = P.diffuse_samples;
+                        }
+                        if (PHOTONS && P.caust) {
+                            /* caustics: irradiance from the caustic map (src/shader.cc:231-236),
+                             * added by k_gather */
+                            GatherReq g;
+                            g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(1u));
+                            g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
+                            g.w = make_float4(W.x, W.y, W.z, 0.0f);
+                            g.kd = kd;
+                            greq[i] = g;
+                        }
+                        V3 nrd = dvs(neg(rd), ray_mag);
+                        float ray_dot_n = dot(nrd, normal);
+                        float sf0 = trans_fact > 0.0f ? (ior - 1.0f) / (ior + 1.0f) : 1.0f;
+                        float fres = fresnel(sf0 * sf0, ray_dot_n);
+                        float refl_energy = refl_fact * energy;
+                        if (refl_energy > P.min_energy && iter > 0) {
+                            V3 nd = reflect(rd, normal);
+                            QRay c;
+                            c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                            c.d = make_float4(nd.x, nd.y, nd.z, refl_energy);
+                            float wf = fres * refl_fact;
+                            c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), R.w.w);
+                            c.m = make_uint4(slot, path * 2u, (uint32_t)(iter - 1), R.m.w);
+                            c.stk = R.stk;
+                            kids[(size_t)i * K + 0] = c;
+                            spawn |= 1u;
+                            c_rl++;
+                        }
+                        float trans_energy = trans_fact * energy;
+                        if (trans_energy > P.min_energy && iter > 0) {
+                            IorStack ts = st;
+                            if (entering) ts.enter(mat_ior);
+                            else ts.leave();
+                            V3 td = mul(refract(dvs(rd, ray_mag), normal, ior), ray_mag);
+                            if (dot(td, normal) > 0.0f) {
+                                if (entering) ts.leave();
+                                else ts.enter(mat_ior);
+                            }
+                            if (ts.overflow) atomicOr(err, 1u);
+                            QRay c;
+                            c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                            c.d = make_float4(td.x, td.y, td.z, trans_energy);
+                            float wf = (1.0f - fres) * trans_fact;
+                            c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), ts.cur);
+                            c.m = make_uint4(slot, path * 2u + 1u, (uint32_t)(iter - 1), ts.n | (flg << 8) | (rq << SLOT_SHIFT));
+                            c.stk = make_float4(ts.s[0], ts.s[1], ts.s[2], ts.s[3]);
+                            kids[(size_t)i * K + 1] = c;
+                            spawn |= 2u;
+                            c_rr++;
+                        }
+                        direct[i] = dir;
+                    }
                 }
             }
+            /* flags of the child slots and the unused shadow slots, written once */
+            kflag[(size_t)i * K + 0] = (uint8_t)(spawn & 1u);
+            kflag[(size_t)i * K + 1] = (uint8_t)((spawn >> 1) & 1u);
+            for (int k = 2 + dwritten; k < K; k++) kflag[(size_t)i * K + k] = 0;
+            for (int l = used; l < L; l++) scon[(size_t)i * L + l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        /* flags of the child slots and the unused shadow slots, written once */
-        kflag[(size_t)i * K + 0] = (uint8_t)(spawn & 1u);
-        kflag[(size_t)i * K + 1] = (uint8_t)((spawn >> 1) & 1u);
-        for (int k = 2 + dwritten; k < K; k++) kflag[(size_t)i * K + k] = 0;
-        for (int l = used; l < L; l++) scon[(size_t)i * L + l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    if (!UNI)
-        wave_keyed_add(slot_rays, i < n ? q[i].m.w >> SLOT_SHIFT : 0u,
-                       (unsigned long long)(c_sh + c_rl + c_rr + c_df - before), i < n);
+        if (!UNI)
+            wave_keyed_add(slot_rays, i < n ? q[i].m.w >> SLOT_SHIFT : 0u,
+                           (unsigned long long)(c_sh + c_rl + c_rr + c_df - before), i < n);
     }
     if (c_sh) atomicAdd(&bcnt[0], c_sh);
     if (c_rl) atomicAdd(&bcnt[1], c_rl);
