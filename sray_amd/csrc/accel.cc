@@ -36,6 +36,12 @@ struct Builder {
     int max_items, max_depth;
     std::vector<BNode> nodes;
     uint32_t height = 0;
+    /* budget: past these the device layout could not hold the tree anyway
+     * (node indices are 23-bit), and a tiny leaf size with a large depth
+     * would otherwise grow without bound where items share points */
+    static constexpr size_t MAX_NODES = size_t(1) << 23, MAX_REFS = size_t(1) << 28;
+    size_t refs = 0;
+    bool overflow = false;
 
     Builder(const std::vector<float> &b, int mi, int md) : box(b), max_items(mi), max_depth(md) {}
 
@@ -57,7 +63,11 @@ struct Builder {
 
     void subdivide(int ni, int lvl)
     {
-        if ((int)nodes[ni].items.size() <= max_items || lvl >= max_depth) return;
+        if (overflow || (int)nodes[ni].items.size() <= max_items || lvl >= max_depth) return;
+        if (nodes.size() + 8 > MAX_NODES) {
+            overflow = true;
+            return;
+        }
         float mn[3], mx[3], c[3];
         for (int k = 0; k < 3; k++) {
             mn[k] = nodes[ni].mn[k];
@@ -68,6 +78,7 @@ struct Builder {
         nodes[ni].split = true;
         std::vector<uint32_t> parent_items;
         parent_items.swap(nodes[ni].items);
+        refs -= parent_items.size();
         for (int r = 0; r < 8; r++) {
             int bits = kRefToBits[r];
             float cmn[3], cmx[3];
@@ -83,6 +94,8 @@ struct Builder {
             std::vector<uint32_t> its;
             for (uint32_t it : parent_items)
                 if (overlap(&box[(size_t)it * 6], cmn, cmx)) its.push_back(it);
+            refs += its.size();
+            if (refs > MAX_REFS) overflow = true;
             nodes[ch].items.swap(its);
             subdivide(ch, lvl + 1);
         }
@@ -104,6 +117,7 @@ struct Builder {
         std::memcpy(nodes[root].mx, mx, sizeof mx);
         nodes[root].items.resize(n);
         for (size_t i = 0; i < n; i++) nodes[root].items[i] = (uint32_t)i;
+        refs = n;
         subdivide(root, 0);
     }
 
@@ -157,6 +171,10 @@ uint32_t build_flat_octree(const std::vector<float> &boxes, int max_items, int m
 {
     Builder b(boxes, max_items, max_depth);
     b.build();
+    if (b.overflow) {
+        out.overflow = true;
+        return 0;
+    }
     uint32_t root = (uint32_t)(out.lo.size() / 4);
     out.lo.resize(out.lo.size() + 4);
     out.hi.resize(out.hi.size() + 4);

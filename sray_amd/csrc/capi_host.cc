@@ -5,6 +5,7 @@
  */
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -83,8 +84,21 @@ int sray_scene_write_xml(const sray_host_scene *h, char *buf, size_t cap, size_t
     return SRAY_OK;
 }
 
+static int octree_build_host(const sray_scene_desc *s, int level, sray_octree_info *info, float *node_lo,
+                             float *node_hi, uint32_t *items, uint32_t cap_nodes, uint32_t cap_items);
+
 int sray_octree_build_host(const sray_scene_desc *s, int level, sray_octree_info *info, float *node_lo,
                            float *node_hi, uint32_t *items, uint32_t cap_nodes, uint32_t cap_items)
+{
+    try {
+        return octree_build_host(s, level, info, node_lo, node_hi, items, cap_nodes, cap_items);
+    } catch (const std::bad_alloc &) {
+        return SRAY_ERR_NOMEM;
+    }
+}
+
+static int octree_build_host(const sray_scene_desc *s, int level, sray_octree_info *info, float *node_lo,
+                             float *node_hi, uint32_t *items, uint32_t cap_nodes, uint32_t cap_items)
 {
     if (!s || !info) return SRAY_ERR_ARG;
     std::vector<float> boxes;
@@ -97,6 +111,7 @@ int sray_octree_build_host(const sray_scene_desc *s, int level, sray_octree_info
         sray::triangle_boxes(s, (uint32_t)level, boxes);
         sray::build_flat_octree(boxes, s->mesh_oct_items, s->mesh_oct_depth, 0, ft);
     }
+    if (ft.overflow) return SRAY_ERR_LIMIT;
     info->num_nodes = (uint32_t)(ft.lo.size() / 4);
     info->num_leaves = ft.num_leaves;
     info->num_item_refs = (uint32_t)ft.items.size();

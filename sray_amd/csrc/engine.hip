@@ -127,15 +127,16 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
     /* without PERSIST the grid has one run per wave: no loop state is kept */
     for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
+        /* every lane of the wave walks (the leaf tests of sparse waves are
+         * shared); a lane past the end has no ray */
         const uint32_t i = base + __lane_id();
-        if (i < n) {
-            const SceneDev S = T.scenes[UNI ? 0u : T.reqs[q[i].m.w >> SLOT_SHIFT].scene];
-            /* the world ray is re-read from the queue when needed instead of
-             * held in registers: 63 VGPRs, 8 waves per SIMD */
-            const RayRecord<QRay> src{q, i};
-            HitRec h = traverse_src<false, COUNT>(S, src, &stk[threadIdx.x], BS, cnt);
-            hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
-        }
+        const bool live = i < n;
+        const SceneDev S = T.scenes[UNI || !live ? 0u : T.reqs[q[i].m.w >> SLOT_SHIFT].scene];
+        /* the world ray is re-read from the queue when needed instead of
+         * held in registers */
+        const RayRecord<QRay> src{q, i};
+        HitRec h = traverse_src<false, COUNT, true>(S, src, live, &stk[threadIdx.x], BS, cnt);
+        if (live) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
     }
     flush_counts<COUNT>(cnt, counts);
 }
@@ -151,21 +152,22 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
     const uint32_t n = step_n(sn) * (uint32_t)L;
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
     for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
+        /* every lane of the wave walks; past the end or in an unused slot it
+         * has no ray */
         const uint32_t i = base + __lane_id();
-        if (i < n) {
-            float4 d = sq[i].d;
-            if (((const float *)scon)[(size_t)i * 4 + 3] == 0.0f) {
-                occ[i] = 0; /* unused slot */
-            } else {
-                /* the world ray stays in registers here: this kernel is at full
-                 * occupancy already, and re-reading it would only add loads */
-                const float4 o = sq[i].o;
-                const SceneDev S = T.scenes[UNI ? 0u : T.reqs[__float_as_uint(d.w)].scene];
-                HitRec h = traverse<true, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w),
-                                                 &stk[threadIdx.x], BS, cnt);
-                occ[i] = h.obj >= 0 ? 1 : 0;
-            }
+        bool live = i < n;
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d = o;
+        if (live) {
+            d = sq[i].d;
+            o = sq[i].o;
+            live = ((const float *)scon)[(size_t)i * 4 + 3] != 0.0f;
         }
+        /* the world ray stays in registers here: this kernel is at full
+         * occupancy already, and re-reading it would only add loads */
+        const SceneDev S = T.scenes[UNI || !live ? 0u : T.reqs[__float_as_uint(d.w)].scene];
+        const RayRegs src{v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w)};
+        HitRec h = traverse_src<true, COUNT, true>(S, src, live, &stk[threadIdx.x], BS, cnt);
+        if (i < n) occ[i] = h.obj >= 0 ? 1 : 0;
     }
     flush_counts<COUNT>(cnt, counts);
 }

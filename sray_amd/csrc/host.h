@@ -76,13 +76,16 @@ struct FlatTree {
     std::vector<uint32_t> items;
     uint32_t height = 0;
     uint32_t num_leaves = 0;
+    bool overflow = false; /* the build passed 2^23 nodes or 2^28 item references */
     float root_min[3] = {0, 0, 0}, root_max[3] = {0, 0, 0};
 };
 
 /* Builds an octree over item boxes (6 floats per item: min xyz, max xyz)
  * with the reference rules and appends its flattened form to out (node
  * indices offset by out's current node count, item values = item_base +
- * local item index).  Returns the root node index. */
+ * local item index).  Returns the root node index; sets out.overflow (and
+ * appends nothing) when the tree would pass 2^23 nodes or 2^28 item
+ * references. */
 uint32_t build_flat_octree(const std::vector<float> &boxes, int max_items, int max_depth, uint32_t item_base,
                            FlatTree &out);
 

@@ -28,6 +28,8 @@
 #include <thread>
 #include <vector>
 
+#include <new>
+
 #include "runtime.h"
 #include "pmap.h"
 #include "surface.h"
@@ -539,6 +541,7 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     std::vector<float> boxes;
     sray::object_boxes(s, boxes);
     uint32_t sroot = sray::build_flat_octree(boxes, s->scene_oct_items, s->scene_oct_depth, 0, ft);
+    if (ft.overflow) return fail(c, SRAY_ERR_LIMIT, "scene octree passes 2^23 nodes or 2^28 item references");
     const size_t scene_items = ft.items.size(); /* mesh item positions follow */
     g->scene_height = ft.height;
     double dx = (double)ft.root_max[0] - ft.root_min[0], dy = (double)ft.root_max[1] - ft.root_min[1],
@@ -552,6 +555,8 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
         /* append into ft: build into tmp then splice with offsets */
         uint32_t node_off = (uint32_t)(ft.lo.size() / 4), item_off = (uint32_t)ft.items.size();
         sray::build_flat_octree(boxes, s->mesh_oct_items, s->mesh_oct_depth, s->meshes[m].tri_start, tmp);
+        if (tmp.overflow)
+            return fail(c, SRAY_ERR_LIMIT, "octree of mesh %u passes 2^23 nodes or 2^28 item references", m);
         mh = std::max(mh, tmp.height);
         for (size_t k = 0; k < tmp.lo.size() / 4; k++) {
             uint32_t first, meta;
@@ -1204,6 +1209,17 @@ int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, float cellr, 
     return SRAY_OK;
 }
 
+/* do_scene_upload with host memory exhaustion (octree build, staging)
+ * reported as an error instead of escaping the C ABI */
+int scene_upload_guarded(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_id)
+{
+    try {
+        return do_scene_upload(c, s, scene_id);
+    } catch (const std::bad_alloc &) {
+        return fail(c, SRAY_ERR_NOMEM, "host memory exhausted while building the scene's octrees");
+    }
+}
+
 int do_trace_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *d_rays, uint32_t n, sray_hit *d_hits,
                     uint32_t flags, unsigned long long *d_counts)
 {
@@ -1279,7 +1295,7 @@ int sray_gpu_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *s
     if (!c || !scene_id) return SRAY_ERR_ARG;
     ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
-    return do_scene_upload(c, s, scene_id);
+    return scene_upload_guarded(c, s, scene_id);
 }
 
 int sray_gpu_autotune_octree(sray_gpu_ctx *c, const sray_scene_desc *s, const sray_render_params *p, int rounds,
@@ -1301,7 +1317,7 @@ int sray_gpu_autotune_octree(sray_gpu_ctx *c, const sray_scene_desc *s, const sr
         d.mesh_oct_depth = depths[k];
         uint32_t sid = 0;
         double ms = 0;
-        if ((rc = do_scene_upload(c, &d, &sid)) == SRAY_OK) {
+        if ((rc = scene_upload_guarded(c, &d, &sid)) == SRAY_OK) {
             rc = time_render(c, sid, p, rounds, dptr<float>(out), &ms);
             auto it = c->scenes.find(sid);
             (void)hipStreamSynchronize(c->stream);

@@ -271,17 +271,6 @@ struct HitRec {
     uint32_t rank;
 };
 
-/* Scene::cast_ray over the octrees.  ANY: stop at the first hit (the
- * reference's shadow test cast_ray(ray) != NULL, src/shader.cc:266).
- * stk points at this lane's entry 0; consecutive entries are `stride`
- * apart (LDS, lane-interleaved).
- *
- * A node record is fetched only for a child whose box the segment meets:
- * when a node is entered its children are tested from its own box
- * (expand_children) and only the hit ones are pushed.  A popped child is
- * tested again from its fetched box (the same floats, so it passes) to
- * prune it against the best distance found since the push, and the pseudo
- * entries of the roots get their real test there. */
 /* where traverse() takes the world ray from: registers, or a queue record
  * re-read when needed (at the start, entering and leaving a mesh), which
  * keeps the world ray out of registers during the walk */
@@ -309,19 +298,66 @@ template <class R> struct RayRecord {
     }
 };
 
-template <bool ANY, bool COUNT, class SRC>
-__device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src, uint32_t *stk, int stride,
-                                               TraceCounts &cnt)
+/* wave-wide minimum of a 64-bit key (every lane of the wave takes part) */
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 {
-    V3 wo, wd;
-    int time;
-    src.get(wo, wd, time);
-    TRay r = make_tray(wo, wd); /* world ray; the mesh's object ray inside a mesh */
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ float read_lane_f(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ uint32_t read_lane_u(uint32_t v, int j)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
+}
+template <class T> __device__ __forceinline__ const T *read_lane_p(const T *p, int j)
+{
+    const uint64_t v = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = read_lane_u((uint32_t)v, j), hi = read_lane_u((uint32_t)(v >> 32), j);
+    return (const T *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+/* Up to this many lanes of a wave in mesh leaves at once, the whole wave
+ * tests each of their leaves in turn, one triangle per lane; with more,
+ * each lane walks its own leaf.  Long rays (grazing a heightfield, say)
+ * keep a wave alive long after its other lanes are done, and set how long
+ * a bounce level takes; in such a wave one step of 64 loads replaces a
+ * leaf's serial chain of loads. */
+constexpr int COOP_LANES = 16;
+
+/* Scene::cast_ray over the octrees.  ANY: stop at the first hit (the
+ * reference's shadow test cast_ray(ray) != NULL, src/shader.cc:266).
+ * stk points at this lane's entry 0; consecutive entries are `stride`
+ * apart (LDS, lane-interleaved).
+ *
+ * A node record is fetched only for a child whose box the segment meets:
+ * when a node is entered its children are tested from its own box
+ * (expand_children) and only the hit ones are pushed.  A popped child is
+ * tested again from its fetched box (the same floats, so it passes) to
+ * prune it against the best distance found since the push, and the pseudo
+ * entries of the roots get their real test there.
+ *
+ * COOP: every lane of the wave calls this (live = false for a lane without
+ * a ray) and stays in the walk until all lanes are done, so that sparse
+ * waves can test leaves together.  A leaf tested by the wave yields the
+ * same winner as tested by its own lane: the smallest (t, item position)
+ * key, merged with the rule the serial loop applies. */
+template <bool ANY, bool COUNT, bool COOP, class SRC>
+__device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src, bool live, uint32_t *stk,
+                                               int stride, TraceCounts &cnt)
+{
     HitRec best;
     best.t = FLT_MAX;
     best.obj = -1;
     best.tri = 0xffffffffu;
     best.rank = 0xffffffffu;
+    TRay r;
     int level = 0;
     float m_t = FLT_MAX;
     uint32_t m_rank = 0xffffffffu; /* item position of the mesh's best triangle */
@@ -331,72 +367,145 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
     uint32_t item_cur = 0, item_end = 0;
     int cur_mesh = -1;
     int sp = 0;
-    /* pseudo entry whose only child (slot 0) is the scene root */
-    stk[0] = (S.scene_root << 8) | (1u << r.dmask);
-    sp = 1;
+    bool done = !live;
+    if (live) {
+        V3 wo, wd;
+        int time;
+        src.get(wo, wd, time);
+        r = make_tray(wo, wd); /* world ray; the mesh's object ray inside a mesh */
+        /* pseudo entry whose only child (slot 0) is the scene root */
+        stk[0] = (S.scene_root << 8) | (1u << r.dmask);
+        sp = 1;
+    } else {
+        r = make_tray(v3(0.0f, 0.0f, 0.0f), v3(1.0f, 1.0f, 1.0f));
+    }
     for (;;) {
-        /* while-while: walk the stack until this lane has items to test, so
-         * lanes meet at the item tests instead of alternating node visits
-         * and leaf loops */
-        while (item_cur >= item_end && sp > 0) {
-            uint32_t e = stk[(sp - 1) * stride];
-            if ((e & E_MARK) == E_MARK) {
-                sp--;
-                uint32_t rank = e & 0x3fffffffu;
-                int oi = (int)S.items[rank];
-                cache_obj = oi;
+        if (!done) {
+            /* while-while: walk the stack until this lane has items to test,
+             * so lanes meet at the item tests instead of alternating node
+             * visits and leaf loops */
+            while (item_cur >= item_end && sp > 0) {
+                uint32_t e = stk[(sp - 1) * stride];
+                if ((e & E_MARK) == E_MARK) {
+                    sp--;
+                    uint32_t rank = e & 0x3fffffffu;
+                    int oi = (int)S.items[rank];
+                    cache_obj = oi;
+                    if (!ANY) {
+                        uint32_t m_tri = m_rank != 0xffffffffu ? S.items[m_rank] : 0xffffffffu;
+                        cache_t = m_t;
+                        cache_tri = m_tri;
+                        if (m_tri != 0xffffffffu && (m_t < best.t || (m_t == best.t && rank < best.rank))) {
+                            best.t = m_t;
+                            best.obj = oi;
+                            best.tri = m_tri;
+                            best.rank = rank;
+                        }
+                    }
+                    {
+                        V3 o2, d2;
+                        int t2;
+                        src.get(o2, d2, t2);
+                        r = make_tray(o2, d2); /* back to the world ray (same floats) */
+                    }
+                    level = 0;
+                    continue;
+                }
+                if (e & E_RES) {
+                    item_end = e & 0x3fffffffu;
+                    item_cur = stk[(sp - 2) * stride];
+                    sp -= 2;
+                    continue;
+                }
+                uint32_t pend = e & 0xffu;
+                uint32_t k = (uint32_t)__builtin_ctz(pend);
+                pend &= pend - 1u;
+                if (pend == 0u) sp--;
+                else stk[(sp - 1) * stride] = (e & ~0xffu) | pend;
+                uint32_t child = (e >> 8) + (k ^ r.dmask);
+                float4 lo = S.nlo[child], hi = S.nhi[child];
+                if (COUNT) cnt.nodes++;
+                float tmin;
+                if (!slab(lo, hi, r, tmin)) continue;
+                float bound = FLT_MAX;
                 if (!ANY) {
-                    uint32_t m_tri = m_rank != 0xffffffffu ? S.items[m_rank] : 0xffffffffu;
-                    cache_t = m_t;
-                    cache_tri = m_tri;
-                    if (m_tri != 0xffffffffu && (m_t < best.t || (m_t == best.t && rank < best.rank))) {
-                        best.t = m_t;
-                        best.obj = oi;
-                        best.tri = m_tri;
-                        best.rank = rank;
+                    bound = level ? fminf(best.t, m_t) : best.t;
+                    if (tmin > bound) continue;
+                }
+                uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
+                if (meta & LEAF) {
+                    item_cur = first;
+                    item_end = first + (meta & ~LEAF);
+                } else {
+                    uint32_t hitm = expand_children(lo, hi, r, order_mask(meta, r.dmask), bound);
+                    if (hitm) stk[(sp++) * stride] = (first << 8) | hitm;
+                }
+            }
+            if (item_cur >= item_end) done = true; /* stack empty */
+        }
+        if (COOP) {
+            if (__ballot(!done) == 0ull) break;
+        } else if (done) {
+            break;
+        }
+        const bool in_leaf = !done && level != 0;
+        if (COOP) {
+            unsigned long long lm = __ballot(in_leaf);
+            if (lm != 0ull && __popcll(lm) <= COOP_LANES) {
+                /* the wave tests each leaf lane's leaf in turn */
+                const uint32_t lane = __lane_id();
+                while (lm != 0ull) {
+                    const int j = __ffsll(lm) - 1;
+                    lm &= lm - 1ull;
+                    TRay rj;
+                    rj.ox = read_lane_f(r.ox, j);
+                    rj.oy = read_lane_f(r.oy, j);
+                    rj.oz = read_lane_f(r.oz, j);
+                    rj.dx = read_lane_f(r.dx, j);
+                    rj.dy = read_lane_f(r.dy, j);
+                    rj.dz = read_lane_f(r.dz, j);
+                    const uint32_t cur = read_lane_u(item_cur, j), end = read_lane_u(item_end, j);
+                    const float bnd = ANY ? FLT_MAX : read_lane_f(fminf(m_t, best.t), j);
+                    const float4 *LT = read_lane_p(S.ltri, j);
+                    unsigned long long key = ~0ull;
+                    for (uint32_t base = cur; base < end; base += 64u) {
+                        const uint32_t k = base + lane;
+                        if (k < end) {
+                            const float4 *L = LT + (size_t)k * 3;
+                            float4 A = L[0], B = L[1], C = L[2];
+                            if (COUNT) {
+                                cnt.item_refs++;
+                                cnt.triangles++;
+                            }
+                            float t;
+                            if (tri_t(A, B, C, rj, t, bnd)) {
+                                const unsigned long long kk = ((unsigned long long)__float_as_uint(t) << 32) | k;
+                                key = kk < key ? kk : key;
+                            }
+                        }
+                    }
+                    key = wave_min_u64(key);
+                    if (lane == (uint32_t)j) {
+                        if (key != ~0ull) {
+                            const float t = __uint_as_float((uint32_t)(key >> 32));
+                            const uint32_t kpos = (uint32_t)key;
+                            if (ANY) {
+                                best.t = t;
+                                best.tri = S.items[kpos];
+                                best.obj = cur_mesh;
+                                done = true;
+                            } else if (t < m_t || (t == m_t && kpos < m_rank)) {
+                                m_t = t;
+                                m_rank = kpos;
+                            }
+                        }
+                        item_cur = item_end;
                     }
                 }
-                {
-                    V3 o2, d2;
-                    int t2;
-                    src.get(o2, d2, t2);
-                    r = make_tray(o2, d2); /* back to the world ray (same floats) */
-                }
-                level = 0;
                 continue;
-            }
-            if (e & E_RES) {
-                item_end = e & 0x3fffffffu;
-                item_cur = stk[(sp - 2) * stride];
-                sp -= 2;
-                continue;
-            }
-            uint32_t pend = e & 0xffu;
-            uint32_t k = (uint32_t)__builtin_ctz(pend);
-            pend &= pend - 1u;
-            if (pend == 0u) sp--;
-            else stk[(sp - 1) * stride] = (e & ~0xffu) | pend;
-            uint32_t child = (e >> 8) + (k ^ r.dmask);
-            float4 lo = S.nlo[child], hi = S.nhi[child];
-            if (COUNT) cnt.nodes++;
-            float tmin;
-            if (!slab(lo, hi, r, tmin)) continue;
-            float bound = FLT_MAX;
-            if (!ANY) {
-                bound = level ? fminf(best.t, m_t) : best.t;
-                if (tmin > bound) continue;
-            }
-            uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
-            if (meta & LEAF) {
-                item_cur = first;
-                item_end = first + (meta & ~LEAF);
-            } else {
-                uint32_t hitm = expand_children(lo, hi, r, order_mask(meta, r.dmask), bound);
-                if (hitm) stk[(sp++) * stride] = (first << 8) | hitm;
             }
         }
-        if (item_cur >= item_end) break; /* stack empty */
-        if (level) {
+        if (in_leaf) {
             /* mesh leaf: all its triangles, from the leaf-ordered copies
              * (one per step: a second triangle's registers in flight cost
              * a wave of occupancy and more than its latency hiding gains) */
@@ -413,7 +522,8 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
                         best.t = t;
                         best.tri = S.items[item_cur];
                         best.obj = cur_mesh;
-                        return best;
+                        done = true;
+                        break;
                     }
                     if (t < m_t || (t == m_t && item_cur < m_rank)) {
                         m_t = t;
@@ -424,6 +534,7 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
             }
             continue;
         }
+        if (done) continue;
         uint32_t rank = item_cur++;
         int oi = (int)S.items[rank];
         if (COUNT) cnt.item_refs++;
@@ -488,7 +599,8 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
             if (ANY) {
                 best.t = t;
                 best.obj = oi;
-                return best;
+                done = true;
+                continue;
             }
             if (t < best.t || (t == best.t && rank < best.rank)) {
                 best.t = t;
@@ -497,7 +609,6 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
                 best.rank = rank;
             }
         }
-        continue;
     }
     return best;
 }
@@ -507,7 +618,7 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                                            TraceCounts &cnt)
 {
     RayRegs src{wo, wd, time};
-    return traverse_src<ANY, COUNT>(S, src, stk, stride, cnt);
+    return traverse_src<ANY, COUNT, false>(S, src, true, stk, stride, cnt);
 }
 
 } // namespace srd
