@@ -328,8 +328,10 @@ template <class T> __device__ __forceinline__ const T *read_lane_p(const T *p, i
  * each lane walks its own leaf.  Long rays (grazing a heightfield, say)
  * keep a wave alive long after its other lanes are done, and set how long
  * a bounce level takes; in such a wave one step of 64 loads replaces a
- * leaf's serial chain of loads. */
-constexpr int COOP_LANES = 16;
+ * leaf's serial chain of loads.  Measured on the heightfield (thresholds
+ * 8 / 16 / 32 / 64): 8 is fastest for closest-hit rays, 16 for any-hit
+ * rays, whose leaves end at the first hit when walked by their own lane. */
+template <bool ANY> constexpr int coop_lanes() { return ANY ? 16 : 8; }
 
 /* Scene::cast_ray over the octrees.  ANY: stop at the first hit (the
  * reference's shadow test cast_ray(ray) != NULL, src/shader.cc:266).
@@ -451,7 +453,7 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
         const bool in_leaf = !done && level != 0;
         if (COOP) {
             unsigned long long lm = __ballot(in_leaf);
-            if (lm != 0ull && __popcll(lm) <= COOP_LANES) {
+            if (lm != 0ull && __popcll(lm) <= coop_lanes<ANY>()) {
                 /* the wave tests each leaf lane's leaf in turn */
                 const uint32_t lane = __lane_id();
                 while (lm != 0ull) {
