@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tuning (no untimed search)")
     ap.add_argument("--tuning", default=None,
-                    help="heightfield: set the tuning instead of searching, e.g. "
+                    help="set the tuning (heightfield: instead of searching), e.g. "
                          "trace_block=64,tile_w=64,chunk_log2=21,graphs=1,persistent=0")
     ap.add_argument("--frames", type=int, default=None, help="room / spheres: frames in flight (8 / 64)")
     ap.add_argument("--scenes", type=int, default=None, help="batch: scenes (256); serve: resident scenes (32)")
@@ -129,6 +129,17 @@ def base_line(metric, value, steps, warmup, ms, dtype, workload, extra):
     return line
 
 
+def tuning_arg(args, ctx):
+    """--tuning applied to the context; the tuning as a dict, or None"""
+    if not args.tuning:
+        return None
+    kw = {k: (float(v) if k == "gather_cell" else int(v))
+          for k, v in (item.split("=") for item in args.tuning.split(","))}
+    t = ctx.set_tuning(**kw)
+    return {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2,
+            "graphs": t.graphs, "persistent": t.persistent, "lanes": t.lanes}
+
+
 def look_at(pos, target, vfov_deg, shutter=0):
     """camera -> world matrix of a camera at pos looking at target (+z forward, +y up)"""
     import sray_amd as S
@@ -153,6 +164,7 @@ def run_spheres(args):
     config, variant, W, H, it, samples = WORKLOADS["spheres"]
     hs = S.HostScene.generate(config)
     ctx = S.Context(0)
+    tuning_arg(args, ctx)
     sid = ctx.upload(hs)
     p = S.render_params(W, H, samples=samples, iter=it)
     out = ctx.malloc(W * H * 16)
@@ -213,6 +225,7 @@ def run_room(args):
     config, variant, W, H, it, samples = WORKLOADS["room"]
     hs = S.HostScene.generate(config, variant=variant)
     ctx = S.Context(0)
+    tuning_arg(args, ctx)
     sid = ctx.upload(hs)
     diag = S.scene_diag(hs.desc_ptr())
     t0 = time.perf_counter()
@@ -267,6 +280,7 @@ def run_batch(args):
     scenes, sids, params = [], [], []
     t0 = time.perf_counter()
     ctx = S.Context(0)
+    tuning_arg(args, ctx)
     for k in range(n):
         ntri = int(math.exp(rng.uniform(math.log(1000), math.log(200_000))))
         hs = S.HostScene.generate(3, seed=0x5EED0400 + k, variant=ntri)
@@ -318,6 +332,7 @@ def run_serve(args):
     import sray_amd as S
     rng = np.random.default_rng(0x5EED0005)
     ctx = S.Context(0)
+    tuning_arg(args, ctx)
     sids, diags = [], []
     t0 = time.perf_counter()
     for k in range(args.scenes):
@@ -402,17 +417,11 @@ def main():
               f"upload+octree build {upload_s:.2f} s, {ctx.scene_bytes(sid) / 2**20:.1f} MiB resident")
     p = S.render_params(W, H, samples=samples, iter=it)
     out = ctx.malloc(W * H * 4 * 4)
-    tuning = None
-    if args.tuning:
-        kw = {k: (float(v) if k == "gather_cell" else int(v))
-              for k, v in (item.split("=") for item in args.tuning.split(","))}
-        t = ctx.set_tuning(**kw)
-        tuning = {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2,
-                  "graphs": t.graphs, "persistent": t.persistent}
-    elif not args.no_autotune:
+    tuning = tuning_arg(args, ctx)
+    if tuning is None and not args.no_autotune:
         t, tms = ctx.autotune(sid, p, rounds=2)
         tuning = {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2,
-                  "graphs": t.graphs, "persistent": t.persistent}
+                  "graphs": t.graphs, "persistent": t.persistent, "lanes": t.lanes}
         log(args, f"autotune: {tuning}, {tms:.2f} ms/frame")
     for _ in range(args.warmup):
         ctx.render_device(sid, p, out)
@@ -428,8 +437,13 @@ def main():
     ctx.synchronize()
     wall = time.perf_counter() - t0
     # the same steps again with every launch bracketed by HIP events on the
-    # library's stream (per-kernel durations for the roofline; kernels are
-    # launched one by one while events are recorded)
+    # stream it runs on (per-kernel durations for the roofline; kernels are
+    # launched one by one while events are recorded).  This pass renders the
+    # frame as one chunk on one stream, so that each k_trace_q launch covers
+    # a whole bounce level and no launch overlaps another (overlapping
+    # launches would each be charged the time they share).
+    timed_tuning = ctx.tuning()
+    ctx.set_tuning(chunk_log2=21, lanes=1)
     ctx.set_profiling(True)
     ctx.kernel_stats(reset=True)
     for _ in range(args.steps):
@@ -437,6 +451,7 @@ def main():
     ctx.synchronize()
     ctx.set_profiling(False)
     ks = ctx.kernel_stats(reset=True)
+    ctx.set_tuning(chunk_log2=timed_tuning.chunk_log2, lanes=timed_tuning.lanes)
     # visit counters of the same frame (separate, untimed run)
     pc = S.render_params(W, H, samples=samples, iter=it, flags=S.abi.SRAY_RENDER_COUNT)
     cst = ctx.render_device(sid, pc, out)
@@ -484,7 +499,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_trace_q", "avg_launch_ms": round(avg_launch_ms, 4),
-                     "launches_per_step": per_frame_launches, "alg_bytes_per_launch": int(bytes_per_launch)},
+                     "launches_per_step": per_frame_launches, "alg_bytes_per_launch": int(bytes_per_launch),
+                     "pass": "one chunk of 2^21 rays per bounce level, one stream"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -363,7 +363,9 @@ typedef struct sray_tuning {
                                  HIP graph; 0: launched kernel by kernel (same kernels, same order) */
     int32_t persistent;       /* 1: traversal kernels run as many waves as stay resident, each
                                  looping over runs of 64 rays; 0: one run per wave */
-    int32_t pad_;
+    int32_t lanes;            /* 1, 2 or 4 HIP streams a render pass spreads its primary chunks over
+                                 (each with its own bounce-level queues), so that one chunk's deep,
+                                 sparse levels overlap another chunk's work */
 } sray_tuning;
 int  sray_gpu_set_tuning(sray_gpu_ctx *ctx, const sray_tuning *t);
 int  sray_gpu_get_tuning(sray_gpu_ctx *ctx, sray_tuning *t);

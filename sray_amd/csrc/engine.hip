@@ -942,13 +942,30 @@ struct Engine {
     DevBuf fb, nsamp, preq, keep, act[2], subpix;
     int cur = 0;
     uint32_t nact = 0;
-    /* ray queues and per-level scratch */
-    std::vector<DevBuf> qlev;
-    DevBuf hits, direct, sq, scon, rslot, occ, kids, kflag, acc, stats, errf, rcounts, slotcnt, slotrays, dreqs,
-        dscenes, greq;
-    /* device-driven levels: per-level ray counts, compaction block counts /
-     * offsets */
-    DevBuf lvcnt, cmpb;
+    /* shared counters and tables */
+    DevBuf stats, errf, rcounts, slotcnt, slotrays, dreqs, dscenes;
+    /* lanes: a stream each, with the ray queues and per-level scratch of the
+     * primary chunks it runs (chunks on different lanes overlap on the GPU,
+     * so one chunk's long-ray tail at a deep level leaves room for another
+     * chunk's work) */
+    struct Lane {
+        hipStream_t s = nullptr; /* lane 0: the context's stream; others: created here */
+        hipEvent_t done = nullptr;
+        std::vector<DevBuf> qlev;
+        DevBuf hits, direct, sq, scon, rslot, occ, kids, kflag, acc, greq;
+        DevBuf lvcnt, cmpb; /* per-level ray counts; compaction block counts / offsets */
+        uint64_t bytes() const
+        {
+            uint64_t b = 0;
+            const DevBuf *bs[] = {&hits, &direct, &sq, &scon, &rslot, &occ, &kids, &kflag, &acc, &greq, &lvcnt, &cmpb};
+            for (const DevBuf *d : bs) b += d->bytes;
+            for (const DevBuf &d : qlev) b += d.bytes;
+            return b;
+        }
+    };
+    std::vector<Lane> lanes;
+    hipEvent_t tables_ready = nullptr;
+    int nlanes = 1; /* lanes a pass spreads its primary chunks over */
     std::map<const void *, uint32_t> pgrid; /* persistent grid per traversal kernel instance */
     /* captured descents, keyed by everything their launches depend on */
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
@@ -1001,10 +1018,9 @@ struct Engine {
     uint64_t fixed_bytes() const
     {
         uint64_t b = 0;
-        const DevBuf *bs[] = {&hits, &direct, &sq, &scon, &rslot, &occ, &kids, &kflag, &acc, &stats, &errf, &rcounts,
-                              &slotcnt, &slotrays, &dreqs, &dscenes, &greq, &lvcnt, &cmpb};
+        const DevBuf *bs[] = {&stats, &errf, &rcounts, &slotcnt, &slotrays, &dreqs, &dscenes};
         for (const DevBuf *d : bs) b += d->bytes;
-        for (const DevBuf &d : qlev) b += d.bytes;
+        for (const Lane &l : lanes) b += l.bytes();
         return b;
     }
     /* the pixel and sample arenas */
@@ -1108,6 +1124,7 @@ void apply_tuning(Engine &E)
     E.chunk = 1u << t.chunk_log2;
     E.use_graphs = t.graphs != 0;
     E.persistent = t.persistent != 0;
+    E.nlanes = t.lanes;
 }
 
 /* places a request in the arena and the active list */
@@ -1199,6 +1216,8 @@ void retire(Engine &E, uint32_t s)
 /* drops every request in flight (after a device error) */
 void abort_all(Engine &E)
 {
+    for (Engine::Lane &ln : E.lanes)
+        if (ln.s) (void)hipStreamSynchronize(ln.s);
     (void)hipStreamSynchronize(E.c->stream);
     for (uint32_t s = 0; s < E.hi; s++)
         if (E.slot_job[s]) {
@@ -1273,6 +1292,7 @@ int build_tables(Engine &E)
 
 struct Level {
     Engine *E;
+    Engine::Lane *ln; /* the stream and scratch of this chunk */
     Tabs T;
     const uint32_t *active;
     uint32_t c0;
@@ -1300,9 +1320,11 @@ template <typename K> uint32_t persistent_grid(Engine &E, K kern, int bs)
 int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
 {
     Engine &E = *LV.E;
+    Engine::Lane &ln = *LV.ln;
     sray_gpu_ctx *c = E.c;
-    const StepN sn{dptr<uint32_t>(E.lvcnt) + b, off, cap};
-    const QRay *q = dptr<QRay>(E.qlev[b]) + off;
+    const hipStream_t st = ln.s;
+    const StepN sn{dptr<uint32_t>(ln.lvcnt) + b, off, cap};
+    const QRay *q = dptr<QRay>(ln.qlev[b]) + off;
     unsigned long long *rc6 = dptr<unsigned long long>(E.rcounts);
     unsigned long long *stats = dptr<unsigned long long>(E.stats);
     unsigned long long *srays = dptr<unsigned long long>(E.slotrays);
@@ -1314,54 +1336,54 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
         return E.persistent ? std::min(g, persistent_grid(E, kern, bs)) : g;
     };
     {
-        ProfScope ps(c, "k_trace_q");
+        ProfScope ps(c, "k_trace_q", st);
         /* the traversal reads only the scene: one scene = the uniform variant */
         const bool P = E.persistent;
         auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs, P) : trace_q_of<true, false>(bs, P))
                                : (E.one_scene ? trace_q_of<false, true>(bs, P) : trace_q_of<false, false>(bs, P));
-        kern<<<tgrid(kern, cap), bs, 0, c->stream>>>(LV.T, q, sn, dptr<int4>(E.hits), rc6);
+        kern<<<tgrid(kern, cap), bs, 0, st>>>(LV.T, q, sn, dptr<int4>(ln.hits), rc6);
     }
     {
-        ProfScope ps(c, "k_shade");
+        ProfScope ps(c, "k_shade", st);
         auto kern = E.photons ? (E.uni ? k_shade<true, true, true>
                                        : E.one_scene ? k_shade<true, false, true> : k_shade<true, false, false>)
                               : (E.uni ? k_shade<false, true, true>
                                        : E.one_scene ? k_shade<false, false, true> : k_shade<false, false, false>);
-        kern<<<small_grid, 256, 0, c->stream>>>(LV.T, LV.active, LV.c0, q, dptr<int4>(E.hits), sn, E.L, E.K,
-                                               dptr<float4>(E.direct), dptr<SRay>(E.sq), dptr<float4>(E.scon),
-                                               dptr<uint32_t>(E.rslot), dptr<QRay>(E.kids),
-                                               dptr<uint8_t>(E.kflag), dptr<GatherReq>(E.greq), stats, srays,
+        kern<<<small_grid, 256, 0, st>>>(LV.T, LV.active, LV.c0, q, dptr<int4>(ln.hits), sn, E.L, E.K,
+                                               dptr<float4>(ln.direct), dptr<SRay>(ln.sq), dptr<float4>(ln.scon),
+                                               dptr<uint32_t>(ln.rslot), dptr<QRay>(ln.kids),
+                                               dptr<uint8_t>(ln.kflag), dptr<GatherReq>(ln.greq), stats, srays,
                                                dptr<uint32_t>(E.errf));
     }
     if (E.photons) {
-        ProfScope ps(c, "k_gather");
+        ProfScope ps(c, "k_gather", st);
         auto kern = E.uni ? gather_of<true>(E.gather_lanes) : gather_of<false>(E.gather_lanes);
         const uint32_t g = std::min<uint32_t>(blocks((size_t)cap * E.gather_lanes, 256), 8192u);
-        kern<<<g, 256, 0, c->stream>>>(LV.T, q, dptr<GatherReq>(E.greq), sn, dptr<float4>(E.direct));
+        kern<<<g, 256, 0, st>>>(LV.T, q, dptr<GatherReq>(ln.greq), sn, dptr<float4>(ln.direct));
     }
     if (E.L) {
-        ProfScope ps(c, "k_trace_shadow");
+        ProfScope ps(c, "k_trace_shadow", st);
         const bool P = E.persistent;
         auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs, P)
                                               : trace_shadow_of<true, false>(bs, P))
                                : (E.one_scene ? trace_shadow_of<false, true>(bs, P)
                                               : trace_shadow_of<false, false>(bs, P));
-        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, c->stream>>>(LV.T, dptr<SRay>(E.sq), dptr<float4>(E.scon),
-                                                                   sn, E.L, dptr<uint8_t>(E.occ), rc6 + 6);
+        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, st>>>(LV.T, dptr<SRay>(ln.sq), dptr<float4>(ln.scon),
+                                                                   sn, E.L, dptr<uint8_t>(ln.occ), rc6 + 6);
     }
     {
-        ProfScope ps(c, "k_accum");
-        k_accum<<<small_grid, 256, 0, c->stream>>>(dptr<uint32_t>(E.rslot), dptr<float4>(E.direct),
-                                                   dptr<float4>(E.scon), dptr<uint8_t>(E.occ), sn, E.L,
-                                                   dptr<float4>(E.acc));
+        ProfScope ps(c, "k_accum", st);
+        k_accum<<<small_grid, 256, 0, st>>>(dptr<uint32_t>(ln.rslot), dptr<float4>(ln.direct),
+                                                   dptr<float4>(ln.scon), dptr<uint8_t>(ln.occ), sn, E.L,
+                                                   dptr<float4>(ln.acc));
     }
     if (b < E.maxiter) {
-        ProfScope ps(c, "k_compact");
-        uint32_t *bc = dptr<uint32_t>(E.cmpb);
-        k_cmp_count<<<CMP_BLOCKS, CMP_THREADS, 0, c->stream>>>(dptr<uint8_t>(E.kflag), sn, E.K, bc);
-        k_cmp_scan<<<1, CMP_BLOCKS, 0, c->stream>>>(bc, bc + CMP_BLOCKS, dptr<uint32_t>(E.lvcnt) + b + 1);
-        k_cmp_scatter<<<CMP_BLOCKS, CMP_THREADS, 0, c->stream>>>(dptr<QRay>(E.kids), dptr<uint8_t>(E.kflag), sn,
-                                                                 E.K, bc + CMP_BLOCKS, dptr<QRay>(E.qlev[b + 1]));
+        ProfScope ps(c, "k_compact", st);
+        uint32_t *bc = dptr<uint32_t>(ln.cmpb);
+        k_cmp_count<<<CMP_BLOCKS, CMP_THREADS, 0, st>>>(dptr<uint8_t>(ln.kflag), sn, E.K, bc);
+        k_cmp_scan<<<1, CMP_BLOCKS, 0, st>>>(bc, bc + CMP_BLOCKS, dptr<uint32_t>(ln.lvcnt) + b + 1);
+        k_cmp_scatter<<<CMP_BLOCKS, CMP_THREADS, 0, st>>>(dptr<QRay>(ln.kids), dptr<uint8_t>(ln.kflag), sn,
+                                                                 E.K, bc + CMP_BLOCKS, dptr<QRay>(ln.qlev[b + 1]));
     }
     HIPCHK(c, hipGetLastError());
     return SRAY_OK;
@@ -1395,8 +1417,9 @@ int finish_levels(Level &LV, int b)
     if (b >= E.maxiter) return SRAY_OK;
     uint32_t n[64];
     const int nl = E.maxiter - b;
-    HIPCHK(c, hipMemcpyAsync(n, dptr<uint32_t>(E.lvcnt) + b + 1, (size_t)nl * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(n, dptr<uint32_t>(LV.ln->lvcnt) + b + 1, (size_t)nl * 4, hipMemcpyDeviceToHost,
+                             LV.ln->s));
+    HIPCHK(c, hipStreamSynchronize(LV.ln->s));
     int rc;
     for (int d = E.maxiter; d > b; d--)
         for (uint64_t off = E.chunk; off < n[d - b - 1]; off += E.chunk) {
@@ -1424,14 +1447,16 @@ bool may_overflow(const Engine &E, uint32_t m)
 int launch_chunk(Level &LV, uint32_t m)
 {
     Engine &E = *LV.E;
+    Engine::Lane &ln = *LV.ln;
     sray_gpu_ctx *c = E.c;
+    const hipStream_t st = ln.s;
     auto body = [&]() -> int {
-        k_fill_f4<<<blocks(m, 256), 256, 0, c->stream>>>(dptr<float4>(E.acc), m, make_float4(0, 0, 0, 0));
+        k_fill_f4<<<blocks(m, 256), 256, 0, st>>>(dptr<float4>(ln.acc), m, make_float4(0, 0, 0, 0));
         {
-            ProfScope ps(c, "k_raygen");
+            ProfScope ps(c, "k_raygen", st);
             auto kern = E.uni ? k_raygen<true> : k_raygen<false>;
-            kern<<<blocks(m, 256), 256, 0, c->stream>>>(LV.T, LV.active, LV.c0, m, dptr<QRay>(E.qlev[0]),
-                                                        dptr<uint32_t>(E.lvcnt));
+            kern<<<blocks(m, 256), 256, 0, st>>>(LV.T, LV.active, LV.c0, m, dptr<QRay>(ln.qlev[0]),
+                                                 dptr<uint32_t>(ln.lvcnt));
         }
         return descend(LV, 0, 0, m);
     };
@@ -1444,20 +1469,21 @@ int launch_chunk(Level &LV, uint32_t m)
                                  (uint64_t)(uintptr_t)LV.T.scenes, (uint64_t)(uintptr_t)LV.T.reqs,
                                  (uint64_t)(uintptr_t)LV.T.preq, (uint64_t)(uintptr_t)LV.T.nsamp,
                                  (uint64_t)(uintptr_t)LV.active};
-    const DevBuf *bufs[] = {&E.hits, &E.direct, &E.sq, &E.scon, &E.rslot, &E.occ, &E.kids, &E.kflag, &E.acc, &E.stats, &E.errf,
-                            &E.rcounts, &E.slotrays, &E.greq, &E.lvcnt, &E.cmpb};
+    const DevBuf *bufs[] = {&ln.hits, &ln.direct, &ln.sq, &ln.scon, &ln.rslot, &ln.occ, &ln.kids, &ln.kflag, &ln.acc,
+                            &E.stats, &E.errf, &E.rcounts, &E.slotrays, &ln.greq, &ln.lvcnt, &ln.cmpb};
     for (const DevBuf *d : bufs) key.push_back((uint64_t)(uintptr_t)d->p);
-    for (int b = 0; b <= E.maxiter; b++) key.push_back((uint64_t)(uintptr_t)E.qlev[b].p);
+    for (int b = 0; b <= E.maxiter; b++) key.push_back((uint64_t)(uintptr_t)ln.qlev[b].p);
+    key.push_back((uint64_t)(uintptr_t)st);
     auto it = E.graphs.find(key);
     if (it == E.graphs.end()) {
         if (E.graphs.size() >= 64) {
             for (auto &kv : E.graphs) (void)hipGraphExecDestroy(kv.second);
             E.graphs.clear();
         }
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = body();
         hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        hipError_t e = hipStreamEndCapture(st, &g);
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
@@ -1469,7 +1495,7 @@ int launch_chunk(Level &LV, uint32_t m)
         if (e != hipSuccess) return fail(c, SRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         it = E.graphs.emplace(key, x).first;
     }
-    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    HIPCHK(c, hipGraphLaunch(it->second, st));
     return SRAY_OK;
 }
 
@@ -1503,41 +1529,96 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
     sray_gpu_ctx *c = E.c;
     int rc;
     if ((rc = build_tables(E))) return rc;
-    uint32_t levels = (uint32_t)E.maxiter + 1;
-    if (E.qlev.size() < levels) E.qlev.resize(levels);
-    for (uint32_t b = 0; b < levels; b++)
-        if ((rc = ensure(c, E.qlev[b], (size_t)E.chunk * E.K * sizeof(QRay)))) return rc;
-    if ((rc = ensure(c, E.hits, (size_t)E.chunk * sizeof(int4))) || (rc = ensure(c, E.direct, (size_t)E.chunk * 16)) ||
-        (rc = ensure(c, E.sq, (size_t)E.chunk * std::max(E.L, 1) * sizeof(SRay))) ||
-        (rc = ensure(c, E.scon, (size_t)E.chunk * std::max(E.L, 1) * sizeof(float4))) ||
-        (rc = ensure(c, E.rslot, (size_t)E.chunk * sizeof(uint32_t))) ||
-        (rc = ensure(c, E.occ, (size_t)E.chunk * std::max(E.L, 1))) ||
-        (rc = ensure(c, E.kids, (size_t)E.chunk * E.K * sizeof(QRay))) || (rc = ensure(c, E.kflag, (size_t)E.chunk * E.K)) ||
-        (rc = ensure(c, E.acc, (size_t)E.chunk * 16)) || (rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) ||
-        (E.photons && (rc = ensure(c, E.greq, (size_t)E.chunk * sizeof(GatherReq)))) ||
-        (rc = ensure(c, E.slotrays, MAX_SLOTS * 8)) || (rc = ensure(c, c->nsel, 4)))
+    if ((rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) || (rc = ensure(c, E.slotrays, MAX_SLOTS * 8)) ||
+        (rc = ensure(c, c->nsel, 4)) || (rc = ensure(c, E.errf, 4)) ||
+        (rc = ensure(c, E.stats, 8 * sizeof(unsigned long long))) ||
+        (rc = ensure(c, E.rcounts, 12 * sizeof(unsigned long long))))
         return rc;
     if (E.sub.cap == 0 && (rc = ensure(c, E.subpix, 16))) return rc;
-    if ((rc = ensure(c, E.errf, 4)) || (rc = ensure(c, E.stats, 8 * sizeof(unsigned long long))) ||
-        (rc = ensure(c, E.rcounts, 12 * sizeof(unsigned long long))) || (rc = ensure(c, E.lvcnt, 64 * 4)) ||
-        (rc = ensure(c, E.cmpb, 2 * CMP_BLOCKS * 4)))
-        return rc;
+    /* primary chunks: with several lanes a pass is spread over them, each
+     * lane running its chunks in order.  A pass of fewer pixels than a
+     * chunk is split only into pieces of at least 2^20 pixels: below that
+     * the extra launches and stream hand-offs cost more than the overlap of
+     * the deep levels gains. */
+    const uint32_t want = (uint32_t)std::max(1, E.nlanes);
+    uint32_t mchunk = E.chunk;
+    if (want > 1) {
+        uint64_t per = ((uint64_t)E.nact + want - 1) / want;
+        per = std::max<uint64_t>((per + 1023) & ~1023ull, 1u << 20);
+        mchunk = (uint32_t)std::min<uint64_t>(per, E.chunk);
+    }
+    const uint32_t nchunks = (E.nact + mchunk - 1) / mchunk;
+    const uint32_t nl = std::min(want, nchunks);
+    /* per-step capacity: a step holds at most mchunk * Kb^depth rays, and at
+     * most one sub-chunk */
+    uint64_t cap_rays = mchunk;
+    for (int d = 1; d <= E.maxiter && cap_rays < E.chunk; d++) cap_rays *= (uint64_t)std::max(E.Kb, 1);
+    cap_rays = std::min<uint64_t>(cap_rays, E.chunk);
+    const size_t L1 = (size_t)std::max(E.L, 1);
+    while (E.lanes.size() < nl) E.lanes.emplace_back();
+    if (!E.tables_ready) HIPCHK(c, hipEventCreateWithFlags(&E.tables_ready, hipEventDisableTiming));
+    for (uint32_t l = 0; l < nl; l++) {
+        Engine::Lane &ln = E.lanes[l];
+        if (l == 0) ln.s = c->stream;
+        else if (!ln.s) HIPCHK(c, hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking)); /* default priority */
+        if (!ln.done) HIPCHK(c, hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
+        if (ln.qlev.size() < (size_t)E.maxiter + 1) ln.qlev.resize((size_t)E.maxiter + 1);
+        if ((rc = ensure(c, ln.qlev[0], (size_t)mchunk * sizeof(QRay)))) return rc;
+        for (int b = 1; b <= E.maxiter; b++)
+            if ((rc = ensure(c, ln.qlev[b], (size_t)cap_rays * E.K * sizeof(QRay)))) return rc;
+        if ((rc = ensure(c, ln.hits, (size_t)cap_rays * sizeof(int4))) ||
+            (rc = ensure(c, ln.direct, (size_t)cap_rays * 16)) ||
+            (rc = ensure(c, ln.sq, (size_t)cap_rays * L1 * sizeof(SRay))) ||
+            (rc = ensure(c, ln.scon, (size_t)cap_rays * L1 * sizeof(float4))) ||
+            (rc = ensure(c, ln.rslot, (size_t)cap_rays * sizeof(uint32_t))) ||
+            (rc = ensure(c, ln.occ, (size_t)cap_rays * L1)) ||
+            (rc = ensure(c, ln.kids, (size_t)cap_rays * E.K * sizeof(QRay))) ||
+            (rc = ensure(c, ln.kflag, (size_t)cap_rays * E.K)) || (rc = ensure(c, ln.acc, (size_t)mchunk * 16)) ||
+            (E.photons && (rc = ensure(c, ln.greq, (size_t)cap_rays * sizeof(GatherReq)))) ||
+            (rc = ensure(c, ln.lvcnt, 64 * 4)) || (rc = ensure(c, ln.cmpb, 2 * CMP_BLOCKS * 4)))
+            return rc;
+    }
 
     HIPCHK(c, hipMemsetAsync(E.errf.p, 0, 4, c->stream));
     Tabs T{dptr<SceneDev>(E.dscenes), dptr<ReqDev>(E.dreqs), dptr<uint32_t>(E.preq), dptr<int>(E.nsamp)};
     if (!E.uni) HIPCHK(c, hipMemsetAsync(E.slotrays.p, 0, (size_t)E.hi * 8, c->stream));
     const uint32_t *A = dptr<uint32_t>(E.act[E.cur]);
-    Level LV{&E, T, A, 0};
-    for (uint32_t c0 = 0; c0 < E.nact; c0 += E.chunk) {
-        uint32_t m = std::min(E.chunk, E.nact - c0);
-        LV.c0 = c0;
-        if ((rc = launch_chunk(LV, m))) return rc;
-        /* a level may have outgrown one sub-chunk: read the counts and run
-         * the remaining sub-chunks (never needed when m * K^maxiter fits) */
-        if (may_overflow(E, m) && (rc = finish_levels(LV, 0))) return rc;
-        auto fin = E.uni ? k_pass_finish<true> : k_pass_finish<false>;
-        fin<<<blocks(m, 256), 256, 0, c->stream>>>(T, A, c0, m, dptr<float4>(E.acc), dptr<float4>(E.fb),
-                                                  dptr<float4>(E.subpix), dptr<int>(E.nsamp), dptr<uint8_t>(E.keep));
+    /* the other lanes start after the tables, admissions and clears above */
+    if (nl > 1) HIPCHK(c, hipEventRecord(E.tables_ready, c->stream));
+    for (uint32_t l = 1; l < nl; l++) HIPCHK(c, hipStreamWaitEvent(E.lanes[l].s, E.tables_ready, 0));
+    auto finish = E.uni ? k_pass_finish<true> : k_pass_finish<false>;
+    /* a level may outgrow one sub-chunk: then each chunk's level counts are
+     * read after its descent and the remaining sub-chunks run before the
+     * lane takes its next chunk (never needed when mchunk * Kb^maxiter fits) */
+    const bool ovf = may_overflow(E, mchunk);
+    for (uint32_t g0 = 0; g0 < nchunks; g0 += nl) {
+        const uint32_t g1 = std::min(nchunks, g0 + nl);
+        for (uint32_t j = g0; j < g1; j++) {
+            const uint32_t c0 = j * mchunk, m = std::min(mchunk, E.nact - c0);
+            Engine::Lane &ln = E.lanes[j - g0];
+            Level LV{&E, &ln, T, A, c0};
+            if ((rc = launch_chunk(LV, m))) return rc;
+            if (!ovf)
+                finish<<<blocks(m, 256), 256, 0, ln.s>>>(T, A, c0, m, dptr<float4>(ln.acc), dptr<float4>(E.fb),
+                                                         dptr<float4>(E.subpix), dptr<int>(E.nsamp),
+                                                         dptr<uint8_t>(E.keep));
+        }
+        if (ovf)
+            for (uint32_t j = g0; j < g1; j++) {
+                const uint32_t c0 = j * mchunk, m = std::min(mchunk, E.nact - c0);
+                Engine::Lane &ln = E.lanes[j - g0];
+                Level LV{&E, &ln, T, A, c0};
+                if ((rc = finish_levels(LV, 0))) return rc;
+                finish<<<blocks(m, 256), 256, 0, ln.s>>>(T, A, c0, m, dptr<float4>(ln.acc), dptr<float4>(E.fb),
+                                                         dptr<float4>(E.subpix), dptr<int>(E.nsamp),
+                                                         dptr<uint8_t>(E.keep));
+            }
+        HIPCHK(c, hipGetLastError());
+    }
+    /* the pass end waits for every lane */
+    for (uint32_t l = 1; l < nl; l++) {
+        HIPCHK(c, hipEventRecord(E.lanes[l].done, E.lanes[l].s));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, E.lanes[l].done, 0));
     }
     /* The pass status comes back with one copy into pinned memory and one
      * synchronisation: the next active list's length (when some request can
@@ -1613,14 +1694,25 @@ void engine_free(Engine *E)
     if (!E) return;
     for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
     E->graphs.clear();
-    DevBuf *bs[] = {&E->fb, &E->nsamp, &E->preq, &E->keep, &E->act[0], &E->act[1], &E->subpix, &E->hits,
-                    &E->direct, &E->sq, &E->scon, &E->rslot, &E->occ, &E->kids, &E->kflag, &E->acc, &E->stats, &E->errf, &E->rcounts,
-                    &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes, &E->greq, &E->lvcnt, &E->cmpb};
+    for (size_t l = 0; l < E->lanes.size(); l++) {
+        Engine::Lane &ln = E->lanes[l];
+        if (l > 0 && ln.s) {
+            (void)hipStreamSynchronize(ln.s);
+            (void)hipStreamDestroy(ln.s);
+        }
+        if (ln.done) (void)hipEventDestroy(ln.done);
+        DevBuf *lb[] = {&ln.hits, &ln.direct, &ln.sq, &ln.scon, &ln.rslot, &ln.occ, &ln.kids, &ln.kflag, &ln.acc,
+                        &ln.greq, &ln.lvcnt, &ln.cmpb};
+        for (DevBuf *b : lb) release(*b);
+        for (DevBuf &b : ln.qlev) release(b);
+    }
+    if (E->tables_ready) (void)hipEventDestroy(E->tables_ready);
+    DevBuf *bs[] = {&E->fb, &E->nsamp, &E->preq, &E->keep, &E->act[0], &E->act[1], &E->subpix, &E->stats,
+                    &E->errf, &E->rcounts, &E->slotcnt, &E->slotrays, &E->dreqs, &E->dscenes};
     for (DevBuf *b : bs) release(*b);
     if (E->hst) (void)hipHostFree(E->hst);
     if (E->hreq_pin) (void)hipHostFree(E->hreq_pin);
     if (E->hscn_pin) (void)hipHostFree(E->hscn_pin);
-    for (DevBuf &b : E->qlev) release(b);
     delete E;
 }
 
@@ -1890,7 +1982,8 @@ int check_tuning(sray_gpu_ctx *c, const sray_tuning &t)
               (t.tile_w == 8 || t.tile_w == 16 || t.tile_w == 32 || t.tile_w == 64) &&
               (t.gather_lanes == 8 || t.gather_lanes == 16 || t.gather_lanes == 32) && t.chunk_log2 >= 16 &&
               t.chunk_log2 <= 21 && (t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f) &&
-              (t.graphs == 0 || t.graphs == 1) && (t.persistent == 0 || t.persistent == 1);
+              (t.graphs == 0 || t.graphs == 1) && (t.persistent == 0 || t.persistent == 1) &&
+              (t.lanes == 1 || t.lanes == 2 || t.lanes == 4);
     return ok ? SRAY_OK : fail(c, SRAY_ERR_ARG, "tuning value outside the supported set");
 }
 
@@ -1928,9 +2021,10 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
     static const int blocks_[] = {64, 128, 256}, tiles[] = {8, 16, 32, 64}, lanes[] = {8, 16, 32}, chunks[] = {18, 19, 20, 21};
     static const float cells[] = {0.5f, 1.0f, 2.0f};
     bool photons = p->caust_photons || p->gi_photons;
-    for (int knob = 0; knob < 7; knob++) {
+    static const int nlanes[] = {1, 2, 4};
+    for (int knob = 0; knob < 8; knob++) {
         if ((knob == 2 || knob == 4) && !photons) continue;
-        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 3 : 2;
+        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 3 : knob == 7 ? 3 : 2;
         for (int k = 0; k < nv; k++) {
             sray_tuning t = cur;
             if (knob == 0) t.trace_block = blocks_[k];
@@ -1939,7 +2033,8 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
             else if (knob == 3) t.chunk_log2 = chunks[k];
             else if (knob == 4) t.gather_cell = cells[k];
             else if (knob == 5) t.graphs = k;
-            else t.persistent = k;
+            else if (knob == 6) t.persistent = k;
+            else t.lanes = nlanes[k];
             if (!memcmp(&t, &cur, sizeof t)) continue;
             c->tune = t;
             double ms = 0;

@@ -97,7 +97,7 @@ struct sray_gpu_ctx {
     };
     std::map<std::string, KStat> kstats;
     /* tuning of the renderers and photon grids */
-    sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 0};
+    sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 4};
     /* renderers: one for synchronous renders, one behind submit/poll */
     srt::Engine *plain = nullptr;
     srt::Engine *serve = nullptr;
@@ -138,10 +138,12 @@ hipEvent_t prof_event(sray_gpu_ctx *c);
 struct ProfScope {
     sray_gpu_ctx *c;
     const char *name;
+    hipStream_t s;
     hipEvent_t a = nullptr;
-    ProfScope(sray_gpu_ctx *cc, const char *n) : c(cc), name(n)
+    /* on the stream the launch goes to (the context's own by default) */
+    ProfScope(sray_gpu_ctx *cc, const char *n, hipStream_t st = nullptr) : c(cc), name(n), s(st ? st : cc->stream)
     {
-        if (c->profiling && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
+        if (c->profiling && (a = prof_event(c))) (void)hipEventRecord(a, s);
     }
     ~ProfScope()
     {
@@ -151,7 +153,7 @@ struct ProfScope {
             c->evpool.push_back(a);
             return;
         }
-        (void)hipEventRecord(b, c->stream);
+        (void)hipEventRecord(b, s);
         c->pending.push_back({name, a, b});
     }
 };

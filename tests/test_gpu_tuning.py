@@ -55,9 +55,31 @@ def test_sub_chunks_match_checker(ctx):
     ctx.release(sid)
 
 
+@pytest.mark.parametrize("config,variant,size", [(1, 4, (480, 270)), (2, 10, (400, 300)), (0, 0, (400, 300))])
+def test_lanes_give_the_same_image(ctx, config, variant, size):
+    """a pass spread over 2 or 4 streams renders the image and ray counts of
+    one stream.  Chunks of 2^16 pixels make these small frames several
+    chunks (the room's levels also outgrow a sub-chunk; the spheres' cannot)"""
+    hs = S.HostScene.generate(config, variant=variant)
+    sid = ctx.upload(hs)
+    p = S.render_params(*size, samples=(1, 2), max_var=0.05, iter=5, shadow_samples=2, seed=3)
+    ctx.set_tuning(chunk_log2=16, lanes=1)
+    try:
+        base, bst = ctx.render(sid, p)
+        for n in (2, 4):
+            ctx.set_tuning(lanes=n)
+            img, st = ctx.render(sid, p)
+            err = float(np.abs(img - base).max())
+            print(f"lanes={n}: max diff {err:.2e}")
+            assert err <= TOL and st.rays() == bst.rays(), n
+    finally:
+        ctx.set_tuning(lanes=4, chunk_log2=21)
+    ctx.release(sid)
+
+
 def test_bad_tuning_rejected(ctx):
     for kw in (dict(trace_block=96), dict(tile_w=12), dict(gather_lanes=4), dict(chunk_log2=22),
-               dict(gather_cell=3.0), dict(graphs=2), dict(persistent=2)):
+               dict(gather_cell=3.0), dict(graphs=2), dict(persistent=2), dict(lanes=3), dict(lanes=0)):
         with pytest.raises(S.SrayError):
             ctx.set_tuning(**kw)
 
@@ -73,7 +95,8 @@ def test_autotune(ctx):
     assert (kept.trace_block, kept.tile_w, kept.chunk_log2) == (t.trace_block, t.tile_w, t.chunk_log2)
     after, _ = ctx.render(sid, p)
     assert float(np.abs(after - before).max()) <= TOL
-    ctx.set_tuning(trace_block=128, tile_w=64, gather_lanes=16, chunk_log2=21, gather_cell=1.0, graphs=1, persistent=0)
+    ctx.set_tuning(trace_block=128, tile_w=64, gather_lanes=16, chunk_log2=21, gather_cell=1.0, graphs=1, persistent=0,
+                   lanes=4)
     ctx.release(sid)
 
 
