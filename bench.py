@@ -137,7 +137,7 @@ def tuning_arg(args, ctx):
           for k, v in (item.split("=") for item in args.tuning.split(","))}
     t = ctx.set_tuning(**kw)
     return {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2,
-            "graphs": t.graphs, "persistent": t.persistent, "lanes": t.lanes}
+            "graphs": t.graphs, "persistent": t.persistent, "lanes": t.lanes, "sparse_from": t.sparse_from}
 
 
 def look_at(pos, target, vfov_deg, shutter=0):
@@ -421,7 +421,7 @@ def main():
     if tuning is None and not args.no_autotune:
         t, tms = ctx.autotune(sid, p, rounds=2)
         tuning = {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2,
-                  "graphs": t.graphs, "persistent": t.persistent, "lanes": t.lanes}
+                  "graphs": t.graphs, "persistent": t.persistent, "lanes": t.lanes, "sparse_from": t.sparse_from}
         log(args, f"autotune: {tuning}, {tms:.2f} ms/frame")
     for _ in range(args.warmup):
         ctx.render_device(sid, p, out)

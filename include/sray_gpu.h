@@ -366,6 +366,9 @@ typedef struct sray_tuning {
     int32_t lanes;            /* 1, 2 or 4 HIP streams a render pass spreads its primary chunks over
                                  (each with its own bounce-level queues), so that one chunk's deep,
                                  sparse levels overlap another chunk's work */
+    int32_t sparse_from;      /* 0..64: first bounce level whose traversal kernels keep four triangles
+                                 in flight per lane (more registers, fewer waves; for levels with
+                                 too few rays to hide load latency); 64: none */
 } sray_tuning;
 int  sray_gpu_set_tuning(sray_gpu_ctx *ctx, const sray_tuning *t);
 int  sray_gpu_get_tuning(sray_gpu_ctx *ctx, sray_tuning *t);

@@ -118,7 +118,7 @@ __device__ __forceinline__ uint32_t first_run() { return (blockIdx.x * blockDim.
 __device__ __forceinline__ uint32_t run_stride() { return gridDim.x * blockDim.x; }
 
 /* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
-template <bool COUNT, bool UNI, int BS, bool PERSIST>
+template <bool COUNT, bool UNI, int BS, bool PERSIST, bool SPARSE>
 __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
                                                 int4 *__restrict__ hits, unsigned long long *counts)
 {
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
         /* the world ray is re-read from the queue when needed instead of
          * held in registers */
         const RayRecord<QRay> src{q, i};
-        HitRec h = traverse_src<false, COUNT, true>(S, src, live, &stk[threadIdx.x], BS, cnt);
+        HitRec h = traverse_src<false, COUNT, true, SPARSE>(S, src, live, &stk[threadIdx.x], BS, cnt);
         if (live) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
     }
     flush_counts<COUNT>(cnt, counts);
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
 
 /* occlusion of every shadow ray (any hit: cast_ray(ray) != NULL,
  * src/shader.cc:266), L slots per ray of the step */
-template <bool COUNT, bool UNI, int BS, bool PERSIST>
+template <bool COUNT, bool UNI, int BS, bool PERSIST, bool SPARSE>
 __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restrict__ sq,
                                                      const float4 *__restrict__ scon, StepN sn, int L,
                                                      uint8_t *__restrict__ occ, unsigned long long *counts)
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
          * occupancy already, and re-reading it would only add loads */
         const SceneDev S = T.scenes[UNI || !live ? 0u : T.reqs[__float_as_uint(d.w)].scene];
         const RayRegs src{v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w)};
-        HitRec h = traverse_src<true, COUNT, true>(S, src, live, &stk[threadIdx.x], BS, cnt);
+        HitRec h = traverse_src<true, COUNT, true, SPARSE>(S, src, live, &stk[threadIdx.x], BS, cnt);
         if (i < n) occ[i] = h.obj >= 0 ? 1 : 0;
     }
     flush_counts<COUNT>(cnt, counts);
@@ -788,27 +788,38 @@ __global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__
 
 
 /* launchable instances of the tunable kernels */
-template <bool COUNT, bool UNI, bool P> auto trace_q_of_p(int bs) -> decltype(&k_trace_q<COUNT, UNI, 128, P>)
+/* launchable instances of the tunable kernels (the SPARSE instances hold
+ * more registers: they are launched for the deep, sparse bounce levels) */
+template <bool COUNT, bool UNI, bool P, bool SP>
+auto trace_q_of_p(int bs) -> decltype(&k_trace_q<COUNT, UNI, 128, P, SP>)
 {
-    return bs == 64    ? k_trace_q<COUNT, UNI, 64, P>
-           : bs == 256 ? k_trace_q<COUNT, UNI, 256, P>
-                       : k_trace_q<COUNT, UNI, 128, P>;
-}
-template <bool COUNT, bool UNI> auto trace_q_of(int bs, bool persist) -> decltype(&k_trace_q<COUNT, UNI, 128, false>)
-{
-    return persist ? trace_q_of_p<COUNT, UNI, true>(bs) : trace_q_of_p<COUNT, UNI, false>(bs);
-}
-template <bool COUNT, bool UNI, bool P>
-auto trace_shadow_of_p(int bs) -> decltype(&k_trace_shadow<COUNT, UNI, 128, P>)
-{
-    return bs == 64    ? k_trace_shadow<COUNT, UNI, 64, P>
-           : bs == 256 ? k_trace_shadow<COUNT, UNI, 256, P>
-                       : k_trace_shadow<COUNT, UNI, 128, P>;
+    return bs == 64    ? k_trace_q<COUNT, UNI, 64, P, SP>
+           : bs == 256 ? k_trace_q<COUNT, UNI, 256, P, SP>
+                       : k_trace_q<COUNT, UNI, 128, P, SP>;
 }
 template <bool COUNT, bool UNI>
-auto trace_shadow_of(int bs, bool persist) -> decltype(&k_trace_shadow<COUNT, UNI, 128, false>)
+auto trace_q_of(int bs, bool persist, bool sparse) -> decltype(&k_trace_q<COUNT, UNI, 128, false, false>)
 {
-    return persist ? trace_shadow_of_p<COUNT, UNI, true>(bs) : trace_shadow_of_p<COUNT, UNI, false>(bs);
+    if constexpr (!COUNT) {
+        if (sparse) return persist ? trace_q_of_p<COUNT, UNI, true, true>(bs) : trace_q_of_p<COUNT, UNI, false, true>(bs);
+    }
+    return persist ? trace_q_of_p<COUNT, UNI, true, false>(bs) : trace_q_of_p<COUNT, UNI, false, false>(bs);
+}
+template <bool COUNT, bool UNI, bool P, bool SP>
+auto trace_shadow_of_p(int bs) -> decltype(&k_trace_shadow<COUNT, UNI, 128, P, SP>)
+{
+    return bs == 64    ? k_trace_shadow<COUNT, UNI, 64, P, SP>
+           : bs == 256 ? k_trace_shadow<COUNT, UNI, 256, P, SP>
+                       : k_trace_shadow<COUNT, UNI, 128, P, SP>;
+}
+template <bool COUNT, bool UNI>
+auto trace_shadow_of(int bs, bool persist, bool sparse) -> decltype(&k_trace_shadow<COUNT, UNI, 128, false, false>)
+{
+    if constexpr (!COUNT) {
+        if (sparse)
+            return persist ? trace_shadow_of_p<COUNT, UNI, true, true>(bs) : trace_shadow_of_p<COUNT, UNI, false, true>(bs);
+    }
+    return persist ? trace_shadow_of_p<COUNT, UNI, true, false>(bs) : trace_shadow_of_p<COUNT, UNI, false, false>(bs);
 }
 template <bool UNI> auto gather_of(int lanes) -> decltype(&k_gather<UNI, 16>)
 {
@@ -966,6 +977,7 @@ struct Engine {
     std::vector<Lane> lanes;
     hipEvent_t tables_ready = nullptr;
     int nlanes = 1; /* lanes a pass spreads its primary chunks over */
+    int sparse_from = 64; /* first bounce level traced by the SPARSE kernel instances */
     std::map<const void *, uint32_t> pgrid; /* persistent grid per traversal kernel instance */
     /* captured descents, keyed by everything their launches depend on */
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
@@ -1125,6 +1137,7 @@ void apply_tuning(Engine &E)
     E.use_graphs = t.graphs != 0;
     E.persistent = t.persistent != 0;
     E.nlanes = t.lanes;
+    E.sparse_from = t.sparse_from;
 }
 
 /* places a request in the arena and the active list */
@@ -1338,9 +1351,9 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
     {
         ProfScope ps(c, "k_trace_q", st);
         /* the traversal reads only the scene: one scene = the uniform variant */
-        const bool P = E.persistent;
-        auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs, P) : trace_q_of<true, false>(bs, P))
-                               : (E.one_scene ? trace_q_of<false, true>(bs, P) : trace_q_of<false, false>(bs, P));
+        const bool P = E.persistent, SP = b >= E.sparse_from;
+        auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs, P, SP) : trace_q_of<true, false>(bs, P, SP))
+                               : (E.one_scene ? trace_q_of<false, true>(bs, P, SP) : trace_q_of<false, false>(bs, P, SP));
         kern<<<tgrid(kern, cap), bs, 0, st>>>(LV.T, q, sn, dptr<int4>(ln.hits), rc6);
     }
     {
@@ -1363,11 +1376,11 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
     }
     if (E.L) {
         ProfScope ps(c, "k_trace_shadow", st);
-        const bool P = E.persistent;
-        auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs, P)
-                                              : trace_shadow_of<true, false>(bs, P))
-                               : (E.one_scene ? trace_shadow_of<false, true>(bs, P)
-                                              : trace_shadow_of<false, false>(bs, P));
+        const bool P = E.persistent, SP = b >= E.sparse_from;
+        auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs, P, SP)
+                                              : trace_shadow_of<true, false>(bs, P, SP))
+                               : (E.one_scene ? trace_shadow_of<false, true>(bs, P, SP)
+                                              : trace_shadow_of<false, false>(bs, P, SP));
         kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, st>>>(LV.T, dptr<SRay>(ln.sq), dptr<float4>(ln.scon),
                                                                    sn, E.L, dptr<uint8_t>(ln.occ), rc6 + 6);
     }
@@ -1530,9 +1543,9 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
     int rc;
     if ((rc = build_tables(E))) return rc;
     if ((rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) || (rc = ensure(c, E.slotrays, MAX_SLOTS * 8)) ||
-        (rc = ensure(c, c->nsel, 4)) || (rc = ensure(c, E.errf, 4)) ||
-        (rc = ensure(c, E.stats, 8 * sizeof(unsigned long long))) ||
-        (rc = ensure(c, E.rcounts, 12 * sizeof(unsigned long long))))
+        (rc = ensure(c, c->nsel, 4)) || (rc = ensure_zero(c, E.errf, 4)) ||
+        (rc = ensure_zero(c, E.stats, 8 * sizeof(unsigned long long))) ||
+        (rc = ensure_zero(c, E.rcounts, 12 * sizeof(unsigned long long))))
         return rc;
     if (E.sub.cap == 0 && (rc = ensure(c, E.subpix, 16))) return rc;
     /* primary chunks: with several lanes a pass is spread over them, each
@@ -1870,8 +1883,9 @@ int do_submit_many(sray_gpu_ctx *c, const sray_request *r, uint32_t n, float *co
             E->started = true;
             E->worker = std::thread(serve_loop, c, E);
         }
-        if ((rc = ensure(c, E->stats, 8 * sizeof(unsigned long long))) || (rc = ensure(c, E->errf, 4)) ||
-            (rc = ensure(c, E->rcounts, 12 * sizeof(unsigned long long))))
+        /* the serving engine's counters accumulate over its lifetime */
+        if ((rc = ensure_zero(c, E->stats, 8 * sizeof(unsigned long long))) || (rc = ensure_zero(c, E->errf, 4)) ||
+            (rc = ensure_zero(c, E->rcounts, 12 * sizeof(unsigned long long))))
             return rc;
     }
     Engine *E = c->serve;
@@ -1983,7 +1997,7 @@ int check_tuning(sray_gpu_ctx *c, const sray_tuning &t)
               (t.gather_lanes == 8 || t.gather_lanes == 16 || t.gather_lanes == 32) && t.chunk_log2 >= 16 &&
               t.chunk_log2 <= 21 && (t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f) &&
               (t.graphs == 0 || t.graphs == 1) && (t.persistent == 0 || t.persistent == 1) &&
-              (t.lanes == 1 || t.lanes == 2 || t.lanes == 4);
+              (t.lanes == 1 || t.lanes == 2 || t.lanes == 4) && t.sparse_from >= 0 && t.sparse_from <= 64;
     return ok ? SRAY_OK : fail(c, SRAY_ERR_ARG, "tuning value outside the supported set");
 }
 
@@ -2021,10 +2035,11 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
     static const int blocks_[] = {64, 128, 256}, tiles[] = {8, 16, 32, 64}, lanes[] = {8, 16, 32}, chunks[] = {18, 19, 20, 21};
     static const float cells[] = {0.5f, 1.0f, 2.0f};
     bool photons = p->caust_photons || p->gi_photons;
-    static const int nlanes[] = {1, 2, 4};
-    for (int knob = 0; knob < 8; knob++) {
+    static const int nlanes[] = {1, 2, 4}, sparse[] = {0, 1, 2, 64};
+    for (int knob = 0; knob < 9; knob++) {
         if ((knob == 2 || knob == 4) && !photons) continue;
-        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 3 : knob == 7 ? 3 : 2;
+        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 3 : knob == 7 ? 3
+               : knob == 8 ? 4 : 2;
         for (int k = 0; k < nv; k++) {
             sray_tuning t = cur;
             if (knob == 0) t.trace_block = blocks_[k];
@@ -2034,7 +2049,8 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
             else if (knob == 4) t.gather_cell = cells[k];
             else if (knob == 5) t.graphs = k;
             else if (knob == 6) t.persistent = k;
-            else t.lanes = nlanes[k];
+            else if (knob == 7) t.lanes = nlanes[k];
+            else t.sparse_from = sparse[k];
             if (!memcmp(&t, &cur, sizeof t)) continue;
             c->tune = t;
             double ms = 0;

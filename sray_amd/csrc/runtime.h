@@ -97,7 +97,7 @@ struct sray_gpu_ctx {
     };
     std::map<std::string, KStat> kstats;
     /* tuning of the renderers and photon grids */
-    sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 4};
+    sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 4, 64};
     /* renderers: one for synchronous renders, one behind submit/poll */
     srt::Engine *plain = nullptr;
     srt::Engine *serve = nullptr;
@@ -130,6 +130,9 @@ int fail(sray_gpu_ctx *c, int code, const char *fmt, ...);
 
 /* grows b to at least bytes (contents are not kept) */
 int ensure(sray_gpu_ctx *c, DevBuf &b, size_t bytes);
+/* as ensure, and a buffer it (re)allocates is cleared on the context's
+ * stream (counters that accumulate across calls) */
+int ensure_zero(sray_gpu_ctx *c, DevBuf &b, size_t bytes);
 void release(DevBuf &b);
 inline uint32_t blocks(size_t n, int bs) { return (uint32_t)((n + (size_t)bs - 1) / (size_t)bs); }
 

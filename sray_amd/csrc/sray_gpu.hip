@@ -448,6 +448,15 @@ int ensure(sray_gpu_ctx *c, DevBuf &b, size_t bytes)
     return SRAY_OK;
 }
 
+int ensure_zero(sray_gpu_ctx *c, DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes) return SRAY_OK;
+    int rc = ensure(c, b, bytes);
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(b.p, 0, b.bytes, c->stream));
+    return SRAY_OK;
+}
+
 void release(DevBuf &b)
 {
     if (b.p) (void)hipFree(b.p);

@@ -333,6 +333,13 @@ template <class T> __device__ __forceinline__ const T *read_lane_p(const T *p, i
  * rays, whose leaves end at the first hit when walked by their own lane. */
 template <bool ANY> constexpr int coop_lanes() { return ANY ? 16 : 8; }
 
+/* Triangles a lane keeps in flight when it walks its own leaf.  The deep
+ * bounce levels hold few rays (two or three waves per SIMD), too few to
+ * hide one load latency per triangle with other waves' work; the
+ * SPARSE instances, launched for those levels, load SPARSE_TRIS triangles
+ * at once, at the cost of registers the dense levels need for occupancy. */
+constexpr int SPARSE_TRIS = 4;
+
 /* Scene::cast_ray over the octrees.  ANY: stop at the first hit (the
  * reference's shadow test cast_ray(ray) != NULL, src/shader.cc:266).
  * stk points at this lane's entry 0; consecutive entries are `stride`
@@ -350,7 +357,7 @@ template <bool ANY> constexpr int coop_lanes() { return ANY ? 16 : 8; }
  * waves can test leaves together.  A leaf tested by the wave yields the
  * same winner as tested by its own lane: the smallest (t, item position)
  * key, merged with the rule the serial loop applies. */
-template <bool ANY, bool COUNT, bool COOP, class SRC>
+template <bool ANY, bool COUNT, bool COOP, bool SPARSE, class SRC>
 __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src, bool live, uint32_t *stk,
                                                int stride, TraceCounts &cnt)
 {
@@ -509,8 +516,37 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
         }
         if (in_leaf) {
             /* mesh leaf: all its triangles, from the leaf-ordered copies
-             * (one per step: a second triangle's registers in flight cost
-             * a wave of occupancy and more than its latency hiding gains) */
+             * (dense levels: one per step, as a second triangle's registers
+             * in flight cost a wave of occupancy and more than its latency
+             * hiding gains; sparse levels: SPARSE_TRIS per step, tested in
+             * order) */
+            if (SPARSE) {
+                constexpr int U = SPARSE_TRIS;
+                for (; item_cur + (uint32_t)U <= item_end && !done; item_cur += U) {
+                    const float4 *L = S.ltri + (size_t)item_cur * 3;
+                    float4 V[3 * U];
+#pragma unroll
+                    for (int u = 0; u < 3 * U; u++) V[u] = L[u];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        float t;
+                        if (!done && tri_t(V[3 * u], V[3 * u + 1], V[3 * u + 2], r, t,
+                                           ANY ? FLT_MAX : fminf(m_t, best.t))) {
+                            const uint32_t k = item_cur + (uint32_t)u;
+                            if (ANY) {
+                                best.t = t;
+                                best.tri = S.items[k];
+                                best.obj = cur_mesh;
+                                done = true;
+                            } else if (t < m_t || (t == m_t && k < m_rank)) {
+                                m_t = t;
+                                m_rank = k;
+                            }
+                        }
+                    }
+                }
+                if (done) continue;
+            }
             for (; item_cur < item_end;) {
                 const float4 *L = S.ltri + (size_t)item_cur * 3;
                 float4 A = L[0], B = L[1], C = L[2];
@@ -620,7 +656,7 @@ __device__ __forceinline__ HitRec traverse(const SceneDev &S, V3 wo, V3 wd, int 
                                            TraceCounts &cnt)
 {
     RayRegs src{wo, wd, time};
-    return traverse_src<ANY, COUNT, false>(S, src, true, stk, stride, cnt);
+    return traverse_src<ANY, COUNT, false, false>(S, src, true, stk, stride, cnt);
 }
 
 } // namespace srd

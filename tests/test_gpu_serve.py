@@ -212,3 +212,23 @@ def test_submit_many_equals_alone(ctx):
         ctx.submit_many(bad)
     for s in sids:
         ctx.release(s)
+
+
+def test_serving_counters_start_at_zero():
+    """a new context's serving engine starts its ray counters from zero,
+    also when its counter buffers land on memory freed dirty just before"""
+    c = S.Context(0)
+    try:
+        hs = S.HostScene.generate(0)
+        sid = c.upload(hs)
+        p = S.render_params(160, 120, iter=5, seed=1)
+        alone, ast = c.render(sid, p)
+        junk = [c.to_device(np.full(n, 0xFF, np.uint8)) for n in (5, 80, 120, 256) for _ in range(8)]
+        for d in junk:
+            c.free(d)
+        img, res = c.poll(c.submit(sid, p))
+        assert float(np.abs(img - alone).max()) <= TOL
+        assert res.rays == ast.rays()
+        c.release(sid)
+    finally:
+        c.close()

@@ -18,7 +18,7 @@ TOL = 1e-6
 def _variants():
     return [dict(trace_block=64), dict(trace_block=256), dict(tile_w=8), dict(tile_w=16),
             dict(chunk_log2=16), dict(gather_lanes=8), dict(gather_lanes=32), dict(gather_cell=0.5),
-            dict(gather_cell=2.0), dict(graphs=0), dict(persistent=1)]
+            dict(gather_cell=2.0), dict(graphs=0), dict(persistent=1), dict(sparse_from=0), dict(sparse_from=2)]
 
 
 def test_tunings_give_the_same_image(ctx):
@@ -79,7 +79,8 @@ def test_lanes_give_the_same_image(ctx, config, variant, size):
 
 def test_bad_tuning_rejected(ctx):
     for kw in (dict(trace_block=96), dict(tile_w=12), dict(gather_lanes=4), dict(chunk_log2=22),
-               dict(gather_cell=3.0), dict(graphs=2), dict(persistent=2), dict(lanes=3), dict(lanes=0)):
+               dict(gather_cell=3.0), dict(graphs=2), dict(persistent=2), dict(lanes=3), dict(lanes=0),
+               dict(sparse_from=-1), dict(sparse_from=65)):
         with pytest.raises(S.SrayError):
             ctx.set_tuning(**kw)
 
@@ -96,7 +97,7 @@ def test_autotune(ctx):
     after, _ = ctx.render(sid, p)
     assert float(np.abs(after - before).max()) <= TOL
     ctx.set_tuning(trace_block=128, tile_w=64, gather_lanes=16, chunk_log2=21, gather_cell=1.0, graphs=1, persistent=0,
-                   lanes=4)
+                   lanes=4, sparse_from=64)
     ctx.release(sid)
 
 
