@@ -962,6 +962,11 @@ struct Engine {
     struct Lane {
         hipStream_t s = nullptr; /* lane 0: the context's stream; others: created here */
         hipEvent_t done = nullptr;
+        /* second stream: a level's shadow rays and accumulation run there,
+         * beside the next level's compaction and closest-hit trace (evA:
+         * the level is shaded; evB: its accumulation is done) */
+        hipStream_t s2 = nullptr;
+        hipEvent_t evA = nullptr, evB = nullptr;
         std::vector<DevBuf> qlev;
         DevBuf hits, direct, sq, scon, rslot, occ, kids, kflag, acc, greq;
         DevBuf lvcnt, cmpb; /* per-level ray counts; compaction block counts / offsets */
@@ -1229,8 +1234,10 @@ void retire(Engine &E, uint32_t s)
 /* drops every request in flight (after a device error) */
 void abort_all(Engine &E)
 {
-    for (Engine::Lane &ln : E.lanes)
+    for (Engine::Lane &ln : E.lanes) {
+        if (ln.s2) (void)hipStreamSynchronize(ln.s2);
         if (ln.s) (void)hipStreamSynchronize(ln.s);
+    }
     (void)hipStreamSynchronize(E.c->stream);
     for (uint32_t s = 0; s < E.hi; s++)
         if (E.slot_job[s]) {
@@ -1329,13 +1336,19 @@ template <typename K> uint32_t persistent_grid(Engine &E, K kern, int bs)
 /* one level step: rays [off, off + chunk) of level b's queue (count on the
  * device) through trace, shade, gather, shadow trace and accumulation,
  * and (below the deepest level) the stable compaction of their children
- * into level b + 1's queue, which also writes that level's count */
-int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
+ * into level b + 1's queue, which also writes that level's count.
+ *
+ * The photon gathers, the shadow trace and the accumulation go to the
+ * lane's second stream once the level is shaded, so they run beside the
+ * compaction and the next step's closest-hit trace; the next step's
+ * shading (which reuses their buffers) waits for them (first = no earlier
+ * step to wait for). */
+int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, bool first)
 {
     Engine &E = *LV.E;
     Engine::Lane &ln = *LV.ln;
     sray_gpu_ctx *c = E.c;
-    const hipStream_t st = ln.s;
+    const hipStream_t st = ln.s, st2 = ln.s2;
     const StepN sn{dptr<uint32_t>(ln.lvcnt) + b, off, cap};
     const QRay *q = dptr<QRay>(ln.qlev[b]) + off;
     unsigned long long *rc6 = dptr<unsigned long long>(E.rcounts);
@@ -1356,6 +1369,7 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
                                : (E.one_scene ? trace_q_of<false, true>(bs, P, SP) : trace_q_of<false, false>(bs, P, SP));
         kern<<<tgrid(kern, cap), bs, 0, st>>>(LV.T, q, sn, dptr<int4>(ln.hits), rc6);
     }
+    if (!first) HIPCHK(c, hipStreamWaitEvent(st, ln.evB, 0));
     {
         ProfScope ps(c, "k_shade", st);
         auto kern = E.photons ? (E.uni ? k_shade<true, true, true>
@@ -1368,28 +1382,31 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap)
                                                dptr<uint8_t>(ln.kflag), dptr<GatherReq>(ln.greq), stats, srays,
                                                dptr<uint32_t>(E.errf));
     }
+    HIPCHK(c, hipEventRecord(ln.evA, st));
+    HIPCHK(c, hipStreamWaitEvent(st2, ln.evA, 0));
     if (E.photons) {
-        ProfScope ps(c, "k_gather", st);
+        ProfScope ps(c, "k_gather", st2);
         auto kern = E.uni ? gather_of<true>(E.gather_lanes) : gather_of<false>(E.gather_lanes);
         const uint32_t g = std::min<uint32_t>(blocks((size_t)cap * E.gather_lanes, 256), 8192u);
-        kern<<<g, 256, 0, st>>>(LV.T, q, dptr<GatherReq>(ln.greq), sn, dptr<float4>(ln.direct));
+        kern<<<g, 256, 0, st2>>>(LV.T, q, dptr<GatherReq>(ln.greq), sn, dptr<float4>(ln.direct));
     }
     if (E.L) {
-        ProfScope ps(c, "k_trace_shadow", st);
+        ProfScope ps(c, "k_trace_shadow", st2);
         const bool P = E.persistent, SP = b >= E.sparse_from;
         auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs, P, SP)
                                               : trace_shadow_of<true, false>(bs, P, SP))
                                : (E.one_scene ? trace_shadow_of<false, true>(bs, P, SP)
                                               : trace_shadow_of<false, false>(bs, P, SP));
-        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, st>>>(LV.T, dptr<SRay>(ln.sq), dptr<float4>(ln.scon),
-                                                                   sn, E.L, dptr<uint8_t>(ln.occ), rc6 + 6);
+        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, st2>>>(LV.T, dptr<SRay>(ln.sq), dptr<float4>(ln.scon),
+                                                                    sn, E.L, dptr<uint8_t>(ln.occ), rc6 + 6);
     }
     {
-        ProfScope ps(c, "k_accum", st);
-        k_accum<<<small_grid, 256, 0, st>>>(dptr<uint32_t>(ln.rslot), dptr<float4>(ln.direct),
-                                                   dptr<float4>(ln.scon), dptr<uint8_t>(ln.occ), sn, E.L,
-                                                   dptr<float4>(ln.acc));
+        ProfScope ps(c, "k_accum", st2);
+        k_accum<<<small_grid, 256, 0, st2>>>(dptr<uint32_t>(ln.rslot), dptr<float4>(ln.direct),
+                                                    dptr<float4>(ln.scon), dptr<uint8_t>(ln.occ), sn, E.L,
+                                                    dptr<float4>(ln.acc));
     }
+    HIPCHK(c, hipEventRecord(ln.evB, st2));
     if (b < E.maxiter) {
         ProfScope ps(c, "k_compact", st);
         uint32_t *bc = dptr<uint32_t>(ln.cmpb);
@@ -1412,9 +1429,12 @@ int descend(Level &LV, int b, uint32_t off, uint32_t cap)
     int rc;
     uint64_t bound = cap;
     for (int d = b; d <= E.maxiter && bound > 0; d++) {
-        if ((rc = launch_step(LV, d, d == b ? off : 0u, (uint32_t)std::min<uint64_t>(bound, E.chunk)))) return rc;
+        if ((rc = launch_step(LV, d, d == b ? off : 0u, (uint32_t)std::min<uint64_t>(bound, E.chunk), d == b)))
+            return rc;
         bound = std::min<uint64_t>(bound * (uint64_t)E.Kb, E.chunk);
     }
+    /* join: what follows on the lane's stream sees the last accumulation */
+    HIPCHK(E.c, hipStreamWaitEvent(LV.ln->s, LV.ln->evB, 0));
     return SRAY_OK;
 }
 
@@ -1575,6 +1595,9 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
         if (l == 0) ln.s = c->stream;
         else if (!ln.s) HIPCHK(c, hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking)); /* default priority */
         if (!ln.done) HIPCHK(c, hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
+        if (!ln.s2) HIPCHK(c, hipStreamCreateWithFlags(&ln.s2, hipStreamNonBlocking)); /* default priority */
+        if (!ln.evA) HIPCHK(c, hipEventCreateWithFlags(&ln.evA, hipEventDisableTiming));
+        if (!ln.evB) HIPCHK(c, hipEventCreateWithFlags(&ln.evB, hipEventDisableTiming));
         if (ln.qlev.size() < (size_t)E.maxiter + 1) ln.qlev.resize((size_t)E.maxiter + 1);
         if ((rc = ensure(c, ln.qlev[0], (size_t)mchunk * sizeof(QRay)))) return rc;
         for (int b = 1; b <= E.maxiter; b++)
@@ -1714,6 +1737,12 @@ void engine_free(Engine *E)
             (void)hipStreamDestroy(ln.s);
         }
         if (ln.done) (void)hipEventDestroy(ln.done);
+        if (ln.s2) {
+            (void)hipStreamSynchronize(ln.s2);
+            (void)hipStreamDestroy(ln.s2);
+        }
+        if (ln.evA) (void)hipEventDestroy(ln.evA);
+        if (ln.evB) (void)hipEventDestroy(ln.evB);
         DevBuf *lb[] = {&ln.hits, &ln.direct, &ln.sq, &ln.scon, &ln.rslot, &ln.occ, &ln.kids, &ln.kflag, &ln.acc,
                         &ln.greq, &ln.lvcnt, &ln.cmpb};
         for (DevBuf *b : lb) release(*b);
