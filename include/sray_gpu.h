@@ -360,7 +360,8 @@ typedef struct sray_tuning {
     int32_t chunk_log2;       /* 16..21: rays per bounce-level sub-chunk = 2^chunk_log2 */
     float   gather_cell;      /* photon-grid cell size in gather radii: 0.5, 1 or 2 */
     int32_t graphs;           /* 1: each primary chunk's bounce levels are replayed as a captured
-                                 HIP graph; 0: launched kernel by kernel (same kernels, same order) */
+                                 HIP graph when the pass runs on one lane; 0: launched kernel by
+                                 kernel (same kernels, same order) */
     int32_t persistent;       /* 1: traversal kernels run as many waves as stay resident, each
                                  looping over runs of 64 rays; 0: one run per wave */
     int32_t lanes;            /* 1, 2 or 4 HIP streams a render pass spreads its primary chunks over
@@ -466,8 +467,10 @@ int  sray_gpu_memcpy(sray_gpu_ctx *ctx, void *dst, const void *src, size_t bytes
 int  sray_gpu_synchronize(sray_gpu_ctx *ctx);
 
 /* Per-kernel device time, measured with HIP events around each launch on
- * the context's stream while profiling is on (no reference counterpart;
- * the reference only prints total wall time, src/sray.cc:67-76). */
+ * the stream it runs on while profiling is on (no reference counterpart;
+ * the reference only prints total wall time, src/sray.cc:67-76).  While
+ * profiling, renders launch kernel by kernel (no graphs), and a bounce
+ * level's kernels run in order on one stream per lane. */
 typedef struct sray_kernel_stat {
     char     name[32];
     uint64_t launches;

@@ -1348,7 +1348,9 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, bool first)
     Engine &E = *LV.E;
     Engine::Lane &ln = *LV.ln;
     sray_gpu_ctx *c = E.c;
-    const hipStream_t st = ln.s, st2 = ln.s2;
+    /* with per-kernel timing on, every kernel of the step goes to the lane's
+     * own stream: an event-timed launch then has the device to itself */
+    const hipStream_t st = ln.s, st2 = c->profiling ? ln.s : ln.s2;
     const StepN sn{dptr<uint32_t>(ln.lvcnt) + b, off, cap};
     const QRay *q = dptr<QRay>(ln.qlev[b]) + off;
     unsigned long long *rc6 = dptr<unsigned long long>(E.rcounts);
@@ -1477,7 +1479,7 @@ bool may_overflow(const Engine &E, uint32_t m)
 /* ray generation and the descent of the primary chunk [c0, c0 + m): as
  * kernel launches, or as one captured graph replayed (same kernels, same
  * order) */
-int launch_chunk(Level &LV, uint32_t m)
+int launch_chunk(Level &LV, uint32_t m, bool one_lane)
 {
     Engine &E = *LV.E;
     Engine::Lane &ln = *LV.ln;
@@ -1493,7 +1495,11 @@ int launch_chunk(Level &LV, uint32_t m)
         }
         return descend(LV, 0, 0, m);
     };
-    if (!E.use_graphs || c->profiling) return body();
+    /* graphs only for a pass on one lane: measured, graphs replayed on two
+     * lanes at once could run half again slower than the same launches made
+     * kernel by kernel (whose speed they match there), depending on the
+     * order the graphs were instantiated in */
+    if (!E.use_graphs || c->profiling || !one_lane) return body();
     std::vector<uint64_t> key = {LV.c0, m, (uint64_t)E.maxiter, (uint64_t)E.L, (uint64_t)E.K, E.chunk,
                                  (uint64_t)E.trace_block, (uint64_t)E.gather_lanes,
                                  (uint64_t)(E.photons | E.uni << 1 | E.one_scene << 2 | E.counting << 3 |
@@ -1633,7 +1639,7 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
             const uint32_t c0 = j * mchunk, m = std::min(mchunk, E.nact - c0);
             Engine::Lane &ln = E.lanes[j - g0];
             Level LV{&E, &ln, T, A, c0};
-            if ((rc = launch_chunk(LV, m))) return rc;
+            if ((rc = launch_chunk(LV, m, nl == 1))) return rc;
             if (!ovf)
                 finish<<<blocks(m, 256), 256, 0, ln.s>>>(T, A, c0, m, dptr<float4>(ln.acc), dptr<float4>(E.fb),
                                                          dptr<float4>(E.subpix), dptr<int>(E.nsamp),
