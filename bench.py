@@ -2,9 +2,10 @@
 
 A step is one full frame of the traversal-bound heightfield workload
 (1000x500-quad fBm terrain = 1,000,000 triangles + 20 glass + 20 mirror
-spheres; 1920x1080, 1 sample/pixel, recursion depth 6, mesh octree depth 8 /
-32 items): primary, shadow, reflect and refract rays through the wavefront
-path, scene and octrees resident in HBM before the timed region.
+spheres; 1920x1080, 1 sample/pixel, recursion depth 6; mesh octree leaf size
+and depth picked by the octree autotuner among 32/8 (the reference's), 48/8,
+64/7 and 64/8): primary, shadow, reflect and refract rays through the
+wavefront path, scene and octrees resident in HBM before the timed region.
 
 Prints one JSON line with the value, a roofline object for the dominant
 kernel (the closest-hit traversal k_trace_q: algorithmic bytes from the
@@ -24,6 +25,13 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+# A render lane uses two HIP streams (4 lanes by default).  With HIP's
+# default of 4 hardware queues per process, streams share queues and wait
+# behind each other's event waits; 8 queues give each stream its own.  Set
+# before the HIP runtime starts (it starts when the library is loaded).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
@@ -48,6 +56,8 @@ def parse():
     ap.add_argument("--tuning", default=None,
                     help="set the tuning (heightfield: instead of searching), e.g. "
                          "trace_block=64,tile_w=64,chunk_log2=21,graphs=1,persistent=0")
+    ap.add_argument("--octree", default=None,
+                    help="heightfield: mesh octree 'items,depth' instead of searching (the reference's is 32,8)")
     ap.add_argument("--frames", type=int, default=None, help="room / spheres: frames in flight (8 / 64)")
     ap.add_argument("--scenes", type=int, default=None, help="batch: scenes (256); serve: resident scenes (32)")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrival rate (req/s)")
@@ -411,11 +421,24 @@ def main():
     t0 = time.perf_counter()
     hs = S.HostScene.generate(config, variant=variant)
     ctx = S.Context(0)
-    sid = ctx.upload(hs)
+    p = S.render_params(W, H, samples=samples, iter=it)
+    # the mesh octree's leaf size and depth: given, searched (untimed; the
+    # candidates change only which of two exactly equal hit distances
+    # wins), or the reference's
+    desc = S.abi.SceneDesc()
+    ctypes.memmove(ctypes.byref(desc), hs.desc_ptr(), ctypes.sizeof(desc))
+    if args.octree:
+        desc.mesh_oct_items, desc.mesh_oct_depth = (int(v) for v in args.octree.split(","))
+    elif args.tuning is None and not args.no_autotune:
+        cands = [(32, 8), (48, 8), (64, 7), (64, 8)]
+        best, oms = ctx.autotune_octree(ctypes.pointer(desc), p, cands, rounds=2)
+        desc.mesh_oct_items, desc.mesh_oct_depth = cands[best]
+        log(args, f"octree autotune: {cands[best]}, {oms:.2f} ms/frame")
+    sid = ctx.upload(ctypes.pointer(desc))
+    octree = {"mesh_items": desc.mesh_oct_items, "mesh_depth": desc.mesh_oct_depth}
     upload_s = time.perf_counter() - t0
     log(args, f"scene {wl}: {hs.desc.num_triangles} triangles, {hs.desc.num_objects} objects, "
-              f"upload+octree build {upload_s:.2f} s, {ctx.scene_bytes(sid) / 2**20:.1f} MiB resident")
-    p = S.render_params(W, H, samples=samples, iter=it)
+              f"upload+octree build {upload_s:.2f} s, {ctx.scene_bytes(sid) / 2**20:.1f} MiB resident, octree {octree}")
     out = ctx.malloc(W * H * 4 * 4)
     tuning = tuning_arg(args, ctx)
     if tuning is None and not args.no_autotune:
@@ -495,7 +518,7 @@ def main():
         "config": {"workload": f"{wl}: {hs.desc.num_triangles} triangles + {hs.desc.num_objects - 1} spheres, "
                                f"{W}x{H}, {samples[0]}-{samples[1]} spp, depth {it}",
                    "rays_per_step": rays // args.steps, "device_ms_per_step": round(dev_ms / args.steps, 3),
-                   "tuning": tuning},
+                   "tuning": tuning, "octree": octree},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_trace_q", "avg_launch_ms": round(avg_launch_ms, 4),
