@@ -29,7 +29,7 @@ extern "C" {
 #define SRAY_ERR_NOMEM      -3   /* allocation failed or admission cap exceeded */
 #define SRAY_ERR_NOTFOUND   -4   /* unknown scene / photon map / ticket */
 #define SRAY_ERR_LIMIT      -5   /* a compiled limit (stack depth, ior stack) is exceeded */
-#define SRAY_ERR_PARSE      -6   /* scene XML could not be parsed/loaded */
+#define SRAY_ERR_PARSE      -6   /* scene XML or a photon dump could not be parsed/loaded */
 #define SRAY_ERR_PENDING    -7   /* sray_gpu_poll: request not finished yet */
 #define SRAY_ERR_NODEVICE   -8   /* no HIP device */
 
@@ -456,6 +456,40 @@ int  sray_scene_generate(int config, uint64_t seed, int variant, sray_host_scene
 /* reference-format XML of a host scene (%.9g floats); *len receives the
  * size including the terminating NUL, buf may be NULL to query it */
 int  sray_scene_write_xml(const sray_host_scene *hs, char *buf, size_t cap, size_t *len);
+
+/* Photon-map wire format.  Replaces PhotonMap::dump (src/pmap.cc:180-205):
+ * writes "energy %f\n", then one line per photon,
+ * "<%f %f %f> <%f %f %f> <%f %f %f> <%f %f %f>\n" (position, direction,
+ * normal, colour rgb), as the reference's printf. */
+int  sray_photon_dump(const char *path, const sray_photon *p, uint32_t n, float energy);
+/* Replaces PhotonMap::restore (src/pmap.cc:207-251): reads a dump (the
+ * reference's scanf formats; a short line is an error) and scales every
+ * colour by energy / (the dump's energy).  *out (NULL when *n is 0) is
+ * freed with sray_photon_free; the photons go to the device with
+ * sray_gpu_pmap_upload. */
+int  sray_photon_restore(const char *path, float energy, sray_photon **out, uint32_t *n);
+void sray_photon_free(sray_photon *p);
+/* Replaces Image::save (src/img.cc:80-83, libimago's float formats, which
+ * are not available here): the RGBA float framebuffer (row-major, top row
+ * first) as a PFM file, "PF\n<w> <h>\n-1.0\n" then RGB float32
+ * little-endian, bottom row first. */
+int  sray_image_write_pfm(const char *path, const float *rgba, int width, int height);
+/* The backend status protocol (emit_status, src/render.cc:279-289): one
+ * record is "%c %6d %6d %6d %6d\n" at the start of SRAY_STATUS_RECORD
+ * bytes, the rest zero.  'f' frame (x blocks, y blocks, block size,
+ * threads), 's' block start (x, y, width, height), 'e' block end (the
+ * same), 'd' done (milliseconds, ...). */
+#define SRAY_STATUS_RECORD 32
+int  sray_status_record(char st, int a1, int a2, int a3, int a4, char *out);
+/* text of the last failure of the I/O calls above (per thread) */
+const char *sray_io_last_error(void);
+/* Status stream of a context (fd < 0: off, the default).  A render writes
+ * 'f' (1, 1, width, lanes), 's' (x0, y0, width, height) when it starts,
+ * 'e' (the same) and 'd' (device milliseconds) when it ends.  A submitted
+ * request is a block: 's' when it is admitted to the GPU, 'e' when it is
+ * done, then 'd' (latency ms, queue ms, 0, 0).  Records are written whole,
+ * under a lock, from the thread that renders. */
+int  sray_gpu_set_status_fd(sray_gpu_ctx *ctx, int fd);
 
 /* Device memory and synchronisation on the context's device, so callers
  * (tests, benchmarks, the serve loop) need no second GPU runtime in the

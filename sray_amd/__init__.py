@@ -192,6 +192,46 @@ def projmap_cell(x, y, z):
     return u.value, v.value
 
 
+def _io_check(rc):
+    if rc:
+        raise SrayError(rc, lib().sray_io_last_error().decode(errors="replace"))
+
+
+def photon_dump(path, photons, energy):
+    """PhotonMap::dump (src/pmap.cc:180-205): photons (n, 12) float32 rows
+    pos, dir, norm, col in the reference's text format"""
+    arr = np.ascontiguousarray(photons, np.float32).reshape(-1, 12)
+    _io_check(lib().sray_photon_dump(os.fsencode(path), arr.ctypes.data_as(C.POINTER(abi.Photon)), len(arr),
+                                     float(energy)))
+
+
+def photon_restore(path, energy):
+    """PhotonMap::restore (src/pmap.cc:207-251): (n, 12) float32 photons,
+    colours scaled by energy / (the dump's energy)"""
+    p, n = C.POINTER(abi.Photon)(), C.c_uint32()
+    _io_check(lib().sray_photon_restore(os.fsencode(path), float(energy), C.byref(p), C.byref(n)))
+    try:
+        if n.value == 0:
+            return np.zeros((0, 12), np.float32)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n.value, 12)).copy()
+    finally:
+        lib().sray_photon_free(p)
+
+
+def write_pfm(path, rgba):
+    """the float framebuffer (h, w, 4), top row first, as a PFM file"""
+    img = np.ascontiguousarray(rgba, np.float32)
+    h, w = img.shape[:2]
+    _io_check(lib().sray_image_write_pfm(os.fsencode(path), img.ctypes.data_as(C.POINTER(C.c_float)), w, h))
+
+
+def status_record(st, a1, a2, a3, a4):
+    """one record of the backend status protocol (src/render.cc:279-289)"""
+    buf = C.create_string_buffer(abi.SRAY_STATUS_RECORD)
+    _io_check(lib().sray_status_record(st.encode() if isinstance(st, str) else st, a1, a2, a3, a4, buf))
+    return buf.raw
+
+
 # ---------------------------------------------------------------------------
 # device context
 # ---------------------------------------------------------------------------
@@ -378,6 +418,10 @@ class Context:
         self._check(lib().sray_gpu_autotune_octree(self._h, desc_ptr, C.byref(params), rounds, items, depths, n,
                                                    C.byref(best), C.byref(ms)))
         return best.value, ms.value
+
+    def set_status_fd(self, fd):
+        """status records of renders and requests go to fd (-1: off)"""
+        self._check(lib().sray_gpu_set_status_fd(self._h, int(fd)))
 
     def set_admission_cap(self, nbytes):
         self._check(lib().sray_gpu_set_admission_cap(self._h, int(nbytes)))

@@ -154,7 +154,10 @@ EXPORTS = [
     "sray_gpu_malloc", "sray_gpu_free", "sray_gpu_memcpy", "sray_gpu_synchronize",
     "sray_gpu_shoot_photons", "sray_gpu_pmap_download", "sray_projmap_build", "sray_projmap_cell",
     "sray_gpu_set_tuning", "sray_gpu_get_tuning", "sray_gpu_autotune", "sray_gpu_autotune_octree",
+    "sray_photon_dump", "sray_photon_restore", "sray_photon_free", "sray_image_write_pfm", "sray_status_record",
+    "sray_io_last_error", "sray_gpu_set_status_fd",
 ]
+SRAY_STATUS_RECORD = 32
 
 
 def declare(lib):
@@ -203,6 +206,13 @@ def declare(lib):
         "sray_gpu_autotune": (i32, [vp, u32, P(RenderParams), C.c_int, P(Tuning), P(f64)]),
         "sray_gpu_autotune_octree": (i32, [vp, P(SceneDesc), P(RenderParams), C.c_int, P(i32), P(i32), C.c_int,
                                            P(C.c_int), P(f64)]),
+        "sray_photon_dump": (i32, [C.c_char_p, P(Photon), u32, f32]),
+        "sray_photon_restore": (i32, [C.c_char_p, f32, P(P(Photon)), P(u32)]),
+        "sray_photon_free": (None, [P(Photon)]),
+        "sray_image_write_pfm": (i32, [C.c_char_p, P(f32), C.c_int, C.c_int]),
+        "sray_status_record": (i32, [C.c_char, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p]),
+        "sray_io_last_error": (C.c_char_p, []),
+        "sray_gpu_set_status_fd": (i32, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -1128,6 +1128,8 @@ void job_done(Engine &E, Job *j, int rc, const char *msg)
         if (msg) j->err = msg;
     }
     j->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - j->t_submit).count();
+    emit_status(E.c, 'e', j->p.x0, j->p.y0, j->p.x1 - j->p.x0, j->p.y1 - j->p.y0);
+    emit_status(E.c, 'd', (int)j->latency_ms, (int)j->queue_ms, 0, 0);
     j->state = 2;
     E.qcv.notify_all();
 }
@@ -1216,6 +1218,7 @@ int admit(Engine &E, Job *j)
     j->state = 1;
     j->t_admit = Clock::now();
     j->queue_ms = std::chrono::duration<double, std::milli>(j->t_admit - j->t_submit).count();
+    emit_status(E.c, 's', j->p.x0, j->p.y0, j->p.x1 - j->p.x0, j->p.y1 - j->p.y0);
     return SRAY_OK;
 }
 
@@ -1496,9 +1499,9 @@ int launch_chunk(Level &LV, uint32_t m, bool one_lane)
         return descend(LV, 0, 0, m);
     };
     /* graphs only for a pass on one lane: measured, graphs replayed on two
-     * lanes at once could run half again slower than the same launches made
-     * kernel by kernel (whose speed they match there), depending on the
-     * order the graphs were instantiated in */
+     * lanes at once ran half again slower than the same launches made kernel
+     * by kernel once a graph had been re-instantiated (4 hardware queues), and
+     * 7% slower with 8 queues */
     if (!E.use_graphs || c->profiling || !one_lane) return body();
     std::vector<uint64_t> key = {LV.c0, m, (uint64_t)E.maxiter, (uint64_t)E.L, (uint64_t)E.K, E.chunk,
                                  (uint64_t)E.trace_block, (uint64_t)E.gather_lanes,
@@ -1784,6 +1787,7 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
     j.dev_out = d_out;
     j.t_submit = Clock::now();
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    emit_status(c, 'f', 1, 1, p->x1 - p->x0, std::max(1, (int)c->tune.lanes));
     if ((rc = admit(E, &j))) return rc;
     uint64_t prim = 0, samples = 0;
     std::vector<Job *> fin;
@@ -1805,10 +1809,13 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
     /* counters as the last pass read them back (nothing after it adds to them) */
     const unsigned long long *hs = E.hst->stats, *rcn = E.hst->rcounts;
     if (j.limit_hit) return fail(c, SRAY_ERR_LIMIT, "ior stack deeper than %d media", IOR_DEPTH);
+    float dev_ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&dev_ms, c->ev0, c->ev1));
+    emit_status(c, 'e', p->x0, p->y0, p->x1 - p->x0, p->y1 - p->y0);
+    emit_status(c, 'd', (int)dev_ms, 0, 0, 0);
     if (st) {
         memset(st, 0, sizeof *st);
-        float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        const float ms = dev_ms;
         st->render_ms = ms;
         st->rays_primary = prim;
         st->rays_shadow = hs[1];

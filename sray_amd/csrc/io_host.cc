@@ -1,0 +1,139 @@
+/*
+ * io_host.cc — host-side input/output of the path: the photon-map text
+ * dump and restore (PhotonMap::dump / restore, src/pmap.cc:180-251), the
+ * float framebuffer written as PFM (Image::save, src/img.cc:80-83), and the
+ * 32-byte status records of the reference's backend protocol (emit_status,
+ * src/render.cc:279-289), which the serving engine also writes.
+ */
+#include <cerrno>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "host.h"
+
+namespace {
+thread_local std::string g_io_err;
+
+int io_fail(int code, const std::string &msg)
+{
+    g_io_err = msg;
+    return code;
+}
+} // namespace
+
+extern "C" {
+
+const char *sray_io_last_error(void) { return g_io_err.c_str(); }
+
+/* PhotonMap::dump, src/pmap.cc:180-205: the energy line, then one line per
+ * photon with position, direction, normal and colour (rgb), printf "%f" */
+int sray_photon_dump(const char *path, const sray_photon *p, uint32_t n, float energy)
+{
+    if (!path || (n && !p)) return io_fail(SRAY_ERR_ARG, "sray_photon_dump: bad arguments");
+    FILE *fp = fopen(path, "w");
+    if (!fp) return io_fail(SRAY_ERR_ARG, std::string("failed to dump photon map: ") + path + ": " + strerror(errno));
+    bool ok = fprintf(fp, "energy %f\n", (double)energy) > 0;
+    for (uint32_t i = 0; i < n && ok; i++) {
+        const sray_photon &q = p[i];
+        ok = fprintf(fp, "<%f %f %f> ", (double)q.pos[0], (double)q.pos[1], (double)q.pos[2]) > 0 &&
+             fprintf(fp, "<%f %f %f> ", (double)q.dir[0], (double)q.dir[1], (double)q.dir[2]) > 0 &&
+             fprintf(fp, "<%f %f %f> ", (double)q.norm[0], (double)q.norm[1], (double)q.norm[2]) > 0 &&
+             fprintf(fp, "<%f %f %f>\n", (double)q.col[0], (double)q.col[1], (double)q.col[2]) > 0;
+    }
+    if (fclose(fp) != 0) ok = false;
+    return ok ? SRAY_OK : io_fail(SRAY_ERR_ARG, std::string("failed to write photon dump: ") + path);
+}
+
+/* PhotonMap::restore, src/pmap.cc:207-251: reads the energy line and the
+ * photons; every colour is scaled by energy / (the dump's energy), the
+ * quotient taken in float and the product in double as the reference's
+ * Vector3 * float, then stored as float */
+int sray_photon_restore(const char *path, float energy, sray_photon **out, uint32_t *n)
+{
+    if (!path || !out || !n) return io_fail(SRAY_ERR_ARG, "sray_photon_restore: bad arguments");
+    *out = nullptr;
+    *n = 0;
+    FILE *fp = fopen(path, "r");
+    if (!fp) return io_fail(SRAY_ERR_ARG, std::string("failed to open photon dump: ") + path);
+    float prev_energy = 0.0f;
+    if (fscanf(fp, "energy %f\n", &prev_energy) < 1) {
+        fclose(fp);
+        return io_fail(SRAY_ERR_PARSE, std::string("failed to restore photon dump: ") + path);
+    }
+    const float scale = energy / prev_energy;
+    std::vector<sray_photon> v;
+    try {
+        for (;;) {
+            float x, y, z, dx, dy, dz, nx, ny, nz, cx, cy, cz;
+            int res = fscanf(fp, "<%f %f %f> <%f %f %f> <%f %f %f> <%f %f %f>\n", &x, &y, &z, &dx, &dy, &dz, &nx,
+                             &ny, &nz, &cx, &cy, &cz);
+            if (res == EOF) break;
+            if (res < 12) {
+                fclose(fp);
+                return io_fail(SRAY_ERR_PARSE, std::string("failed to restore photon dump: ") + path);
+            }
+            sray_photon q;
+            q.pos[0] = x, q.pos[1] = y, q.pos[2] = z;
+            q.dir[0] = dx, q.dir[1] = dy, q.dir[2] = dz;
+            q.norm[0] = nx, q.norm[1] = ny, q.norm[2] = nz;
+            q.col[0] = (float)((double)cx * (double)scale);
+            q.col[1] = (float)((double)cy * (double)scale);
+            q.col[2] = (float)((double)cz * (double)scale);
+            v.push_back(q);
+        }
+    } catch (const std::bad_alloc &) {
+        fclose(fp);
+        return io_fail(SRAY_ERR_NOMEM, "photon dump too large for host memory");
+    }
+    fclose(fp);
+    if (!v.empty()) {
+        sray_photon *buf = new (std::nothrow) sray_photon[v.size()];
+        if (!buf) return io_fail(SRAY_ERR_NOMEM, "photon dump too large for host memory");
+        memcpy(buf, v.data(), v.size() * sizeof(sray_photon));
+        *out = buf;
+    }
+    *n = (uint32_t)v.size();
+    return SRAY_OK;
+}
+
+void sray_photon_free(sray_photon *p) { delete[] p; }
+
+/* The float framebuffer (RGBA, row-major, top row first) as a PFM file:
+ * "PF\n<w> <h>\n-1.0\n", then RGB float32 little-endian, bottom row first */
+int sray_image_write_pfm(const char *path, const float *rgba, int width, int height)
+{
+    if (!path || !rgba || width <= 0 || height <= 0) return io_fail(SRAY_ERR_ARG, "sray_image_write_pfm: bad arguments");
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return io_fail(SRAY_ERR_ARG, std::string("failed to open image file: ") + path + ": " + strerror(errno));
+    bool ok = fprintf(fp, "PF\n%d %d\n-1.0\n", width, height) > 0;
+    std::vector<float> row((size_t)width * 3);
+    for (int y = height - 1; y >= 0 && ok; y--) {
+        const float *src = rgba + (size_t)y * width * 4;
+        for (int x = 0; x < width; x++) {
+            row[(size_t)x * 3] = src[(size_t)x * 4];
+            row[(size_t)x * 3 + 1] = src[(size_t)x * 4 + 1];
+            row[(size_t)x * 3 + 2] = src[(size_t)x * 4 + 2];
+        }
+        ok = fwrite(row.data(), sizeof(float), row.size(), fp) == row.size();
+    }
+    if (fclose(fp) != 0) ok = false;
+    return ok ? SRAY_OK : io_fail(SRAY_ERR_ARG, std::string("failed to write image: ") + path);
+}
+
+/* emit_status, src/render.cc:279-289: "%c %6d %6d %6d %6d\n" in a 32-byte
+ * record, the rest zero */
+int sray_status_record(char st, int a1, int a2, int a3, int a4, char *out)
+{
+    if (!out) return SRAY_ERR_ARG;
+    char buf[64];
+    int len = snprintf(buf, sizeof buf, "%c %6d %6d %6d %6d\n", st, a1, a2, a3, a4);
+    if (len < 0 || len >= SRAY_STATUS_RECORD) return io_fail(SRAY_ERR_ARG, "status record longer than 31 bytes");
+    memset(out, 0, SRAY_STATUS_RECORD);
+    memcpy(out, buf, (size_t)len);
+    return SRAY_OK;
+}
+
+} /* extern "C" */

@@ -98,6 +98,9 @@ struct sray_gpu_ctx {
     std::map<std::string, KStat> kstats;
     /* tuning of the renderers and photon grids */
     sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 4, 64};
+    /* status stream (fd < 0: off) and the lock that keeps records whole */
+    std::atomic<int> status_fd{-1};
+    std::mutex status_mu;
     /* renderers: one for synchronous renders, one behind submit/poll */
     srt::Engine *plain = nullptr;
     srt::Engine *serve = nullptr;
@@ -164,6 +167,9 @@ struct ProfScope {
 void prof_resolve(sray_gpu_ctx *c);
 
 GpuScene *find_scene(sray_gpu_ctx *c, uint32_t id);
+
+/* writes one status record to the context's status stream, if it has one */
+void emit_status(sray_gpu_ctx *c, char st, int a1, int a2, int a3, int a4);
 
 /* stable selection of the flagged elements (order kept), count to the host */
 template <class T>

@@ -17,7 +17,9 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cerrno>
 #include <cstdarg>
+#include <unistd.h>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -507,6 +509,22 @@ void prof_resolve(sray_gpu_ctx *c)
         c->evpool.push_back(p.b);
     }
     c->pending.clear();
+}
+
+void emit_status(sray_gpu_ctx *c, char st, int a1, int a2, int a3, int a4)
+{
+    const int fd = c->status_fd.load();
+    if (fd < 0) return;
+    char rec[SRAY_STATUS_RECORD];
+    if (sray_status_record(st, a1, a2, a3, a4, rec) != SRAY_OK) return;
+    std::lock_guard<std::mutex> lk(c->status_mu);
+    size_t off = 0;
+    while (off < sizeof rec) {
+        const ssize_t w = write(fd, rec + off, sizeof rec - off);
+        if (w < 0 && errno == EINTR) continue;
+        if (w <= 0) break; /* a closed or full stream drops the record */
+        off += (size_t)w;
+    }
 }
 
 GpuScene *find_scene(sray_gpu_ctx *c, uint32_t id)
@@ -1526,6 +1544,13 @@ int sray_gpu_synchronize(sray_gpu_ctx *c)
     ApiLock lk(c);
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipDeviceSynchronize());
+    return SRAY_OK;
+}
+
+int sray_gpu_set_status_fd(sray_gpu_ctx *c, int fd)
+{
+    if (!c) return SRAY_ERR_ARG;
+    c->status_fd.store(fd < 0 ? -1 : fd);
     return SRAY_OK;
 }
 

@@ -9,6 +9,7 @@ ray counts; admission by memory footprint, per-request cameras and the
 scene-release guard behave as documented.
 """
 import ctypes as C
+import os
 import time
 
 import numpy as np
@@ -232,3 +233,56 @@ def test_serving_counters_start_at_zero():
         c.release(sid)
     finally:
         c.close()
+
+
+def _records(fd_r):
+    data = b""
+    while True:
+        try:
+            chunk = os.read(fd_r, 65536)
+        except BlockingIOError:
+            break
+        if not chunk:
+            break
+        data += chunk
+    assert len(data) % abi.SRAY_STATUS_RECORD == 0
+    recs = []
+    for k in range(0, len(data), abi.SRAY_STATUS_RECORD):
+        rec = data[k:k + abi.SRAY_STATUS_RECORD]
+        text = rec.split(b"\0", 1)[0].decode()
+        assert rec == text.encode() + b"\0" * (abi.SRAY_STATUS_RECORD - len(text))
+        st, *args = text.split()
+        recs.append((st, [int(a) for a in args]))
+    return recs
+
+
+def test_status_stream(ctx):
+    """renders and requests report on the status stream in the reference's
+    record format (src/render.cc:279-289): a render is one frame of one
+    block, a request one block; 'd' carries the time"""
+    hs = S.HostScene.generate(0)
+    sid = ctx.upload(hs)
+    r, w = os.pipe()
+    os.set_blocking(r, False)
+    ctx.set_status_fd(w)
+    try:
+        p = S.render_params(160, 120, rect=(10, 20, 110, 80), iter=3, seed=1)
+        ctx.render(sid, p)
+        recs = _records(r)
+        assert [x[0] for x in recs] == ["f", "s", "e", "d"]
+        assert recs[0][1][:3] == [1, 1, 100] and recs[1][1] == [10, 20, 100, 60] and recs[2][1] == [10, 20, 100, 60]
+        assert recs[3][1][0] >= 0
+        tickets = [ctx.submit(sid, S.render_params(64, 48, rect=(k, 0, 32 + k, 16), iter=3, seed=k)) for k in range(3)]
+        for t in tickets:
+            ctx.poll(t)
+        recs = _records(r)
+        assert sorted(x[0] for x in recs) == ["d"] * 3 + ["e"] * 3 + ["s"] * 3
+        assert sorted(tuple(x[1]) for x in recs if x[0] == "s") == [(k, 0, 32, 16) for k in range(3)]
+        for k, (st, args) in enumerate(recs):
+            if st == "e":
+                assert recs[k + 1][0] == "d" and recs[k + 1][1][0] >= recs[k + 1][1][1] >= 0
+    finally:
+        ctx.set_status_fd(-1)
+        os.close(r)
+        os.close(w)
+        ctx.release(sid)

@@ -55,6 +55,27 @@ def test_sub_chunks_match_checker(ctx):
     ctx.release(sid)
 
 
+@pytest.mark.parametrize("sparse_from", [0, 1])
+def test_sparse_levels_match_checker(ctx, sparse_from):
+    """the traversal instances that keep four triangles in flight (from
+    bounce level sparse_from on) give the heightfield image of the default
+    instances bit for bit, and match the checker"""
+    hs = S.HostScene.generate(1, variant=4)
+    sid = ctx.upload(hs)
+    p = S.render_params(480, 270, iter=6)
+    base, bst = ctx.render(sid, p)
+    ctx.set_tuning(sparse_from=sparse_from)
+    try:
+        img, st = ctx.render(sid, p)
+    finally:
+        ctx.set_tuning(sparse_from=64)
+    assert np.array_equal(img, base) and st.rays() == bst.rays()
+    oimg, ost = oracle.OracleScene(hs.desc_ptr(), 32).render(p)
+    e = float(np.sqrt(np.mean((img.astype(np.float64) - oimg) ** 2)))
+    assert e <= 1e-3 and st.rays() == ost.rays(), e
+    ctx.release(sid)
+
+
 @pytest.mark.parametrize("config,variant,size", [(1, 4, (480, 270)), (2, 10, (400, 300)), (0, 0, (400, 300))])
 def test_lanes_give_the_same_image(ctx, config, variant, size):
     """a pass spread over 2 or 4 streams renders the image and ray counts of
