@@ -370,6 +370,10 @@ typedef struct sray_tuning {
     int32_t sparse_from;      /* 0..64: first bounce level whose traversal kernels keep four triangles
                                  in flight per lane (more registers, fewer waves; for levels with
                                  too few rays to hide load latency); 64: none */
+    int32_t chain;            /* 1: when no ray of the requests in flight can spawn more than one
+                                 child and none uses photon maps (and at most 4 shadow rays per
+                                 hit), one lane follows each sample's rays to the end in a single
+                                 kernel; 0: always the level queues (same image) */
 } sray_tuning;
 int  sray_gpu_set_tuning(sray_gpu_ctx *ctx, const sray_tuning *t);
 int  sray_gpu_get_tuning(sray_gpu_ctx *ctx, sray_tuning *t);

@@ -210,8 +210,279 @@ __device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uin
     return v3(lt.pos[0], lt.pos[1], lt.pos[2]);
 }
 
-/* shade_phong (src/shader.cc:83-247) / shade_undef (:36-71) for one hit,
- * emitting direct terms, shadow rays, children and photon-gather requests */
+/* Shading of one queued ray and its hit (shade_phong, src/shader.cc:83-247,
+ * calc_lighting :250-283, shade_undef :36-71; the miss colour of
+ * Scene::trace_ray, src/scene.cc:568): the direct term, the shadow rays
+ * with their contributions, and the children (reflect, refract, diffuse),
+ * written through the sink (the level arrays of the wavefront, or a
+ * lane's registers on the one-child chain).  Counts of spawned rays by
+ * kind go to c_sh, c_rl, c_rr, c_df. */
+template <bool PHOTONS, bool UNI, bool SCN1, bool ATTR, class Sink>
+__device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restrict__ active, uint32_t c0,
+                                          const QRay &R, int4 h, int L, int K, Sink &sink, uint32_t &c_sh,
+                                          uint32_t &c_rl, uint32_t &c_rr, uint32_t &c_df)
+{
+    uint32_t slot = R.m.x, path = R.m.y, flg = (R.m.w >> 8) & 0xffu;
+    uint32_t rq = UNI ? 0u : R.m.w >> SLOT_SHIFT;
+    const ReqDev &Q = T.reqs[rq];
+    const RenderDev &P = Q.P;
+    /* SCN1: every request in flight renders scene 0 */
+    const SceneDev S = T.scenes[(UNI || SCN1) ? 0u : Q.scene];
+    uint32_t pr = active[c0 + slot];
+    uint32_t lp = pr - Q.pix_base;
+    int sample = T.nsamp[pr];
+    int iter = (int)R.m.z;
+    V3 W = v3(R.w.x, R.w.y, R.w.z);
+    float energy = R.d.w;
+    int time = __float_as_int(R.o.w);
+    V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
+    float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    int used = 0;        /* shadow slots written */
+    uint32_t spawn = 0;  /* reflect (bit 0) / refract (bit 1) child written */
+    int dwritten = 0;    /* diffuse child flags written */
+    sink.slot(slot);
+    if (PHOTONS) sink.gather_clear();
+    if (h.y < 0) {
+        /* miss: env_color (src/scene.cc:568) */
+        dir.x = W.x * S.env_color.x;
+        dir.y = W.y * S.env_color.y;
+        dir.z = W.z * S.env_color.z;
+        if (path == 1u) dir.w = S.env_color.w;
+        sink.set_direct(dir);
+    } else {
+        float t = __int_as_float(h.x);
+        const DevObj &ob = S.objs[h.y];
+        const DevMat &mat = S.mats[ob.mat];
+        bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
+        V3 Pw, N;
+        float tu, tv;
+        hit_surface<ATTR>(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong, Pw, N, tu, tv);
+        if (path == 1u) dir.w = 1.0f;
+        uint32_t pix = (uint32_t)(P.y0 + (int)(lp / (uint32_t)P.W)) * (uint32_t)P.width + (uint32_t)(P.x0 + (int)(lp % (uint32_t)P.W));
+        int nl = S.nlights;
+        if (!phong) {
+            /* shade_undef: checker texture from texcoords */
+            int cx = (int)(tu * 20.0f), cy = (int)(tv * 10.0f);
+            V3 tex = ((cx % 2) == (cy % 2)) ? v3(1.0f, 0.3f, 0.2f) : v3(0.2f, 0.3f, 1.0f);
+            dir.x = W.x * (0.1f * tex.x);
+            dir.y = W.y * (0.1f * tex.y);
+            dir.z = W.z * (0.1f * tex.z);
+            for (int l = 0; l < nl; l++) {
+                const DevLight &lt = S.lights[l];
+                V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u);
+                V3 ldir = sub(lpos, Pw);
+                float ldist = length(ldir);
+                V3 sd = ldir;
+                ldir = dvs(ldir, ldist);
+                V3 vdir = neg(normalized(rd));
+                V3 hdir = normalized(add(ldir, vdir));
+                float ndotl = fmaxf(dot(N, ldir), 0.0f);
+                float ndoth = fmaxf(dot(N, hdir), 0.0f);
+                float sp = powf(ndoth, 50.0f);
+                SRay s;
+                s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
+                s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
+                sink.shadow(l, s,
+                            make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
+                                        W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
+                                        W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp), 1.0f));
+                c_sh++;
+            }
+            used = nl;
+            sink.set_direct(dir);
+        } else {
+            bool entering;
+            V3 normal;
+            if (dot(rd, N) > 0.0f) {
+                normal = neg(N);
+                entering = false;
+            } else {
+                normal = N;
+                entering = true;
+            }
+            float ray_mag = length(rd);
+            V3 incident = dvs(rd, ray_mag);
+            float4 kd = mat.kd;
+            float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            if (flg & QF_DIFFUSE) {
+                /* diffuse GI ray: radiance estimate from the global map
+                 * (src/shader.cc:119-123) */
+                if (PHOTONS) {
+                    GatherReq g;
+                    g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(2u));
+                    g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
+                    g.w = make_float4(W.x, W.y, W.z, 0.0f);
+                    g.kd = kd;
+                    sink.gather(g);
+                } else {
+                    dir.x = W.x * (0.0f * kd.x);
+                    dir.y = W.y * (0.0f * kd.y);
+                    dir.z = W.z * (0.0f * kd.z);
+                }
+                sink.set_direct(dir);
+            } else {
+                float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
+                float trans_fact = (mat.flags & SRAY_MAT_HAS_REFRACT) ? mat.refract : 0.0f;
+                float mat_ior = mat.ior;
+                IorStack st;
+                st.cur = R.w.w;
+                st.s[0] = R.stk.x;
+                st.s[1] = R.stk.y;
+                st.s[2] = R.stk.z;
+                st.s[3] = R.stk.w;
+                st.n = R.m.w & 0xffu;
+                st.overflow = false;
+                float ior = st.calc(entering, mat_ior);
+                /* ambient: (env_ambient * kd) * kd, src/shader.cc:142,246 */
+                dir.x = W.x * ((S.env_ambient.x * kd.x) * kd.x);
+                dir.y = W.y * ((S.env_ambient.y * kd.y) * kd.y);
+                dir.z = W.z * ((S.env_ambient.z * kd.z) * kd.z);
+                int slotk = 0;
+                V3 vref = reflect(incident, normal);
+                for (int l = 0; l < nl; l++) {
+                    const DevLight &lt = S.lights[l];
+                    int ns = lt.type != SRAY_LIGHT_POINT ? P.shadow_samples : 1;
+                    float scale = ns > 1 ? 1.0f / (float)ns : 1.0f;
+                    for (int si = 0; si < ns; si++) {
+                        V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si);
+                        V3 ldir = sub(lpos, Pw);
+                        float ldist = length(ldir);
+                        V3 sd = ldir;
+                        ldir = dvs(ldir, ldist);
+                        float ndotl = fmaxf(dot(normal, ldir), 0.0f);
+                        float rdotl = fmaxf(dot(vref, ldir), 0.0f);
+                        float sp = powf(rdotl, mat.shininess);
+                        SRay s;
+                        s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
+                        s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
+                        sink.shadow(slotk, s,
+                                    make_float4(W.x * ((kd.x * (lt.color[0] * ndotl) + ks.x * (lt.color[0] * sp)) * scale),
+                                                W.y * ((kd.y * (lt.color[1] * ndotl) + ks.y * (lt.color[1] * sp)) * scale),
+                                                W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale),
+                                                1.0f));
+                        slotk++;
+                        c_sh++;
+                    }
+                }
+                used = slotk;
+                if (PHOTONS && P.gi) {
+                    /* diffuse hemisphere rays (src/shader.cc:158-185); a
+                     * direction flipped into the hemisphere keeps its
+                     * negative ndotl and so spawns no ray */
+                    float dsc = P.diffuse_samples > 1 ? 1.0f / (float)P.diffuse_samples : 1.0f;
+                    for (int di = 0; di < P.diffuse_samples; di++) {
+                        V3 gd = sph_rand(P.seed, pix, (uint32_t)sample, path, K_DIFFUSE, (uint32_t)di, 0u, 1.0f);
+                        float ndotl = dot(gd, normal);
+                        if (ndotl < 0.0f) gd = neg(gd);
+                        float diff_energy = ndotl * energy;
+                        if (diff_energy > P.min_energy) {
+                            QRay cr;
+                            V3 dd = mul(gd, ray_mag);
+                            cr.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                            cr.d = make_float4(dd.x, dd.y, dd.z, diff_energy);
+                            float wf = ndotl * dsc;
+                            cr.w = make_float4(W.x * (kd.x * wf), W.y * (kd.y * wf), W.z * (kd.z * wf), R.w.w);
+                            cr.m = make_uint4(slot, path | 0x40000000u | ((uint32_t)di << 20), 0x7fffffffu,
+                                              (R.m.w & 0xffu) | (QF_DIFFUSE << 8) | (rq << SLOT_SHIFT));
+                            cr.stk = R.stk;
+                            sink.child(2 + di, cr);
+                            c_df++;
+                        }
+                        sink.flag(2 + di, diff_energy > P.min_energy ? 1 : 0);
+                    }
+                    dwritten = P.diffuse_samples;
+                }
+                if (PHOTONS && P.caust) {
+                    /* caustics: irradiance from the caustic map (src/shader.cc:231-236),
+                     * added by k_gather */
+                    GatherReq g;
+                    g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(1u));
+                    g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
+                    g.w = make_float4(W.x, W.y, W.z, 0.0f);
+                    g.kd = kd;
+                    sink.gather(g);
+                }
+                V3 nrd = dvs(neg(rd), ray_mag);
+                float ray_dot_n = dot(nrd, normal);
+                float sf0 = trans_fact > 0.0f ? (ior - 1.0f) / (ior + 1.0f) : 1.0f;
+                float fres = fresnel(sf0 * sf0, ray_dot_n);
+                float refl_energy = refl_fact * energy;
+                if (refl_energy > P.min_energy && iter > 0) {
+                    V3 nd = reflect(rd, normal);
+                    QRay c;
+                    c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                    c.d = make_float4(nd.x, nd.y, nd.z, refl_energy);
+                    float wf = fres * refl_fact;
+                    c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), R.w.w);
+                    c.m = make_uint4(slot, path * 2u, (uint32_t)(iter - 1), R.m.w);
+                    c.stk = R.stk;
+                    sink.child(0, c);
+                    spawn |= 1u;
+                    c_rl++;
+                }
+                float trans_energy = trans_fact * energy;
+                if (trans_energy > P.min_energy && iter > 0) {
+                    IorStack ts = st;
+                    if (entering) ts.enter(mat_ior);
+                    else ts.leave();
+                    V3 td = mul(refract(dvs(rd, ray_mag), normal, ior), ray_mag);
+                    if (dot(td, normal) > 0.0f) {
+                        if (entering) ts.leave();
+                        else ts.enter(mat_ior);
+                    }
+                    if (ts.overflow) sink.ior_overflow();
+                    QRay c;
+                    c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
+                    c.d = make_float4(td.x, td.y, td.z, trans_energy);
+                    float wf = (1.0f - fres) * trans_fact;
+                    c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), ts.cur);
+                    c.m = make_uint4(slot, path * 2u + 1u, (uint32_t)(iter - 1), ts.n | (flg << 8) | (rq << SLOT_SHIFT));
+                    c.stk = make_float4(ts.s[0], ts.s[1], ts.s[2], ts.s[3]);
+                    sink.child(1, c);
+                    spawn |= 2u;
+                    c_rr++;
+                }
+                sink.set_direct(dir);
+            }
+        }
+    }
+    /* flags of the child slots and the unused shadow slots, written once */
+    sink.flag(0, (uint8_t)(spawn & 1u));
+    sink.flag(1, (uint8_t)((spawn >> 1) & 1u));
+    for (int k = 2 + dwritten; k < K; k++) sink.flag(k, 0);
+    for (int l = used; l < L; l++) sink.shadow_unused(l);
+}
+
+/* the level arrays of a step: ray i's outputs at its own offsets */
+struct ArraySink {
+    uint32_t i;
+    int L, K;
+    float4 *direct;
+    SRay *sq;
+    float4 *scon;
+    uint32_t *rslot;
+    QRay *kids;
+    uint8_t *kflag;
+    GatherReq *greq;
+    uint32_t *err;
+    __device__ void slot(uint32_t s) { rslot[i] = s; }
+    __device__ void gather_clear() { greq[i].pos = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u)); }
+    __device__ void gather(const GatherReq &g) { greq[i] = g; }
+    __device__ void set_direct(float4 d) { direct[i] = d; }
+    __device__ void shadow(int l, const SRay &s, float4 c)
+    {
+        sq[(size_t)i * L + l] = s;
+        scon[(size_t)i * L + l] = c;
+    }
+    __device__ void shadow_unused(int l) { scon[(size_t)i * L + l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
+    __device__ void child(int k, const QRay &c) { kids[(size_t)i * K + k] = c; }
+    __device__ void flag(int k, uint8_t f) { kflag[(size_t)i * K + k] = f; }
+    __device__ void ior_overflow() { atomicOr(err, 1u); }
+};
+
+/* shade_phong (src/shader.cc:83-247) / shade_undef (:36-71) for every hit
+ * of a level step, emitting direct terms, shadow rays, children and
+ * photon-gather requests into the level arrays */
 template <bool PHOTONS, bool UNI, bool SCN1>
 __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restrict__ active, uint32_t c0,
                                                const QRay *__restrict__ q, const int4 *__restrict__ hits, StepN sn,
@@ -231,243 +502,138 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
         const uint32_t i = gbase + threadIdx.x;
         const uint32_t before = c_sh + c_rl + c_rr + c_df;
         if (i < n) {
-            QRay R = q[i];
-            int4 h = hits[i];
-            uint32_t slot = R.m.x, path = R.m.y, flg = (R.m.w >> 8) & 0xffu;
-            uint32_t rq = UNI ? 0u : R.m.w >> SLOT_SHIFT;
-            const ReqDev &Q = T.reqs[rq];
-            const RenderDev &P = Q.P;
-            /* SCN1: every request in flight renders scene 0 */
-            const SceneDev S = T.scenes[(UNI || SCN1) ? 0u : Q.scene];
-            uint32_t pr = active[c0 + slot];
-            uint32_t lp = pr - Q.pix_base;
-            int sample = T.nsamp[pr];
-            int iter = (int)R.m.z;
-            V3 W = v3(R.w.x, R.w.y, R.w.z);
-            float energy = R.d.w;
-            int time = __float_as_int(R.o.w);
-            V3 ro = v3(R.o.x, R.o.y, R.o.z), rd = v3(R.d.x, R.d.y, R.d.z);
-            float4 dir = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            int used = 0;        /* shadow slots written */
-            uint32_t spawn = 0;  /* reflect (bit 0) / refract (bit 1) child written */
-            int dwritten = 0;    /* diffuse child flags written */
-            rslot[i] = slot;
-            if (PHOTONS) greq[i].pos = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-            if (h.y < 0) {
-                /* miss: env_color (src/scene.cc:568) */
-                dir.x = W.x * S.env_color.x;
-                dir.y = W.y * S.env_color.y;
-                dir.z = W.z * S.env_color.z;
-                if (path == 1u) dir.w = S.env_color.w;
-                direct[i] = dir;
-            } else {
-                float t = __int_as_float(h.x);
-                const DevObj &ob = S.objs[h.y];
-                const DevMat &mat = S.mats[ob.mat];
-                bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
-                V3 Pw, N;
-                float tu, tv;
-                hit_surface(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong, Pw, N, tu, tv);
-                if (path == 1u) dir.w = 1.0f;
-                uint32_t pix = (uint32_t)(P.y0 + (int)(lp / (uint32_t)P.W)) * (uint32_t)P.width + (uint32_t)(P.x0 + (int)(lp % (uint32_t)P.W));
-                int nl = S.nlights;
-                if (!phong) {
-                    /* shade_undef: checker texture from texcoords */
-                    int cx = (int)(tu * 20.0f), cy = (int)(tv * 10.0f);
-                    V3 tex = ((cx % 2) == (cy % 2)) ? v3(1.0f, 0.3f, 0.2f) : v3(0.2f, 0.3f, 1.0f);
-                    dir.x = W.x * (0.1f * tex.x);
-                    dir.y = W.y * (0.1f * tex.y);
-                    dir.z = W.z * (0.1f * tex.z);
-                    for (int l = 0; l < nl; l++) {
-                        const DevLight &lt = S.lights[l];
-                        V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u);
-                        V3 ldir = sub(lpos, Pw);
-                        float ldist = length(ldir);
-                        V3 sd = ldir;
-                        ldir = dvs(ldir, ldist);
-                        V3 vdir = neg(normalized(rd));
-                        V3 hdir = normalized(add(ldir, vdir));
-                        float ndotl = fmaxf(dot(N, ldir), 0.0f);
-                        float ndoth = fmaxf(dot(N, hdir), 0.0f);
-                        float sp = powf(ndoth, 50.0f);
-                        SRay s;
-                        s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
-                        s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
-                        sq[(size_t)i * L + l] = s;
-                        scon[(size_t)i * L + l] = make_float4(W.x * (tex.x * (lt.color[0] * ndotl) + lt.color[0] * sp),
-                                                              W.y * (tex.y * (lt.color[1] * ndotl) + lt.color[1] * sp),
-                                                              W.z * (tex.z * (lt.color[2] * ndotl) + lt.color[2] * sp),
-                                                              1.0f);
-                        c_sh++;
-                    }
-                    used = nl;
-                    direct[i] = dir;
-                } else {
-                    bool entering;
-                    V3 normal;
-                    if (dot(rd, N) > 0.0f) {
-                        normal = neg(N);
-                        entering = false;
-                    } else {
-                        normal = N;
-                        entering = true;
-                    }
-                    float ray_mag = length(rd);
-                    V3 incident = dvs(rd, ray_mag);
-                    float4 kd = mat.kd;
-                    float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-                    if (flg & QF_DIFFUSE) {
-                        /* diffuse GI ray: radiance estimate from the global map
-                         * (src/shader.cc:119-123) */
-                        if (PHOTONS) {
-                            GatherReq g;
-                            g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(2u));
-                            g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
-                            g.w = make_float4(W.x, W.y, W.z, 0.0f);
-                            g.kd = kd;
-                            greq[i] = g;
-                        } else {
-                            dir.x = W.x * (0.0f * kd.x);
-                            dir.y = W.y * (0.0f * kd.y);
-                            dir.z = W.z * (0.0f * kd.z);
-                        }
-                        direct[i] = dir;
-                    } else {
-                        float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
-                        float trans_fact = (mat.flags & SRAY_MAT_HAS_REFRACT) ? mat.refract : 0.0f;
-                        float mat_ior = mat.ior;
-                        IorStack st;
-                        st.cur = R.w.w;
-                        st.s[0] = R.stk.x;
-                        st.s[1] = R.stk.y;
-                        st.s[2] = R.stk.z;
-                        st.s[3] = R.stk.w;
-                        st.n = R.m.w & 0xffu;
-                        st.overflow = false;
-                        float ior = st.calc(entering, mat_ior);
-                        /* ambient: (env_ambient * kd) * kd, src/shader.cc:142,246 */
-                        dir.x = W.x * ((S.env_ambient.x * kd.x) * kd.x);
-                        dir.y = W.y * ((S.env_ambient.y * kd.y) * kd.y);
-                        dir.z = W.z * ((S.env_ambient.z * kd.z) * kd.z);
-                        int slotk = 0;
-                        V3 vref = reflect(incident, normal);
-                        for (int l = 0; l < nl; l++) {
-                            const DevLight &lt = S.lights[l];
-                            int ns = lt.type != SRAY_LIGHT_POINT ? P.shadow_samples : 1;
-                            float scale = ns > 1 ? 1.0f / (float)ns : 1.0f;
-                            for (int si = 0; si < ns; si++) {
-                                V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si);
-                                V3 ldir = sub(lpos, Pw);
-                                float ldist = length(ldir);
-                                V3 sd = ldir;
-                                ldir = dvs(ldir, ldist);
-                                float ndotl = fmaxf(dot(normal, ldir), 0.0f);
-                                float rdotl = fmaxf(dot(vref, ldir), 0.0f);
-                                float sp = powf(rdotl, mat.shininess);
-                                SRay s;
-                                s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
-                                s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));
-                                sq[(size_t)i * L + slotk] = s;
-                                scon[(size_t)i * L + slotk] = make_float4(
-                                    W.x * ((kd.x * (lt.color[0] * ndotl) + ks.x * (lt.color[0] * sp)) * scale),
-                                    W.y * ((kd.y * (lt.color[1] * ndotl) + ks.y * (lt.color[1] * sp)) * scale),
-                                    W.z * ((kd.z * (lt.color[2] * ndotl) + ks.z * (lt.color[2] * sp)) * scale), 1.0f);
-                                slotk++;
-                                c_sh++;
-                            }
-                        }
-                        used = slotk;
-                        if (PHOTONS && P.gi) {
-                            /* diffuse hemisphere rays (src/shader.cc:158-185); a
-                             * direction flipped into the hemisphere keeps its
-                             * negative ndotl and so spawns no ray */
-                            float dsc = P.diffuse_samples > 1 ? 1.0f / (float)P.diffuse_samples : 1.0f;
-                            for (int di = 0; di < P.diffuse_samples; di++) {
-                                V3 gd = sph_rand(P.seed, pix, (uint32_t)sample, path, K_DIFFUSE, (uint32_t)di, 0u, 1.0f);
-                                float ndotl = dot(gd, normal);
-                                if (ndotl < 0.0f) gd = neg(gd);
-                                float diff_energy = ndotl * energy;
-                                if (diff_energy > P.min_energy) {
-                                    QRay cr;
-                                    V3 dd = mul(gd, ray_mag);
-                                    cr.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                                    cr.d = make_float4(dd.x, dd.y, dd.z, diff_energy);
-                                    float wf = ndotl * dsc;
-                                    cr.w = make_float4(W.x * (kd.x * wf), W.y * (kd.y * wf), W.z * (kd.z * wf), R.w.w);
-                                    cr.m = make_uint4(slot, path | 0x40000000u | ((uint32_t)di << 20), 0x7fffffffu,
-                                                      (R.m.w & 0xffu) | (QF_DIFFUSE << 8) | (rq << SLOT_SHIFT));
-                                    cr.stk = R.stk;
-                                    kids[(size_t)i * K + 2 + di] = cr;
-                                    c_df++;
-                                }
-                                kflag[(size_t)i * K + 2 + di] = diff_energy > P.min_energy ? 1 : 0;
-                            }
-                            dwritten = P.diffuse_samples;
-                        }
-                        if (PHOTONS && P.caust) {
-                            /* caustics: irradiance from the caustic map (src/shader.cc:231-236),
-                             * added by k_gather */
-                            GatherReq g;
-                            g.pos = make_float4(Pw.x, Pw.y, Pw.z, __uint_as_float(1u));
-                            g.nrm = make_float4(normal.x, normal.y, normal.z, 0.0f);
-                            g.w = make_float4(W.x, W.y, W.z, 0.0f);
-                            g.kd = kd;
-                            greq[i] = g;
-                        }
-                        V3 nrd = dvs(neg(rd), ray_mag);
-                        float ray_dot_n = dot(nrd, normal);
-                        float sf0 = trans_fact > 0.0f ? (ior - 1.0f) / (ior + 1.0f) : 1.0f;
-                        float fres = fresnel(sf0 * sf0, ray_dot_n);
-                        float refl_energy = refl_fact * energy;
-                        if (refl_energy > P.min_energy && iter > 0) {
-                            V3 nd = reflect(rd, normal);
-                            QRay c;
-                            c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                            c.d = make_float4(nd.x, nd.y, nd.z, refl_energy);
-                            float wf = fres * refl_fact;
-                            c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), R.w.w);
-                            c.m = make_uint4(slot, path * 2u, (uint32_t)(iter - 1), R.m.w);
-                            c.stk = R.stk;
-                            kids[(size_t)i * K + 0] = c;
-                            spawn |= 1u;
-                            c_rl++;
-                        }
-                        float trans_energy = trans_fact * energy;
-                        if (trans_energy > P.min_energy && iter > 0) {
-                            IorStack ts = st;
-                            if (entering) ts.enter(mat_ior);
-                            else ts.leave();
-                            V3 td = mul(refract(dvs(rd, ray_mag), normal, ior), ray_mag);
-                            if (dot(td, normal) > 0.0f) {
-                                if (entering) ts.leave();
-                                else ts.enter(mat_ior);
-                            }
-                            if (ts.overflow) atomicOr(err, 1u);
-                            QRay c;
-                            c.o = make_float4(Pw.x, Pw.y, Pw.z, R.o.w);
-                            c.d = make_float4(td.x, td.y, td.z, trans_energy);
-                            float wf = (1.0f - fres) * trans_fact;
-                            c.w = make_float4(W.x * (ks.x * wf), W.y * (ks.y * wf), W.z * (ks.z * wf), ts.cur);
-                            c.m = make_uint4(slot, path * 2u + 1u, (uint32_t)(iter - 1), ts.n | (flg << 8) | (rq << SLOT_SHIFT));
-                            c.stk = make_float4(ts.s[0], ts.s[1], ts.s[2], ts.s[3]);
-                            kids[(size_t)i * K + 1] = c;
-                            spawn |= 2u;
-                            c_rr++;
-                        }
-                        direct[i] = dir;
-                    }
-                }
-            }
-            /* flags of the child slots and the unused shadow slots, written once */
-            kflag[(size_t)i * K + 0] = (uint8_t)(spawn & 1u);
-            kflag[(size_t)i * K + 1] = (uint8_t)((spawn >> 1) & 1u);
-            for (int k = 2 + dwritten; k < K; k++) kflag[(size_t)i * K + k] = 0;
-            for (int l = used; l < L; l++) scon[(size_t)i * L + l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ArraySink sink{i, L, K, direct, sq, scon, rslot, kids, kflag, greq, err};
+            shade_ray<PHOTONS, UNI, SCN1, true>(T, active, c0, q[i], hits[i], L, K, sink, c_sh, c_rl, c_rr, c_df);
         }
         if (!UNI)
             wave_keyed_add(slot_rays, i < n ? q[i].m.w >> SLOT_SHIFT : 0u,
                            (unsigned long long)(c_sh + c_rl + c_rr + c_df - before), i < n);
     }
+    if (c_sh) atomicAdd(&bcnt[0], c_sh);
+    if (c_rl) atomicAdd(&bcnt[1], c_rl);
+    if (c_rr) atomicAdd(&bcnt[2], c_rr);
+    if (c_df) atomicAdd(&bcnt[3], c_df);
+    __syncthreads();
+    if (threadIdx.x < 4 && bcnt[threadIdx.x]) atomicAdd(&stats[1 + threadIdx.x], (unsigned long long)bcnt[threadIdx.x]);
+}
+
+/* A lane's registers as the shading sink of the one-child chain: the direct
+ * term, up to LMAX shadow rays with their contributions, and the one child
+ * (every request in flight spawns at most one child per ray and has no
+ * photon maps, so there are no gather requests and no diffuse children). */
+template <int LMAX> struct ChainSink {
+    float4 *acc;   /* the sample's sum: the direct term is added when set */
+    SRay sh[LMAX];
+    float4 con[LMAX];
+    QRay *kid;     /* the child, in LDS (it would hold 20 registers through the shading) */
+    uint32_t kids; /* bit k: child k spawned */
+    uint32_t *err;
+    __device__ void slot(uint32_t) {}
+    __device__ void gather_clear() {}
+    __device__ void gather(const GatherReq &) {}
+    __device__ void set_direct(float4 d)
+    {
+        acc->x = acc->x + d.x;
+        acc->y = acc->y + d.y;
+        acc->z = acc->z + d.z;
+        acc->w = acc->w + d.w;
+    }
+    /* one slot: its index is 0 (L <= 1), which keeps the arrays in registers */
+    __device__ void shadow(int l, const SRay &s, float4 c)
+    {
+        sh[LMAX == 1 ? 0 : l] = s;
+        con[LMAX == 1 ? 0 : l] = c;
+    }
+    __device__ void shadow_unused(int l) { con[LMAX == 1 ? 0 : l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
+    __device__ void child(int, const QRay &c) { *kid = c; }
+    __device__ void flag(int k, uint8_t f) { kids |= f ? 1u << k : 0u; }
+    __device__ void ior_overflow() { atomicOr(err, 1u); }
+};
+
+/* The one-child chain: when no ray of the requests in flight can spawn more
+ * than one child and none has photon maps, a sample's rays form a single
+ * line, and one lane follows it from the primary ray to its end: closest
+ * hit, shading, the shadow rays, and the sample's sum.  Each ray's
+ * arithmetic is that of the level kernels (k_raygen, k_trace_q, k_shade,
+ * k_trace_shadow, k_accum), and with one ray per sample per level the sum
+ * is added in the same order, so the image is the same; there are no level
+ * queues, compactions or per-level launches.  acc[i] receives sample i's
+ * sum for k_pass_finish. */
+template <bool UNI, bool SCN1, int LMAX, bool ATTR, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ATTR || LMAX > 1 ? 1 : 4))) void k_chain(Tabs T, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m,
+                                              int L, int K, float4 *__restrict__ acc,
+                                              unsigned long long *__restrict__ stats,
+                                              unsigned long long *__restrict__ slot_rays, uint32_t *__restrict__ err)
+{
+    /* 20 KB a block of two waves: four waves per SIMD */
+    __shared__ uint32_t stk[STACK * BS];
+    __shared__ QRay kids[BS];
+    uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    for (uint32_t gbase = blockIdx.x * (uint32_t)BS; gbase < m; gbase += gridDim.x * (uint32_t)BS) {
+        const uint32_t i = gbase + threadIdx.x;
+        const uint32_t before = c_sh + c_rl + c_rr + c_df;
+        uint32_t rq = 0;
+        if (i < m) {
+            /* k_raygen */
+            const uint32_t pr = active[c0 + i];
+            rq = UNI ? 0u : T.preq[pr];
+            const ReqDev &Q = T.reqs[rq];
+            const RenderDev &P = Q.P;
+            const uint32_t lp = pr - Q.pix_base;
+            const int x = P.x0 + (int)(lp % (uint32_t)P.W), y = P.y0 + (int)(lp / (uint32_t)P.W);
+            V3 o, d;
+            int time;
+            primary_ray(Q.C, P, x, y, T.nsamp[pr], o, d, time);
+            QRay R;
+            R.o = make_float4(o.x, o.y, o.z, __int_as_float(time));
+            R.d = make_float4(d.x, d.y, d.z, 1.0f);
+            R.w = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            R.m = make_uint4(i, 1u, (uint32_t)P.iter, rq << SLOT_SHIFT);
+            R.stk = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const SceneDev S = T.scenes[(UNI || SCN1) ? 0u : Q.scene];
+            float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (;;) {
+                /* k_trace_q */
+                const HitRec hr = traverse<false, false>(S, v3(R.o.x, R.o.y, R.o.z), v3(R.d.x, R.d.y, R.d.z),
+                                                         __float_as_int(R.o.w), &stk[threadIdx.x], BS, cnt);
+                const int4 h = make_int4(__float_as_int(hr.t), hr.obj, (int)hr.tri, (int)hr.rank);
+                /* k_shade */
+                ChainSink<LMAX> sk;
+                sk.acc = &a;
+                sk.kid = &kids[threadIdx.x];
+                sk.kids = 0u;
+                sk.err = err;
+                /* k_shade, with k_accum's first addition (the direct term) */
+                shade_ray<false, UNI, SCN1, ATTR>(T, active, c0, R, h, L, K, sk, c_sh, c_rl, c_rr, c_df);
+                /* k_trace_shadow and the rest of k_accum (accum_run's order) */
+#pragma unroll
+                for (int l = 0; l < LMAX; l++) {
+                    const float4 cc = sk.con[l];
+                    if (l < L && cc.w != 0.0f) {
+                        const SRay &sr = sk.sh[l];
+                        const HitRec oc = traverse<true, false>(S, v3(sr.o.x, sr.o.y, sr.o.z), v3(sr.d.x, sr.d.y, sr.d.z),
+                                                                __float_as_int(sr.o.w), &stk[threadIdx.x], BS, cnt);
+                        if (oc.obj < 0) {
+                            a.x = a.x + cc.x;
+                            a.y = a.y + cc.y;
+                            a.z = a.z + cc.z;
+                        }
+                    }
+                }
+                if (!sk.kids) break;
+                R = kids[threadIdx.x];
+            }
+            acc[i] = a;
+        }
+        if (!UNI)
+            wave_keyed_add(slot_rays, rq, (unsigned long long)(c_sh + c_rl + c_rr + c_df - before), i < m);
+    }
+    /* block sums of the spawn counts, in the stack's LDS (no walk is left) */
+    __syncthreads();
+    uint32_t *bcnt = stk;
+    if (threadIdx.x < 4) bcnt[threadIdx.x] = 0;
+    __syncthreads();
     if (c_sh) atomicAdd(&bcnt[0], c_sh);
     if (c_rl) atomicAdd(&bcnt[1], c_rl);
     if (c_rr) atomicAdd(&bcnt[2], c_rr);
@@ -983,6 +1149,8 @@ struct Engine {
     hipEvent_t tables_ready = nullptr;
     int nlanes = 1; /* lanes a pass spreads its primary chunks over */
     int sparse_from = 64; /* first bounce level traced by the SPARSE kernel instances */
+    bool chain = true;    /* one-child chains in one kernel when the pass allows it */
+    bool attr = true;     /* some scene in flight has meshes with vertex attributes */
     std::map<const void *, uint32_t> pgrid; /* persistent grid per traversal kernel instance */
     /* captured descents, keyed by everything their launches depend on */
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
@@ -1145,6 +1313,7 @@ void apply_tuning(Engine &E)
     E.persistent = t.persistent != 0;
     E.nlanes = t.lanes;
     E.sparse_from = t.sparse_from;
+    E.chain = t.chain != 0;
 }
 
 /* places a request in the arena and the active list */
@@ -1259,6 +1428,7 @@ int build_tables(Engine &E)
     E.L = 0;
     E.K = 2;
     E.Kb = 0;
+    E.attr = false;
     E.maxiter = 0;
     E.photons = false;
     E.more = false;
@@ -1296,6 +1466,7 @@ int build_tables(Engine &E)
         const int br = ((g->pos_reflect || any) ? 1 : 0) + ((g->pos_refract || any) ? 1 : 0) +
                        (Q.P.gi ? Q.P.diffuse_samples : 0);
         E.Kb = std::max(E.Kb, br);
+        E.attr = E.attr || g->attr_meshes;
     }
     E.uni = E.live == 1 && E.slot_job[0] != nullptr;
     E.one_scene = E.hscenes.size() == 1;
@@ -1479,6 +1650,36 @@ bool may_overflow(const Engine &E, uint32_t m)
     return false;
 }
 
+/* whether this pass's samples are one-child chains (k_chain) */
+bool chain_pass(const Engine &E)
+{
+    return E.chain && !E.photons && !E.counting && E.Kb <= 1 && E.L <= 4;
+}
+
+/* the primary chunk [c0, c0 + m) as one-child chains: one kernel */
+int launch_chain(Level &LV, uint32_t m)
+{
+    Engine &E = *LV.E;
+    Engine::Lane &ln = *LV.ln;
+    sray_gpu_ctx *c = E.c;
+    constexpr int BS = 128;
+    ProfScope ps(c, "k_chain", ln.s);
+    auto pick = [&](auto u, auto o, auto n) { return E.uni ? u : E.one_scene ? o : n; };
+    auto kern = E.attr ? (E.L <= 1 ? pick(k_chain<true, true, 1, true, BS>, k_chain<false, true, 1, true, BS>,
+                                          k_chain<false, false, 1, true, BS>)
+                                   : pick(k_chain<true, true, 4, true, BS>, k_chain<false, true, 4, true, BS>,
+                                          k_chain<false, false, 4, true, BS>))
+                       : (E.L <= 1 ? pick(k_chain<true, true, 1, false, BS>, k_chain<false, true, 1, false, BS>,
+                                          k_chain<false, false, 1, false, BS>)
+                                   : pick(k_chain<true, true, 4, false, BS>, k_chain<false, true, 4, false, BS>,
+                                          k_chain<false, false, 4, false, BS>));
+    kern<<<blocks(m, BS), BS, 0, ln.s>>>(LV.T, LV.active, LV.c0, m, E.L, E.K, dptr<float4>(ln.acc),
+                                         dptr<unsigned long long>(E.stats), dptr<unsigned long long>(E.slotrays),
+                                         dptr<uint32_t>(E.errf));
+    HIPCHK(c, hipGetLastError());
+    return SRAY_OK;
+}
+
 /* ray generation and the descent of the primary chunk [c0, c0 + m): as
  * kernel launches, or as one captured graph replayed (same kernels, same
  * order) */
@@ -1635,14 +1836,15 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
     /* a level may outgrow one sub-chunk: then each chunk's level counts are
      * read after its descent and the remaining sub-chunks run before the
      * lane takes its next chunk (never needed when mchunk * Kb^maxiter fits) */
-    const bool ovf = may_overflow(E, mchunk);
+    const bool chain = chain_pass(E);
+    const bool ovf = !chain && may_overflow(E, mchunk);
     for (uint32_t g0 = 0; g0 < nchunks; g0 += nl) {
         const uint32_t g1 = std::min(nchunks, g0 + nl);
         for (uint32_t j = g0; j < g1; j++) {
             const uint32_t c0 = j * mchunk, m = std::min(mchunk, E.nact - c0);
             Engine::Lane &ln = E.lanes[j - g0];
             Level LV{&E, &ln, T, A, c0};
-            if ((rc = launch_chunk(LV, m, nl == 1))) return rc;
+            if ((rc = chain ? launch_chain(LV, m) : launch_chunk(LV, m, nl == 1))) return rc;
             if (!ovf)
                 finish<<<blocks(m, 256), 256, 0, ln.s>>>(T, A, c0, m, dptr<float4>(ln.acc), dptr<float4>(E.fb),
                                                          dptr<float4>(E.subpix), dptr<int>(E.nsamp),
@@ -2039,7 +2241,8 @@ int check_tuning(sray_gpu_ctx *c, const sray_tuning &t)
               (t.gather_lanes == 8 || t.gather_lanes == 16 || t.gather_lanes == 32) && t.chunk_log2 >= 16 &&
               t.chunk_log2 <= 21 && (t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f) &&
               (t.graphs == 0 || t.graphs == 1) && (t.persistent == 0 || t.persistent == 1) &&
-              (t.lanes == 1 || t.lanes == 2 || t.lanes == 4) && t.sparse_from >= 0 && t.sparse_from <= 64;
+              (t.lanes == 1 || t.lanes == 2 || t.lanes == 4) && t.sparse_from >= 0 && t.sparse_from <= 64 &&
+              (t.chain == 0 || t.chain == 1);
     return ok ? SRAY_OK : fail(c, SRAY_ERR_ARG, "tuning value outside the supported set");
 }
 
@@ -2078,7 +2281,7 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
     static const float cells[] = {0.5f, 1.0f, 2.0f};
     bool photons = p->caust_photons || p->gi_photons;
     static const int nlanes[] = {1, 2, 4}, sparse[] = {0, 1, 2, 64};
-    for (int knob = 0; knob < 9; knob++) {
+    for (int knob = 0; knob < 10; knob++) {
         if ((knob == 2 || knob == 4) && !photons) continue;
         int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 3 : knob == 7 ? 3
                : knob == 8 ? 4 : 2;
@@ -2092,7 +2295,8 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
             else if (knob == 5) t.graphs = k;
             else if (knob == 6) t.persistent = k;
             else if (knob == 7) t.lanes = nlanes[k];
-            else t.sparse_from = sparse[k];
+            else if (knob == 8) t.sparse_from = sparse[k];
+            else t.chain = k;
             if (!memcmp(&t, &cur, sizeof t)) continue;
             c->tune = t;
             double ms = 0;

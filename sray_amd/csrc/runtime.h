@@ -63,6 +63,8 @@ struct GpuScene {
      * the bounce levels): any Phong material, one with reflect > 0, one
      * with refract > 0 */
     bool has_phong = false, pos_reflect = false, pos_refract = false;
+    /* some mesh object carries vertex attributes (interpolated at hits) */
+    bool attr_meshes = false;
 };
 
 struct Engine; /* multi-request wavefront renderer (engine.hip) */
@@ -97,7 +99,7 @@ struct sray_gpu_ctx {
     };
     std::map<std::string, KStat> kstats;
     /* tuning of the renderers and photon grids */
-    sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 4, 64};
+    sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 4, 64, 1};
     /* status stream (fd < 0: off) and the lock that keeps records whole */
     std::atomic<int> status_fd{-1};
     std::mutex status_mu;

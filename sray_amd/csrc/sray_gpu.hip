@@ -628,6 +628,7 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
             d.root = mesh_root[o.mesh];
             d.tri_base = s->meshes[o.mesh].tri_start;
             d.has_attr = (s->tri_normal ? 1 : 0) | (s->tri_texcoord ? 2 : 0) | (s->tri_tangent ? 4 : 0);
+            if (d.has_attr) g->attr_meshes = true;
         }
     }
     std::vector<DevMat> mats(nmat + 1);

@@ -168,7 +168,9 @@ using IorStack = IorStackT<IOR_DEPTH>;
  * world ray (ro, rd) at time; N is normalised for triangles only, as
  * SurfPoint::normal in the reference.  want_uv: texture coordinates of
  * spheres and cylinders (meshes get them from interp_face when they carry
- * attributes). */
+ * attributes; ATTR = false leaves that path out, for scenes without such
+ * meshes, which keeps its registers out of the caller). */
+template <bool ATTR = true>
 __device__ __forceinline__ void hit_surface(const SceneDev &S, const DevObj &ob, int time, V3 ro, V3 rd, float t,
                                             uint32_t ti, bool want_uv, V3 &Pw, V3 &N, float &tu, float &tv)
 {
@@ -207,7 +209,7 @@ __device__ __forceinline__ void hit_surface(const SceneDev &S, const DevObj &ob,
         float4 A = S.tri[(size_t)ti * 3], B = S.tri[(size_t)ti * 3 + 1], C = S.tri[(size_t)ti * 3 + 2];
         V3 gn = v3(A.w, B.w, C.w); /* (v1-v0) x (v2-v0), stored at upload */
         V3 nrm = gn;
-        if (ob.has_attr) {
+        if (ATTR && ob.has_attr) {
             V3 in, ig;
             interp_face(S.tri, (ob.has_attr & 1) ? S.tri_n : nullptr, (ob.has_attr & 4) ? S.tri_g : nullptr,
                         (ob.has_attr & 2) ? S.tri_uv : nullptr, ti, p, gn, in, ig, tu, tv);

@@ -98,10 +98,51 @@ def test_lanes_give_the_same_image(ctx, config, variant, size):
     ctx.release(sid)
 
 
+@pytest.mark.parametrize("config,variant,size,samples", [(0, 0, (320, 240), (1, 1)), (0, 0, (160, 120), (1, 4)),
+                                                          (3, 2000, (128, 96), (8, 8))])
+def test_chain_gives_the_same_image(ctx, config, variant, size, samples):
+    """one-child chains in one kernel (reflect-only spheres; a motion-blurred
+    batch scene) render the image and ray counts of the level queues"""
+    hs = S.HostScene.generate(config, variant=variant)
+    sid = ctx.upload(hs)
+    p = S.render_params(*size, samples=samples, max_var=0.01, iter=5, mblur=1 if config == 3 else 0, seed=11)
+    ctx.set_tuning(chain=0)
+    try:
+        base, bst = ctx.render(sid, p)
+        ctx.set_tuning(chain=1)
+        img, st = ctx.render(sid, p)
+    finally:
+        ctx.set_tuning(chain=1)
+    assert np.array_equal(img, base)
+    assert st.rays() == bst.rays() and st.rays_shadow == bst.rays_shadow and st.rays_reflect == bst.rays_reflect
+    ctx.release(sid)
+
+
+def test_chain_mixed_scenes_serve(ctx):
+    """requests on two sphere scenes in flight together (one-child chains of
+    several scenes and requests) equal the same requests on the level queues"""
+    a = S.HostScene.generate(0, seed=5)
+    b = S.HostScene.generate(0, seed=6)
+    sa, sb = ctx.upload(a), ctx.upload(b)
+    jobs = [(sa, S.render_params(96, 64, samples=(1, 4), max_var=0.01, iter=5, seed=1), None, None),
+            (sb, S.render_params(64, 80, rect=(8, 8, 56, 72), iter=4, seed=2), None, None),
+            (sa, S.render_params(50, 40, iter=6, seed=3), None, None)]
+    out = {}
+    for ch in (0, 1):
+        ctx.set_tuning(chain=ch)
+        res = [ctx.poll(t) for t in ctx.submit_many(jobs)]
+        out[ch] = [(img, r.rays) for img, r in res]
+    ctx.set_tuning(chain=1)
+    for (i0, r0), (i1, r1) in zip(out[0], out[1]):
+        assert np.array_equal(i0, i1) and r0 == r1
+    ctx.release(sa)
+    ctx.release(sb)
+
+
 def test_bad_tuning_rejected(ctx):
     for kw in (dict(trace_block=96), dict(tile_w=12), dict(gather_lanes=4), dict(chunk_log2=22),
                dict(gather_cell=3.0), dict(graphs=2), dict(persistent=2), dict(lanes=3), dict(lanes=0),
-               dict(sparse_from=-1), dict(sparse_from=65)):
+               dict(sparse_from=-1), dict(sparse_from=65), dict(chain=2)):
         with pytest.raises(S.SrayError):
             ctx.set_tuning(**kw)
 
@@ -118,7 +159,7 @@ def test_autotune(ctx):
     after, _ = ctx.render(sid, p)
     assert float(np.abs(after - before).max()) <= TOL
     ctx.set_tuning(trace_block=128, tile_w=64, gather_lanes=16, chunk_log2=21, gather_cell=1.0, graphs=1, persistent=0,
-                   lanes=4, sparse_from=64)
+                   lanes=4, sparse_from=64, chain=1)
     ctx.release(sid)
 
 
