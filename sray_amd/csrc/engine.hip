@@ -1130,9 +1130,14 @@ struct Engine {
         hipEvent_t done = nullptr;
         /* second stream: a level's shadow rays and accumulation run there,
          * beside the next level's compaction and closest-hit trace (evA:
-         * the level is shaded; evB: its accumulation is done) */
+         * the level is shaded; evB[p]: the accumulation of the last level of
+         * parity p is done) */
         hipStream_t s2 = nullptr;
-        hipEvent_t evA = nullptr, evB = nullptr;
+        hipEvent_t evA = nullptr, evB[2] = {nullptr, nullptr};
+        /* the shading outputs (direct, sq, scon, rslot, occ, greq) are held
+         * twice, one set per level parity, pstride rays apart, so a level
+         * can be shaded while the previous one's shadow rays are traced */
+        uint64_t pstride = 0;
         std::vector<DevBuf> qlev;
         DevBuf hits, direct, sq, scon, rslot, occ, kids, kflag, acc, greq;
         DevBuf lvcnt, cmpb; /* per-level ray counts; compaction block counts / offsets */
@@ -1517,7 +1522,7 @@ template <typename K> uint32_t persistent_grid(Engine &E, K kern, int bs)
  * compaction and the next step's closest-hit trace; the next step's
  * shading (which reuses their buffers) waits for them (first = no earlier
  * step to wait for). */
-int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, bool first)
+int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
 {
     Engine &E = *LV.E;
     Engine::Lane &ln = *LV.ln;
@@ -1530,6 +1535,15 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, bool first)
     unsigned long long *rc6 = dptr<unsigned long long>(E.rcounts);
     unsigned long long *stats = dptr<unsigned long long>(E.stats);
     unsigned long long *srays = dptr<unsigned long long>(E.slotrays);
+    /* this level's set of shading outputs */
+    const int par = b & 1;
+    const size_t po = (size_t)par * ln.pstride, pl = po * (size_t)std::max(E.L, 1);
+    float4 *direct = dptr<float4>(ln.direct) + po;
+    SRay *sq = dptr<SRay>(ln.sq) + pl;
+    float4 *scon = dptr<float4>(ln.scon) + pl;
+    uint32_t *rslot = dptr<uint32_t>(ln.rslot) + po;
+    uint8_t *occ = dptr<uint8_t>(ln.occ) + pl;
+    GatherReq *greq = E.photons ? dptr<GatherReq>(ln.greq) + po : nullptr;
     const int bs = E.trace_block;
     const uint32_t small_grid = std::min<uint32_t>(blocks(cap, 256), 2048u);
     /* traversal grid: one run of 64 rays per wave, or the resident waves */
@@ -1545,7 +1559,9 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, bool first)
                                : (E.one_scene ? trace_q_of<false, true>(bs, P, SP) : trace_q_of<false, false>(bs, P, SP));
         kern<<<tgrid(kern, cap), bs, 0, st>>>(LV.T, q, sn, dptr<int4>(ln.hits), rc6);
     }
-    if (!first) HIPCHK(c, hipStreamWaitEvent(st, ln.evB, 0));
+    /* the set was last used two steps back (k-th step of this descent; the
+     * steps before it were joined) */
+    if (k >= 2) HIPCHK(c, hipStreamWaitEvent(st, ln.evB[par], 0));
     {
         ProfScope ps(c, "k_shade", st);
         auto kern = E.photons ? (E.uni ? k_shade<true, true, true>
@@ -1553,9 +1569,8 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, bool first)
                               : (E.uni ? k_shade<false, true, true>
                                        : E.one_scene ? k_shade<false, false, true> : k_shade<false, false, false>);
         kern<<<small_grid, 256, 0, st>>>(LV.T, LV.active, LV.c0, q, dptr<int4>(ln.hits), sn, E.L, E.K,
-                                               dptr<float4>(ln.direct), dptr<SRay>(ln.sq), dptr<float4>(ln.scon),
-                                               dptr<uint32_t>(ln.rslot), dptr<QRay>(ln.kids),
-                                               dptr<uint8_t>(ln.kflag), dptr<GatherReq>(ln.greq), stats, srays,
+                                               direct, sq, scon, rslot, dptr<QRay>(ln.kids),
+                                               dptr<uint8_t>(ln.kflag), greq, stats, srays,
                                                dptr<uint32_t>(E.errf));
     }
     HIPCHK(c, hipEventRecord(ln.evA, st));
@@ -1564,7 +1579,7 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, bool first)
         ProfScope ps(c, "k_gather", st2);
         auto kern = E.uni ? gather_of<true>(E.gather_lanes) : gather_of<false>(E.gather_lanes);
         const uint32_t g = std::min<uint32_t>(blocks((size_t)cap * E.gather_lanes, 256), 8192u);
-        kern<<<g, 256, 0, st2>>>(LV.T, q, dptr<GatherReq>(ln.greq), sn, dptr<float4>(ln.direct));
+        kern<<<g, 256, 0, st2>>>(LV.T, q, greq, sn, direct);
     }
     if (E.L) {
         ProfScope ps(c, "k_trace_shadow", st2);
@@ -1573,16 +1588,14 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, bool first)
                                               : trace_shadow_of<true, false>(bs, P, SP))
                                : (E.one_scene ? trace_shadow_of<false, true>(bs, P, SP)
                                               : trace_shadow_of<false, false>(bs, P, SP));
-        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, st2>>>(LV.T, dptr<SRay>(ln.sq), dptr<float4>(ln.scon),
-                                                                    sn, E.L, dptr<uint8_t>(ln.occ), rc6 + 6);
+        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, st2>>>(LV.T, sq, scon, sn, E.L, occ, rc6 + 6);
     }
     {
         ProfScope ps(c, "k_accum", st2);
-        k_accum<<<small_grid, 256, 0, st2>>>(dptr<uint32_t>(ln.rslot), dptr<float4>(ln.direct),
-                                                    dptr<float4>(ln.scon), dptr<uint8_t>(ln.occ), sn, E.L,
+        k_accum<<<small_grid, 256, 0, st2>>>(rslot, direct, scon, occ, sn, E.L,
                                                     dptr<float4>(ln.acc));
     }
-    HIPCHK(c, hipEventRecord(ln.evB, st2));
+    HIPCHK(c, hipEventRecord(ln.evB[par], st2));
     if (b < E.maxiter) {
         ProfScope ps(c, "k_compact", st);
         uint32_t *bc = dptr<uint32_t>(ln.cmpb);
@@ -1604,13 +1617,16 @@ int descend(Level &LV, int b, uint32_t off, uint32_t cap)
     Engine &E = *LV.E;
     int rc;
     uint64_t bound = cap;
+    int last = b;
     for (int d = b; d <= E.maxiter && bound > 0; d++) {
-        if ((rc = launch_step(LV, d, d == b ? off : 0u, (uint32_t)std::min<uint64_t>(bound, E.chunk), d == b)))
+        if ((rc = launch_step(LV, d, d == b ? off : 0u, (uint32_t)std::min<uint64_t>(bound, E.chunk), d - b)))
             return rc;
+        last = d;
         bound = std::min<uint64_t>(bound * (uint64_t)E.Kb, E.chunk);
     }
-    /* join: what follows on the lane's stream sees the last accumulation */
-    HIPCHK(E.c, hipStreamWaitEvent(LV.ln->s, LV.ln->evB, 0));
+    /* join: what follows on the lane's stream sees the last accumulation
+     * (the second stream runs its work in order) */
+    HIPCHK(E.c, hipStreamWaitEvent(LV.ln->s, LV.ln->evB[last & 1], 0));
     return SRAY_OK;
 }
 
@@ -1715,6 +1731,7 @@ int launch_chunk(Level &LV, uint32_t m, bool one_lane)
     const DevBuf *bufs[] = {&ln.hits, &ln.direct, &ln.sq, &ln.scon, &ln.rslot, &ln.occ, &ln.kids, &ln.kflag, &ln.acc,
                             &E.stats, &E.errf, &E.rcounts, &E.slotrays, &ln.greq, &ln.lvcnt, &ln.cmpb};
     for (const DevBuf *d : bufs) key.push_back((uint64_t)(uintptr_t)d->p);
+    key.push_back(ln.pstride);
     for (int b = 0; b <= E.maxiter; b++) key.push_back((uint64_t)(uintptr_t)ln.qlev[b].p);
     key.push_back((uint64_t)(uintptr_t)st);
     auto it = E.graphs.find(key);
@@ -1807,22 +1824,24 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
         if (!ln.done) HIPCHK(c, hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
         if (!ln.s2) HIPCHK(c, hipStreamCreateWithFlags(&ln.s2, hipStreamNonBlocking)); /* default priority */
         if (!ln.evA) HIPCHK(c, hipEventCreateWithFlags(&ln.evA, hipEventDisableTiming));
-        if (!ln.evB) HIPCHK(c, hipEventCreateWithFlags(&ln.evB, hipEventDisableTiming));
+        for (hipEvent_t &ev : ln.evB)
+            if (!ev) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         if (ln.qlev.size() < (size_t)E.maxiter + 1) ln.qlev.resize((size_t)E.maxiter + 1);
         if ((rc = ensure(c, ln.qlev[0], (size_t)mchunk * sizeof(QRay)))) return rc;
         for (int b = 1; b <= E.maxiter; b++)
             if ((rc = ensure(c, ln.qlev[b], (size_t)cap_rays * E.K * sizeof(QRay)))) return rc;
         if ((rc = ensure(c, ln.hits, (size_t)cap_rays * sizeof(int4))) ||
-            (rc = ensure(c, ln.direct, (size_t)cap_rays * 16)) ||
-            (rc = ensure(c, ln.sq, (size_t)cap_rays * L1 * sizeof(SRay))) ||
-            (rc = ensure(c, ln.scon, (size_t)cap_rays * L1 * sizeof(float4))) ||
-            (rc = ensure(c, ln.rslot, (size_t)cap_rays * sizeof(uint32_t))) ||
-            (rc = ensure(c, ln.occ, (size_t)cap_rays * L1)) ||
+            (rc = ensure(c, ln.direct, 2 * (size_t)cap_rays * 16)) ||
+            (rc = ensure(c, ln.sq, 2 * (size_t)cap_rays * L1 * sizeof(SRay))) ||
+            (rc = ensure(c, ln.scon, 2 * (size_t)cap_rays * L1 * sizeof(float4))) ||
+            (rc = ensure(c, ln.rslot, 2 * (size_t)cap_rays * sizeof(uint32_t))) ||
+            (rc = ensure(c, ln.occ, 2 * (size_t)cap_rays * L1)) ||
             (rc = ensure(c, ln.kids, (size_t)cap_rays * E.K * sizeof(QRay))) ||
             (rc = ensure(c, ln.kflag, (size_t)cap_rays * E.K)) || (rc = ensure(c, ln.acc, (size_t)mchunk * 16)) ||
-            (E.photons && (rc = ensure(c, ln.greq, (size_t)cap_rays * sizeof(GatherReq)))) ||
+            (E.photons && (rc = ensure(c, ln.greq, 2 * (size_t)cap_rays * sizeof(GatherReq)))) ||
             (rc = ensure(c, ln.lvcnt, 64 * 4)) || (rc = ensure(c, ln.cmpb, 2 * CMP_BLOCKS * 4)))
             return rc;
+        ln.pstride = cap_rays;
     }
 
     HIPCHK(c, hipMemsetAsync(E.errf.p, 0, 4, c->stream));
@@ -1953,7 +1972,8 @@ void engine_free(Engine *E)
             (void)hipStreamDestroy(ln.s2);
         }
         if (ln.evA) (void)hipEventDestroy(ln.evA);
-        if (ln.evB) (void)hipEventDestroy(ln.evB);
+        for (hipEvent_t ev : ln.evB)
+            if (ev) (void)hipEventDestroy(ev);
         DevBuf *lb[] = {&ln.hits, &ln.direct, &ln.sq, &ln.scon, &ln.rslot, &ln.occ, &ln.kids, &ln.kflag, &ln.acc,
                         &ln.greq, &ln.lvcnt, &ln.cmpb};
         for (DevBuf *b : lb) release(*b);
