@@ -131,12 +131,26 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
          * shared); a lane past the end has no ray */
         const uint32_t i = base + __lane_id();
         const bool live = i < n;
-        const SceneDev S = T.scenes[UNI || !live ? 0u : T.reqs[q[i].m.w >> SLOT_SHIFT].scene];
         /* the world ray is re-read from the queue when needed instead of
          * held in registers */
         const RayRecord<QRay> src{q, i};
-        HitRec h = traverse_src<false, COUNT, true, SPARSE>(S, src, live, &stk[threadIdx.x], BS, cnt);
-        if (live) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+        if (UNI) {
+            HitRec h = traverse_src<false, COUNT, true, SPARSE>(T.scenes[0], src, live, &stk[threadIdx.x], BS, cnt);
+            if (live) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+        } else {
+            /* one walk per distinct scene of the wave (a wave's rays nearly
+             * always come from one request), each with the scene's
+             * descriptor wave-uniform */
+            const uint32_t sidx = live ? T.reqs[q[i].m.w >> SLOT_SHIFT].scene : 0u;
+            unsigned long long todo = __ballot(live);
+            while (todo) {
+                const uint32_t s0 = read_lane_u(sidx, __ffsll(todo) - 1);
+                const bool mine = live && sidx == s0;
+                todo &= ~__ballot(mine);
+                HitRec h = traverse_src<false, COUNT, true, SPARSE>(T.scenes[s0], src, mine, &stk[threadIdx.x], BS, cnt);
+                if (mine) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+            }
+        }
     }
     flush_counts<COUNT>(cnt, counts);
 }
@@ -164,10 +178,23 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
         }
         /* the world ray stays in registers here: this kernel is at full
          * occupancy already, and re-reading it would only add loads */
-        const SceneDev S = T.scenes[UNI || !live ? 0u : T.reqs[__float_as_uint(d.w)].scene];
         const RayRegs src{v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w)};
-        HitRec h = traverse_src<true, COUNT, true, SPARSE>(S, src, live, &stk[threadIdx.x], BS, cnt);
-        if (i < n) occ[i] = h.obj >= 0 ? 1 : 0;
+        if (UNI) {
+            HitRec h = traverse_src<true, COUNT, true, SPARSE>(T.scenes[0], src, live, &stk[threadIdx.x], BS, cnt);
+            if (i < n) occ[i] = h.obj >= 0 ? 1 : 0;
+        } else {
+            /* one walk per distinct scene of the wave, as in k_trace_q */
+            const uint32_t sidx = live ? T.reqs[__float_as_uint(d.w)].scene : 0u;
+            if (i < n && !live) occ[i] = 0;
+            unsigned long long todo = __ballot(live);
+            while (todo) {
+                const uint32_t s0 = read_lane_u(sidx, __ffsll(todo) - 1);
+                const bool mine = live && sidx == s0;
+                todo &= ~__ballot(mine);
+                HitRec h = traverse_src<true, COUNT, true, SPARSE>(T.scenes[s0], src, mine, &stk[threadIdx.x], BS, cnt);
+                if (mine) occ[i] = h.obj >= 0 ? 1 : 0;
+            }
+        }
     }
     flush_counts<COUNT>(cnt, counts);
 }
