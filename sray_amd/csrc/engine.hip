@@ -520,7 +520,12 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
                                                unsigned long long *__restrict__ slot_rays, uint32_t *__restrict__ err)
 {
     __shared__ uint32_t bcnt[5];
+    /* per-request ray counts of the block (many requests: one global atomic
+     * per request and block instead of one per request and wave) */
+    __shared__ uint32_t scnt[UNI ? 1 : MAX_SLOTS];
     if (threadIdx.x < 5) bcnt[threadIdx.x] = 0;
+    if (!UNI)
+        for (uint32_t k = threadIdx.x; k < MAX_SLOTS; k += blockDim.x) scnt[k] = 0;
     __syncthreads();
     const uint32_t n = step_n(sn);
     uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
@@ -533,8 +538,7 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
             shade_ray<PHOTONS, UNI, SCN1, true>(T, active, c0, q[i], hits[i], L, K, sink, c_sh, c_rl, c_rr, c_df);
         }
         if (!UNI)
-            wave_keyed_add(slot_rays, i < n ? q[i].m.w >> SLOT_SHIFT : 0u,
-                           (unsigned long long)(c_sh + c_rl + c_rr + c_df - before), i < n);
+            wave_keyed_add(scnt, i < n ? q[i].m.w >> SLOT_SHIFT : 0u, c_sh + c_rl + c_rr + c_df - before, i < n);
     }
     if (c_sh) atomicAdd(&bcnt[0], c_sh);
     if (c_rl) atomicAdd(&bcnt[1], c_rl);
@@ -542,6 +546,9 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
     if (c_df) atomicAdd(&bcnt[3], c_df);
     __syncthreads();
     if (threadIdx.x < 4 && bcnt[threadIdx.x]) atomicAdd(&stats[1 + threadIdx.x], (unsigned long long)bcnt[threadIdx.x]);
+    if (!UNI)
+        for (uint32_t k = threadIdx.x; k < MAX_SLOTS; k += blockDim.x)
+            if (scnt[k]) atomicAdd(&slot_rays[k], (unsigned long long)scnt[k]);
 }
 
 /* A lane's registers as the shading sink of the one-child chain: the direct
@@ -597,6 +604,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ATTR || LMAX
     __shared__ QRay kids[BS];
     uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
     TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    /* this lane's rays of its last request, added to the block's counts at
+     * the end (the grid has a lane per sample, so that is all of them) */
+    uint32_t last_rq = 0, last_cnt = 0;
     for (uint32_t gbase = blockIdx.x * (uint32_t)BS; gbase < m; gbase += gridDim.x * (uint32_t)BS) {
         const uint32_t i = gbase + threadIdx.x;
         const uint32_t before = c_sh + c_rl + c_rr + c_df;
@@ -653,20 +663,31 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ATTR || LMAX
             }
             acc[i] = a;
         }
-        if (!UNI)
-            wave_keyed_add(slot_rays, rq, (unsigned long long)(c_sh + c_rl + c_rr + c_df - before), i < m);
+        if (!UNI && i < m) {
+            const uint32_t add = c_sh + c_rl + c_rr + c_df - before;
+            if (last_cnt && rq != last_rq) atomicAdd(&slot_rays[last_rq], (unsigned long long)last_cnt);
+            if (rq != last_rq) last_cnt = 0;
+            last_rq = rq;
+            last_cnt += add;
+        }
     }
-    /* block sums of the spawn counts, in the stack's LDS (no walk is left) */
+    /* block sums of the spawn counts and of the per-request counts, in the
+     * stack's LDS (no walk is left) */
     __syncthreads();
-    uint32_t *bcnt = stk;
-    if (threadIdx.x < 4) bcnt[threadIdx.x] = 0;
+    uint32_t *bcnt = stk, *scnt = stk + 4;
+    static_assert(STACK * BS >= 4 + MAX_SLOTS, "the block sums fit in the stack's LDS");
+    for (uint32_t k = threadIdx.x; k < (UNI ? 4u : 4u + MAX_SLOTS); k += BS) stk[k] = 0;
     __syncthreads();
     if (c_sh) atomicAdd(&bcnt[0], c_sh);
     if (c_rl) atomicAdd(&bcnt[1], c_rl);
     if (c_rr) atomicAdd(&bcnt[2], c_rr);
     if (c_df) atomicAdd(&bcnt[3], c_df);
+    if (!UNI) wave_keyed_add(scnt, last_rq, last_cnt, last_cnt != 0u);
     __syncthreads();
     if (threadIdx.x < 4 && bcnt[threadIdx.x]) atomicAdd(&stats[1 + threadIdx.x], (unsigned long long)bcnt[threadIdx.x]);
+    if (!UNI)
+        for (uint32_t k = threadIdx.x; k < MAX_SLOTS; k += BS)
+            if (scnt[k]) atomicAdd(&slot_rays[k], (unsigned long long)scnt[k]);
 }
 
 /* photon gathers requested by k_shade: caustic irradiance (>= 8 photons,
@@ -881,22 +902,30 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
+/* active pixels per request slot over a grid-stride loop: counted per
+ * block in LDS, then one global atomic per request and block */
+__device__ __forceinline__ void slot_count(const uint32_t *__restrict__ active, uint32_t n,
+                                           const uint32_t *__restrict__ preq, uint32_t *__restrict__ cnt)
+{
+    __shared__ uint32_t scnt[MAX_SLOTS];
+    for (uint32_t k = threadIdx.x; k < MAX_SLOTS; k += blockDim.x) scnt[k] = 0;
+    __syncthreads();
+    for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
+        const uint32_t i = b + threadIdx.x;
+        wave_keyed_add(scnt, i < n ? preq[active[i]] : 0u, 1u, i < n);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < MAX_SLOTS; k += blockDim.x)
+        if (scnt[k]) atomicAdd(&cnt[k], scnt[k]);
+}
+
 /* active pixels per request slot, the list's length read on the device */
 __global__ void k_slot_count_dev(const uint32_t *__restrict__ active, const uint32_t *__restrict__ np,
                                  const uint32_t *__restrict__ preq, uint32_t *__restrict__ cnt)
 {
-    const uint32_t n = *np;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    wave_keyed_add(cnt, i < n ? preq[active[i]] : 0u, 1u, i < n);
+    slot_count(active, *np, preq, cnt);
 }
 
-/* active pixels per request slot */
-__global__ void k_slot_count(const uint32_t *__restrict__ active, uint32_t n, const uint32_t *__restrict__ preq,
-                             uint32_t *__restrict__ cnt)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    wave_keyed_add(cnt, i < n ? preq[active[i]] : 0u, 1u, i < n);
-}
 
 /* end of one sample pass: framebuffer accumulation in float
  * (src/render.cc:226-229) and the variance stop test (:232-237, 261-270) */
@@ -1929,7 +1958,7 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
         if (!E.uni) {
             /* per-request active counts of the new list, bounded by the old length */
             k_fill_u32<<<blocks(E.hi, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.slotcnt), E.hi, 0u);
-            k_slot_count_dev<<<blocks(E.nact, 256), 256, 0, c->stream>>>(
+            k_slot_count_dev<<<std::min<uint32_t>(blocks(E.nact, 256), 1024u), 256, 0, c->stream>>>(
                 dptr<uint32_t>(E.act[E.cur ^ 1]), dptr<uint32_t>(c->nsel), dptr<uint32_t>(E.preq),
                 dptr<uint32_t>(E.slotcnt));
             HIPCHK(c, hipMemcpyAsync(H->slotcnt, E.slotcnt.p, (size_t)E.hi * 4, hipMemcpyDeviceToHost, c->stream));
