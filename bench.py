@@ -399,8 +399,8 @@ def run_serve(args):
 def main():
     args = parse()
     if args.gpus > 1:
-        sys.exit("bench.py measures one GPU only: every box here has one MI355X and no multi-GPU "
-                 "figure is produced (run with --gpus 1)")
+        sys.exit("bench.py measures one GPU: the path runs on a single MI355X, and sharding over several "
+                 "GPUs is a design only (run with --gpus 1)")
     if args.copy_calibration:
         import sray_amd as S
         ctx = S.Context(0)
