@@ -66,6 +66,7 @@ def lib(fast=False):
         L.oracle_projmap.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, vp,
                                      C.POINTER(C.c_double)]
         L.oracle_surface.argtypes = [vp, vp, C.c_uint32, vp]
+        L.oracle_light_points.argtypes = [vp, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, vp]
         L.oracle_cell_from_dir.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]
         _libs[name] = L
@@ -165,6 +166,13 @@ class OracleScene:
 
     def diag(self):
         return self.L.oracle_scene_diag(self.h)
+
+    def light_points(self, light, n, seed=1, pixel=0):
+        """(n, 3) float64: sample points of one light, samples 0..n-1 of a pixel"""
+        out = np.zeros((n, 3), np.float64)
+        if self.L.oracle_light_points(self.h, light, seed, pixel, n, out.ctypes.data):
+            raise RuntimeError("oracle_light_points failed")
+        return out
 
     def close(self):
         if self.h:

@@ -2124,6 +2124,26 @@ static int do_projmap(Scene<T> &S, int light, int kind, int t0, int t1, uint64_t
 
 using namespace orc;
 
+/* n sample points of one light (Scene::light_point, shadow sample si = 0,
+ * path 0), sample index i for point i: the draws the renderer makes for
+ * sample i of a pixel */
+template <class T>
+static int do_light_points(const Scene<T> &S, int light, uint64_t seed, uint32_t pixel, uint32_t n, double *out)
+{
+    if (light < 0 || (size_t)light >= S.lights.size()) return -1;
+    typename Scene<T>::ShadeCtx sc{};
+    sc.seed = seed;
+    sc.pixel = pixel;
+    for (uint32_t i = 0; i < n; i++) {
+        sc.sample = i;
+        V3<T> v = S.light_point(S.lights[(size_t)light], sc, 0u, (uint32_t)light, 0u);
+        out[(size_t)i * 3] = (double)v.x;
+        out[(size_t)i * 3 + 1] = (double)v.y;
+        out[(size_t)i * 3 + 2] = (double)v.z;
+    }
+    return 0;
+}
+
 struct oracle_scene {
     AnyScene s;
 };
@@ -2233,6 +2253,13 @@ double oracle_scene_diag(oracle_scene *s)
 {
     if (!s) return 0;
     return s->s.prec == 32 ? s->s.f->diag : s->s.d->diag;
+}
+
+int oracle_light_points(oracle_scene *s, int light, uint64_t seed, uint32_t pixel, uint32_t n, double *out)
+{
+    if (!s || (n && !out)) return -1;
+    if (s->s.prec == 32) return do_light_points(*s->s.f, light, seed, pixel, n, out);
+    return do_light_points(*s->s.d, light, seed, pixel, n, out);
 }
 
 uint32_t oracle_rng_u32(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d)

@@ -105,6 +105,10 @@ int oracle_surface(oracle_scene *s, const sray_ray *rays, uint32_t n, float *out
 /* scene octree root diagonal (Scene::build_tree, src/scene.cc:384-387) */
 double oracle_scene_diag(oracle_scene *s);
 
+/* n sample points of light `light` (Light::get_point, src/light.cc:106-109,
+ * 272-275, 346-354), the draws of samples 0..n-1 of one pixel; 3 doubles each */
+int oracle_light_points(oracle_scene *s, int light, uint64_t seed, uint32_t pixel, uint32_t n, double *out);
+
 /* counter-based RNG used by both implementations (exposed for KAT tests) */
 uint32_t oracle_rng_u32(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d);
 uint64_t oracle_splitmix64(uint64_t x);
