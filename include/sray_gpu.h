@@ -478,6 +478,12 @@ void sray_photon_free(sray_photon *p);
  * first) as a PFM file, "PF\n<w> <h>\n-1.0\n" then RGB float32
  * little-endian, bottom row first. */
 int  sray_image_write_pfm(const char *path, const float *rgba, int width, int height);
+/* Replaces Image::save (src/img.cc:80-83) for the reference's default
+ * output, output.png / frame%04d.png (src/sray.cc:150-155): the RGBA float
+ * framebuffer (row-major, top row first) as an 8-bit RGBA PNG, each channel
+ * value * 255 truncated toward zero and clamped to [0, 255], the float to
+ * byte conversion libimago applies before its PNG writer. */
+int  sray_image_write_png(const char *path, const float *rgba, int width, int height);
 /* The backend status protocol (emit_status, src/render.cc:279-289): one
  * record is "%c %6d %6d %6d %6d\n" at the start of SRAY_STATUS_RECORD
  * bytes, the rest zero.  'f' frame (x blocks, y blocks, block size,

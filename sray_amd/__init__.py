@@ -225,6 +225,13 @@ def write_pfm(path, rgba):
     _io_check(lib().sray_image_write_pfm(os.fsencode(path), img.ctypes.data_as(C.POINTER(C.c_float)), w, h))
 
 
+def write_png(path, rgba):
+    """the float framebuffer (h, w, 4), top row first, as an 8-bit RGBA PNG"""
+    img = np.ascontiguousarray(rgba, np.float32)
+    h, w = img.shape[:2]
+    _io_check(lib().sray_image_write_png(os.fsencode(path), img.ctypes.data_as(C.POINTER(C.c_float)), w, h))
+
+
 def status_record(st, a1, a2, a3, a4):
     """one record of the backend status protocol (src/render.cc:279-289)"""
     buf = C.create_string_buffer(abi.SRAY_STATUS_RECORD)

@@ -154,7 +154,8 @@ EXPORTS = [
     "sray_gpu_malloc", "sray_gpu_free", "sray_gpu_memcpy", "sray_gpu_synchronize",
     "sray_gpu_shoot_photons", "sray_gpu_pmap_download", "sray_projmap_build", "sray_projmap_cell",
     "sray_gpu_set_tuning", "sray_gpu_get_tuning", "sray_gpu_autotune", "sray_gpu_autotune_octree",
-    "sray_photon_dump", "sray_photon_restore", "sray_photon_free", "sray_image_write_pfm", "sray_status_record",
+    "sray_photon_dump", "sray_photon_restore", "sray_photon_free", "sray_image_write_pfm", "sray_image_write_png",
+    "sray_status_record",
     "sray_io_last_error", "sray_gpu_set_status_fd",
 ]
 SRAY_STATUS_RECORD = 32
@@ -210,6 +211,7 @@ def declare(lib):
         "sray_photon_restore": (i32, [C.c_char_p, f32, P(P(Photon)), P(u32)]),
         "sray_photon_free": (None, [P(Photon)]),
         "sray_image_write_pfm": (i32, [C.c_char_p, P(f32), C.c_int, C.c_int]),
+        "sray_image_write_png": (i32, [C.c_char_p, P(f32), C.c_int, C.c_int]),
         "sray_status_record": (i32, [C.c_char, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p]),
         "sray_io_last_error": (C.c_char_p, []),
         "sray_gpu_set_status_fd": (i32, [vp, C.c_int]),

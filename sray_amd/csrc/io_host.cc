@@ -1,7 +1,7 @@
 /*
  * io_host.cc — host-side input/output of the path: the photon-map text
  * dump and restore (PhotonMap::dump / restore, src/pmap.cc:180-251), the
- * float framebuffer written as PFM (Image::save, src/img.cc:80-83), and the
+ * framebuffer written as PFM or PNG (Image::save, src/img.cc:80-83), and the
  * 32-byte status records of the reference's backend protocol (emit_status,
  * src/render.cc:279-289), which the serving engine also writes.
  */
@@ -11,6 +11,8 @@
 #include <new>
 #include <string>
 #include <vector>
+
+#include <zlib.h>
 
 #include "host.h"
 
@@ -119,6 +121,79 @@ int sray_image_write_pfm(const char *path, const float *rgba, int width, int hei
         }
         ok = fwrite(row.data(), sizeof(float), row.size(), fp) == row.size();
     }
+    if (fclose(fp) != 0) ok = false;
+    return ok ? SRAY_OK : io_fail(SRAY_ERR_ARG, std::string("failed to write image: ") + path);
+}
+
+namespace {
+void put_be32(std::vector<unsigned char> &v, uint32_t x)
+{
+    v.push_back((unsigned char)(x >> 24));
+    v.push_back((unsigned char)(x >> 16));
+    v.push_back((unsigned char)(x >> 8));
+    v.push_back((unsigned char)x);
+}
+
+/* one PNG chunk: length, type, data, CRC-32 of type and data */
+bool put_chunk(FILE *fp, const char type[4], const unsigned char *data, size_t n)
+{
+    std::vector<unsigned char> head;
+    put_be32(head, (uint32_t)n);
+    head.insert(head.end(), type, type + 4);
+    uLong crc = crc32(0L, (const Bytef *)type, 4);
+    if (n) crc = crc32(crc, data, (uInt)n);
+    std::vector<unsigned char> tail;
+    put_be32(tail, (uint32_t)crc);
+    return fwrite(head.data(), 1, head.size(), fp) == head.size() && (n == 0 || fwrite(data, 1, n, fp) == n) &&
+           fwrite(tail.data(), 1, tail.size(), fp) == tail.size();
+}
+
+/* a channel as a byte: value * 255 truncated toward zero, clamped to [0, 255] */
+unsigned char to_byte(float f)
+{
+    const double v = (double)f * 255.0;
+    if (!(v > 0.0)) return 0; /* negative, zero or NaN */
+    if (v >= 255.0) return 255;
+    return (unsigned char)(int)v;
+}
+} // namespace
+
+/* The output image of the reference's main program (output.png or
+ * frame%04d.png, src/sray.cc:150-155, saved by Image::save, src/img.cc:80-83,
+ * through libimago, which converts the float image to 8-bit RGBA): a PNG of
+ * colour type 6 (RGBA, 8 bits per channel), top row first, every scanline
+ * with filter 0, one zlib stream in one IDAT chunk */
+int sray_image_write_png(const char *path, const float *rgba, int width, int height)
+{
+    if (!path || !rgba || width <= 0 || height <= 0 || (uint64_t)width * (uint64_t)height > (1ull << 30))
+        return io_fail(SRAY_ERR_ARG, "sray_image_write_png: bad arguments");
+    const size_t stride = (size_t)width * 4 + 1;
+    std::vector<unsigned char> raw, packed;
+    try {
+        raw.resize(stride * (size_t)height);
+        packed.resize(compressBound((uLong)raw.size()));
+    } catch (const std::bad_alloc &) {
+        return io_fail(SRAY_ERR_NOMEM, "image too large for host memory");
+    }
+    for (int y = 0; y < height; y++) {
+        unsigned char *row = &raw[(size_t)y * stride];
+        row[0] = 0; /* filter type None */
+        const float *src = rgba + (size_t)y * width * 4;
+        for (size_t k = 0; k < (size_t)width * 4; k++) row[1 + k] = to_byte(src[k]);
+    }
+    uLongf plen = (uLongf)packed.size();
+    if (compress2(packed.data(), &plen, raw.data(), (uLong)raw.size(), 6) != Z_OK)
+        return io_fail(SRAY_ERR_ARG, "sray_image_write_png: compression failed");
+    std::vector<unsigned char> ihdr;
+    put_be32(ihdr, (uint32_t)width);
+    put_be32(ihdr, (uint32_t)height);
+    const unsigned char rest[5] = {8, 6, 0, 0, 0}; /* depth, RGBA, deflate, filter set 0, no interlace */
+    ihdr.insert(ihdr.end(), rest, rest + 5);
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return io_fail(SRAY_ERR_ARG, std::string("failed to open image file: ") + path + ": " + strerror(errno));
+    static const unsigned char sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    bool ok = fwrite(sig, 1, 8, fp) == 8 && put_chunk(fp, "IHDR", ihdr.data(), ihdr.size()) &&
+              put_chunk(fp, "IDAT", packed.data(), plen) && put_chunk(fp, "IEND", nullptr, 0);
     if (fclose(fp) != 0) ok = false;
     return ok ? SRAY_OK : io_fail(SRAY_ERR_ARG, std::string("failed to write image: ") + path);
 }

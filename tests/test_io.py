@@ -107,3 +107,47 @@ def test_status_records():
         assert len(rec) == 32 and rec == want + b"\0" * (32 - len(want))
     with pytest.raises(S.SrayError):
         S.status_record("d", 10 ** 9, 10 ** 9, 0, 0)
+
+
+def _read_png(data):
+    """chunks of a PNG (CRCs checked) -> (IHDR fields, raw scanlines)"""
+    import struct
+    import zlib
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, []
+    while pos < len(data):
+        n, = struct.unpack(">I", data[pos:pos + 4])
+        typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        crc, = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert crc == zlib.crc32(typ + body)
+        chunks.append((typ, body))
+        pos += 12 + n
+    assert [c[0] for c in chunks] == [b"IHDR", b"IDAT", b"IEND"]
+    ihdr = struct.unpack(">IIBBBBB", chunks[0][1])
+    return ihdr, zlib.decompress(chunks[1][1])
+
+
+def test_png_rgba8(tmp_path):
+    """8-bit RGBA, colour type 6, top row first, filter 0 on every row; each
+    channel is value * 255 truncated toward zero and clamped to [0, 255]"""
+    rng = np.random.default_rng(2)
+    h, w = 7, 13
+    img = rng.uniform(-0.2, 1.2, size=(h, w, 4)).astype(np.float32)
+    img[0, :6, 0] = [0.0, 1.0 / 255, 0.5, 254.999 / 255, 1.0, np.nan]
+    path = tmp_path / "out.png"
+    S.write_png(path, img)
+    (W, H, depth, ctype, comp, filt, inter), raw = _read_png(path.read_bytes())
+    assert (W, H, depth, ctype, comp, filt, inter) == (w, h, 8, 6, 0, 0, 0)
+    rows = np.frombuffer(raw, np.uint8).reshape(h, 1 + w * 4)
+    assert np.all(rows[:, 0] == 0)
+    v = img.astype(np.float64) * 255.0
+    want = np.where(np.isnan(v), 0, np.clip(np.trunc(np.nan_to_num(v)), 0, 255)).astype(np.uint8)
+    assert np.array_equal(rows[:, 1:].reshape(h, w, 4), want)
+    assert list(rows[0, 1:25:4]) == [0, 1, 127, 254, 255, 0]
+
+
+def test_png_errors(tmp_path):
+    with pytest.raises(S.SrayError):
+        S.write_png(tmp_path / "missing_dir" / "x.png", np.zeros((2, 2, 4), np.float32))
+    with pytest.raises(S.SrayError):
+        S.write_png(tmp_path / "empty.png", np.zeros((0, 2, 4), np.float32))
