@@ -78,7 +78,7 @@ struct SceneDev {
     uint32_t scene_root;
     int32_t nlights;
     int32_t nobjs;
-    int32_t pad;
+    int32_t fast_boxes;      /* every octree box finite, lo <= hi, |bounds| <= 2^120 */
     float4 env_ambient;
     float4 env_color;
     float cam[12];

@@ -727,6 +727,13 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     D.scene_root = sroot;
     D.nlights = (int32_t)s->num_lights;
     D.nobjs = (int32_t)s->num_objects;
+    D.fast_boxes = 1;
+    for (size_t k = 0; k < ft.lo.size() / 4 && D.fast_boxes; k++)
+        for (int a = 0; a < 3; a++) {
+            const float l = ft.lo[k * 4 + a], h = ft.hi[k * 4 + a];
+            if (!(std::isfinite(l) && std::isfinite(h) && l <= h && std::fabs(l) <= 0x1p120f && std::fabs(h) <= 0x1p120f))
+                D.fast_boxes = 0;
+        }
     D.env_ambient = make_float4(s->env_ambient[0], s->env_ambient[1], s->env_ambient[2], s->env_ambient[3]);
     D.env_color = make_float4(s->env_color[0], s->env_color[1], s->env_color[2], s->env_color[3]);
     memcpy(D.cam, s->camera.xform, sizeof D.cam);

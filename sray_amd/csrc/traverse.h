@@ -43,6 +43,8 @@ constexpr uint32_t E_RES = 0x80000000u, E_MARK = 0xC0000000u;
 struct TRay {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
     uint32_t dmask;
+    /* the box tests may take their min/max form (box_tests_exact) */
+    bool fast;
 };
 
 __device__ __forceinline__ TRay make_tray(V3 o, V3 d)
@@ -58,6 +60,33 @@ __device__ __forceinline__ TRay make_tray(V3 o, V3 d)
     r.iy = 1.0f / d.y;
     r.iz = 1.0f / d.z;
     r.dmask = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    r.fast = false;
+    return r;
+}
+
+/* When the box tests may use their min/max form.  slab() below is the
+ * reference's sequence of comparisons and updates.  With no NaN among its
+ * values and near <= far on every axis, that sequence accepts exactly when
+ * max(entry) <= min(exit), entry < 1 and exit > 0, and its final entry and
+ * exit are that max and min: the pairwise checks it makes, together with
+ * near <= far per axis, are all nine "entry <= exit" pairs.  Near <= far
+ * and no NaN hold when the origin is finite, every 1/d component is finite
+ * and non-zero (d finite, non-zero, not subnormal enough to overflow; its
+ * sign then matches d's, which picks near and far), and every box of the
+ * scene is finite with lo <= hi and |bounds| <= 2^120 (so that no midpoint
+ * overflows: lo <= (lo + hi) * 0.5 <= hi).  The scene's condition is
+ * checked once at upload (SceneDev::fast_boxes). */
+__device__ __forceinline__ bool box_tests_exact(const TRay &r)
+{
+    auto fin = [](float v) { return __builtin_isfinite(v); };
+    auto fnz = [](float v) { return __builtin_isfinite(v) && v != 0.0f; };
+    return fin(r.ox) && fin(r.oy) && fin(r.oz) && fnz(r.ix) && fnz(r.iy) && fnz(r.iz);
+}
+
+__device__ __forceinline__ TRay make_tray_s(V3 o, V3 d, bool fast_boxes)
+{
+    TRay r = make_tray(o, d);
+    r.fast = fast_boxes && box_tests_exact(r);
     return r;
 }
 
@@ -69,6 +98,13 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, const TRay &r, float 
     float tmax = ((xs ? lo.x : hi.x) - r.ox) * r.ix;
     float tymin = ((ys ? hi.y : lo.y) - r.oy) * r.iy;
     float tymax = ((ys ? lo.y : hi.y) - r.oy) * r.iy;
+    if (r.fast) {
+        const float tzmin = ((zs ? hi.z : lo.z) - r.oz) * r.iz;
+        const float tzmax = ((zs ? lo.z : hi.z) - r.oz) * r.iz;
+        const float a = fmaxf(fmaxf(tmin, tymin), tzmin), b = fminf(fminf(tmax, tymax), tzmax);
+        tmin_out = a;
+        return a <= b && a < 1.0f && b > 0.0f;
+    }
     if ((tmin > tymax) || (tymin > tmax)) return false;
     if (tymin > tmin) tmin = tymin;
     if (tymax < tmax) tmax = tymax;
@@ -102,6 +138,18 @@ __device__ __forceinline__ uint32_t expand_children(float4 lo, float4 hi, const 
     const float ny = ys ? y1 : y0, fy = ys ? y0 : y1;
     const float nz = zs ? z1 : z0, fz = zs ? z0 : z1;
     uint32_t out = 0;
+    if (r.fast) {
+        /* the same tests in min/max form (box_tests_exact) */
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float txmin = (k & 1) ? xc : nx, txmax = (k & 1) ? fx : xc;
+            const float tymin = (k & 2) ? yc : ny, tymax = (k & 2) ? fy : yc;
+            const float tzmin = (k & 4) ? zc : nz, tzmax = (k & 4) ? fz : zc;
+            const float a = fmaxf(fmaxf(txmin, tymin), tzmin), b = fminf(fminf(txmax, tymax), tzmax);
+            out |= (a <= b && a < 1.0f && b > 0.0f && a <= bound) ? (1u << k) : 0u;
+        }
+        return out & exist;
+    }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const float txmin = (k & 1) ? xc : nx, txmax = (k & 1) ? fx : xc;
@@ -381,7 +429,7 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
         V3 wo, wd;
         int time;
         src.get(wo, wd, time);
-        r = make_tray(wo, wd); /* world ray; the mesh's object ray inside a mesh */
+        r = make_tray_s(wo, wd, SPARSE && S.fast_boxes); /* world ray; the mesh's object ray inside a mesh */
         /* pseudo entry whose only child (slot 0) is the scene root */
         stk[0] = (S.scene_root << 8) | (1u << r.dmask);
         sp = 1;
@@ -415,7 +463,7 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
                         V3 o2, d2;
                         int t2;
                         src.get(o2, d2, t2);
-                        r = make_tray(o2, d2); /* back to the world ray (same floats) */
+                        r = make_tray_s(o2, d2, SPARSE && S.fast_boxes); /* back to the world ray (same floats) */
                     }
                     level = 0;
                     continue;
@@ -604,7 +652,7 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
                 int t2;
                 src.get(o2, d2, t2);
                 obj_inv_at(ob, t2, I);
-                r = make_tray(xf_point(I, o2), xf_vec(I, d2));
+                r = make_tray_s(xf_point(I, o2), xf_vec(I, d2), SPARSE && S.fast_boxes);
             }
             stk[(sp++) * stride] = (ob.root << 8) | (1u << r.dmask);
             item_cur = item_end = 0;

@@ -76,6 +76,25 @@ def test_sparse_levels_match_checker(ctx, sparse_from):
     ctx.release(sid)
 
 
+@pytest.mark.parametrize("config,variant,size", [(2, 10, (320, 240)), (3, 2000, (256, 256)), (0, 0, (320, 240))])
+def test_sparse_instances_same_image(ctx, config, variant, size):
+    """the sparse instances (four triangles in flight, box tests in min/max
+    form where exact) render the image and ray counts of the default
+    instances on the room (axis-aligned walls), a batch scene (moving
+    objects) and the spheres"""
+    hs = S.HostScene.generate(config, variant=variant)
+    sid = ctx.upload(hs)
+    p = S.render_params(*size, samples=(1, 2), max_var=0.05, iter=5, shadow_samples=2, seed=5)
+    base, bst = ctx.render(sid, p)
+    ctx.set_tuning(sparse_from=0, chain=0)
+    try:
+        img, st = ctx.render(sid, p)
+    finally:
+        ctx.set_tuning(sparse_from=64, chain=1)
+    assert np.array_equal(img, base) and st.rays() == bst.rays()
+    ctx.release(sid)
+
+
 @pytest.mark.parametrize("config,variant,size", [(1, 4, (480, 270)), (2, 10, (400, 300)), (0, 0, (400, 300))])
 def test_lanes_give_the_same_image(ctx, config, variant, size):
     """a pass spread over 2 or 4 streams renders the image and ray counts of
