@@ -429,7 +429,7 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
         V3 wo, wd;
         int time;
         src.get(wo, wd, time);
-        r = make_tray_s(wo, wd, SPARSE && S.fast_boxes); /* world ray; the mesh's object ray inside a mesh */
+        r = make_tray_s(wo, wd, S.fast_boxes); /* world ray; the mesh's object ray inside a mesh */
         /* pseudo entry whose only child (slot 0) is the scene root */
         stk[0] = (S.scene_root << 8) | (1u << r.dmask);
         sp = 1;
@@ -463,7 +463,7 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
                         V3 o2, d2;
                         int t2;
                         src.get(o2, d2, t2);
-                        r = make_tray_s(o2, d2, SPARSE && S.fast_boxes); /* back to the world ray (same floats) */
+                        r = make_tray_s(o2, d2, S.fast_boxes); /* back to the world ray (same floats) */
                     }
                     level = 0;
                     continue;
@@ -652,7 +652,7 @@ __device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src
                 int t2;
                 src.get(o2, d2, t2);
                 obj_inv_at(ob, t2, I);
-                r = make_tray_s(xf_point(I, o2), xf_vec(I, d2), SPARSE && S.fast_boxes);
+                r = make_tray_s(xf_point(I, o2), xf_vec(I, d2), S.fast_boxes);
             }
             stk[(sp++) * stride] = (ob.root << 8) | (1u << r.dmask);
             item_cur = item_end = 0;
