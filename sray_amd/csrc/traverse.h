@@ -131,22 +131,29 @@ __device__ __forceinline__ uint32_t expand_children(float4 lo, float4 hi, const 
 {
     const float cx = (lo.x + hi.x) * 0.5f, cy = (lo.y + hi.y) * 0.5f, cz = (lo.z + hi.z) * 0.5f;
     const bool xs = r.dx < 0.0f, ys = r.dy < 0.0f, zs = r.dz < 0.0f;
-    const float x0 = (lo.x - r.ox) * r.ix, xc = (cx - r.ox) * r.ix, x1 = (hi.x - r.ox) * r.ix;
-    const float y0 = (lo.y - r.oy) * r.iy, yc = (cy - r.oy) * r.iy, y1 = (hi.y - r.oy) * r.iy;
-    const float z0 = (lo.z - r.oz) * r.iz, zc = (cz - r.oz) * r.iz, z1 = (hi.z - r.oz) * r.iz;
-    const float nx = xs ? x1 : x0, fx = xs ? x0 : x1;
-    const float ny = ys ? y1 : y0, fy = ys ? y0 : y1;
-    const float nz = zs ? z1 : z0, fz = zs ? z0 : z1;
+    /* near / far bound picked before the arithmetic: the same values, and
+     * results of arithmetic, which the min/max form takes as they are */
+    const float nx = ((xs ? hi.x : lo.x) - r.ox) * r.ix, xc = (cx - r.ox) * r.ix, fx = ((xs ? lo.x : hi.x) - r.ox) * r.ix;
+    const float ny = ((ys ? hi.y : lo.y) - r.oy) * r.iy, yc = (cy - r.oy) * r.iy, fy = ((ys ? lo.y : hi.y) - r.oy) * r.iy;
+    const float nz = ((zs ? hi.z : lo.z) - r.oz) * r.iz, zc = (cz - r.oz) * r.iz, fz = ((zs ? lo.z : hi.z) - r.oz) * r.iz;
     uint32_t out = 0;
     if (r.fast) {
-        /* the same tests in min/max form (box_tests_exact) */
+        /* the same tests in min/max form (box_tests_exact), as one compare
+         * per child: with no NaN, "exit > 0" is "exit >= 2^-149" (the
+         * smallest positive float; denormals are kept), "entry < 1" is
+         * "entry <= 1 - 2^-24", so entry <= exit, entry < 1, exit > 0 and
+         * entry <= bound together are max(entry, 2^-149) <= min(exit, c)
+         * with c = min(bound, 1 - 2^-24); the extra terms go into the
+         * per-axis values shared by the children */
+        const float lo_t = 0x1p-149f, c = fminf(bound, 0x1.fffffep-1f);
+        const float nx2 = fmaxf(nx, lo_t), xc2 = fmaxf(xc, lo_t), fx2 = fminf(fx, c), xc3 = fminf(xc, c);
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const float txmin = (k & 1) ? xc : nx, txmax = (k & 1) ? fx : xc;
+            const float txmin = (k & 1) ? xc2 : nx2, txmax = (k & 1) ? fx2 : xc3;
             const float tymin = (k & 2) ? yc : ny, tymax = (k & 2) ? fy : yc;
             const float tzmin = (k & 4) ? zc : nz, tzmax = (k & 4) ? fz : zc;
             const float a = fmaxf(fmaxf(txmin, tymin), tzmin), b = fminf(fminf(txmax, tymax), tzmax);
-            out |= (a <= b && a < 1.0f && b > 0.0f && a <= bound) ? (1u << k) : 0u;
+            out |= (a <= b) ? (1u << k) : 0u;
         }
         return out & exist;
     }
