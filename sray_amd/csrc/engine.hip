@@ -135,7 +135,7 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
          * held in registers */
         const RayRecord<QRay> src{q, i};
         if (UNI) {
-            HitRec h = traverse_src<false, COUNT, true, SPARSE>(T.scenes[0], src, live, &stk[threadIdx.x], BS, cnt);
+            HitRec h = traverse_src<false, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[0]), src, live, &stk[threadIdx.x], BS, cnt);
             if (live) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
         } else {
             /* one walk per distinct scene of the wave (a wave's rays nearly
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
                 const uint32_t s0 = read_lane_u(sidx, __ffsll(todo) - 1);
                 const bool mine = live && sidx == s0;
                 todo &= ~__ballot(mine);
-                HitRec h = traverse_src<false, COUNT, true, SPARSE>(T.scenes[s0], src, mine, &stk[threadIdx.x], BS, cnt);
+                HitRec h = traverse_src<false, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[s0]), src, mine, &stk[threadIdx.x], BS, cnt);
                 if (mine) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
             }
         }
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
          * occupancy already, and re-reading it would only add loads */
         const RayRegs src{v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w)};
         if (UNI) {
-            HitRec h = traverse_src<true, COUNT, true, SPARSE>(T.scenes[0], src, live, &stk[threadIdx.x], BS, cnt);
+            HitRec h = traverse_src<true, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[0]), src, live, &stk[threadIdx.x], BS, cnt);
             if (i < n) occ[i] = h.obj >= 0 ? 1 : 0;
         } else {
             /* one walk per distinct scene of the wave, as in k_trace_q */
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
                 const uint32_t s0 = read_lane_u(sidx, __ffsll(todo) - 1);
                 const bool mine = live && sidx == s0;
                 todo &= ~__ballot(mine);
-                HitRec h = traverse_src<true, COUNT, true, SPARSE>(T.scenes[s0], src, mine, &stk[threadIdx.x], BS, cnt);
+                HitRec h = traverse_src<true, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[s0]), src, mine, &stk[threadIdx.x], BS, cnt);
                 if (mine) occ[i] = h.obj >= 0 ? 1 : 0;
             }
         }

@@ -395,6 +395,43 @@ template <bool ANY> constexpr int coop_lanes() { return ANY ? 16 : 8; }
  * at once, at the cost of registers the dense levels need for occupancy. */
 constexpr int SPARSE_TRIS = 4;
 
+/* The fields of a scene descriptor the walk reads, for a wave whose lanes
+ * all walk the same scene: read once into scalar registers
+ * (trav_scene_uniform).  Read through the descriptor in memory instead,
+ * the node arrays' addresses are loaded again at every node visit, a
+ * second memory round trip in front of the node's own load. */
+struct TravScene {
+    const float4 *nlo;
+    const float4 *nhi;
+    const uint32_t *items;
+    const DevObj *objs;
+    const float4 *ltri;
+    uint32_t scene_root;
+    int32_t fast_boxes;
+};
+
+template <class T> __device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const uint64_t v = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+/* S must be the same descriptor in every active lane */
+__device__ __forceinline__ TravScene trav_scene_uniform(const SceneDev &S)
+{
+    TravScene t;
+    t.nlo = uniform_ptr(S.nlo);
+    t.nhi = uniform_ptr(S.nhi);
+    t.items = uniform_ptr(S.items);
+    t.objs = uniform_ptr(S.objs);
+    t.ltri = uniform_ptr(S.ltri);
+    t.scene_root = __builtin_amdgcn_readfirstlane(S.scene_root);
+    t.fast_boxes = __builtin_amdgcn_readfirstlane(S.fast_boxes);
+    return t;
+}
+
 /* Scene::cast_ray over the octrees.  ANY: stop at the first hit (the
  * reference's shadow test cast_ray(ray) != NULL, src/shader.cc:266).
  * stk points at this lane's entry 0; consecutive entries are `stride`
@@ -412,8 +449,8 @@ constexpr int SPARSE_TRIS = 4;
  * waves can test leaves together.  A leaf tested by the wave yields the
  * same winner as tested by its own lane: the smallest (t, item position)
  * key, merged with the rule the serial loop applies. */
-template <bool ANY, bool COUNT, bool COOP, bool SPARSE, class SRC>
-__device__ __forceinline__ HitRec traverse_src(const SceneDev &S, const SRC &src, bool live, uint32_t *stk,
+template <bool ANY, bool COUNT, bool COOP, bool SPARSE, class SRC, class SC>
+__device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool live, uint32_t *stk,
                                                int stride, TraceCounts &cnt)
 {
     HitRec best;
