@@ -442,7 +442,9 @@ __device__ __forceinline__ TravScene trav_scene_uniform(const SceneDev &S)
  * (expand_children) and only the hit ones are pushed.  A popped child is
  * tested again from its fetched box (the same floats, so it passes) to
  * prune it against the best distance found since the push, and the pseudo
- * entries of the roots get their real test there.
+ * entries of the roots get their real test there.  In min/max form that
+ * test is left out for inner nodes: their children's tests, against the
+ * same bound, then all fail.
  *
  * COOP: every lane of the wave calls this (live = false for a lane without
  * a ray) and stays in the walk until all lanes are done, so that sparse
@@ -526,14 +528,22 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 uint32_t child = (e >> 8) + (k ^ r.dmask);
                 float4 lo = S.nlo[child], hi = S.nhi[child];
                 if (COUNT) cnt.nodes++;
+                uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
+                float bound = FLT_MAX;
+                if (!ANY) bound = level ? fminf(best.t, m_t) : best.t;
+                if (r.fast && !(meta & LEAF)) {
+                    /* an inner node in min/max form: every child's entry is
+                     * at least the node's and its exit at most the node's,
+                     * so when the node's own test fails (a miss, or an
+                     * entry beyond bound) every child test fails too, and
+                     * the node's test is left out */
+                    uint32_t hitm = expand_children(lo, hi, r, order_mask(meta, r.dmask), bound);
+                    if (hitm) stk[(sp++) * stride] = (first << 8) | hitm;
+                    continue;
+                }
                 float tmin;
                 if (!slab(lo, hi, r, tmin)) continue;
-                float bound = FLT_MAX;
-                if (!ANY) {
-                    bound = level ? fminf(best.t, m_t) : best.t;
-                    if (tmin > bound) continue;
-                }
-                uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
+                if (!ANY && tmin > bound) continue;
                 if (meta & LEAF) {
                     item_cur = first;
                     item_end = first + (meta & ~LEAF);
