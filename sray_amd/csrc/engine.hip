@@ -118,8 +118,10 @@ __device__ __forceinline__ uint32_t first_run() { return (blockIdx.x * blockDim.
 __device__ __forceinline__ uint32_t run_stride() { return gridDim.x * blockDim.x; }
 
 /* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
+/* The dense instances fit 7 waves per SIMD (72 VGPRs) without spilling
+ * when asked to; left to itself the compiler takes 73-76 VGPRs (6 waves). */
 template <bool COUNT, bool UNI, int BS, bool PERSIST, bool SPARSE>
-__global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(SPARSE || COUNT || PERSIST ? 1 : 7))) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
                                                 int4 *__restrict__ hits, unsigned long long *counts)
 {
     __shared__ uint32_t stk[STACK * BS];
