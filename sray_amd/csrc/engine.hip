@@ -1631,6 +1631,18 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
                                                dptr<uint8_t>(ln.kflag), greq, stats, srays,
                                                dptr<uint32_t>(E.errf));
     }
+    /* the children are compacted into the next level before the second
+     * stream starts: the compaction reads only what k_shade wrote, and
+     * launched beside the shadow trace its few blocks wait for free CUs
+     * while the next level's closest-hit trace waits for them */
+    if (b < E.maxiter) {
+        ProfScope ps(c, "k_compact", st);
+        uint32_t *bc = dptr<uint32_t>(ln.cmpb);
+        k_cmp_count<<<CMP_BLOCKS, CMP_THREADS, 0, st>>>(dptr<uint8_t>(ln.kflag), sn, E.K, bc);
+        k_cmp_scan<<<1, CMP_BLOCKS, 0, st>>>(bc, bc + CMP_BLOCKS, dptr<uint32_t>(ln.lvcnt) + b + 1);
+        k_cmp_scatter<<<CMP_BLOCKS, CMP_THREADS, 0, st>>>(dptr<QRay>(ln.kids), dptr<uint8_t>(ln.kflag), sn,
+                                                                 E.K, bc + CMP_BLOCKS, dptr<QRay>(ln.qlev[b + 1]));
+    }
     HIPCHK(c, hipEventRecord(ln.evA, st));
     HIPCHK(c, hipStreamWaitEvent(st2, ln.evA, 0));
     if (E.photons) {
@@ -1654,14 +1666,6 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
                                                     dptr<float4>(ln.acc));
     }
     HIPCHK(c, hipEventRecord(ln.evB[par], st2));
-    if (b < E.maxiter) {
-        ProfScope ps(c, "k_compact", st);
-        uint32_t *bc = dptr<uint32_t>(ln.cmpb);
-        k_cmp_count<<<CMP_BLOCKS, CMP_THREADS, 0, st>>>(dptr<uint8_t>(ln.kflag), sn, E.K, bc);
-        k_cmp_scan<<<1, CMP_BLOCKS, 0, st>>>(bc, bc + CMP_BLOCKS, dptr<uint32_t>(ln.lvcnt) + b + 1);
-        k_cmp_scatter<<<CMP_BLOCKS, CMP_THREADS, 0, st>>>(dptr<QRay>(ln.kids), dptr<uint8_t>(ln.kflag), sn,
-                                                                 E.K, bc + CMP_BLOCKS, dptr<QRay>(ln.qlev[b + 1]));
-    }
     HIPCHK(c, hipGetLastError());
     return SRAY_OK;
 }
