@@ -814,40 +814,29 @@ __global__ __launch_bounds__(CMP_THREADS) void k_cmp_count(const uint8_t *__rest
     }
 }
 
-/* pass 2 (one block of CMP_BLOCKS threads): exclusive offsets of the
- * blocks, and the total = the next level's count */
-__global__ __launch_bounds__(CMP_BLOCKS) void k_cmp_scan(const uint32_t *__restrict__ bcount,
-                                                         uint32_t *__restrict__ boff, uint32_t *__restrict__ total)
-{
-    __shared__ uint32_t wsum[CMP_BLOCKS / 64];
-    const uint32_t t = threadIdx.x, lane = __lane_id(), w = t / 64u;
-    const uint32_t v = bcount[t];
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = (uint32_t)__shfl_up((int)x, o);
-        if ((int)lane >= o) x += y;
-    }
-    if (lane == 63u) wsum[w] = x;
-    __syncthreads();
-    uint32_t pre = 0;
-    for (uint32_t k = 0; k < w; k++) pre += wsum[k];
-    const uint32_t ex = pre + x - v;
-    boff[t] = ex;
-    if (t == CMP_BLOCKS - 1u) *total = ex + v;
-}
-
-/* pass 3: each block writes its flagged items in order from its offset */
+/* pass 2: each block writes its flagged items in order from its offset,
+ * the sum of the counts of the blocks before it (at most CMP_BLOCKS - 1
+ * values, summed by the block itself: no separate scan launch); the last
+ * block writes the total, the next level's count */
 __global__ __launch_bounds__(CMP_THREADS) void k_cmp_scatter(const QRay *__restrict__ in,
                                                              const uint8_t *__restrict__ flags, StepN sn, int K,
-                                                             const uint32_t *__restrict__ boff,
-                                                             QRay *__restrict__ out)
+                                                             const uint32_t *__restrict__ bcount,
+                                                             uint32_t *__restrict__ total, QRay *__restrict__ out)
 {
     __shared__ uint32_t wsum[CMP_THREADS / 64];
+    const uint32_t lane = __lane_id(), w = threadIdx.x / 64u;
+    uint32_t pre = 0;
+    for (uint32_t k = threadIdx.x; k < blockIdx.x; k += CMP_THREADS) pre += bcount[k];
+    for (int off = 32; off > 0; off >>= 1) pre += (uint32_t)__shfl_xor((int)pre, off);
+    if (lane == 0u) wsum[w] = pre;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < CMP_THREADS / 64; k++) base += wsum[k];
+    __syncthreads();
+    if (blockIdx.x == gridDim.x - 1u && threadIdx.x == 0) *total = base + bcount[blockIdx.x];
     const uint32_t items = step_n(sn) * (uint32_t)K;
     uint32_t lo, hi;
     cmp_range(items, blockIdx.x, lo, hi);
-    const uint32_t lane = __lane_id(), w = threadIdx.x / 64u;
-    uint32_t base = boff[blockIdx.x];
     for (uint32_t t0 = lo; t0 < hi; t0 += CMP_THREADS) {
         const uint32_t j = t0 + threadIdx.x;
         const bool f = j < hi && flags[j] != 0;
@@ -1639,9 +1628,9 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
         ProfScope ps(c, "k_compact", st);
         uint32_t *bc = dptr<uint32_t>(ln.cmpb);
         k_cmp_count<<<CMP_BLOCKS, CMP_THREADS, 0, st>>>(dptr<uint8_t>(ln.kflag), sn, E.K, bc);
-        k_cmp_scan<<<1, CMP_BLOCKS, 0, st>>>(bc, bc + CMP_BLOCKS, dptr<uint32_t>(ln.lvcnt) + b + 1);
         k_cmp_scatter<<<CMP_BLOCKS, CMP_THREADS, 0, st>>>(dptr<QRay>(ln.kids), dptr<uint8_t>(ln.kflag), sn,
-                                                                 E.K, bc + CMP_BLOCKS, dptr<QRay>(ln.qlev[b + 1]));
+                                                                 E.K, bc, dptr<uint32_t>(ln.lvcnt) + b + 1,
+                                                                 dptr<QRay>(ln.qlev[b + 1]));
     }
     HIPCHK(c, hipEventRecord(ln.evA, st));
     HIPCHK(c, hipStreamWaitEvent(st2, ln.evA, 0));
