@@ -143,8 +143,9 @@ __device__ __forceinline__ uint32_t expand_children(float4 lo, float4 hi, const 
          * smallest positive float; denormals are kept), "entry < 1" is
          * "entry <= 1 - 2^-24", so entry <= exit, entry < 1, exit > 0 and
          * entry <= bound together are max(entry, 2^-149) <= min(exit, c)
-         * with c = min(bound, 1 - 2^-24); the extra terms go into the
-         * per-axis values shared by the children */
+         * with c = min(bound, 1 - 2^-24), given bound >= 2^-149 (it is
+         * FLT_MAX or a hit distance, and hits are at t >= 1e-6); the extra
+         * terms go into the per-axis values shared by the children */
         const float lo_t = 0x1p-149f, c = fminf(bound, 0x1.fffffep-1f);
         const float nx2 = fmaxf(nx, lo_t), xc2 = fmaxf(xc, lo_t), fx2 = fminf(fx, c), xc3 = fminf(xc, c);
 #pragma unroll
