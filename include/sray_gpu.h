@@ -154,13 +154,24 @@ typedef struct sray_hit {
     float    t;            /* parametric distance, FLT_MAX on miss */
     int32_t  obj;          /* object index, -1 on miss */
     int32_t  prim;         /* triangle index within the mesh, -1 otherwise */
-    uint32_t rank;         /* occurrence rank of the winner in the scene octree's
-                              depth-first leaf order (tie-break key) */
+    uint32_t rank;         /* position of the winning occurrence in the scene
+                              octree's depth-first item order, among the
+                              leaves the walk visited (leaves beyond the hit
+                              are skipped); when distinct items tie at t
+                              the walk visits every leaf the segment meets,
+                              and it is the reference's first occurrence */
 } sray_hit;
 
 #define SRAY_TRACE_CLOSEST   0u    /* Scene::cast_ray with a SurfPoint */
 #define SRAY_TRACE_ANY       1u    /* Scene::cast_ray(ray) == NULL test (shadow) */
 #define SRAY_TRACE_COUNT     2u    /* also fill visit counters (slower) */
+#define SRAY_TRACE_WAVE      4u    /* walk with the instance the render path
+                                      uses: a wave tests the leaves of its
+                                      few lanes still in mesh leaves together
+                                      (same results; for tests) */
+#define SRAY_TRACE_SPARSE    8u    /* with SRAY_TRACE_WAVE: the instance of the
+                                      deep bounce levels (four triangles in
+                                      flight per lane) */
 
 typedef struct sray_trace_counts {
     uint64_t nodes;        /* octree node records fetched (children are

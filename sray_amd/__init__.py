@@ -271,13 +271,17 @@ class Context:
         self._check(lib().sray_gpu_scene_bytes(self._h, sid, C.byref(b)))
         return b.value
 
-    def trace(self, sid, rays, any_hit=False, counts=False):
+    def trace(self, sid, rays, any_hit=False, counts=False, kernel="plain"):
+        """kernel: "plain" (one walk per lane), "wave" or "sparse" (the
+        render path's level-kernel instances)"""
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.zeros(len(rays), HIT_DTYPE)
         cnt = TraceCounts()
+        flags = abi.SRAY_TRACE_ANY if any_hit else 0
+        flags |= {"plain": 0, "wave": abi.SRAY_TRACE_WAVE,
+                  "sparse": abi.SRAY_TRACE_WAVE | abi.SRAY_TRACE_SPARSE}[kernel]
         self._check(lib().sray_gpu_trace(self._h, sid, rays.ctypes.data_as(C.POINTER(Ray)), len(rays),
-                                         hits.ctypes.data_as(C.POINTER(Hit)),
-                                         abi.SRAY_TRACE_ANY if any_hit else 0,
+                                         hits.ctypes.data_as(C.POINTER(Hit)), flags,
                                          C.byref(cnt) if counts else None))
         return (hits, cnt) if counts else hits
 

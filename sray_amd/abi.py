@@ -15,7 +15,7 @@ SRAY_ERR_NODEVICE = -8
 SRAY_OBJ_SPHERE, SRAY_OBJ_CYLINDER, SRAY_OBJ_MESH = 0, 1, 2
 SRAY_LIGHT_POINT, SRAY_LIGHT_SPHERE, SRAY_LIGHT_BOX = 0, 1, 2
 SRAY_MAT_HAS_SPECULAR, SRAY_MAT_HAS_REFLECT, SRAY_MAT_HAS_REFRACT, SRAY_MAT_PHONG = 1, 2, 4, 8
-SRAY_TRACE_CLOSEST, SRAY_TRACE_ANY, SRAY_TRACE_COUNT = 0, 1, 2
+SRAY_TRACE_CLOSEST, SRAY_TRACE_ANY, SRAY_TRACE_COUNT, SRAY_TRACE_WAVE, SRAY_TRACE_SPARSE = 0, 1, 2, 4, 8
 SRAY_RENDER_REF_Q1 = 1
 SRAY_RENDER_COUNT = 2
 SRAY_PMAP_CAUSTIC, SRAY_PMAP_GLOBAL = 0, 1

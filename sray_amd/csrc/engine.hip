@@ -118,10 +118,13 @@ __device__ __forceinline__ uint32_t first_run() { return (blockIdx.x * blockDim.
 __device__ __forceinline__ uint32_t run_stride() { return gridDim.x * blockDim.x; }
 
 /* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
-/* The dense instances fit 7 waves per SIMD (72 VGPRs) without spilling
- * when asked to; left to itself the compiler takes 73-76 VGPRs (6 waves). */
+/* The dense single-scene instance fits 7 waves per SIMD (72 VGPRs) without
+ * spilling when asked to; left to itself the compiler takes 73-76 VGPRs (6
+ * waves).  The multi-scene instance spills under the same hint (its walk
+ * loops over the wave's scenes), and gained nothing from it when it did
+ * not, so it takes no hint. */
 template <bool COUNT, bool UNI, int BS, bool PERSIST, bool SPARSE>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(SPARSE || COUNT || PERSIST ? 1 : 7))) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(SPARSE || COUNT || PERSIST || !UNI ? 1 : 7))) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
                                                 int4 *__restrict__ hits, unsigned long long *counts)
 {
     __shared__ uint32_t stk[STACK * BS];
@@ -1747,6 +1750,23 @@ int launch_chain(Level &LV, uint32_t m)
     return SRAY_OK;
 }
 
+/* destroys every captured descent.  A graph launch returns before its
+ * kernels run, and destroying an executable graph frees its kernel
+ * argument memory, which kernels still queued (and a profiler's dispatch
+ * records) read: so the streams the graphs were launched on are drained
+ * first. */
+void drop_graphs(Engine &E)
+{
+    if (E.graphs.empty()) return;
+    for (Engine::Lane &ln : E.lanes) {
+        if (ln.s) (void)hipStreamSynchronize(ln.s);
+        if (ln.s2) (void)hipStreamSynchronize(ln.s2);
+    }
+    (void)hipStreamSynchronize(E.c->stream);
+    for (auto &kv : E.graphs) (void)hipGraphExecDestroy(kv.second);
+    E.graphs.clear();
+}
+
 /* ray generation and the descent of the primary chunk [c0, c0 + m): as
  * kernel launches, or as one captured graph replayed (same kernels, same
  * order) */
@@ -1776,6 +1796,8 @@ int launch_chunk(Level &LV, uint32_t m, bool one_lane)
                                  (uint64_t)(E.photons | E.uni << 1 | E.one_scene << 2 | E.counting << 3 |
                                             E.persistent << 4),
                                  (uint64_t)E.Kb,
+                                 /* the first level traced by the sparse instances */
+                                 (uint64_t)std::min(E.sparse_from, E.maxiter + 1),
                                  (uint64_t)(uintptr_t)LV.T.scenes, (uint64_t)(uintptr_t)LV.T.reqs,
                                  (uint64_t)(uintptr_t)LV.T.preq, (uint64_t)(uintptr_t)LV.T.nsamp,
                                  (uint64_t)(uintptr_t)LV.active};
@@ -1787,10 +1809,7 @@ int launch_chunk(Level &LV, uint32_t m, bool one_lane)
     key.push_back((uint64_t)(uintptr_t)st);
     auto it = E.graphs.find(key);
     if (it == E.graphs.end()) {
-        if (E.graphs.size() >= 64) {
-            for (auto &kv : E.graphs) (void)hipGraphExecDestroy(kv.second);
-            E.graphs.clear();
-        }
+        if (E.graphs.size() >= 64) drop_graphs(E);
         HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = body();
         hipGraph_t g = nullptr;
@@ -2009,8 +2028,7 @@ Engine *engine_get(sray_gpu_ctx *c, Engine *&slot)
 void engine_free(Engine *E)
 {
     if (!E) return;
-    for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
-    E->graphs.clear();
+    drop_graphs(*E);
     for (size_t l = 0; l < E->lanes.size(); l++) {
         Engine::Lane &ln = E->lanes[l];
         if (l > 0 && ln.s) {

@@ -77,6 +77,40 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneDev S, const sr
     }
 }
 
+/* the same through the instance of the render path's level kernels: every
+ * lane of a wave walks until all are done, and waves with few lanes in
+ * mesh leaves test those leaves together (SPARSE: four triangles in flight
+ * per lane in its own leaf) */
+template <bool ANY, bool SPARSE>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays_wave(SceneDev S, const sray_ray *__restrict__ rays,
+                                                               uint32_t n, sray_hit *__restrict__ hits)
+{
+    __shared__ uint32_t stk[STACK * TRACE_BLOCK];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    RayRegs src;
+    if (live) {
+        const sray_ray r = rays[i];
+        src = RayRegs{v3(r.origin[0], r.origin[1], r.origin[2]), v3(r.dir[0], r.dir[1], r.dir[2]), r.time};
+    } else {
+        src = RayRegs{v3(0.0f, 0.0f, 0.0f), v3(1.0f, 1.0f, 1.0f), 0};
+    }
+    const HitRec h = traverse_src<ANY, false, true, SPARSE>(trav_scene_uniform(S), src, live, &stk[threadIdx.x],
+                                                            TRACE_BLOCK, cnt);
+    if (!live) return;
+    sray_hit o;
+    o.t = h.t;
+    o.obj = h.obj;
+    o.prim = (h.obj >= 0 && h.tri != 0xffffffffu) ? (int32_t)(h.tri - S.objs[h.obj].tri_base) : -1;
+    o.rank = h.obj >= 0 ? h.rank : 0u;
+    if (h.obj < 0) {
+        o.t = FLT_MAX;
+        o.rank = 0;
+    }
+    hits[i] = o;
+}
+
 /* ------------------------------------------------------------------ */
 /* photon-map kd-tree build (level by level) and gathers                */
 /* ------------------------------------------------------------------ */
@@ -1263,6 +1297,16 @@ int do_trace_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *d_rays, 
     if (n == 0) return SRAY_OK;
     bool any = flags & SRAY_TRACE_ANY, count = (flags & SRAY_TRACE_COUNT) && d_counts;
     uint32_t nb = blocks(n, TRACE_BLOCK);
+    if (flags & SRAY_TRACE_WAVE) {
+        if (count) return fail(c, SRAY_ERR_ARG, "SRAY_TRACE_WAVE takes no visit counters");
+        ProfScope ps(c, "k_trace_rays_wave");
+        const bool sp = flags & SRAY_TRACE_SPARSE;
+        auto kern = any ? (sp ? k_trace_rays_wave<true, true> : k_trace_rays_wave<true, false>)
+                        : (sp ? k_trace_rays_wave<false, true> : k_trace_rays_wave<false, false>);
+        kern<<<nb, TRACE_BLOCK, 0, c->stream>>>(g->dev, d_rays, n, d_hits);
+        HIPCHK(c, hipGetLastError());
+        return SRAY_OK;
+    }
     ProfScope ps(c, any ? "k_trace_rays_any" : "k_trace_rays");
     if (any && count) k_trace_rays<true, true><<<nb, TRACE_BLOCK, 0, c->stream>>>(g->dev, d_rays, n, d_hits, d_counts);
     else if (any) k_trace_rays<true, false><<<nb, TRACE_BLOCK, 0, c->stream>>>(g->dev, d_rays, n, d_hits, d_counts);

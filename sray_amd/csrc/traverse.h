@@ -8,11 +8,29 @@
  * The reference walks every node whose box meets the segment, children in
  * fixed order 0..7, and keeps the first occurrence of the smallest t (strict
  * '<').  Here children are visited near-to-far (octant bits XOR the ray's
- * sign mask), a node is skipped only when its entry distance is strictly
+ * sign mask), a node is skipped when its entry distance is strictly
  * greater than the best t found so far, and candidates are ranked by the
  * pair (t, occurrence rank) — the rank being the item's position in the
- * depth-first item array — so equal distances resolve to the same winner
- * as the reference's first-occurrence rule.
+ * depth-first item array.
+ *
+ * Ties.  Skipping far nodes is exact for the distance, and for the winner
+ * whenever one item alone holds the smallest t.  When two distinct items
+ * tie at it, the reference's winner is the one whose first occurrence
+ * comes first in depth-first order, and that occurrence may sit in a leaf
+ * the near-to-far walk skipped: a leaf beyond the hit that comes earlier
+ * in depth-first order and holds an item whose box reaches into it.  So
+ * the walk notes every exact tie between distinct items at the running
+ * best t (at either level, a mesh's internal ties included), and a ray that
+ * ends with one is walked again without skipping any node: then every
+ * occurrence the reference tests is tested, and the (t, rank) order picks
+ * the reference's winner.  Ties are rare, so the second walk costs little.
+ * To see a tie, the first walk must reach both tied items.  A tied item
+ * can sit in a node whose box the ray enters exactly where it hits the
+ * item (a flat mesh's box, a triangle whose edge lies on a split plane);
+ * the box's entry distance and the item's hit distance are then equal in
+ * exact arithmetic but rounded differently.  So a node is skipped only
+ * when its entry exceeds the best t by a relative PRUNE_SLACK, far more
+ * than the few roundings between the two.
  *
  * Per-lane stack in LDS, 32-bit entries:
  *   ITER   first child block << 8 | pending children whose boxes the
@@ -39,6 +57,9 @@
 namespace srd {
 
 constexpr uint32_t E_RES = 0x80000000u, E_MARK = 0xC0000000u;
+
+/* a node is skipped when its entry distance exceeds best t * PRUNE_SLACK */
+constexpr float PRUNE_SLACK = 1.0f + 0x1p-14f;
 
 struct TRay {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
@@ -144,8 +165,9 @@ __device__ __forceinline__ uint32_t expand_children(float4 lo, float4 hi, const 
          * "entry <= 1 - 2^-24", so entry <= exit, entry < 1, exit > 0 and
          * entry <= bound together are max(entry, 2^-149) <= min(exit, c)
          * with c = min(bound, 1 - 2^-24), given bound >= 2^-149 (it is
-         * FLT_MAX or a hit distance, and hits are at t >= 1e-6); the extra
-         * terms go into the per-axis values shared by the children */
+         * +inf, FLT_MAX or a hit distance times PRUNE_SLACK, and hits are
+         * at t >= 1e-6); the extra terms go into the per-axis values
+         * shared by the children */
         const float lo_t = 0x1p-149f, c = fminf(bound, 0x1.fffffep-1f);
         const float nx2 = fmaxf(nx, lo_t), xc2 = fmaxf(xc, lo_t), fx2 = fminf(fx, c), xc3 = fminf(xc, c);
 #pragma unroll
@@ -472,6 +494,33 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
     int cur_mesh = -1;
     int sp = 0;
     bool done = !live;
+    /* TF_TIE: distinct items tie at best.t; TF_MTIE: the same within the
+     * mesh being walked; TF_EXH: the second, exhaustive walk of a tie ray */
+    uint32_t tf = 0;
+    constexpr uint32_t TF_TIE = 1u, TF_MTIE = 2u, TF_EXH = 4u;
+    /* a triangle of the mesh being walked at item position k (tie_in: it
+     * ties with another triangle of its leaf) */
+    auto mesh_offer = [&](float t, uint32_t k, bool tie_in) {
+        const bool eq = t == m_t;
+        if (eq && !(tf & TF_MTIE) && (tie_in || S.items[k] != S.items[m_rank])) tf |= TF_MTIE;
+        if (t < m_t || (eq && k < m_rank)) {
+            if (!eq) tf = (tf & ~TF_MTIE) | (tie_in ? TF_MTIE : 0u);
+            m_t = t;
+            m_rank = k;
+        }
+    };
+    /* a scene item at occurrence rank: object oi (triangle tri of a mesh) */
+    auto scene_offer = [&](float t, int oi, uint32_t tri, uint32_t rank, bool tie_in) {
+        const bool eq = t == best.t;
+        if (eq && (tie_in || oi != best.obj || tri != best.tri)) tf |= TF_TIE;
+        if (t < best.t || (eq && rank < best.rank)) {
+            if (!eq) tf = (tf & ~TF_TIE) | (tie_in ? TF_TIE : 0u);
+            best.t = t;
+            best.obj = oi;
+            best.tri = tri;
+            best.rank = rank;
+        }
+    };
     if (live) {
         V3 wo, wd;
         int time;
@@ -484,7 +533,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
         r = make_tray(v3(0.0f, 0.0f, 0.0f), v3(1.0f, 1.0f, 1.0f));
     }
     for (;;) {
-        if (!done) {
+        if (!done) for (;;) {
             /* while-while: walk the stack until this lane has items to test,
              * so lanes meet at the item tests instead of alternating node
              * visits and leaf loops */
@@ -499,12 +548,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                         uint32_t m_tri = m_rank != 0xffffffffu ? S.items[m_rank] : 0xffffffffu;
                         cache_t = m_t;
                         cache_tri = m_tri;
-                        if (m_tri != 0xffffffffu && (m_t < best.t || (m_t == best.t && rank < best.rank))) {
-                            best.t = m_t;
-                            best.obj = oi;
-                            best.tri = m_tri;
-                            best.rank = rank;
-                        }
+                        if (m_tri != 0xffffffffu) scene_offer(m_t, oi, m_tri, rank, (tf & TF_MTIE) != 0u);
                     }
                     {
                         V3 o2, d2;
@@ -531,7 +575,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 if (COUNT) cnt.nodes++;
                 uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
                 float bound = FLT_MAX;
-                if (!ANY) bound = level ? fminf(best.t, m_t) : best.t;
+                if (!ANY && !(tf & TF_EXH)) bound = (level ? fminf(best.t, m_t) : best.t) * PRUNE_SLACK;
                 if (r.fast && !(meta & LEAF)) {
                     /* an inner node in min/max form: every child's entry is
                      * at least the node's and its exit at most the node's,
@@ -553,7 +597,31 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                     if (hitm) stk[(sp++) * stride] = (first << 8) | hitm;
                 }
             }
-            if (item_cur >= item_end) done = true; /* stack empty */
+            if (item_cur < item_end) break;
+            /* stack empty: a tie ray is walked again, skipping nothing */
+            if (!ANY && (tf & (TF_TIE | TF_EXH)) == TF_TIE) {
+                tf = TF_EXH;
+                best.t = FLT_MAX;
+                best.obj = -1;
+                best.tri = 0xffffffffu;
+                best.rank = 0xffffffffu;
+                m_t = FLT_MAX;
+                m_rank = 0xffffffffu;
+                cache_obj = -1;
+                cache_t = FLT_MAX;
+                cache_tri = 0xffffffffu;
+                cur_mesh = -1;
+                level = 0;
+                V3 wo, wd;
+                int time;
+                src.get(wo, wd, time);
+                r = make_tray_s(wo, wd, S.fast_boxes);
+                stk[0] = (S.scene_root << 8) | (1u << r.dmask);
+                sp = 1;
+                continue;
+            }
+            done = true;
+            break;
         }
         if (COOP) {
             if (__ballot(!done) == 0ull) break;
@@ -580,6 +648,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                     const float bnd = ANY ? FLT_MAX : read_lane_f(fminf(m_t, best.t), j);
                     const float4 *LT = read_lane_p(S.ltri, j);
                     unsigned long long key = ~0ull;
+                    bool ltie = false; /* two of this lane's triangles tie at its best */
                     for (uint32_t base = cur; base < end; base += 64u) {
                         const uint32_t k = base + lane;
                         if (k < end) {
@@ -592,11 +661,19 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                             float t;
                             if (tri_t(A, B, C, rj, t, bnd)) {
                                 const unsigned long long kk = ((unsigned long long)__float_as_uint(t) << 32) | k;
+                                if (!ANY && (uint32_t)(key >> 32) == __float_as_uint(t)) ltie = true;
                                 key = kk < key ? kk : key;
                             }
                         }
                     }
+                    const unsigned long long mine = key;
                     key = wave_min_u64(key);
+                    /* positions within one leaf are distinct triangles: any
+                     * other lane at the leaf's best t is a tie */
+                    bool leaf_tie = false;
+                    if (!ANY)
+                        leaf_tie = __ballot(mine != ~0ull && (uint32_t)(mine >> 32) == (uint32_t)(key >> 32) &&
+                                            (mine != key || ltie)) != 0ull;
                     if (lane == (uint32_t)j) {
                         if (key != ~0ull) {
                             const float t = __uint_as_float((uint32_t)(key >> 32));
@@ -606,9 +683,8 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                                 best.tri = S.items[kpos];
                                 best.obj = cur_mesh;
                                 done = true;
-                            } else if (t < m_t || (t == m_t && kpos < m_rank)) {
-                                m_t = t;
-                                m_rank = kpos;
+                            } else {
+                                mesh_offer(t, kpos, leaf_tie);
                             }
                         }
                         item_cur = item_end;
@@ -641,9 +717,8 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                                 best.tri = S.items[k];
                                 best.obj = cur_mesh;
                                 done = true;
-                            } else if (t < m_t || (t == m_t && k < m_rank)) {
-                                m_t = t;
-                                m_rank = k;
+                            } else {
+                                mesh_offer(t, k, false);
                             }
                         }
                     }
@@ -666,10 +741,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                         done = true;
                         break;
                     }
-                    if (t < m_t || (t == m_t && item_cur < m_rank)) {
-                        m_t = t;
-                        m_rank = item_cur;
-                    }
+                    mesh_offer(t, item_cur, false);
                 }
                 item_cur++;
             }
@@ -686,13 +758,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 /* this ray has traversed this mesh already (in another scene
                  * leaf): same result; an any-hit ray only gets here after
                  * finding nothing in it */
-                if (!ANY && cache_tri != 0xffffffffu &&
-                    (cache_t < best.t || (cache_t == best.t && rank < best.rank))) {
-                    best.t = cache_t;
-                    best.obj = oi;
-                    best.tri = cache_tri;
-                    best.rank = rank;
-                }
+                if (!ANY && cache_tri != 0xffffffffu) scene_offer(cache_t, oi, cache_tri, rank, false);
                 continue;
             }
             if (COUNT) cnt.mesh_enters++;
@@ -715,6 +781,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
             cur_mesh = oi;
             m_t = FLT_MAX;
             m_rank = 0xffffffffu;
+            tf &= ~TF_MTIE;
             continue;
         }
         float I[12];
@@ -743,12 +810,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 done = true;
                 continue;
             }
-            if (t < best.t || (t == best.t && rank < best.rank)) {
-                best.t = t;
-                best.obj = oi;
-                best.tri = 0xffffffffu;
-                best.rank = rank;
-            }
+            scene_offer(t, oi, 0xffffffffu, rank, false);
         }
     }
     return best;

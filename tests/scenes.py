@@ -150,3 +150,104 @@ def ray(o, d):
 
 def rays(os_, ds):
     return S.make_rays(np.asarray(os_, np.float32), np.asarray(ds, np.float32))
+
+
+# ---------------------------------------------------------------- exact ties
+# Scenes where distinct items tie at exactly the same distance, built so
+# that the reference's first-depth-first-occurrence rule (strict '<' over
+# children 0..7 and leaf lists, src/octree.inl:211-234) picks an item whose
+# first occurrence lies in a leaf *beyond* the hit.  Every coordinate is a
+# small dyadic number, so hit distances and hit points are exact in float:
+# the tied items sit in the plane y = 0, their plane normals are (0, ny, 0),
+# and every ray is o = p - 0.5 d with p on the tied items, so t = 0.5.
+#
+# The octree has one split (depth 1, one item per node): extender items at
+# the corners make the root box [-8, 8]^3, so the children are the octants
+# around the origin (numbered as src/octree.inl:161-169: bit 0 = high x,
+# bit 1 = high y, bit 2 = high z, except that children 2/3 and 6/7 run x
+# high-then-low: 2 = (+x,+y,-z), 3 = (-x,+y,-z), 6 = (+x,+y,+z),
+# 7 = (-x,+y,+z)).
+#
+# Pair k: item B (lower index) spans x in [0.5, 1.5], item A spans x in
+# [-1.5, 0.5]; they share the edge x = 0.5 (mesh level) or are the same
+# triangle (scene level, A's box widened by an extra triangle).  A ray
+# heading to -x crosses y = 0 at p (x >= 0.5) and then enters the
+# low-x low-y octant (child 4 for z > 0, child 0 for z < 0) at t = 0.625:
+# that octant holds A only and comes first in depth-first order, so the
+# reference returns A; a walk that skips leaves beyond the best hit sees
+# B first in every leaf it visits and returns B.  Rays heading to +x
+# meet B first in depth-first order (child 5 / child 1) and return B.
+TIE_Z = [(0.5, 1.25), (2.0, 2.75), (3.5, 4.25), (5.0, 5.75),
+         (-1.25, -0.5), (-2.75, -2.0), (-4.25, -3.5), (-5.75, -5.0)]
+
+
+def _extenders():
+    return [[(-8, -8, -8), (-7.5, -8, -8), (-8, -7.5, -8)], [(8, 8, 8), (7.5, 8, 8), (8, 7.5, 8)]]
+
+
+def tie_mesh_scene():
+    """one mesh: two corner extenders, then for each pair B_k, A_k (B first).
+    Returns (built scene, list of (index of B, index of A, z0, z1))."""
+    tris = _extenders()
+    pairs = []
+    for z0, z1 in TIE_Z:
+        b = [(0.5, 0, z0), (1.5, 0, z0), (0.5, 0, z1)]
+        a = [(0.5, 0, z0), (0.5, 0, z1), (-1.5, 0, z0)]
+        pairs.append((len(tris), len(tris) + 1, z0, z1))
+        tris += [b, a]
+    return build(meshes=[(np.asarray(tris, np.float32), (0, 0, 0), 0)], mesh_oct=(1, 1)), pairs
+
+
+def tie_object_scene():
+    """objects: a pair of coincident spheres (same centre and radius, objects
+    0 and 1) away from the pairs, a corner-extender mesh, then per pair mesh
+    B_k = [T_k] and mesh A_k = [T_k, X_k] (X_k widens A's box to x = -1.5).
+    Returns (built scene, list of (object of B, object of A, z0, z1))."""
+    meshes = [(np.asarray(_extenders(), np.float32), (0, 0, 0), 0)]
+    pairs = []
+    for z0, z1 in TIE_Z:
+        t = [(0.5, 0, z0), (1.5, 0, z0), (0.5, 0, z1)]
+        x = [(-1.5, 0, z0), (-1.25, 0, z0), (-1.5, 0, z0 + 0.25)]
+        pairs.append((len(meshes), len(meshes) + 1, z0, z1))
+        meshes += [(np.asarray([t], np.float32), (0, 0, 0), 0), (np.asarray([t, x], np.float32), (0, 0, 0), 0)]
+    spheres = [((-4.0, 4.0, 0.0), 1.0, 0), ((-4.0, 4.0, 0.0), 1.0, 0)]
+    # build() numbers spheres first, then meshes
+    pairs = [(ib + len(spheres), ia + len(spheres), z0, z1) for ib, ia, z0, z1 in pairs]
+    return build(spheres=spheres, meshes=meshes, scene_oct=(1, 1), mesh_oct=(1, 14)), pairs
+
+
+def tie_rays(pairs, per_pair=32, seed=0, edge=True):
+    """rays through points p of each pair's tied region at t = 0.5:
+    p on the shared edge x = 0.5 (edge=True) or inside the triangle at
+    x = 0.75; half head to -x (the reference picks A), half to +x (B).
+    Returns (rays, expected item index, mask of -x rays)."""
+    rng = np.random.default_rng(seed)
+    o, d, want, neg = [], [], [], []
+    px = 0.5 if edge else 0.75
+    for ib, ia, z0, z1 in pairs:
+        lo, hi = (z0 + 0.125, z1 - 0.25) if not edge else (z0, z1)
+        for j in range(per_pair):
+            # dyadic z in [lo, hi] with 1/64 steps
+            pz = lo + np.floor(rng.uniform(0, 1) * (hi - lo) * 64) / 64
+            for sx in (-1.0, 1.0):
+                dz = 0.25 if rng.uniform() < 0.5 else -0.25
+                dd = np.array([4.0 * sx, -2.0, dz])
+                p = np.array([px, 0.0, pz])
+                o.append(p - 0.5 * dd)
+                d.append(dd)
+                want.append(ia if sx < 0 else ib)
+                neg.append(sx < 0)
+    return rays(o, d), np.asarray(want), np.asarray(neg)
+
+
+def sparse_lanes(r, every=16):
+    """spreads rays one per `every` lanes, the rest missing everything (so
+    that a wave has few lanes in mesh leaves); returns (rays, index mask)"""
+    n = len(r)
+    out = np.zeros(n * every, r.dtype)
+    out["origin"] = np.array([0.0, 50.0, 0.0], np.float32)
+    out["dir"] = np.array([0.0, 1.0, 0.0], np.float32)
+    out[::every] = r
+    m = np.zeros(n * every, bool)
+    m[::every] = True
+    return out, m

@@ -1,10 +1,10 @@
 """Device traversal parity: sray_gpu_trace (C ABI) against the CPU checker.
 
 Bar: hit object / triangle IDs bit-exact and t bit-exact against the strict
-float restatement (same operation order), on every ray without an exact-t
-tie between distinct items (the checker flags those; on them t must still
-match exactly).  Agreement with the double restatement (the reference's own
-precision) is reported and must stay above 99.9%.
+float restatement (same operation order), on every ray, including rays
+where distinct items tie at exactly the same distance (the checker flags
+those; their count is reported).  Agreement with the double restatement
+(the reference's own precision) is reported and must stay above 99.9%.
 """
 import numpy as np
 import pytest
@@ -17,9 +17,8 @@ pytestmark = pytest.mark.gpu
 
 def _compare(hits, ohits, ext, label):
     tie = ext["tie"] != 0
-    nt = ~tie
-    bad_obj = np.flatnonzero(nt & (hits["obj"] != ohits["obj"]))
-    bad_prim = np.flatnonzero(nt & (hits["prim"] != ohits["prim"]))
+    bad_obj = np.flatnonzero(hits["obj"] != ohits["obj"])
+    bad_prim = np.flatnonzero(hits["prim"] != ohits["prim"])
     hit = ohits["obj"] >= 0
     bad_t = np.flatnonzero(hit & (hits["t"].view(np.uint32) != ohits["t"].view(np.uint32)))
     msg = (f"{label}: {len(hits)} rays, {hit.sum()} hits, {tie.sum()} ties; "
