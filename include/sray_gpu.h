@@ -348,6 +348,22 @@ int  sray_gpu_pmap_release(sray_gpu_ctx *ctx, uint32_t pmap_id);
 int  sray_gpu_pmap_gather(sray_gpu_ctx *ctx, uint32_t pmap_id, const sray_gather_query *q,
                           uint32_t n, float radius, int k, int mode, float *rgb);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the fixed-radius gather of
+ * sray_gpu_pmap_gather (k == 0, kd-tree; mode SRAY_GATHER_IRRADIANCE or
+ * SRAY_GATHER_RADIANCE, no SRAY_GATHER_GRID) on queries d_q and results
+ * d_rgb (n * 3 floats) already in device memory.  It allocates nothing:
+ * its per-query counters live in a fixed device array of the library,
+ * one per GPU, sized for SRAY_NOALLOC_MAX_QUERIES queries; larger n is
+ * rejected with SRAY_ERR_LIMIT.  Asynchronous on the context's stream
+ * (sray_gpu_synchronize waits).  Calls on one GPU share that array, so
+ * calls that overlap on one GPU (other contexts, other threads) race. */
+#define SRAY_NOALLOC_MAX_QUERIES (1u << 20)
+int  sray_gpu_pmap_gather_device_noalloc(sray_gpu_ctx *ctx, uint32_t pmap_id,
+                                         const sray_gather_query *d_q, uint32_t n, float radius,
+                                         int mode, float *d_rgb);
+
 /* Replaces shoot_photons + Scene::build_caustics_map/build_global_map +
  * Light::gen_photon/build_projmap (src/render.cc:67-122, src/scene.cc:445-760,
  * src/light.cc:111-205, src/projmap.cc): projection maps on the host,

@@ -17,6 +17,7 @@ SRAY_LIGHT_POINT, SRAY_LIGHT_SPHERE, SRAY_LIGHT_BOX = 0, 1, 2
 SRAY_MAT_HAS_SPECULAR, SRAY_MAT_HAS_REFLECT, SRAY_MAT_HAS_REFRACT, SRAY_MAT_PHONG = 1, 2, 4, 8
 SRAY_TRACE_CLOSEST, SRAY_TRACE_ANY, SRAY_TRACE_COUNT, SRAY_TRACE_WAVE, SRAY_TRACE_SPARSE = 0, 1, 2, 4, 8
 SRAY_RENDER_REF_Q1 = 1
+SRAY_NOALLOC_MAX_QUERIES = 1 << 20
 SRAY_RENDER_COUNT = 2
 SRAY_PMAP_CAUSTIC, SRAY_PMAP_GLOBAL = 0, 1
 SRAY_GATHER_IRRADIANCE, SRAY_GATHER_RADIANCE = 0, 1
@@ -156,7 +157,7 @@ EXPORTS = [
     "sray_gpu_set_tuning", "sray_gpu_get_tuning", "sray_gpu_autotune", "sray_gpu_autotune_octree",
     "sray_photon_dump", "sray_photon_restore", "sray_photon_free", "sray_image_write_pfm", "sray_image_write_png",
     "sray_status_record",
-    "sray_io_last_error", "sray_gpu_set_status_fd",
+    "sray_io_last_error", "sray_gpu_set_status_fd", "sray_gpu_pmap_gather_device_noalloc",
 ]
 SRAY_STATUS_RECORD = 32
 
@@ -215,6 +216,7 @@ def declare(lib):
         "sray_status_record": (i32, [C.c_char, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p]),
         "sray_io_last_error": (C.c_char_p, []),
         "sray_gpu_set_status_fd": (i32, [vp, C.c_int]),
+        "sray_gpu_pmap_gather_device_noalloc": (i32, [vp, u32, vp, u32, f32, C.c_int, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -1319,6 +1319,44 @@ int do_trace_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *d_rays, 
 } // namespace
 
 /* ================================================================== */
+/* the gather that allocates nothing                                    */
+/* ================================================================== */
+/* Per-query counters (4 B count + 1 B selection flag) of
+ * sray_gpu_pmap_gather_device_noalloc.  Each GPU holds its own copy, at
+ * its own address, until the process ends.  In host code `scratch` names
+ * a host-side stand-in, not the device array: its device address comes
+ * from hipGetSymbolAddress, once per device, with that device current. */
+static __device__ __align__(256) uint8_t scratch[(size_t)SRAY_NOALLOC_MAX_QUERIES * 5];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+/* the device address of `scratch` on the current device (dev), looked up
+ * once; hipGetSymbolAddress allocates nothing */
+int scratch_init(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return SRAY_ERR_LIMIT;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return SRAY_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SRAY_ERR_HIP;
+    g_scratch_ptr[dev] = (uint8_t *)p;
+    return SRAY_OK;
+}
+
+uint8_t *scratch_ptr(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
+
+} // namespace
+
+/* ================================================================== */
 /* C ABI                                                                */
 /* ================================================================== */
 extern "C" {
@@ -1334,7 +1372,11 @@ int sray_gpu_create(int device, uint32_t flags, sray_gpu_ctx **out)
     auto *c = new sray_gpu_ctx;
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        scratch_init(device) != SRAY_OK) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
         delete c;
         return SRAY_ERR_HIP;
     }
@@ -1535,6 +1577,32 @@ int sray_gpu_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_qu
     ApiLock lk(c);
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
     return do_pmap_gather(c, pmap_id, q, n, radius, k, mode, rgb);
+}
+
+int sray_gpu_pmap_gather_device_noalloc(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *d_q, uint32_t n,
+                                        float radius, int mode, float *d_rgb)
+{
+    if (!c || (n && (!d_q || !d_rgb))) return SRAY_ERR_ARG;
+    ApiLock lk(c);
+    if (n > SRAY_NOALLOC_MAX_QUERIES)
+        return fail(c, SRAY_ERR_LIMIT, "%u queries: the gather that allocates nothing takes at most %u", n,
+                    SRAY_NOALLOC_MAX_QUERIES);
+    if (mode != SRAY_GATHER_IRRADIANCE && mode != SRAY_GATHER_RADIANCE)
+        return fail(c, SRAY_ERR_ARG, "bad gather mode (irradiance or radiance, kd-tree only)");
+    if (!(radius > 0.0f)) return fail(c, SRAY_ERR_ARG, "radius must be > 0");
+    auto it = c->pmaps.find(pmap_id);
+    if (it == c->pmaps.end()) return fail(c, SRAY_ERR_NOTFOUND, "unknown photon map %u", pmap_id);
+    if (n == 0) return SRAY_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, SRAY_ERR_HIP, "hipSetDevice failed");
+    uint8_t *scr = scratch_ptr(c->device);
+    if (!scr) return fail(c, SRAY_ERR_HIP, "no scratch address for device %d", c->device);
+    /* counts in the first 4 * max bytes, selection flags after them; k = 0
+     * writes both for every query, so nothing needs clearing first */
+    uint32_t *cnt = (uint32_t *)scr;
+    uint8_t *need = scr + (size_t)SRAY_NOALLOC_MAX_QUERIES * 4;
+    k_pm_gather<<<blocks(n, 128), 128, 0, c->stream>>>(it->second->tree, d_q, n, radius, mode, 0, d_rgb, cnt, need);
+    HIPCHK(c, hipGetLastError());
+    return SRAY_OK;
 }
 
 int sray_gpu_shoot_photons(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon_params *pp, uint32_t *caustic,
