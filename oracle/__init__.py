@@ -75,6 +75,7 @@ def lib(fast=False):
 
 Q9_RETRAVERSE = 0x1
 BRUTE_FORCE = 0x2
+BRUTE_GATHER = 0x4
 
 
 class OracleScene:

@@ -419,6 +419,21 @@ struct Counters {
     uint64_t nodes = 0, prims = 0;
 };
 
+/* Photons bucketed in a uniform grid of cells of side h >= the gather
+ * radius, in index order within a cell.  A photon that passes the range
+ * test d^2 <= r^2 is within r (to a rounding) of the query on every axis,
+ * so with h = 1.01 r its cell is the query's or a neighbour: the 27 cells
+ * around the query's hold every photon the brute-force loop would accept.
+ * The candidates then take the brute-force test and order, so results are
+ * identical to testing every photon (kdtree's kd_nearest_range3f is a
+ * range search, src/pmap.cc:114,149: only the found set matters). */
+struct PhotonGrid {
+    double rf = -1.0; /* float-rounded radius it was built for (-1: none) */
+    double o[3] = {0.0, 0.0, 0.0}, h = 1.0;
+    int64_t n[3] = {0, 0, 0};
+    std::vector<uint32_t> start, idx;
+};
+
 template <class T> struct Scene {
     std::vector<Object<T>> objs;
     std::vector<MeshT<T>> meshes;
@@ -436,7 +451,59 @@ template <class T> struct Scene {
     double gather_dist_abs = 0;
     double diag = 0;
     std::vector<Photon> pm[2];
+    PhotonGrid grid[2];
+    bool brute_gather = false;
     static constexpr T eps = (T)SRAY_ERROR_MARGIN;
+
+    /* (re)builds the bucket grid of map kind for a gather radius (not
+     * thread-safe: called before a render's or a gather's workers start) */
+    void build_grid(int kind, double radius)
+    {
+        const double rf = (double)(float)radius;
+        PhotonGrid &g = grid[kind];
+        if (brute_gather || g.rf == rf || !(rf > 0.0)) return;
+        g = PhotonGrid();
+        const std::vector<Photon> &ph = pm[kind];
+        double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        for (const Photon &p : ph)
+            for (int d = 0; d < 3; d++) {
+                mn[d] = std::min(mn[d], (double)p.pos[d]);
+                mx[d] = std::max(mx[d], (double)p.pos[d]);
+            }
+        if (ph.empty())
+            for (int d = 0; d < 3; d++) mn[d] = mx[d] = 0.0;
+        for (int d = 0; d < 3; d++)
+            if (!std::isfinite(mn[d]) || !std::isfinite(mx[d])) return; /* brute force for this map */
+        double h = rf * 1.01;
+        for (;;) {
+            double cells = 1.0;
+            for (int d = 0; d < 3; d++) cells *= std::floor((mx[d] - mn[d]) / h) + 1.0;
+            if (cells <= (double)(1u << 24)) break;
+            h *= 2.0;
+        }
+        for (int d = 0; d < 3; d++) {
+            g.o[d] = mn[d];
+            g.n[d] = (int64_t)std::floor((mx[d] - mn[d]) / h) + 1;
+        }
+        g.h = h;
+        const size_t nc = (size_t)(g.n[0] * g.n[1] * g.n[2]);
+        std::vector<uint32_t> cell(ph.size());
+        g.start.assign(nc + 1, 0);
+        for (size_t i = 0; i < ph.size(); i++) {
+            int64_t c[3];
+            for (int d = 0; d < 3; d++)
+                c[d] = std::min<int64_t>(std::max<int64_t>((int64_t)std::floor(((double)ph[i].pos[d] - g.o[d]) / h), 0),
+                                         g.n[d] - 1);
+            cell[i] = (uint32_t)((c[2] * g.n[1] + c[1]) * g.n[0] + c[0]);
+            g.start[cell[i] + 1]++;
+        }
+        for (size_t c = 0; c < nc; c++) g.start[c + 1] += g.start[c];
+        g.idx.resize(ph.size());
+        std::vector<uint32_t> fill(g.start.begin(), g.start.end() - 1);
+        for (size_t i = 0; i < ph.size(); i++) g.idx[fill[cell[i]]++] = (uint32_t)i;
+        g.rf = rf;
+    }
+    void photons_changed(int kind) { grid[kind] = PhotonGrid(); }
 
     /* object matrices at a time (linear translation keys, src/anim.inl:188-261) */
     void xform_at(const Object<T> &o, int time, Mat<T> &M, Mat<T> &I) const
@@ -886,12 +953,33 @@ template <class T> struct Scene {
         double rf = (double)(float)radius;
         double r2 = rf * rf;
         std::vector<std::pair<double, uint32_t>> cand;
-        for (uint32_t i = 0; i < ph.size(); i++) {
+        auto test = [&](uint32_t i) {
             double dx = (double)ph[i].pos[0] - (double)(float)pos.x;
             double dy = (double)ph[i].pos[1] - (double)(float)pos.y;
             double dz = (double)ph[i].pos[2] - (double)(float)pos.z;
             double d2 = dx * dx + dy * dy + dz * dz;
             if (d2 <= r2) cand.push_back({d2, i});
+        };
+        const PhotonGrid &g = grid[kind];
+        if (!brute_gather && g.rf == rf) {
+            const double q[3] = {(double)(float)pos.x, (double)(float)pos.y, (double)(float)pos.z};
+            double lo[3], hi[3];
+            for (int d = 0; d < 3; d++) {
+                const double f = std::floor((q[d] - g.o[d]) / g.h);
+                lo[d] = std::max(f - 1.0, 0.0);
+                hi[d] = std::min(f + 1.0, (double)(g.n[d] - 1));
+                if (!(lo[d] <= hi[d])) return; /* no photon within reach */
+            }
+            for (int64_t z = (int64_t)lo[2]; z <= (int64_t)hi[2]; z++)
+                for (int64_t y = (int64_t)lo[1]; y <= (int64_t)hi[1]; y++)
+                    for (int64_t x = (int64_t)lo[0]; x <= (int64_t)hi[0]; x++) {
+                        const size_t c = (size_t)((z * g.n[1] + y) * g.n[0] + x);
+                        for (uint32_t j = g.start[c]; j < g.start[c + 1]; j++) test(g.idx[j]);
+                    }
+            /* (the candidates are put in ascending index order below, the
+             * brute-force loop's order) */
+        } else {
+            for (uint32_t i = 0; i < ph.size(); i++) test(i);
         }
         if (k > 0 && (int)cand.size() > k) {
             std::sort(cand.begin(), cand.end());
@@ -1680,6 +1768,7 @@ template <class T> static void load(Scene<T> &S, const sray_scene_desc *d, uint3
 {
     S.use_tree = !(flags & ORACLE_BRUTE_FORCE);
     S.q9_retraverse = (flags & ORACLE_Q9_RETRAVERSE) != 0;
+    S.brute_gather = (flags & ORACLE_BRUTE_GATHER) != 0;
     S.tris.resize(d->num_triangles);
     for (uint32_t i = 0; i < d->num_triangles; i++) {
         Tri<T> &t = S.tris[i];
@@ -2026,6 +2115,7 @@ template <class T> static int do_render(Scene<T> &S, const sray_render_params *p
 {
     /* gather radius relative to the scene octree diagonal (Q19) */
     S.gather_dist_abs = S.diag * (double)p->gather_dist;
+    for (int kind = 0; kind < 2; kind++) S.build_grid(kind, S.gather_dist_abs);
     Renderer<T> R(S, p, rgba);
     R.run(nthreads);
     if (st) {
@@ -2062,6 +2152,10 @@ template <class T> static int do_primary(Scene<T> &S, const sray_render_params *
 template <class T> static int do_gather(Scene<T> &S, int kind, const sray_gather_query *q, uint32_t n, float radius,
                                         int k, int mode, float *rgb)
 {
+    S.build_grid(kind, (double)radius);
+    /* the estimates read the caustic (irradiance) or global (radiance) map */
+    const int slot = mode == SRAY_GATHER_IRRADIANCE ? SRAY_PMAP_CAUSTIC : SRAY_PMAP_GLOBAL;
+    std::swap(S.grid[slot], S.grid[kind]);
     for (uint32_t i = 0; i < n; i++) {
         V3<T> pos = v3<T>(q[i].pos[0], q[i].pos[1], q[i].pos[2]);
         V3<T> nrm = v3<T>(q[i].norm[0], q[i].norm[1], q[i].norm[2]);
@@ -2080,6 +2174,7 @@ template <class T> static int do_gather(Scene<T> &S, int kind, const sray_gather
         rgb[i * 3 + 1] = (float)c.y;
         rgb[i * 3 + 2] = (float)c.z;
     }
+    std::swap(S.grid[slot], S.grid[kind]);
     return 0;
 }
 
@@ -2102,6 +2197,8 @@ static int do_shoot(Scene<T> &S, const sray_photon_params *pp, sray_photon *caus
     S.pm[1].resize(g.size());
     for (size_t i = 0; i < c.size(); i++) memcpy(&S.pm[0][i], &c[i], sizeof(Photon));
     for (size_t i = 0; i < g.size(); i++) memcpy(&S.pm[1][i], &g[i], sizeof(Photon));
+    S.photons_changed(0);
+    S.photons_changed(1);
     return 0;
 }
 
@@ -2204,8 +2301,13 @@ int oracle_set_photons(oracle_scene *s, int kind, const sray_photon *p, uint32_t
     if (!s || kind < 0 || kind > 1) return -1;
     std::vector<Photon> v(n);
     for (uint32_t i = 0; i < n; i++) memcpy(&v[i], &p[i], sizeof(Photon));
-    if (s->s.prec == 32) s->s.f->pm[kind] = v;
-    else s->s.d->pm[kind] = v;
+    if (s->s.prec == 32) {
+        s->s.f->pm[kind] = v;
+        s->s.f->photons_changed(kind);
+    } else {
+        s->s.d->pm[kind] = v;
+        s->s.d->photons_changed(kind);
+    }
     return 0;
 }
 

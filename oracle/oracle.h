@@ -53,6 +53,7 @@ typedef struct oracle_tree_info {
 /* flags for oracle_scene_create */
 #define ORACLE_Q9_RETRAVERSE 0x1u  /* re-traverse a mesh in every scene leaf (src/octree.inl:211-222) */
 #define ORACLE_BRUTE_FORCE   0x2u  /* Scene::cast_ray / Mesh::intersect without octrees */
+#define ORACLE_BRUTE_GATHER  0x4u  /* photon gathers test every photon (no bucket grid) */
 
 oracle_scene *oracle_scene_create(const sray_scene_desc *desc, int prec, uint32_t flags);
 void oracle_scene_destroy(oracle_scene *s);

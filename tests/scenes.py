@@ -251,3 +251,32 @@ def sparse_lanes(r, every=16):
     m = np.zeros(n * every, bool)
     m[::every] = True
     return out, m
+
+
+def look_at(pos, target, vfov_deg, shutter=0):
+    """camera -> world matrix of a camera at pos looking at target (+z
+    forward, +y up), as an sray_camera_desc"""
+    f = np.asarray(target, np.float64) - np.asarray(pos, np.float64)
+    f /= np.linalg.norm(f)
+    r = np.cross([0.0, 1.0, 0.0], f)
+    r /= np.linalg.norm(r)
+    u = np.cross(f, r)
+    cam = abi.CameraDesc()
+    cam.xform[:] = np.array([[r[0], u[0], f[0], pos[0]], [r[1], u[1], f[1], pos[1]],
+                             [r[2], u[2], f[2], pos[2]]], np.float32).ravel()
+    cam.vfov = math.radians(vfov_deg)
+    cam.max_dist = 10000.0
+    cam.shutter = shutter
+    return cam
+
+
+def desc_copy(desc_ptr, camera=None, shutter=None):
+    """a copy of a scene description (same arrays) with another camera or
+    shutter"""
+    d = abi.SceneDesc()
+    C.memmove(C.byref(d), desc_ptr, C.sizeof(d))
+    if camera is not None:
+        d.camera = camera
+    if shutter is not None:
+        d.camera.shutter = shutter
+    return d
