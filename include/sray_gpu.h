@@ -79,6 +79,12 @@ typedef struct sray_object_desc {
      * all zero = static object. */
     float   motion[3];
     int32_t motion_t0, motion_t1;
+    /* keyframed motion (XFormNode tracks: rotation, scale, pivot, more
+     * than two position keys), evaluated on the host at every msec of the
+     * scene's [frame_t0, frame_t1]: 24 floats per msec (xform, then
+     * inv_xform), frame_t1 - frame_t0 + 1 rows; NULL = xform / motion
+     * above.  A ray outside the interval takes the nearest row. */
+    const float *time_xforms;
 } sray_object_desc;
 
 typedef struct sray_mesh_desc {
@@ -104,6 +110,9 @@ typedef struct sray_light_desc {
     float   radius;        /* sphere light */
     float   dim[3];        /* box light */
     float   xform[12];     /* box light: get_xform_matrix() at time 0 */
+    /* point / sphere light with a position track: the position at every
+     * msec of [frame_t0, frame_t1] (3 floats per msec), or NULL */
+    const float *time_pos;
 } sray_light_desc;
 
 typedef struct sray_camera_desc {
@@ -113,6 +122,10 @@ typedef struct sray_camera_desc {
     int32_t shutter;       /* msec */
     /* the aspect ratio is not a scene property: Camera() takes it from the
      * render size (src/camera.cc:31), so it comes from sray_render_params */
+    /* an animated camera (Camera::get_matrix(ray.time), src/camera.cc:141):
+     * xform at every msec of the scene's [frame_t0, frame_t1] (12 floats
+     * per msec), or NULL.  Ignored in a request's own camera. */
+    const float *time_xforms;
 } sray_camera_desc;
 
 typedef struct sray_scene_desc {
@@ -451,6 +464,13 @@ typedef struct sray_host_scene sray_host_scene;
  * returned handle. */
 int  sray_scene_load_xml(const char *path, const char *xml_text, sray_host_scene **out);
 const sray_scene_desc *sray_scene_desc_of(const sray_host_scene *hs);
+/* Replaces the per-frame part of render(msec) (src/render.cc:34-41) and the
+ * XFormNode track evaluation (src/anim.inl:188-275, src/anim.cc:391-427)
+ * for a scene loaded from XML: the frame interval msec - shutter/2 ..
+ * + shutter, every transform at msec, and per-msec rows of the animated
+ * objects, lights and camera over the interval (upload the scene again to
+ * render the new frame).  The load itself evaluates msec 0. */
+int  sray_scene_set_frame(sray_host_scene *hs, int32_t msec);
 void sray_scene_free(sray_host_scene *hs);
 const char *sray_host_last_error(void);
 

@@ -383,6 +383,7 @@ template <class T> struct Object {
     V3<T> motion;
     int mt0, mt1;
     bool moving;
+    std::vector<float> rows; /* xform + inverse per msec of the frame interval (keyframed) */
 };
 
 template <class T> struct Tri {
@@ -403,6 +404,7 @@ template <class T> struct Light {
     T radius;
     V3<T> dim;
     Mat<T> xf;
+    std::vector<float> rows; /* position per msec of the frame interval (a position track) */
 };
 
 template <class T> struct Material {
@@ -450,6 +452,8 @@ template <class T> struct Scene {
     bool q9_retraverse = false;
     double gather_dist_abs = 0;
     double diag = 0;
+    int frame_t0 = 0, nrow = 1;
+    std::vector<float> cam_rows; /* camera matrix per msec (an animated camera) */
     std::vector<Photon> pm[2];
     PhotonGrid grid[2];
     bool brute_gather = false;
@@ -505,9 +509,33 @@ template <class T> struct Scene {
     }
     void photons_changed(int kind) { grid[kind] = PhotonGrid(); }
 
-    /* object matrices at a time (linear translation keys, src/anim.inl:188-261) */
+    /* the row of a per-msec table at a time: the scene's frame interval,
+     * the nearest row outside it */
+    size_t row_of(int time) const
+    {
+        int k = time - frame_t0;
+        return (size_t)(k < 0 ? 0 : k >= nrow ? nrow - 1 : k);
+    }
+    /* Light::get_position(msec) (src/anim.cc XFormNode tracks, evaluated
+     * per msec by the scene loader) */
+    V3<T> light_pos(const Light<T> &lt, int time) const
+    {
+        if (lt.rows.empty()) return lt.pos;
+        const float *r = &lt.rows[row_of(time) * 3];
+        return v3<T>(r[0], r[1], r[2]);
+    }
+
+    /* object matrices at a time: keyframed rows (XFormNode::get_xform_matrix
+     * and its inverse at each msec, src/anim.cc:391-427), or linear
+     * translation keys (src/anim.inl:188-261) */
     void xform_at(const Object<T> &o, int time, Mat<T> &M, Mat<T> &I) const
     {
+        if (!o.rows.empty()) {
+            const float *r = &o.rows[row_of(time) * 24];
+            M = mat_from<T>(r);
+            I = mat_from<T>(r + 12);
+            return;
+        }
         M = o.xf;
         I = o.inv;
         if (!o.moving) return;
@@ -1043,7 +1071,7 @@ template <class T> struct Scene {
 
     /* light sample point: Light::get_point src/light.cc:106-109,
      * SphLight src/light.cc:272-275, BoxLight src/light.cc:346-354 (time 0) */
-    V3<T> light_point(const Light<T> &lt, const ShadeCtx &sc, uint32_t path, uint32_t li, uint32_t si) const
+    V3<T> light_point(const Light<T> &lt, const ShadeCtx &sc, uint32_t path, uint32_t li, uint32_t si, int tm) const
     {
         if (lt.type == SRAY_LIGHT_BOX) {
             T fx = (T)rng_unit(sc.seed, sc.pixel, sc.sample, path, slot(K_LIGHT, li, si, 0, 0));
@@ -1055,9 +1083,9 @@ template <class T> struct Scene {
         }
         if (lt.type == SRAY_LIGHT_SPHERE) {
             V3<T> s = sph_rand(sc, path, K_LIGHT, li, si, lt.radius);
-            return lt.pos + s;
+            return light_pos(lt, tm) + s;
         }
-        return lt.pos;
+        return light_pos(lt, tm);
     }
 
     /* uniform point on a sphere of radius r (Marsaglia 1972), a bounded
@@ -1086,7 +1114,7 @@ template <class T> struct Scene {
         diff = c4<T>(0, 0, 0, 0);
         spec = c4<T>(0, 0, 0, 0);
         for (int i = 0; i < ns; i++) {
-            V3<T> lpos = light_point(lt, sc, path, li, (uint32_t)i);
+            V3<T> lpos = light_point(lt, sc, path, li, (uint32_t)i, tm);
             V3<T> ldir = lpos - pt;
             T ldist = length(ldir);
             Ray<T> sr;
@@ -1129,7 +1157,7 @@ template <class T> struct Scene {
         C4<T> diff = c4<T>((T)0.1, (T)0.1, (T)0.1, 1), spec = c4<T>(0, 0, 0, 1);
         for (size_t i = 0; i < lights.size(); i++) {
             const Light<T> &lt = lights[i];
-            V3<T> lpos = light_point(lt, sc, path, (uint32_t)i, 0);
+            V3<T> lpos = light_point(lt, sc, path, (uint32_t)i, 0, ray.time);
             V3<T> ldir = lpos - sp.pos;
             T ldist = length(ldir);
             Ray<T> sr;
@@ -1295,9 +1323,11 @@ template <class T> struct Scene {
             uint32_t u = rng_u32(seed, pix, (uint32_t)sub, 0, slot(K_CAMERA, 0, 0, 0, 2));
             r.time += (int)(u % (uint32_t)shutter) - shutter / 2;
         }
+        /* ray.transformed(get_matrix(ray.time)) */
+        const Mat<T> cm = cam_rows.empty() ? cam_xf : mat_from<T>(&cam_rows[row_of(r.time) * 12]);
         Ray<T> w = r;
-        w.o = xf_point(cam_xf, r.o);
-        w.d = xf_vec(cam_xf, r.d);
+        w.o = xf_point(cm, r.o);
+        w.d = xf_vec(cm, r.d);
         return w;
     }
 };
@@ -1448,9 +1478,10 @@ template <class T> struct PhotonTracer {
 
     /* light point: point = position, sphere = pos + sphrand(r), box = box
      * point at time 0 (src/light.cc:106-109, 272-275, 346-354) */
-    bool light_point(const Light<T> &lt, uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t purpose,
+    bool light_point(const Light<T> &lt, uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t purpose, int msec,
                      double out[3]) const
     {
+        const V3<T> lpos = S.light_pos(lt, msec);
         if (lt.type == SRAY_LIGHT_BOX) {
             T f[3];
             for (int k = 0; k < 3; k++) f[k] = (T)prand(seed, a, b, c, purpose, 0, (uint32_t)k);
@@ -1464,13 +1495,13 @@ template <class T> struct PhotonTracer {
             for (uint32_t at = 0; at < 128; at++) {
                 float d[3];
                 if (marsaglia(prand(seed, a, b, c, purpose, at, 0), prand(seed, a, b, c, purpose, at, 1), d)) {
-                    V3<T> w = lt.pos + v3<T>((T)d[0], (T)d[1], (T)d[2]) * lt.radius;
+                    V3<T> w = lpos + v3<T>((T)d[0], (T)d[1], (T)d[2]) * lt.radius;
                     out[0] = (double)w.x, out[1] = (double)w.y, out[2] = (double)w.z;
                     return true;
                 }
             }
         }
-        out[0] = (double)lt.pos.x, out[1] = (double)lt.pos.y, out[2] = (double)lt.pos.z;
+        out[0] = (double)lpos.x, out[1] = (double)lpos.y, out[2] = (double)lpos.z;
         return true;
     }
 
@@ -1497,7 +1528,7 @@ template <class T> struct PhotonTracer {
                 }
                 for (int sm = 0; sm < samples; sm++) {
                     double lp[3];
-                    light_point(lt, pp->seed, 0xffffffffu, (uint32_t)li, (uint32_t)(ti << 8 | sm), P_PROJ, lp);
+                    light_point(lt, pp->seed, 0xffffffffu, (uint32_t)li, (uint32_t)(ti << 8 | sm), P_PROJ, tm[ti], lp);
                     build_projmap(proj[li], lp, all);
                     build_projmap(spec[li], lp, caust);
                 }
@@ -1692,11 +1723,10 @@ template <class T> struct PhotonTracer {
     Ph gen_photon(size_t li, int kind, uint32_t a, uint32_t b, int msec, Ray<T> &r)
     {
         const Light<T> &lt = S.lights[li];
-        (void)msec;
         Ph ph;
         ph.type = 0;
         double lp[3];
-        light_point(lt, pp->seed, a, b, 0, P_LPOINT, lp);
+        light_point(lt, pp->seed, a, b, 0, P_LPOINT, msec, lp);
         const ProjMapT &pm = kind == 0 ? spec[li] : proj[li];
         double cov = pm.get_coverage();
         bool use = cov != 0.0;
@@ -1769,6 +1799,9 @@ template <class T> static void load(Scene<T> &S, const sray_scene_desc *d, uint3
     S.use_tree = !(flags & ORACLE_BRUTE_FORCE);
     S.q9_retraverse = (flags & ORACLE_Q9_RETRAVERSE) != 0;
     S.brute_gather = (flags & ORACLE_BRUTE_GATHER) != 0;
+    S.frame_t0 = d->frame_t0;
+    S.nrow = d->frame_t1 >= d->frame_t0 ? d->frame_t1 - d->frame_t0 + 1 : 1;
+    const size_t nrow = (size_t)S.nrow;
     S.tris.resize(d->num_triangles);
     for (uint32_t i = 0; i < d->num_triangles; i++) {
         Tri<T> &t = S.tris[i];
@@ -1816,6 +1849,7 @@ template <class T> static void load(Scene<T> &S, const sray_scene_desc *d, uint3
         o.mt0 = od.motion_t0;
         o.mt1 = od.motion_t1;
         o.moving = (od.motion[0] != 0 || od.motion[1] != 0 || od.motion[2] != 0) && od.motion_t1 > od.motion_t0;
+        if (od.time_xforms) o.rows.assign(od.time_xforms, od.time_xforms + (size_t)nrow * 24);
     }
     S.mats.resize(d->num_materials);
     for (uint32_t i = 0; i < d->num_materials; i++) {
@@ -1839,8 +1873,10 @@ template <class T> static void load(Scene<T> &S, const sray_scene_desc *d, uint3
         l.radius = ld.radius;
         l.dim = v3<T>(ld.dim[0], ld.dim[1], ld.dim[2]);
         l.xf = mat_from<T>(ld.xform);
+        if (ld.time_pos && ld.type != SRAY_LIGHT_BOX) l.rows.assign(ld.time_pos, ld.time_pos + (size_t)nrow * 3);
     }
     S.cam_xf = mat_from<T>(d->camera.xform);
+    if (d->camera.time_xforms) S.cam_rows.assign(d->camera.time_xforms, d->camera.time_xforms + (size_t)nrow * 12);
     S.vfov = d->camera.vfov;
     S.max_dist = d->camera.max_dist;
     S.shutter = d->camera.shutter;
@@ -2233,7 +2269,7 @@ static int do_light_points(const Scene<T> &S, int light, uint64_t seed, uint32_t
     sc.pixel = pixel;
     for (uint32_t i = 0; i < n; i++) {
         sc.sample = i;
-        V3<T> v = S.light_point(S.lights[(size_t)light], sc, 0u, (uint32_t)light, 0u);
+        V3<T> v = S.light_point(S.lights[(size_t)light], sc, 0u, (uint32_t)light, 0u, S.frame_t0);
         out[(size_t)i * 3] = (double)v.x;
         out[(size_t)i * 3 + 1] = (double)v.y;
         out[(size_t)i * 3 + 2] = (double)v.z;

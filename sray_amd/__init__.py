@@ -85,6 +85,13 @@ class HostScene:
     def desc_ptr(self):
         return lib().sray_scene_desc_of(self._h)
 
+    def set_frame(self, msec):
+        """evaluates the scene's keyframe tracks for render(msec): the frame
+        interval, the transforms at msec and per-msec rows over the interval"""
+        rc = lib().sray_scene_set_frame(self._h, int(msec))
+        if rc:
+            raise SrayError(rc, _host_err())
+
     def to_xml(self):
         n = C.c_size_t()
         lib().sray_scene_write_xml(self._h, None, 0, C.byref(n))

@@ -34,7 +34,7 @@ f32, i32, u32, u64, f64 = C.c_float, C.c_int32, C.c_uint32, C.c_uint64, C.c_doub
 class ObjectDesc(C.Structure):
     _fields_ = [("type", i32), ("material", i32), ("mesh", i32), ("radius", f32), ("start", f32),
                 ("end", f32), ("xform", f32 * 12), ("inv_xform", f32 * 12), ("motion", f32 * 3),
-                ("motion_t0", i32), ("motion_t1", i32)]
+                ("motion_t0", i32), ("motion_t1", i32), ("time_xforms", C.POINTER(f32))]
 
 
 class MeshDesc(C.Structure):
@@ -48,11 +48,12 @@ class MaterialDesc(C.Structure):
 
 class LightDesc(C.Structure):
     _fields_ = [("type", i32), ("pos", f32 * 3), ("color", f32 * 4), ("radius", f32), ("dim", f32 * 3),
-                ("xform", f32 * 12)]
+                ("xform", f32 * 12), ("time_pos", C.POINTER(f32))]
 
 
 class CameraDesc(C.Structure):
-    _fields_ = [("xform", f32 * 12), ("vfov", f64), ("max_dist", f32), ("shutter", i32)]
+    _fields_ = [("xform", f32 * 12), ("vfov", f64), ("max_dist", f32), ("shutter", i32),
+                ("time_xforms", C.POINTER(f32))]
 
 
 class SceneDesc(C.Structure):
@@ -158,6 +159,7 @@ EXPORTS = [
     "sray_photon_dump", "sray_photon_restore", "sray_photon_free", "sray_image_write_pfm", "sray_image_write_png",
     "sray_status_record",
     "sray_io_last_error", "sray_gpu_set_status_fd", "sray_gpu_pmap_gather_device_noalloc",
+    "sray_scene_set_frame",
 ]
 SRAY_STATUS_RECORD = 32
 
@@ -217,6 +219,7 @@ def declare(lib):
         "sray_io_last_error": (C.c_char_p, []),
         "sray_gpu_set_status_fd": (i32, [vp, C.c_int]),
         "sray_gpu_pmap_gather_device_noalloc": (i32, [vp, u32, vp, u32, f32, C.c_int, vp]),
+        "sray_scene_set_frame": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

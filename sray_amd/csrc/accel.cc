@@ -201,9 +201,17 @@ inline void xf_point(const float *m, float x, float y, float z, float out[3])
     out[2] = m[8] * x + m[9] * y + m[10] * z + m[11];
 }
 
-/* object matrix at time t: linear translation keys */
-void xform_at(const sray_object_desc &o, int time, float m[12])
+/* object matrix at time t: the keyframed row of that msec (the nearest row
+ * outside the frame interval), or linear translation keys */
+void xform_at(const sray_scene_desc *sd, const sray_object_desc &o, int time, float m[12])
 {
+    if (o.time_xforms) {
+        const int n = sd->frame_t1 - sd->frame_t0 + 1;
+        int k = time - sd->frame_t0;
+        k = k < 0 ? 0 : k >= n ? n - 1 : k;
+        std::memcpy(m, o.time_xforms + (size_t)k * 24, sizeof(float) * 12);
+        return;
+    }
     std::memcpy(m, o.xform, sizeof(float) * 12);
     bool moving = (o.motion[0] != 0 || o.motion[1] != 0 || o.motion[2] != 0) && o.motion_t1 > o.motion_t0;
     if (!moving) return;
@@ -221,7 +229,7 @@ void xform_at(const sray_object_desc &o, int time, float m[12])
 void object_box(const sray_scene_desc *s, const sray_object_desc &o, int time, float b[6])
 {
     float m[12];
-    xform_at(o, time, m);
+    xform_at(s, o, time, m);
     if (o.type == SRAY_OBJ_SPHERE) {
         float p[3];
         xf_point(m, 0.0f, 0.0f, 0.0f, p);
@@ -260,6 +268,11 @@ void object_box(const sray_scene_desc *s, const sray_object_desc &o, int time, f
 }
 
 } // namespace
+
+void object_box_at(const sray_scene_desc *s, const sray_object_desc &o, int time, float b[6])
+{
+    object_box(s, o, time, b);
+}
 
 /* Scene::build_tree, src/scene.cc:364-378: union of bounds at t0 and t1 */
 void object_boxes(const sray_scene_desc *s, std::vector<float> &boxes)
@@ -388,6 +401,12 @@ const sray_scene_desc *HostScene::finish()
     desc.mesh_oct_items = mesh_oct_items;
     desc.frame_t0 = frame_t0;
     desc.frame_t1 = frame_t1;
+    for (size_t i = 0; i < objects.size(); i++)
+        objects[i].time_xforms = i < obj_tab.size() && !obj_tab[i].empty() ? obj_tab[i].data() : nullptr;
+    for (size_t i = 0; i < lights.size(); i++)
+        lights[i].time_pos = i < light_tab.size() && !light_tab[i].empty() ? light_tab[i].data() : nullptr;
+    camera.time_xforms = cam_tab.empty() ? nullptr : cam_tab.data();
+    desc.camera = camera;
     return &desc;
 }
 

@@ -55,6 +55,17 @@ int sray_scene_load_xml(const char *path, const char *xml_text, sray_host_scene 
 
 const sray_scene_desc *sray_scene_desc_of(const sray_host_scene *h) { return h ? &h->hs.desc : nullptr; }
 
+int sray_scene_set_frame(sray_host_scene *h, int32_t msec)
+{
+    if (!h) return SRAY_ERR_ARG;
+    std::string err;
+    if (!h->hs.set_frame(msec, err)) {
+        g_host_err = err;
+        return SRAY_ERR_ARG;
+    }
+    return SRAY_OK;
+}
+
 void sray_scene_free(sray_host_scene *h) { delete h; }
 
 int sray_scene_generate(int config, uint64_t seed, int variant, sray_host_scene **out)

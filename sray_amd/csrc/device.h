@@ -38,7 +38,7 @@ struct alignas(16) DevObj {
     int32_t mat;     /* material index (default material resolved on upload) */
     uint32_t root;   /* mesh: root node of its octree */
     uint32_t tri_base;
-    int32_t moving;
+    int32_t moving;  /* 0 static, 1 linear translation (motion), 2 keyframed rows (tab) */
     float xf[12];    /* object -> world */
     float motion[3];
     int32_t mt0;
@@ -46,6 +46,9 @@ struct alignas(16) DevObj {
     int32_t mesh;
     int32_t has_attr; /* mesh: 1 normals, 2 texcoords, 4 tangents */
     int32_t pad;
+    /* keyframed motion: xform + inverse (24 floats) per msec from tab_t0 */
+    const float *tab;
+    int32_t tab_t0, tab_n;
 };
 
 struct alignas(16) DevMat {
@@ -61,7 +64,18 @@ struct alignas(16) DevLight {
     float radius;
     float dim[3];
     float xf[12];
+    /* a point / sphere light's position (3 floats) per msec from tab_t0 */
+    const float *tab;
+    int32_t tab_t0, tab_n;
 };
+
+/* the row of a per-msec table at a time (the nearest row outside it) */
+__device__ __forceinline__ const float *time_row(const float *tab, int t0, int n, int time, int width)
+{
+    int k = time - t0;
+    k = k < 0 ? 0 : (k >= n ? n - 1 : k);
+    return tab + (size_t)k * (size_t)width;
+}
 
 struct SceneDev {
     const float4 *nlo;
@@ -86,6 +100,9 @@ struct SceneDev {
     float max_dist;
     int32_t shutter;
     float gather_abs;        /* gather radius (absolute) */
+    /* animated camera: xform (12 floats) per msec from cam_t0 */
+    const float *cam_tab;
+    int32_t cam_t0, cam_n;
 };
 
 /* ------------------------------------------------------------------ */

@@ -56,8 +56,10 @@ __device__ __forceinline__ void primary_ray(const CamDev &C, const RenderDev &P,
         uint32_t u = rng_u32(P.seed, pix, (uint32_t)sample, 0u, rslot(K_CAMERA, 0, 0, 0, 2));
         time += (int)(u % (uint32_t)C.shutter) - C.shutter / 2;
     }
-    o = xf_point(C.cam, v3(0.0f, 0.0f, 0.0f));
-    d = xf_vec(C.cam, dl);
+    /* ray.transformed(get_matrix(ray.time)) */
+    const float *cm = C.tab ? time_row(C.tab, C.t0, C.n, time, 12) : C.cam;
+    o = xf_point(cm, v3(0.0f, 0.0f, 0.0f));
+    d = xf_vec(cm, dl);
 }
 
 __global__ void k_primary(CamDev C, RenderDev P, int sample, sray_ray *out, uint32_t n)
@@ -225,7 +227,7 @@ __device__ __forceinline__ V3 sph_rand(uint64_t seed, uint32_t pix, uint32_t sam
 }
 
 __device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uint32_t pix, uint32_t sample,
-                                          uint32_t path, uint32_t li, uint32_t si)
+                                          uint32_t path, uint32_t li, uint32_t si, int time)
 {
     if (lt.type == SRAY_LIGHT_BOX) {
         float fx = rng_unit(seed, pix, sample, path, rslot(K_LIGHT, li, si, 0, 0));
@@ -235,11 +237,17 @@ __device__ __forceinline__ V3 light_point(const DevLight &lt, uint64_t seed, uin
                   fz * lt.dim[2] - 0.5f * lt.dim[2]);
         return xf_point(lt.xf, v);
     }
+    /* get_position(msec) of a point / sphere light */
+    V3 p = v3(lt.pos[0], lt.pos[1], lt.pos[2]);
+    if (lt.tab) {
+        const float *r = time_row(lt.tab, lt.tab_t0, lt.tab_n, time, 3);
+        p = v3(r[0], r[1], r[2]);
+    }
     if (lt.type == SRAY_LIGHT_SPHERE) {
         V3 s = sph_rand(seed, pix, sample, path, K_LIGHT, li, si, lt.radius);
-        return add(v3(lt.pos[0], lt.pos[1], lt.pos[2]), s);
+        return add(p, s);
     }
-    return v3(lt.pos[0], lt.pos[1], lt.pos[2]);
+    return p;
 }
 
 /* Shading of one queued ray and its hit (shade_phong, src/shader.cc:83-247,
@@ -301,7 +309,7 @@ __device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restr
             dir.z = W.z * (0.1f * tex.z);
             for (int l = 0; l < nl; l++) {
                 const DevLight &lt = S.lights[l];
-                V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u);
+                V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, 0u, time);
                 V3 ldir = sub(lpos, Pw);
                 float ldist = length(ldir);
                 V3 sd = ldir;
@@ -376,7 +384,8 @@ __device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restr
                     int ns = lt.type != SRAY_LIGHT_POINT ? P.shadow_samples : 1;
                     float scale = ns > 1 ? 1.0f / (float)ns : 1.0f;
                     for (int si = 0; si < ns; si++) {
-                        V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si);
+                        V3 lpos = light_point(lt, P.seed, pix, (uint32_t)sample, path, (uint32_t)l, (uint32_t)si,
+                                              time);
                         V3 ldir = sub(lpos, Pw);
                         float ldist = length(ldir);
                         V3 sd = ldir;
@@ -1312,11 +1321,15 @@ int grow_sub(Engine &E, uint64_t need)
     return SRAY_OK;
 }
 
-CamDev cam_dev(const sray_camera_desc &d)
+/* a request's own camera (tab null), or the scene's with its rows */
+CamDev cam_dev(const sray_camera_desc &d, const float *tab = nullptr, int t0 = 0, int n = 0)
 {
     CamDev C;
     memset(&C, 0, sizeof C);
     memcpy(C.cam, d.xform, sizeof C.cam);
+    C.tab = tab;
+    C.t0 = t0;
+    C.n = n;
     C.fz = (float)(1.0 / std::tan(0.5 * d.vfov));
     C.max_dist = d.max_dist;
     C.shutter = d.shutter;
@@ -1380,13 +1393,25 @@ int admit(Engine &E, Job *j)
     if (rc) return rc;
     GpuScene *g = find_scene(c, j->scene_id);
     if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", j->scene_id);
+    if (g->timed) {
+        /* ray times: frame_time, or frame_time - shutter/2 + [0, shutter)
+         * with motion blur (Camera::get_primary_ray, src/camera.cc:137-139) */
+        const int sh = j->use_cam ? j->cam.shutter : g->camera.shutter;
+        const bool blur = j->p.mblur && sh > 0;
+        const int lo = blur ? j->p.frame_time - sh / 2 : j->p.frame_time;
+        const int hi = blur ? lo + sh - 1 : j->p.frame_time;
+        if (lo < g->frame_t0 || hi > g->frame_t1)
+            return fail(c, SRAY_ERR_ARG,
+                        "ray times %d..%d msec leave the scene's keyframed interval %d..%d (set the frame first)", lo,
+                        hi, g->frame_t0, g->frame_t1);
+    }
     uint32_t s = 0;
     while (s < MAX_SLOTS && E.slot_job[s]) s++;
     if (s == MAX_SLOTS) return fail(c, SRAY_ERR_LIMIT, "more than %u requests in flight", MAX_SLOTS);
     ReqDev Q;
     memset(&Q, 0, sizeof Q);
     fill_render_dev(&j->p, Q.P);
-    Q.C = cam_dev(j->use_cam ? j->cam : g->camera);
+    Q.C = j->use_cam ? cam_dev(j->cam) : cam_dev(g->camera, g->dev.cam_tab, g->dev.cam_t0, g->dev.cam_n);
     for (int kind = 0; kind < 2; kind++) {
         bool want = kind == SRAY_PMAP_CAUSTIC ? Q.P.caust != 0 : Q.P.gi != 0;
         if (!want) continue;
@@ -2427,7 +2452,8 @@ int sray_gpu_primary_rays_device(sray_gpu_ctx *c, uint32_t scene_id, const sray_
     RenderDev R;
     fill_render_dev(p, R);
     uint32_t n = (uint32_t)R.W * (uint32_t)(p->y1 - p->y0);
-    k_primary<<<blocks(n, 256), 256, 0, c->stream>>>(cam_dev(g->camera), R, sample, d_rays, n);
+    k_primary<<<blocks(n, 256), 256, 0, c->stream>>>(cam_dev(g->camera, g->dev.cam_tab, g->dev.cam_t0, g->dev.cam_n), R,
+                                                     sample, d_rays, n);
     HIPCHK(c, hipGetLastError());
     return SRAY_OK;
 }

@@ -19,6 +19,44 @@
 namespace sray {
 
 /* ------------------------------------------------------------------ */
+/* keyframe tracks of an XFormNode (anim_host.cc)                        */
+/* ------------------------------------------------------------------ */
+enum { SR_INTERP_STEP, SR_INTERP_LINEAR, SR_INTERP_CUBIC };
+enum { SR_EXTRAP_CLAMP, SR_EXTRAP_REPEAT, SR_EXTRAP_PINGPONG };
+struct TrackKey3 {
+    int time;
+    double v[3];
+};
+struct TrackKeyQ {
+    int time;
+    double q[4];     /* s, x, y, z (normalised, or from an axis-angle) */
+    float raw[4];    /* the XML values, for writing the scene back */
+    int axisrot;     /* raw is "angle(deg) axis" rather than "s x y z" */
+};
+struct XFormTracks {
+    std::vector<TrackKey3> pos, scale;
+    std::vector<TrackKeyQ> rot;
+    double pivot[3] = {0, 0, 0};
+    int interp = SR_INTERP_LINEAR, extrap = SR_EXTRAP_CLAMP;
+    void set_pos(int t, double x, double y, double z);
+    void set_scale(int t, double x, double y, double z);
+    void set_rot(int t, const double q[4], const float raw[4], int axisrot);
+    bool animated() const { return pos.size() > 1 || rot.size() > 1 || scale.size() > 1; }
+    /* two position keys, linear, clamped, no rotation or scale keys beyond
+     * one, no pivot: a translation the device interpolates itself */
+    bool linear_translation() const;
+    /* Track::operator() of a vector track (false: no keys, def returned) */
+    bool vec(const std::vector<TrackKey3> &tr, int time, const double def[3], double out[3]) const;
+    void position(int time, double out[3]) const;
+    void scaling(int time, double out[3]) const;
+    void rotation(int time, double q[4]) const;
+    /* XFormNode::get_xform_matrix: pivot * T * R * S * pivot^-1 */
+    void matrix(int time, double m[12]) const;
+};
+void quat_from_axis_angle(double ax, double ay, double az, double angle, double q[4]);
+void quat_to_matrix(const double q[4], double m[12]);
+
+/* ------------------------------------------------------------------ */
 /* an owning flat scene: the vectors behind an sray_scene_desc           */
 /* ------------------------------------------------------------------ */
 struct HostScene {
@@ -40,8 +78,25 @@ struct HostScene {
     int frame_t0 = 0, frame_t1 = 0;
     sray_scene_desc desc{};
 
+    /* keyframe tracks of a scene loaded from XML (one per object / light;
+     * the camera's, and a target camera's target) and the frame they were
+     * last evaluated at */
+    bool has_tracks = false;
+    std::vector<XFormTracks> obj_tracks, light_tracks;
+    XFormTracks cam_tracks, cam_target_tracks;
+    int frame_msec = 0;
+    /* per-millisecond rows over [frame_t0, frame_t1] of the animated ones:
+     * object xform + inverse (24 floats), light position (3), camera
+     * xform (12); empty = not animated */
+    std::vector<std::vector<float>> obj_tab, light_tab;
+    std::vector<float> cam_tab;
+
     /* refresh desc's pointers/counts from the vectors */
     const sray_scene_desc *finish();
+    /* evaluates the tracks for render(msec) (src/render.cc:34-55): the
+     * frame interval msec - shutter/2 .. + shutter, the matrices at msec
+     * and the rows of the animated transforms over the interval */
+    bool set_frame(int msec, std::string &err);
 };
 
 /* TargetCamera::get_matrix (src/camera.cc:216-234), double precision */
@@ -92,6 +147,8 @@ uint32_t build_flat_octree(const std::vector<float> &boxes, int max_items, int m
 /* item boxes in float, same arithmetic as the device:
  * scene level = union of the object's bounds at t0 and t1 */
 void object_boxes(const sray_scene_desc *s, std::vector<float> &boxes);
+/* one object's bounds at a time (Object::calc_bounds) */
+void object_box_at(const sray_scene_desc *s, const sray_object_desc &o, int time, float b[6]);
 void triangle_boxes(const sray_scene_desc *s, uint32_t mesh, std::vector<float> &boxes);
 
 /* ------------------------------------------------------------------ */
@@ -106,11 +163,19 @@ struct PhotonObj {
     bool moving = false;
     bool specular = false; /* caustic caster (reflect or refract attribute > 0) */
     float lin[6] = {};     /* mesh: vertex bounds under the matrix without translation */
+    /* keyframed object: its bounds (6 floats) per msec from the scene's
+     * frame_t0 (empty otherwise) */
+    std::vector<float> box_rows;
+    int row_t0 = 0;
 };
 /* what the projection maps need of a scene, kept after upload */
 struct PhotonScene {
     std::vector<PhotonObj> objs;
-    std::vector<sray_light_desc> lights;
+    std::vector<sray_light_desc> lights; /* time_pos cleared: see light_rows */
+    std::vector<std::vector<float>> light_rows; /* a light's position per msec, or empty */
+    int frame_t0 = 0, nrow = 0;
+    /* Light::get_position(msec) of light li */
+    void light_pos(int li, int time, float out[3]) const;
 };
 struct ProjMap {
     uint8_t cells[SRAY_PROJ_U * SRAY_PROJ_V]; /* v-major */
@@ -120,8 +185,8 @@ void photon_scene_from_desc(const sray_scene_desc *s, PhotonScene &ps);
 void photon_object_box(const PhotonObj &o, int time, float b[6]);
 void cell_from_dir(float x, float y, float z, int &uc, int &vc);
 void cell_tables(float cv[SRAY_PROJ_V + 1], float cu[SRAY_PROJ_U + 1]);
-void photon_light_point(const sray_light_desc &l, uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t purpose,
-                        float out[3]);
+void photon_light_point(const PhotonScene &ps, int li, int time, uint64_t seed, uint32_t a, uint32_t b, uint32_t c,
+                        uint32_t purpose, float out[3]);
 /* all = map of every object, spec = map of the caustic casters */
 void build_projmaps(const PhotonScene &ps, int light, int t0, int t1, uint64_t seed, ProjMap &all, ProjMap &spec);
 void photon_counts(const PhotonScene &ps, int num, std::vector<uint32_t> &nphot);

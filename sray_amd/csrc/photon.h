@@ -77,7 +77,7 @@ __device__ __forceinline__ void cell_of(const CellTab &T, float x, float y, floa
 /* light sample point for photon work (Light::get_point, src/light.cc:106-109,
  * 272-275, 346-354), same draws as the host's photon_light_point */
 __device__ __forceinline__ V3 photon_light_point(const DevLight &lt, uint64_t seed, uint32_t a, uint32_t b, uint32_t c,
-                                                 uint32_t purpose)
+                                                 uint32_t purpose, int msec)
 {
     if (lt.type == SRAY_LIGHT_BOX) {
         float f0 = prand(seed, a, b, c, purpose, 0, 0);
@@ -86,14 +86,19 @@ __device__ __forceinline__ V3 photon_light_point(const DevLight &lt, uint64_t se
         V3 v = v3(f0 * lt.dim[0] - 0.5f * lt.dim[0], f1 * lt.dim[1] - 0.5f * lt.dim[1], f2 * lt.dim[2] - 0.5f * lt.dim[2]);
         return xf_point(lt.xf, v);
     }
+    float p[3] = {lt.pos[0], lt.pos[1], lt.pos[2]};
+    if (lt.tab) {
+        const float *r = time_row(lt.tab, lt.tab_t0, lt.tab_n, msec, 3);
+        p[0] = r[0], p[1] = r[1], p[2] = r[2];
+    }
     if (lt.type == SRAY_LIGHT_SPHERE) {
         for (uint32_t at = 0; at < PHOTON_SPHERE_ATTEMPTS; at++) {
             float d[3];
             if (marsaglia(prand(seed, a, b, c, purpose, at, 0), prand(seed, a, b, c, purpose, at, 1), d))
-                return v3(lt.pos[0] + d[0] * lt.radius, lt.pos[1] + d[1] * lt.radius, lt.pos[2] + d[2] * lt.radius);
+                return v3(p[0] + d[0] * lt.radius, p[1] + d[1] * lt.radius, p[2] + d[2] * lt.radius);
         }
     }
-    return v3(lt.pos[0], lt.pos[1], lt.pos[2]);
+    return v3(p[0], p[1], p[2]);
 }
 
 template <bool COUNT>
@@ -118,7 +123,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon(SceneDev S, PhotonJob J,
         int msec = J.t0 == J.t1 ? J.t0
                                 : (int)(rng_u32(seed, a, b, 0, rslot(K_PHOTON, P_TIME, 0, 0, 0)) %
                                         (uint32_t)(J.t1 - J.t0)) + J.t0;
-        V3 ro = photon_light_point(lt, seed, a, b, 0, P_LPOINT);
+        V3 ro = photon_light_point(lt, seed, a, b, 0, P_LPOINT, msec);
         float col[3] = {PL.col0[0], PL.col0[1], PL.col0[2]};
         /* emission direction: uniform, rejected until it falls in a full
          * cell of the projection map (ProjMap, src/projmap.cc:194-208) */

@@ -187,6 +187,12 @@ void photon_scene_from_desc(const sray_scene_desc *s, PhotonScene &ps)
         o.mt0 = d.motion_t0;
         o.mt1 = d.motion_t1;
         o.moving = (d.motion[0] != 0 || d.motion[1] != 0 || d.motion[2] != 0) && d.motion_t1 > d.motion_t0;
+        if (d.time_xforms) {
+            const int n = s->frame_t1 - s->frame_t0 + 1;
+            o.box_rows.resize((size_t)n * 6);
+            o.row_t0 = s->frame_t0;
+            for (int t = 0; t < n; t++) object_box_at(s, d, s->frame_t0 + t, &o.box_rows[(size_t)t * 6]);
+        }
         /* caustic casters: Material::get_value falls back to 1 for a missing
          * reflect/refract attribute (src/material.cc:164-168) */
         o.specular = false;
@@ -223,12 +229,40 @@ void photon_scene_from_desc(const sray_scene_desc *s, PhotonScene &ps)
         }
     }
     ps.lights.assign(s->lights, s->lights + s->num_lights);
+    ps.frame_t0 = s->frame_t0;
+    ps.nrow = s->frame_t1 - s->frame_t0 + 1;
+    ps.light_rows.assign(s->num_lights, std::vector<float>());
+    for (uint32_t i = 0; i < s->num_lights; i++) {
+        if (s->lights[i].time_pos && s->lights[i].type != SRAY_LIGHT_BOX)
+            ps.light_rows[i].assign(s->lights[i].time_pos, s->lights[i].time_pos + (size_t)ps.nrow * 3);
+        ps.lights[i].time_pos = nullptr;
+    }
+}
+
+void PhotonScene::light_pos(int li, int time, float out[3]) const
+{
+    const std::vector<float> &r = light_rows[(size_t)li];
+    if (r.empty()) {
+        for (int k = 0; k < 3; k++) out[k] = lights[(size_t)li].pos[k];
+        return;
+    }
+    int k = time - frame_t0;
+    k = k < 0 ? 0 : k >= nrow ? nrow - 1 : k;
+    for (int a = 0; a < 3; a++) out[a] = r[(size_t)k * 3 + a];
 }
 
 /* Object::calc_bounds at a time (sphere src/sphere.cc:27-34, cylinder
  * src/cylinder.cc:63-86, mesh src/mesh.cc:169-188) */
 void photon_object_box(const PhotonObj &o, int time, float b[6])
 {
+    if (!o.box_rows.empty()) {
+        /* keyframed: the bounds at that msec (the nearest row outside) */
+        const int n = (int)(o.box_rows.size() / 6);
+        int k = time - o.row_t0;
+        k = k < 0 ? 0 : k >= n ? n - 1 : k;
+        memcpy(b, &o.box_rows[(size_t)k * 6], 6 * sizeof(float));
+        return;
+    }
     float m[12];
     memcpy(m, o.xform, sizeof m);
     float t[3];
@@ -263,10 +297,11 @@ void photon_object_box(const PhotonObj &o, int time, float b[6])
  * sphere light = position + radius * uniform unit vector, box light = the
  * box matrix applied to a uniform point of the box (Light::get_point,
  * src/light.cc:106-109, 272-275, 346-354) */
-void photon_light_point(const sray_light_desc &l, uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t purpose,
-                        float out[3])
+void photon_light_point(const PhotonScene &ps, int li, int time, uint64_t seed, uint32_t a, uint32_t b, uint32_t c,
+                        uint32_t purpose, float out[3])
 {
     using namespace srd;
+    const sray_light_desc &l = ps.lights[(size_t)li];
     if (l.type == SRAY_LIGHT_BOX) {
         float f[3];
         for (int k = 0; k < 3; k++) f[k] = prand(seed, a, b, c, purpose, 0, (uint32_t)k);
@@ -274,16 +309,18 @@ void photon_light_point(const sray_light_desc &l, uint64_t seed, uint32_t a, uin
                   f[2] * l.dim[2] - 0.5f * l.dim[2], out);
         return;
     }
+    float pos[3];
+    ps.light_pos(li, time, pos);
     if (l.type == SRAY_LIGHT_SPHERE) {
         for (uint32_t at = 0; at < PHOTON_SPHERE_ATTEMPTS; at++) {
             float d[3];
             if (marsaglia(prand(seed, a, b, c, purpose, at, 0), prand(seed, a, b, c, purpose, at, 1), d)) {
-                for (int k = 0; k < 3; k++) out[k] = l.pos[k] + d[k] * l.radius;
+                for (int k = 0; k < 3; k++) out[k] = pos[k] + d[k] * l.radius;
                 return;
             }
         }
     }
-    for (int k = 0; k < 3; k++) out[k] = l.pos[k];
+    for (int k = 0; k < 3; k++) out[k] = pos[k];
 }
 
 /* Light::build_proj_intern (src/light.cc:151-205): the maps of one light
@@ -309,7 +346,8 @@ void build_projmaps(const PhotonScene &ps, int light, int t0, int t1, uint64_t s
         }
         for (int sm = 0; sm < samples; sm++) {
             float lpf[3];
-            photon_light_point(lt, seed, 0xffffffffu, (uint32_t)light, (uint32_t)(ti << 8 | sm), srd::P_PROJ, lpf);
+            photon_light_point(ps, light, tm[ti], seed, 0xffffffffu, (uint32_t)light, (uint32_t)(ti << 8 | sm), srd::P_PROJ,
+                               lpf);
             double lp[3] = {(double)lpf[0], (double)lpf[1], (double)lpf[2]};
             add_boxes(all, lp, boxes);
             add_boxes(spec, lp, caust);

@@ -73,6 +73,9 @@ struct CamDev {
     float max_dist;
     int32_t shutter;
     int32_t pad;
+    /* animated camera: xform (12 floats) per msec from t0, or null */
+    const float *tab;
+    int32_t t0, n;
 };
 
 /* one request of a batch: its parameters, camera, photon maps and where its

@@ -65,6 +65,11 @@ struct GpuScene {
     bool has_phong = false, pos_reflect = false, pos_refract = false;
     /* some mesh object carries vertex attributes (interpolated at hits) */
     bool attr_meshes = false;
+    /* the frame interval of the scene's keyframed rows (timed: some object,
+     * light or the camera has them; ray times outside it take the nearest
+     * row, so renders and photon shooting must stay inside) */
+    int frame_t0 = 0, frame_t1 = 0;
+    bool timed = false;
 };
 
 struct Engine; /* multi-request wavefront renderer (engine.hip) */

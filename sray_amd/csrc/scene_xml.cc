@@ -243,200 +243,48 @@ bool xml_parse(const std::string &text, XmlNode &root, std::string &err)
 }
 
 /* ================================================================ */
-/* transforms: XFormNode (src/anim.cc)                                */
+/* transforms: XFormNode::load_xml (src/anim.cc:68-121)               */
 /* ================================================================ */
 namespace {
 
-struct Quat {
-    double s = 1, x = 0, y = 0, z = 0;
-};
-
-Quat quat_axis_angle(double ax, double ay, double az, double angle)
+void load_xform(const XmlNode &n, XFormTracks &xf)
 {
-    /* Quaternion(axis, angle): s = cos(a/2), v = axis * sin(a/2) */
-    Quat q;
-    double h = angle / 2.0;
-    q.s = cos(h);
-    q.x = ax * sin(h);
-    q.y = ay * sin(h);
-    q.z = az * sin(h);
-    return q;
+    int msec = 0;
+    const XmlAttr *a;
+    if ((a = n.get("interpolate"))) {
+        if (a->str == "linear") xf.interp = SR_INTERP_LINEAR;
+        else if (a->str == "step") xf.interp = SR_INTERP_STEP;
+        else if (a->str == "cubic") xf.interp = SR_INTERP_CUBIC;
+    }
+    if ((a = n.get("extrapolate"))) {
+        if (a->str == "clamp") xf.extrap = SR_EXTRAP_CLAMP;
+        else if (a->str == "repeat") xf.extrap = SR_EXTRAP_REPEAT;
+        else if (a->str == "pingpong") xf.extrap = SR_EXTRAP_PINGPONG;
+    }
+    if ((a = n.get("time")) && a->type == ATYPE_INT) msec = a->ival;
+    if ((a = n.get("pos")) && a->type == ATYPE_VEC) xf.set_pos(msec, a->vval[0], a->vval[1], a->vval[2]);
+    if ((a = n.get("rot")) && a->type == ATYPE_VEC) {
+        /* Quaternion(s, x, y, z), normalised */
+        double q[4] = {a->vval[0], a->vval[1], a->vval[2], a->vval[3]};
+        double l = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        if (l > 0)
+            for (double &v : q) v /= l;
+        xf.set_rot(msec, q, a->vval, 0);
+    }
+    if ((a = n.get("axisrot")) && a->type == ATYPE_VEC) {
+        double q[4];
+        quat_from_axis_angle(a->vval[1], a->vval[2], a->vval[3], a->vval[0] * M_PI / 180.0, q);
+        xf.set_rot(msec, q, a->vval, 1);
+    }
+    if ((a = n.get("scale")) && a->type == ATYPE_VEC) xf.set_scale(msec, a->vval[0], a->vval[1], a->vval[2]);
+    if ((a = n.get("pivot")) && a->type == ATYPE_VEC)
+        for (int k = 0; k < 3; k++) xf.pivot[k] = a->vval[k];
 }
 
-void quat_to_mat(const Quat &q, double m[12])
-{
-    double s = q.s, x = q.x, y = q.y, z = q.z;
-    mat_identity(m);
-    m[0] = 1 - 2 * y * y - 2 * z * z;
-    m[1] = 2 * x * y - 2 * s * z;
-    m[2] = 2 * z * x + 2 * s * y;
-    m[4] = 2 * x * y + 2 * s * z;
-    m[5] = 1 - 2 * x * x - 2 * z * z;
-    m[6] = 2 * y * z - 2 * s * x;
-    m[8] = 2 * z * x - 2 * s * y;
-    m[9] = 2 * y * z + 2 * s * x;
-    m[10] = 1 - 2 * x * x - 2 * y * y;
-}
-
-template <class V> struct Key {
-    int time;
-    V val;
-};
-
-struct V3d {
-    double x = 0, y = 0, z = 0;
-};
-
-/* XFormNode: position / rotation / scale tracks + pivot */
-struct XForm {
-    std::vector<Key<V3d>> pos, scale;
-    std::vector<Key<Quat>> rot;
-    V3d pivot;
-    int interp_step = 0;
-
-    template <class V> static void set_key(std::vector<Key<V>> &tr, int t, const V &v)
-    {
-        for (auto &k : tr)
-            if (k.time == t) {
-                k.val = v;
-                return;
-            }
-        tr.push_back({t, v});
-        std::sort(tr.begin(), tr.end(), [](const Key<V> &a, const Key<V> &b) { return a.time < b.time; });
-    }
-
-    /* XFormNode::load_xml, src/anim.cc:68-121 */
-    void load(const XmlNode &n)
-    {
-        int msec = 0;
-        const XmlAttr *a;
-        if ((a = n.get("interpolate")) && a->str == "step") interp_step = 1;
-        if ((a = n.get("time")) && a->type == ATYPE_INT) msec = a->ival;
-        if ((a = n.get("pos")) && a->type == ATYPE_VEC) set_key(pos, msec, V3d{a->vval[0], a->vval[1], a->vval[2]});
-        if ((a = n.get("rot")) && a->type == ATYPE_VEC) {
-            Quat q;
-            q.s = a->vval[0];
-            q.x = a->vval[1];
-            q.y = a->vval[2];
-            q.z = a->vval[3];
-            double l = sqrt(q.s * q.s + q.x * q.x + q.y * q.y + q.z * q.z);
-            if (l > 0) {
-                q.s /= l;
-                q.x /= l;
-                q.y /= l;
-                q.z /= l;
-            }
-            set_key(rot, msec, q);
-        }
-        if ((a = n.get("axisrot")) && a->type == ATYPE_VEC) {
-            double ang = a->vval[0] * M_PI / 180.0;
-            set_key(rot, msec, quat_axis_angle(a->vval[1], a->vval[2], a->vval[3], ang));
-        }
-        if ((a = n.get("scale")) && a->type == ATYPE_VEC) set_key(scale, msec, V3d{a->vval[0], a->vval[1], a->vval[2]});
-        if ((a = n.get("pivot")) && a->type == ATYPE_VEC) pivot = V3d{a->vval[0], a->vval[1], a->vval[2]};
-    }
-
-    /* Track<T>::operator(), src/anim.inl:188-261 (clamped, linear or step) */
-    V3d eval_v(const std::vector<Key<V3d>> &tr, int t, V3d def) const
-    {
-        if (tr.empty()) return def;
-        if (t <= tr.front().time) return tr.front().val;
-        if (t >= tr.back().time) return tr.back().val;
-        for (size_t i = 0; i + 1 < tr.size(); i++) {
-            if (t >= tr[i].time && t < tr[i + 1].time) {
-                if (interp_step) return tr[i].val;
-                double u = (double)(t - tr[i].time) / (double)(tr[i + 1].time - tr[i].time);
-                const V3d &a = tr[i].val, &b = tr[i + 1].val;
-                return V3d{a.x + (b.x - a.x) * u, a.y + (b.y - a.y) * u, a.z + (b.z - a.z) * u};
-            }
-        }
-        return tr.back().val;
-    }
-    Quat eval_q(int t) const
-    {
-        if (rot.empty()) return Quat();
-        if (t <= rot.front().time) return rot.front().val;
-        if (t >= rot.back().time) return rot.back().val;
-        for (size_t i = 0; i + 1 < rot.size(); i++) {
-            if (t >= rot[i].time && t < rot[i + 1].time) {
-                if (interp_step) return rot[i].val;
-                double u = (double)(t - rot[i].time) / (double)(rot[i + 1].time - rot[i].time);
-                /* slerp */
-                Quat a = rot[i].val, b = rot[i + 1].val;
-                double c = a.s * b.s + a.x * b.x + a.y * b.y + a.z * b.z;
-                if (c < 0) {
-                    c = -c;
-                    b.s = -b.s;
-                    b.x = -b.x;
-                    b.y = -b.y;
-                    b.z = -b.z;
-                }
-                double wa, wb;
-                if (c > 0.9995) {
-                    wa = 1 - u;
-                    wb = u;
-                } else {
-                    double th = acos(c), st = sin(th);
-                    wa = sin((1 - u) * th) / st;
-                    wb = sin(u * th) / st;
-                }
-                Quat q;
-                q.s = a.s * wa + b.s * wb;
-                q.x = a.x * wa + b.x * wb;
-                q.y = a.y * wa + b.y * wb;
-                q.z = a.z * wa + b.z * wb;
-                return q;
-            }
-        }
-        return rot.back().val;
-    }
-    V3d position(int t) const { return eval_v(pos, t, V3d{0, 0, 0}); }
-
-    /* XFormNode::get_xform_matrix, src/anim.cc:391-419:
-     * pivot * T * R * S * pivot^-1 */
-    void matrix(int t, double m[12]) const
-    {
-        V3d p = position(t);
-        V3d s = eval_v(scale, t, V3d{1, 1, 1});
-        Quat q = eval_q(t);
-        double P[12], T[12], R[12], S[12], NP[12], tmp[12];
-        mat_identity(P);
-        P[3] = pivot.x;
-        P[7] = pivot.y;
-        P[11] = pivot.z;
-        mat_identity(NP);
-        NP[3] = -pivot.x;
-        NP[7] = -pivot.y;
-        NP[11] = -pivot.z;
-        mat_identity(T);
-        T[3] = p.x;
-        T[7] = p.y;
-        T[11] = p.z;
-        quat_to_mat(q, R);
-        mat_identity(S);
-        S[0] = s.x;
-        S[5] = s.y;
-        S[10] = s.z;
-        mat_mul(P, T, tmp);
-        mat_mul(tmp, R, tmp);
-        mat_mul(tmp, S, tmp);
-        mat_mul(tmp, NP, m);
-    }
-    /* two position keys and static rotation/scale -> linear translation */
-    bool linear_motion(V3d &delta, int &t0, int &t1) const
-    {
-        if (pos.size() != 2 || rot.size() > 1 || scale.size() > 1 || interp_step) return false;
-        delta = V3d{pos[1].val.x - pos[0].val.x, pos[1].val.y - pos[0].val.y, pos[1].val.z - pos[0].val.z};
-        t0 = pos[0].time;
-        t1 = pos[1].time;
-        return true;
-    }
-};
-
-void load_xforms(const XmlNode &n, XForm &xf)
+void load_xforms(const XmlNode &n, XFormTracks &xf)
 {
     for (const XmlNode &c : n.chld)
-        if (c.name == "xform") xf.load(c);
+        if (c.name == "xform") load_xform(c, xf);
 }
 
 /* ================================================================ */
@@ -501,30 +349,6 @@ bool load_material(const XmlNode &n, HostScene &hs, std::string &err)
     }
     hs.materials.push_back(m);
     hs.material_names.push_back(name);
-    return true;
-}
-
-bool set_xform(sray_object_desc &o, const XForm &xf, int frame_time, std::string &err)
-{
-    double m[12], inv[12];
-    V3d dlt;
-    int t0, t1;
-    if (xf.linear_motion(dlt, t0, t1)) {
-        xf.matrix(t0, m);
-        o.motion[0] = (float)dlt.x;
-        o.motion[1] = (float)dlt.y;
-        o.motion[2] = (float)dlt.z;
-        o.motion_t0 = t0;
-        o.motion_t1 = t1;
-    } else {
-        xf.matrix(frame_time, m);
-    }
-    if (!mat_inverse(m, inv)) {
-        err = "singular object transform";
-        return false;
-    }
-    mat_to_float(m, o.xform);
-    mat_to_float(inv, o.inv_xform);
     return true;
 }
 
@@ -638,7 +462,7 @@ bool load_mesh(const XmlNode &obj, HostScene &hs, sray_object_desc &o, std::stri
 
 /* load_object + Object::load_xml + Sphere/Cylinder/Mesh::load_xml,
  * src/scene.cc:797-816, src/object.cc:116-135, 34-77 */
-bool load_object(const XmlNode &n, HostScene &hs, int frame_time, std::string &err)
+bool load_object(const XmlNode &n, HostScene &hs, std::string &err)
 {
     const XmlAttr *a = n.get("type");
     if (!a) {
@@ -659,7 +483,7 @@ bool load_object(const XmlNode &n, HostScene &hs, int frame_time, std::string &e
         err = "invalid object: unknown type: " + a->str;
         return false;
     }
-    XForm xf;
+    XFormTracks xf;
     for (const XmlNode &c : n.chld) {
         if (c.name == "matref") {
             const XmlAttr *mn = c.get("name");
@@ -676,7 +500,7 @@ bool load_object(const XmlNode &n, HostScene &hs, int frame_time, std::string &e
             }
             o.material = found;
         } else if (c.name == "xform") {
-            xf.load(c);
+            load_xform(c, xf);
         }
     }
     if (o.type == SRAY_OBJ_SPHERE || o.type == SRAY_OBJ_CYLINDER) {
@@ -699,13 +523,14 @@ bool load_object(const XmlNode &n, HostScene &hs, int frame_time, std::string &e
     } else if (!load_mesh(n, hs, o, err)) {
         return false;
     }
-    if (!set_xform(o, xf, frame_time, err)) return false;
+    /* the matrices come from the tracks (HostScene::set_frame) */
     hs.objects.push_back(o);
+    hs.obj_tracks.push_back(xf);
     return true;
 }
 
 /* load_light + Light/SphLight/BoxLight::load_xml, src/light.cc:39-65,236-255,310-329 */
-bool load_light(const XmlNode &n, HostScene &hs, int frame_time, std::string &err)
+bool load_light(const XmlNode &n, HostScene &hs, std::string &err)
 {
     const XmlAttr *a = n.get("type");
     if (!a) {
@@ -733,7 +558,7 @@ bool load_light(const XmlNode &n, HostScene &hs, int frame_time, std::string &er
     }
     l.radius = 1.0f;
     l.dim[0] = l.dim[1] = l.dim[2] = 1.0f;
-    XForm xf;
+    XFormTracks xf;
     load_xforms(n, xf);
     for (const XmlNode &c : n.chld) {
         if (l.type == SRAY_LIGHT_SPHERE && c.name == "sphere") {
@@ -753,14 +578,9 @@ bool load_light(const XmlNode &n, HostScene &hs, int frame_time, std::string &er
             for (int k = 0; k < 3; k++) l.dim[k] = mx->vval[k] - mn->vval[k];
         }
     }
-    V3d p = xf.position(frame_time);
-    l.pos[0] = (float)p.x;
-    l.pos[1] = (float)p.y;
-    l.pos[2] = (float)p.z;
-    double m[12];
-    xf.matrix(0, m); /* BoxLight::get_point uses get_xform_matrix() at time 0 (Q13) */
-    mat_to_float(m, l.xform);
+    /* position and box matrix come from the tracks (HostScene::set_frame) */
     hs.lights.push_back(l);
+    hs.light_tracks.push_back(xf);
     return true;
 }
 
@@ -785,8 +605,9 @@ void target_camera_matrix(const double pos[3], const double tg[3], double roll, 
     uy = dz * rx - dx * rz;
     uz = dx * ry - dy * rx;
     if (roll != 0.0) {
-        double R[12];
-        quat_to_mat(quat_axis_angle(dx, dy, dz, roll), R);
+        double R[12], q[4];
+        quat_from_axis_angle(dx, dy, dz, roll, q);
+        quat_to_matrix(q, R);
         double nux = R[0] * ux + R[1] * uy + R[2] * uz, nuy = R[4] * ux + R[5] * uy + R[6] * uz,
                nuz = R[8] * ux + R[9] * uy + R[10] * uz;
         double nrx = R[0] * rx + R[1] * ry + R[2] * rz, nry = R[4] * rx + R[5] * ry + R[6] * rz,
@@ -801,7 +622,7 @@ namespace {
 
 /* load_camera + Camera/TargetCamera::load_xml + get_matrix,
  * src/camera.cc:39-67, 161-192, 216-234 */
-bool load_camera(const XmlNode &n, HostScene &hs, int frame_time, std::string &err)
+bool load_camera(const XmlNode &n, HostScene &hs, std::string &err)
 {
     const XmlAttr *a = n.get("type");
     if (!a) {
@@ -817,13 +638,9 @@ bool load_camera(const XmlNode &n, HostScene &hs, int frame_time, std::string &e
     sray_camera_desc c = default_camera();
     if ((a = n.get("shutter"))) c.shutter = (int)a->fval;
     if ((a = n.get("fov")) && (a->type == ATYPE_FLT || a->type == ATYPE_INT)) c.vfov = (double)a->fval * M_PI / 180.0;
-    XForm xf;
+    XFormTracks xf, tx;
     load_xforms(n, xf);
-    double m[12];
-    if (!target) {
-        xf.matrix(frame_time, m);
-    } else {
-        XForm tx;
+    if (target) {
         const XmlNode *tn = nullptr;
         for (const XmlNode &ch : n.chld)
             if (ch.name == "target") {
@@ -835,18 +652,12 @@ bool load_camera(const XmlNode &n, HostScene &hs, int frame_time, std::string &e
             return false;
         }
         load_xforms(*tn, tx);
-        V3d pos = xf.position(frame_time), tg = tx.position(frame_time);
-        double roll = tx.eval_v(tx.scale, frame_time, V3d{0, 0, 0}).x;
-        double P[3] = {pos.x, pos.y, pos.z}, Tg[3] = {tg.x, tg.y, tg.z};
-        target_camera_matrix(P, Tg, roll, m);
-        hs.cam_has_target = true;
-        for (int k = 0; k < 3; k++) {
-            hs.cam_pos[k] = (float)P[k];
-            hs.cam_target[k] = (float)Tg[k];
-        }
     }
-    mat_to_float(m, c.xform);
+    /* the matrix comes from the tracks (HostScene::set_frame) */
     hs.camera = c;
+    hs.cam_tracks = xf;
+    hs.cam_target_tracks = tx;
+    hs.cam_has_target = target;
     return true;
 }
 
@@ -860,7 +671,6 @@ bool load_scene_xml(const std::string &text, HostScene &hs, std::string &err)
     if (!xml_parse(text, root, err)) return false;
     hs = HostScene();
     hs.camera = default_camera();
-    int frame_time = 0;
     for (const XmlNode &n : root.chld)
         if (n.name == "material" && !load_material(n, hs, err)) return false;
     bool have_cam = false;
@@ -886,17 +696,18 @@ bool load_scene_xml(const std::string &text, HostScene &hs, std::string &err)
             }
             if ((a = n.get("ior")) && a->type == ATYPE_FLT) hs.env_ior = a->fval;
         } else if (n.name == "object") {
-            if (!load_object(n, hs, frame_time, err)) return false;
+            if (!load_object(n, hs, err)) return false;
         } else if (n.name == "light") {
-            if (!load_light(n, hs, frame_time, err)) return false;
+            if (!load_light(n, hs, err)) return false;
         } else if (n.name == "camera") {
             HostScene tmp;
-            if (!load_camera(n, hs.camera.max_dist != 0 && have_cam ? tmp : hs, frame_time, err)) return false;
+            if (!load_camera(n, hs.camera.max_dist != 0 && have_cam ? tmp : hs, err)) return false;
             have_cam = true;
         }
     }
-    hs.finish();
-    return true;
+    /* render(0): every transform at msec 0, rows over the frame interval */
+    hs.has_tracks = true;
+    return hs.set_frame(0, err);
 }
 
 /* ================================================================ */
@@ -916,6 +727,26 @@ std::string f9(double v)
     return s;
 }
 std::string vec3(const float *v) { return f9(v[0]) + " " + f9(v[1]) + " " + f9(v[2]); }
+std::string vec3d(const double *v) { return f9(v[0]) + " " + f9(v[1]) + " " + f9(v[2]); }
+
+/* the keys of an XFormNode's tracks as <xform> elements (the values as
+ * they were read, so the scene reads back to the same tracks) */
+void write_tracks(std::ostringstream &o, const XFormTracks &xf, const char *ind)
+{
+    static const char *ip[] = {"step", "linear", "cubic"}, *ep[] = {"clamp", "repeat", "pingpong"};
+    const bool pivot = xf.pivot[0] != 0 || xf.pivot[1] != 0 || xf.pivot[2] != 0;
+    if (xf.interp != SR_INTERP_LINEAR || xf.extrap != SR_EXTRAP_CLAMP || pivot) {
+        o << ind << "<xform interpolate=\"" << ip[xf.interp] << "\" extrapolate=\"" << ep[xf.extrap] << "\"";
+        if (pivot) o << " pivot=\"" << vec3d(xf.pivot) << "\"";
+        o << "/>\n";
+    }
+    for (const TrackKey3 &k : xf.pos) o << ind << "<xform time=\"" << k.time << "\" pos=\"" << vec3d(k.v) << "\"/>\n";
+    for (const TrackKeyQ &k : xf.rot)
+        o << ind << "<xform time=\"" << k.time << "\" " << (k.axisrot ? "axisrot" : "rot") << "=\"" << f9(k.raw[0]) << " "
+          << f9(k.raw[1]) << " " << f9(k.raw[2]) << " " << f9(k.raw[3]) << "\"/>\n";
+    for (const TrackKey3 &k : xf.scale)
+        o << ind << "<xform time=\"" << k.time << "\" scale=\"" << vec3d(k.v) << "\"/>\n";
+}
 } // namespace
 
 std::string write_scene_xml(const HostScene &hs)
@@ -937,7 +768,8 @@ std::string write_scene_xml(const HostScene &hs)
         o << "    <mattr name=\"ior\" value=\"" << f9(m.ior) << "\"/>\n";
         o << "  </material>\n";
     }
-    for (const sray_object_desc &ob : hs.objects) {
+    for (size_t oi = 0; oi < hs.objects.size(); oi++) {
+        const sray_object_desc &ob = hs.objects[oi];
         const char *ty = ob.type == SRAY_OBJ_SPHERE ? "sphere" : ob.type == SRAY_OBJ_CYLINDER ? "cylinder" : "mesh";
         o << "  <object type=\"" << ty << "\">\n";
         if (ob.material >= 0 && ob.material < (int)hs.material_names.size())
@@ -945,7 +777,9 @@ std::string write_scene_xml(const HostScene &hs)
         /* generators only place objects by translation (+ optional motion) */
         float p0[3] = {ob.xform[3], ob.xform[7], ob.xform[11]};
         bool moving = (ob.motion[0] != 0 || ob.motion[1] != 0 || ob.motion[2] != 0) && ob.motion_t1 > ob.motion_t0;
-        if (moving) {
+        if (hs.has_tracks && oi < hs.obj_tracks.size()) {
+            write_tracks(o, hs.obj_tracks[oi], "    ");
+        } else if (moving) {
             float p1[3] = {p0[0] + ob.motion[0], p0[1] + ob.motion[1], p0[2] + ob.motion[2]};
             o << "    <xform time=\"" << ob.motion_t0 << "\" pos=\"" << vec3(p0) << "\"/>\n";
             o << "    <xform time=\"" << ob.motion_t1 << "\" pos=\"" << vec3(p1) << "\"/>\n";
@@ -993,10 +827,12 @@ std::string write_scene_xml(const HostScene &hs)
         }
         o << "  </object>\n";
     }
-    for (const sray_light_desc &l : hs.lights) {
+    for (size_t li = 0; li < hs.lights.size(); li++) {
+        const sray_light_desc &l = hs.lights[li];
         const char *ty = l.type == SRAY_LIGHT_POINT ? "point" : l.type == SRAY_LIGHT_SPHERE ? "sphere" : "box";
         o << "  <light type=\"" << ty << "\" color=\"" << vec3(l.color) << "\">\n";
-        o << "    <xform pos=\"" << vec3(l.pos) << "\"/>\n";
+        if (hs.has_tracks && li < hs.light_tracks.size()) write_tracks(o, hs.light_tracks[li], "    ");
+        else o << "    <xform pos=\"" << vec3(l.pos) << "\"/>\n";
         if (l.type == SRAY_LIGHT_SPHERE) o << "    <sphere rad=\"" << f9(l.radius) << "\"/>\n";
         if (l.type == SRAY_LIGHT_BOX) {
             float mn[3] = {-0.5f * l.dim[0], -0.5f * l.dim[1], -0.5f * l.dim[2]};
@@ -1015,8 +851,20 @@ std::string write_scene_xml(const HostScene &hs)
     bool ident = c.xform[0] == 1 && c.xform[5] == 1 && c.xform[10] == 1 && c.xform[1] == 0 && c.xform[2] == 0 &&
                  c.xform[4] == 0 && c.xform[6] == 0 && c.xform[8] == 0 && c.xform[9] == 0;
     float cp[3] = {c.xform[3], c.xform[7], c.xform[11]};
-    if (ident) {
-        o << "  <camera type=\"free\" fov=\"" << fs << "\">\n    <xform pos=\"" << vec3(cp) << "\"/>\n  </camera>\n";
+    const std::string shut = c.shutter > 0 ? " shutter=\"" + std::to_string(c.shutter) + "\"" : "";
+    if (hs.has_tracks) {
+        o << "  <camera type=\"" << (hs.cam_has_target ? "target" : "free") << "\" fov=\"" << fs << "\"" << shut
+          << ">\n";
+        write_tracks(o, hs.cam_tracks, "    ");
+        if (hs.cam_has_target) {
+            o << "    <target>\n";
+            write_tracks(o, hs.cam_target_tracks, "      ");
+            o << "    </target>\n";
+        }
+        o << "  </camera>\n";
+    } else if (ident) {
+        o << "  <camera type=\"free\" fov=\"" << fs << "\"" << shut << ">\n    <xform pos=\"" << vec3(cp)
+          << "\"/>\n  </camera>\n";
     } else {
         float tg[3] = {cp[0] + c.xform[2], cp[1] + c.xform[6], cp[2] + c.xform[10]};
         if (hs.cam_has_target) {
@@ -1025,7 +873,7 @@ std::string write_scene_xml(const HostScene &hs)
                 tg[k] = hs.cam_target[k];
             }
         }
-        o << "  <camera type=\"target\" fov=\"" << fs << "\">\n    <xform pos=\"" << vec3(cp) << "\"/>\n"
+        o << "  <camera type=\"target\" fov=\"" << fs << "\"" << shut << ">\n    <xform pos=\"" << vec3(cp) << "\"/>\n"
           << "    <target><xform pos=\"" << vec3(tg) << "\"/></target>\n  </camera>\n";
     }
     o << "</scene>\n";

@@ -589,6 +589,12 @@ int validate_desc(sray_gpu_ctx *c, const sray_scene_desc *s)
         uint64_t e = (uint64_t)s->meshes[m].tri_start + s->meshes[m].tri_count;
         if (e > s->num_triangles) return fail(c, SRAY_ERR_ARG, "mesh %u: triangle range out of bounds", m);
     }
+    bool rows = s->camera.time_xforms != nullptr;
+    for (uint32_t i = 0; i < s->num_objects; i++) rows = rows || s->objects[i].time_xforms;
+    for (uint32_t i = 0; i < s->num_lights; i++) rows = rows || s->lights[i].time_pos;
+    if (rows && (s->frame_t1 < s->frame_t0 || (int64_t)s->frame_t1 - s->frame_t0 >= (1 << 20)))
+        return fail(c, SRAY_ERR_ARG, "keyframed rows need a frame interval of 1 .. 2^20 msec (frame_t0 %d, frame_t1 %d)",
+                    s->frame_t0, s->frame_t1);
     return SRAY_OK;
 }
 
@@ -730,6 +736,51 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     SceneDev &D = g->dev;
     memset(&D, 0, sizeof D);
     const void *p;
+    /* keyframed rows (objects, lights, camera) in one buffer, each table
+     * 16-byte aligned: per msec of [frame_t0, frame_t1] */
+    {
+        const int nrow = s->frame_t1 - s->frame_t0 + 1;
+        std::vector<float> tab;
+        std::vector<size_t> oofs(s->num_objects, SIZE_MAX), lofs(s->num_lights, SIZE_MAX);
+        size_t cofs = SIZE_MAX;
+        auto add = [&](const float *src, size_t n) {
+            size_t at = (tab.size() + 3) & ~(size_t)3;
+            tab.resize(at + n);
+            memcpy(&tab[at], src, n * sizeof(float));
+            return at;
+        };
+        for (uint32_t i = 0; i < s->num_objects; i++)
+            if (s->objects[i].time_xforms) oofs[i] = add(s->objects[i].time_xforms, (size_t)nrow * 24);
+        for (uint32_t i = 0; i < s->num_lights; i++)
+            if (s->lights[i].time_pos && s->lights[i].type != SRAY_LIGHT_BOX)
+                lofs[i] = add(s->lights[i].time_pos, (size_t)nrow * 3);
+        if (s->camera.time_xforms) cofs = add(s->camera.time_xforms, (size_t)nrow * 12);
+        if (!tab.empty()) {
+            if ((rc = upload(c, *g, tab.data(), tab.size() * sizeof(float), &p))) return rc;
+            const float *base = (const float *)p;
+            for (uint32_t i = 0; i < s->num_objects; i++)
+                if (oofs[i] != SIZE_MAX) {
+                    objs[i].moving = 2;
+                    objs[i].tab = base + oofs[i];
+                    objs[i].tab_t0 = s->frame_t0;
+                    objs[i].tab_n = nrow;
+                }
+            for (uint32_t i = 0; i < s->num_lights; i++)
+                if (lofs[i] != SIZE_MAX) {
+                    lights[i].tab = base + lofs[i];
+                    lights[i].tab_t0 = s->frame_t0;
+                    lights[i].tab_n = nrow;
+                }
+            if (cofs != SIZE_MAX) {
+                D.cam_tab = base + cofs;
+                D.cam_t0 = s->frame_t0;
+                D.cam_n = nrow;
+            }
+            g->timed = true;
+        }
+        g->frame_t0 = s->frame_t0;
+        g->frame_t1 = s->frame_t1;
+    }
     if ((rc = upload(c, *g, ft.lo.data(), ft.lo.size() * 4, &p))) return rc;
     D.nlo = (const float4 *)p;
     if ((rc = upload(c, *g, ft.hi.data(), ft.hi.size() * 4, &p))) return rc;
@@ -777,6 +828,7 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     g->nobjs = s->num_objects;
     sray::photon_scene_from_desc(s, g->photon);
     g->camera = s->camera;
+    g->camera.time_xforms = nullptr; /* the caller's rows: only D.cam_tab is kept */
     HIPCHK(c, hipStreamSynchronize(c->stream));
     uint32_t id = c->next_scene++;
     c->scenes[id] = std::move(g);
@@ -1022,6 +1074,9 @@ int do_shoot(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon_params *pp, u
     if (!g) return fail(c, SRAY_ERR_NOTFOUND, "unknown scene %u", scene_id);
     if (pp->caust_photons < 0 || pp->gi_photons < 0) return fail(c, SRAY_ERR_ARG, "negative photon count");
     if (pp->t1 < pp->t0) return fail(c, SRAY_ERR_ARG, "photon interval t1 < t0");
+    if (g->timed && (pp->t0 < g->frame_t0 || pp->t1 > g->frame_t1))
+        return fail(c, SRAY_ERR_ARG, "photon interval %d..%d msec leaves the scene's keyframed interval %d..%d",
+                    pp->t0, pp->t1, g->frame_t0, g->frame_t1);
     const sray::PhotonScene &ps = g->photon;
     size_t nl = ps.lights.size();
     if (st) memset(st, 0, sizeof *st);

@@ -207,12 +207,19 @@ __device__ __forceinline__ uint32_t order_mask(uint32_t m, uint32_t dm)
     return m;
 }
 
-/* object matrices at a ray time (linear translation keys) */
+/* object matrices at a ray time: linear translation keys, or the
+ * keyframed row of that msec */
 __device__ __forceinline__ void obj_inv_at(const DevObj &o, int time, float I[12])
 {
 #pragma unroll
     for (int k = 0; k < 12; k++) I[k] = o.inv[k];
     if (!o.moving) return;
+    if (o.moving == 2) {
+        const float *r = time_row(o.tab, o.tab_t0, o.tab_n, time, 24) + 12;
+#pragma unroll
+        for (int k = 0; k < 12; k++) I[k] = r[k];
+        return;
+    }
     float s;
     if (time <= o.mt0) s = 0.0f;
     else if (time >= o.mt1) s = 1.0f;
@@ -228,6 +235,12 @@ __device__ __forceinline__ void obj_xf_at(const DevObj &o, int time, float M[12]
 #pragma unroll
     for (int k = 0; k < 12; k++) M[k] = o.xf[k];
     if (!o.moving) return;
+    if (o.moving == 2) {
+        const float *r = time_row(o.tab, o.tab_t0, o.tab_n, time, 24);
+#pragma unroll
+        for (int k = 0; k < 12; k++) M[k] = r[k];
+        return;
+    }
     float s;
     if (time <= o.mt0) s = 0.0f;
     else if (time >= o.mt1) s = 1.0f;

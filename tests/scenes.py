@@ -280,3 +280,92 @@ def desc_copy(desc_ptr, camera=None, shutter=None):
     if shutter is not None:
         d.camera.shutter = shutter
     return d
+
+
+# ---------------------------------------------------------------- keyframes
+def _mat(name, kd, ks=0.5, shin=30.0, refl=0.0, refr=0.0, ior=1.5):
+    return (f'  <material name="{name}" shader="phong">\n'
+            f'    <mattr name="diffuse" value="{kd[0]} {kd[1]} {kd[2]}"/>\n'
+            f'    <mattr name="specular" value="{ks} {ks} {ks}"/>\n'
+            f'    <mattr name="shininess" value="{shin}"/>\n'
+            f'    <mattr name="reflect" value="{refl}"/>\n'
+            f'    <mattr name="refract" value="{refr}"/>\n'
+            f'    <mattr name="ior" value="{ior}"/>\n  </material>\n')
+
+
+def _pyramid():
+    v = [(-1.0, 0.0, -1.0), (1.0, 0.0, -1.0), (1.0, 0.0, 1.0), (-1.0, 0.0, 1.0), (0.0, 1.5, 0.0)]
+    faces = [(0, 1, 4), (1, 2, 4), (2, 3, 4), (3, 0, 4), (0, 2, 1), (0, 3, 2)]
+    s = "    <mesh>\n"
+    for i, p in enumerate(v):
+        s += f'      <vertex id="{i}" val="{p[0]} {p[1]} {p[2]}"/>\n'
+    for f in faces:
+        s += "      <face>" + "".join(f'<vref vertex="{k}"/>' for k in f) + "</face>\n"
+    return s + "    </mesh>\n"
+
+
+def animated_scene_xml(shutter=40):
+    """a scene whose transforms are keyframed: a pyramid turning 90 degrees
+    about y while it scales, a sphere scaled into an ellipsoid with a
+    repeating position track, a cylinder turning about a pivot (quaternion
+    keys), a step-interpolated sphere, a cubic position track, a moving
+    point light and a target camera whose position and target move"""
+    x = "<scene>\n"
+    x += '  <env ambient="0.05 0.05 0.05" bgcolor="0.1 0.1 0.2" ior="1.0"/>\n'
+    x += _mat("red", (0.8, 0.2, 0.2)) + _mat("green", (0.2, 0.8, 0.3), refl=0.3)
+    x += _mat("blue", (0.2, 0.3, 0.9), ks=1.0, shin=60.0) + _mat("glass", (0.0, 0.0, 0.0), ks=1.0, shin=80.0,
+                                                                 refl=0.1, refr=0.9)
+    x += _mat("grey", (0.7, 0.7, 0.7), ks=0.0)
+    # a sphere's normal is transformed back without normalising
+    # (src/sphere.cc:127-129): under a non-uniform scale a specular term
+    # raises a reflected vector longer than 1 to the shininess, so the
+    # ellipsoid below is diffuse only
+    x += _mat("matte", (0.3, 0.4, 0.9), ks=0.0)
+    # floor
+    x += ('  <object type="mesh">\n    <matref name="grey"/>\n    <mesh>\n'
+          '      <vertex id="0" val="-8.0 -1.0 -8.0"/><vertex id="1" val="8.0 -1.0 -8.0"/>\n'
+          '      <vertex id="2" val="8.0 -1.0 8.0"/><vertex id="3" val="-8.0 -1.0 8.0"/>\n'
+          '      <face><vref vertex="0"/><vref vertex="2"/><vref vertex="1"/></face>\n'
+          '      <face><vref vertex="0"/><vref vertex="3"/><vref vertex="2"/></face>\n'
+          '    </mesh>\n  </object>\n')
+    x += ('  <object type="mesh">\n    <matref name="red"/>\n'
+          '    <xform time="0" pos="-2.5 -1.0 1.0" axisrot="0.0 0.0 1.0 0.0" scale="1.0 1.0 1.0"/>\n'
+          '    <xform time="40" axisrot="90.0 0.0 1.0 0.0" scale="1.5 0.6 1.0"/>\n' + _pyramid() + '  </object>\n')
+    x += ('  <object type="sphere">\n    <matref name="matte"/>\n'
+          '    <xform extrapolate="repeat" time="0" pos="1.5 0.2 0.0" scale="1.0 1.0 1.0"/>\n'
+          '    <xform time="30" pos="1.5 1.2 0.5" scale="1.6 0.7 1.0"/>\n'
+          '    <sphere rad="0.9"/>\n  </object>\n')
+    x += ('  <object type="cylinder">\n    <matref name="green"/>\n'
+          '    <xform pivot="0.5 0.0 0.0" time="0" pos="0.0 -1.0 3.0" rot="1.0 0.0 0.0 0.0"/>\n'
+          '    <xform time="40" rot="0.9238795 0.0 0.0 0.3826834"/>\n'
+          '    <cylinder rad="0.5" start="0.0" end="2.0"/>\n  </object>\n')
+    x += ('  <object type="sphere">\n    <matref name="glass"/>\n'
+          '    <xform interpolate="step" time="0" pos="-0.5 0.0 -1.5"/>\n'
+          '    <xform time="20" pos="0.0 0.3 -1.5"/>\n'
+          '    <sphere rad="0.6"/>\n  </object>\n')
+    x += ('  <object type="sphere">\n    <matref name="red"/>\n'
+          '    <xform interpolate="cubic" time="0" pos="3.0 2.0 2.0"/>\n    <xform time="10" pos="3.5 2.5 2.0"/>\n'
+          '    <xform time="25" pos="3.0 3.0 2.5"/>\n    <xform time="40" pos="2.5 2.0 2.0"/>\n'
+          '    <sphere rad="0.4"/>\n  </object>\n')
+    x += ('  <light type="point" color="1.0 1.0 1.0">\n'
+          '    <xform time="0" pos="4.0 8.0 -6.0"/>\n    <xform time="40" pos="-4.0 9.0 -5.0"/>\n  </light>\n')
+    x += (f'  <camera type="target" fov="50.0" shutter="{shutter}">\n'
+          '    <xform time="0" pos="0.0 2.5 -9.0"/>\n    <xform time="40" pos="1.0 3.0 -8.5"/>\n'
+          '    <target><xform time="0" pos="0.0 0.0 0.0"/><xform time="40" pos="0.3 0.2 0.5"/></target>\n'
+          '  </camera>\n')
+    return x + "</scene>\n"
+
+
+def rows_of(desc, what, index=0):
+    """the per-msec rows of an object (24 floats), light (3) or the camera
+    (12) as an array, or None"""
+    n = desc.frame_t1 - desc.frame_t0 + 1
+    if what == "object":
+        p, w = desc.objects[index].time_xforms, 24
+    elif what == "light":
+        p, w = desc.lights[index].time_pos, 3
+    else:
+        p, w = desc.camera.time_xforms, 12
+    if not p:
+        return None
+    return np.ctypeslib.as_array(p, shape=(n * w,)).reshape(n, w).copy()
