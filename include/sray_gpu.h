@@ -60,6 +60,15 @@ extern "C" {
 #define SRAY_MAT_HAS_REFLECT   0x2u
 #define SRAY_MAT_HAS_REFRACT   0x4u
 #define SRAY_MAT_PHONG         0x8u   /* shader="phong"; otherwise shade_undef */
+#define SRAY_MAT_HAS_NORMAL    0x10u  /* a "normal" attribute (bump map when mapped) */
+
+/* texture maps of material attributes (MatAttrib::tex, src/material.cc:64-72) */
+enum { SRAY_MAP_DIFFUSE, SRAY_MAP_SPECULAR, SRAY_MAP_SHININESS, SRAY_MAP_REFLECT, SRAY_MAP_REFRACT,
+       SRAY_MAP_IOR, SRAY_MAP_NORMAL, SRAY_NUM_MAPS };
+#define SRAY_TEX_FILTER_LINEAR  0     /* 5-tap cross average (Texture(), src/texture.cc:21-25) */
+#define SRAY_TEX_FILTER_NEAREST 1
+#define SRAY_TEX_WRAP_REPEAT    0
+#define SRAY_TEX_WRAP_CLAMP     1
 
 /* ------------------------------------------------------------------ */
 /* flat scene description (host side, what Scene::load produces)       */
@@ -100,8 +109,22 @@ typedef struct sray_material_desc {
     float    refract;
     float    ior;
     uint32_t flags;        /* SRAY_MAT_* */
-    uint32_t pad_;
+    float    normal;       /* the "normal" attribute's value (bump scale) */
+    /* texture of each attribute (SRAY_MAP_*): 1 + index into the scene's
+     * textures, 0 = none (a zeroed descriptor has no maps).  The attribute
+     * is texture x value (MatAttrib::get_color, src/material.inl:23-28). */
+    int32_t  map[SRAY_NUM_MAPS];
+    int32_t  pad_;
 } sray_material_desc;
+
+typedef struct sray_texture_desc {
+    int32_t      width, height;
+    const float *texels;   /* width * height RGBA floats, row-major, top row first */
+    int32_t      filter;   /* SRAY_TEX_FILTER_* */
+    int32_t      wrap;     /* SRAY_TEX_WRAP_* */
+    float        xform[12];/* texture-coordinate matrix (a Texture is an XFormNode;
+                              identity as loaded from a scene) */
+} sray_texture_desc;
 
 typedef struct sray_light_desc {
     int32_t type;          /* SRAY_LIGHT_* */
@@ -151,6 +174,8 @@ typedef struct sray_scene_desc {
     int32_t                   mesh_oct_depth;   /* -moctdepth (default 7) */
     int32_t                   mesh_oct_items;   /* -moctitems (default 14) */
     int32_t                   frame_t0, frame_t1; /* build_tree(t0, t1) interval */
+    uint32_t                  num_textures;
+    const sray_texture_desc  *textures;
 } sray_scene_desc;
 
 /* ------------------------------------------------------------------ */
@@ -460,8 +485,13 @@ int  sray_gpu_set_admission_cap(sray_gpu_ctx *ctx, uint64_t bytes);
 typedef struct sray_host_scene sray_host_scene;
 
 /* Replaces Scene::load(const char*) / load(FILE*) (src/scene.cc:75-183):
- * parses reference-format scene XML into a flat description owned by the
- * returned handle. */
+ * parses reference-format scene XML (from path, or xml_text when given)
+ * into a flat description owned by the returned handle.  A material
+ * attribute's "map" file is looked for as find_file does
+ * (src/datapath.c:56-77): in the directory of path ("." without one),
+ * then in each entry of SRAY_DATA_PATH; an absolute name as it is.  A map
+ * that cannot be found or read is reported on stderr and the attribute
+ * keeps its plain value (src/material.cc:64-71). */
 int  sray_scene_load_xml(const char *path, const char *xml_text, sray_host_scene **out);
 const sray_scene_desc *sray_scene_desc_of(const sray_host_scene *hs);
 /* Replaces the per-frame part of render(msec) (src/render.cc:34-41) and the
@@ -538,6 +568,15 @@ int  sray_image_write_png(const char *path, const float *rgba, int width, int he
  * same), 'd' done (milliseconds, ...). */
 #define SRAY_STATUS_RECORD 32
 int  sray_status_record(char st, int a1, int a2, int a3, int a4, char *out);
+/* Replaces Image::load (src/img.cc:64-78, which reads through libimago
+ * as RGBA float): a PNG (every colour type and depth, tRNS transparency;
+ * not interlaced), PPM/PGM (P2, P3, P5, P6; maxval up to 65535) or PFM
+ * ("PF"/"Pf", either byte order) file, told apart by its first bytes, as
+ * RGBA floats, row-major, top row first.  An integer channel reads
+ * value / maxval (8-bit: / 255); a file without alpha reads alpha 1.
+ * *rgba is freed with sray_image_free. */
+int  sray_image_read(const char *path, float **rgba, int *width, int *height);
+void sray_image_free(float *rgba);
 /* text of the last failure of the I/O calls above (per thread) */
 const char *sray_io_last_error(void);
 /* Status stream of a context (fd < 0: off, the default).  A render writes

@@ -66,6 +66,8 @@ def lib(fast=False):
         L.oracle_projmap.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, vp,
                                      C.POINTER(C.c_double)]
         L.oracle_surface.argtypes = [vp, vp, C.c_uint32, vp]
+        L.oracle_shade_normal.argtypes = [vp, vp, C.c_uint32, vp]
+        L.oracle_tex_lookup.argtypes = [vp, C.c_int, vp, C.c_uint32, vp]
         L.oracle_light_points.argtypes = [vp, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, vp]
         L.oracle_cell_from_dir.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]
@@ -163,6 +165,23 @@ class OracleScene:
         out = np.zeros((len(rays), 8), np.float32)
         if self.L.oracle_surface(self.h, rays.ctypes.data, len(rays), out.ctypes.data):
             raise RuntimeError("oracle_surface failed")
+        return out
+
+    def shade_normal(self, rays):
+        """(n, 6) float32: the shading normal at each closest hit (facing
+        the ray, or the material's "normal" attribute's) and the tangent"""
+        rays = np.ascontiguousarray(rays)
+        out = np.zeros((len(rays), 6), np.float32)
+        if self.L.oracle_shade_normal(self.h, rays.ctypes.data, len(rays), out.ctypes.data):
+            raise RuntimeError("oracle_shade_normal failed")
+        return out
+
+    def tex_lookup(self, tex, uv):
+        """(n, 4) float32: texture tex of the scene looked up at (n, 2) (u, v)"""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((len(uv), 4), np.float32)
+        if self.L.oracle_tex_lookup(self.h, tex, uv.ctypes.data, len(uv), out.ctypes.data):
+            raise RuntimeError("oracle_tex_lookup failed")
         return out
 
     def diag(self):

@@ -411,6 +411,15 @@ template <class T> struct Material {
     C4<T> kd, ks;
     T shininess, reflect, refract, ior;
     uint32_t flags;
+    int map[SRAY_NUM_MAPS] = {0, 0, 0, 0, 0, 0, 0}; /* 1 + texture index, 0 = none */
+    T nscale = 0;                                  /* the "normal" attribute's value */
+};
+
+/* Texture (src/texture.h:37-60): RGBA float texels, top row first */
+struct Texture {
+    std::vector<float> px;
+    int w = 0, h = 0, filter = SRAY_TEX_FILTER_LINEAR, wrap = SRAY_TEX_WRAP_REPEAT;
+    float xf[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
 };
 
 struct Photon {
@@ -441,6 +450,7 @@ template <class T> struct Scene {
     std::vector<MeshT<T>> meshes;
     std::vector<Tri<T>> tris;
     std::vector<Material<T>> mats;
+    std::vector<Texture> texs;
     std::vector<Light<T>> lights;
     Mat<T> cam_xf;
     double vfov;
@@ -1140,6 +1150,80 @@ template <class T> struct Scene {
         }
     }
 
+    /* Texture::get_texel, src/texture.cc:85-101: repeat wraps with C's
+     * remainder (negatives moved up by the size); clamp holds the index to
+     * [0, size] — the size itself included, one texel past the row — and the
+     * row-major index is kept, where one past the last texel reads the last */
+    static C4<T> texel(const Texture &tx, int x, int y)
+    {
+        if (tx.wrap == SRAY_TEX_WRAP_CLAMP) {
+            x = x < 0 ? 0 : (x > tx.w ? tx.w : x);
+            y = y < 0 ? 0 : (y > tx.h ? tx.h : y);
+        } else {
+            x %= tx.w;
+            y %= tx.h;
+            if (x < 0) x += tx.w;
+            if (y < 0) y += tx.h;
+        }
+        long long k = (long long)y * tx.w + x;
+        const long long last = (long long)tx.w * tx.h - 1;
+        if (k > last) k = last;
+        const float *p = &tx.px[(size_t)k * 4];
+        return c4<T>(p[0], p[1], p[2], p[3]);
+    }
+    /* (int)(c * size), C truncation, held to +-2^30 texels */
+    static int tex_coord(T c, int size)
+    {
+        T f = c * (T)size;
+        f = std::max(std::min(f, (T)1073741824.0), (T)-1073741824.0);
+        return (int)f;
+    }
+    /* Texture::lookup, src/texture.cc:60-83: (u, v, 0, 1) through the
+     * texture matrix, then one texel or the 5-texel cross averaged */
+    static C4<T> tex_lookup(const Texture &tx, T u, T v)
+    {
+        const T x = (T)tx.xf[0] * u + (T)tx.xf[1] * v + (T)tx.xf[2] * (T)0 + (T)tx.xf[3];
+        const T y = (T)tx.xf[4] * u + (T)tx.xf[5] * v + (T)tx.xf[6] * (T)0 + (T)tx.xf[7];
+        const int ix = tex_coord(x, tx.w), iy = tex_coord(y, tx.h);
+        if (tx.filter == SRAY_TEX_FILTER_NEAREST) return texel(tx, ix, iy);
+        C4<T> a = texel(tx, ix, iy), b = texel(tx, ix + 1, iy), c = texel(tx, ix - 1, iy), d = texel(tx, ix, iy + 1),
+              e = texel(tx, ix, iy - 1);
+        C4<T> r;
+        r.x = ((((a.x + b.x) + c.x) + d.x) + e.x) / (T)5;
+        r.y = ((((a.y + b.y) + c.y) + d.y) + e.y) / (T)5;
+        r.z = ((((a.z + b.z) + c.z) + d.z) + e.z) / (T)5;
+        r.w = ((((a.w + b.w) + c.w) + d.w) + e.w) / (T)5;
+        return r;
+    }
+    /* MatAttrib::get_color / get_value, src/material.inl:18-28 */
+    C4<T> attr_color(int map, const C4<T> &col, const SurfPoint<T> &sp) const
+    {
+        if (map <= 0) return col;
+        C4<T> t = tex_lookup(texs[map - 1], sp.u, sp.v);
+        return c4<T>(t.x * col.x, t.y * col.y, t.z * col.z, t.w * col.w);
+    }
+    T attr_value(int map, T val, const SurfPoint<T> &sp) const
+    {
+        if (map <= 0) return val;
+        return tex_lookup(texs[map - 1], sp.u, sp.v).x * val;
+    }
+    /* get_bump_normal, src/shader.cc:293-316: without a map the unflipped
+     * surface normal; with one, the map's colour * 2 - 1 (x, y scaled by
+     * the attribute's value), normalised, in the frame (tangent,
+     * normal x tangent, normal) */
+    V3<T> bump_normal(const Material<T> &m, const SurfPoint<T> &sp) const
+    {
+        if (m.map[SRAY_MAP_NORMAL] <= 0) return sp.normal;
+        C4<T> c = tex_lookup(texs[m.map[SRAY_MAP_NORMAL] - 1], sp.u, sp.v);
+        V3<T> n = v3<T>(c.x * (T)2 - (T)1, c.y * (T)2 - (T)1, c.z * (T)2 - (T)1);
+        n.x = n.x * m.nscale;
+        n.y = n.y * m.nscale;
+        n = normalized(n);
+        V3<T> b = cross(sp.normal, sp.tangent);
+        return v3<T>(sp.tangent.x * n.x + b.x * n.y + sp.normal.x * n.z, sp.tangent.y * n.x + b.y * n.y + sp.normal.y * n.z,
+                     sp.tangent.z * n.x + b.z * n.y + sp.normal.z * n.z);
+    }
+
     static T fresnel(T r, T cosa)
     {
         /* src/shader.cc:285-291 */
@@ -1197,19 +1281,20 @@ template <class T> struct Scene {
 
         C4<T> ks = c4<T>(0, 0, 0, 1);
         T refl_fact = 0, trans_fact = 0;
-        C4<T> kd = mat.kd;
-        if (mat.flags & SRAY_MAT_HAS_SPECULAR) ks = mat.ks;
-        T shininess = mat.shininess;
+        C4<T> kd = attr_color(mat.map[SRAY_MAP_DIFFUSE], mat.kd, sp);
+        if (mat.flags & SRAY_MAT_HAS_SPECULAR) ks = attr_color(mat.map[SRAY_MAP_SPECULAR], mat.ks, sp);
+        T shininess = attr_value(mat.map[SRAY_MAP_SHININESS], mat.shininess, sp);
 
         if (ray.iter == INT_MAX) {
             /* diffuse (GI) ray: radiance estimate from the global map */
             C4<T> rad = radiance(sp.pos, normal, gather_dist_abs, 0);
             return rad * kd;
         }
-        if (mat.flags & SRAY_MAT_HAS_REFLECT) refl_fact = mat.reflect;
-        if (mat.flags & SRAY_MAT_HAS_REFRACT) trans_fact = mat.refract;
-        T mat_ior = mat.ior;
+        if (mat.flags & SRAY_MAT_HAS_REFLECT) refl_fact = attr_value(mat.map[SRAY_MAP_REFLECT], mat.reflect, sp);
+        if (mat.flags & SRAY_MAT_HAS_REFRACT) trans_fact = attr_value(mat.map[SRAY_MAP_REFRACT], mat.refract, sp);
+        T mat_ior = attr_value(mat.map[SRAY_MAP_IOR], mat.ior, sp);
         T ior = ray.calc_ior(entering, mat_ior);
+        if (mat.flags & SRAY_MAT_HAS_NORMAL) normal = bump_normal(mat, sp);
 
         C4<T> diff = env_ambient * kd;
         C4<T> spec = c4<T>(0, 0, 0, 1);
@@ -1523,7 +1608,14 @@ template <class T> struct PhotonTracer {
                                                 (double)b.mx.x, (double)b.mx.y, (double)b.mx.z};
                     all.push_back(bb);
                     int m = S.objs[i].mat;
-                    bool specobj = m >= 0 && (attr_reflect(S.mats[m]) > 0 || attr_refract(S.mats[m]) > 0);
+                    /* the attributes' colours, without their maps (src/light.cc:173-176) */
+                    bool specobj = false;
+                    if (m >= 0) {
+                        const Material<T> &mm = S.mats[m];
+                        T rl = (mm.flags & SRAY_MAT_HAS_REFLECT) ? mm.reflect : (T)1;
+                        T rr = (mm.flags & SRAY_MAT_HAS_REFRACT) ? mm.refract : (T)1;
+                        specobj = rl > (T)0 || rr > (T)0;
+                    }
                     if (specobj) caust.push_back(bb);
                 }
                 for (int sm = 0; sm < samples; sm++) {
@@ -1583,12 +1675,18 @@ template <class T> struct PhotonTracer {
 
     /* Material::get_color / get_value return the default attribute
      * (1,1,1,1) for a missing attribute (src/material.cc:77,164-168) */
-    static C4<T> attr_specular(const Material<T> &m)
+    C4<T> attr_specular(const Material<T> &m, const SurfPoint<T> &sp) const
     {
-        return (m.flags & SRAY_MAT_HAS_SPECULAR) ? m.ks : c4<T>(1, 1, 1, 1);
+        return (m.flags & SRAY_MAT_HAS_SPECULAR) ? S.attr_color(m.map[SRAY_MAP_SPECULAR], m.ks, sp) : c4<T>(1, 1, 1, 1);
     }
-    static T attr_reflect(const Material<T> &m) { return (m.flags & SRAY_MAT_HAS_REFLECT) ? m.reflect : (T)1; }
-    static T attr_refract(const Material<T> &m) { return (m.flags & SRAY_MAT_HAS_REFRACT) ? m.refract : (T)1; }
+    T attr_reflect(const Material<T> &m, const SurfPoint<T> &sp) const
+    {
+        return (m.flags & SRAY_MAT_HAS_REFLECT) ? S.attr_value(m.map[SRAY_MAP_REFLECT], m.reflect, sp) : (T)1;
+    }
+    T attr_refract(const Material<T> &m, const SurfPoint<T> &sp) const
+    {
+        return (m.flags & SRAY_MAT_HAS_REFRACT) ? S.attr_value(m.map[SRAY_MAP_REFRACT], m.refract, sp) : (T)1;
+    }
 
     /* reflect_ray / refract_ray (libvmath, fixed here as in shade_phong) */
     static Ray<T> reflect_ray(const Ray<T> &r, V3<T> n)
@@ -1619,9 +1717,10 @@ template <class T> struct PhotonTracer {
             Hit h;
             if (!cast_hit(r, h)) return false;
             const Material<T> &m = *h.mat;
-            T refl = attr_reflect(m), refr = attr_refract(m), mat_ior = m.ior;
-            C4<T> spc = attr_specular(m);
+            T refl = attr_reflect(m, h.sp), refr = attr_refract(m, h.sp), mat_ior = S.attr_value(m.map[SRAY_MAP_IOR], m.ior, h.sp);
+            C4<T> spc = attr_specular(m, h.sp);
             T ior = r.calc_ior(h.entering, mat_ior);
+            if (m.flags & SRAY_MAT_HAS_NORMAL) h.normal = S.bump_normal(m, h.sp); /* src/scene.cc:603-606 */
             T ray_dot_n = dot(-r.d / h.ray_mag, h.normal);
             T sf0 = refr > (T)0 ? (ior - (T)1) / (ior + (T)1) : (T)1;
             T fres = Scene<T>::fresnel(sf0 * sf0, ray_dot_n);
@@ -1661,7 +1760,7 @@ template <class T> struct PhotonTracer {
             Hit h;
             if (!cast_hit(r, h)) return false;
             const Material<T> &m = *h.mat;
-            C4<T> diff = m.kd, spc = attr_specular(m);
+            C4<T> diff = S.attr_color(m.map[SRAY_MAP_DIFFUSE], m.kd, h.sp), spc = attr_specular(m, h.sp);
             T diff_avg = (diff.x + diff.y + diff.z) / (T)3;
             T spec_avg = (spc.x + spc.y + spc.z) / (T)3;
             T rnum = (T)prand(pp->seed, a, b, bounce, P_RR, 0, 0);
@@ -1685,7 +1784,8 @@ template <class T> struct PhotonTracer {
                 ph.col[2] *= ndotl * (diff.z / diff_avg);
                 r = nr;
             } else if (rnum < diff_avg + spec_avg) {
-                T refl = attr_reflect(m), refr = attr_refract(m), mat_ior = m.ior;
+                T refl = attr_reflect(m, h.sp), refr = attr_refract(m, h.sp),
+                  mat_ior = S.attr_value(m.map[SRAY_MAP_IOR], m.ior, h.sp);
                 T ior = r.calc_ior(h.entering, mat_ior);
                 T ray_dot_n = dot(-r.d / h.ray_mag, h.normal);
                 T sf0 = refr > (T)0 ? (ior - (T)1) / (ior + (T)1) : (T)1;
@@ -1862,6 +1962,19 @@ template <class T> static void load(Scene<T> &S, const sray_scene_desc *d, uint3
         m.refract = md.refract;
         m.ior = md.ior;
         m.flags = md.flags;
+        for (int k = 0; k < SRAY_NUM_MAPS; k++) m.map[k] = md.map[k];
+        m.nscale = md.normal;
+    }
+    S.texs.resize(d->num_textures);
+    for (uint32_t i = 0; i < d->num_textures; i++) {
+        const sray_texture_desc &td = d->textures[i];
+        Texture &t = S.texs[i];
+        t.w = td.width;
+        t.h = td.height;
+        t.filter = td.filter;
+        t.wrap = td.wrap;
+        std::memcpy(t.xf, td.xform, sizeof t.xf);
+        t.px.assign(td.texels, td.texels + (size_t)td.width * td.height * 4);
     }
     S.lights.resize(d->num_lights);
     for (uint32_t i = 0; i < d->num_lights; i++) {
@@ -2058,6 +2171,44 @@ template <class T> static int do_surface(Scene<T> &S, const sray_ray *rays, uint
         o[0] = (float)sp.pos.x, o[1] = (float)sp.pos.y, o[2] = (float)sp.pos.z;
         o[3] = (float)sp.normal.x, o[4] = (float)sp.normal.y, o[5] = (float)sp.normal.z;
         o[6] = (float)sp.u, o[7] = (float)sp.v;
+    }
+    return 0;
+}
+
+/* the shading normal shade_phong uses at each closest hit (the normal
+ * facing the ray, then the "normal" attribute's, src/shader.cc:88-96,
+ * 137-138) and the surface tangent: 6 floats, NaN on a miss */
+template <class T> static int do_shade_normal(Scene<T> &S, const sray_ray *rays, uint32_t n, float *out)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        Ray<T> r;
+        r.o = v3<T>(rays[i].origin[0], rays[i].origin[1], rays[i].origin[2]);
+        r.d = v3<T>(rays[i].dir[0], rays[i].dir[1], rays[i].dir[2]);
+        r.time = rays[i].time;
+        Counters cnt;
+        Best<T> b = S.cast(r, cnt, false);
+        float *o = out + (size_t)i * 6;
+        if (!b.found()) {
+            for (int k = 0; k < 6; k++) o[k] = NAN;
+            continue;
+        }
+        SurfPoint<T> sp = S.surf(r, (int)b.item, b.sub, b.t);
+        V3<T> nrm = dot(r.d, sp.normal) > (T)0 ? -sp.normal : sp.normal;
+        const int m = S.objs[b.item].mat;
+        if (m >= 0 && (S.mats[m].flags & SRAY_MAT_HAS_NORMAL)) nrm = S.bump_normal(S.mats[m], sp);
+        o[0] = (float)nrm.x, o[1] = (float)nrm.y, o[2] = (float)nrm.z;
+        o[3] = (float)sp.tangent.x, o[4] = (float)sp.tangent.y, o[5] = (float)sp.tangent.z;
+    }
+    return 0;
+}
+
+/* Texture::lookup of texture tex at n (u, v) pairs: RGBA */
+template <class T> static int do_tex_lookup(Scene<T> &S, int tex, const float *uv, uint32_t n, float *out)
+{
+    if (tex < 0 || tex >= (int)S.texs.size()) return -1;
+    for (uint32_t i = 0; i < n; i++) {
+        C4<T> c = Scene<T>::tex_lookup(S.texs[tex], (T)uv[2 * i], (T)uv[2 * i + 1]);
+        out[4 * i] = (float)c.x, out[4 * i + 1] = (float)c.y, out[4 * i + 2] = (float)c.z, out[4 * i + 3] = (float)c.w;
     }
     return 0;
 }
@@ -2385,6 +2536,20 @@ int oracle_surface(oracle_scene *s, const sray_ray *rays, uint32_t n, float *out
     if (!s || (n && (!rays || !out))) return -1;
     if (s->s.prec == 32) return do_surface(*s->s.f, rays, n, out);
     return do_surface(*s->s.d, rays, n, out);
+}
+
+int oracle_shade_normal(oracle_scene *s, const sray_ray *rays, uint32_t n, float *out)
+{
+    if (!s || (n && (!rays || !out))) return -1;
+    if (s->s.prec == 32) return do_shade_normal(*s->s.f, rays, n, out);
+    return do_shade_normal(*s->s.d, rays, n, out);
+}
+
+int oracle_tex_lookup(oracle_scene *s, int tex, const float *uv, uint32_t n, float *out)
+{
+    if (!s || (n && (!uv || !out))) return -1;
+    if (s->s.prec == 32) return do_tex_lookup(*s->s.f, tex, uv, n, out);
+    return do_tex_lookup(*s->s.d, tex, uv, n, out);
 }
 
 double oracle_scene_diag(oracle_scene *s)

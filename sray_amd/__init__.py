@@ -239,6 +239,17 @@ def write_png(path, rgba):
     _io_check(lib().sray_image_write_png(os.fsencode(path), img.ctypes.data_as(C.POINTER(C.c_float)), w, h))
 
 
+def read_image(path):
+    """Image::load (src/img.cc:64-78): a PNG, PPM/PGM or PFM file as an
+    (h, w, 4) float32 RGBA array, top row first"""
+    p, w, h = C.POINTER(C.c_float)(), C.c_int(), C.c_int()
+    _io_check(lib().sray_image_read(os.fsencode(path), C.byref(p), C.byref(w), C.byref(h)))
+    try:
+        return np.ctypeslib.as_array(p, shape=(h.value, w.value, 4)).copy()
+    finally:
+        lib().sray_image_free(p)
+
+
 def status_record(st, a1, a2, a3, a4):
     """one record of the backend status protocol (src/render.cc:279-289)"""
     buf = C.create_string_buffer(abi.SRAY_STATUS_RECORD)

@@ -14,7 +14,10 @@ SRAY_ERR_NODEVICE = -8
 
 SRAY_OBJ_SPHERE, SRAY_OBJ_CYLINDER, SRAY_OBJ_MESH = 0, 1, 2
 SRAY_LIGHT_POINT, SRAY_LIGHT_SPHERE, SRAY_LIGHT_BOX = 0, 1, 2
-SRAY_MAT_HAS_SPECULAR, SRAY_MAT_HAS_REFLECT, SRAY_MAT_HAS_REFRACT, SRAY_MAT_PHONG = 1, 2, 4, 8
+SRAY_MAT_HAS_SPECULAR, SRAY_MAT_HAS_REFLECT, SRAY_MAT_HAS_REFRACT, SRAY_MAT_PHONG, SRAY_MAT_HAS_NORMAL = 1, 2, 4, 8, 16
+(SRAY_MAP_DIFFUSE, SRAY_MAP_SPECULAR, SRAY_MAP_SHININESS, SRAY_MAP_REFLECT, SRAY_MAP_REFRACT, SRAY_MAP_IOR,
+ SRAY_MAP_NORMAL) = range(7)
+SRAY_TEX_FILTER_LINEAR, SRAY_TEX_FILTER_NEAREST, SRAY_TEX_WRAP_REPEAT, SRAY_TEX_WRAP_CLAMP = 0, 1, 0, 1
 SRAY_TRACE_CLOSEST, SRAY_TRACE_ANY, SRAY_TRACE_COUNT, SRAY_TRACE_WAVE, SRAY_TRACE_SPARSE = 0, 1, 2, 4, 8
 SRAY_RENDER_REF_Q1 = 1
 SRAY_NOALLOC_MAX_QUERIES = 1 << 20
@@ -43,7 +46,12 @@ class MeshDesc(C.Structure):
 
 class MaterialDesc(C.Structure):
     _fields_ = [("diffuse", f32 * 4), ("specular", f32 * 4), ("shininess", f32), ("reflect", f32),
-                ("refract", f32), ("ior", f32), ("flags", u32), ("pad_", u32)]
+                ("refract", f32), ("ior", f32), ("flags", u32), ("normal", f32), ("map", i32 * 7), ("pad_", i32)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", i32), ("height", i32), ("texels", C.POINTER(f32)), ("filter", i32), ("wrap", i32),
+                ("xform", f32 * 12)]
 
 
 class LightDesc(C.Structure):
@@ -65,7 +73,8 @@ class SceneDesc(C.Structure):
                 ("num_lights", u32), ("lights", C.POINTER(LightDesc)),
                 ("camera", CameraDesc), ("env_ambient", f32 * 4), ("env_color", f32 * 4), ("env_ior", f32),
                 ("scene_oct_depth", i32), ("scene_oct_items", i32), ("mesh_oct_depth", i32),
-                ("mesh_oct_items", i32), ("frame_t0", i32), ("frame_t1", i32)]
+                ("mesh_oct_items", i32), ("frame_t0", i32), ("frame_t1", i32),
+                ("num_textures", u32), ("textures", C.POINTER(TextureDesc))]
 
 
 class Ray(C.Structure):
@@ -140,7 +149,7 @@ class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", u64), ("total_ms", f64)]
 
 
-ABI_STRUCTS = [ObjectDesc, MeshDesc, MaterialDesc, LightDesc, CameraDesc, SceneDesc, Ray, Hit, TraceCounts,
+ABI_STRUCTS = [ObjectDesc, MeshDesc, MaterialDesc, TextureDesc, LightDesc, CameraDesc, SceneDesc, Ray, Hit, TraceCounts,
                RenderParams, Stats, Photon, GatherQuery, Request, Result, OctreeInfo, KernelStat, PhotonParams,
                PhotonStats, Tuning]
 
@@ -159,7 +168,7 @@ EXPORTS = [
     "sray_photon_dump", "sray_photon_restore", "sray_photon_free", "sray_image_write_pfm", "sray_image_write_png",
     "sray_status_record",
     "sray_io_last_error", "sray_gpu_set_status_fd", "sray_gpu_pmap_gather_device_noalloc",
-    "sray_scene_set_frame",
+    "sray_scene_set_frame", "sray_image_read", "sray_image_free",
 ]
 SRAY_STATUS_RECORD = 32
 
@@ -220,6 +229,8 @@ def declare(lib):
         "sray_gpu_set_status_fd": (i32, [vp, C.c_int]),
         "sray_gpu_pmap_gather_device_noalloc": (i32, [vp, u32, vp, u32, f32, C.c_int, vp]),
         "sray_scene_set_frame": (i32, [vp, i32]),
+        "sray_image_read": (i32, [C.c_char_p, P(P(f32)), P(C.c_int), P(C.c_int)]),
+        "sray_image_free": (None, [P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

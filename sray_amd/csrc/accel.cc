@@ -391,6 +391,9 @@ const sray_scene_desc *HostScene::finish()
     desc.materials = materials.data();
     desc.num_lights = (uint32_t)lights.size();
     desc.lights = lights.data();
+    for (size_t i = 0; i < textures.size() && i < texels.size(); i++) textures[i].texels = texels[i].data();
+    desc.num_textures = (uint32_t)textures.size();
+    desc.textures = textures.empty() ? nullptr : textures.data();
     desc.camera = camera;
     std::memcpy(desc.env_ambient, env_ambient, sizeof env_ambient);
     std::memcpy(desc.env_color, env_color, sizeof env_color);

@@ -42,9 +42,16 @@ int sray_scene_load_xml(const char *path, const char *xml_text, sray_host_scene 
         ss << f.rdbuf();
         text = ss.str();
     }
+    /* texture maps are searched for in the scene file's directory first */
+    std::string dir;
+    if (path) {
+        const char *slash = strrchr(path, '/');
+        if (slash) dir.assign(path, (size_t)(slash - path));
+        if (slash == path) dir = "/";
+    }
     auto *h = new sray_host_scene;
     std::string err;
-    if (!sray::load_scene_xml(text, h->hs, err)) {
+    if (!sray::load_scene_xml(text, h->hs, err, dir)) {
         g_host_err = "failed to load scene: " + err;
         delete h;
         return SRAY_ERR_PARSE;
@@ -168,7 +175,8 @@ int sray_projmap_cell(float x, float y, float z, int *uc, int *vc)
 extern "C" int sray_abi_sizes(uint32_t *out, int n)
 {
     const uint32_t sz[] = {(uint32_t)sizeof(sray_object_desc),   (uint32_t)sizeof(sray_mesh_desc),
-                           (uint32_t)sizeof(sray_material_desc), (uint32_t)sizeof(sray_light_desc),
+                           (uint32_t)sizeof(sray_material_desc), (uint32_t)sizeof(sray_texture_desc),
+                           (uint32_t)sizeof(sray_light_desc),
                            (uint32_t)sizeof(sray_camera_desc),   (uint32_t)sizeof(sray_scene_desc),
                            (uint32_t)sizeof(sray_ray),           (uint32_t)sizeof(sray_hit),
                            (uint32_t)sizeof(sray_trace_counts),  (uint32_t)sizeof(sray_render_params),

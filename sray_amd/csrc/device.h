@@ -55,6 +55,17 @@ struct alignas(16) DevMat {
     float4 kd, ks;
     float shininess, reflect, refract, ior;
     uint32_t flags, pad0, pad1, pad2;
+    /* textures of the attributes (SRAY_MAP_*): 1 + texture index, 0 = none */
+    int32_t map[SRAY_NUM_MAPS];
+    float nscale; /* the "normal" attribute's value */
+};
+constexpr uint32_t MAT_MAPPED = 0x100u; /* some attribute has a texture */
+
+/* Texture (src/texture.cc): RGBA texels, filter, wrap, coordinate matrix */
+struct alignas(16) DevTex {
+    const float4 *texels;
+    int32_t w, h, filter, wrap;
+    float xf[12];
 };
 
 struct alignas(16) DevLight {
@@ -103,6 +114,8 @@ struct SceneDev {
     /* animated camera: xform (12 floats) per msec from cam_t0 */
     const float *cam_tab;
     int32_t cam_t0, cam_n;
+    const DevTex *texs;
+    int32_t ntex;
 };
 
 /* ------------------------------------------------------------------ */

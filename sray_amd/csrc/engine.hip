@@ -294,9 +294,14 @@ __device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restr
         const DevObj &ob = S.objs[h.y];
         const DevMat &mat = S.mats[ob.mat];
         bool phong = (mat.flags & SRAY_MAT_PHONG) != 0;
-        V3 Pw, N;
+        /* texture maps: compiled into the ATTR instances only (scenes with
+         * mapped materials count as scenes with attributes) */
+        const bool mapped = ATTR && phong && (mat.flags & MAT_MAPPED);
+        const bool has_normal = phong && (mat.flags & SRAY_MAT_HAS_NORMAL);
+        const bool bump = mapped && has_normal && mat.map[SRAY_MAP_NORMAL] > 0;
+        V3 Pw, N, G;
         float tu, tv;
-        hit_surface<ATTR>(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong, Pw, N, tu, tv);
+        hit_surface<ATTR>(S, ob, time, ro, rd, t, (uint32_t)h.z, !phong || mapped, Pw, N, tu, tv, bump, G);
         if (path == 1u) dir.w = 1.0f;
         uint32_t pix = (uint32_t)(P.y0 + (int)(lp / (uint32_t)P.W)) * (uint32_t)P.width + (uint32_t)(P.x0 + (int)(lp % (uint32_t)P.W));
         int nl = S.nlights;
@@ -344,6 +349,12 @@ __device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restr
             V3 incident = dvs(rd, ray_mag);
             float4 kd = mat.kd;
             float4 ks = (mat.flags & SRAY_MAT_HAS_SPECULAR) ? mat.ks : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            float shininess = mat.shininess;
+            if (mapped) {
+                kd = mat_attr(S, mat.map[SRAY_MAP_DIFFUSE], kd, tu, tv);
+                if (mat.flags & SRAY_MAT_HAS_SPECULAR) ks = mat_attr(S, mat.map[SRAY_MAP_SPECULAR], ks, tu, tv);
+                shininess = mat_value(S, mat.map[SRAY_MAP_SHININESS], shininess, tu, tv);
+            }
             if (flg & QF_DIFFUSE) {
                 /* diffuse GI ray: radiance estimate from the global map
                  * (src/shader.cc:119-123) */
@@ -364,6 +375,13 @@ __device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restr
                 float refl_fact = (mat.flags & SRAY_MAT_HAS_REFLECT) ? mat.reflect : 0.0f;
                 float trans_fact = (mat.flags & SRAY_MAT_HAS_REFRACT) ? mat.refract : 0.0f;
                 float mat_ior = mat.ior;
+                if (mapped) {
+                    if (mat.flags & SRAY_MAT_HAS_REFLECT)
+                        refl_fact = mat_value(S, mat.map[SRAY_MAP_REFLECT], refl_fact, tu, tv);
+                    if (mat.flags & SRAY_MAT_HAS_REFRACT)
+                        trans_fact = mat_value(S, mat.map[SRAY_MAP_REFRACT], trans_fact, tu, tv);
+                    mat_ior = mat_value(S, mat.map[SRAY_MAP_IOR], mat_ior, tu, tv);
+                }
                 IorStack st;
                 st.cur = R.w.w;
                 st.s[0] = R.stk.x;
@@ -373,6 +391,9 @@ __device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restr
                 st.n = R.m.w & 0xffu;
                 st.overflow = false;
                 float ior = st.calc(entering, mat_ior);
+                /* a "normal" attribute replaces the flipped normal, src/shader.cc:137-138:
+                 * by the mapped normal, or without a map by the unflipped one */
+                if (has_normal) normal = bump ? bump_normal(S, mat.map[SRAY_MAP_NORMAL], mat.nscale, N, G, tu, tv) : N;
                 /* ambient: (env_ambient * kd) * kd, src/shader.cc:142,246 */
                 dir.x = W.x * ((S.env_ambient.x * kd.x) * kd.x);
                 dir.y = W.y * ((S.env_ambient.y * kd.y) * kd.y);
@@ -392,7 +413,7 @@ __device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restr
                         ldir = dvs(ldir, ldist);
                         float ndotl = fmaxf(dot(normal, ldir), 0.0f);
                         float rdotl = fmaxf(dot(vref, ldir), 0.0f);
-                        float sp = powf(rdotl, mat.shininess);
+                        float sp = powf(rdotl, shininess);
                         SRay s;
                         s.o = make_float4(Pw.x, Pw.y, Pw.z, __int_as_float(time));
                         s.d = make_float4(sd.x, sd.y, sd.z, __uint_as_float(rq));

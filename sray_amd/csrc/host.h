@@ -66,6 +66,12 @@ struct HostScene {
     bool has_normal = false, has_tangent = false, has_texcoord = false;
     std::vector<sray_material_desc> materials;
     std::vector<std::string> material_names;
+    /* textures of the "map" attributes: descriptors, their texels (RGBA,
+     * top row first), the file each came from and the name the scene
+     * gave it */
+    std::vector<sray_texture_desc> textures;
+    std::vector<std::vector<float>> texels;
+    std::vector<std::string> texture_paths, texture_names;
     std::vector<sray_light_desc> lights;
     sray_camera_desc camera{};
     bool cam_has_target = false; /* target camera: position + target point */
@@ -212,7 +218,10 @@ struct XmlNode {
 bool xml_parse(const std::string &text, XmlNode &root, std::string &err);
 XmlAttr xml_make_attr(const std::string &name, const std::string &val);
 
-bool load_scene_xml(const std::string &text, HostScene &hs, std::string &err);
+bool load_scene_xml(const std::string &text, HostScene &hs, std::string &err, const std::string &scene_dir = "");
+/* Image::load (src/img.cc:64-78): a PNG, PPM/PGM or PFM file as RGBA
+ * floats, row-major, top row first */
+bool read_image_rgbaf(const std::string &path, int &w, int &h, std::vector<float> &px, std::string &err);
 std::string write_scene_xml(const HostScene &hs);
 
 /* ------------------------------------------------------------------ */

@@ -5,7 +5,9 @@
  * 32-byte status records of the reference's backend protocol (emit_status,
  * src/render.cc:279-289), which the serving engine also writes.
  */
+#include <cctype>
 #include <cerrno>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -26,9 +28,342 @@ int io_fail(int code, const std::string &msg)
 }
 } // namespace
 
+namespace sray {
+namespace {
+
+bool read_file(const std::string &path, std::vector<unsigned char> &buf, std::string &err)
+{
+    FILE *fp = fopen(path.c_str(), "rb");
+    if (!fp) {
+        err = "failed to open image: " + path + ": " + strerror(errno);
+        return false;
+    }
+    unsigned char tmp[65536];
+    size_t n;
+    try {
+        while ((n = fread(tmp, 1, sizeof tmp, fp)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+    } catch (const std::bad_alloc &) {
+        fclose(fp);
+        err = "image too large for host memory: " + path;
+        return false;
+    }
+    bool bad = ferror(fp) != 0;
+    fclose(fp);
+    if (bad) err = "failed to read image: " + path;
+    return !bad;
+}
+
+/* netpbm header tokens: whitespace-separated, '#' starts a comment to the
+ * end of the line */
+bool pnm_token(const std::vector<unsigned char> &b, size_t &at, std::string &tok)
+{
+    tok.clear();
+    while (at < b.size()) {
+        if (b[at] == '#') {
+            while (at < b.size() && b[at] != '\n') at++;
+        } else if (isspace(b[at])) at++;
+        else break;
+    }
+    while (at < b.size() && !isspace(b[at]) && b[at] != '#') tok.push_back((char)b[at++]);
+    return !tok.empty();
+}
+bool pnm_int(const std::vector<unsigned char> &b, size_t &at, long &v)
+{
+    std::string t;
+    if (!pnm_token(b, at, t)) return false;
+    char *end = nullptr;
+    v = strtol(t.c_str(), &end, 10);
+    return *end == 0;
+}
+
+constexpr long MAX_SIDE = 1L << 15; /* 32768 texels a side */
+
+/* P2/P3 (text) and P5/P6 (binary) grey / RGB, maxval up to 65535 (two
+ * bytes, most significant first, above 255): value / maxval, alpha 1 */
+bool read_pnm(const std::vector<unsigned char> &b, const std::string &path, int &w, int &h, std::vector<float> &px,
+              std::string &err)
+{
+    const char kind = (char)b[1];
+    const int ch = (kind == '3' || kind == '6') ? 3 : 1;
+    const bool text = kind == '2' || kind == '3';
+    size_t at = 2;
+    long W, H, maxv;
+    if (!pnm_int(b, at, W) || !pnm_int(b, at, H) || !pnm_int(b, at, maxv) || W <= 0 || H <= 0 || W > MAX_SIDE ||
+        H > MAX_SIDE || maxv <= 0 || maxv > 65535) {
+        err = "bad netpbm header: " + path;
+        return false;
+    }
+    at++; /* the single whitespace byte after maxval */
+    const size_t n = (size_t)W * (size_t)H * ch, bps = maxv > 255 ? 2 : 1;
+    if (!text && b.size() < at + n * bps) {
+        err = "truncated netpbm image: " + path;
+        return false;
+    }
+    px.assign((size_t)W * H * 4, 1.0f);
+    const float scale = (float)maxv;
+    for (size_t i = 0; i < n; i++) {
+        long v;
+        if (text) {
+            if (!pnm_int(b, at, v)) {
+                err = "truncated netpbm image: " + path;
+                return false;
+            }
+        } else v = bps == 2 ? ((long)b[at + 2 * i] << 8) | b[at + 2 * i + 1] : (long)b[at + i];
+        const float f = (float)v / scale;
+        const size_t p = i / ch;
+        if (ch == 3) px[p * 4 + i % 3] = f;
+        else px[p * 4] = px[p * 4 + 1] = px[p * 4 + 2] = f;
+    }
+    w = (int)W;
+    h = (int)H;
+    return true;
+}
+
+/* PFM: "PF" (RGB) or "Pf" (grey), width height, a scale whose sign gives
+ * the byte order (negative: little-endian), float32 rows bottom row first */
+bool read_pfm(const std::vector<unsigned char> &b, const std::string &path, int &w, int &h, std::vector<float> &px,
+              std::string &err)
+{
+    const int ch = b[1] == 'F' ? 3 : 1;
+    size_t at = 2;
+    long W, H;
+    std::string st;
+    if (!pnm_int(b, at, W) || !pnm_int(b, at, H) || !pnm_token(b, at, st) || W <= 0 || H <= 0 || W > MAX_SIDE ||
+        H > MAX_SIDE) {
+        err = "bad PFM header: " + path;
+        return false;
+    }
+    const double scale = strtod(st.c_str(), nullptr);
+    if (scale == 0.0) {
+        err = "bad PFM header: " + path;
+        return false;
+    }
+    at++;
+    const bool little = scale < 0.0;
+    const size_t n = (size_t)W * (size_t)H * ch;
+    if (b.size() < at + n * 4) {
+        err = "truncated PFM image: " + path;
+        return false;
+    }
+    px.assign((size_t)W * H * 4, 1.0f);
+    for (size_t i = 0; i < n; i++) {
+        const unsigned char *q = &b[at + 4 * i];
+        uint32_t u = little ? (uint32_t)q[0] | (uint32_t)q[1] << 8 | (uint32_t)q[2] << 16 | (uint32_t)q[3] << 24
+                            : (uint32_t)q[3] | (uint32_t)q[2] << 8 | (uint32_t)q[1] << 16 | (uint32_t)q[0] << 24;
+        float f;
+        memcpy(&f, &u, 4);
+        const size_t pf = i / ch, x = pf % (size_t)W, yb = pf / (size_t)W;
+        const size_t p = ((size_t)H - 1 - yb) * (size_t)W + x; /* top row first */
+        if (ch == 3) px[p * 4 + i % 3] = f;
+        else px[p * 4] = px[p * 4 + 1] = px[p * 4 + 2] = f;
+    }
+    w = (int)W;
+    h = (int)H;
+    return true;
+}
+
+uint32_t be32(const unsigned char *p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+/* PNG (ISO/IEC 15948): grey, RGB, palette, grey + alpha and RGBA at every
+ * bit depth, tRNS transparency, filters 0-4; the IDAT stream inflated with
+ * zlib.  Interlaced (Adam7) files are refused.  A channel of depth d reads
+ * value / (2^d - 1). */
+bool read_png(const std::vector<unsigned char> &b, const std::string &path, int &w, int &h, std::vector<float> &px,
+              std::string &err)
+{
+    size_t at = 8;
+    uint32_t W = 0, H = 0;
+    int depth = 0, ctype = -1, interlace = 0;
+    std::vector<unsigned char> idat, plte, trns;
+    bool end = false;
+    while (!end) {
+        if (at + 12 > b.size()) {
+            err = "truncated PNG: " + path;
+            return false;
+        }
+        const uint32_t len = be32(&b[at]);
+        const std::string type((const char *)&b[at + 4], 4);
+        if (len > b.size() - at - 12) {
+            err = "truncated PNG: " + path;
+            return false;
+        }
+        const unsigned char *d = &b[at + 8];
+        uLong crc = crc32(0L, &b[at + 4], 4 + len);
+        if ((uint32_t)crc != be32(d + len)) {
+            err = "PNG chunk " + type + " fails its CRC: " + path;
+            return false;
+        }
+        if (type == "IHDR") {
+            if (len < 13) {
+                err = "bad PNG header: " + path;
+                return false;
+            }
+            W = be32(d);
+            H = be32(d + 4);
+            depth = d[8];
+            ctype = d[9];
+            interlace = d[12];
+            if (d[10] != 0 || d[11] != 0) {
+                err = "unknown PNG compression or filter method: " + path;
+                return false;
+            }
+        } else if (type == "PLTE") plte.assign(d, d + len);
+        else if (type == "tRNS") trns.assign(d, d + len);
+        else if (type == "IDAT") idat.insert(idat.end(), d, d + len);
+        else if (type == "IEND") end = true;
+        at += 12 + (size_t)len;
+    }
+    static const int chans[7] = {1, 0, 3, 1, 2, 0, 4};
+    const bool okdepth = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) ||
+                         (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8)) ||
+                         ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+    if (W == 0 || H == 0 || W > (uint32_t)MAX_SIDE || H > (uint32_t)MAX_SIDE || !okdepth) {
+        err = "unsupported PNG size, colour type or depth: " + path;
+        return false;
+    }
+    if (interlace != 0) {
+        err = "interlaced PNG not supported: " + path;
+        return false;
+    }
+    if (ctype == 3 && (plte.empty() || plte.size() % 3 != 0)) {
+        err = "PNG palette missing: " + path;
+        return false;
+    }
+    const int nch = chans[ctype];
+    const size_t bpp = ((size_t)nch * depth + 7) / 8; /* filter unit, at least one byte */
+    const size_t stride = ((size_t)W * nch * depth + 7) / 8;
+    std::vector<unsigned char> raw;
+    try {
+        raw.resize((stride + 1) * H);
+    } catch (const std::bad_alloc &) {
+        err = "image too large for host memory: " + path;
+        return false;
+    }
+    uLongf rlen = (uLongf)raw.size();
+    int zr = uncompress(raw.data(), &rlen, idat.data(), (uLong)idat.size());
+    if (zr != Z_OK || rlen != raw.size()) {
+        err = "corrupt PNG image data: " + path;
+        return false;
+    }
+    /* undo the scanline filters in place */
+    std::vector<unsigned char> prev(stride, 0);
+    for (uint32_t y = 0; y < H; y++) {
+        unsigned char *row = &raw[(size_t)y * (stride + 1)];
+        const int f = row[0];
+        unsigned char *c = row + 1;
+        for (size_t i = 0; i < stride; i++) {
+            const int a = i >= bpp ? c[i - bpp] : 0, up = prev[i], ul = i >= bpp ? prev[i - bpp] : 0;
+            int v = c[i];
+            switch (f) {
+            case 0: break;
+            case 1: v += a; break;
+            case 2: v += up; break;
+            case 3: v += (a + up) >> 1; break;
+            case 4: {
+                const int p = a + up - ul, pa = abs(p - a), pb = abs(p - up), pc = abs(p - ul);
+                v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? up : ul);
+                break;
+            }
+            default:
+                err = "bad PNG filter type: " + path;
+                return false;
+            }
+            c[i] = (unsigned char)v;
+        }
+        memcpy(prev.data(), c, stride);
+    }
+    px.assign((size_t)W * H * 4, 1.0f);
+    const float maxv = (float)((1u << depth) - 1u);
+    auto sample = [&](const unsigned char *row, size_t k) -> uint32_t {
+        if (depth == 16) return (uint32_t)row[2 * k] << 8 | row[2 * k + 1];
+        if (depth == 8) return row[k];
+        const size_t bit = k * (size_t)depth;
+        return (uint32_t)(row[bit >> 3] >> (8 - depth - (int)(bit & 7))) & ((1u << depth) - 1u);
+    };
+    for (uint32_t y = 0; y < H; y++) {
+        const unsigned char *row = &raw[(size_t)y * (stride + 1) + 1];
+        for (uint32_t x = 0; x < W; x++) {
+            float *o = &px[((size_t)y * W + x) * 4];
+            if (ctype == 3) {
+                const uint32_t ix = sample(row, x);
+                if ((size_t)ix * 3 + 2 >= plte.size()) {
+                    err = "PNG palette index out of range: " + path;
+                    return false;
+                }
+                o[0] = (float)plte[ix * 3] / 255.0f;
+                o[1] = (float)plte[ix * 3 + 1] / 255.0f;
+                o[2] = (float)plte[ix * 3 + 2] / 255.0f;
+                o[3] = ix < trns.size() ? (float)trns[ix] / 255.0f : 1.0f;
+                continue;
+            }
+            uint32_t s[4];
+            for (int k = 0; k < nch; k++) s[k] = sample(row, (size_t)x * nch + k);
+            if (ctype == 0 || ctype == 4) {
+                o[0] = o[1] = o[2] = (float)s[0] / maxv;
+                if (ctype == 4) o[3] = (float)s[1] / maxv;
+                else if (trns.size() >= 2 && s[0] == ((uint32_t)trns[0] << 8 | trns[1])) o[3] = 0.0f;
+            } else {
+                o[0] = (float)s[0] / maxv;
+                o[1] = (float)s[1] / maxv;
+                o[2] = (float)s[2] / maxv;
+                if (ctype == 6) o[3] = (float)s[3] / maxv;
+                else if (trns.size() >= 6 && s[0] == ((uint32_t)trns[0] << 8 | trns[1]) &&
+                         s[1] == ((uint32_t)trns[2] << 8 | trns[3]) && s[2] == ((uint32_t)trns[4] << 8 | trns[5]))
+                    o[3] = 0.0f;
+            }
+        }
+    }
+    w = (int)W;
+    h = (int)H;
+    return true;
+}
+
+} // namespace
+
+bool read_image_rgbaf(const std::string &path, int &w, int &h, std::vector<float> &px, std::string &err)
+{
+    std::vector<unsigned char> b;
+    if (!read_file(path, b, err)) return false;
+    static const unsigned char sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    try {
+        if (b.size() >= 8 && memcmp(b.data(), sig, 8) == 0) return read_png(b, path, w, h, px, err);
+        if (b.size() >= 3 && b[0] == 'P' && (b[1] == 'F' || b[1] == 'f')) return read_pfm(b, path, w, h, px, err);
+        if (b.size() >= 3 && b[0] == 'P' && (b[1] == '2' || b[1] == '3' || b[1] == '5' || b[1] == '6'))
+            return read_pnm(b, path, w, h, px, err);
+    } catch (const std::bad_alloc &) {
+        err = "image too large for host memory: " + path;
+        return false;
+    }
+    err = "unknown image format (PNG, PPM/PGM and PFM are read): " + path;
+    return false;
+}
+
+} // namespace sray
+
 extern "C" {
 
 const char *sray_io_last_error(void) { return g_io_err.c_str(); }
+
+/* Image::load, src/img.cc:64-78 */
+int sray_image_read(const char *path, float **rgba, int *width, int *height)
+{
+    if (!path || !rgba || !width || !height) return io_fail(SRAY_ERR_ARG, "sray_image_read: bad arguments");
+    *rgba = nullptr;
+    *width = *height = 0;
+    std::vector<float> px;
+    int w, h;
+    std::string err;
+    if (!sray::read_image_rgbaf(path, w, h, px, err)) return io_fail(SRAY_ERR_PARSE, err);
+    float *buf = new (std::nothrow) float[px.size()];
+    if (!buf) return io_fail(SRAY_ERR_NOMEM, "image too large for host memory");
+    memcpy(buf, px.data(), px.size() * sizeof(float));
+    *rgba = buf;
+    *width = w;
+    *height = h;
+    return SRAY_OK;
+}
+
+void sray_image_free(float *rgba) { delete[] rgba; }
 
 /* PhotonMap::dump, src/pmap.cc:180-205: the energy line, then one line per
  * photon with position, direction, normal and colour (rgb), printf "%f" */

@@ -159,9 +159,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon(SceneDev S, PhotonJob J,
             nseg++;
             if (h.obj < 0) break;
             const DevObj &ob = S.objs[h.obj];
-            V3 Pw, N;
+            const DevMat &m = S.mats[ob.mat];
+            const bool mapped = (m.flags & MAT_MAPPED) != 0;
+            const bool has_normal = (m.flags & SRAY_MAT_HAS_NORMAL) != 0;
+            const bool bump = mapped && has_normal && m.map[SRAY_MAP_NORMAL] > 0;
+            V3 Pw, N, G;
             float tu, tv;
-            hit_surface(S, ob, msec, ro, rd, h.t, h.tri, false, Pw, N, tu, tv);
+            hit_surface(S, ob, msec, ro, rd, h.t, h.tri, mapped, Pw, N, tu, tv, caustic_mode && bump, G);
             float ray_mag = length(rd);
             bool entering;
             V3 normal;
@@ -172,17 +176,28 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon(SceneDev S, PhotonJob J,
                 normal = N;
                 entering = true;
             }
-            const DevMat &m = S.mats[ob.mat];
-            /* a missing attribute reads as 1 (Material::def_attr, src/material.cc:77) */
+            /* a missing attribute reads as 1 (Material::def_attr, src/material.cc:77);
+             * mapped ones at the hit's texture coordinates (src/scene.cc:583-586, 677-711) */
             float4 spc = (m.flags & SRAY_MAT_HAS_SPECULAR) ? m.ks : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
             float refl = (m.flags & SRAY_MAT_HAS_REFLECT) ? m.reflect : 1.0f;
             float refr = (m.flags & SRAY_MAT_HAS_REFRACT) ? m.refract : 1.0f;
             float mat_ior = m.ior;
+            float4 kd = m.kd;
+            if (mapped) {
+                if (m.flags & SRAY_MAT_HAS_SPECULAR) spc = mat_attr(S, m.map[SRAY_MAP_SPECULAR], spc, tu, tv);
+                if (m.flags & SRAY_MAT_HAS_REFLECT) refl = mat_value(S, m.map[SRAY_MAP_REFLECT], refl, tu, tv);
+                if (m.flags & SRAY_MAT_HAS_REFRACT) refr = mat_value(S, m.map[SRAY_MAP_REFRACT], refr, tu, tv);
+                mat_ior = mat_value(S, m.map[SRAY_MAP_IOR], mat_ior, tu, tv);
+                kd = mat_attr(S, m.map[SRAY_MAP_DIFFUSE], kd, tu, tv);
+            }
             float rnum = prand(seed, a, b, bounce, P_RR, 0, 0);
             uint32_t here = bounce++;
             int act; /* 0 store, 1 reflect, 2 refract, 3 diffuse, 4 absorb */
             if (caustic_mode) {
                 float ior = st.calc(entering, mat_ior);
+                /* trace_caustics_photon takes the "normal" attribute's
+                 * normal (src/scene.cc:603-606); trace_global_photon does not */
+                if (has_normal) normal = bump ? bump_normal(S, m.map[SRAY_MAP_NORMAL], m.nscale, N, G, tu, tv) : N;
                 float ray_dot_n = dot(dvs(neg(rd), ray_mag), normal);
                 float sf0 = refr > 0.0f ? (ior - 1.0f) / (ior + 1.0f) : 1.0f;
                 float fres = fresnel(sf0 * sf0, ray_dot_n);
@@ -190,7 +205,6 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon(SceneDev S, PhotonJob J,
                 refr = (1.0f - fres) * refr;
                 act = rnum < refl ? 1 : rnum < refr ? 2 : 0;
             } else {
-                float4 kd = m.kd;
                 float diff_avg = (kd.x + kd.y + kd.z) / 3.0f;
                 float spec_avg = (spc.x + spc.y + spc.z) / 3.0f;
                 if (rnum < diff_avg) {

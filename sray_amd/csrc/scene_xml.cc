@@ -16,6 +16,8 @@
 #include <map>
 #include <sstream>
 
+#include <unistd.h>
+
 #include "host.h"
 
 namespace sray {
@@ -305,8 +307,71 @@ sray_material_desc default_material()
     return m;
 }
 
+/* find_file, src/datapath.c:56-77: the first "<dir>/<name>" that exists
+ * over the search list — the scene file's directory (src/sray.cc:103-112;
+ * "." for a scene given as text, src/sray.cc:116-118, or a scene file named
+ * without a directory), then the entries of SRAY_DATA_PATH, ':'-separated.
+ * An absolute name is taken as it is. */
+std::string find_data_file(const std::string &name, const std::string &scene_dir)
+{
+    if (!name.empty() && name[0] == '/') return access(name.c_str(), F_OK) == 0 ? name : std::string();
+    std::vector<std::string> dirs;
+    dirs.push_back(scene_dir.empty() ? std::string(".") : scene_dir);
+    if (const char *env = getenv("SRAY_DATA_PATH")) {
+        std::string e(env);
+        size_t at = 0;
+        while (true) {
+            size_t c = e.find(':', at);
+            dirs.push_back(e.substr(at, c == std::string::npos ? std::string::npos : c - at));
+            if (c == std::string::npos) break;
+            at = c + 1;
+        }
+    }
+    for (const std::string &d : dirs) {
+        std::string p = d + "/" + name;
+        if (access(p.c_str(), F_OK) == 0) return p;
+    }
+    return std::string();
+}
+
+/* Texture::load (src/texture.cc:28-37) for a "map" attribute: 1 + the
+ * scene's index of the texture (one per file, shared by the attributes
+ * that name it; linear filter, repeat wrap, identity matrix, Texture(),
+ * src/texture.cc:21-25), or 0 when it cannot be loaded — the reference
+ * then reports it and the attribute keeps its plain colour
+ * (src/material.cc:64-71) */
+int load_texture(const std::string &name, const std::string &scene_dir, HostScene &hs)
+{
+    const std::string path = find_data_file(name, scene_dir);
+    if (path.empty()) {
+        fprintf(stderr, "failed to load texture: %s\n", name.c_str());
+        return 0;
+    }
+    for (size_t i = 0; i < hs.texture_paths.size(); i++)
+        if (hs.texture_paths[i] == path) return (int)i + 1;
+    int w, h;
+    std::vector<float> px;
+    std::string ierr;
+    if (!read_image_rgbaf(path, w, h, px, ierr)) {
+        fprintf(stderr, "failed to load texture: %s (%s)\n", name.c_str(), ierr.c_str());
+        return 0;
+    }
+    sray_texture_desc t;
+    std::memset(&t, 0, sizeof t);
+    t.width = w;
+    t.height = h;
+    t.filter = SRAY_TEX_FILTER_LINEAR;
+    t.wrap = SRAY_TEX_WRAP_REPEAT;
+    t.xform[0] = t.xform[5] = t.xform[10] = 1.0f;
+    hs.textures.push_back(t);
+    hs.texels.push_back(std::move(px));
+    hs.texture_paths.push_back(path);
+    hs.texture_names.push_back(name);
+    return (int)hs.textures.size();
+}
+
 /* Material::load_xml + MatAttrib::load_xml, src/material.cc:36-124 */
-bool load_material(const XmlNode &n, HostScene &hs, std::string &err)
+bool load_material(const XmlNode &n, HostScene &hs, const std::string &scene_dir, std::string &err)
 {
     const XmlAttr *a = n.get("name");
     if (!a) {
@@ -330,22 +395,39 @@ bool load_material(const XmlNode &n, HostScene &hs, std::string &err)
             col[0] = col[1] = col[2] = av->fval;
             col[3] = 1.0f;
         } else continue;
-        /* a "map" texture would need libimago; it fails to load and the
-         * attribute keeps its plain colour, as the reference does when
-         * Texture::load fails */
         const std::string &nm = an->str;
-        if (nm == "diffuse") std::memcpy(m.diffuse, col, sizeof col);
-        else if (nm == "specular") {
+        int slot = -1;
+        if (nm == "diffuse") {
+            std::memcpy(m.diffuse, col, sizeof col);
+            slot = SRAY_MAP_DIFFUSE;
+        } else if (nm == "specular") {
             std::memcpy(m.specular, col, sizeof col);
             m.flags |= SRAY_MAT_HAS_SPECULAR;
-        } else if (nm == "shininess") m.shininess = col[0];
-        else if (nm == "reflect") {
+            slot = SRAY_MAP_SPECULAR;
+        } else if (nm == "shininess") {
+            m.shininess = col[0];
+            slot = SRAY_MAP_SHININESS;
+        } else if (nm == "reflect") {
             m.reflect = col[0];
             m.flags |= SRAY_MAT_HAS_REFLECT;
+            slot = SRAY_MAP_REFLECT;
         } else if (nm == "refract") {
             m.refract = col[0];
             m.flags |= SRAY_MAT_HAS_REFRACT;
-        } else if (nm == "ior") m.ior = col[0];
+            slot = SRAY_MAP_REFRACT;
+        } else if (nm == "ior") {
+            m.ior = col[0];
+            slot = SRAY_MAP_IOR;
+        } else if (nm == "normal") {
+            m.normal = col[0];
+            m.flags |= SRAY_MAT_HAS_NORMAL;
+            slot = SRAY_MAP_NORMAL;
+        }
+        /* set_attribute replaces colour and texture together (src/material.cc:150-161) */
+        if (slot >= 0) {
+            const XmlAttr *am = c.get("map");
+            m.map[slot] = am ? load_texture(am->str, scene_dir, hs) : 0;
+        }
     }
     hs.materials.push_back(m);
     hs.material_names.push_back(name);
@@ -665,14 +747,14 @@ bool load_camera(const XmlNode &n, HostScene &hs, std::string &err)
 
 /* Scene::load(FILE*), src/scene.cc:93-183: materials first, then env,
  * objects, lights and the (first) camera in document order */
-bool load_scene_xml(const std::string &text, HostScene &hs, std::string &err)
+bool load_scene_xml(const std::string &text, HostScene &hs, std::string &err, const std::string &scene_dir)
 {
     XmlNode root;
     if (!xml_parse(text, root, err)) return false;
     hs = HostScene();
     hs.camera = default_camera();
     for (const XmlNode &n : root.chld)
-        if (n.name == "material" && !load_material(n, hs, err)) return false;
+        if (n.name == "material" && !load_material(n, hs, scene_dir, err)) return false;
     bool have_cam = false;
     for (const XmlNode &n : root.chld) {
         if (n.name == "env") {
@@ -760,12 +842,20 @@ std::string write_scene_xml(const HostScene &hs)
         const sray_material_desc &m = hs.materials[i];
         std::string nm = i < hs.material_names.size() ? hs.material_names[i] : ("mat" + std::to_string(i));
         o << "  <material name=\"" << nm << "\"" << ((m.flags & SRAY_MAT_PHONG) ? " shader=\"phong\"" : "") << ">\n";
-        o << "    <mattr name=\"diffuse\" value=\"" << vec3(m.diffuse) << "\"/>\n";
-        o << "    <mattr name=\"specular\" value=\"" << vec3(m.specular) << "\"/>\n";
-        o << "    <mattr name=\"shininess\" value=\"" << f9(m.shininess) << "\"/>\n";
-        o << "    <mattr name=\"reflect\" value=\"" << f9(m.reflect) << "\"/>\n";
-        o << "    <mattr name=\"refract\" value=\"" << f9(m.refract) << "\"/>\n";
-        o << "    <mattr name=\"ior\" value=\"" << f9(m.ior) << "\"/>\n";
+        /* a mapped attribute names its texture as the scene it came from did */
+        auto mp = [&](int k) -> std::string {
+            const int t = m.map[k];
+            if (t <= 0 || t > (int)hs.texture_names.size()) return "";
+            return " map=\"" + hs.texture_names[t - 1] + "\"";
+        };
+        o << "    <mattr name=\"diffuse\" value=\"" << vec3(m.diffuse) << "\"" << mp(SRAY_MAP_DIFFUSE) << "/>\n";
+        o << "    <mattr name=\"specular\" value=\"" << vec3(m.specular) << "\"" << mp(SRAY_MAP_SPECULAR) << "/>\n";
+        o << "    <mattr name=\"shininess\" value=\"" << f9(m.shininess) << "\"" << mp(SRAY_MAP_SHININESS) << "/>\n";
+        o << "    <mattr name=\"reflect\" value=\"" << f9(m.reflect) << "\"" << mp(SRAY_MAP_REFLECT) << "/>\n";
+        o << "    <mattr name=\"refract\" value=\"" << f9(m.refract) << "\"" << mp(SRAY_MAP_REFRACT) << "/>\n";
+        o << "    <mattr name=\"ior\" value=\"" << f9(m.ior) << "\"" << mp(SRAY_MAP_IOR) << "/>\n";
+        if (m.flags & SRAY_MAT_HAS_NORMAL)
+            o << "    <mattr name=\"normal\" value=\"" << f9(m.normal) << "\"" << mp(SRAY_MAP_NORMAL) << "/>\n";
         o << "  </material>\n";
     }
     for (size_t oi = 0; oi < hs.objects.size(); oi++) {

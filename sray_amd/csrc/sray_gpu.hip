@@ -589,6 +589,19 @@ int validate_desc(sray_gpu_ctx *c, const sray_scene_desc *s)
         uint64_t e = (uint64_t)s->meshes[m].tri_start + s->meshes[m].tri_count;
         if (e > s->num_triangles) return fail(c, SRAY_ERR_ARG, "mesh %u: triangle range out of bounds", m);
     }
+    if (s->num_textures && !s->textures) return fail(c, SRAY_ERR_ARG, "textures is NULL");
+    for (uint32_t i = 0; i < s->num_textures; i++) {
+        const sray_texture_desc &t = s->textures[i];
+        if (t.width <= 0 || t.height <= 0 || (int64_t)t.width * t.height > (1ll << 28) || !t.texels)
+            return fail(c, SRAY_ERR_ARG, "texture %u: bad size %dx%d or no texels", i, t.width, t.height);
+        if (t.filter < 0 || t.filter > 1 || t.wrap < 0 || t.wrap > 1)
+            return fail(c, SRAY_ERR_ARG, "texture %u: bad filter or wrap", i);
+    }
+    for (uint32_t i = 0; i < s->num_materials; i++)
+        for (int k = 0; k < SRAY_NUM_MAPS; k++)
+            if (s->materials[i].map[k] < 0 || (uint32_t)s->materials[i].map[k] > s->num_textures)
+                return fail(c, SRAY_ERR_ARG, "material %u: map %d names texture %d of %u", i, k,
+                            s->materials[i].map[k], s->num_textures);
     bool rows = s->camera.time_xforms != nullptr;
     for (uint32_t i = 0; i < s->num_objects; i++) rows = rows || s->objects[i].time_xforms;
     for (uint32_t i = 0; i < s->num_lights; i++) rows = rows || s->lights[i].time_pos;
@@ -691,13 +704,20 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
         }
         DevMat &d = mats[i];
         memset(&d, 0, sizeof d);
+        for (int k = 0; k < SRAY_NUM_MAPS; k++) {
+            d.map[k] = md.map[k];
+            if (md.map[k] > 0) d.flags |= MAT_MAPPED;
+        }
+        d.nscale = md.normal;
+        /* mapped materials need the shading instances with surface attributes */
+        if (d.flags & MAT_MAPPED) g->attr_meshes = true;
         d.kd = make_float4(md.diffuse[0], md.diffuse[1], md.diffuse[2], md.diffuse[3]);
         d.ks = make_float4(md.specular[0], md.specular[1], md.specular[2], md.specular[3]);
         d.shininess = md.shininess;
         d.reflect = md.reflect;
         d.refract = md.refract;
         d.ior = md.ior;
-        d.flags = md.flags;
+        d.flags |= md.flags;
     }
     std::vector<DevLight> lights(s->num_lights);
     for (uint32_t i = 0; i < s->num_lights; i++) {
@@ -807,6 +827,25 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     }
     if ((rc = upload(c, *g, mats.data(), mats.size() * sizeof(DevMat), &p))) return rc;
     D.mats = (const DevMat *)p;
+    /* textures: texels, then the table */
+    if (s->num_textures) {
+        std::vector<DevTex> texs(s->num_textures);
+        for (uint32_t i = 0; i < s->num_textures; i++) {
+            const sray_texture_desc &td = s->textures[i];
+            DevTex &t = texs[i];
+            memset(&t, 0, sizeof t);
+            t.w = td.width;
+            t.h = td.height;
+            t.filter = td.filter;
+            t.wrap = td.wrap;
+            memcpy(t.xf, td.xform, sizeof t.xf);
+            if ((rc = upload(c, *g, td.texels, (size_t)td.width * td.height * 16, &p))) return rc;
+            t.texels = (const float4 *)p;
+        }
+        if ((rc = upload(c, *g, texs.data(), texs.size() * sizeof(DevTex), &p))) return rc;
+        D.texs = (const DevTex *)p;
+        D.ntex = (int32_t)s->num_textures;
+    }
     if ((rc = upload(c, *g, lights.data(), lights.size() * sizeof(DevLight), &p))) return rc;
     D.lights = (const DevLight *)p;
     D.scene_root = sroot;

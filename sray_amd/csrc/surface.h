@@ -169,10 +169,15 @@ using IorStack = IorStackT<IOR_DEPTH>;
  * SurfPoint::normal in the reference.  want_uv: texture coordinates of
  * spheres and cylinders (meshes get them from interp_face when they carry
  * attributes; ATTR = false leaves that path out, for scenes without such
- * meshes, which keeps its registers out of the caller). */
+ * meshes, which keeps its registers out of the caller).  G (want_tan):
+ * SurfPoint::tangent: (0,1,0) x the local normal of a quadric, transformed
+ * as a normal (src/sphere.cc:121,128, src/cylinder.cc:173,181); a mesh's
+ * interpolated tangent, transformed and normalised (src/mesh.cc:298-304;
+ * a zero tangent stays zero). */
 template <bool ATTR = true>
 __device__ __forceinline__ void hit_surface(const SceneDev &S, const DevObj &ob, int time, V3 ro, V3 rd, float t,
-                                            uint32_t ti, bool want_uv, V3 &Pw, V3 &N, float &tu, float &tv)
+                                            uint32_t ti, bool want_uv, V3 &Pw, V3 &N, float &tu, float &tv,
+                                            bool want_tan, V3 &G)
 {
     float I[12], M[12];
     obj_inv_at(ob, time, I);
@@ -190,9 +195,11 @@ __device__ __forceinline__ void hit_surface(const SceneDev &S, const DevObj &ob,
     }
     tu = 0.0f;
     tv = 0.0f;
+    G = v3(0.0f, 0.0f, 0.0f);
     if (ob.type == SRAY_OBJ_SPHERE) {
         V3 nrm = dvs(p, ob.rad);
         N = xf_normal(I, nrm);
+        if (ATTR && want_tan) G = xf_normal(I, cross(v3(0.0f, 1.0f, 0.0f), nrm));
         if (want_uv) {
             float r = length(p);
             tu = (atan2f(p.z, p.x) + 3.14159265358979f) / 6.28318530717959f;
@@ -201,6 +208,7 @@ __device__ __forceinline__ void hit_surface(const SceneDev &S, const DevObj &ob,
     } else if (ob.type == SRAY_OBJ_CYLINDER) {
         V3 nrm = dvs(v3(p.x, 0.0f * p.y, p.z), ob.rad);
         N = xf_normal(I, nrm);
+        if (ATTR && want_tan) G = xf_normal(I, cross(v3(0.0f, 1.0f, 0.0f), nrm));
         if (want_uv) {
             tu = (atan2f(p.z, p.x) + 3.14159265358979f) / 6.28318530717959f;
             tv = p.y;
@@ -214,10 +222,85 @@ __device__ __forceinline__ void hit_surface(const SceneDev &S, const DevObj &ob,
             interp_face(S.tri, (ob.has_attr & 1) ? S.tri_n : nullptr, (ob.has_attr & 4) ? S.tri_g : nullptr,
                         (ob.has_attr & 2) ? S.tri_uv : nullptr, ti, p, gn, in, ig, tu, tv);
             nrm = dot(in, in) < XSMALL ? gn : in;
+            if (want_tan) {
+                V3 tg = xf_normal(I, ig);
+                float tl = length(tg);
+                G = tl > 0.0f ? dvs(tg, tl) : tg;
+            }
         }
         N = normalized(xf_normal(I, nrm));
     }
     Pw = xf_point(M, p);
+}
+
+/* Texture::lookup (src/texture.cc:60-83) with get_texel (:85-101): the
+ * coordinates through the texture matrix, truncated to texels, then one
+ * texel (nearest) or the average of the 5-texel cross (linear), each
+ * wrapped (repeat) or clamped to [0, size] — the reference's clamp lets
+ * the index reach the size itself, one texel past the row; the row-major
+ * index is kept, and one past the last texel reads the last texel. */
+__device__ __forceinline__ float4 tex_texel(const DevTex &T, int tx, int ty)
+{
+    if (T.wrap == SRAY_TEX_WRAP_CLAMP) {
+        tx = tx < 0 ? 0 : (tx > T.w ? T.w : tx);
+        ty = ty < 0 ? 0 : (ty > T.h ? T.h : ty);
+    } else {
+        tx %= T.w;
+        ty %= T.h;
+        if (tx < 0) tx += T.w;
+        if (ty < 0) ty += T.h;
+    }
+    long long k = (long long)ty * T.w + tx;
+    const long long last = (long long)T.w * T.h - 1;
+    return T.texels[k > last ? last : k];
+}
+__device__ __forceinline__ int tex_coord(float c, int size)
+{
+    /* (int)(c * size), C truncation; coordinates beyond +-2^30 texels are
+     * held there (a float-to-int conversion out of range is undefined) */
+    float f = c * (float)size;
+    f = fmaxf(fminf(f, 1073741824.0f), -1073741824.0f);
+    return (int)f;
+}
+__device__ __forceinline__ float4 tex_lookup(const DevTex &T, float u, float v)
+{
+    const float x = T.xf[0] * u + T.xf[1] * v + T.xf[2] * 0.0f + T.xf[3];
+    const float y = T.xf[4] * u + T.xf[5] * v + T.xf[6] * 0.0f + T.xf[7];
+    const int tx = tex_coord(x, T.w), ty = tex_coord(y, T.h);
+    if (T.filter == SRAY_TEX_FILTER_NEAREST) return tex_texel(T, tx, ty);
+    const float4 a = tex_texel(T, tx, ty), b = tex_texel(T, tx + 1, ty), c = tex_texel(T, tx - 1, ty),
+                 d = tex_texel(T, tx, ty + 1), e = tex_texel(T, tx, ty - 1);
+    return make_float4(((((a.x + b.x) + c.x) + d.x) + e.x) / 5.0f, ((((a.y + b.y) + c.y) + d.y) + e.y) / 5.0f,
+                       ((((a.z + b.z) + c.z) + d.z) + e.z) / 5.0f, ((((a.w + b.w) + c.w) + d.w) + e.w) / 5.0f);
+}
+
+/* MatAttrib::get_color (src/material.inl:23-28): texture x colour, or the
+ * colour alone (map 0); get_value is its x component */
+__device__ __forceinline__ float4 mat_attr(const SceneDev &S, int map, float4 col, float u, float v)
+{
+    if (map <= 0) return col;
+    const float4 t = tex_lookup(S.texs[map - 1], u, v);
+    return make_float4(t.x * col.x, t.y * col.y, t.z * col.z, t.w * col.w);
+}
+__device__ __forceinline__ float mat_value(const SceneDev &S, int map, float val, float u, float v)
+{
+    if (map <= 0) return val;
+    return tex_lookup(S.texs[map - 1], u, v).x * val;
+}
+
+/* get_bump_normal (src/shader.cc:293-316): the normal map's colour * 2 - 1,
+ * x and y scaled by the attribute's value, normalised, then taken from the
+ * tangent frame (tangent, normal x tangent, normal) of the unflipped
+ * surface normal */
+__device__ __forceinline__ V3 bump_normal(const SceneDev &S, int map, float scale, V3 N, V3 G, float u, float v)
+{
+    const float4 c = tex_lookup(S.texs[map - 1], u, v);
+    V3 n = v3(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f);
+    n.x = n.x * scale;
+    n.y = n.y * scale;
+    n = normalized(n);
+    const V3 b = cross(N, G);
+    return v3(G.x * n.x + b.x * n.y + N.x * n.z, G.y * n.x + b.y * n.y + N.y * n.z, G.z * n.x + b.z * n.y + N.z * n.z);
 }
 
 } // namespace srd

@@ -369,3 +369,113 @@ def rows_of(desc, what, index=0):
     if not p:
         return None
     return np.ctypeslib.as_array(p, shape=(n * w,)).reshape(n, w).copy()
+
+
+def _normal_map(n, amp=0.35, freq=3):
+    """an 8-bit RGB normal map of bumps h = amp sin(2 pi f u) sin(2 pi f v):
+    n = normalise(-dh/du, -dh/dv, 1) encoded as (n + 1) / 2"""
+    u = (np.arange(n) + 0.5) / n
+    U, V = np.meshgrid(u, u)
+    k = 2 * np.pi * freq
+    du = amp * k * np.cos(k * U) * np.sin(k * V)
+    dv = amp * k * np.sin(k * U) * np.cos(k * V)
+    nv = np.stack([-du, -dv, np.ones_like(du)], -1)
+    nv /= np.linalg.norm(nv, axis=-1, keepdims=True)
+    return np.clip(np.round((nv + 1) * 127.5), 0, 255).astype(np.int64)
+
+
+def write_textures(d, seed=7):
+    """the texture files of textured_scene_xml, written into directory d:
+    checker.png (RGB, random texel colours, filters 0-4), bumps.png (a
+    normal map), spec.ppm (P3 text), mask.pfm (grey, little-endian),
+    shin.pgm (P5, 16-bit), alpha.png (palette + tRNS)"""
+    import imgfiles as I
+    rng = np.random.default_rng(seed)
+    I.write(f"{d}/checker.png", I.png_bytes(rng.integers(40, 256, (16, 16, 3)), 2, 8, filters=(0, 1, 2, 3, 4)))
+    I.write(f"{d}/bumps.png", I.png_bytes(_normal_map(32), 2, 8, filters=(4, 1)))
+    I.write(f"{d}/spec.ppm", I.pnm_bytes(rng.integers(0, 256, (8, 8, 3)), 255, "3"))
+    I.write(f"{d}/mask.pfm", I.pfm_bytes((rng.random((4, 8)) > 0.4).astype(np.float32) * 0.9 + 0.05))
+    I.write(f"{d}/shin.pgm", I.pnm_bytes(rng.integers(20000, 65536, (8, 8)), 65535, "5"))
+    pal = rng.integers(0, 256, (4, 3))
+    I.write(f"{d}/alpha.png", I.png_bytes(rng.integers(0, 4, (8, 8)), 3, 2, palette=pal, trns=[255, 128]))
+
+
+def textured_scene_xml(bump_scale=1.5):
+    """a scene whose materials are mapped (the file names of
+    write_textures): a textured floor with tangents and texture
+    coordinates running to 3 (repeat wrap), a bump-mapped sphere, a
+    bump-mapped pyramid without tangents, a sphere whose reflectance is
+    masked, a glass sphere with a mapped index of refraction, a bump-mapped
+    cylinder that also reflects, and a back-facing wall with a "normal"
+    attribute but no map (the unflipped normal)"""
+    def mat(name, attrs):
+        s = f'  <material name="{name}" shader="phong">\n'
+        for a in attrs:
+            s += "    <mattr " + " ".join(f'{k}="{v}"' for k, v in a.items()) + "/>\n"
+        return s + "  </material>\n"
+    x = "<scene>\n"
+    x += '  <env ambient="0.08 0.08 0.08" bgcolor="0.1 0.1 0.2" ior="1.0"/>\n'
+    x += mat("floor", [dict(name="diffuse", value="1.0 0.9 0.8", map="checker.png"),
+                       dict(name="specular", value="0.6 0.6 0.6", map="spec.ppm"),
+                       dict(name="shininess", value="60.0", map="shin.pgm")])
+    x += mat("bumpy", [dict(name="diffuse", value="0.7 0.5 0.3"), dict(name="specular", value="0.7 0.7 0.7"),
+                       dict(name="shininess", value="40.0"), dict(name="normal", value=str(bump_scale), map="bumps.png")])
+    x += mat("mirror", [dict(name="diffuse", value="0.3 0.3 0.35"), dict(name="specular", value="0.9 0.9 0.9"),
+                        dict(name="shininess", value="80.0"), dict(name="reflect", value="0.9", map="mask.pfm")])
+    x += mat("glass", [dict(name="diffuse", value="0.0 0.0 0.0"), dict(name="specular", value="1.0 1.0 1.0"),
+                       dict(name="shininess", value="80.0"), dict(name="reflect", value="0.1"),
+                       dict(name="refract", value="0.9"), dict(name="ior", value="1.6", map="shin.pgm")])
+    x += mat("bumpmirror", [dict(name="diffuse", value="0.4 0.6 0.4", map="alpha.png"),
+                            dict(name="specular", value="0.5 0.5 0.5"), dict(name="reflect", value="0.4"),
+                            dict(name="normal", value="0.8", map="bumps.png")])
+    x += mat("flipwall", [dict(name="diffuse", value="0.6 0.6 0.9"), dict(name="specular", value="0.4 0.4 0.4"),
+                          dict(name="normal", value="1.0")])
+    x += mat("missing", [dict(name="diffuse", value="0.9 0.3 0.3", map="no_such_file.png")])
+    # floor: texture coordinates 0..3 over the quad, tangent +x
+    x += ('  <object type="mesh">\n    <matref name="floor"/>\n    <mesh>\n'
+          '      <vertex id="0" val="-8.0 -1.0 -8.0"/><vertex id="1" val="8.0 -1.0 -8.0"/>\n'
+          '      <vertex id="2" val="8.0 -1.0 8.0"/><vertex id="3" val="-8.0 -1.0 8.0"/>\n'
+          '      <normal id="0" val="0.0 1.0 0.0"/>\n      <tangent id="0" val="1.0 0.0 0.0"/>\n'
+          '      <texcoord id="0" val="0.0 0.0 0.0"/><texcoord id="1" val="3.0 0.0 0.0"/>\n'
+          '      <texcoord id="2" val="3.0 3.0 0.0"/><texcoord id="3" val="0.0 3.0 0.0"/>\n'
+          '      <face><vref vertex="0" normal="0" tangent="0" texcoord="0"/>'
+          '<vref vertex="2" normal="0" tangent="0" texcoord="2"/>'
+          '<vref vertex="1" normal="0" tangent="0" texcoord="1"/></face>\n'
+          '      <face><vref vertex="0" normal="0" tangent="0" texcoord="0"/>'
+          '<vref vertex="3" normal="0" tangent="0" texcoord="3"/>'
+          '<vref vertex="2" normal="0" tangent="0" texcoord="2"/></face>\n'
+          '    </mesh>\n  </object>\n')
+    x += ('  <object type="sphere">\n    <matref name="bumpy"/>\n    <xform pos="-1.8 0.2 0.5"/>\n'
+          '    <sphere rad="1.1"/>\n  </object>\n')
+    x += ('  <object type="mesh">\n    <matref name="bumpy"/>\n    <xform pos="-4.0 -1.0 2.5"/>\n'
+          + _pyramid() + '  </object>\n')
+    x += ('  <object type="sphere">\n    <matref name="mirror"/>\n    <xform pos="1.2 0.0 1.5"/>\n'
+          '    <sphere rad="1.0"/>\n  </object>\n')
+    x += ('  <object type="sphere">\n    <matref name="glass"/>\n    <xform pos="0.2 -0.3 -1.6"/>\n'
+          '    <sphere rad="0.7"/>\n  </object>\n')
+    x += ('  <object type="cylinder">\n    <matref name="bumpmirror"/>\n    <xform pos="3.3 -1.0 0.5"/>\n'
+          '    <cylinder rad="0.6" start="0.0" end="2.2"/>\n  </object>\n')
+    x += ('  <object type="sphere">\n    <matref name="missing"/>\n    <xform pos="-0.6 -0.6 -3.2"/>\n'
+          '    <sphere rad="0.4"/>\n  </object>\n')
+    # a wall whose geometric normal faces away from the camera
+    x += ('  <object type="mesh">\n    <matref name="flipwall"/>\n    <mesh>\n'
+          '      <vertex id="0" val="-8.0 -1.0 5.0"/><vertex id="1" val="8.0 -1.0 5.0"/>\n'
+          '      <vertex id="2" val="8.0 5.0 5.0"/><vertex id="3" val="-8.0 5.0 5.0"/>\n'
+          '      <face><vref vertex="0"/><vref vertex="1"/><vref vertex="2"/></face>\n'
+          '      <face><vref vertex="0"/><vref vertex="2"/><vref vertex="3"/></face>\n'
+          '    </mesh>\n  </object>\n')
+    x += ('  <light type="point" color="1.0 1.0 1.0">\n    <xform pos="3.0 6.0 -5.0"/>\n  </light>\n'
+          '  <light type="point" color="0.4 0.4 0.5">\n    <xform pos="-5.0 3.0 -2.0"/>\n  </light>\n')
+    x += ('  <camera type="target" fov="55.0">\n    <xform pos="0.5 2.5 -8.0"/>\n'
+          '    <target><xform pos="0.0 0.0 0.5"/></target>\n  </camera>\n')
+    return x + "</scene>\n"
+
+
+def textured_scene(d, bump_scale=1.5):
+    """writes the textures and textured_scene_xml into directory d and
+    loads the scene from there (maps are found next to the scene file)"""
+    write_textures(d)
+    path = f"{d}/scene.xml"
+    with open(path, "w") as f:
+        f.write(textured_scene_xml(bump_scale))
+    return S.HostScene.load_xml(path=path)
