@@ -460,12 +460,14 @@ def run_spheres(args):
 def room_params(W, H, diag, seed):
     import sray_amd as S
     return S.render_params(W, H, samples=(1, 16), max_var=0.05, iter=5, shadow_samples=4, diffuse_samples=1,
-                           caust_photons=500_000, gi_photons=2_000_000, gather_dist=0.1 / diag, seed=seed)
+                           caust_photons=500_000, gi_photons=2_000_000, gather_dist=0.1 / diag, seed=seed,
+                           gather_k=200)
 
 
 def run_room(args):
     """photon-mapped room, 1280x720, adaptive 1-16 spp, 2M global + 500k
-    caustic photons shot on the device: K frames one after another
+    caustic photons shot on the device, gathers of the 200 nearest within
+    0.1: K frames one after another
     (plain) against the same K frames in flight together (batched)"""
     import sray_amd as S
     config, variant, W, H, it, samples = WORKLOADS["room"]
@@ -501,7 +503,7 @@ def run_room(args):
         cpu = cpu_rows(args, cpu_checker(hs.desc_ptr(), photons), frames[0], "room")
     return base_line("Mrays/s on 1 GPU (batched)", batched, K, 1, batch_wall * 1e3 / K,
                      f"room: 12 quads + 2 glass spheres + 50k-triangle icosphere, box light, {W}x{H}, 1-16 spp "
-                     f"(variance 0.05), 2M+500k photons, {K} frames",
+                     f"(variance 0.05), 2M+500k photons, k=200 nearest within 0.1, {K} frames",
                      {"batched": {"value": round(batched, 3), "frames_in_flight": K, "wall_s": round(batch_wall, 3),
                                   "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)},
                       "plain": {"value": round(plain, 3), "wall_s": round(plain_wall, 3)},
@@ -610,7 +612,7 @@ def run_serve(args):
         v /= np.linalg.norm(v)
         cam = look_at(np.array([0.0, 4.0, 0.0]) + 3.5 * v, (0.0, 4.0, 0.0), 60.0)
         p = S.render_params(w, h, samples=(1, 16), max_var=0.05, iter=5, shadow_samples=4, caust_photons=1,
-                            gi_photons=1, gather_dist=0.1 / diags[sc], seed=k)
+                            gi_photons=1, gather_dist=0.1 / diags[sc], seed=k, gather_k=200)
         return (sids[sc], p, cam)
 
     # capacity: the completion rate of a stream of the same mix offered

@@ -245,7 +245,13 @@ typedef struct sray_render_params {
     float    gather_dist;           /* -gatherdisc, relative to the scene octree diagonal */
     uint64_t seed;                  /* counter-based RNG seed (sample jitter, area lights, GI) */
     uint32_t flags;                 /* SRAY_RENDER_* */
-    uint32_t pad_;
+    /* photons per gather estimate, the max_photons of
+     * PhotonMap::irradiance_est / radiance_est (src/pmap.h:59-60, which
+     * the reference leaves unused): 0 (or below) every photon within the
+     * radius, as the reference; k > 0 the k nearest within the radius,
+     * ties at equal distance by photon index.  The estimate keeps the
+     * reference's 1 / (pi r^2) density. */
+    int32_t  gather_k;
 } sray_render_params;
 
 typedef struct sray_stats {
@@ -275,9 +281,10 @@ typedef struct sray_photon {
 #define SRAY_PMAP_GLOBAL     1
 #define SRAY_GATHER_IRRADIANCE 0   /* PhotonMap::irradiance_est, src/pmap.cc:109-139 */
 #define SRAY_GATHER_RADIANCE   1   /* PhotonMap::radiance_est,  src/pmap.cc:143-178 */
-/* or-ed into the gather mode: fixed-radius gather over a uniform grid of
- * radius-sized cells (built per radius on first use, the structure the
- * renderer's shading gathers use) instead of the kd-tree; same found set */
+/* or-ed into the gather mode: gather over a uniform grid of radius-sized
+ * cells (built per radius on first use, the structure the renderer's
+ * shading gathers use) instead of the kd-tree; same found set, and with
+ * k > 0 the same k nearest (a radix select of dist^2, then photon index) */
 #define SRAY_GATHER_GRID       0x10
 
 typedef struct sray_gather_query {

@@ -1287,7 +1287,7 @@ template <class T> struct Scene {
 
         if (ray.iter == INT_MAX) {
             /* diffuse (GI) ray: radiance estimate from the global map */
-            C4<T> rad = radiance(sp.pos, normal, gather_dist_abs, 0);
+            C4<T> rad = radiance(sp.pos, normal, gather_dist_abs, std::max(sc.p->gather_k, 0));
             return rad * kd;
         }
         if (mat.flags & SRAY_MAT_HAS_REFLECT) refl_fact = attr_value(mat.map[SRAY_MAP_REFLECT], mat.reflect, sp);
@@ -1355,7 +1355,7 @@ template <class T> struct Scene {
         }
 
         C4<T> irrad = c4<T>(0, 0, 0, 0);
-        if (sc.p->caust_photons) irrad = irradiance(sp.pos, normal, gather_dist_abs, 0);
+        if (sc.p->caust_photons) irrad = irradiance(sp.pos, normal, gather_dist_abs, std::max(sc.p->gather_k, 0));
 
         T ray_dot_n = dot(-ray.d / ray_mag, normal);
         T sqrt_fres_0 = trans_fact > (T)0 ? (ior - (T)1) / (ior + (T)1) : (T)1;

@@ -160,7 +160,7 @@ def make_rays(origins, dirs, time=0):
 
 def render_params(width, height, rect=None, samples=(1, 1), max_var=0.005, min_energy=1e-4, iter=7,
                   shadow_samples=1, diffuse_samples=1, frame_time=0, mblur=0, caust_photons=0, gi_photons=0,
-                  gather_dist=0.001, seed=0x5EED, flags=0):
+                  gather_dist=0.001, seed=0x5EED, flags=0, gather_k=0):
     """sray_render_params with the reference's option defaults (src/opt.c:357-388)."""
     p = RenderParams()
     p.width, p.height = width, height
@@ -172,6 +172,7 @@ def render_params(width, height, rect=None, samples=(1, 1), max_var=0.005, min_e
     p.frame_time, p.mblur = frame_time, mblur
     p.caust_photons, p.gi_photons, p.gather_dist = caust_photons, gi_photons, gather_dist
     p.seed, p.flags = seed, flags
+    p.gather_k = gather_k
     return p
 
 

@@ -95,7 +95,7 @@ class RenderParams(C.Structure):
                 ("min_samples", i32), ("max_samples", i32), ("max_var", f32), ("min_energy", f32),
                 ("iter", i32), ("shadow_samples", i32), ("diffuse_samples", i32), ("frame_time", i32),
                 ("mblur", i32), ("caust_photons", i32), ("gi_photons", i32), ("gather_dist", f32),
-                ("seed", u64), ("flags", u32), ("pad_", u32)]
+                ("seed", u64), ("flags", u32), ("gather_k", i32)]
 
 
 class Stats(C.Structure):

@@ -728,9 +728,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ATTR || LMAX
 /* photon gathers requested by k_shade: caustic irradiance (>= 8 photons,
  * src/pmap.cc:109-139) added to the direct term, or the radiance estimate
  * (src/pmap.cc:141-178) of a diffuse GI ray times kd */
-template <bool UNI, int GATHER_LANES>
+template <bool UNI, int GATHER_LANES, bool KNN>
 __device__ __forceinline__ void gather_one(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
-                                           uint32_t i, int lane, float4 *__restrict__ direct)
+                                           uint32_t i, int lane, float4 *__restrict__ direct, uint32_t *hist)
 {
     float4 pos = greq[i].pos;
     uint32_t mode = __float_as_uint(pos.w);
@@ -739,9 +739,14 @@ __device__ __forceinline__ void gather_one(Tabs T, const QRay *__restrict__ q, c
     const ShadePmaps &PM = T.reqs[rq].PM;
     float4 nn = greq[i].nrm;
     GatherAcc a;
-    gather_grid_coop<GATHER_LANES>(mode == 1u ? PM.caust : PM.glob, pos.x, pos.y, pos.z, PM.radius,
-                                   mode == 1u ? SRAY_GATHER_IRRADIANCE : SRAY_GATHER_RADIANCE, (double)nn.x,
-                                   (double)nn.y, (double)nn.z, lane, a);
+    const PmapGrid &Gd = mode == 1u ? PM.caust : PM.glob;
+    const int gm = mode == 1u ? SRAY_GATHER_IRRADIANCE : SRAY_GATHER_RADIANCE;
+    if (KNN && PM.k > 0)
+        gather_grid_kcap<GATHER_LANES>(Gd, pos.x, pos.y, pos.z, PM.radius, gm, (double)nn.x, (double)nn.y,
+                                       (double)nn.z, lane, (uint32_t)PM.k, hist, a);
+    else
+        gather_grid_coop<GATHER_LANES>(Gd, pos.x, pos.y, pos.z, PM.radius, gm, (double)nn.x, (double)nn.y,
+                                       (double)nn.z, lane, a);
     if (lane != 0) return;
     float4 w = greq[i].w;
     float4 d = direct[i];
@@ -761,17 +766,22 @@ __device__ __forceinline__ void gather_one(Tabs T, const QRay *__restrict__ q, c
     direct[i] = d;
 }
 
-template <bool UNI, int GATHER_LANES>
+/* KNN: the instance for passes where a request gathers the k nearest
+ * (its selection code is left out of the other, whose registers it would
+ * raise) */
+template <bool UNI, int GATHER_LANES, bool KNN>
 __global__ __launch_bounds__(256) void k_gather(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
                                                 StepN sn, float4 *__restrict__ direct)
 {
+    /* a 256-bin histogram per group for the k-nearest selection */
+    __shared__ uint32_t hist[KNN ? 256 / GATHER_LANES : 1][256];
     const uint32_t n = step_n(sn);
     const uint32_t groups = gridDim.x * (blockDim.x / GATHER_LANES);
     const int lane = (int)(threadIdx.x % GATHER_LANES);
     /* grid-stride over the requests, one group of lanes each (whole groups
      * move and leave together) */
     for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / GATHER_LANES; i < n; i += groups)
-        gather_one<UNI, GATHER_LANES>(T, q, greq, i, lane, direct);
+        gather_one<UNI, GATHER_LANES, KNN>(T, q, greq, i, lane, direct, hist[KNN ? threadIdx.x / GATHER_LANES : 0]);
 }
 
 /* the sum of the run of equal sample slots starting at i, if i starts one */
@@ -1067,9 +1077,9 @@ auto trace_shadow_of(int bs, bool persist, bool sparse) -> decltype(&k_trace_sha
     }
     return persist ? trace_shadow_of_p<COUNT, UNI, true, false>(bs) : trace_shadow_of_p<COUNT, UNI, false, false>(bs);
 }
-template <bool UNI> auto gather_of(int lanes) -> decltype(&k_gather<UNI, 16>)
+template <bool UNI, bool KNN> auto gather_of(int lanes) -> decltype(&k_gather<UNI, 16, KNN>)
 {
-    return lanes == 8 ? k_gather<UNI, 8> : lanes == 32 ? k_gather<UNI, 32> : k_gather<UNI, 16>;
+    return lanes == 8 ? k_gather<UNI, 8, KNN> : lanes == 32 ? k_gather<UNI, 32, KNN> : k_gather<UNI, 16, KNN>;
 }
 
 } // namespace
@@ -1250,6 +1260,7 @@ struct Engine {
     /* this pass's launch shape */
     int L = 0, K = 2, maxiter = 0;
     bool photons = false, uni = true, counting = false;
+    bool knn = false; /* a request gathers the k nearest photons */
     bool one_scene = true; /* every request in flight renders the same scene */
     int Kb = 2;            /* most children one ray can spawn (growth bound of the levels) */
     bool more = true;      /* some request in flight can take another sample */
@@ -1443,6 +1454,7 @@ int admit(Engine &E, Job *j)
     /* gather distance relative to the scene octree diagonal (src/scene.cc:384-387) */
     double gabs = g->diag * (double)j->p.gather_dist;
     Q.PM.radius = (float)gabs;
+    Q.PM.k = j->p.gather_k > 0 ? j->p.gather_k : 0;
     Q.PM.scale = gabs > 0.0 ? 1.0 / (M_PI * (gabs * gabs)) : 0.0;
     for (int kind = 0; kind < 2; kind++) {
         bool want = kind == SRAY_PMAP_CAUSTIC ? Q.P.caust != 0 : Q.P.gi != 0;
@@ -1532,6 +1544,7 @@ int build_tables(Engine &E)
     E.attr = false;
     E.maxiter = 0;
     E.photons = false;
+    E.knn = false;
     E.more = false;
     if (!E.hst) {
         if (hipHostMalloc((void **)&E.hst, sizeof(Engine::PassStatus), hipHostMallocDefault) != hipSuccess ||
@@ -1558,6 +1571,7 @@ int build_tables(Engine &E)
         E.K = std::max(E.K, 2 + (Q.P.gi ? Q.P.diffuse_samples : 0));
         E.maxiter = std::max(E.maxiter, Q.P.iter);
         E.photons = E.photons || Q.P.gi || Q.P.caust;
+        E.knn = E.knn || ((Q.P.gi || Q.P.caust) && Q.PM.k > 0);
         E.more = E.more || Q.P.max_samples > 1;
         /* children a ray of this request can spawn: reflect / refract only
          * with a Phong material whose factor is > 0 (or any Phong material
@@ -1685,7 +1699,8 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
     HIPCHK(c, hipStreamWaitEvent(st2, ln.evA, 0));
     if (E.photons) {
         ProfScope ps(c, "k_gather", st2);
-        auto kern = E.uni ? gather_of<true>(E.gather_lanes) : gather_of<false>(E.gather_lanes);
+        auto kern = E.knn ? (E.uni ? gather_of<true, true>(E.gather_lanes) : gather_of<false, true>(E.gather_lanes))
+                          : (E.uni ? gather_of<true, false>(E.gather_lanes) : gather_of<false, false>(E.gather_lanes));
         const uint32_t g = std::min<uint32_t>(blocks((size_t)cap * E.gather_lanes, 256), 8192u);
         kern<<<g, 256, 0, st2>>>(LV.T, q, greq, sn, direct);
     }
@@ -1840,7 +1855,7 @@ int launch_chunk(Level &LV, uint32_t m, bool one_lane)
     std::vector<uint64_t> key = {LV.c0, m, (uint64_t)E.maxiter, (uint64_t)E.L, (uint64_t)E.K, E.chunk,
                                  (uint64_t)E.trace_block, (uint64_t)E.gather_lanes,
                                  (uint64_t)(E.photons | E.uni << 1 | E.one_scene << 2 | E.counting << 3 |
-                                            E.persistent << 4),
+                                            E.persistent << 4 | E.knn << 5),
                                  (uint64_t)E.Kb,
                                  /* the first level traced by the sparse instances */
                                  (uint64_t)std::min(E.sparse_from, E.maxiter + 1),

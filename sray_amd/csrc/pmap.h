@@ -187,6 +187,212 @@ __device__ __forceinline__ void gather_grid_coop(const PmapGrid &Gd, float qx, f
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* k nearest within the radius (PhotonMap::*_est's max_photons, unused */
+/* by the reference, src/pmap.h:59-60): the photons of the range gather */
+/* ordered by (dist^2, photon index), the first k kept                  */
+/* ------------------------------------------------------------------ */
+/* the photons of the query's cell rows, each lane of a group of G a
+ * strided share, passed to f(m, d2) when within the radius */
+template <int G, class F>
+__device__ __forceinline__ void grid_scan(const PmapGrid &Gd, float qx, float qy, float qz, float radius, int lane, F f)
+{
+    const double r2 = (double)radius * (double)radius;
+    float mx = 1e-5f * (fabsf(qx) + radius), my = 1e-5f * (fabsf(qy) + radius), mz = 1e-5f * (fabsf(qz) + radius);
+    int x0 = grid_cell(qx - radius - mx, Gd.ox, Gd.inv_cs, Gd.dx), x1 = grid_cell(qx + radius + mx, Gd.ox, Gd.inv_cs, Gd.dx);
+    int y0 = grid_cell(qy - radius - my, Gd.oy, Gd.inv_cs, Gd.dy), y1 = grid_cell(qy + radius + my, Gd.oy, Gd.inv_cs, Gd.dy);
+    int z0 = grid_cell(qz - radius - mz, Gd.oz, Gd.inv_cs, Gd.dz), z1 = grid_cell(qz + radius + mz, Gd.oz, Gd.inv_cs, Gd.dz);
+    for (int z = z0; z <= z1; z++)
+        for (int y = y0; y <= y1; y++) {
+            uint32_t row = ((uint32_t)z * (uint32_t)Gd.dy + (uint32_t)y) * (uint32_t)Gd.dx;
+            uint32_t b = Gd.cell_start[row + (uint32_t)x0], e = Gd.cell_start[row + (uint32_t)x1 + 1u];
+            for (uint32_t m = b + (uint32_t)lane; m < e; m += (uint32_t)G) {
+                float4 p = Gd.v.pos[m];
+                double ddx = (double)p.x - (double)qx, ddy = (double)p.y - (double)qy, ddz = (double)p.z - (double)qz;
+                double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (d2 <= r2) f(m, d2, __float_as_uint(p.w));
+            }
+        }
+}
+
+template <int G> __device__ __forceinline__ uint32_t group_sum(uint32_t x)
+{
+    for (int off = G / 2; off > 0; off >>= 1) x += (uint32_t)__shfl_xor((int)x, off, G);
+    return x;
+}
+
+/* LDS written by some lanes of this wave and read by others: every lane's
+ * operation is issued and completed in program order, so only the
+ * compiler's ordering needs holding */
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+/* the digit of the need-th smallest (1-based) of a group's 256-bin
+ * histogram: digit and the count of smaller digits */
+template <int G>
+__device__ __forceinline__ void hist_select(const uint32_t *hist, int lane, uint32_t need, uint32_t &digit,
+                                            uint32_t &below)
+{
+    constexpr int B = 256 / G;
+    uint32_t loc = 0;
+    for (int b = 0; b < B; b++) loc += hist[lane * B + b];
+    uint32_t incl = loc;
+    for (int d = 1; d < G; d <<= 1) {
+        uint32_t y = (uint32_t)__shfl_up((int)incl, d, G);
+        if (lane >= d) incl += y;
+    }
+    const uint32_t excl = incl - loc;
+    uint32_t dg = 0, bl = 0;
+    if (excl < need && need <= incl) {
+        uint32_t run = excl;
+        for (int b = 0; b < B; b++) {
+            const uint32_t h = hist[lane * B + b];
+            if (run + h >= need) {
+                dg = (uint32_t)(lane * B + b);
+                bl = run;
+                break;
+            }
+            run += h;
+        }
+    }
+    digit = group_sum<G>(dg); /* one lane holds the crossing */
+    below = group_sum<G>(bl);
+}
+
+/* clears a group's histogram */
+template <int G> __device__ __forceinline__ void hist_clear(uint32_t *hist, int lane)
+{
+    constexpr int B = 256 / G;
+    for (int b = 0; b < B; b++) hist[lane * B + b] = 0u;
+}
+
+/* The gather of the renderer's k-nearest mode by a group of G lanes:
+ * the range gather (the same sums, in the same order, as
+ * gather_grid_coop) and, when more than k photons are within the radius,
+ * the estimate over the k nearest by (dist^2, photon index).  Selection:
+ * a 32-bit key, dist^2 / r^2 in fixed point (monotone in dist^2), radix
+ * selected a byte at a time over a 256-bin histogram in hist (this
+ * group's LDS; the first byte's histogram is taken during the range
+ * gather), stopping as soon as the k-th photon's bin holds exactly the
+ * photons still needed; photons that share the k-th key go to an exact
+ * select of dist^2's double bits and then of the photon index.  Every
+ * lane returns the total. */
+template <int G>
+__device__ __forceinline__ void gather_grid_kcap(const PmapGrid &Gd, float qx, float qy, float qz, float radius,
+                                                 int mode, double nx, double ny, double nz, int lane, uint32_t k,
+                                                 uint32_t *hist, GatherAcc &a)
+{
+    a.r = a.g = a.b = 0.0;
+    a.count = 0;
+    if (Gd.v.n == 0) return;
+    const double r2 = (double)radius * (double)radius;
+    const double kscale = 4294967296.0 / r2;
+    auto key32 = [&](double d2) -> uint32_t {
+        const double f = d2 * kscale;
+        return f >= 4294967295.0 ? 0xffffffffu : (uint32_t)f;
+    };
+    hist_clear<G>(hist, lane);
+    wave_lds_sync();
+    grid_scan<G>(Gd, qx, qy, qz, radius, lane, [&](uint32_t m, double d2, uint32_t) {
+        gather_photon(Gd.v, m, mode, nx, ny, nz, a);
+        atomicAdd(&hist[key32(d2) >> 24], 1u);
+    });
+    wave_lds_sync();
+    for (int off = G / 2; off > 0; off >>= 1) {
+        a.r += __shfl_xor(a.r, off, G);
+        a.g += __shfl_xor(a.g, off, G);
+        a.b += __shfl_xor(a.b, off, G);
+        a.count += (uint32_t)__shfl_xor((int)a.count, off, G);
+    }
+    if (a.count <= k) return;
+    /* the k-th key, a byte at a time */
+    uint32_t need = k, prefix = 0, pmask = 0, digit, below;
+    bool whole = false;
+    for (int shift = 24;; shift -= 8) {
+        hist_select<G>(hist, lane, need, digit, below);
+        need -= below;
+        prefix |= digit << shift;
+        pmask |= 0xffu << shift;
+        const uint32_t in_bin = hist[digit];
+        wave_lds_sync();
+        if (in_bin == need) {
+            whole = true; /* every photon of the bin is taken */
+            break;
+        }
+        if (shift == 0) break; /* more photons share the k-th key than are needed */
+        hist_clear<G>(hist, lane);
+        wave_lds_sync();
+        grid_scan<G>(Gd, qx, qy, qz, radius, lane, [&](uint32_t, double d2, uint32_t) {
+            const uint32_t kk = key32(d2);
+            if ((kk & pmask) == prefix) atomicAdd(&hist[(kk >> (shift - 8)) & 255u], 1u);
+        });
+        wave_lds_sync();
+    }
+    /* the photons at the k-th key: need of them, by dist^2's bits, then
+     * by photon index */
+    uint64_t p64 = ~0ull;
+    uint32_t ilim = 0xffffffffu;
+    if (!whole) {
+        uint64_t pre = 0, msk = 0;
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            hist_clear<G>(hist, lane);
+            wave_lds_sync();
+            grid_scan<G>(Gd, qx, qy, qz, radius, lane, [&](uint32_t, double d2, uint32_t) {
+                const uint64_t kb = (uint64_t)__double_as_longlong(d2);
+                if (key32(d2) == prefix && (kb & msk) == pre) atomicAdd(&hist[(uint32_t)(kb >> shift) & 255u], 1u);
+            });
+            wave_lds_sync();
+            hist_select<G>(hist, lane, need, digit, below);
+            wave_lds_sync();
+            need -= below;
+            pre |= (uint64_t)digit << shift;
+            msk |= 0xffull << shift;
+        }
+        p64 = pre;
+        const uint32_t at_key = hist[digit];
+        wave_lds_sync();
+        if (need < at_key) {
+            uint32_t ipre = 0, imask = 0;
+            for (int shift = 24; shift >= 0; shift -= 8) {
+                hist_clear<G>(hist, lane);
+                wave_lds_sync();
+                grid_scan<G>(Gd, qx, qy, qz, radius, lane, [&](uint32_t, double d2, uint32_t idx) {
+                    if ((uint64_t)__double_as_longlong(d2) == p64 && (idx & imask) == ipre)
+                        atomicAdd(&hist[(idx >> shift) & 255u], 1u);
+                });
+                wave_lds_sync();
+                hist_select<G>(hist, lane, need, digit, below);
+                wave_lds_sync();
+                need -= below;
+                ipre |= digit << shift;
+                imask |= 0xffu << shift;
+            }
+            ilim = ipre;
+        }
+    }
+    a.r = a.g = a.b = 0.0;
+    a.count = 0;
+    grid_scan<G>(Gd, qx, qy, qz, radius, lane, [&](uint32_t m, double d2, uint32_t idx) {
+        const uint32_t kk = key32(d2);
+        bool take;
+        if (whole) take = (kk & pmask) <= prefix;
+        else {
+            const uint64_t kb = (uint64_t)__double_as_longlong(d2);
+            take = kk < prefix || (kk == prefix && (kb < p64 || (kb == p64 && idx <= ilim)));
+        }
+        if (take) gather_photon(Gd.v, m, mode, nx, ny, nz, a);
+    });
+    for (int off = G / 2; off > 0; off >>= 1) {
+        a.r += __shfl_xor(a.r, off, G);
+        a.g += __shfl_xor(a.g, off, G);
+        a.b += __shfl_xor(a.b, off, G);
+        a.count += (uint32_t)__shfl_xor((int)a.count, off, G);
+    }
+}
+
 } // namespace srd
 
 #endif

@@ -52,6 +52,7 @@ struct RenderDev {
 struct ShadePmaps {
     PmapGrid caust, glob;
     float radius; /* inclusion radius (float, as kd_nearest_range3f) */
+    int32_t k;    /* > 0: the k nearest within the radius */
     double scale; /* 1 / (pi r^2) with the double gather distance */
 };
 

@@ -331,16 +331,22 @@ __global__ __launch_bounds__(128) void k_pm_gather(PmapTree T, const sray_gather
 /* fixed-radius gather over the uniform grid (same found set and estimate),
  * 16 lanes per query */
 __global__ __launch_bounds__(256) void k_pm_gather_grid(PmapGrid G, const sray_gather_query *__restrict__ q,
-                                                        uint32_t n, float radius, int mode, float *__restrict__ rgb)
+                                                        uint32_t n, float radius, int mode, int k,
+                                                        float *__restrict__ rgb)
 {
+    __shared__ uint32_t hist[256 / 16][256];
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t i = t / 16u;
     int lane = (int)(t % 16u);
     if (i >= n) return;
     const sray_gather_query &Q = q[i];
     GatherAcc a;
-    gather_grid_coop<16>(G, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
-                         (double)Q.norm[2], lane, a);
+    if (k > 0)
+        gather_grid_kcap<16>(G, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
+                             (double)Q.norm[2], lane, (uint32_t)k, hist[threadIdx.x / 16], a);
+    else
+        gather_grid_coop<16>(G, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
+                             (double)Q.norm[2], lane, a);
     if (lane == 0) finish_estimate(a, mode, 1.0 / (M_PI * ((double)radius * (double)radius)), rgb + (size_t)i * 3);
 }
 
@@ -1035,7 +1041,6 @@ int do_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q
     mode &= ~SRAY_GATHER_GRID;
     if (mode != SRAY_GATHER_IRRADIANCE && mode != SRAY_GATHER_RADIANCE) return fail(c, SRAY_ERR_ARG, "bad gather mode");
     if (k < 0) return fail(c, SRAY_ERR_ARG, "k < 0");
-    if (grid && k > 0) return fail(c, SRAY_ERR_ARG, "the grid gather is fixed-radius (k must be 0)");
     if (!(radius > 0.0f)) return fail(c, SRAY_ERR_ARG, "radius must be > 0");
     if (n == 0) return SRAY_OK;
     const PmapTree &T = it->second->tree;
@@ -1049,7 +1054,7 @@ int do_pmap_gather(sray_gpu_ctx *c, uint32_t pmap_id, const sray_gather_query *q
         {
             ProfScope ps(c, "k_pm_gather_grid");
             k_pm_gather_grid<<<blocks((size_t)n * 16, 256), 256, 0, c->stream>>>(
-                G, dptr<sray_gather_query>(c->gq), n, radius, mode, dptr<float>(c->grgb));
+                G, dptr<sray_gather_query>(c->gq), n, radius, mode, k, dptr<float>(c->grgb));
         }
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(rgb, c->grgb.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));

@@ -113,3 +113,29 @@ def test_render_from_device_photons(ctx):
     assert e <= 1e-3
     assert st.rays_diffuse == ost.rays_diffuse
     ctx.release(sid)
+
+
+def test_render_with_k_nearest(ctx):
+    """gather_k: the k nearest photons within the radius in the render's
+    gathers (k = 200 as the photon room workload), against the checker's
+    render with the same k; a k no gather reaches renders the range
+    image bit for bit"""
+    hs = S.HostScene.generate(2, variant=10)
+    sid = ctx.upload(hs)
+    ctx.shoot_photons(sid, 20000, 20000, 300.0, 5, 0, 0, SEED)
+    ref = oracle.OracleScene(hs.desc_ptr(), 32)
+    ref.shoot_photons(S.photon_params(20000, 20000, 300.0, 5, 0, 0, SEED))
+    imgs = {}
+    for k in (0, 10, 200, 1 << 30):
+        p = S.render_params(96, 54, iter=3, caust_photons=20000, gi_photons=20000, diffuse_samples=2,
+                            gather_dist=0.08, seed=17, gather_k=k)
+        img, st = ctx.render(sid, p)
+        imgs[k] = img
+        if k in (10, 200):
+            oimg, ost = ref.render(p)
+            e = float(np.sqrt(np.mean((img.astype(np.float64) - oimg) ** 2)))
+            print(f"render with k = {k}: rmse {e:.3e}, mean {img.mean():.4f}", flush=True)
+            assert e <= 1e-3 and st.rays_diffuse == ost.rays_diffuse
+    assert np.array_equal(imgs[1 << 30], imgs[0])
+    assert not np.array_equal(imgs[10], imgs[0])  # the cap binds
+    ctx.release(sid)

@@ -110,21 +110,27 @@ def test_grid_gather_matches_checker(ctx, n, planar):
     S.lib().sray_gpu_pmap_release(ctx._h, pid)
 
 
+@pytest.mark.parametrize("grid", [0, S.abi.SRAY_GATHER_GRID], ids=["kdtree", "grid"])
 @pytest.mark.parametrize("k", [1, 8, 20, 200])
-def test_knn_gather_tie_break_by_index(ctx, k):
+def test_knn_gather_tie_break_by_index(ctx, k, grid):
+    """the k nearest by (dist^2, photon index): the kd-tree's quickselect
+    and the grid's radix select of the renderer's gathers"""
     rng = np.random.default_rng(k)
     ph = _photons(rng, 50_000)
     # duplicated positions force exact distance ties
     ph[25_000:30_000, 0:3] = ph[0:5_000, 0:3]
+    # 600 photons on one point: more than k tie at every distance
+    ph[40_000:40_600, 0:3] = ph[7, 0:3]
     q = _queries(rng, 3000)
     q[:200, 0:3] = ph[:200, 0:3]
+    q[200:260, 0:3] = ph[7, 0:3] + rng.normal(0, 0.01, (60, 3))
     pid = _upload(ctx, ph)
     b = build()
     ref = oracle.OracleScene(b.ptr(), 64)
     ref.set_photons(0, ph)
     for radius in (0.05, 0.2):
         for mode in (0, 1):
-            got = _gpu_gather(ctx, pid, q, radius, k, mode)
+            got = _gpu_gather(ctx, pid, q, radius, k, mode | grid)
             want = ref.gather(0, q, radius, k, mode)
             assert np.allclose(got, want, rtol=RTOL, atol=ATOL), (k, radius, mode, np.abs(got - want).max())
     S.lib().sray_gpu_pmap_release(ctx._h, pid)
