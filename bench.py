@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--rate", type=float, default=None, help="serve: fixed Poisson rate (req/s) instead")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device three times (scales the HBM byte counters)")
+    ap.add_argument("--gather-k", type=int, default=200,
+                    help="room / serve: photons per gather estimate (0: every photon within the radius)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -457,11 +459,11 @@ def run_spheres(args):
                       "image": rm})
 
 
-def room_params(W, H, diag, seed):
+def room_params(W, H, diag, seed, gather_k=200):
     import sray_amd as S
     return S.render_params(W, H, samples=(1, 16), max_var=0.05, iter=5, shadow_samples=4, diffuse_samples=1,
                            caust_photons=500_000, gi_photons=2_000_000, gather_dist=0.1 / diag, seed=seed,
-                           gather_k=200)
+                           gather_k=gather_k)
 
 
 def run_room(args):
@@ -482,7 +484,7 @@ def run_room(args):
     log(args, f"photons: shot {pst.shot[0]}+{pst.shot[1]}, stored {pst.stored[0]}+{pst.stored[1]}, "
               f"{pst.segments} segments, trace {pst.trace_ms:.1f} ms, kd build {pst.build_ms:.1f} ms")
     K = args.frames or 8
-    frames = [room_params(W, H, diag, 100 + k) for k in range(K)]
+    frames = [room_params(W, H, diag, 100 + k, args.gather_k) for k in range(K)]
     out = ctx.malloc(W * H * 16)
     ctx.render_device(sid, frames[0], out)  # warm-up (allocations)
     ctx.synchronize()
@@ -503,7 +505,7 @@ def run_room(args):
         cpu = cpu_rows(args, cpu_checker(hs.desc_ptr(), photons), frames[0], "room")
     return base_line("Mrays/s on 1 GPU (batched)", batched, K, 1, batch_wall * 1e3 / K,
                      f"room: 12 quads + 2 glass spheres + 50k-triangle icosphere, box light, {W}x{H}, 1-16 spp "
-                     f"(variance 0.05), 2M+500k photons, k=200 nearest within 0.1, {K} frames",
+                     f"(variance 0.05), 2M+500k photons, k={args.gather_k} nearest within 0.1, {K} frames",
                      {"batched": {"value": round(batched, 3), "frames_in_flight": K, "wall_s": round(batch_wall, 3),
                                   "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)},
                       "plain": {"value": round(plain, 3), "wall_s": round(plain_wall, 3)},
@@ -612,7 +614,7 @@ def run_serve(args):
         v /= np.linalg.norm(v)
         cam = look_at(np.array([0.0, 4.0, 0.0]) + 3.5 * v, (0.0, 4.0, 0.0), 60.0)
         p = S.render_params(w, h, samples=(1, 16), max_var=0.05, iter=5, shadow_samples=4, caust_photons=1,
-                            gi_photons=1, gather_dist=0.1 / diags[sc], seed=k, gather_k=200)
+                            gi_photons=1, gather_dist=0.1 / diags[sc], seed=k, gather_k=args.gather_k)
         return (sids[sc], p, cam)
 
     # capacity: the completion rate of a stream of the same mix offered

@@ -728,26 +728,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ATTR || LMAX
 /* photon gathers requested by k_shade: caustic irradiance (>= 8 photons,
  * src/pmap.cc:109-139) added to the direct term, or the radiance estimate
  * (src/pmap.cc:141-178) of a diffuse GI ray times kd */
-template <bool UNI, int GATHER_LANES, bool KNN>
-__device__ __forceinline__ void gather_one(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
-                                           uint32_t i, int lane, float4 *__restrict__ direct, uint32_t *hist)
+/* the estimate of request i into its direct term: mode 1 adds
+ * W * caustic irradiance (no estimate from fewer than 8 photons), mode 2
+ * sets W * (GI radiance * kd) */
+__device__ __forceinline__ void gather_finish(const GatherReq *__restrict__ greq, uint32_t i, uint32_t mode,
+                                              const ShadePmaps &PM, const GatherAcc &a, float4 *__restrict__ direct)
 {
-    float4 pos = greq[i].pos;
-    uint32_t mode = __float_as_uint(pos.w);
-    if (mode == 0u) return;
-    uint32_t rq = UNI ? 0u : q[i].m.w >> SLOT_SHIFT;
-    const ShadePmaps &PM = T.reqs[rq].PM;
-    float4 nn = greq[i].nrm;
-    GatherAcc a;
-    const PmapGrid &Gd = mode == 1u ? PM.caust : PM.glob;
-    const int gm = mode == 1u ? SRAY_GATHER_IRRADIANCE : SRAY_GATHER_RADIANCE;
-    if (KNN && PM.k > 0)
-        gather_grid_kcap<GATHER_LANES>(Gd, pos.x, pos.y, pos.z, PM.radius, gm, (double)nn.x, (double)nn.y,
-                                       (double)nn.z, lane, (uint32_t)PM.k, hist, a);
-    else
-        gather_grid_coop<GATHER_LANES>(Gd, pos.x, pos.y, pos.z, PM.radius, gm, (double)nn.x, (double)nn.y,
-                                       (double)nn.z, lane, a);
-    if (lane != 0) return;
     float4 w = greq[i].w;
     float4 d = direct[i];
     if (mode == 1u) {
@@ -766,22 +752,83 @@ __device__ __forceinline__ void gather_one(Tabs T, const QRay *__restrict__ q, c
     direct[i] = d;
 }
 
-/* KNN: the instance for passes where a request gathers the k nearest
- * (its selection code is left out of the other, whose registers it would
- * raise) */
+/* KNN: a request with gather_k > 0 whose range holds more than k photons
+ * is listed in knn (count at knn[0], then (index, photons in range)
+ * pairs) for k_gather_knn, which writes its estimate */
+template <bool UNI, int GATHER_LANES, bool KNN>
+__device__ __forceinline__ void gather_one(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
+                                           uint32_t i, int lane, float4 *__restrict__ direct, uint32_t *__restrict__ knn)
+{
+    float4 pos = greq[i].pos;
+    uint32_t mode = __float_as_uint(pos.w);
+    if (mode == 0u) return;
+    uint32_t rq = UNI ? 0u : q[i].m.w >> SLOT_SHIFT;
+    const ShadePmaps &PM = T.reqs[rq].PM;
+    float4 nn = greq[i].nrm;
+    GatherAcc a;
+    gather_grid_coop<GATHER_LANES>(mode == 1u ? PM.caust : PM.glob, pos.x, pos.y, pos.z, PM.radius,
+                                   mode == 1u ? SRAY_GATHER_IRRADIANCE : SRAY_GATHER_RADIANCE, (double)nn.x,
+                                   (double)nn.y, (double)nn.z, lane, a);
+    if (lane != 0) return;
+    if (KNN && PM.k > 0 && a.count > (uint32_t)PM.k) {
+        const uint32_t j = atomicAdd(knn, 1u);
+        knn[1u + 2u * j] = i;
+        knn[2u + 2u * j] = a.count;
+        return;
+    }
+    gather_finish(greq, i, mode, PM, a, direct);
+}
+
+/* KNN: the instance for passes where a request gathers the k nearest */
 template <bool UNI, int GATHER_LANES, bool KNN>
 __global__ __launch_bounds__(256) void k_gather(Tabs T, const QRay *__restrict__ q, const GatherReq *__restrict__ greq,
-                                                StepN sn, float4 *__restrict__ direct)
+                                                StepN sn, float4 *__restrict__ direct, uint32_t *__restrict__ knn)
 {
-    /* a 256-bin histogram per group for the k-nearest selection */
-    __shared__ uint32_t hist[KNN ? 256 / GATHER_LANES : 1][256];
     const uint32_t n = step_n(sn);
     const uint32_t groups = gridDim.x * (blockDim.x / GATHER_LANES);
     const int lane = (int)(threadIdx.x % GATHER_LANES);
     /* grid-stride over the requests, one group of lanes each (whole groups
      * move and leave together) */
     for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / GATHER_LANES; i < n; i += groups)
-        gather_one<UNI, GATHER_LANES, KNN>(T, q, greq, i, lane, direct, hist[KNN ? threadIdx.x / GATHER_LANES : 0]);
+        gather_one<UNI, GATHER_LANES, KNN>(T, q, greq, i, lane, direct, knn);
+}
+
+/* the requests k_gather listed: the estimate over the k nearest photons
+ * (gather_grid_kcap), a group of lanes each.  The k nearest are looked
+ * for first within r sqrt(1.5 k / n), n the photons within r (the radius
+ * that holds 1.5 k photons where their density is even, photons lying on
+ * surfaces), then, if that holds fewer than k, within r. */
+template <bool UNI, int GATHER_LANES>
+__global__ __launch_bounds__(256) void k_gather_knn(Tabs T, const QRay *__restrict__ q,
+                                                    const GatherReq *__restrict__ greq,
+                                                    const uint32_t *__restrict__ knn, float4 *__restrict__ direct)
+{
+    __shared__ KcapLds<GATHER_LANES> lds[256 / GATHER_LANES];
+    const uint32_t n = knn[0];
+    const uint32_t groups = gridDim.x * (blockDim.x / GATHER_LANES);
+    const int lane = (int)(threadIdx.x % GATHER_LANES);
+    KcapLds<GATHER_LANES> &L = lds[threadIdx.x / GATHER_LANES];
+    for (uint32_t j = (blockIdx.x * blockDim.x + threadIdx.x) / GATHER_LANES; j < n; j += groups) {
+        const uint32_t i = knn[1u + 2u * j], inr = knn[2u + 2u * j];
+        float4 pos = greq[i].pos;
+        uint32_t mode = __float_as_uint(pos.w);
+        uint32_t rq = UNI ? 0u : q[i].m.w >> SLOT_SHIFT;
+        const ShadePmaps &PM = T.reqs[rq].PM;
+        float4 nn = greq[i].nrm;
+        const PmapGrid &Gd = mode == 1u ? PM.caust : PM.glob;
+        const int gm = mode == 1u ? SRAY_GATHER_IRRADIANCE : SRAY_GATHER_RADIANCE;
+        GatherAcc a;
+        const float f = 1.5f * (float)PM.k / (float)inr;
+        bool done = false;
+        if (f < 0.8f)
+            done = gather_grid_kcap<GATHER_LANES, false>(Gd, pos.x, pos.y, pos.z, PM.radius * sqrtf(f), gm,
+                                                         (double)nn.x, (double)nn.y, (double)nn.z, lane,
+                                                         (uint32_t)PM.k, L, a);
+        if (!done)
+            gather_grid_kcap<GATHER_LANES, false>(Gd, pos.x, pos.y, pos.z, PM.radius, gm, (double)nn.x, (double)nn.y,
+                                                  (double)nn.z, lane, (uint32_t)PM.k, L, a);
+        if (lane == 0) gather_finish(greq, i, mode, PM, a, direct);
+    }
 }
 
 /* the sum of the run of equal sample slots starting at i, if i starts one */
@@ -815,8 +862,10 @@ __device__ __forceinline__ void accum_run(const uint32_t *__restrict__ rslot, co
  * sums the run (queues are ordered by slot) */
 __global__ void k_accum(const uint32_t *__restrict__ rslot, const float4 *__restrict__ direct,
                         const float4 *__restrict__ scon, const uint8_t *__restrict__ occ, StepN sn, int L,
-                        float4 *__restrict__ acc)
+                        float4 *__restrict__ acc, uint32_t *__restrict__ knn)
 {
+    /* the step's k-nearest list is spent: empty it for the next step */
+    if (knn && blockIdx.x == 0 && threadIdx.x == 0) knn[0] = 0u;
     const uint32_t n = step_n(sn);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         accum_run(rslot, direct, scon, occ, n, L, acc, i);
@@ -1077,6 +1126,10 @@ auto trace_shadow_of(int bs, bool persist, bool sparse) -> decltype(&k_trace_sha
     }
     return persist ? trace_shadow_of_p<COUNT, UNI, true, false>(bs) : trace_shadow_of_p<COUNT, UNI, false, false>(bs);
 }
+template <bool UNI> auto gather_knn_of(int lanes) -> decltype(&k_gather_knn<UNI, 16>)
+{
+    return lanes == 8 ? k_gather_knn<UNI, 8> : lanes == 32 ? k_gather_knn<UNI, 32> : k_gather_knn<UNI, 16>;
+}
 template <bool UNI, bool KNN> auto gather_of(int lanes) -> decltype(&k_gather<UNI, 16, KNN>)
 {
     return lanes == 8 ? k_gather<UNI, 8, KNN> : lanes == 32 ? k_gather<UNI, 32, KNN> : k_gather<UNI, 16, KNN>;
@@ -1229,12 +1282,13 @@ struct Engine {
          * can be shaded while the previous one's shadow rays are traced */
         uint64_t pstride = 0;
         std::vector<DevBuf> qlev;
-        DevBuf hits, direct, sq, scon, rslot, occ, kids, kflag, acc, greq;
+        DevBuf hits, direct, sq, scon, rslot, occ, kids, kflag, acc, greq, knnq;
         DevBuf lvcnt, cmpb; /* per-level ray counts; compaction block counts / offsets */
         uint64_t bytes() const
         {
             uint64_t b = 0;
-            const DevBuf *bs[] = {&hits, &direct, &sq, &scon, &rslot, &occ, &kids, &kflag, &acc, &greq, &lvcnt, &cmpb};
+            const DevBuf *bs[] = {&hits, &direct, &sq, &scon, &rslot, &occ, &kids, &kflag, &acc, &greq, &knnq,
+                                  &lvcnt, &cmpb};
             for (const DevBuf *d : bs) b += d->bytes;
             for (const DevBuf &d : qlev) b += d.bytes;
             return b;
@@ -1654,6 +1708,7 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
     uint32_t *rslot = dptr<uint32_t>(ln.rslot) + po;
     uint8_t *occ = dptr<uint8_t>(ln.occ) + pl;
     GatherReq *greq = E.photons ? dptr<GatherReq>(ln.greq) + po : nullptr;
+    uint32_t *knn = E.photons && E.knn ? dptr<uint32_t>(ln.knnq) : nullptr;
     const int bs = E.trace_block;
     const uint32_t small_grid = std::min<uint32_t>(blocks(cap, 256), 2048u);
     /* traversal grid: one run of 64 rays per wave, or the resident waves */
@@ -1702,7 +1757,14 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
         auto kern = E.knn ? (E.uni ? gather_of<true, true>(E.gather_lanes) : gather_of<false, true>(E.gather_lanes))
                           : (E.uni ? gather_of<true, false>(E.gather_lanes) : gather_of<false, false>(E.gather_lanes));
         const uint32_t g = std::min<uint32_t>(blocks((size_t)cap * E.gather_lanes, 256), 8192u);
-        kern<<<g, 256, 0, st2>>>(LV.T, q, greq, sn, direct);
+        kern<<<g, 256, 0, st2>>>(LV.T, q, greq, sn, direct, knn);
+    }
+    if (E.photons && E.knn) {
+        /* the requests with more than k photons in range (a few percent) */
+        ProfScope ps(c, "k_gather_knn", st2);
+        auto kern = E.uni ? gather_knn_of<true>(E.gather_lanes) : gather_knn_of<false>(E.gather_lanes);
+        const uint32_t g = std::min<uint32_t>(blocks((size_t)cap * E.gather_lanes, 256), 1024u);
+        kern<<<g, 256, 0, st2>>>(LV.T, q, greq, knn, direct);
     }
     if (E.L) {
         ProfScope ps(c, "k_trace_shadow", st2);
@@ -1716,7 +1778,7 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
     {
         ProfScope ps(c, "k_accum", st2);
         k_accum<<<small_grid, 256, 0, st2>>>(rslot, direct, scon, occ, sn, E.L,
-                                                    dptr<float4>(ln.acc));
+                                                    dptr<float4>(ln.acc), knn);
     }
     HIPCHK(c, hipEventRecord(ln.evB[par], st2));
     HIPCHK(c, hipGetLastError());
@@ -1863,7 +1925,7 @@ int launch_chunk(Level &LV, uint32_t m, bool one_lane)
                                  (uint64_t)(uintptr_t)LV.T.preq, (uint64_t)(uintptr_t)LV.T.nsamp,
                                  (uint64_t)(uintptr_t)LV.active};
     const DevBuf *bufs[] = {&ln.hits, &ln.direct, &ln.sq, &ln.scon, &ln.rslot, &ln.occ, &ln.kids, &ln.kflag, &ln.acc,
-                            &E.stats, &E.errf, &E.rcounts, &E.slotrays, &ln.greq, &ln.lvcnt, &ln.cmpb};
+                            &E.stats, &E.errf, &E.rcounts, &E.slotrays, &ln.greq, &ln.knnq, &ln.lvcnt, &ln.cmpb};
     for (const DevBuf *d : bufs) key.push_back((uint64_t)(uintptr_t)d->p);
     key.push_back(ln.pstride);
     for (int b = 0; b <= E.maxiter; b++) key.push_back((uint64_t)(uintptr_t)ln.qlev[b].p);
@@ -1970,8 +2032,11 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
             (rc = ensure(c, ln.kids, (size_t)cap_rays * E.K * sizeof(QRay))) ||
             (rc = ensure(c, ln.kflag, (size_t)cap_rays * E.K)) || (rc = ensure(c, ln.acc, (size_t)mchunk * 16)) ||
             (E.photons && (rc = ensure(c, ln.greq, 2 * (size_t)cap_rays * sizeof(GatherReq)))) ||
+            (E.photons && E.knn && (rc = ensure(c, ln.knnq, (2 * (size_t)cap_rays + 1) * 4))) ||
             (rc = ensure(c, ln.lvcnt, 64 * 4)) || (rc = ensure(c, ln.cmpb, 2 * CMP_BLOCKS * 4)))
             return rc;
+        /* the k-nearest list starts empty (each step's k_accum empties it) */
+        if (E.photons && E.knn) HIPCHK(c, hipMemsetAsync(ln.knnq.p, 0, 4, c->stream));
         ln.pstride = cap_rays;
     }
 
@@ -2105,7 +2170,7 @@ void engine_free(Engine *E)
         for (hipEvent_t ev : ln.evB)
             if (ev) (void)hipEventDestroy(ev);
         DevBuf *lb[] = {&ln.hits, &ln.direct, &ln.sq, &ln.scon, &ln.rslot, &ln.occ, &ln.kids, &ln.kflag, &ln.acc,
-                        &ln.greq, &ln.lvcnt, &ln.cmpb};
+                        &ln.greq, &ln.knnq, &ln.lvcnt, &ln.cmpb};
         for (DevBuf *b : lb) release(*b);
         for (DevBuf &b : ln.qlev) release(b);
     }
@@ -2389,7 +2454,7 @@ int check_tuning(sray_gpu_ctx *c, const sray_tuning &t)
     bool ok = (t.trace_block == 64 || t.trace_block == 128 || t.trace_block == 256) &&
               (t.tile_w == 8 || t.tile_w == 16 || t.tile_w == 32 || t.tile_w == 64) &&
               (t.gather_lanes == 8 || t.gather_lanes == 16 || t.gather_lanes == 32) && t.chunk_log2 >= 16 &&
-              t.chunk_log2 <= 21 && (t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f) &&
+              t.chunk_log2 <= 21 && (t.gather_cell == 0.25f || t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f) &&
               (t.graphs == 0 || t.graphs == 1) && (t.persistent == 0 || t.persistent == 1) &&
               (t.lanes == 1 || t.lanes == 2 || t.lanes == 4) && t.sparse_from >= 0 && t.sparse_from <= 64 &&
               (t.chain == 0 || t.chain == 1);
@@ -2428,12 +2493,12 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
         return rc;
     }
     static const int blocks_[] = {64, 128, 256}, tiles[] = {8, 16, 32, 64}, lanes[] = {8, 16, 32}, chunks[] = {18, 19, 20, 21};
-    static const float cells[] = {0.5f, 1.0f, 2.0f};
+    static const float cells[] = {0.25f, 0.5f, 1.0f, 2.0f};
     bool photons = p->caust_photons || p->gi_photons;
     static const int nlanes[] = {1, 2, 4}, sparse[] = {0, 1, 2, 64};
     for (int knob = 0; knob < 10; knob++) {
         if ((knob == 2 || knob == 4) && !photons) continue;
-        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 3 : knob == 7 ? 3
+        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 4 : knob == 7 ? 3
                : knob == 8 ? 4 : 2;
         for (int k = 0; k < nv; k++) {
             sray_tuning t = cur;

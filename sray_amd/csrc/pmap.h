@@ -269,60 +269,120 @@ template <int G> __device__ __forceinline__ void hist_clear(uint32_t *hist, int 
     for (int b = 0; b < B; b++) hist[lane * B + b] = 0u;
 }
 
+/* a group's LDS for the k-nearest selection: the histogram and the list
+ * of the photons in the k-th photon's first-byte bin */
+template <int G> struct KcapLds {
+    static constexpr uint32_t CAP = 4 * G;
+    uint32_t hist[256];
+    uint32_t cnt, pad_[3];
+    double d2[CAP];
+    uint32_t idx[CAP], m[CAP];
+};
+
 /* The gather of the renderer's k-nearest mode by a group of G lanes:
  * the range gather (the same sums, in the same order, as
  * gather_grid_coop) and, when more than k photons are within the radius,
  * the estimate over the k nearest by (dist^2, photon index).  Selection:
- * a 32-bit key, dist^2 / r^2 in fixed point (monotone in dist^2), radix
- * selected a byte at a time over a 256-bin histogram in hist (this
- * group's LDS; the first byte's histogram is taken during the range
- * gather), stopping as soon as the k-th photon's bin holds exactly the
- * photons still needed; photons that share the k-th key go to an exact
- * select of dist^2's double bits and then of the photon index.  Every
- * lane returns the total. */
-template <int G>
-__device__ __forceinline__ void gather_grid_kcap(const PmapGrid &Gd, float qx, float qy, float qz, float radius,
+ * a 32-bit key, dist^2 / r^2 in fixed point (monotone in dist^2), with a
+ * 256-bin histogram of its first byte taken during the range gather.
+ * The k-th photon's bin is then, most often, small: one more scan sums
+ * the photons of the bins below it and lists the bin's photons in LDS,
+ * which are ranked by (dist^2, index) there.  A larger bin is radix
+ * selected a byte at a time, stopping as soon as the k-th photon's bin
+ * holds exactly the photons still needed; photons that share the k-th key
+ * go to an exact select of dist^2's double bits and then of the photon
+ * index.  Every lane returns the total.  SUMS0 = false: the first scan
+ * only counts and takes the histogram, and a radius holding fewer than k
+ * photons returns false with no estimate (a query known to hold more than
+ * k photons in a larger radius looks first in a smaller one, where its k
+ * nearest most often are: every photon outside a radius is farther than
+ * every photon inside it). */
+template <int G, bool SUMS0 = true>
+__device__ __forceinline__ bool gather_grid_kcap(const PmapGrid &Gd, float qx, float qy, float qz, float radius,
                                                  int mode, double nx, double ny, double nz, int lane, uint32_t k,
-                                                 uint32_t *hist, GatherAcc &a)
+                                                 KcapLds<G> &L, GatherAcc &a)
 {
+    uint32_t *hist = L.hist;
     a.r = a.g = a.b = 0.0;
     a.count = 0;
-    if (Gd.v.n == 0) return;
+    if (Gd.v.n == 0) return true;
     const double r2 = (double)radius * (double)radius;
     const double kscale = 4294967296.0 / r2;
     auto key32 = [&](double d2) -> uint32_t {
         const double f = d2 * kscale;
         return f >= 4294967295.0 ? 0xffffffffu : (uint32_t)f;
     };
+    auto reduce = [&]() {
+        for (int off = G / 2; off > 0; off >>= 1) {
+            a.r += __shfl_xor(a.r, off, G);
+            a.g += __shfl_xor(a.g, off, G);
+            a.b += __shfl_xor(a.b, off, G);
+            a.count += (uint32_t)__shfl_xor((int)a.count, off, G);
+        }
+    };
     hist_clear<G>(hist, lane);
     wave_lds_sync();
     grid_scan<G>(Gd, qx, qy, qz, radius, lane, [&](uint32_t m, double d2, uint32_t) {
-        gather_photon(Gd.v, m, mode, nx, ny, nz, a);
+        if (SUMS0) gather_photon(Gd.v, m, mode, nx, ny, nz, a);
+        else a.count++;
         atomicAdd(&hist[key32(d2) >> 24], 1u);
     });
     wave_lds_sync();
-    for (int off = G / 2; off > 0; off >>= 1) {
-        a.r += __shfl_xor(a.r, off, G);
-        a.g += __shfl_xor(a.g, off, G);
-        a.b += __shfl_xor(a.b, off, G);
-        a.count += (uint32_t)__shfl_xor((int)a.count, off, G);
-    }
-    if (a.count <= k) return;
-    /* the k-th key, a byte at a time */
-    uint32_t need = k, prefix = 0, pmask = 0, digit, below;
-    bool whole = false;
-    for (int shift = 24;; shift -= 8) {
-        hist_select<G>(hist, lane, need, digit, below);
-        need -= below;
-        prefix |= digit << shift;
-        pmask |= 0xffu << shift;
-        const uint32_t in_bin = hist[digit];
-        wave_lds_sync();
-        if (in_bin == need) {
-            whole = true; /* every photon of the bin is taken */
-            break;
+    reduce();
+    if (!SUMS0 && a.count < k) return false;
+    if (a.count <= k) {
+        if (!SUMS0) {
+            a.r = a.g = a.b = 0.0;
+            a.count = 0;
+            grid_scan<G>(Gd, qx, qy, qz, radius, lane,
+                         [&](uint32_t m, double, uint32_t) { gather_photon(Gd.v, m, mode, nx, ny, nz, a); });
+            reduce();
         }
-        if (shift == 0) break; /* more photons share the k-th key than are needed */
+        return true;
+    }
+    /* the first byte of the k-th key */
+    uint32_t need = k, digit, below;
+    hist_select<G>(hist, lane, need, digit, below);
+    need -= below;
+    const uint32_t in_bin0 = hist[digit];
+    wave_lds_sync();
+    if (in_bin0 <= KcapLds<G>::CAP) {
+        if (lane == 0) L.cnt = 0u;
+        wave_lds_sync();
+        const bool all = in_bin0 == need;
+        a.r = a.g = a.b = 0.0;
+        a.count = 0;
+        grid_scan<G>(Gd, qx, qy, qz, radius, lane, [&](uint32_t m, double d2, uint32_t idx) {
+            const uint32_t kb = key32(d2) >> 24;
+            if (kb < digit || (kb == digit && all)) {
+                gather_photon(Gd.v, m, mode, nx, ny, nz, a);
+            } else if (kb == digit) {
+                const uint32_t sl = atomicAdd(&L.cnt, 1u);
+                L.d2[sl] = d2;
+                L.idx[sl] = idx;
+                L.m[sl] = m;
+            }
+        });
+        wave_lds_sync();
+        if (!all)
+            for (uint32_t e = (uint32_t)lane; e < in_bin0; e += (uint32_t)G) {
+                const double de = L.d2[e];
+                const uint32_t ie = L.idx[e];
+                uint32_t rank = 0;
+                for (uint32_t f = 0; f < in_bin0; f++) {
+                    const double df = L.d2[f];
+                    rank += (df < de || (df == de && L.idx[f] < ie)) ? 1u : 0u;
+                }
+                if (rank < need) gather_photon(Gd.v, L.m[e], mode, nx, ny, nz, a);
+            }
+        wave_lds_sync();
+        reduce();
+        return true;
+    }
+    /* a large bin: the rest of the k-th key, a byte at a time */
+    uint32_t prefix = digit << 24, pmask = 0xff000000u;
+    bool whole = in_bin0 == need; /* every photon of the bin is taken */
+    for (int shift = 24; !whole && shift > 0;) {
         hist_clear<G>(hist, lane);
         wave_lds_sync();
         grid_scan<G>(Gd, qx, qy, qz, radius, lane, [&](uint32_t, double d2, uint32_t) {
@@ -330,6 +390,14 @@ __device__ __forceinline__ void gather_grid_kcap(const PmapGrid &Gd, float qx, f
             if ((kk & pmask) == prefix) atomicAdd(&hist[(kk >> (shift - 8)) & 255u], 1u);
         });
         wave_lds_sync();
+        shift -= 8;
+        hist_select<G>(hist, lane, need, digit, below);
+        need -= below;
+        prefix |= digit << shift;
+        pmask |= 0xffu << shift;
+        const uint32_t in_bin = hist[digit];
+        wave_lds_sync();
+        whole = in_bin == need;
     }
     /* the photons at the k-th key: need of them, by dist^2's bits, then
      * by photon index */
@@ -385,12 +453,8 @@ __device__ __forceinline__ void gather_grid_kcap(const PmapGrid &Gd, float qx, f
         }
         if (take) gather_photon(Gd.v, m, mode, nx, ny, nz, a);
     });
-    for (int off = G / 2; off > 0; off >>= 1) {
-        a.r += __shfl_xor(a.r, off, G);
-        a.g += __shfl_xor(a.g, off, G);
-        a.b += __shfl_xor(a.b, off, G);
-        a.count += (uint32_t)__shfl_xor((int)a.count, off, G);
-    }
+    reduce();
+    return true;
 }
 
 } // namespace srd

@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_pm_gather_grid(PmapGrid G, const sray_g
                                                         uint32_t n, float radius, int mode, int k,
                                                         float *__restrict__ rgb)
 {
-    __shared__ uint32_t hist[256 / 16][256];
+    __shared__ KcapLds<16> lds[256 / 16];
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t i = t / 16u;
     int lane = (int)(t % 16u);
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void k_pm_gather_grid(PmapGrid G, const sray_g
     GatherAcc a;
     if (k > 0)
         gather_grid_kcap<16>(G, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
-                             (double)Q.norm[2], lane, (uint32_t)k, hist[threadIdx.x / 16], a);
+                             (double)Q.norm[2], lane, (uint32_t)k, lds[threadIdx.x / 16], a);
     else
         gather_grid_coop<16>(G, Q.pos[0], Q.pos[1], Q.pos[2], radius, mode, (double)Q.norm[0], (double)Q.norm[1],
                              (double)Q.norm[2], lane, a);
