@@ -8,8 +8,9 @@
  * The reference walks every node whose box meets the segment, children in
  * fixed order 0..7, and keeps the first occurrence of the smallest t (strict
  * '<').  Here children are visited near-to-far (octant bits XOR the ray's
- * sign mask), a node is skipped when its entry distance is strictly
- * greater than the best t found so far, and candidates are ranked by the
+ * sign mask), a node is skipped when its entry distance exceeds the best
+ * t found so far (by a small relative slack, below), and candidates are
+ * ranked by the
  * pair (t, occurrence rank) — the rank being the item's position in the
  * depth-first item array.
  *
