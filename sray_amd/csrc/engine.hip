@@ -875,77 +875,21 @@ __global__ void k_accum(const uint32_t *__restrict__ rslot, const float4 *__rest
 /* stable compaction of a level step's flagged children into the next   */
 /* level's queue, with the count read and written on the device         */
 /* ------------------------------------------------------------------ */
-constexpr uint32_t CMP_BLOCKS = 1024, CMP_THREADS = 256;
-
-/* block b's items: an equal share of whole 256-item tiles, in order */
-__device__ __forceinline__ void cmp_range(uint32_t items, uint32_t b, uint32_t &lo, uint32_t &hi)
-{
-    const uint32_t tiles = (items + CMP_THREADS - 1u) / CMP_THREADS;
-    const uint32_t per = (tiles + CMP_BLOCKS - 1u) / CMP_BLOCKS;
-    lo = min(b * per * CMP_THREADS, items);
-    hi = min((b + 1u) * per * CMP_THREADS, items);
-}
-
-/* pass 1: flagged items per block */
+/* pass 1: flagged children per block (devsort.h) */
 __global__ __launch_bounds__(CMP_THREADS) void k_cmp_count(const uint8_t *__restrict__ flags, StepN sn, int K,
                                                            uint32_t *__restrict__ bcount)
 {
-    __shared__ uint32_t wsum[CMP_THREADS / 64];
-    const uint32_t items = step_n(sn) * (uint32_t)K;
-    uint32_t lo, hi;
-    cmp_range(items, blockIdx.x, lo, hi);
-    uint32_t c = 0;
-    for (uint32_t j = lo + threadIdx.x; j < hi; j += CMP_THREADS) c += flags[j] ? 1u : 0u;
-    for (int off = 32; off > 0; off >>= 1) c += (uint32_t)__shfl_xor((int)c, off);
-    if (__lane_id() == 0) wsum[threadIdx.x / 64] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (uint32_t w = 0; w < CMP_THREADS / 64; w++) t += wsum[w];
-        bcount[blockIdx.x] = t;
-    }
+    cmp_block_count(flags, step_n(sn) * (uint32_t)K, bcount);
 }
 
-/* pass 2: each block writes its flagged items in order from its offset,
- * the sum of the counts of the blocks before it (at most CMP_BLOCKS - 1
- * values, summed by the block itself: no separate scan launch); the last
- * block writes the total, the next level's count */
+/* pass 2: each block writes its flagged children in order from its
+ * offset; the last block writes the total, the next level's count */
 __global__ __launch_bounds__(CMP_THREADS) void k_cmp_scatter(const QRay *__restrict__ in,
                                                              const uint8_t *__restrict__ flags, StepN sn, int K,
                                                              const uint32_t *__restrict__ bcount,
                                                              uint32_t *__restrict__ total, QRay *__restrict__ out)
 {
-    __shared__ uint32_t wsum[CMP_THREADS / 64];
-    const uint32_t lane = __lane_id(), w = threadIdx.x / 64u;
-    uint32_t pre = 0;
-    for (uint32_t k = threadIdx.x; k < blockIdx.x; k += CMP_THREADS) pre += bcount[k];
-    for (int off = 32; off > 0; off >>= 1) pre += (uint32_t)__shfl_xor((int)pre, off);
-    if (lane == 0u) wsum[w] = pre;
-    __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t k = 0; k < CMP_THREADS / 64; k++) base += wsum[k];
-    __syncthreads();
-    if (blockIdx.x == gridDim.x - 1u && threadIdx.x == 0) *total = base + bcount[blockIdx.x];
-    const uint32_t items = step_n(sn) * (uint32_t)K;
-    uint32_t lo, hi;
-    cmp_range(items, blockIdx.x, lo, hi);
-    for (uint32_t t0 = lo; t0 < hi; t0 += CMP_THREADS) {
-        const uint32_t j = t0 + threadIdx.x;
-        const bool f = j < hi && flags[j] != 0;
-        const unsigned long long bal = __ballot(f);
-        const uint32_t r = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0u) wsum[w] = (uint32_t)__popcll(bal);
-        __syncthreads();
-        uint32_t wpre = 0, tot = 0;
-        for (uint32_t k = 0; k < CMP_THREADS / 64; k++) {
-            const uint32_t c = wsum[k];
-            wpre += k < w ? c : 0u;
-            tot += c;
-        }
-        if (f) out[base + wpre + r] = in[j];
-        base += tot;
-        __syncthreads();
-    }
+    cmp_block_scatter(in, flags, step_n(sn) * (uint32_t)K, bcount, total, out);
 }
 
 __global__ void k_fill_f4(float4 *p, size_t n, float4 v)

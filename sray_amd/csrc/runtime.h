@@ -8,8 +8,8 @@
 #define SRAY_RUNTIME_H_
 
 #include <hip/hip_runtime.h>
-#include <hipcub/device/device_select.hpp>
 
+#include <algorithm>
 #include <atomic>
 #include <condition_variable>
 #include <map>
@@ -19,6 +19,7 @@
 #include <thread>
 #include <vector>
 
+#include "devsort.h"
 #include "host.h"
 #include "render.h"
 
@@ -88,7 +89,7 @@ struct sray_gpu_ctx {
     uint32_t next_pmap = 1;
     uint64_t admission_cap = 200ull << 30;
     /* trace / gather scratch, grown on demand */
-    srt::DevBuf cubtmp, nsel, rays, hitbuf, counts, gq, grgb, gcnt, gneed, gfq, goff, gcand, giota;
+    srt::DevBuf seltmp, sorttmp, nsel, rays, hitbuf, counts, gq, grgb, gcnt, gneed, gfq, goff, gcand, giota;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     /* per-kernel HIP-event timing */
     bool profiling = false;
@@ -178,31 +179,59 @@ GpuScene *find_scene(sray_gpu_ctx *c, uint32_t id);
 /* writes one status record to the context's status stream, if it has one */
 void emit_status(sray_gpu_ctx *c, char st, int a1, int a2, int a3, int a4);
 
-/* stable selection of the flagged elements (order kept), count to the host */
+/* stable selection of the flagged elements (order kept), the count left
+ * on the device (no synchronisation) */
+template <typename T>
+int select_flagged_async(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, uint32_t n, uint32_t *d_count)
+{
+    int rc = ensure(c, c->seltmp, (size_t)CMP_BLOCKS * 4);
+    if (rc) return rc;
+    k_sel_count<T><<<CMP_BLOCKS, CMP_THREADS, 0, c->stream>>>(flags, n, dptr<uint32_t>(c->seltmp));
+    k_sel_scatter<T><<<CMP_BLOCKS, CMP_THREADS, 0, c->stream>>>(in, flags, n, dptr<uint32_t>(c->seltmp), d_count,
+                                                                out);
+    HIPCHK(c, hipGetLastError());
+    return SRAY_OK;
+}
+
+/* as select_flagged_async, with the count copied to the host */
 template <class T>
 int select_flagged(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, uint32_t n, uint32_t *count)
 {
-    size_t tmp = 0;
-    HIPCHK(c, hipcub::DeviceSelect::Flagged(nullptr, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
-    int rc = ensure(c, c->cubtmp, tmp);
-    if (rc) return rc;
-    tmp = c->cubtmp.bytes;
-    HIPCHK(c, hipcub::DeviceSelect::Flagged(c->cubtmp.p, tmp, in, flags, out, (uint32_t *)c->nsel.p, (int)n, c->stream));
+    int rc = ensure(c, c->nsel, 4);
+    if (rc || (rc = select_flagged_async(c, in, flags, out, n, dptr<uint32_t>(c->nsel)))) return rc;
     HIPCHK(c, hipMemcpyAsync(count, c->nsel.p, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SRAY_OK;
 }
 
-/* as select_flagged, with the count left on the device (no synchronisation) */
-template <typename T>
-int select_flagged_async(sray_gpu_ctx *c, const T *in, const uint8_t *flags, T *out, uint32_t n, uint32_t *d_count)
+/* stable LSD radix sort of n (key, value) pairs on bits [0, end_bit) of
+ * the keys, from (k0, v0) into (k1, v1); both pairs of buffers are
+ * overwritten on the way (an odd number of digit passes of at most 8 bits
+ * each ends in k1, v1) */
+template <class K>
+int radix_sort_pairs(sray_gpu_ctx *c, K *k0, K *k1, uint32_t *v0, uint32_t *v1, uint32_t n, int end_bit)
 {
-    size_t tmp = 0;
-    HIPCHK(c, hipcub::DeviceSelect::Flagged(nullptr, tmp, in, flags, out, d_count, (int)n, c->stream));
-    int rc = ensure(c, c->cubtmp, tmp);
+    int rc = ensure(c, c->sorttmp, ((size_t)256 * CMP_BLOCKS + 256) * 4);
     if (rc) return rc;
-    tmp = c->cubtmp.bytes;
-    HIPCHK(c, hipcub::DeviceSelect::Flagged(c->cubtmp.p, tmp, in, flags, out, d_count, (int)n, c->stream));
+    if (end_bit < 1) end_bit = 1;
+    int passes = (end_bit + 7) / 8;
+    if (passes % 2 == 0) passes++;
+    const int width = (end_bit + passes - 1) / passes;
+    uint32_t *bh = dptr<uint32_t>(c->sorttmp), *dt = bh + (size_t)256 * CMP_BLOCKS;
+    K *ks = k0, *kd = k1;
+    uint32_t *vs = v0, *vd = v1;
+    int shift = 0;
+    for (int p = 0; p < passes; p++) {
+        const int wdt = std::max(0, std::min(width, end_bit - shift));
+        const int sh = wdt ? shift : 0; /* a zero-width pass is a stable copy */
+        k_rs_hist<K><<<CMP_BLOCKS, CMP_THREADS, 0, c->stream>>>(ks, n, sh, wdt, bh);
+        k_rs_scan<<<1u << wdt, CMP_BLOCKS, 0, c->stream>>>(bh, dt);
+        k_rs_scatter<K><<<CMP_BLOCKS, CMP_THREADS, 0, c->stream>>>(ks, vs, n, sh, wdt, bh, dt, kd, vd);
+        std::swap(ks, kd);
+        std::swap(vs, vd);
+        shift += wdt;
+    }
+    HIPCHK(c, hipGetLastError());
     return SRAY_OK;
 }
 

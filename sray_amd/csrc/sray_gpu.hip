@@ -8,9 +8,6 @@
  * every float operation is the IEEE one the CPU checker performs.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/device/device_radix_sort.hpp>
-#include <hipcub/device/device_scan.hpp>
-#include <hipcub/device/device_select.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -944,10 +941,10 @@ int do_pmap_build(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, boo
     uint32_t depth = 0;
     if (n) {
         /* scratch for the build */
-        DevBuf pos0, ord[2], key[2], seg, bb, slo, shi, sax, tmp;
+        DevBuf pos0, ord[2], key[2], seg, bb, slo, shi, sax;
         sray_photon *dph = (sray_photon *)store;
         auto cleanup = [&]() {
-            DevBuf *bs[] = {&pos0, &ord[0], &ord[1], &key[0], &key[1], &seg, &bb, &slo, &shi, &sax, &tmp};
+            DevBuf *bs[] = {&pos0, &ord[0], &ord[1], &key[0], &key[1], &seg, &bb, &slo, &shi, &sax};
             for (DevBuf *b : bs) release(*b);
         };
         std::vector<std::vector<uint32_t>> L, H;
@@ -989,21 +986,10 @@ int do_pmap_build(sray_gpu_ctx *c, uint32_t scene_id, const sray_photon *ph, boo
                                                               dptr<int32_t>(seg), dptr<uint32_t>(slo),
                                                               dptr<uint8_t>(sax), n,
                                                               dptr<unsigned long long>(key[0]));
-            size_t tb = 0;
-            (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, dptr<unsigned long long>(key[0]),
-                                                     dptr<unsigned long long>(key[1]), dptr<uint32_t>(ord[cur]),
-                                                     dptr<uint32_t>(ord[cur ^ 1]), (int)n, 0, endbit, c->stream);
-            if ((rc = ensure(c, tmp, tb))) {
+            if ((rc = radix_sort_pairs(c, dptr<unsigned long long>(key[0]), dptr<unsigned long long>(key[1]),
+                                       dptr<uint32_t>(ord[cur]), dptr<uint32_t>(ord[cur ^ 1]), n, endbit))) {
                 cleanup();
                 return rc;
-            }
-            tb = tmp.bytes;
-            e = hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, dptr<unsigned long long>(key[0]),
-                                                   dptr<unsigned long long>(key[1]), dptr<uint32_t>(ord[cur]),
-                                                   dptr<uint32_t>(ord[cur ^ 1]), (int)n, 0, endbit, c->stream);
-            if (e != hipSuccess) {
-                cleanup();
-                return fail(c, SRAY_ERR_HIP, "kd-tree level sort failed: %s", hipGetErrorString(e));
             }
             cur ^= 1;
             k_pm_nodes<<<blocks(ns, 256), 256, 0, c->stream>>>(dptr<uint32_t>(slo), dptr<uint32_t>(shi),
@@ -1289,9 +1275,9 @@ int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, float cellr, 
         pm.grids[key] = std::move(G);
         return SRAY_OK;
     }
-    DevBuf bb, keys[2], vals[2], tmp;
+    DevBuf bb, keys[2], vals[2];
     auto cleanup = [&]() {
-        DevBuf *bs[] = {&bb, &keys[0], &keys[1], &vals[0], &vals[1], &tmp};
+        DevBuf *bs[] = {&bb, &keys[0], &keys[1], &vals[0], &vals[1]};
         for (DevBuf *b : bs) release(*b);
     };
     if ((rc = ensure(c, bb, 24))) return rc;
@@ -1340,22 +1326,12 @@ int pmap_grid_get(sray_gpu_ctx *c, uint32_t pmap_id, float radius, float cellr, 
     k_grid_key<<<blocks(n, 256), 256, 0, c->stream>>>(T.pos, n, pg, dptr<uint32_t>(keys[0]), dptr<uint32_t>(vals[0]));
     int endbit = 1;
     while (endbit < 32 && (1ull << endbit) <= ncell) endbit++;
-    size_t tb = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, dptr<uint32_t>(keys[0]), dptr<uint32_t>(keys[1]),
-                                                      dptr<uint32_t>(vals[0]), dptr<uint32_t>(vals[1]), (int)n, 0,
-                                                      endbit, c->stream);
-    if (e != hipSuccess || (rc = ensure(c, tmp, tb))) {
+    if ((rc = radix_sort_pairs(c, dptr<uint32_t>(keys[0]), dptr<uint32_t>(keys[1]), dptr<uint32_t>(vals[0]),
+                               dptr<uint32_t>(vals[1]), n, endbit))) {
         cleanup();
-        return rc ? rc : fail(c, SRAY_ERR_HIP, "grid sort sizing failed");
+        return rc;
     }
-    tb = tmp.bytes;
-    e = hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, dptr<uint32_t>(keys[0]), dptr<uint32_t>(keys[1]),
-                                           dptr<uint32_t>(vals[0]), dptr<uint32_t>(vals[1]), (int)n, 0, endbit,
-                                           c->stream);
-    if (e != hipSuccess) {
-        cleanup();
-        return fail(c, SRAY_ERR_HIP, "grid sort failed: %s", hipGetErrorString(e));
-    }
+    hipError_t e;
     k_grid_start<<<blocks((size_t)ncell + 1, 256), 256, 0, c->stream>>>(dptr<uint32_t>(keys[1]), n, ncell,
                                                                         dptr<uint32_t>(G->cell));
     k_grid_final<<<blocks(n, 256), 256, 0, c->stream>>>(T, dptr<uint32_t>(vals[1]), n, dptr<float4>(G->pos),
@@ -1496,7 +1472,7 @@ void sray_gpu_destroy(sray_gpu_ctx *c)
         c->scenes.clear();
         for (auto &kv : c->pmaps) free_pmap(*kv.second);
         c->pmaps.clear();
-        DevBuf *bs[] = {&c->cubtmp, &c->nsel, &c->rays, &c->hitbuf, &c->counts, &c->gq, &c->grgb, &c->gcnt,
+        DevBuf *bs[] = {&c->seltmp, &c->sorttmp, &c->nsel, &c->rays, &c->hitbuf, &c->counts, &c->gq, &c->grgb, &c->gcnt,
                         &c->gneed, &c->gfq, &c->goff, &c->gcand, &c->giota};
         for (DevBuf *b : bs) release(*b);
         prof_resolve(c);
@@ -1589,7 +1565,7 @@ int sray_gpu_scene_bytes(sray_gpu_ctx *c, uint32_t scene_id, uint64_t *bytes)
                 t += g.second->pos.bytes + g.second->dir.bytes + g.second->nrm.bytes + g.second->colb.bytes +
                      g.second->cell.bytes;
         }
-        DevBuf *bs[] = {&c->cubtmp, &c->nsel, &c->rays, &c->hitbuf, &c->counts, &c->gq, &c->grgb, &c->gcnt,
+        DevBuf *bs[] = {&c->seltmp, &c->sorttmp, &c->nsel, &c->rays, &c->hitbuf, &c->counts, &c->gq, &c->grgb, &c->gcnt,
                         &c->gneed, &c->gfq, &c->goff, &c->gcand, &c->giota};
         for (DevBuf *d : bs) t += d->bytes;
         *bytes = t;
