@@ -430,7 +430,8 @@ typedef struct sray_tuning {
     int32_t tile_w;           /* 8, 16, 32 or 64: primary rays in tiles of tile_w x (64 / tile_w) pixels */
     int32_t gather_lanes;     /* 8, 16 or 32 lanes sharing one photon-gather query */
     int32_t chunk_log2;       /* 16..21: rays per bounce-level sub-chunk = 2^chunk_log2 */
-    float   gather_cell;      /* photon-grid cell size in gather radii: 0.25, 0.5, 1 or 2 */
+    float   gather_cell;      /* photon-grid cell size in gather radii: 0.25, 0.5, 1 or 2
+                                 (renders with gather_k > 0 use half of it) */
     int32_t graphs;           /* 1: each primary chunk's bounce levels are replayed as a captured
                                  HIP graph when the pass runs on one lane; 0: launched kernel by
                                  kernel (same kernels, same order) */

@@ -1456,7 +1456,10 @@ int admit(Engine &E, Job *j)
     Q.PM.scale = gabs > 0.0 ? 1.0 / (M_PI * (gabs * gabs)) : 0.0;
     for (int kind = 0; kind < 2; kind++) {
         bool want = kind == SRAY_PMAP_CAUSTIC ? Q.P.caust != 0 : Q.P.gi != 0;
-        if (want && (rc = pmap_grid_get(c, g->pmap[kind], Q.PM.radius, c->tune.gather_cell,
+        /* k-nearest gathers scan smaller boxes (their first radius is a
+         * fraction of the gather radius): cells of half the tuned size */
+        const float cell = c->tune.gather_cell * (Q.PM.k > 0 ? 0.5f : 1.0f);
+        if (want && (rc = pmap_grid_get(c, g->pmap[kind], Q.PM.radius, cell,
                                         kind == SRAY_PMAP_CAUSTIC ? &Q.PM.caust : &Q.PM.glob)))
             return rc;
         j->pmaps[kind] = want ? g->pmap[kind] : 0u;
