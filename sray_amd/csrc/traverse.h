@@ -212,15 +212,12 @@ __device__ __forceinline__ uint32_t order_mask(uint32_t m, uint32_t dm)
  * keyframed row of that msec */
 __device__ __forceinline__ void obj_inv_at(const DevObj &o, int time, float I[12])
 {
+    /* the matrix's source chosen first, then loaded once (loading the
+     * static one and overwriting it held both in registers) */
+    const float *m = o.moving == 2 ? time_row(o.tab, o.tab_t0, o.tab_n, time, 24) + 12 : o.inv;
 #pragma unroll
-    for (int k = 0; k < 12; k++) I[k] = o.inv[k];
-    if (!o.moving) return;
-    if (o.moving == 2) {
-        const float *r = time_row(o.tab, o.tab_t0, o.tab_n, time, 24) + 12;
-#pragma unroll
-        for (int k = 0; k < 12; k++) I[k] = r[k];
-        return;
-    }
+    for (int k = 0; k < 12; k++) I[k] = m[k];
+    if (o.moving != 1) return;
     float s;
     if (time <= o.mt0) s = 0.0f;
     else if (time >= o.mt1) s = 1.0f;
@@ -233,15 +230,10 @@ __device__ __forceinline__ void obj_inv_at(const DevObj &o, int time, float I[12
 }
 __device__ __forceinline__ void obj_xf_at(const DevObj &o, int time, float M[12])
 {
+    const float *m = o.moving == 2 ? time_row(o.tab, o.tab_t0, o.tab_n, time, 24) : o.xf;
 #pragma unroll
-    for (int k = 0; k < 12; k++) M[k] = o.xf[k];
-    if (!o.moving) return;
-    if (o.moving == 2) {
-        const float *r = time_row(o.tab, o.tab_t0, o.tab_n, time, 24);
-#pragma unroll
-        for (int k = 0; k < 12; k++) M[k] = r[k];
-        return;
-    }
+    for (int k = 0; k < 12; k++) M[k] = m[k];
+    if (o.moving != 1) return;
     float s;
     if (time <= o.mt0) s = 0.0f;
     else if (time >= o.mt1) s = 1.0f;
