@@ -32,30 +32,53 @@ const XmlAttr *XmlNode::get(const char *n) const
     return nullptr;
 }
 
-/* xml_create_attr, src/xmltree.c:51-99: a value that starts with a digit
- * (or a sign and a digit) is a number — FLT if it contains '.', else INT —
- * and a VEC if it holds more than one space-separated number; unspecified
- * vector components are 1.0.  ".5" stays a string (Q16). */
+/* Attribute typing with the outcome of xml_create_attr, src/xmltree.c:51-99
+ * (Q16).  The value is split into whitespace-separated words.  It counts as
+ * numeric only when its first character is a digit, or a sign followed by a
+ * digit, so ".5" and " 1" stay strings.  A numeric value is a VEC when it has
+ * two or more words (the first four become components, the rest default to
+ * 1.0), otherwise FLT when a '.' appears anywhere in it and INT when none
+ * does.  Each component is the leading number of its word, as strtod reads
+ * it ("1,2" gives 1). */
+namespace {
+bool opens_number(const std::string &v)
+{
+    auto digit = [&](size_t k) { return k < v.size() && isdigit((unsigned char)v[k]); };
+    if (digit(0)) return true;
+    return !v.empty() && (v[0] == '+' || v[0] == '-') && digit(1);
+}
+
+std::vector<std::string> words_of(const std::string &v, size_t max_words)
+{
+    std::vector<std::string> w;
+    size_t k = 0;
+    while (w.size() < max_words) {
+        while (k < v.size() && isspace((unsigned char)v[k])) k++;
+        if (k == v.size()) break;
+        size_t e = k;
+        while (e < v.size() && !isspace((unsigned char)v[e])) e++;
+        w.push_back(v.substr(k, e - k));
+        k = e;
+    }
+    return w;
+}
+} // namespace
+
 XmlAttr xml_make_attr(const std::string &name, const std::string &val)
 {
     XmlAttr a;
     a.name = name;
     a.str = val;
-    const char *v = val.c_str();
-    if (isdigit((unsigned char)v[0]) || ((v[0] == '+' || v[0] == '-') && isdigit((unsigned char)v[1]))) {
-        a.type = strchr(v, '.') ? ATYPE_FLT : ATYPE_INT;
-        a.ival = atoi(v);
-        a.fval = (float)atof(v);
-        a.vval[0] = a.vval[1] = a.vval[2] = a.vval[3] = 1.0f;
-        int i;
-        for (i = 0; i < 4; i++) {
-            if (!*v) break;
-            a.vval[i] = (float)atof(v);
-            while (*v && !isspace((unsigned char)*v)) v++;
-            while (*v && isspace((unsigned char)*v)) v++;
-        }
-        if (i > 1) a.type = ATYPE_VEC;
-    }
+    if (!opens_number(val)) return a;
+    const std::vector<std::string> w = words_of(val, 4);
+    a.ival = (int)strtol(w[0].c_str(), nullptr, 10);
+    a.fval = (float)strtod(w[0].c_str(), nullptr);
+    for (float &c : a.vval) c = 1.0f;
+    for (size_t k = 0; k < w.size(); k++) a.vval[k] = (float)strtod(w[k].c_str(), nullptr);
+    if (w.size() >= 2)
+        a.type = ATYPE_VEC;
+    else
+        a.type = val.find('.') != std::string::npos ? ATYPE_FLT : ATYPE_INT;
     return a;
 }
 

@@ -198,3 +198,25 @@ def test_scene_diag_equals_checker():
     for cfg, var in ((0, 0), (2, 10), (3, 500)):
         hs = S.HostScene.generate(cfg, variant=var)
         assert S.scene_diag(hs.desc_ptr()) == oracle.OracleScene(hs.desc_ptr(), 32).diag()
+
+
+@pytest.mark.parametrize("ambient,bgcolor,ior,exp_amb,exp_bg,exp_ior", [
+    # two words make a VEC; missing components are 1.0
+    ("0.25 0.5", "2,5 1 7 9 11", "1.25 ", [0.25, 0.5, 1.0, 1.0], [2.0, 1.0, 7.0, 1.0], 1.25),
+    # one word with a '.' is FLT, a sign and a digit open a number
+    ("+0.5", "1 1e-1 0x10", "+1.5", [0.5, 0.5, 0.5, 1.0], [1.0, 0.1, 16.0, 1.0], 1.5),
+    # INT: no '.'; ior takes only FLT, a single-word bgcolor only VEC
+    ("3", "4", "3", [3.0, 3.0, 3.0, 1.0], None, 1.0),
+    # leading space, a bare sign, a leading '.': strings, ignored
+    (" 1", "- 1 2", ".5", None, None, 1.0),
+])
+def test_attribute_words(ambient, bgcolor, ior, exp_amb, exp_bg, exp_ior):
+    """the word rules of src/xmltree.c:76-95 on values the loader reads as
+    scalars or vectors: numbers open with a digit or sign+digit, two or more
+    words make a VEC of the first four, a '.' anywhere makes FLT"""
+    base = S.HostScene.load_xml(text="<scene><env/></scene>").desc
+    xml = (f'<scene><env ambient="{ambient}" bgcolor="{bgcolor}" ior="{ior}"/></scene>')
+    d = S.HostScene.load_xml(text=xml).desc
+    assert list(d.env_ambient) == pytest.approx(exp_amb if exp_amb else list(base.env_ambient))
+    assert list(d.env_color) == pytest.approx(exp_bg if exp_bg else list(base.env_color))
+    assert d.env_ior == pytest.approx(exp_ior)
