@@ -138,11 +138,13 @@ static int octree_build_host(const sray_scene_desc *s, int level, sray_octree_in
         info->root_min[k] = ft.root_min[k];
         info->root_max[k] = ft.root_max[k];
     }
-    if (node_lo && node_hi && info->num_nodes <= cap_nodes) {
+    /* (an empty tree has nothing to copy, and memcpy takes no null source) */
+    if (node_lo && node_hi && info->num_nodes <= cap_nodes && !ft.lo.empty()) {
         memcpy(node_lo, ft.lo.data(), ft.lo.size() * 4);
         memcpy(node_hi, ft.hi.data(), ft.hi.size() * 4);
     }
-    if (items && info->num_item_refs <= cap_items) memcpy(items, ft.items.data(), ft.items.size() * 4);
+    if (items && info->num_item_refs <= cap_items && !ft.items.empty())
+        memcpy(items, ft.items.data(), ft.items.size() * 4);
     return SRAY_OK;
 }
 
