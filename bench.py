@@ -344,9 +344,10 @@ def run_heightfield(args):
     closest_rays = cst.rays_primary + cst.rays_reflect + cst.rays_refract + cst.rays_diffuse
     # per ray 32 B in + 16 B hit out; per fetched node 32 B; per triangle its
     # three vertices with the face normal (48 B); per sphere / cylinder the
-    # inverse matrix and radius (+ extent) (52 / 60 B); per scene item its index
+    # inverse matrix and radius (+ extent) (52 / 60 B); per scene item its
+    # index; per mesh leaf item box tested its 8 B of codes
     alg_bytes = (32 + 16) * closest_rays + 32 * cl.nodes + 48 * cl.triangles + 52 * cl.spheres + \
-        60 * cl.cylinders + 4 * (cl.item_refs - cl.triangles)
+        60 * cl.cylinders + 4 * (cl.item_refs - cl.triangles) + 8 * cl.item_boxes
     launches, tq_ms = ks.get("k_trace_q", (0, 0.0))
     per_frame_launches = launches / args.steps
     avg_launch_ms = tq_ms / max(launches, 1)
@@ -356,7 +357,7 @@ def run_heightfield(args):
     kernels = {k: round(v[1] / args.steps, 4) for k, v in ks.items()}
     log(args, "per-kernel ms/step: " + ", ".join(f"{k} {v:.3f}" for k, v in kernels.items()))
     log(args, f"closest-hit visits/frame: rays {closest_rays} nodes {cl.nodes} tris {cl.triangles} "
-              f"spheres {cl.spheres} idx {cl.item_refs}; alg bytes/frame {alg_bytes / 1e6:.1f} MB")
+              f"spheres {cl.spheres} idx {cl.item_refs} boxes {cl.item_boxes}; alg bytes/frame {alg_bytes / 1e6:.1f} MB")
     ctx.free(out)
     ctx.close()
     cpu, image = None, None

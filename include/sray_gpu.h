@@ -220,6 +220,8 @@ typedef struct sray_trace_counts {
     uint64_t triangles;
     uint64_t item_refs;    /* leaf items tested (scene objects, triangles) */
     uint64_t mesh_enters;  /* mesh (second level) traversals started */
+    uint64_t item_boxes;   /* mesh leaf item boxes tested (a triangle whose
+                              box the segment misses is not tested) */
 } sray_trace_counts;
 
 /* ------------------------------------------------------------------ */

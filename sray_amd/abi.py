@@ -87,7 +87,8 @@ class Hit(C.Structure):
 
 class TraceCounts(C.Structure):
     _fields_ = [("nodes", u64), ("spheres", u64), ("cylinders", u64), ("triangles", u64),
-                ("item_refs", u64), ("mesh_enters", u64)]
+                ("item_refs", u64), ("mesh_enters", u64),
+                ("item_boxes", u64)]
 
 
 class RenderParams(C.Structure):

@@ -30,6 +30,18 @@ constexpr int IOR_DEPTH = 4;   /* ior stack carried by queued rays */
 constexpr float EPS = (float)SRAY_ERROR_MARGIN;
 constexpr float XSMALL = (float)SRAY_XSMALL_NUMBER;
 
+/* Item boxes of mesh leaves.  Every item position of a mesh leaf holds a
+ * uint2 whose low six bytes are codes x0 y0 z0 x1 y1 z1 of its triangle's
+ * bounding box, relative to the leaf's cell [lo, hi]: per axis, with
+ * ext = hi - lo, base = lo - (ext + ext) and step = ext * IBOX_STEP (float
+ * operations), code q in 1..254 is the plane base + (q - 1) * step (so the
+ * codes span the cell grown by two cell sizes on each side); code 0 as a
+ * lower side and code 255 as an upper side are unbounded.  Lower codes are
+ * at or below the triangle's minimum, upper codes at or above its maximum;
+ * a triangle whose test could accept points off its box (non-finite or
+ * badly shaped) has every side unbounded. */
+constexpr float IBOX_STEP = 5.0f / 253.0f;
+
 /* object record: traversal reads the first 80 bytes */
 struct alignas(16) DevObj {
     float inv[12];   /* world -> object */
@@ -49,6 +61,19 @@ struct alignas(16) DevObj {
     /* keyframed motion: xform + inverse (24 floats) per msec from tab_t0 */
     const float *tab;
     int32_t tab_t0, tab_n;
+};
+
+/* A scene-level item occurrence: the part of its object's record the
+ * traversal reads, stored per occurrence so that one load by occurrence
+ * rank brings it (a moving object's matrices still come from its DevObj) */
+struct alignas(16) SceneItem {
+    float inv[12];
+    float rad, start, end;
+    int32_t type;
+    int32_t obj;     /* object index */
+    uint32_t root;   /* mesh: root node of its octree */
+    int32_t moving;  /* DevObj::moving */
+    int32_t pad;
 };
 
 struct alignas(16) DevMat {
@@ -89,12 +114,13 @@ __device__ __forceinline__ const float *time_row(const float *tab, int t0, int n
 }
 
 struct SceneDev {
-    const float4 *nlo;
-    const float4 *nhi;
+    const float4 *node;      /* 2 per node: lo (w = first child or item), hi (w = child mask or LEAF | count) */
     const uint32_t *items;
+    const SceneItem *sitem;  /* per scene item position (the scene octree's leaves) */
     const DevObj *objs;
     const float4 *tri;       /* 3 per triangle: v0, v1, v2; w = face normal (v1-v0) x (v2-v0) */
     const float4 *ltri;      /* the same, 3 per item position: mesh leaves' triangles in leaf order */
+    const uint2 *lbox;       /* item box codes per item position (IBOX_STEP) */
     const float *tri_n;      /* 9 per triangle or null */
     const float *tri_g;      /* 9 per triangle or null */
     const float *tri_uv;     /* 6 per triangle or null */

@@ -131,32 +131,46 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(SPARSE || CO
 {
     __shared__ uint32_t stk[STACK * BS];
     const uint32_t n = step_n(sn);
-    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0, 0};
     /* without PERSIST the grid has one run per wave: no loop state is kept */
     for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
         /* every lane of the wave walks (the leaf tests of sparse waves are
          * shared); a lane past the end has no ray */
         const uint32_t i = base + __lane_id();
         const bool live = i < n;
-        /* the world ray is re-read from the queue when needed instead of
-         * held in registers */
-        const RayRecord<QRay> src{q, i};
-        if (UNI) {
-            HitRec h = traverse_src<false, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[0]), src, live, &stk[threadIdx.x], BS, cnt);
-            if (live) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
-        } else {
-            /* one walk per distinct scene of the wave (a wave's rays nearly
-             * always come from one request), each with the scene's
-             * descriptor wave-uniform */
-            const uint32_t sidx = live ? T.reqs[q[i].m.w >> SLOT_SHIFT].scene : 0u;
-            unsigned long long todo = __ballot(live);
-            while (todo) {
-                const uint32_t s0 = read_lane_u(sidx, __ffsll(todo) - 1);
-                const bool mine = live && sidx == s0;
-                todo &= ~__ballot(mine);
-                HitRec h = traverse_src<false, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[s0]), src, mine, &stk[threadIdx.x], BS, cnt);
-                if (mine) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+        auto walk = [&](const auto &src) {
+            if (UNI) {
+                HitRec h = traverse_src<false, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[0]), src, live, &stk[threadIdx.x], BS, cnt);
+                if (live) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+            } else {
+                /* one walk per distinct scene of the wave (a wave's rays
+                 * nearly always come from one request), each with the
+                 * scene's descriptor wave-uniform */
+                const uint32_t sidx = live ? T.reqs[q[i].m.w >> SLOT_SHIFT].scene : 0u;
+                unsigned long long todo = __ballot(live);
+                while (todo) {
+                    const uint32_t s0 = read_lane_u(sidx, __ffsll(todo) - 1);
+                    const bool mine = live && sidx == s0;
+                    todo &= ~__ballot(mine);
+                    HitRec h = traverse_src<false, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[s0]), src, mine, &stk[threadIdx.x], BS, cnt);
+                    if (mine) hits[i] = make_int4(__float_as_int(h.t), h.obj, (int)h.tri, (int)h.rank);
+                }
             }
+        };
+        if (SPARSE) {
+            /* sparse levels hold few waves per SIMD, so registers do not
+             * cost occupancy: the world ray stays in registers (no reload
+             * on entering or leaving a mesh) */
+            float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            if (live) {
+                o = q[i].o;
+                d = q[i].d;
+            }
+            walk(RayRegs{v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w)});
+        } else {
+            /* the world ray is re-read from the queue when needed instead
+             * of held in registers */
+            walk(RayRecord<QRay>{q, i});
         }
     }
     flush_counts<COUNT>(cnt, counts);
@@ -171,7 +185,7 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
 {
     __shared__ uint32_t stk[STACK * BS];
     const uint32_t n = step_n(sn) * (uint32_t)L;
-    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0, 0};
     for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
         /* every lane of the wave walks; past the end or in an unused slot it
          * has no ray */
@@ -638,7 +652,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ATTR || LMAX
     __shared__ uint32_t stk[STACK * BS];
     __shared__ QRay kids[BS];
     uint32_t c_sh = 0, c_rl = 0, c_rr = 0, c_df = 0;
-    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0, 0};
     /* this lane's rays of its last request, added to the block's counts at
      * the end (the grid has a lane per sample, so that is all of them) */
     uint32_t last_rq = 0, last_cnt = 0;
@@ -1283,7 +1297,7 @@ struct Engine {
      * one synchronisation, and the staging of the request / scene tables */
     struct PassStatus {
         uint32_t nsel, errf;
-        unsigned long long stats[8], rcounts[12];
+        unsigned long long stats[8], rcounts[2 * NCOUNT];
         uint32_t slotcnt[MAX_SLOTS];
         unsigned long long srays[MAX_SLOTS];
     };
@@ -1720,7 +1734,7 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
                                               : trace_shadow_of<true, false>(bs, P, SP))
                                : (E.one_scene ? trace_shadow_of<false, true>(bs, P, SP)
                                               : trace_shadow_of<false, false>(bs, P, SP));
-        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, st2>>>(LV.T, sq, scon, sn, E.L, occ, rc6 + 6);
+        kern<<<tgrid(kern, (size_t)cap * E.L), bs, 0, st2>>>(LV.T, sq, scon, sn, E.L, occ, rc6 + NCOUNT);
     }
     {
         ProfScope ps(c, "k_accum", st2);
@@ -1932,7 +1946,7 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
     if ((rc = ensure(c, E.slotcnt, MAX_SLOTS * 4)) || (rc = ensure(c, E.slotrays, MAX_SLOTS * 8)) ||
         (rc = ensure(c, c->nsel, 4)) || (rc = ensure_zero(c, E.errf, 4)) ||
         (rc = ensure_zero(c, E.stats, 8 * sizeof(unsigned long long))) ||
-        (rc = ensure_zero(c, E.rcounts, 12 * sizeof(unsigned long long))))
+        (rc = ensure_zero(c, E.rcounts, 2 * NCOUNT * sizeof(unsigned long long))))
         return rc;
     if (E.sub.cap == 0 && (rc = ensure(c, E.subpix, 16))) return rc;
     /* primary chunks: with several lanes a pass is spread over them, each
@@ -2138,9 +2152,9 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
     if (rc) return rc;
     Engine &E = *engine_get(c, c->plain);
     if ((rc = ensure(c, E.stats, 8 * sizeof(unsigned long long))) || (rc = ensure(c, E.errf, 4)) ||
-        (rc = ensure(c, E.rcounts, 12 * sizeof(unsigned long long))))
+        (rc = ensure(c, E.rcounts, 2 * NCOUNT * sizeof(unsigned long long))))
         return rc;
-    HIPCHK(c, hipMemsetAsync(E.rcounts.p, 0, 12 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(E.rcounts.p, 0, 2 * NCOUNT * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemsetAsync(E.stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemsetAsync(E.errf.p, 0, 4, c->stream));
     E.sec_seen = 0;
@@ -2189,12 +2203,13 @@ int do_render(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p, f
         st->samples = samples;
         sray_trace_counts *tc[2] = {&st->closest, &st->shadow};
         for (int k = 0; k < 2; k++) {
-            tc[k]->nodes = rcn[k * 6 + 0];
-            tc[k]->spheres = rcn[k * 6 + 1];
-            tc[k]->cylinders = rcn[k * 6 + 2];
-            tc[k]->triangles = rcn[k * 6 + 3];
-            tc[k]->item_refs = rcn[k * 6 + 4];
-            tc[k]->mesh_enters = rcn[k * 6 + 5];
+            tc[k]->nodes = rcn[k * NCOUNT + 0];
+            tc[k]->spheres = rcn[k * NCOUNT + 1];
+            tc[k]->cylinders = rcn[k * NCOUNT + 2];
+            tc[k]->triangles = rcn[k * NCOUNT + 3];
+            tc[k]->item_refs = rcn[k * NCOUNT + 4];
+            tc[k]->mesh_enters = rcn[k * NCOUNT + 5];
+            tc[k]->item_boxes = rcn[k * NCOUNT + 6];
         }
     }
     return SRAY_OK;
@@ -2291,7 +2306,7 @@ int do_submit_many(sray_gpu_ctx *c, const sray_request *r, uint32_t n, float *co
         }
         /* the serving engine's counters accumulate over its lifetime */
         if ((rc = ensure_zero(c, E->stats, 8 * sizeof(unsigned long long))) || (rc = ensure_zero(c, E->errf, 4)) ||
-            (rc = ensure_zero(c, E->rcounts, 12 * sizeof(unsigned long long))))
+            (rc = ensure_zero(c, E->rcounts, 2 * NCOUNT * sizeof(unsigned long long))))
             return rc;
     }
     Engine *E = c->serve;

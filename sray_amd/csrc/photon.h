@@ -149,7 +149,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon(SceneDev S, PhotonJob J,
         bool stored = false;
         V3 pdir = v3(0.0f, 0.0f, 0.0f), ppos = pdir, pnrm = pdir;
         int guard = 0;
-        TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+        TraceCounts cnt = {0, 0, 0, 0, 0, 0, 0};
         while (true) {
             if (guard >= PHOTON_MAX_BOUNCES) break;
             guard++;

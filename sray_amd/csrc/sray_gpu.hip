@@ -48,7 +48,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneDev S, const sr
 {
     __shared__ uint32_t stk[STACK * TRACE_BLOCK];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0, 0};
     if (i < n) {
         sray_ray r = rays[i];
         HitRec h = traverse<ANY, COUNT>(S, v3(r.origin[0], r.origin[1], r.origin[2]), v3(r.dir[0], r.dir[1], r.dir[2]),
@@ -64,14 +64,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(SceneDev S, const sr
         }
         hits[i] = o;
     }
-    if (COUNT) {
-        atomicAdd(&counts[0], (unsigned long long)cnt.nodes);
-        atomicAdd(&counts[1], (unsigned long long)cnt.spheres);
-        atomicAdd(&counts[2], (unsigned long long)cnt.cylinders);
-        atomicAdd(&counts[3], (unsigned long long)cnt.triangles);
-        atomicAdd(&counts[4], (unsigned long long)cnt.item_refs);
-        atomicAdd(&counts[5], (unsigned long long)cnt.mesh_enters);
-    }
+    flush_counts<COUNT>(cnt, counts);
 }
 
 /* the same through the instance of the render path's level kernels: every
@@ -85,7 +78,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays_wave(SceneDev S, con
     __shared__ uint32_t stk[STACK * TRACE_BLOCK];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < n;
-    TraceCounts cnt = {0, 0, 0, 0, 0, 0};
+    TraceCounts cnt = {0, 0, 0, 0, 0, 0, 0};
     RayRegs src;
     if (live) {
         const sray_ray r = rays[i];
@@ -448,6 +441,84 @@ __global__ void k_pm_select(PmapTree T, const sray_gather_query *__restrict__ q,
 /* ================================================================== */
 namespace srt {
 
+namespace {
+/* The item box codes of one triangle (v: its three float4 records) in a
+ * leaf frame (per axis base and step as the device computes them).  The
+ * planes base + (q - 1) * step are evaluated exactly (in double, from the
+ * same float base and step), and the codes taken on the safe side: the
+ * lower plane at or below the triangle's minimum, the upper at or above its
+ * maximum.  A triangle whose float test could accept points measurably off
+ * its box — a non-finite vertex, or twice its area below 2^-10 of its
+ * longest edge squared (a sliver) — gets every side unbounded. */
+uint2 ibox_codes(const float4 *v, const float base[3], const float step[3])
+{
+    const uint2 open = make_uint2(0xff000000u, 0xffffu); /* x0 y0 z0 = 0, x1 y1 z1 = 255 */
+    double p[3][3];
+    for (int i = 0; i < 3; i++) {
+        p[i][0] = v[i].x;
+        p[i][1] = v[i].y;
+        p[i][2] = v[i].z;
+        for (int a = 0; a < 3; a++)
+            if (!std::isfinite(p[i][a])) return open;
+    }
+    double e[3][3], len2 = 0;
+    for (int a = 0; a < 3; a++) {
+        e[0][a] = p[1][a] - p[0][a];
+        e[1][a] = p[2][a] - p[0][a];
+        e[2][a] = p[2][a] - p[1][a];
+    }
+    for (int k = 0; k < 3; k++) len2 = std::max(len2, e[k][0] * e[k][0] + e[k][1] * e[k][1] + e[k][2] * e[k][2]);
+    const double nx = e[0][1] * e[1][2] - e[0][2] * e[1][1], ny = e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                 nz = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+    if (!(std::sqrt(nx * nx + ny * ny + nz * nz) >= len2 * 0x1p-10)) return open;
+    uint32_t code[6];
+    for (int a = 0; a < 3; a++) {
+        const double lo = std::min(p[0][a], std::min(p[1][a], p[2][a]));
+        const double hi = std::max(p[0][a], std::max(p[1][a], p[2][a]));
+        const double b = base[a], s = step[a];
+        auto plane = [&](int q) { return b + (double)(q - 1) * s; };
+        int ql = 0, qh = 255;
+        if (plane(1) <= lo) {
+            ql = s > 0 ? (int)std::min(254.0, std::floor((lo - b) / s) + 1.0) : 1;
+            while (ql > 1 && plane(ql) > lo) ql--;
+            while (ql < 254 && plane(ql + 1) <= lo) ql++;
+        }
+        if (plane(254) >= hi) {
+            qh = s > 0 ? (int)std::max(1.0, std::ceil((hi - b) / s) + 1.0) : 1;
+            while (qh < 254 && plane(qh) < hi) qh++;
+            while (qh > 1 && plane(qh - 1) >= hi) qh--;
+        }
+        code[a] = (uint32_t)ql;
+        code[3 + a] = (uint32_t)qh;
+    }
+    return make_uint2(code[0] | code[1] << 8 | code[2] << 16 | code[3] << 24, code[4] | code[5] << 8);
+}
+
+/* item box codes of every mesh leaf's item positions (scene positions and
+ * unused slots stay zero) */
+void leaf_item_boxes(const sray::FlatTree &ft, size_t scene_items, const std::vector<float4> &tri,
+                     std::vector<uint2> &out)
+{
+    out.assign(ft.items.size(), make_uint2(0, 0));
+    const size_t nn = ft.lo.size() / 4;
+    for (size_t nd = 0; nd < nn; nd++) {
+        uint32_t first, meta;
+        memcpy(&first, &ft.lo[nd * 4 + 3], 4);
+        memcpy(&meta, &ft.hi[nd * 4 + 3], 4);
+        if (!(meta & LEAF) || first < scene_items) continue;
+        float base[3], step[3];
+        for (int a = 0; a < 3; a++) {
+            const float lo = ft.lo[nd * 4 + a], hi = ft.hi[nd * 4 + a];
+            const float ext = hi - lo;
+            base[a] = lo - (ext + ext);
+            step[a] = ext * IBOX_STEP;
+        }
+        const uint32_t end = first + (meta & ~LEAF);
+        for (uint32_t k = first; k < end; k++) out[k] = ibox_codes(&tri[(size_t)ft.items[k] * 3], base, step);
+    }
+}
+} // namespace
+
 float host_unkey(uint32_t k)
 {
     uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
@@ -750,6 +821,17 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     std::vector<float4> ltri(ft.items.size() * 3, make_float4(0, 0, 0, 0));
     for (size_t k = scene_items; k < ft.items.size(); k++)
         for (int v = 0; v < 3; v++) ltri[k * 3 + v] = tri[(size_t)ft.items[k] * 3 + v];
+    std::vector<uint2> lbox;
+    leaf_item_boxes(ft, scene_items, tri, lbox);
+    /* scene item records, filled in below once the objects' keyframe
+     * tables are known */
+    std::vector<SceneItem> sitem(scene_items);
+    /* node records interleaved, lo then hi: one 32-byte record per node */
+    std::vector<float> nodes(ft.lo.size() * 2);
+    for (size_t k = 0; k < ft.lo.size() / 4; k++) {
+        memcpy(&nodes[k * 8], &ft.lo[k * 4], 16);
+        memcpy(&nodes[k * 8 + 4], &ft.hi[k * 4], 16);
+    }
     /* the host vectors above are the sources of asynchronous copies: on
      * every way out (errors included) the stream drains before they go */
     struct Drain {
@@ -804,10 +886,23 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
         g->frame_t0 = s->frame_t0;
         g->frame_t1 = s->frame_t1;
     }
-    if ((rc = upload(c, *g, ft.lo.data(), ft.lo.size() * 4, &p))) return rc;
-    D.nlo = (const float4 *)p;
-    if ((rc = upload(c, *g, ft.hi.data(), ft.hi.size() * 4, &p))) return rc;
-    D.nhi = (const float4 *)p;
+    if ((rc = upload(c, *g, nodes.data(), nodes.size() * 4, &p))) return rc;
+    D.node = (const float4 *)p;
+    for (size_t k = 0; k < scene_items; k++) {
+        const DevObj &o = objs[ft.items[k]];
+        SceneItem &d = sitem[k];
+        memset(&d, 0, sizeof d);
+        memcpy(d.inv, o.inv, sizeof d.inv);
+        d.rad = o.rad;
+        d.start = o.start;
+        d.end = o.end;
+        d.type = o.type;
+        d.obj = (int32_t)ft.items[k];
+        d.root = o.root;
+        d.moving = o.moving;
+    }
+    if ((rc = upload(c, *g, sitem.data(), sitem.size() * sizeof(SceneItem), &p))) return rc;
+    D.sitem = (const SceneItem *)p;
     if ((rc = upload(c, *g, ft.items.data(), ft.items.size() * 4, &p))) return rc;
     D.items = (const uint32_t *)p;
     if ((rc = upload(c, *g, objs.data(), objs.size() * sizeof(DevObj), &p))) return rc;
@@ -816,6 +911,8 @@ int do_scene_upload(sray_gpu_ctx *c, const sray_scene_desc *s, uint32_t *scene_i
     D.tri = (const float4 *)p;
     if ((rc = upload(c, *g, ltri.data(), ltri.size() * sizeof(float4), &p))) return rc;
     D.ltri = (const float4 *)p;
+    if ((rc = upload(c, *g, lbox.data(), lbox.size() * sizeof(uint2), &p))) return rc;
+    D.lbox = (const uint2 *)p;
     if (s->tri_normal) {
         if ((rc = upload(c, *g, s->tri_normal, (size_t)s->num_triangles * 9 * 4, &p))) return rc;
         D.tri_n = (const float *)p;
@@ -1615,6 +1712,7 @@ int sray_gpu_trace(sray_gpu_ctx *c, uint32_t scene_id, const sray_ray *rays, uin
         counts->triangles = hc[3];
         counts->item_refs = hc[4];
         counts->mesh_enters = hc[5];
+        counts->item_boxes = hc[6];
     }
     return SRAY_OK;
 }

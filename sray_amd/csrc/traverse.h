@@ -332,10 +332,11 @@ __device__ __forceinline__ bool tri_t(float4 A, float4 B, float4 C, const TRay &
 }
 
 struct TraceCounts {
-    uint32_t nodes, spheres, cylinders, triangles, item_refs, mesh_enters;
+    uint32_t nodes, spheres, cylinders, triangles, item_refs, mesh_enters, item_boxes;
 };
+constexpr int NCOUNT = 7; /* device counters per kind of traversal */
 
-/* adds a lane's visit counts to the six device counters */
+/* adds a lane's visit counts to the device counters */
 template <bool COUNT>
 __device__ __forceinline__ void flush_counts(const TraceCounts &cnt, unsigned long long *c)
 {
@@ -346,6 +347,7 @@ __device__ __forceinline__ void flush_counts(const TraceCounts &cnt, unsigned lo
     atomicAdd(&c[3], (unsigned long long)cnt.triangles);
     atomicAdd(&c[4], (unsigned long long)cnt.item_refs);
     atomicAdd(&c[5], (unsigned long long)cnt.mesh_enters);
+    atomicAdd(&c[6], (unsigned long long)cnt.item_boxes);
 }
 
 struct HitRec {
@@ -430,11 +432,12 @@ constexpr int SPARSE_TRIS = 4;
  * the node arrays' addresses are loaded again at every node visit, a
  * second memory round trip in front of the node's own load. */
 struct TravScene {
-    const float4 *nlo;
-    const float4 *nhi;
+    const float4 *node;
     const uint32_t *items;
+    const SceneItem *sitem;
     const DevObj *objs;
     const float4 *ltri;
+    const uint2 *lbox;
     uint32_t scene_root;
     int32_t fast_boxes;
 };
@@ -451,11 +454,12 @@ template <class T> __device__ __forceinline__ T *uniform_ptr(T *p)
 __device__ __forceinline__ TravScene trav_scene_uniform(const SceneDev &S)
 {
     TravScene t;
-    t.nlo = uniform_ptr(S.nlo);
-    t.nhi = uniform_ptr(S.nhi);
+    t.node = uniform_ptr(S.node);
     t.items = uniform_ptr(S.items);
+    t.sitem = uniform_ptr(S.sitem);
     t.objs = uniform_ptr(S.objs);
     t.ltri = uniform_ptr(S.ltri);
+    t.lbox = uniform_ptr(S.lbox);
     t.scene_root = __builtin_amdgcn_readfirstlane(S.scene_root);
     t.fast_boxes = __builtin_amdgcn_readfirstlane(S.fast_boxes);
     return t;
@@ -500,6 +504,9 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
     int cur_mesh = -1;
     int sp = 0;
     bool done = !live;
+    /* the cell of the mesh leaf being tested (its item boxes are coded
+     * relative to it) */
+    float4 c_lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c_hi = c_lo;
     /* TF_TIE: distinct items tie at best.t; TF_MTIE: the same within the
      * mesh being walked; TF_EXH: the second, exhaustive walk of a tie ray */
     uint32_t tf = 0;
@@ -548,7 +555,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 if ((e & E_MARK) == E_MARK) {
                     sp--;
                     uint32_t rank = e & 0x3fffffffu;
-                    int oi = (int)S.items[rank];
+                    const int oi = cur_mesh; /* the object at rank */
                     cache_obj = oi;
                     if (!ANY) {
                         uint32_t m_tri = m_rank != 0xffffffffu ? S.items[m_rank] : 0xffffffffu;
@@ -577,7 +584,8 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 if (pend == 0u) sp--;
                 else stk[(sp - 1) * stride] = (e & ~0xffu) | pend;
                 uint32_t child = (e >> 8) + (k ^ r.dmask);
-                float4 lo = S.nlo[child], hi = S.nhi[child];
+                const float4 *nd = S.node + 2 * (size_t)child;
+                float4 lo = nd[0], hi = nd[1];
                 if (COUNT) cnt.nodes++;
                 uint32_t meta = __float_as_uint(hi.w), first = __float_as_uint(lo.w);
                 float bound = FLT_MAX;
@@ -598,6 +606,8 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 if (meta & LEAF) {
                     item_cur = first;
                     item_end = first + (meta & ~LEAF);
+                    c_lo = lo;
+                    c_hi = hi;
                 } else {
                     uint32_t hitm = expand_children(lo, hi, r, order_mask(meta, r.dmask), bound);
                     if (hitm) stk[(sp++) * stride] = (first << 8) | hitm;
@@ -699,6 +709,98 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 continue;
             }
         }
+        if (in_leaf && r.fast) {
+            /* mesh leaf, item boxes first: a triangle is tested only when
+             * the segment meets its box, grown by a margin far above the
+             * roundings of the triangle test (which accepts points a few
+             * ulps of the coordinates off the triangle) and of these box
+             * tests, so no hit the triangle test would report is skipped.
+             * Per axis, the t of a code-q plane is t0 + (q - 1) * dt with
+             * t0 the padded code-1 plane's t (IBOX_STEP in device.h). */
+            const float lo3[3] = {c_lo.x, c_lo.y, c_lo.z}, hi3[3] = {c_hi.x, c_hi.y, c_hi.z};
+            const float o3[3] = {r.ox, r.oy, r.oz}, i3[3] = {r.ix, r.iy, r.iz};
+            float mag = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 3; a++) mag = fmaxf(mag, fmaxf(fabsf(o3[a]), fmaxf(fabsf(lo3[a]), fabsf(hi3[a]))));
+            const float mg = mag * 0x1p-14f;
+            float tl0[3], th0[3], dt[3], tinf[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const float ext = hi3[a] - lo3[a];
+                const float base = lo3[a] - (ext + ext);
+                tl0[a] = ((base - mg) - o3[a]) * i3[a];
+                th0[a] = ((base + mg) - o3[a]) * i3[a];
+                dt[a] = (ext * IBOX_STEP) * i3[a];
+                tinf[a] = i3[a] > 0.0f ? -INFINITY : INFINITY; /* t of an unbounded lower side */
+            }
+            /* a hit the triangle test reports lies at t <= tb */
+            const float tb = ANY ? 1.0f : fminf(fminf(m_t, best.t), 1.0f);
+            auto box_met = [&](uint2 q) {
+                float near = -INFINITY, far = INFINITY;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const uint32_t ql = (q.x >> (8 * a)) & 0xffu;
+                    const uint32_t qh = a == 0 ? q.x >> 24 : (q.y >> (8 * (a - 1))) & 0xffu;
+                    const float tl = ql != 0u ? tl0[a] + (float)(int)(ql - 1u) * dt[a] : tinf[a];
+                    const float th = qh != 255u ? th0[a] + (float)(int)(qh - 1u) * dt[a] : -tinf[a];
+                    near = fmaxf(near, fminf(tl, th));
+                    far = fminf(far, fmaxf(tl, th));
+                }
+                return near <= far && far >= 0.0f && near <= tb;
+            };
+            constexpr uint32_t IB = 8;                      /* boxes per load round */
+            constexpr int TG = SPARSE ? SPARSE_TRIS : 1;    /* triangles per load round */
+            while (item_cur < item_end && !done) {
+                const uint32_t nb = min(IB, item_end - item_cur);
+                uint2 qb[IB];
+#pragma unroll
+                for (uint32_t j = 0; j < IB; j++) qb[j] = j < nb ? S.lbox[item_cur + j] : make_uint2(0u, 0u);
+                if (COUNT) cnt.item_boxes += nb;
+                uint32_t mask = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < IB; j++) mask |= (j < nb && box_met(qb[j])) ? 1u << j : 0u;
+                while (mask != 0u && !done) {
+                    uint32_t kk[TG];
+                    bool ok[TG];
+                    float4 V[3 * TG];
+#pragma unroll
+                    for (int u = 0; u < TG; u++) {
+                        ok[u] = mask != 0u;
+                        kk[u] = item_cur + (ok[u] ? (uint32_t)__builtin_ctz(mask) : 0u);
+                        mask &= ok[u] ? mask - 1u : ~0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < TG; u++)
+                        if (ok[u]) {
+                            const float4 *L = S.ltri + (size_t)kk[u] * 3;
+                            V[3 * u] = L[0];
+                            V[3 * u + 1] = L[1];
+                            V[3 * u + 2] = L[2];
+                        }
+#pragma unroll
+                    for (int u = 0; u < TG; u++) {
+                        if (!ok[u] || done) continue;
+                        if (COUNT) {
+                            cnt.item_refs++;
+                            cnt.triangles++;
+                        }
+                        float t;
+                        if (tri_t(V[3 * u], V[3 * u + 1], V[3 * u + 2], r, t, ANY ? FLT_MAX : fminf(m_t, best.t))) {
+                            if (ANY) {
+                                best.t = t;
+                                best.tri = S.items[kk[u]];
+                                best.obj = cur_mesh;
+                                done = true;
+                            } else {
+                                mesh_offer(t, kk[u], false);
+                            }
+                        }
+                    }
+                }
+                item_cur += nb;
+            }
+            continue;
+        }
         if (in_leaf) {
             /* mesh leaf: all its triangles, from the leaf-ordered copies
              * (dense levels: one per step, as a second triangle's registers
@@ -755,10 +857,20 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
         }
         if (done) continue;
         uint32_t rank = item_cur++;
-        int oi = (int)S.items[rank];
+        /* the occurrence's record (one load); a moving object's matrices
+         * from its DevObj */
+        const SceneItem &ob = S.sitem[rank];
+        const int oi = ob.obj;
         if (COUNT) cnt.item_refs++;
-        const DevObj &ob = S.objs[oi];
-        int type = ob.type;
+        const int type = ob.type;
+        auto inv_at = [&](int time, float I[12]) {
+            if (ob.moving == 0) {
+#pragma unroll
+                for (int k = 0; k < 12; k++) I[k] = ob.inv[k];
+            } else {
+                obj_inv_at(S.objs[oi], time, I);
+            }
+        };
         if (type == SRAY_OBJ_MESH) {
             if (oi == cache_obj) {
                 /* this ray has traversed this mesh already (in another scene
@@ -778,7 +890,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 V3 o2, d2;
                 int t2;
                 src.get(o2, d2, t2);
-                obj_inv_at(ob, t2, I);
+                inv_at(t2, I);
                 r = make_tray_s(xf_point(I, o2), xf_vec(I, d2), S.fast_boxes);
             }
             stk[(sp++) * stride] = (ob.root << 8) | (1u << r.dmask);
@@ -796,7 +908,7 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
             V3 o2, d2;
             int t2;
             src.get(o2, d2, t2);
-            obj_inv_at(ob, t2, I);
+            inv_at(t2, I);
             lo = xf_point(I, o2);
             ld = xf_vec(I, d2);
         }
