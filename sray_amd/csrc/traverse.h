@@ -410,14 +410,15 @@ template <class T> __device__ __forceinline__ const T *read_lane_p(const T *p, i
 }
 
 /* Up to this many lanes of a wave in mesh leaves at once, the whole wave
- * tests each of their leaves in turn, one triangle per lane; with more,
- * each lane walks its own leaf.  Long rays (grazing a heightfield, say)
- * keep a wave alive long after its other lanes are done, and set how long
- * a bounce level takes; in such a wave one step of 64 loads replaces a
- * leaf's serial chain of loads.  Measured on the heightfield (thresholds
- * 8 / 16 / 32 / 64): 8 is fastest for closest-hit rays, 16 for any-hit
- * rays, whose leaves end at the first hit when walked by their own lane. */
-template <bool ANY> constexpr int coop_lanes() { return ANY ? 16 : 8; }
+ * tests their leaves, one triangle per lane, several leaves per round when
+ * their items fit the wave; with more, each lane walks its own leaf (item
+ * boxes first).  Long rays (grazing a heightfield, say) keep a wave alive
+ * long after its other lanes are done, and set how long a bounce level
+ * takes; in such a wave one step of 64 loads replaces a leaf's serial chain
+ * of loads.  Measured on the heightfield with one leaf per round
+ * (closest-hit / any-hit thresholds 1/2, 2/4, 4/8, 8/8, 8/16, 12/8, 16/8
+ * gave 6.06, 5.84, 5.36, 5.23, 5.29, 5.37, 5.65 ms per frame): 8 for both. */
+template <bool ANY> constexpr int coop_lanes() { return 8; }
 
 /* Node records a lane fetches per round of the walk before the wave moves
  * on to the item tests.  Walking on until every lane has items (no cap)
@@ -660,62 +661,155 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
         if (COOP) {
             unsigned long long lm = __ballot(in_leaf);
             if (lm != 0ull && __popcll(lm) <= coop_lanes<ANY>()) {
-                /* the wave tests each leaf lane's leaf in turn */
+                /* The wave tests the leaf lanes' leaves, one triangle per
+                 * lane: in rounds of consecutive leaf lanes whose items fit
+                 * the 64 lanes together (a leaf of more items alone, in
+                 * passes of 64).  Each leaf's winner is the smallest (t, item
+                 * position) key over its own lanes, merged with the rule
+                 * the serial loop applies. */
                 const uint32_t lane = __lane_id();
+                const uint32_t mycnt = in_leaf ? item_end - item_cur : 0u;
                 while (lm != 0ull) {
-                    const int j = __ffsll(lm) - 1;
-                    lm &= lm - 1ull;
-                    TRay rj;
-                    rj.ox = read_lane_f(r.ox, j);
-                    rj.oy = read_lane_f(r.oy, j);
-                    rj.oz = read_lane_f(r.oz, j);
-                    rj.dx = read_lane_f(r.dx, j);
-                    rj.dy = read_lane_f(r.dy, j);
-                    rj.dz = read_lane_f(r.dz, j);
-                    const uint32_t cur = read_lane_u(item_cur, j), end = read_lane_u(item_end, j);
-                    const float bnd = ANY ? FLT_MAX : read_lane_f(fminf(m_t, best.t), j);
-                    const float4 *LT = read_lane_p(S.ltri, j);
-                    unsigned long long key = ~0ull;
-                    bool ltie = false; /* two of this lane's triangles tie at its best */
-                    for (uint32_t base = cur; base < end; base += 64u) {
-                        const uint32_t k = base + lane;
-                        if (k < end) {
-                            const float4 *L = LT + (size_t)k * 3;
-                            float4 A = L[0], B = L[1], C = L[2];
-                            if (COUNT) {
-                                cnt.item_refs++;
-                                cnt.triangles++;
-                            }
-                            float t;
-                            if (tri_t(A, B, C, rj, t, bnd)) {
-                                const unsigned long long kk = ((unsigned long long)__float_as_uint(t) << 32) | k;
-                                if (!ANY && (uint32_t)(key >> 32) == __float_as_uint(t)) ltie = true;
-                                key = kk < key ? kk : key;
+                    constexpr int NSEG = coop_lanes<ANY>();
+                    int segl[NSEG];
+                    uint32_t segp[NSEG];
+                    bool segv[NSEG];
+                    uint32_t total = 0;
+                    bool open = true;
+#pragma unroll
+                    for (int q = 0; q < NSEG; q++) {
+                        segv[q] = false;
+                        segl[q] = 0;
+                        segp[q] = total;
+                        if (open && lm != 0ull) {
+                            const int j = __ffsll(lm) - 1;
+                            const uint32_t cj = read_lane_u(mycnt, j);
+                            if (q > 0 && total + cj > 64u) {
+                                open = false;
+                            } else {
+                                segv[q] = true;
+                                segl[q] = j;
+                                segp[q] = total;
+                                total += cj;
+                                lm &= lm - 1ull;
+                                if (total > 64u) open = false;
                             }
                         }
                     }
-                    const unsigned long long mine = key;
-                    key = wave_min_u64(key);
-                    /* positions within one leaf are distinct triangles: any
-                     * other lane at the leaf's best t is a tie */
-                    bool leaf_tie = false;
-                    if (!ANY)
-                        leaf_tie = __ballot(mine != ~0ull && (uint32_t)(mine >> 32) == (uint32_t)(key >> 32) &&
-                                            (mine != key || ltie)) != 0ull;
-                    if (lane == (uint32_t)j) {
-                        if (key != ~0ull) {
-                            const float t = __uint_as_float((uint32_t)(key >> 32));
-                            const uint32_t kpos = (uint32_t)key;
-                            if (ANY) {
-                                best.t = t;
-                                best.tri = S.items[kpos];
-                                best.obj = cur_mesh;
-                                done = true;
-                            } else {
-                                mesh_offer(t, kpos, leaf_tie);
+                    if (total > 64u) {
+                        /* one leaf of more than 64 items: passes of 64 */
+                        const int j = segl[0];
+                        TRay rj;
+                        rj.ox = read_lane_f(r.ox, j);
+                        rj.oy = read_lane_f(r.oy, j);
+                        rj.oz = read_lane_f(r.oz, j);
+                        rj.dx = read_lane_f(r.dx, j);
+                        rj.dy = read_lane_f(r.dy, j);
+                        rj.dz = read_lane_f(r.dz, j);
+                        const uint32_t cur = read_lane_u(item_cur, j), end = read_lane_u(item_end, j);
+                        const float bnd = ANY ? FLT_MAX : read_lane_f(fminf(m_t, best.t), j);
+                        unsigned long long key = ~0ull;
+                        bool ltie = false; /* two of this lane's triangles tie at its best */
+                        for (uint32_t base = cur; base < end; base += 64u) {
+                            const uint32_t k = base + lane;
+                            if (k < end) {
+                                const float4 *L = S.ltri + (size_t)k * 3;
+                                float4 A = L[0], B = L[1], C = L[2];
+                                if (COUNT) {
+                                    cnt.item_refs++;
+                                    cnt.triangles++;
+                                }
+                                float t;
+                                if (tri_t(A, B, C, rj, t, bnd)) {
+                                    const unsigned long long kk = ((unsigned long long)__float_as_uint(t) << 32) | k;
+                                    if (!ANY && (uint32_t)(key >> 32) == __float_as_uint(t)) ltie = true;
+                                    key = kk < key ? kk : key;
+                                }
                             }
                         }
-                        item_cur = item_end;
+                        const unsigned long long mine = key;
+                        key = wave_min_u64(key);
+                        /* positions within one leaf are distinct triangles:
+                         * any other lane at the leaf's best t is a tie */
+                        bool leaf_tie = false;
+                        if (!ANY)
+                            leaf_tie = __ballot(mine != ~0ull && (uint32_t)(mine >> 32) == (uint32_t)(key >> 32) &&
+                                                (mine != key || ltie)) != 0ull;
+                        if (lane == (uint32_t)j) {
+                            if (key != ~0ull) {
+                                const float t = __uint_as_float((uint32_t)(key >> 32));
+                                const uint32_t kpos = (uint32_t)key;
+                                if (ANY) {
+                                    best.t = t;
+                                    best.tri = S.items[kpos];
+                                    best.obj = cur_mesh;
+                                    done = true;
+                                } else {
+                                    mesh_offer(t, kpos, leaf_tie);
+                                }
+                            }
+                            item_cur = item_end;
+                        }
+                        continue;
+                    }
+                    /* a packed round: lane l tests item l - segp[q] of the
+                     * leaf of segment q (segp[q] <= l < segp[q + 1]) */
+                    int mseg = 0;
+#pragma unroll
+                    for (int q = 1; q < NSEG; q++)
+                        if (segv[q] && lane >= segp[q]) mseg = q;
+                    int src = segl[0];
+                    uint32_t off = lane - segp[0];
+#pragma unroll
+                    for (int q = 1; q < NSEG; q++)
+                        if (mseg == q) {
+                            src = segl[q];
+                            off = lane - segp[q];
+                        }
+                    TRay rj;
+                    rj.ox = __shfl(r.ox, src);
+                    rj.oy = __shfl(r.oy, src);
+                    rj.oz = __shfl(r.oz, src);
+                    rj.dx = __shfl(r.dx, src);
+                    rj.dy = __shfl(r.dy, src);
+                    rj.dz = __shfl(r.dz, src);
+                    const uint32_t k = (uint32_t)__shfl((int)item_cur, src) + off;
+                    const float bnd = ANY ? FLT_MAX : __shfl(fminf(m_t, best.t), src);
+                    unsigned long long key = ~0ull;
+                    if (lane < total) {
+                        const float4 *L = S.ltri + (size_t)k * 3;
+                        float4 A = L[0], B = L[1], C = L[2];
+                        if (COUNT) {
+                            cnt.item_refs++;
+                            cnt.triangles++;
+                        }
+                        float t;
+                        if (tri_t(A, B, C, rj, t, bnd)) key = ((unsigned long long)__float_as_uint(t) << 32) | k;
+                    }
+#pragma unroll
+                    for (int q = 0; q < NSEG; q++) {
+                        if (!segv[q]) continue;
+                        const unsigned long long mine = (lane < total && mseg == q) ? key : ~0ull;
+                        const unsigned long long kmin = wave_min_u64(mine);
+                        bool leaf_tie = false;
+                        if (!ANY)
+                            leaf_tie = __ballot(mine != ~0ull && (uint32_t)(mine >> 32) == (uint32_t)(kmin >> 32) &&
+                                                mine != kmin) != 0ull;
+                        if (lane == (uint32_t)segl[q]) {
+                            if (kmin != ~0ull) {
+                                const float t = __uint_as_float((uint32_t)(kmin >> 32));
+                                const uint32_t kpos = (uint32_t)kmin;
+                                if (ANY) {
+                                    best.t = t;
+                                    best.tri = S.items[kpos];
+                                    best.obj = cur_mesh;
+                                    done = true;
+                                } else {
+                                    mesh_offer(t, kpos, leaf_tie);
+                                }
+                            }
+                            item_cur = item_end;
+                        }
                     }
                 }
                 continue;
