@@ -304,7 +304,7 @@ def run_heightfield(args):
     out = ctx.malloc(W * H * 4 * 4)
     tuning = tuning_arg(args, ctx)
     if tuning is None and not args.no_autotune:
-        t, tms = ctx.autotune(sid, p, rounds=2)
+        t, tms = ctx.autotune(sid, p, rounds=3)
         tuning = tuning_dict(t)
         log(args, f"autotune: {tuning}, {tms:.2f} ms/frame")
     if tuning is None:

@@ -442,9 +442,10 @@ typedef struct sray_tuning {
     int32_t lanes;            /* 1, 2 or 4 HIP streams a render pass spreads its primary chunks over
                                  (each with its own bounce-level queues), so that one chunk's deep,
                                  sparse levels overlap another chunk's work */
-    int32_t sparse_from;      /* 0..64: first bounce level whose traversal kernels keep four triangles
-                                 in flight per lane (more registers, fewer waves; for levels with
-                                 too few rays to hide load latency); 64: none */
+    int32_t sparse_from;      /* 0..64: first bounce level whose traversal kernels keep two triangles
+                                 in flight per lane and the world ray in registers (more registers,
+                                 fewer waves; for levels with too few rays to hide load latency);
+                                 64: none */
     int32_t chain;            /* 1: when no ray of the requests in flight can spawn more than one
                                  child and none uses photon maps (and at most 4 shadow rays per
                                  hit), one lane follows each sample's rays to the end in a single

@@ -120,13 +120,10 @@ __device__ __forceinline__ uint32_t first_run() { return (blockIdx.x * blockDim.
 __device__ __forceinline__ uint32_t run_stride() { return gridDim.x * blockDim.x; }
 
 /* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
-/* The dense single-scene instance fits 7 waves per SIMD (72 VGPRs) without
- * spilling when asked to; left to itself the compiler takes 73-76 VGPRs (6
- * waves).  The multi-scene instance spills under the same hint (its walk
- * loops over the wave's scenes), and gained nothing from it when it did
- * not, so it takes no hint. */
+/* No occupancy hints: with item boxes in the walk, the dense instance
+ * spilled under a 6- or 7-wave hint (80 / 72 VGPRs). */
 template <bool COUNT, bool UNI, int BS, bool PERSIST, bool SPARSE>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(SPARSE || COUNT || PERSIST || !UNI ? 1 : 7))) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
+__global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__ q, StepN sn,
                                                 int4 *__restrict__ hits, unsigned long long *counts)
 {
     __shared__ uint32_t stk[STACK * BS];
@@ -2482,7 +2479,9 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
                 release(out);
                 return rc;
             }
-            if (ms < cur_ms) {
+            /* a change must beat the current choice by more than the
+             * round-to-round noise of the timings (about 1%) */
+            if (ms < cur_ms * 0.985) {
                 cur = t;
                 cur_ms = ms;
             }

@@ -816,7 +816,11 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 continue;
             }
         }
-        if (in_leaf && r.fast) {
+        if (COOP && in_leaf && r.fast) {
+            /* (only in the wave-wide walks of the level kernels: in the
+             * one-lane walks of the chain and photon kernels the registers
+             * it holds cost more than it saves; spheres 1601 -> 1742 Mrays/s
+             * without it there) */
             /* mesh leaf, item boxes first: a triangle is tested only when
              * the segment meets its box, grown by a margin far above the
              * roundings of the triangle test (which accepts points a few

@@ -18,7 +18,7 @@ TOL = 1e-6
 def _variants():
     return [dict(trace_block=64), dict(trace_block=256), dict(tile_w=8), dict(tile_w=16),
             dict(chunk_log2=16), dict(gather_lanes=8), dict(gather_lanes=32), dict(gather_cell=0.25),
-            dict(gather_cell=0.5), dict(gather_cell=2.0), dict(graphs=0), dict(persistent=1), dict(sparse_from=0), dict(sparse_from=2)]
+            dict(gather_cell=0.5), dict(gather_cell=2.0), dict(graphs=0), dict(persistent=1), dict(sparse_from=64), dict(sparse_from=2)]
 
 
 def test_tunings_give_the_same_image(ctx):
@@ -57,18 +57,20 @@ def test_sub_chunks_match_checker(ctx):
 
 @pytest.mark.parametrize("sparse_from", [0, 1])
 def test_sparse_levels_match_checker(ctx, sparse_from):
-    """the traversal instances that keep four triangles in flight (from
-    bounce level sparse_from on) give the heightfield image of the default
-    instances bit for bit, and match the checker"""
+    """the sparse traversal instances (two triangles in flight, the world
+    ray in registers; from bounce level sparse_from on) give the heightfield
+    image of the dense instances bit for bit, and match the checker"""
     hs = S.HostScene.generate(1, variant=4)
     sid = ctx.upload(hs)
     p = S.render_params(480, 270, iter=6)
-    base, bst = ctx.render(sid, p)
-    ctx.set_tuning(sparse_from=sparse_from)
+    default = ctx.tuning().sparse_from
+    ctx.set_tuning(sparse_from=64)
     try:
+        base, bst = ctx.render(sid, p)
+        ctx.set_tuning(sparse_from=sparse_from)
         img, st = ctx.render(sid, p)
     finally:
-        ctx.set_tuning(sparse_from=64)
+        ctx.set_tuning(sparse_from=default)
     assert np.array_equal(img, base) and st.rays() == bst.rays()
     oimg, ost = oracle.OracleScene(hs.desc_ptr(), 32).render(p)
     e = float(np.sqrt(np.mean((img.astype(np.float64) - oimg) ** 2)))
@@ -78,19 +80,21 @@ def test_sparse_levels_match_checker(ctx, sparse_from):
 
 @pytest.mark.parametrize("config,variant,size", [(2, 10, (320, 240)), (3, 2000, (256, 256)), (0, 0, (320, 240))])
 def test_sparse_instances_same_image(ctx, config, variant, size):
-    """the sparse instances (four triangles in flight, box tests in min/max
-    form where exact) render the image and ray counts of the default
+    """the sparse instances (two triangles in flight, box tests in min/max
+    form where exact) render the image and ray counts of the dense
     instances on the room (axis-aligned walls), a batch scene (moving
     objects) and the spheres"""
     hs = S.HostScene.generate(config, variant=variant)
     sid = ctx.upload(hs)
     p = S.render_params(*size, samples=(1, 2), max_var=0.05, iter=5, shadow_samples=2, seed=5)
-    base, bst = ctx.render(sid, p)
-    ctx.set_tuning(sparse_from=0, chain=0)
+    default = ctx.tuning()
+    ctx.set_tuning(sparse_from=64, chain=1)
     try:
+        base, bst = ctx.render(sid, p)
+        ctx.set_tuning(sparse_from=0, chain=0)
         img, st = ctx.render(sid, p)
     finally:
-        ctx.set_tuning(sparse_from=64, chain=1)
+        ctx.set_tuning(sparse_from=default.sparse_from, chain=default.chain)
     assert np.array_equal(img, base) and st.rays() == bst.rays()
     ctx.release(sid)
 
