@@ -33,14 +33,16 @@ constexpr float XSMALL = (float)SRAY_XSMALL_NUMBER;
 /* Item boxes of mesh leaves.  Every item position of a mesh leaf holds a
  * uint2 whose low six bytes are codes x0 y0 z0 x1 y1 z1 of its triangle's
  * bounding box, relative to the leaf's cell [lo, hi]: per axis, with
- * ext = hi - lo, base = lo - (ext + ext) and step = ext * IBOX_STEP (float
- * operations), code q in 1..254 is the plane base + (q - 1) * step (so the
- * codes span the cell grown by two cell sizes on each side); code 0 as a
- * lower side and code 255 as an upper side are unbounded.  Lower codes are
- * at or below the triangle's minimum, upper codes at or above its maximum;
- * a triangle whose test could accept points off its box (non-finite or
- * badly shaped) has every side unbounded. */
-constexpr float IBOX_STEP = 5.0f / 253.0f;
+ * ext = hi - lo, base = lo - ext * IBOX_GROW and step = ext * IBOX_STEP
+ * (float operations), code q in 0..255 is the plane base + q * step (the
+ * codes span the cell grown by four cell sizes on each side).  Lower codes
+ * are at or below the triangle's minimum, upper codes at or above its
+ * maximum.  IBOX_OPEN in the high word marks a triangle whose box leaves
+ * that range, or whose test could accept points off its box (non-finite or
+ * badly shaped): it is always tested. */
+constexpr float IBOX_GROW = 4.0f;
+constexpr float IBOX_STEP = 9.0f / 255.0f;
+constexpr uint32_t IBOX_OPEN = 0x10000u;
 
 /* object record: traversal reads the first 80 bytes */
 struct alignas(16) DevObj {

@@ -444,15 +444,16 @@ namespace srt {
 namespace {
 /* The item box codes of one triangle (v: its three float4 records) in a
  * leaf frame (per axis base and step as the device computes them).  The
- * planes base + (q - 1) * step are evaluated exactly (in double, from the
- * same float base and step), and the codes taken on the safe side: the
- * lower plane at or below the triangle's minimum, the upper at or above its
- * maximum.  A triangle whose float test could accept points measurably off
- * its box — a non-finite vertex, or twice its area below 2^-10 of its
- * longest edge squared (a sliver) — gets every side unbounded. */
+ * planes base + q * step are evaluated exactly (in double, from the same
+ * float base and step), and the codes taken on the safe side: the lower
+ * plane at or below the triangle's minimum, the upper at or above its
+ * maximum.  A triangle outside the frame's planes, or whose float test
+ * could accept points measurably off its box (a non-finite vertex, or twice
+ * its area below 2^-10 of its longest edge squared: a sliver), is marked
+ * IBOX_OPEN. */
 uint2 ibox_codes(const float4 *v, const float base[3], const float step[3])
 {
-    const uint2 open = make_uint2(0xff000000u, 0xffffu); /* x0 y0 z0 = 0, x1 y1 z1 = 255 */
+    const uint2 open = make_uint2(0xff000000u, 0xffffu | IBOX_OPEN); /* x0 y0 z0 = 0, x1 y1 z1 = 255 */
     double p[3][3];
     for (int i = 0; i < 3; i++) {
         p[i][0] = v[i].x;
@@ -476,18 +477,14 @@ uint2 ibox_codes(const float4 *v, const float base[3], const float step[3])
         const double lo = std::min(p[0][a], std::min(p[1][a], p[2][a]));
         const double hi = std::max(p[0][a], std::max(p[1][a], p[2][a]));
         const double b = base[a], s = step[a];
-        auto plane = [&](int q) { return b + (double)(q - 1) * s; };
-        int ql = 0, qh = 255;
-        if (plane(1) <= lo) {
-            ql = s > 0 ? (int)std::min(254.0, std::floor((lo - b) / s) + 1.0) : 1;
-            while (ql > 1 && plane(ql) > lo) ql--;
-            while (ql < 254 && plane(ql + 1) <= lo) ql++;
-        }
-        if (plane(254) >= hi) {
-            qh = s > 0 ? (int)std::max(1.0, std::ceil((hi - b) / s) + 1.0) : 1;
-            while (qh < 254 && plane(qh) < hi) qh++;
-            while (qh > 1 && plane(qh - 1) >= hi) qh--;
-        }
+        auto plane = [&](int q) { return b + (double)q * s; };
+        if (!(plane(0) <= lo) || !(plane(255) >= hi)) return open;
+        int ql = s > 0 ? (int)std::min(255.0, std::max(0.0, std::floor((lo - b) / s))) : 0;
+        while (ql > 0 && plane(ql) > lo) ql--;
+        while (ql < 255 && plane(ql + 1) <= lo) ql++;
+        int qh = s > 0 ? (int)std::min(255.0, std::max(0.0, std::ceil((hi - b) / s))) : 0;
+        while (qh < 255 && plane(qh) < hi) qh++;
+        while (qh > 0 && plane(qh - 1) >= hi) qh--;
         code[a] = (uint32_t)ql;
         code[3 + a] = (uint32_t)qh;
     }
@@ -510,7 +507,7 @@ void leaf_item_boxes(const sray::FlatTree &ft, size_t scene_items, const std::ve
         for (int a = 0; a < 3; a++) {
             const float lo = ft.lo[nd * 4 + a], hi = ft.hi[nd * 4 + a];
             const float ext = hi - lo;
-            base[a] = lo - (ext + ext);
+            base[a] = lo - ext * IBOX_GROW;
             step[a] = ext * IBOX_STEP;
         }
         const uint32_t end = first + (meta & ~LEAF);

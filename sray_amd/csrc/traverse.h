@@ -834,17 +834,16 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
             for (int a = 0; a < 3; a++) mag = fmaxf(mag, fmaxf(fabsf(o3[a]), fmaxf(fabsf(lo3[a]), fabsf(hi3[a]))));
             const float mg = mag * 0x1p-14f;
             /* per axis, the padded planes of code q lie at t = fma(q, dt, t0)
-             * (t0 for code 0, the padding folded in); the roundings of this
-             * form are far below the padding */
-            float tl0[3], th0[3], dt[3], tinf[3];
+             * (the padding folded into t0); the roundings of this form are
+             * far below the padding */
+            float tl0[3], th0[3], dt[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 const float ext = hi3[a] - lo3[a];
-                const float base = lo3[a] - (ext + ext);
+                const float base = lo3[a] - ext * IBOX_GROW;
                 dt[a] = (ext * IBOX_STEP) * i3[a];
-                tl0[a] = ((base - mg) - o3[a]) * i3[a] - dt[a];
-                th0[a] = ((base + mg) - o3[a]) * i3[a] - dt[a];
-                tinf[a] = i3[a] > 0.0f ? -INFINITY : INFINITY; /* t of an unbounded lower side */
+                tl0[a] = ((base - mg) - o3[a]) * i3[a];
+                th0[a] = ((base + mg) - o3[a]) * i3[a];
             }
             /* a hit the triangle test reports lies at t <= tb */
             const float tb = ANY ? 1.0f : fminf(fminf(m_t, best.t), 1.0f);
@@ -854,12 +853,11 @@ __device__ __forceinline__ HitRec traverse_src(const SC &S, const SRC &src, bool
                 for (int a = 0; a < 3; a++) {
                     const float ql = (float)((q.x >> (8 * a)) & 0xffu);
                     const float qh = (float)(a == 0 ? q.x >> 24 : (q.y >> (8 * (a - 1))) & 0xffu);
-                    const float tl = ql != 0.0f ? fmaf(ql, dt[a], tl0[a]) : tinf[a];
-                    const float th = qh != 255.0f ? fmaf(qh, dt[a], th0[a]) : -tinf[a];
+                    const float tl = fmaf(ql, dt[a], tl0[a]), th = fmaf(qh, dt[a], th0[a]);
                     near = fmaxf(near, fminf(tl, th));
                     far = fminf(far, fmaxf(tl, th));
                 }
-                return near <= far && far >= 0.0f && near <= tb;
+                return (q.y & IBOX_OPEN) != 0u || (near <= far && far >= 0.0f && near <= tb);
             };
             /* boxes per load round: 8 for closest-hit rays, 16 for any-hit
              * rays, which often end at their first hit (measured: shadow
