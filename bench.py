@@ -113,12 +113,12 @@ def cpu_share():
     return n, {"online_cores": online, "affinity_cores": aff, "threads": n, "limited_by": why}
 
 
-def read_traffic(workload):
-    """HBM bytes per k_trace_q launch from the committed rocprofv3 --pmc summary"""
+def read_pmc(workload):
+    """the committed rocprofv3 --pmc summary of k_trace_q (key: value lines)"""
     path = os.path.join(ROOT, "profiles", f"pmc_k_trace_q_{workload}.txt")
-    if not os.path.exists(path):
-        return None
     vals = {}
+    if not os.path.exists(path):
+        return vals
     with open(path) as f:
         for line in f:
             if ":" in line and not line.startswith("#"):
@@ -127,7 +127,22 @@ def read_traffic(workload):
                     vals[k.strip()] = float(v.split()[0])
                 except (ValueError, IndexError):
                     pass
-    return vals.get("hbm_bytes_per_dispatch")
+    return vals
+
+
+def read_traffic(workload):
+    """HBM bytes per k_trace_q launch from the committed rocprofv3 --pmc summary"""
+    return read_pmc(workload).get("hbm_bytes_per_dispatch")
+
+
+def read_valu(workload):
+    """VALU issue busy of k_trace_q from the same summary (all levels, level 0, levels 1-6)"""
+    v = read_pmc(workload)
+    keys = ("valu_busy_all_levels", "valu_busy_level0", "valu_busy_levels1_6")
+    if not all(k in v for k in keys):
+        return None
+    return {"all_levels": v[keys[0]], "level0": v[keys[1]], "levels1_6": v[keys[2]],
+            "source": f"profiles/pmc_k_trace_q_{workload}.txt"}
 
 
 def rmse(a, b):
@@ -278,6 +293,14 @@ def latency(res):
             "queue_p50_ms": pct(q, 50), "queue_p99_ms": pct(q, 99)}
 
 
+def latency_by_size(jobs, res):
+    """latency percentiles per request size (width x height)"""
+    groups = {}
+    for (_, p, _), r in zip(jobs, res):
+        groups.setdefault(f"{p.x1 - p.x0}x{p.y1 - p.y0}", []).append(r.latency_ms)
+    return {k: {"n": len(v), "p50_ms": pct(v, 50), "p99_ms": pct(v, 99)} for k, v in sorted(groups.items())}
+
+
 # ------------------------------------------------------------------ legs
 def run_heightfield(args):
     """the traversal-bound frame: value, roofline and CPU baseline"""
@@ -384,6 +407,7 @@ def run_heightfield(args):
                    "tuning": tuning, "octree": octree, "kernel_ms_per_step": kernels},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic("heightfield"),
+                     "valu_busy": read_valu("heightfield"),
                      "kernel": "k_trace_q", "avg_launch_ms": round(avg_launch_ms, 4),
                      "launches_per_step": per_frame_launches, "alg_bytes_per_launch": int(bytes_per_launch),
                      "pass": "one chunk of 2^21 rays per bounce level, one stream"},
@@ -638,7 +662,8 @@ def run_serve(args):
     return base_line("Mrays/s on 1 GPU (served)", value, len(reqs), 1, wall * 1e3 / len(reqs),
                      f"serve: Poisson stream over {len(sids)} resident photon-mapped rooms (2M+500k photons "
                      f"each), tiles 64^2/256^2/1920x1080 at .6/.3/.1, 1-16 spp",
-                     {"latency_ms": lat, "requests": len(reqs), "offered_rate_per_s": round(rate, 2),
+                     {"latency_ms": lat, "latency_by_tile": latency_by_size(reqs, res),
+                      "requests": len(reqs), "offered_rate_per_s": round(rate, 2),
                       "capacity_req_per_s": round(capacity, 2), "burst_req_per_s": round(len(burst) / bwall, 2),
                       "load": round(rate / capacity, 3),
                       "completed_rate_per_s": round(len(reqs) / wall, 2), "wall_s": round(wall, 2),
