@@ -208,8 +208,9 @@ typedef struct sray_hit {
                                       few lanes still in mesh leaves together
                                       (same results; for tests) */
 #define SRAY_TRACE_SPARSE    8u    /* with SRAY_TRACE_WAVE: the instance of the
-                                      deep bounce levels (four triangles in
-                                      flight per lane) */
+                                      sparse bounce levels (two triangles in
+                                      flight per lane, world ray in
+                                      registers) */
 
 typedef struct sray_trace_counts {
     uint64_t nodes;        /* octree node records fetched (children are
