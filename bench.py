@@ -10,6 +10,8 @@ The line also carries:
   roofline      the closest-hit traversal kernel k_trace_q: algorithmic bytes
                 from the library's visit counters over its HIP-event-timed
                 launch duration;
+  batched       8 heightfield frames of other sample seeds in flight
+                together (one submit_many), against the plain frames;
   cpu_baseline  the CPU restatement of the reference renderer (double
                 precision, -O3 -ffast-math, reference tiles and mesh
                 re-traversal) on whole frames, on the host cores this job
@@ -68,7 +70,7 @@ def parse():
                     help="heightfield: mesh octree 'items,depth' (default: the configured 32,8)")
     ap.add_argument("--octree-autotune", action="store_true",
                     help="heightfield: search the mesh octree among 32/8, 48/8, 64/7, 64/8 (untimed)")
-    ap.add_argument("--frames", type=int, default=None, help="room / spheres: frames in flight (8 / 64)")
+    ap.add_argument("--frames", type=int, default=None, help="heightfield / room / spheres: frames in flight (8 / 8 / 64)")
     ap.add_argument("--scenes", type=int, default=None, help="batch: scenes (256); serve: resident scenes (32)")
     ap.add_argument("--requests", type=int, default=2000, help="serve: requests in the Poisson stream")
     ap.add_argument("--load", type=float, default=0.7,
@@ -344,6 +346,13 @@ def run_heightfield(args):
     ctx.synchronize()
     wall = time.perf_counter() - t0
     img = ctx.to_host(out, W * H * 16, np.float32).reshape(H, W, 4)
+    # K frames of other sample seeds in flight together (one submit_many):
+    # one frame's deep bounce levels hold few waves, and the other frames'
+    # levels fill the GPU beside them
+    K = args.frames or 8
+    frames = [S.render_params(W, H, samples=samples, iter=it, seed=0x5EED + k) for k in range(K)]
+    batched, bwall, _ = batched_leg(ctx, [(sid, q, None) for q in frames])
+    log(args, f"heightfield: plain {rays / wall / 1e6:.1f} Mrays/s, {K} frames in flight {batched:.1f} Mrays/s")
     # the same steps again with every launch bracketed by HIP events on the
     # stream it runs on (per-kernel durations for the roofline; kernels are
     # launched one by one while events are recorded).  This pass renders the
@@ -411,6 +420,8 @@ def run_heightfield(args):
                      "kernel": "k_trace_q", "avg_launch_ms": round(avg_launch_ms, 4),
                      "launches_per_step": per_frame_launches, "alg_bytes_per_launch": int(bytes_per_launch),
                      "pass": "one chunk of 2^21 rays per bounce level, one stream"},
+        "batched": {"value": round(batched, 3), "frames_in_flight": K, "wall_s": round(bwall, 4),
+                    "batched_vs_plain": round(batched / value, 3)},
         "cpu_baseline": cpu,
         "image": image,
     }
@@ -691,6 +702,7 @@ def run_all(args):
         "spheres_batched_vs_cpu": sph.get("batched_vs_cpu"),
         "spheres_plain_vs_cpu": sph.get("plain_vs_cpu"),
         "spheres_gpu_ge_100x_cpu": (sph["batched_vs_cpu"] >= 100) if sph.get("batched_vs_cpu") else None,
+        "heightfield_batched_vs_plain": line["batched"]["batched_vs_plain"],
         "room_batched_vs_plain": room["batched_vs_plain"],
         "room_batched_ge_3x_plain": room["batched_vs_plain"] >= 3,
         "batch_poisson_p99_ms": bat["poisson"]["p99_ms"],

@@ -439,7 +439,9 @@ typedef struct sray_tuning {
                                  HIP graph when the pass runs on one lane; 0: launched kernel by
                                  kernel (same kernels, same order) */
     int32_t persistent;       /* 1: traversal kernels run as many waves as stay resident, each
-                                 looping over runs of 64 rays; 0: one run per wave */
+                                 looping over runs of rays (16, 32 or 64: the fewest that cover
+                                 the level in one run per wave); 2: so below the primary level
+                                 only; 0: one run of 64 rays per wave */
     int32_t lanes;            /* 1, 2 or 4 HIP streams a render pass spreads its primary chunks over
                                  (each with its own bounce-level queues), so that one chunk's deep,
                                  sparse levels overlap another chunk's work */

@@ -110,14 +110,29 @@ __global__ void k_raygen(Tabs T, const uint32_t *__restrict__ active, uint32_t c
     q[i] = r;
 }
 
-/* The traversal kernels walk the step in runs of 64 rays per wave, wave w
- * taking runs w, w + W, w + 2W, ... of the W waves of the grid: with a
- * grid as large as the step (one run per wave) the hardware dispatcher
- * balances the load; with a persistent grid (as many waves as stay
- * resident) each wave loops.  Either way the launch shape does not depend
- * on the step's count, which the kernels read from the device. */
-__device__ __forceinline__ uint32_t first_run() { return (blockIdx.x * blockDim.x + threadIdx.x) / 64u * 64u; }
-__device__ __forceinline__ uint32_t run_stride() { return gridDim.x * blockDim.x; }
+/* The traversal kernels walk the step in runs of rays, one run per wave
+ * at a time, wave w taking runs w, w + W, w + 2W, ... of the W waves of
+ * the grid: with a grid as large as the step (runs of 64, one per wave)
+ * the hardware dispatcher balances the load; with a persistent grid (as
+ * many waves as stay resident) each wave loops, and a run holds the
+ * smallest power of two of rays in [RUN_MIN, 64] that covers the step in
+ * one run per wave.  A deep bounce level of a few tens of thousands of
+ * rays then spreads over every SIMD in waves of 16 or 32 rays: more waves
+ * hide each other's load latency, and a wave lasts as long as the slowest
+ * of fewer rays (the other lanes help in the wave-wide leaf tests).  Either
+ * way the launch shape does not depend on the step's count, which the
+ * kernels read from the device, and every ray keeps its own walk. */
+constexpr uint32_t RUN_MIN = 16;
+template <bool PERSIST> __device__ __forceinline__ uint32_t run_len(uint32_t n)
+{
+    if (!PERSIST) return 64u;
+    const uint32_t waves = gridDim.x * blockDim.x / 64u;
+    uint32_t r = RUN_MIN;
+    while (r < 64u && (uint64_t)r * waves < (uint64_t)n) r <<= 1;
+    return r;
+}
+__device__ __forceinline__ uint32_t wave_index() { return (blockIdx.x * blockDim.x + threadIdx.x) / 64u; }
+__device__ __forceinline__ uint32_t grid_waves() { return gridDim.x * blockDim.x / 64u; }
 
 /* closest hit of every queued ray (Scene::cast_ray with a SurfPoint) */
 /* No occupancy hints: with item boxes in the walk, the dense instance
@@ -130,11 +145,12 @@ __global__ __launch_bounds__(BS) void k_trace_q(Tabs T, const QRay *__restrict__
     const uint32_t n = step_n(sn);
     TraceCounts cnt = {0, 0, 0, 0, 0, 0, 0};
     /* without PERSIST the grid has one run per wave: no loop state is kept */
-    for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
+    const uint32_t run = run_len<PERSIST>(n), lane = __lane_id();
+    for (uint32_t base = wave_index() * run; base < n; base = PERSIST ? base + grid_waves() * run : n) {
         /* every lane of the wave walks (the leaf tests of sparse waves are
-         * shared); a lane past the end has no ray */
-        const uint32_t i = base + __lane_id();
-        const bool live = i < n;
+         * shared); a lane past the run or the end has no ray */
+        const uint32_t i = base + lane;
+        const bool live = lane < run && i < n;
         auto walk = [&](const auto &src) {
             if (UNI) {
                 HitRec h = traverse_src<false, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[0]), src, live, &stk[threadIdx.x], BS, cnt);
@@ -183,11 +199,13 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
     __shared__ uint32_t stk[STACK * BS];
     const uint32_t n = step_n(sn) * (uint32_t)L;
     TraceCounts cnt = {0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t base = first_run(); base < n; base = PERSIST ? base + run_stride() : n) {
-        /* every lane of the wave walks; past the end or in an unused slot it
-         * has no ray */
-        const uint32_t i = base + __lane_id();
-        bool live = i < n;
+    const uint32_t run = run_len<PERSIST>(n), lane = __lane_id();
+    for (uint32_t base = wave_index() * run; base < n; base = PERSIST ? base + grid_waves() * run : n) {
+        /* every lane of the wave walks; past the run or the end, or in an
+         * unused slot, it has no ray.  own: the slot is this lane's */
+        const uint32_t i = base + lane;
+        const bool own = lane < run && i < n;
+        bool live = own;
         float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d = o;
         if (live) {
             d = sq[i].d;
@@ -199,11 +217,11 @@ __global__ __launch_bounds__(BS) void k_trace_shadow(Tabs T, const SRay *__restr
         const RayRegs src{v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), __float_as_int(o.w)};
         if (UNI) {
             HitRec h = traverse_src<true, COUNT, true, SPARSE>(trav_scene_uniform(T.scenes[0]), src, live, &stk[threadIdx.x], BS, cnt);
-            if (i < n) occ[i] = h.obj >= 0 ? 1 : 0;
+            if (own) occ[i] = h.obj >= 0 ? 1 : 0;
         } else {
             /* one walk per distinct scene of the wave, as in k_trace_q */
             const uint32_t sidx = live ? T.reqs[__float_as_uint(d.w)].scene : 0u;
-            if (i < n && !live) occ[i] = 0;
+            if (own && !live) occ[i] = 0;
             unsigned long long todo = __ballot(live);
             while (todo) {
                 const uint32_t s0 = read_lane_u(sidx, __ffsll(todo) - 1);
@@ -1259,7 +1277,9 @@ struct Engine {
     /* captured descents, keyed by everything their launches depend on */
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
     bool use_graphs = true;
-    bool persistent = false; /* traversal grids of resident size (each wave loops) */
+    /* traversal grids of resident size (each wave loops): 0 never, 1 at
+     * every bounce level, 2 below the primary level */
+    int persistent = 0;
     /* requests in flight */
     std::vector<ReqDev> hreqs = std::vector<ReqDev>(MAX_SLOTS);
     std::vector<SceneDev> hscenes;
@@ -1419,7 +1439,7 @@ void apply_tuning(Engine &E)
     E.gather_lanes = t.gather_lanes;
     E.chunk = 1u << t.chunk_log2;
     E.use_graphs = t.graphs != 0;
-    E.persistent = t.persistent != 0;
+    E.persistent = t.persistent;
     E.nlanes = t.lanes;
     E.sparse_from = t.sparse_from;
     E.chain = t.chain != 0;
@@ -1670,14 +1690,14 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
     const int bs = E.trace_block;
     const uint32_t small_grid = std::min<uint32_t>(blocks(cap, 256), 2048u);
     /* traversal grid: one run of 64 rays per wave, or the resident waves */
+    const bool P = E.persistent == 1 || (E.persistent == 2 && b > 0), SP = b >= E.sparse_from;
     auto tgrid = [&](auto kern, size_t items) {
         uint32_t g = blocks(items, bs);
-        return E.persistent ? std::min(g, persistent_grid(E, kern, bs)) : g;
+        return P ? std::min(g, persistent_grid(E, kern, bs)) : g;
     };
     {
         ProfScope ps(c, "k_trace_q", st);
         /* the traversal reads only the scene: one scene = the uniform variant */
-        const bool P = E.persistent, SP = b >= E.sparse_from;
         auto kern = E.counting ? (E.one_scene ? trace_q_of<true, true>(bs, P, SP) : trace_q_of<true, false>(bs, P, SP))
                                : (E.one_scene ? trace_q_of<false, true>(bs, P, SP) : trace_q_of<false, false>(bs, P, SP));
         kern<<<tgrid(kern, cap), bs, 0, st>>>(LV.T, q, sn, dptr<int4>(ln.hits), rc6);
@@ -1726,7 +1746,6 @@ int launch_step(Level &LV, int b, uint32_t off, uint32_t cap, int k)
     }
     if (E.L) {
         ProfScope ps(c, "k_trace_shadow", st2);
-        const bool P = E.persistent, SP = b >= E.sparse_from;
         auto kern = E.counting ? (E.one_scene ? trace_shadow_of<true, true>(bs, P, SP)
                                               : trace_shadow_of<true, false>(bs, P, SP))
                                : (E.one_scene ? trace_shadow_of<false, true>(bs, P, SP)
@@ -1875,7 +1894,7 @@ int launch_chunk(Level &LV, uint32_t m, bool one_lane)
     std::vector<uint64_t> key = {LV.c0, m, (uint64_t)E.maxiter, (uint64_t)E.L, (uint64_t)E.K, E.chunk,
                                  (uint64_t)E.trace_block, (uint64_t)E.gather_lanes,
                                  (uint64_t)(E.photons | E.uni << 1 | E.one_scene << 2 | E.counting << 3 |
-                                            E.persistent << 4 | E.knn << 5),
+                                            E.persistent << 4 | E.knn << 6),
                                  (uint64_t)E.Kb,
                                  /* the first level traced by the sparse instances */
                                  (uint64_t)std::min(E.sparse_from, E.maxiter + 1),
@@ -2414,7 +2433,7 @@ int check_tuning(sray_gpu_ctx *c, const sray_tuning &t)
               (t.tile_w == 8 || t.tile_w == 16 || t.tile_w == 32 || t.tile_w == 64) &&
               (t.gather_lanes == 8 || t.gather_lanes == 16 || t.gather_lanes == 32) && t.chunk_log2 >= 16 &&
               t.chunk_log2 <= 21 && (t.gather_cell == 0.25f || t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f) &&
-              (t.graphs == 0 || t.graphs == 1) && (t.persistent == 0 || t.persistent == 1) &&
+              (t.graphs == 0 || t.graphs == 1) && (t.persistent >= 0 && t.persistent <= 2) &&
               (t.lanes == 1 || t.lanes == 2 || t.lanes == 4) && t.sparse_from >= 0 && t.sparse_from <= 64 &&
               (t.chain == 0 || t.chain == 1);
     return ok ? SRAY_OK : fail(c, SRAY_ERR_ARG, "tuning value outside the supported set");
@@ -2457,8 +2476,8 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
     static const int nlanes[] = {1, 2, 4}, sparse[] = {0, 1, 2, 64};
     for (int knob = 0; knob < 10; knob++) {
         if ((knob == 2 || knob == 4) && !photons) continue;
-        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 4 : knob == 7 ? 3
-               : knob == 8 ? 4 : 2;
+        int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 4 : knob == 6 ? 3
+               : knob == 7 ? 3 : knob == 8 ? 4 : 2;
         for (int k = 0; k < nv; k++) {
             sray_tuning t = cur;
             if (knob == 0) t.trace_block = blocks_[k];

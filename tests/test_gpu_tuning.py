@@ -18,7 +18,7 @@ TOL = 1e-6
 def _variants():
     return [dict(trace_block=64), dict(trace_block=256), dict(tile_w=8), dict(tile_w=16),
             dict(chunk_log2=16), dict(gather_lanes=8), dict(gather_lanes=32), dict(gather_cell=0.25),
-            dict(gather_cell=0.5), dict(gather_cell=2.0), dict(graphs=0), dict(persistent=1), dict(sparse_from=64), dict(sparse_from=2)]
+            dict(gather_cell=0.5), dict(gather_cell=2.0), dict(graphs=0), dict(persistent=1), dict(persistent=2), dict(sparse_from=64), dict(sparse_from=2)]
 
 
 def test_tunings_give_the_same_image(ctx):
@@ -75,6 +75,34 @@ def test_sparse_levels_match_checker(ctx, sparse_from):
     oimg, ost = oracle.OracleScene(hs.desc_ptr(), 32).render(p)
     e = float(np.sqrt(np.mean((img.astype(np.float64) - oimg) ** 2)))
     assert e <= 1e-3 and st.rays() == ost.rays(), e
+    ctx.release(sid)
+
+
+@pytest.mark.parametrize("persistent", [1, 2])
+@pytest.mark.parametrize("config,variant,size", [(1, 4, (480, 270)), (2, 10, (320, 240)), (3, 2000, (256, 256))])
+def test_persistent_runs_same_image(ctx, persistent, config, variant, size):
+    """persistent grids walk each level in runs of 16, 32 or 64 rays per
+    wave (the fewest that cover the level), the other lanes idle or helping
+    in the wave-wide leaf tests: every ray keeps its own walk, so the image
+    and the ray counts equal those of one run of 64 per wave bit for bit,
+    and the heightfield matches the checker"""
+    hs = S.HostScene.generate(config, variant=variant)
+    sid = ctx.upload(hs)
+    p = S.render_params(*size, samples=(1, 2), max_var=0.05, iter=6, shadow_samples=2, seed=7)
+    default = ctx.tuning()
+    ctx.set_tuning(persistent=0, chain=0)
+    try:
+        base, bst = ctx.render(sid, p)
+        ctx.set_tuning(persistent=persistent)
+        img, st = ctx.render(sid, p)
+    finally:
+        ctx.set_tuning(persistent=default.persistent, chain=default.chain)
+    assert np.array_equal(img, base)
+    assert st.rays() == bst.rays() and st.rays_shadow == bst.rays_shadow
+    if config == 1:
+        oimg, ost = oracle.OracleScene(hs.desc_ptr(), 32).render(p)
+        e = float(np.sqrt(np.mean((img.astype(np.float64) - oimg) ** 2)))
+        assert e <= 1e-3 and st.rays() == ost.rays(), e
     ctx.release(sid)
 
 
@@ -164,7 +192,7 @@ def test_chain_mixed_scenes_serve(ctx):
 
 def test_bad_tuning_rejected(ctx):
     for kw in (dict(trace_block=96), dict(tile_w=12), dict(gather_lanes=4), dict(chunk_log2=22),
-               dict(gather_cell=3.0), dict(graphs=2), dict(persistent=2), dict(lanes=3), dict(lanes=0),
+               dict(gather_cell=3.0), dict(graphs=2), dict(persistent=3), dict(lanes=3), dict(lanes=0),
                dict(sparse_from=-1), dict(sparse_from=65), dict(chain=2)):
         with pytest.raises(S.SrayError):
             ctx.set_tuning(**kw)
