@@ -213,7 +213,8 @@ def pct(v, q):
 
 def tuning_dict(t):
     return {"trace_block": t.trace_block, "tile_w": t.tile_w, "chunk_log2": t.chunk_log2, "graphs": t.graphs,
-            "persistent": t.persistent, "lanes": t.lanes, "sparse_from": t.sparse_from}
+            "persistent": t.persistent, "lanes": t.lanes, "sparse_from": t.sparse_from,
+            "sample_runs": t.sample_runs}
 
 
 def tuning_arg(args, ctx):

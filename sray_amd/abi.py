@@ -125,7 +125,7 @@ class PhotonStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("trace_block", i32), ("tile_w", i32), ("gather_lanes", i32), ("chunk_log2", i32),
                 ("gather_cell", f32), ("graphs", i32), ("persistent", i32), ("lanes", i32),
-                ("sparse_from", i32), ("chain", i32)]
+                ("sparse_from", i32), ("chain", i32), ("sample_runs", i32)]
 
 
 class GatherQuery(C.Structure):

@@ -91,13 +91,13 @@ __global__ void k_raygen(Tabs T, const uint32_t *__restrict__ active, uint32_t c
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *count = m; /* level 0's count for the level kernels */
     if (i >= m) return;
-    uint32_t pr = active[c0 + i];
+    const uint32_t e = active[c0 + i], pr = ent_pix(e);
     uint32_t rq = UNI ? 0u : T.preq[pr];
     const ReqDev &Q = T.reqs[rq];
     const RenderDev &P = Q.P;
     uint32_t lp = pr - Q.pix_base;
     int x = P.x0 + (int)(lp % (uint32_t)P.W), y = P.y0 + (int)(lp / (uint32_t)P.W);
-    int sample = T.nsamp[pr];
+    int sample = T.nsamp[pr] + (int)ent_off(e);
     V3 o, d;
     int time;
     primary_ray(Q.C, P, x, y, sample, o, d, time);
@@ -297,9 +297,9 @@ __device__ __forceinline__ void shade_ray(const Tabs &T, const uint32_t *__restr
     const RenderDev &P = Q.P;
     /* SCN1: every request in flight renders scene 0 */
     const SceneDev S = T.scenes[(UNI || SCN1) ? 0u : Q.scene];
-    uint32_t pr = active[c0 + slot];
+    const uint32_t e = active[c0 + slot], pr = ent_pix(e);
     uint32_t lp = pr - Q.pix_base;
-    int sample = T.nsamp[pr];
+    int sample = T.nsamp[pr] + (int)ent_off(e);
     int iter = (int)R.m.z;
     V3 W = v3(R.w.x, R.w.y, R.w.z);
     float energy = R.d.w;
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ATTR || LMAX
         uint32_t rq = 0;
         if (i < m) {
             /* k_raygen */
-            const uint32_t pr = active[c0 + i];
+            const uint32_t e = active[c0 + i], pr = ent_pix(e);
             rq = UNI ? 0u : T.preq[pr];
             const ReqDev &Q = T.reqs[rq];
             const RenderDev &P = Q.P;
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ATTR || LMAX
             const int x = P.x0 + (int)(lp % (uint32_t)P.W), y = P.y0 + (int)(lp / (uint32_t)P.W);
             V3 o, d;
             int time;
-            primary_ray(Q.C, P, x, y, T.nsamp[pr], o, d, time);
+            primary_ray(Q.C, P, x, y, T.nsamp[pr] + (int)ent_off(e), o, d, time);
             QRay R;
             R.o = make_float4(o.x, o.y, o.z, __int_as_float(time));
             R.d = make_float4(d.x, d.y, d.z, 1.0f);
@@ -937,10 +937,13 @@ __global__ void k_iota(uint32_t *p, uint32_t n, uint32_t base)
  * pixels (tw * th = 64, one wave), rows of tiles top to bottom, tiles left
  * to right, row-major inside a tile (tw = 64: plain row order) */
 __global__ void k_tile_order(uint32_t *__restrict__ act, uint32_t npix, uint32_t base, uint32_t W, uint32_t H,
-                             uint32_t tw)
+                             uint32_t tw, uint32_t run)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npix) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npix * run) return;
+    /* run entries per pixel: sample offsets 0..run-1 */
+    const uint32_t i = t / run, off = t - i * run;
+    act += (size_t)i * (run - 1u) + off;
     uint32_t th = 64u / tw;
     uint32_t ty = i / (th * W);
     uint32_t o = i - ty * th * W;
@@ -949,7 +952,7 @@ __global__ void k_tile_order(uint32_t *__restrict__ act, uint32_t npix, uint32_t
     uint32_t wt = min(tw, W - tw * tx);
     uint32_t j = o - tx * tw * hr;
     uint32_t y = th * ty + j / wt, x = tw * tx + j % wt;
-    act[i] = base + y * W + x;
+    act[i] = (base + y * W + x) | (off << ENT_SHIFT);
 }
 
 __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
@@ -968,7 +971,7 @@ __device__ __forceinline__ void slot_count(const uint32_t *__restrict__ active, 
     __syncthreads();
     for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
         const uint32_t i = b + threadIdx.x;
-        wave_keyed_add(scnt, i < n ? preq[active[i]] : 0u, 1u, i < n);
+        wave_keyed_add(scnt, i < n ? preq[ent_pix(active[i])] : 0u, 1u, i < n);
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < MAX_SLOTS; k += blockDim.x)
@@ -984,7 +987,11 @@ __global__ void k_slot_count_dev(const uint32_t *__restrict__ active, const uint
 
 
 /* end of one sample pass: framebuffer accumulation in float
- * (src/render.cc:226-229) and the variance stop test (:232-237, 261-270) */
+ * (src/render.cc:226-229) and the variance stop test (:232-237, 261-270).
+ * A pixel whose request takes several samples per pass has them in
+ * consecutive entries of the chunk: its first entry adds them one by one,
+ * in sample order, with the stop test after each (the others only leave
+ * the list), so the sums and the stop are those of one sample per pass. */
 template <bool UNI>
 __global__ void k_pass_finish(Tabs T, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m,
                               const float4 *__restrict__ acc, float4 *__restrict__ fb, float4 *__restrict__ subpix,
@@ -992,39 +999,47 @@ __global__ void k_pass_finish(Tabs T, const uint32_t *__restrict__ active, uint3
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    uint32_t pr = active[c0 + i];
+    const uint32_t e = active[c0 + i], pr = ent_pix(e);
+    if (ent_off(e) != 0u) {
+        keep[c0 + i] = 0;
+        return;
+    }
     uint32_t rq = UNI ? 0u : T.preq[pr];
     const ReqDev &Q = T.reqs[rq];
     const RenderDev &P = Q.P;
+    const int maxs = P.max_samples;
     int sample = nsamp[pr];
-    float4 c = acc[i];
     float4 f = fb[pr];
-    f.x = f.x + c.x;
-    f.y = f.y + c.y;
-    f.z = f.z + c.z;
-    f.w = f.w + c.w;
-    fb[pr] = f;
-    int ns = sample + 1;
-    nsamp[pr] = ns;
-    int maxs = P.max_samples;
-    bool k = ns < maxs;
-    if (maxs > 1) {
-        float4 *sp = subpix + Q.sub_base + (size_t)(pr - Q.pix_base) * (size_t)maxs;
-        sp[sample] = c;
-        if (ns >= P.min_samples && ns > 1) {
-            float rcp = 1.0f / (float)ns;
-            float4 mean = make_float4(f.x * rcp, f.y * rcp, f.z * rcp, f.w * rcp);
-            float sq = 0.0f;
-            for (int j = 0; j < ns; j++) {
-                float4 s = sp[j];
-                float dx = mean.x - s.x, dy = mean.y - s.y, dz = mean.z - s.z, dw = mean.w - s.w;
-                float l2 = dx * dx + dy * dy + dz * dz + dw * dw;
-                sq = sq + l2;
+    bool k = true;
+    for (uint32_t o = 0; o < Q.run && k; o++) {
+        float4 c = acc[i + o];
+        f.x = f.x + c.x;
+        f.y = f.y + c.y;
+        f.z = f.z + c.z;
+        f.w = f.w + c.w;
+        int ns = sample + 1;
+        k = ns < maxs;
+        if (maxs > 1) {
+            float4 *sp = subpix + Q.sub_base + (size_t)(pr - Q.pix_base) * (size_t)maxs;
+            sp[sample] = c;
+            if (ns >= P.min_samples && ns > 1) {
+                float rcp = 1.0f / (float)ns;
+                float4 mean = make_float4(f.x * rcp, f.y * rcp, f.z * rcp, f.w * rcp);
+                float sq = 0.0f;
+                for (int j = 0; j < ns; j++) {
+                    float4 s = sp[j];
+                    float dx = mean.x - s.x, dy = mean.y - s.y, dz = mean.z - s.z, dw = mean.w - s.w;
+                    float l2 = dx * dx + dy * dy + dz * dz + dw * dw;
+                    sq = sq + l2;
+                }
+                float v = sq * rcp;
+                if (v < P.max_var) k = false;
             }
-            float v = sq * rcp;
-            if (v < P.max_var) k = false;
         }
+        sample = ns;
     }
+    fb[pr] = f;
+    nsamp[pr] = sample;
     keep[c0 + i] = k ? 1 : 0;
 }
 
@@ -1272,6 +1287,7 @@ struct Engine {
     int nlanes = 1; /* lanes a pass spreads its primary chunks over */
     int sparse_from = 64; /* first bounce level traced by the SPARSE kernel instances */
     bool chain = true;    /* one-child chains in one kernel when the pass allows it */
+    bool sample_runs = true; /* a fixed-count request takes all its samples in its first pass */
     bool attr = true;     /* some scene in flight has meshes with vertex attributes */
     std::map<const void *, uint32_t> pgrid; /* persistent grid per traversal kernel instance */
     /* captured descents, keyed by everything their launches depend on */
@@ -1362,7 +1378,12 @@ int grow_pixels(Engine &E, uint64_t need)
 {
     sray_gpu_ctx *c = E.c;
     uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(E.pix.cap * 2, 1u << 20), E.pix.cap + need);
-    if (cap > 0xfffff000ull) return fail(c, SRAY_ERR_LIMIT, "pixel arena beyond 2^32 pixels");
+    /* active-list entries hold an arena pixel in 28 bits */
+    if (cap > (uint64_t)ENT_PIX + 1u) {
+        if (E.pix.cap + need > (uint64_t)ENT_PIX + 1u)
+            return fail(c, SRAY_ERR_LIMIT, "pixel arena beyond 2^28 pixels in flight");
+        cap = (uint64_t)ENT_PIX + 1u;
+    }
     uint64_t old = E.pix.cap;
     int rc;
     if ((rc = regrow(c, E.fb, cap * 16, old * 16)) || (rc = regrow(c, E.nsamp, cap * 4, old * 4)) ||
@@ -1370,6 +1391,22 @@ int grow_pixels(Engine &E, uint64_t need)
         (rc = regrow(c, E.act[E.cur], cap * 4, (size_t)E.nact * 4)) || (rc = regrow(c, E.act[E.cur ^ 1], cap * 4, 0)))
         return rc;
     E.pix.grow(cap);
+    return SRAY_OK;
+}
+
+/* room for `need` active-list entries (a pixel may have several: one per
+ * sample its request takes in a pass), keeping the current list */
+int grow_list(Engine &E, uint64_t need)
+{
+    sray_gpu_ctx *c = E.c;
+    const uint64_t have = std::min<uint64_t>(std::min(E.act[0].bytes, E.act[1].bytes) / 4, E.keep.bytes);
+    if (need <= have) return SRAY_OK;
+    if (need > 0xfffff000ull) return fail(c, SRAY_ERR_LIMIT, "active list beyond 2^32 entries");
+    const uint64_t cap = std::min<uint64_t>(std::max(need, have * 2), 0xfffff000ull);
+    int rc;
+    if ((rc = regrow(c, E.act[E.cur], cap * 4, (size_t)E.nact * 4)) || (rc = regrow(c, E.act[E.cur ^ 1], cap * 4, 0)) ||
+        (rc = regrow(c, E.keep, cap, 0)))
+        return rc;
     return SRAY_OK;
 }
 
@@ -1443,6 +1480,7 @@ void apply_tuning(Engine &E)
     E.nlanes = t.lanes;
     E.sparse_from = t.sparse_from;
     E.chain = t.chain != 0;
+    E.sample_runs = t.sample_runs != 0;
 }
 
 /* places a request in the arena and the active list */
@@ -1510,17 +1548,37 @@ int admit(Engine &E, Job *j)
     }
     Q.pix_base = (uint32_t)base;
     Q.sub_base = sbase;
+    /* A request whose pixels all take exactly max_samples (min_samples >=
+     * max_samples, so the variance test never stops one early) takes them
+     * all in its first pass when max_samples is a power of two <= 16: a
+     * pixel's samples are consecutive entries of one aligned group of 16,
+     * which no primary chunk (a multiple of 1024 entries) splits.  The
+     * entries start at the list's last multiple of 16; the fewer than 16
+     * entries past it (one sample each, as every entry of a group that a
+     * run does not fill) move behind the request's. */
+    const bool runs = E.sample_runs && Q.P.min_samples >= Q.P.max_samples &&
+                      (maxs == 2 || maxs == 4 || maxs == 8 || maxs == 16) && (npix * maxs) % 16 == 0;
+    Q.run = runs ? (uint32_t)maxs : 1u;
+    const uint64_t nent = npix * Q.run;
+    const uint64_t at = runs ? (E.nact & ~15ull) : E.nact, tail = E.nact - at;
+    if ((rc = grow_list(E, E.nact + nent))) {
+        E.pix.release(base, npix);
+        if (maxs > 1) E.sub.release(sbase, npix * maxs);
+        return rc;
+    }
+    uint32_t *act = dptr<uint32_t>(E.act[E.cur]);
+    if (tail) HIPCHK(c, hipMemcpyAsync(act + at + nent, act + at, tail * 4, hipMemcpyDeviceToDevice, c->stream));
     k_fill_f4<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<float4>(E.fb) + base, npix, make_float4(0, 0, 0, 0));
     k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.nsamp) + base, npix, 0u);
     k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.preq) + base, npix, s);
-    k_tile_order<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.act[E.cur]) + E.nact, (uint32_t)npix,
-                                                           (uint32_t)base, (uint32_t)Q.P.W,
-                                                           (uint32_t)(Q.P.y1 - Q.P.y0), E.tile_w);
+    k_tile_order<<<blocks(nent, 256), 256, 0, c->stream>>>(act + at, (uint32_t)npix, (uint32_t)base,
+                                                           (uint32_t)Q.P.W, (uint32_t)(Q.P.y1 - Q.P.y0),
+                                                           E.tile_w, Q.run);
     HIPCHK(c, hipGetLastError());
-    E.nact += (uint32_t)npix;
+    E.nact += (uint32_t)nent;
     E.hreqs[s] = Q;
     E.slot_job[s] = j;
-    E.slot_active[s] = (uint32_t)npix;
+    E.slot_active[s] = (uint32_t)nent;
     E.live++;
     E.live_bytes += job_bytes(*j);
     E.hi = std::max(E.hi, s + 1);
@@ -2435,7 +2493,7 @@ int check_tuning(sray_gpu_ctx *c, const sray_tuning &t)
               t.chunk_log2 <= 21 && (t.gather_cell == 0.25f || t.gather_cell == 0.5f || t.gather_cell == 1.0f || t.gather_cell == 2.0f) &&
               (t.graphs == 0 || t.graphs == 1) && (t.persistent >= 0 && t.persistent <= 2) &&
               (t.lanes == 1 || t.lanes == 2 || t.lanes == 4) && t.sparse_from >= 0 && t.sparse_from <= 64 &&
-              (t.chain == 0 || t.chain == 1);
+              (t.chain == 0 || t.chain == 1) && (t.sample_runs == 0 || t.sample_runs == 1);
     return ok ? SRAY_OK : fail(c, SRAY_ERR_ARG, "tuning value outside the supported set");
 }
 
@@ -2474,7 +2532,7 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
     static const float cells[] = {0.25f, 0.5f, 1.0f, 2.0f};
     bool photons = p->caust_photons || p->gi_photons;
     static const int nlanes[] = {1, 2, 4}, sparse[] = {0, 1, 2, 64};
-    for (int knob = 0; knob < 10; knob++) {
+    for (int knob = 0; knob < 11; knob++) {
         if ((knob == 2 || knob == 4) && !photons) continue;
         int nv = knob == 0 ? 3 : knob == 1 ? 4 : knob == 2 ? 3 : knob == 3 ? 4 : knob == 4 ? 4 : knob == 6 ? 3
                : knob == 7 ? 3 : knob == 8 ? 4 : 2;
@@ -2489,7 +2547,8 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
             else if (knob == 6) t.persistent = k;
             else if (knob == 7) t.lanes = nlanes[k];
             else if (knob == 8) t.sparse_from = sparse[k];
-            else t.chain = k;
+            else if (knob == 9) t.chain = k;
+            else t.sample_runs = k;
             if (!memcmp(&t, &cur, sizeof t)) continue;
             c->tune = t;
             double ms = 0;

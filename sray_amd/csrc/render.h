@@ -88,7 +88,19 @@ struct ReqDev {
     uint32_t scene;    /* index into the batch's scene table */
     uint32_t pix_base; /* first arena pixel */
     uint64_t sub_base; /* first per-sample record (max_samples per pixel) */
+    uint32_t run;      /* samples a pass takes of each active pixel: 1, or all of
+                          max_samples (a power of two <= 16) for a request whose
+                          pixels take exactly max_samples */
+    uint32_t pad_;
 };
+
+/* An active-list entry: the arena pixel in its low 28 bits and, in the top
+ * four, the sample's offset from the pixel's samples taken.  A pixel of a
+ * request with run r > 1 has r consecutive entries (offsets 0..r-1) in one
+ * aligned group of 16 entries, so a primary chunk never splits them. */
+constexpr uint32_t ENT_SHIFT = 28, ENT_PIX = (1u << ENT_SHIFT) - 1u;
+__host__ __device__ __forceinline__ uint32_t ent_pix(uint32_t e) { return e & ENT_PIX; }
+__host__ __device__ __forceinline__ uint32_t ent_off(uint32_t e) { return e >> ENT_SHIFT; }
 
 /* kernel view of a batch */
 struct Tabs {

@@ -190,10 +190,60 @@ def test_chain_mixed_scenes_serve(ctx):
     ctx.release(sb)
 
 
+def _runs_jobs(ctx):
+    hb = S.HostScene.generate(3, variant=2000)
+    hr = S.HostScene.generate(2, variant=10)
+    sb, sr = ctx.upload(hb), ctx.upload(hr)
+    ctx.shoot_photons(sr, 20000, 20000, 300.0, 5, 0, 0, 3)
+    room = dict(shadow_samples=2, caust_photons=1, gi_photons=1, gather_dist=0.03)
+    jobs = [
+        # 231 pixels, adaptive: one sample per pass, and the list length
+        # is not a multiple of 16 when the next request joins
+        (sr, S.render_params(33, 7, samples=(1, 4), max_var=0.05, iter=4, seed=1, **room)),
+        (sb, S.render_params(64, 48, samples=(8, 8), iter=4, mblur=1, seed=2)),   # 8 per pass
+        (sr, S.render_params(40, 30, samples=(4, 4), iter=4, seed=3, **room)),    # 4 per pass
+        (sb, S.render_params(50, 30, samples=(16, 16), iter=3, seed=4)),          # 16 per pass
+        (sb, S.render_params(31, 17, samples=(8, 8), iter=3, seed=5)),            # 527 x 8 entries: one per pass
+        (sr, S.render_params(24, 16, samples=(2, 2), iter=4, seed=6, **room)),    # 2 per pass
+    ]
+    return (sb, sr), jobs
+
+
+def test_sample_runs_same_images(ctx):
+    """requests whose pixels take exactly max_samples (a power of two) take
+    them all in their first pass, one queue entry per pixel and sample,
+    summed in sample order: each image and ray count equals one sample per
+    pass bit for bit, rendered alone and submitted together with requests
+    of one sample per pass (whose entries leave the list's length off a
+    multiple of 16, so the run request's entries start at the last multiple
+    and the entries past it move behind them)"""
+    sids, jobs = _runs_jobs(ctx)
+    default = ctx.tuning().sample_runs
+    try:
+        ctx.set_tuning(sample_runs=0)
+        base = [ctx.render(sid, p) for sid, p in jobs]
+        ctx.set_tuning(sample_runs=1)
+        for k, (sid, p) in enumerate(jobs):
+            img, st = ctx.render(sid, p)
+            assert np.array_equal(img, base[k][0]), k
+            assert st.rays() == base[k][1].rays() and st.rays_shadow == base[k][1].rays_shadow, k
+        for rnd in range(2):
+            tickets = ctx.submit_many([(sid, p, None, None) for sid, p in jobs])
+            for k, t in enumerate(tickets):
+                img, res = ctx.poll(t)
+                print(f"round {rnd} request {k}: rays {res.rays} (one per pass {base[k][1].rays()})")
+                assert np.array_equal(img, base[k][0]), (rnd, k)
+                assert res.rays == base[k][1].rays(), (rnd, k)
+    finally:
+        ctx.set_tuning(sample_runs=default)
+    for sid in sids:
+        ctx.release(sid)
+
+
 def test_bad_tuning_rejected(ctx):
     for kw in (dict(trace_block=96), dict(tile_w=12), dict(gather_lanes=4), dict(chunk_log2=22),
                dict(gather_cell=3.0), dict(graphs=2), dict(persistent=3), dict(lanes=3), dict(lanes=0),
-               dict(sparse_from=-1), dict(sparse_from=65), dict(chain=2)):
+               dict(sparse_from=-1), dict(sparse_from=65), dict(chain=2), dict(sample_runs=2)):
         with pytest.raises(S.SrayError):
             ctx.set_tuning(**kw)
 
@@ -210,7 +260,7 @@ def test_autotune(ctx):
     after, _ = ctx.render(sid, p)
     assert float(np.abs(after - before).max()) <= TOL
     ctx.set_tuning(trace_block=128, tile_w=64, gather_lanes=16, chunk_log2=21, gather_cell=1.0, graphs=1, persistent=0,
-                   lanes=4, sparse_from=64, chain=1)
+                   lanes=4, sparse_from=64, chain=1, sample_runs=1)
     ctx.release(sid)
 
 
