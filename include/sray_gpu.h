@@ -453,11 +453,11 @@ typedef struct sray_tuning {
                                  child and none uses photon maps (and at most 4 shadow rays per
                                  hit), one lane follows each sample's rays to the end in a single
                                  kernel; 0: always the level queues (same image) */
-    int32_t sample_runs;      /* 1: a request whose pixels all take exactly max_samples samples
-                                 (min_samples >= max_samples, max_samples 2, 4, 8 or 16) takes
-                                 them all in its first pass, each pixel's samples in consecutive
-                                 queue entries and summed in sample order (same image, one pass
-                                 instead of max_samples); 0: one sample per pass */
+    int32_t sample_runs;      /* 1: a request's first pass takes the samples every pixel takes
+                                 (max(min_samples, 2) up to max_samples, as a power of two <= 16:
+                                 all of them when min_samples >= max_samples), each pixel's
+                                 samples in consecutive queue entries and summed in sample
+                                 order (same image, fewer passes); 0: one sample per pass */
 } sray_tuning;
 int  sray_gpu_set_tuning(sray_gpu_ctx *ctx, const sray_tuning *t);
 int  sray_gpu_get_tuning(sray_gpu_ctx *ctx, sray_tuning *t);

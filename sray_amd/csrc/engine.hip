@@ -1011,7 +1011,9 @@ __global__ void k_pass_finish(Tabs T, const uint32_t *__restrict__ active, uint3
     int sample = nsamp[pr];
     float4 f = fb[pr];
     bool k = true;
-    for (uint32_t o = 0; o < Q.run && k; o++) {
+    /* the first pass's run (every sample of it is taken, see admit) */
+    const uint32_t run = sample == 0 ? Q.run : 1u;
+    for (uint32_t o = 0; o < run && k; o++) {
         float4 c = acc[i + o];
         f.x = f.x + c.x;
         f.y = f.y + c.y;
@@ -1548,17 +1550,21 @@ int admit(Engine &E, Job *j)
     }
     Q.pix_base = (uint32_t)base;
     Q.sub_base = sbase;
-    /* A request whose pixels all take exactly max_samples (min_samples >=
-     * max_samples, so the variance test never stops one early) takes them
-     * all in its first pass when max_samples is a power of two <= 16: a
-     * pixel's samples are consecutive entries of one aligned group of 16,
-     * which no primary chunk (a multiple of 1024 entries) splits.  The
-     * entries start at the list's last multiple of 16; the fewer than 16
-     * entries past it (one sample each, as every entry of a group that a
-     * run does not fill) move behind the request's. */
-    const bool runs = E.sample_runs && Q.P.min_samples >= Q.P.max_samples &&
-                      (maxs == 2 || maxs == 4 || maxs == 8 || maxs == 16) && (npix * maxs) % 16 == 0;
-    Q.run = runs ? (uint32_t)maxs : 1u;
+    /* The first pass takes the samples every pixel takes: the variance
+     * test runs from max(min_samples, 2) samples on (all of max_samples
+     * when min_samples >= max_samples), so that many are always taken.
+     * When that count, or its largest power of two <= 16, is above one, a
+     * pixel's first samples are consecutive entries of one aligned group
+     * of 16, which no primary chunk (a multiple of 1024 entries) splits;
+     * later passes take one sample per pixel.  The entries start at the
+     * list's last multiple of 16; the fewer than 16 entries past it (one
+     * sample each, as every entry of a group that a run does not fill)
+     * move behind the request's. */
+    uint32_t first = (uint32_t)std::min<uint64_t>(maxs, (uint64_t)std::max(Q.P.min_samples, 2));
+    first = std::min(first, 16u);
+    while (first & (first - 1u)) first &= first - 1u;
+    const bool runs = E.sample_runs && first > 1u && (npix * first) % 16 == 0;
+    Q.run = runs ? first : 1u;
     const uint64_t nent = npix * Q.run;
     const uint64_t at = runs ? (E.nact & ~15ull) : E.nact, tail = E.nact - at;
     if ((rc = grow_list(E, E.nact + nent))) {

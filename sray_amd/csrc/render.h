@@ -88,9 +88,9 @@ struct ReqDev {
     uint32_t scene;    /* index into the batch's scene table */
     uint32_t pix_base; /* first arena pixel */
     uint64_t sub_base; /* first per-sample record (max_samples per pixel) */
-    uint32_t run;      /* samples a pass takes of each active pixel: 1, or all of
-                          max_samples (a power of two <= 16) for a request whose
-                          pixels take exactly max_samples */
+    uint32_t run;      /* samples the first pass takes of each pixel (a power of
+                          two <= 16 that every pixel takes: min_samples, at
+                          least 2, at most max_samples); later passes take one */
     uint32_t pad_;
 };
 

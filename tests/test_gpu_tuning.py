@@ -205,18 +205,23 @@ def _runs_jobs(ctx):
         (sb, S.render_params(50, 30, samples=(16, 16), iter=3, seed=4)),          # 16 per pass
         (sb, S.render_params(31, 17, samples=(8, 8), iter=3, seed=5)),            # 527 x 8 entries: one per pass
         (sr, S.render_params(24, 16, samples=(2, 2), iter=4, seed=6, **room)),    # 2 per pass
+        # adaptive: the first pass takes the two samples every pixel takes
+        # (the variance test starts at two), then one per pass
+        (sr, S.render_params(32, 16, samples=(1, 8), max_var=0.05, iter=4, seed=7, **room)),
+        (sb, S.render_params(48, 20, samples=(3, 16), max_var=0.02, iter=3, seed=8)),
     ]
     return (sb, sr), jobs
 
 
 def test_sample_runs_same_images(ctx):
-    """requests whose pixels take exactly max_samples (a power of two) take
-    them all in their first pass, one queue entry per pixel and sample,
-    summed in sample order: each image and ray count equals one sample per
-    pass bit for bit, rendered alone and submitted together with requests
-    of one sample per pass (whose entries leave the list's length off a
-    multiple of 16, so the run request's entries start at the last multiple
-    and the entries past it move behind them)"""
+    """a request's first pass takes the samples every pixel takes (all of
+    max_samples when min_samples >= max_samples; else max(min_samples, 2)),
+    one queue entry per pixel and sample, summed in sample order with the
+    variance test after each: each image and ray count equals one sample
+    per pass bit for bit, rendered alone and submitted together with
+    requests of one sample per pass (whose entries leave the list's length
+    off a multiple of 16, so the run request's entries start at the last
+    multiple and the entries past it move behind them)"""
     sids, jobs = _runs_jobs(ctx)
     default = ctx.tuning().sample_runs
     try:
