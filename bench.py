@@ -330,7 +330,7 @@ def run_heightfield(args):
     out = ctx.malloc(W * H * 4 * 4)
     tuning = tuning_arg(args, ctx)
     if tuning is None and not args.no_autotune:
-        t, tms = ctx.autotune(sid, p, rounds=3)
+        t, tms = ctx.autotune(sid, p, rounds=5)
         tuning = tuning_dict(t)
         log(args, f"autotune: {tuning}, {tms:.2f} ms/frame")
     if tuning is None:
@@ -350,9 +350,18 @@ def run_heightfield(args):
     # K frames of other sample seeds in flight together (one submit_many):
     # one frame's deep bounce levels hold few waves, and the other frames'
     # levels fill the GPU beside them
+    # The frames take 2^21-ray chunks: a bounce level may outgrow its chunk
+    # (two children per ray), and each chunk that may is finished with a
+    # read-back of its level counts, a host synchronisation that holds the
+    # other lanes' next chunks back, so smaller chunks (which the one-frame
+    # autotune may pick within its noise) cost the frames in flight half
+    # their rate.
     K = args.frames or 8
     frames = [S.render_params(W, H, samples=samples, iter=it, seed=0x5EED + k) for k in range(K)]
+    plain_chunk = ctx.tuning().chunk_log2
+    ctx.set_tuning(chunk_log2=21)
     batched, bwall, _ = batched_leg(ctx, [(sid, q, None) for q in frames])
+    ctx.set_tuning(chunk_log2=plain_chunk)
     log(args, f"heightfield: plain {rays / wall / 1e6:.1f} Mrays/s, {K} frames in flight {batched:.1f} Mrays/s")
     # the same steps again with every launch bracketed by HIP events on the
     # stream it runs on (per-kernel durations for the roofline; kernels are
@@ -422,7 +431,7 @@ def run_heightfield(args):
                      "launches_per_step": per_frame_launches, "alg_bytes_per_launch": int(bytes_per_launch),
                      "pass": "one chunk of 2^21 rays per bounce level, one stream"},
         "batched": {"value": round(batched, 3), "frames_in_flight": K, "wall_s": round(bwall, 4),
-                    "batched_vs_plain": round(batched / value, 3)},
+                    "chunk_log2": 21, "batched_vs_plain": round(batched / value, 3)},
         "cpu_baseline": cpu,
         "image": image,
     }

@@ -619,6 +619,40 @@ __global__ __launch_bounds__(256) void k_shade(Tabs T, const uint32_t *__restric
  * term, up to LMAX shadow rays with their contributions, and the one child
  * (every request in flight spawns at most one child per ray and has no
  * photon maps, so there are no gather requests and no diffuse children). */
+/* one sample c of arena pixel pr added to its sum f in float
+ * (src/render.cc:226-229), its record kept for the variance stop test
+ * (:232-237, 261-270); false when the pixel stops after it */
+__device__ __forceinline__ bool add_sample(const ReqDev &Q, uint32_t pr, int sample, float4 c, float4 &f,
+                                           float4 *__restrict__ subpix)
+{
+    const RenderDev &P = Q.P;
+    const int maxs = P.max_samples;
+    f.x = f.x + c.x;
+    f.y = f.y + c.y;
+    f.z = f.z + c.z;
+    f.w = f.w + c.w;
+    int ns = sample + 1;
+    bool k = ns < maxs;
+    if (maxs > 1) {
+        float4 *sp = subpix + Q.sub_base + (size_t)(pr - Q.pix_base) * (size_t)maxs;
+        sp[sample] = c;
+        if (ns >= P.min_samples && ns > 1) {
+            float rcp = 1.0f / (float)ns;
+            float4 mean = make_float4(f.x * rcp, f.y * rcp, f.z * rcp, f.w * rcp);
+            float sq = 0.0f;
+            for (int j = 0; j < ns; j++) {
+                float4 s = sp[j];
+                float dx = mean.x - s.x, dy = mean.y - s.y, dz = mean.z - s.z, dw = mean.w - s.w;
+                float l2 = dx * dx + dy * dy + dz * dz + dw * dw;
+                sq = sq + l2;
+            }
+            float v = sq * rcp;
+            if (v < P.max_var) k = false;
+        }
+    }
+    return k;
+}
+
 template <int LMAX> struct ChainSink {
     float4 *acc;   /* the sample's sum: the direct term is added when set */
     SRay sh[LMAX];
@@ -933,15 +967,23 @@ __global__ void k_iota(uint32_t *p, uint32_t n, uint32_t base)
     if (i < n) p[i] = base + i;
 }
 
-/* the arena pixels of a W x H rectangle in tile order: tiles of tw x th
- * pixels (tw * th = 64, one wave), rows of tiles top to bottom, tiles left
- * to right, row-major inside a tile (tw = 64: plain row order) */
-__global__ void k_tile_order(uint32_t *__restrict__ act, uint32_t npix, uint32_t base, uint32_t W, uint32_t H,
-                             uint32_t tw, uint32_t run)
+/* a request's admission in one launch: its arena pixels cleared (sum,
+ * samples taken, request slot) and its active-list entries, the pixels of
+ * a W x H rectangle in tile order: tiles of tw x th pixels (tw * th = 64,
+ * one wave), rows of tiles top to bottom, tiles left to right, row-major
+ * inside a tile (tw = 64: plain row order), run entries per pixel (sample
+ * offsets 0..run-1) */
+__global__ void k_admit(uint32_t *__restrict__ act, float4 *__restrict__ fb, uint32_t *__restrict__ nsamp,
+                        uint32_t *__restrict__ preq, uint32_t npix, uint32_t base, uint32_t W, uint32_t H,
+                        uint32_t tw, uint32_t run, uint32_t slot)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < npix) {
+        fb[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        nsamp[t] = 0u;
+        preq[t] = slot;
+    }
     if (t >= npix * run) return;
-    /* run entries per pixel: sample offsets 0..run-1 */
     const uint32_t i = t / run, off = t - i * run;
     act += (size_t)i * (run - 1u) + off;
     uint32_t th = 64u / tw;
@@ -986,12 +1028,12 @@ __global__ void k_slot_count_dev(const uint32_t *__restrict__ active, const uint
 }
 
 
-/* end of one sample pass: framebuffer accumulation in float
- * (src/render.cc:226-229) and the variance stop test (:232-237, 261-270).
- * A pixel whose request takes several samples per pass has them in
- * consecutive entries of the chunk: its first entry adds them one by one,
- * in sample order, with the stop test after each (the others only leave
- * the list), so the sums and the stop are those of one sample per pass. */
+/* end of one sample pass: each active pixel's sample added (add_sample).
+ * A pixel whose request takes several samples in its first pass has them
+ * in consecutive entries of the chunk: its first entry adds them one by
+ * one, in sample order, with the stop test after each (the others only
+ * leave the list), so the sums and the stop are those of one sample per
+ * pass. */
 template <bool UNI>
 __global__ void k_pass_finish(Tabs T, const uint32_t *__restrict__ active, uint32_t c0, uint32_t m,
                               const float4 *__restrict__ acc, float4 *__restrict__ fb, float4 *__restrict__ subpix,
@@ -1006,39 +1048,14 @@ __global__ void k_pass_finish(Tabs T, const uint32_t *__restrict__ active, uint3
     }
     uint32_t rq = UNI ? 0u : T.preq[pr];
     const ReqDev &Q = T.reqs[rq];
-    const RenderDev &P = Q.P;
-    const int maxs = P.max_samples;
     int sample = nsamp[pr];
     float4 f = fb[pr];
     bool k = true;
     /* the first pass's run (every sample of it is taken, see admit) */
     const uint32_t run = sample == 0 ? Q.run : 1u;
     for (uint32_t o = 0; o < run && k; o++) {
-        float4 c = acc[i + o];
-        f.x = f.x + c.x;
-        f.y = f.y + c.y;
-        f.z = f.z + c.z;
-        f.w = f.w + c.w;
-        int ns = sample + 1;
-        k = ns < maxs;
-        if (maxs > 1) {
-            float4 *sp = subpix + Q.sub_base + (size_t)(pr - Q.pix_base) * (size_t)maxs;
-            sp[sample] = c;
-            if (ns >= P.min_samples && ns > 1) {
-                float rcp = 1.0f / (float)ns;
-                float4 mean = make_float4(f.x * rcp, f.y * rcp, f.z * rcp, f.w * rcp);
-                float sq = 0.0f;
-                for (int j = 0; j < ns; j++) {
-                    float4 s = sp[j];
-                    float dx = mean.x - s.x, dy = mean.y - s.y, dz = mean.z - s.z, dw = mean.w - s.w;
-                    float l2 = dx * dx + dy * dy + dz * dz + dw * dw;
-                    sq = sq + l2;
-                }
-                float v = sq * rcp;
-                if (v < P.max_var) k = false;
-            }
-        }
-        sample = ns;
+        k = add_sample(Q, pr, sample, acc[i + o], f, subpix);
+        sample++;
     }
     fb[pr] = f;
     nsamp[pr] = sample;
@@ -1065,6 +1082,42 @@ __global__ void k_finalize(RenderDev P, const float4 *__restrict__ fb, const int
     out[(size_t)i * 4 + 1] = f.y * rcp;
     out[(size_t)i * 4 + 2] = f.z * rcp;
     out[(size_t)i * 4 + 3] = f.w * rcp;
+}
+
+/* the images of several finished requests in one launch (k_finalize's
+ * division, for requests without the Q1 pattern): job k covers pixels
+ * [start_k, start_k + npix_k) of the launch.  The table is the kernel's
+ * argument (FIN_MAX jobs keep it under the 4 KB argument limit), so no
+ * staging buffer outlives the launch call. */
+struct FinJob {
+    const float4 *fb;
+    const int *nsamp;
+    float *dst;
+    uint32_t npix, start;
+};
+constexpr uint32_t FIN_MAX = 112;
+struct FinBatch {
+    uint32_t nj, total;
+    FinJob j[FIN_MAX];
+};
+__global__ void k_finalize_many(const FinBatch B)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < B.total; g += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = B.nj; /* j[lo].start <= g < j[hi].start (or the end) */
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (B.j[mid].start <= g) lo = mid;
+            else hi = mid;
+        }
+        const FinJob &J = B.j[lo];
+        const uint32_t i = g - J.start;
+        const float4 f = J.fb[i];
+        const float rcp = 1.0f / (float)J.nsamp[i];
+        J.dst[(size_t)i * 4 + 0] = f.x * rcp;
+        J.dst[(size_t)i * 4 + 1] = f.y * rcp;
+        J.dst[(size_t)i * 4 + 2] = f.z * rcp;
+        J.dst[(size_t)i * 4 + 3] = f.w * rcp;
+    }
 }
 
 __global__ void k_q1_scale(RenderDev P, const int *__restrict__ nsamp, float *__restrict__ out, uint32_t npix)
@@ -1289,7 +1342,7 @@ struct Engine {
     int nlanes = 1; /* lanes a pass spreads its primary chunks over */
     int sparse_from = 64; /* first bounce level traced by the SPARSE kernel instances */
     bool chain = true;    /* one-child chains in one kernel when the pass allows it */
-    bool sample_runs = true; /* a fixed-count request takes all its samples in its first pass */
+    bool sample_runs = true; /* a request's first pass may take several samples per pixel */
     bool attr = true;     /* some scene in flight has meshes with vertex attributes */
     std::map<const void *, uint32_t> pgrid; /* persistent grid per traversal kernel instance */
     /* captured descents, keyed by everything their launches depend on */
@@ -1574,12 +1627,10 @@ int admit(Engine &E, Job *j)
     }
     uint32_t *act = dptr<uint32_t>(E.act[E.cur]);
     if (tail) HIPCHK(c, hipMemcpyAsync(act + at + nent, act + at, tail * 4, hipMemcpyDeviceToDevice, c->stream));
-    k_fill_f4<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<float4>(E.fb) + base, npix, make_float4(0, 0, 0, 0));
-    k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.nsamp) + base, npix, 0u);
-    k_fill_u32<<<blocks(npix, 256), 256, 0, c->stream>>>(dptr<uint32_t>(E.preq) + base, npix, s);
-    k_tile_order<<<blocks(nent, 256), 256, 0, c->stream>>>(act + at, (uint32_t)npix, (uint32_t)base,
-                                                           (uint32_t)Q.P.W, (uint32_t)(Q.P.y1 - Q.P.y0),
-                                                           E.tile_w, Q.run);
+    k_admit<<<blocks(nent, 256), 256, 0, c->stream>>>(act + at, dptr<float4>(E.fb) + base,
+                                                      dptr<uint32_t>(E.nsamp) + base, dptr<uint32_t>(E.preq) + base,
+                                                      (uint32_t)npix, (uint32_t)base, (uint32_t)Q.P.W,
+                                                      (uint32_t)(Q.P.y1 - Q.P.y0), E.tile_w, Q.run, s);
     HIPCHK(c, hipGetLastError());
     E.nact += (uint32_t)nent;
     E.hreqs[s] = Q;
@@ -2017,6 +2068,52 @@ int finalize_job(Engine &E, uint32_t s)
     return SRAY_OK;
 }
 
+/* the images of the finished requests of a pass: those without the Q1
+ * pattern FIN_MAX to a k_finalize_many launch, the others one by one;
+ * then the host copies */
+int finalize_jobs(Engine &E, const std::vector<uint32_t> &slots)
+{
+    sray_gpu_ctx *c = E.c;
+    int rc;
+    FinBatch B;
+    B.nj = 0;
+    B.total = 0;
+    auto flush = [&]() -> int {
+        if (!B.nj) return SRAY_OK;
+        k_finalize_many<<<std::min<uint32_t>(blocks(B.total, 256), 4096u), 256, 0, c->stream>>>(B);
+        HIPCHK(c, hipGetLastError());
+        B.nj = 0;
+        B.total = 0;
+        return SRAY_OK;
+    };
+    for (uint32_t s : slots) {
+        Job *j = E.slot_job[s];
+        const ReqDev &Q = E.hreqs[s];
+        if (Q.P.flags & SRAY_RENDER_REF_Q1) {
+            if ((rc = finalize_job(E, s))) return rc;
+            continue;
+        }
+        float *dst = j->dev_out;
+        if (!dst) {
+            if ((rc = ensure(c, j->stage, (size_t)j->npix * 16))) return rc;
+            dst = dptr<float>(j->stage);
+        }
+        if (B.nj == FIN_MAX || (uint64_t)B.total + j->npix > 0xffffffffull)
+            if ((rc = flush())) return rc;
+        B.j[B.nj++] = FinJob{dptr<float4>(E.fb) + j->pix_base, dptr<int>(E.nsamp) + j->pix_base, dst,
+                             (uint32_t)j->npix, B.total};
+        B.total += (uint32_t)j->npix;
+    }
+    if ((rc = flush())) return rc;
+    for (uint32_t s : slots) {
+        Job *j = E.slot_job[s];
+        if (j->host_out && !(E.hreqs[s].P.flags & SRAY_RENDER_REF_Q1))
+            HIPCHK(c, hipMemcpyAsync(j->host_out, dptr<float>(j->stage), (size_t)j->npix * 16,
+                                     hipMemcpyDeviceToHost, c->stream));
+    }
+    return SRAY_OK;
+}
+
 /* one sample of every active pixel; retires the requests that finish */
 int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
 {
@@ -2145,8 +2242,10 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
             HIPCHK(c, hipMemcpyAsync(H->slotcnt, E.slotcnt.p, (size_t)E.hi * 4, hipMemcpyDeviceToHost, c->stream));
         }
     } else {
+        std::vector<uint32_t> all;
         for (uint32_t s = 0; s < E.hi; s++)
-            if (E.slot_job[s] && (rc = finalize_job(E, s))) return rc;
+            if (E.slot_job[s]) all.push_back(s);
+        if ((rc = finalize_jobs(E, all))) return rc;
     }
     if (!E.uni) HIPCHK(c, hipMemcpyAsync(H->srays, E.slotrays.p, (size_t)E.hi * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(&H->errf, E.errf.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -2160,6 +2259,7 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
     if (E.uni) H->srays[0] = sec - E.sec_seen;
     E.sec_seen = sec;
     bool host_copies = false;
+    std::vector<uint32_t> stopped;
     for (uint32_t s = 0; s < E.hi; s++) {
         Job *j = E.slot_job[s];
         if (!j) continue;
@@ -2168,9 +2268,12 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
         j->rays += E.slot_active[s] + H->srays[s];
         if (H->errf) j->limit_hit = true;
         E.slot_active[s] = cnt;
-        if (cnt) continue;
-        /* all pixels stopped: the image (src/render.cc:238-241) */
-        if (E.more && (rc = finalize_job(E, s))) return rc;
+        if (!cnt) stopped.push_back(s);
+    }
+    /* all pixels stopped: the images (src/render.cc:238-241) */
+    if (E.more && !stopped.empty() && (rc = finalize_jobs(E, stopped))) return rc;
+    for (uint32_t s : stopped) {
+        Job *j = E.slot_job[s];
         host_copies = host_copies || j->host_out != nullptr;
         retire(E, s);
         finished.push_back(j);
