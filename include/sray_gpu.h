@@ -463,7 +463,9 @@ int  sray_gpu_set_tuning(sray_gpu_ctx *ctx, const sray_tuning *t);
 int  sray_gpu_get_tuning(sray_gpu_ctx *ctx, sray_tuning *t);
 /* Autotuner: renders params of the scene `rounds` times per candidate,
  * varying one knob at a time from the current tuning (coordinate descent
- * over the values above), keeps the fastest tuning in the context and
+ * over the values above); a change is kept when its median render time
+ * beats the current tuning's by 1.5%, twice (the second time with both
+ * timed again, one after the other).  Keeps the tuning in the context and
  * returns it with its median render time. */
 int  sray_gpu_autotune(sray_gpu_ctx *ctx, uint32_t scene_id, const sray_render_params *p, int rounds,
                        sray_tuning *best, double *best_ms);

@@ -2667,10 +2667,27 @@ int do_autotune(sray_gpu_ctx *c, uint32_t scene_id, const sray_render_params *p,
                 return rc;
             }
             /* a change must beat the current choice by more than the
-             * round-to-round noise of the timings (about 1%) */
+             * round-to-round noise of the timings (about 1%), and again
+             * when both are timed once more, one right after the other
+             * (one lucky median would otherwise move the tuning) */
             if (ms < cur_ms * 0.985) {
-                cur = t;
-                cur_ms = ms;
+                double again_cur = 0, again_t = 0;
+                c->tune = cur;
+                if (!(rc = time_render(c, scene_id, p, rounds, dptr<float>(out), &again_cur))) {
+                    c->tune = t;
+                    rc = time_render(c, scene_id, p, rounds, dptr<float>(out), &again_t);
+                }
+                if (rc) {
+                    c->tune = cur;
+                    release(out);
+                    return rc;
+                }
+                if (again_t < again_cur * 0.985) {
+                    cur = t;
+                    cur_ms = again_t;
+                } else {
+                    cur_ms = again_cur;
+                }
             }
         }
         c->tune = cur;

@@ -105,7 +105,7 @@ struct sray_gpu_ctx {
     };
     std::map<std::string, KStat> kstats;
     /* tuning of the renderers and photon grids */
-    sray_tuning tune{128, 64, 16, 21, 1.0f, 1, 0, 4, 0, 1, 1};
+    sray_tuning tune{64, 64, 16, 21, 1.0f, 1, 0, 4, 0, 1, 1};
     /* status stream (fd < 0: off) and the lock that keeps records whole */
     std::atomic<int> status_fd{-1};
     std::mutex status_mu;

@@ -16,7 +16,7 @@ TOL = 1e-6
 
 
 def _variants():
-    return [dict(trace_block=64), dict(trace_block=256), dict(tile_w=8), dict(tile_w=16),
+    return [dict(trace_block=128), dict(trace_block=256), dict(tile_w=8), dict(tile_w=16),
             dict(chunk_log2=16), dict(gather_lanes=8), dict(gather_lanes=32), dict(gather_cell=0.25),
             dict(gather_cell=0.5), dict(gather_cell=2.0), dict(graphs=0), dict(persistent=1), dict(persistent=2), dict(sparse_from=64), dict(sparse_from=2)]
 
@@ -264,7 +264,7 @@ def test_autotune(ctx):
     assert (kept.trace_block, kept.tile_w, kept.chunk_log2) == (t.trace_block, t.tile_w, t.chunk_log2)
     after, _ = ctx.render(sid, p)
     assert float(np.abs(after - before).max()) <= TOL
-    ctx.set_tuning(trace_block=128, tile_w=64, gather_lanes=16, chunk_log2=21, gather_cell=1.0, graphs=1, persistent=0,
+    ctx.set_tuning(trace_block=64, tile_w=64, gather_lanes=16, chunk_log2=21, gather_cell=1.0, graphs=1, persistent=0,
                    lanes=4, sparse_from=64, chain=1, sample_runs=1)
     ctx.release(sid)
 
