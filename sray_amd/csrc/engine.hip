@@ -2191,29 +2191,50 @@ int run_pass(Engine &E, std::vector<Job *> &finished, bool sync_done)
      * lane takes its next chunk (never needed when mchunk * Kb^maxiter fits) */
     const bool chain = chain_pass(E);
     const bool ovf = !chain && may_overflow(E, mchunk);
-    for (uint32_t g0 = 0; g0 < nchunks; g0 += nl) {
-        const uint32_t g1 = std::min(nchunks, g0 + nl);
-        for (uint32_t j = g0; j < g1; j++) {
-            const uint32_t c0 = j * mchunk, m = std::min(mchunk, E.nact - c0);
-            Engine::Lane &ln = E.lanes[j - g0];
-            Level LV{&E, &ln, T, A, c0};
-            if ((rc = chain ? launch_chain(LV, m) : launch_chunk(LV, m, nl == 1))) return rc;
-            if (!ovf)
-                finish<<<blocks(m, 256), 256, 0, ln.s>>>(T, A, c0, m, dptr<float4>(ln.acc), dptr<float4>(E.fb),
-                                                         dptr<float4>(E.subpix), dptr<int>(E.nsamp),
-                                                         dptr<uint8_t>(E.keep));
-        }
-        if (ovf)
+    auto finish_chunk = [&](Engine::Lane &ln, uint32_t c0, uint32_t m) {
+        finish<<<blocks(m, 256), 256, 0, ln.s>>>(T, A, c0, m, dptr<float4>(ln.acc), dptr<float4>(E.fb),
+                                                 dptr<float4>(E.subpix), dptr<int>(E.nsamp), dptr<uint8_t>(E.keep));
+    };
+    if (!ovf) {
+        for (uint32_t g0 = 0; g0 < nchunks; g0 += nl) {
+            const uint32_t g1 = std::min(nchunks, g0 + nl);
             for (uint32_t j = g0; j < g1; j++) {
                 const uint32_t c0 = j * mchunk, m = std::min(mchunk, E.nact - c0);
                 Engine::Lane &ln = E.lanes[j - g0];
                 Level LV{&E, &ln, T, A, c0};
-                if ((rc = finish_levels(LV, 0))) return rc;
-                finish<<<blocks(m, 256), 256, 0, ln.s>>>(T, A, c0, m, dptr<float4>(ln.acc), dptr<float4>(E.fb),
-                                                         dptr<float4>(E.subpix), dptr<int>(E.nsamp),
-                                                         dptr<uint8_t>(E.keep));
+                if ((rc = chain ? launch_chain(LV, m) : launch_chunk(LV, m, nl == 1))) return rc;
+                finish_chunk(ln, c0, m);
             }
-        HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipGetLastError());
+        }
+    } else {
+        /* Chunk j runs on lane j % nl.  A lane's level counts are read
+         * after its descent (finish_levels), and the lane takes its next
+         * chunk as soon as that chunk is finished, in the order the chunks
+         * were started: no lane waits while the host reads another's. */
+        std::deque<std::pair<uint32_t, uint32_t>> fifo; /* (lane, chunk) in start order */
+        uint32_t next = 0;
+        auto start = [&](uint32_t l) -> int {
+            const uint32_t c0 = next * mchunk, m = std::min(mchunk, E.nact - c0);
+            Level LV{&E, &E.lanes[l], T, A, c0};
+            int r = launch_chunk(LV, m, nl == 1);
+            if (r) return r;
+            fifo.emplace_back(l, next++);
+            return SRAY_OK;
+        };
+        for (uint32_t l = 0; l < nl && next < nchunks; l++)
+            if ((rc = start(l))) return rc;
+        while (!fifo.empty()) {
+            const uint32_t l = fifo.front().first, j = fifo.front().second;
+            fifo.pop_front();
+            const uint32_t c0 = j * mchunk, m = std::min(mchunk, E.nact - c0);
+            Engine::Lane &ln = E.lanes[l];
+            Level LV{&E, &ln, T, A, c0};
+            if ((rc = finish_levels(LV, 0))) return rc;
+            finish_chunk(ln, c0, m);
+            HIPCHK(c, hipGetLastError());
+            if (next < nchunks && (rc = start(l))) return rc;
+        }
     }
     /* the pass end waits for every lane */
     for (uint32_t l = 1; l < nl; l++) {
