@@ -27,7 +27,8 @@ The line also carries:
                            arriving as a Poisson stream (latency p50/p99);
                   serve    32 resident rooms, a Poisson stream of tile
                            requests at a rate below the measured capacity;
-  targets       the north-star ratios, from the legs above.
+  targets       the target ratios (GPU against CPU, batched against plain,
+                latencies and image error), from the legs above.
 `--workload NAME` runs one of them alone and prints its own line.
 """
 import argparse
